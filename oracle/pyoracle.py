@@ -1,0 +1,323 @@
+"""ctypes front-end to the CPU checker libraries.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the cpu_baseline leg
+of bench.py, never by the product package.  ``Oracle`` wraps oracle/libftlm_oracle.so (this
+repository's CPU restatement of the reference hot path); ``Reference`` wraps
+oracle/_ref/libftlm_ref.so (the reference's own sources compiled here, present only where the
+reference checkout was available at build time).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libftlm_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libftlm_ref.so")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+_u32p = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
+
+
+def build(with_ref: bool | None = None) -> None:
+    """Compile the checker libraries (make -C oracle)."""
+    targets = ["oracle"]
+    if with_ref is None:
+        with_ref = os.path.isdir("/root/reference/src")
+    if with_ref:
+        targets.append("ref")
+    subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
+
+
+def params8(Lx, Ly, t=1.0, tp=0.0, U=0.0, V=0.0, phi_x=0.0, phi_y=0.0):
+    return np.array([Lx, Ly, t, tp, U, V, phi_x, phi_y], dtype=np.float64)
+
+
+class _Common:
+    prefix = ""
+
+    def _f(self, name):
+        return getattr(self.lib, self.prefix + name)
+
+    def geometry(self, p8):
+        cap = 4096
+        fr, to, sp = (np.zeros(cap, np.int32) for _ in range(3))
+        re, im = np.zeros(cap), np.zeros(cap)
+        pi, pj = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        nh, np_ = C.c_int(0), C.c_int(0)
+        rc = self._f("geometry")(p8, fr, to, sp, re, im, cap, C.byref(nh), pi, pj, cap, C.byref(np_))
+        assert rc == 0
+        n, m = nh.value, np_.value
+        hops = np.zeros(n, dtype=[("from", np.int32), ("to", np.int32), ("spin", np.int32),
+                                  ("re", np.float64), ("im", np.float64)])
+        hops["from"], hops["to"], hops["spin"] = fr[:n], to[:n], sp[:n]
+        hops["re"], hops["im"] = re[:n], im[:n]
+        pairs = np.stack([pi[:m], pj[:m]], axis=1)
+        return hops, pairs
+
+    def apply(self, p8, nu, nd, x):
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        y = np.zeros_like(x)
+        rc = self._f("apply")(p8, nu, nd, x.view(np.float64), y.view(np.float64))
+        assert rc == 0
+        return y
+
+    def lanczos(self, p8, nu, nd, M, seed):
+        a, b = np.zeros(max(M, 1)), np.zeros(max(M, 1))
+        nb = C.c_int(0)
+        if self.prefix == "orc_":
+            used = self._f("lanczos")(p8, nu, nd, M, seed, None, a, b, C.byref(nb))
+        else:
+            used = self._f("lanczos")(p8, nu, nd, M, seed, a, b, C.byref(nb))
+        return used, a[:used].copy(), b[: nb.value].copy()
+
+    def lanczos_extrema(self, p8, nu, nd, M, seed):
+        lo, hi = C.c_double(0), C.c_double(0)
+        used = self._f("lanczos_extrema")(p8, nu, nd, M, seed, C.byref(lo), C.byref(hi))
+        return used, lo.value, hi.value
+
+    def tridiagonal_extrema(self, alpha, beta):
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        beta = np.ascontiguousarray(beta if len(beta) else np.zeros(1), np.float64)
+        lo, hi = C.c_double(0), C.c_double(0)
+        self._f("tridiagonal_extrema")(alpha, len(alpha), beta, len(beta) if len(beta) else 0,
+                                       C.byref(lo), C.byref(hi))
+        return lo.value, hi.value
+
+    def ftlm_log_partition(self, p8, nu, nd, beta, R, M, seed, real=False):
+        return self._f("ftlm_log_partition")(p8, nu, nd, beta, R, M, seed, int(real))
+
+    def orbit_basis(self, lx, ly, nu, nd, kx, ky):
+        cap = 1 << 20
+        reps = np.zeros(cap, np.uint32)
+        osz, ssz = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        A = np.zeros(2 * cap)
+        if self.prefix == "orc_":
+            dk = self._f("orbit_basis")(lx, ly, nu, nd, kx, ky, 1e-12, reps, osz, ssz, A, cap)
+        else:
+            dk = self._f("orbit_basis")(lx, ly, nu, nd, kx, ky, reps, osz, ssz, A, cap)
+        assert dk >= 0
+        return reps[:dk].copy(), osz[:dk].copy(), ssz[:dk].copy(), A[: 2 * dk].view(np.complex128).copy()
+
+    def orbit_map(self, lx, ly, nu, nd, kx, ky, states):
+        states = np.ascontiguousarray(states, np.uint32)
+        n = len(states)
+        row, dx, dy = (np.zeros(n, np.int32) for _ in range(3))
+        eta = np.zeros(n)
+        if self.prefix == "orc_":
+            self._f("orbit_map")(lx, ly, nu, nd, kx, ky, 1e-12, states, n, row, dx, dy, eta)
+        else:
+            self._f("orbit_map")(lx, ly, nu, nd, kx, ky, states, n, row, dx, dy, eta)
+        return row, dx, dy, eta
+
+    def orbit_apply(self, p8, nu, nd, kx, ky, x):
+        x = np.ascontiguousarray(x, np.complex128)
+        y = np.zeros_like(x)
+        if self.prefix == "orc_":
+            self._f("orbit_apply")(p8, nu, nd, kx, ky, 1e-12, x.view(np.float64), y.view(np.float64))
+        else:
+            self._f("orbit_apply")(p8, nu, nd, kx, ky, x.view(np.float64), y.view(np.float64))
+        return y
+
+    def orbit_ftlm_log_partition(self, p8, nu, nd, kx, ky, beta, R, M, seed):
+        return self._f("orbit_ftlm_log_partition")(p8, nu, nd, kx, ky, beta, R, M, seed)
+
+
+class Oracle(_Common):
+    prefix = "orc_"
+
+    def __init__(self, path: str = ORACLE_SO):
+        if not os.path.exists(path):
+            build(with_ref=False)
+        L = C.CDLL(path)
+        self.lib = L
+        L.orc_geometry.argtypes = [_dp, _ip, _ip, _ip, _dp, _dp, C.c_int, C.POINTER(C.c_int), _ip, _ip,
+                                   C.c_int, C.POINTER(C.c_int)]
+        L.orc_basis_dim.argtypes = [C.c_int] * 3
+        L.orc_basis_dim.restype = C.c_long
+        L.orc_basis_masks.argtypes = [C.c_int, C.c_int, C.c_int, _up, _up]
+        L.orc_index_of.argtypes = [C.c_int, C.c_int, C.c_int, _up, _up, C.c_long, _lp]
+        L.orc_apply.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp]
+        L.orc_apply_cols.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp, C.c_long, C.c_long]
+        L.orc_rank_closed_form.argtypes = [C.c_int, C.c_int, _up, C.c_long, _lp]
+        L.orc_gauss_start.argtypes = [C.c_long, C.c_uint, _dp]
+        L.orc_lanczos.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_void_p, _dp, _dp,
+                                  C.POINTER(C.c_int)]
+        L.orc_tridiagonal_extrema.argtypes = [_dp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double)]
+        L.orc_lanczos_extrema.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
+        L.orc_quadrature.argtypes = [_dp, C.c_int, _dp, C.c_int, C.c_double]
+        L.orc_quadrature.restype = C.c_double
+        L.orc_quadrature_ritz.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, _dp]
+        L.orc_tridiag_eig_ql.argtypes = [_dp, C.c_int, _dp, _dp, _dp]
+        L.orc_ftlm_log_partition.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_uint,
+                                             C.c_int]
+        L.orc_ftlm_log_partition.restype = C.c_double
+        L.orc_gc_density.argtypes = [_dp, _ip, C.c_int, C.c_double, _dp, C.c_int, C.c_int, _dp]
+        L.orc_gc_density.restype = None
+        L.orc_orbit_basis.argtypes = [C.c_int] * 6 + [C.c_double, _u32p, _ip, _ip, _dp, C.c_long]
+        L.orc_orbit_map.argtypes = [C.c_int] * 6 + [C.c_double, _u32p, C.c_long, _ip, _ip, _ip, _dp]
+        L.orc_orbit_apply.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp]
+        L.orc_translation_sign.argtypes = [C.c_uint32, C.c_uint32] + [C.c_int] * 4
+        L.orc_translation_sign.restype = C.c_double
+        L.orc_canonical.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_int]
+        L.orc_canonical.restype = C.c_uint32
+        L.orc_orbit_ftlm_log_partition.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                   C.c_int, C.c_int, C.c_uint]
+        L.orc_orbit_ftlm_log_partition.restype = C.c_double
+        L.orc_exact_log_partition.argtypes = [_dp, C.c_int, C.c_double]
+        L.orc_exact_log_partition.restype = C.c_double
+        L.orc_dense_hamiltonian.argtypes = [_dp, C.c_int, C.c_int, _dp]
+        L.orc_dense_extrema.argtypes = [_dp, C.c_int, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+
+    # ---- oracle-only helpers
+    def basis_dim(self, n, nu, nd):
+        return self.lib.orc_basis_dim(n, nu, nd)
+
+    def basis_masks(self, n, nu, nd):
+        d = self.basis_dim(n, nu, nd)
+        assert d >= 0
+        up, dn = np.zeros(d, np.uint64), np.zeros(d, np.uint64)
+        assert self.lib.orc_basis_masks(n, nu, nd, up, dn) == 0
+        return up, dn
+
+    def index_of(self, n, nu, nd, up, dn):
+        up = np.ascontiguousarray(up, np.uint64)
+        dn = np.ascontiguousarray(dn, np.uint64)
+        out = np.zeros(len(up), np.int64)
+        assert self.lib.orc_index_of(n, nu, nd, up, dn, len(up), out) == 0
+        return out
+
+    def rank_closed_form(self, n, k, masks):
+        masks = np.ascontiguousarray(masks, np.uint64)
+        out = np.zeros(len(masks), np.int64)
+        self.lib.orc_rank_closed_form(n, k, masks, len(masks), out)
+        return out
+
+    def gauss_start(self, dim, seed):
+        q = np.zeros(dim, np.complex128)
+        self.lib.orc_gauss_start(dim, seed, q.view(np.float64))
+        return q
+
+    def lanczos_from(self, p8, nu, nd, M, q0):
+        """Lanczos with a supplied (unnormalised) start vector."""
+        q0 = np.ascontiguousarray(q0, np.complex128)
+        a, b = np.zeros(max(M, 1)), np.zeros(max(M, 1))
+        nb = C.c_int(0)
+        used = self.lib.orc_lanczos(p8, nu, nd, M, 0, q0.ctypes.data, a, b, C.byref(nb))
+        return used, a[:used].copy(), b[: nb.value].copy()
+
+    def apply_cols(self, p8, nu, nd, x, y, c0, c1):
+        assert self.lib.orc_apply_cols(p8, nu, nd, x.view(np.float64), y.view(np.float64), c0, c1) == 0
+
+    def quadrature(self, alpha, beta, bt):
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        b = np.ascontiguousarray(beta if len(beta) else np.zeros(1), np.float64)
+        return self.lib.orc_quadrature(alpha, len(alpha), b, len(beta), bt)
+
+    def quadrature_ritz(self, alpha, beta):
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        b = np.ascontiguousarray(beta if len(beta) else np.zeros(1), np.float64)
+        n = len(alpha)
+        lam, w = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        self.lib.orc_quadrature_ritz(alpha, n, b, len(beta), lam, w)
+        return lam[:n], w[:n]
+
+    def tridiag_eig_ql(self, alpha, beta):
+        alpha = np.ascontiguousarray(alpha, np.float64)
+        n = len(alpha)
+        b = np.zeros(max(n, 1))
+        b[: min(len(beta), max(n - 1, 0))] = beta[: max(n - 1, 0)]
+        lam, w = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+        rc = self.lib.orc_tridiag_eig_ql(alpha, n, b, lam, w)
+        assert rc == 0
+        return lam[:n], w[:n]
+
+    def gc_density(self, lnZ, N, beta, mu, n_sites):
+        lnZ = np.ascontiguousarray(lnZ, np.float64)
+        N = np.ascontiguousarray(N, np.int32)
+        mu = np.ascontiguousarray(mu, np.float64)
+        out = np.zeros(len(mu))
+        self.lib.orc_gc_density(lnZ, N, len(lnZ), beta, mu, len(mu), n_sites, out)
+        return out
+
+    def translation_sign(self, u, d, lx, ly, dx, dy):
+        return self.lib.orc_translation_sign(u, d, lx, ly, dx, dy)
+
+    def canonical(self, u, d, lx, ly):
+        return self.lib.orc_canonical(u, d, lx, ly)
+
+    def exact_log_partition(self, H, beta):
+        H = np.ascontiguousarray(H, np.complex128)
+        return self.lib.orc_exact_log_partition(H.view(np.float64).ravel(), H.shape[0], beta)
+
+    def dense_hamiltonian(self, p8, nu, nd):
+        d = self.basis_dim(int(p8[0] * p8[1]), nu, nd)
+        H = np.zeros((d, d), np.complex128)
+        assert self.lib.orc_dense_hamiltonian(p8, nu, nd, H.view(np.float64).ravel()) == 0
+        return H
+
+    def dense_extrema(self, Hreal, cap=1600):
+        A = np.ascontiguousarray(Hreal, np.float64)
+        lo, hi = C.c_double(0), C.c_double(0)
+        self.lib.orc_dense_extrema(A.ravel(), A.shape[0], cap, C.byref(lo), C.byref(hi))
+        return lo.value, hi.value
+
+
+class Reference(_Common):
+    prefix = "ref_"
+
+    def __init__(self, path: str = REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = C.CDLL(path)
+        self.lib = L
+        L.ref_version.restype = C.c_char_p
+        L.ref_geometry.argtypes = [_dp, _ip, _ip, _ip, _dp, _dp, C.c_int, C.POINTER(C.c_int), _ip, _ip,
+                                   C.c_int, C.POINTER(C.c_int)]
+        L.ref_basis_masks.argtypes = [C.c_int, C.c_int, C.c_int, _up, _up, C.c_int]
+        L.ref_index_of.argtypes = [C.c_int, C.c_int, C.c_int, _up, _up, C.c_int, _ip]
+        L.ref_apply.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp]
+        L.ref_lanczos.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_uint, _dp, _dp, C.POINTER(C.c_int)]
+        L.ref_lanczos_extrema.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double)]
+        L.ref_tridiagonal_extrema.argtypes = [_dp, C.c_int, _dp, C.c_int, C.POINTER(C.c_double),
+                                              C.POINTER(C.c_double)]
+        L.ref_ftlm_log_partition.argtypes = [_dp, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_uint,
+                                             C.c_int]
+        L.ref_ftlm_log_partition.restype = C.c_double
+        L.ref_ftlm_log_partition_dense.argtypes = [_dp, C.c_int, C.c_double, C.c_int, C.c_int, C.c_uint]
+        L.ref_ftlm_log_partition_dense.restype = C.c_double
+        L.ref_orbit_basis.argtypes = [C.c_int] * 6 + [_u32p, _ip, _ip, _dp, C.c_int]
+        L.ref_orbit_map.argtypes = [C.c_int] * 6 + [_u32p, C.c_int, _ip, _ip, _ip, _dp]
+        L.ref_orbit_apply.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.ref_orbit_ftlm_log_partition.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                   C.c_int, C.c_int, C.c_uint]
+        L.ref_orbit_ftlm_log_partition.restype = C.c_double
+
+    def basis_masks(self, n, nu, nd):
+        cap = 1 << 22
+        up, dn = np.zeros(cap, np.uint64), np.zeros(cap, np.uint64)
+        d = self.lib.ref_basis_masks(n, nu, nd, up, dn, cap)
+        if d < 0:
+            raise ValueError("invalid sector")
+        return up[:d].copy(), dn[:d].copy()
+
+    def index_of(self, n, nu, nd, up, dn):
+        up = np.ascontiguousarray(up, np.uint64)
+        dn = np.ascontiguousarray(dn, np.uint64)
+        out = np.zeros(len(up), np.int32)
+        assert self.lib.ref_index_of(n, nu, nd, up, dn, len(up), out) == 0
+        return out.astype(np.int64)
+
+    def ftlm_dense(self, H, beta, R, M, seed):
+        H = np.ascontiguousarray(H, np.complex128)
+        return self.lib.ref_ftlm_log_partition_dense(H.view(np.float64).ravel(), H.shape[0], beta, R, M, seed)
