@@ -1,0 +1,239 @@
+"""FTLM hot-path benchmark on one MI355X: 4x4 (8,8) extended Hubbard, dim 165,636,900.
+
+A "step" is one Lanczos step of a block of R random-phase start vectors (one block H*V, the FP64
+dot/norm reductions and the three-term update), so one step = R sample-steps.  value = sample-steps
+per second of the batched complex64 path (R = 16); the plain path (R = 1, complex128, the
+reference's arithmetic) is timed beside it.  Inputs are resident in HBM when timing starts.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--R 16] [--no-validate] [--no-cpu]
+
+Prints progress on stderr and ONE JSON line on stdout.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "4x4 (8,8) FTLM sample-steps/s, batched vs plain at |ΔlnZ|≤1e-6; H·v GB/s"
+MODEL = dict(Lx=4, Ly=4, t=1.0, tp=-0.25, U=8.0, V=1.5, phi_x=0.1, phi_y=0.0)
+NU = ND = 8
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def time_steps(sess, K, torch):
+    st = torch.cuda.current_stream()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    sess.run(K, st.cuda_stream)
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / 1e3
+
+
+def kernel_roofline(F, L, op, R, dtype, torch, reps=5):
+    """Average duration of the H*V kernel (one launch for R <= 16 at complex64) via HIP events on
+    the launch stream; achieved = algorithmic bytes (read x, write y once) / duration."""
+    dim = op.dim()
+    ctype = torch.complex64 if dtype == L.FTLM_C64 else torch.complex128
+    s = 8 if dtype == L.FTLM_C64 else 16
+    X = torch.randn(dim * R, dtype=ctype, device="cuda")
+    Y = torch.empty_like(X)
+    st = torch.cuda.current_stream()
+    op.apply_device(X, Y, R=R, dtype=dtype, stream=st.cuda_stream)  # warm
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        op.apply_device(X, Y, R=R, dtype=dtype, stream=st.cuda_stream)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) / 1e3)
+    del X, Y
+    torch.cuda.empty_cache()
+    t = sum(ts) / len(ts)
+    B = R * dim * 2 * s
+    return t, B
+
+
+def traffic_from_profile():
+    path = os.path.join(ROOT, "profiles", "hv_traffic_4x4_c64_r16.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return float(json.load(f)["hbm_bytes_per_launch"])
+    except Exception:  # noqa: BLE001
+        return None
+
+
+def cpu_baseline(budget_s=12.0):
+    """The oracle's push-form complex128 H*v (reference arithmetic, closed-form rank instead of the
+    hash map), 1 thread, on a bounded column slice of the same 4x4 (8,8) sector, scaled by dim;
+    plus the Lanczos vector passes timed on a slice.  Returns sample-steps/s of the plain method."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle, params8
+
+    orc = Oracle()
+    p8 = params8(MODEL["Lx"], MODEL["Ly"], MODEL["t"], MODEL["tp"], MODEL["U"], MODEL["V"], MODEL["phi_x"],
+                 MODEL["phi_y"])
+    dim = 12870 * 12870
+    x = np.zeros(dim, np.complex128)
+    y = np.zeros(dim, np.complex128)
+    rng = np.random.default_rng(0)
+    c0, chunk, done, t_used = 0, 4096, 0, 0.0
+    while t_used < budget_s:
+        x[c0:c0 + chunk] = rng.standard_normal(chunk) + 1j * rng.standard_normal(chunk)
+        t0 = time.perf_counter()
+        orc.apply_cols(p8, NU, ND, x, y, c0, c0 + chunk)
+        t_used += time.perf_counter() - t0
+        c0 += chunk
+        done += chunk
+    t_hv = t_used / done * dim
+    # recurrence: dot, two axpy, norm, scale over complex128 vectors (reference lanczos.cpp:180-206)
+    n = 1 << 22
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    t0 = time.perf_counter()
+    z = np.vdot(a, b)
+    b -= z * a
+    b -= 0.5 * a
+    nw = np.vdot(b, b).real
+    a = b / math.sqrt(nw)
+    t_rec = (time.perf_counter() - t0) / n * dim
+    per_step = t_hv + t_rec
+    return {
+        "value": 1.0 / per_step,
+        "unit": "sample-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle push-form complex128 H*v over columns [0,{done}) of the 4x4 (8,8) sector "
+                   f"({t_used:.1f} s, scaled by dim={dim}: {t_hv:.0f} s per H*v) + Lanczos vector passes "
+                   f"timed on 2^22 elements and scaled ({t_rec:.2f} s); 1 thread"),
+    }
+
+
+def ftlm_lnz(F, L, op, R, M, dtype, betas):
+    par = F.FtlmParams(n_random=R, lanczos_steps=M, seed=1, start=L.FTLM_START_PHASE, dtype=dtype,
+                       quad=L.FTLM_QUAD_REFERENCE)
+    rd = F.ftlm_ritz([op], par)
+    return F.ftlm_thermo(rd, betas)["lnZ"][0]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--R", type=int, default=16)
+    ap.add_argument("--validate-M", type=int, default=150)
+    ap.add_argument("--no-validate", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(sectors and samples would shard across GPUs with no collective on the critical path).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import ftlm_extended_hubbard_amd as F
+    from ftlm_extended_hubbard_amd import _lib as L
+
+    assert torch.cuda.is_available(), "no GPU"
+    K, W, R = args.steps, args.warmup, args.R
+    t0 = time.perf_counter()
+    op = F.HubbardOperator.from_params(F.HubbardParams(**MODEL), NU, ND)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    dim = op.dim()
+    log(f"sector built: dim={dim} in {t_setup:.2f} s")
+
+    # ---- batched complex64, R vectors
+    sess = F.KrylovSession(op, R=R, max_steps=W + K, dtype=L.FTLM_C64, start=L.FTLM_START_PHASE, seed=1)
+    sess.run(W)
+    torch.cuda.synchronize()
+    log(f"batched c64 R={R}: warmup {W} done")
+    secs = time_steps(sess, K, torch)
+    a, b, used, alive = sess.result()
+    sess.close()
+    ms_per_step = secs / K * 1e3
+    value = K * R / secs
+    log(f"batched c64 R={R}: {ms_per_step:.2f} ms/step, {value:.1f} sample-steps/s")
+
+    # ---- plain complex128, R = 1 (the reference's arithmetic)
+    plain = None
+    if not args.no_plain:
+        ps = F.KrylovSession(op, R=1, max_steps=W + K, dtype=L.FTLM_C128, start=L.FTLM_START_PHASE, seed=1)
+        ps.run(W)
+        torch.cuda.synchronize()
+        psecs = time_steps(ps, K, torch)
+        ps.close()
+        plain = {"value": K / psecs, "unit": "sample-steps/s", "ms_per_step": psecs / K * 1e3, "R": 1,
+                 "dtype": "c128"}
+        log(f"plain c128 R=1: {plain['ms_per_step']:.2f} ms/step, {plain['value']:.1f} sample-steps/s")
+
+    # ---- roofline of the dominant kernel (block H*V)
+    t_k, B = kernel_roofline(F, L, op, R, L.FTLM_C64, torch)
+    achieved = B / t_k / 1e9
+    log(f"k_hv_sector c64 R={R}: {t_k * 1e3:.3f} ms/launch, {achieved:.0f} GB/s algorithmic")
+    t_k1, B1 = kernel_roofline(F, L, op, 1, L.FTLM_C128, torch)
+    log(f"k_hv_sector c128 R=1: {t_k1 * 1e3:.3f} ms/launch, {B1 / t_k1 / 1e9:.0f} GB/s algorithmic")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
+                "kernel": f"k_hv_sector<float,{min(R, 16)},false>", "launch_ms": round(t_k * 1e3, 4),
+                "algorithmic_bytes_per_launch": B,
+                "plain_c128_r1": {"launch_ms": round(t_k1 * 1e3, 4), "achieved": round(B1 / t_k1 / 1e9, 1)}}
+
+    # ---- |dlnZ| of the batched complex64 FTLM against the FP64 path, same start vectors
+    dlnz = None
+    if not args.no_validate:
+        betas = [0.5, 2.0, 8.0]
+        M = args.validate_M
+        t1 = time.perf_counter()
+        l64 = ftlm_lnz(F, L, op, R, M, L.FTLM_C64, betas)
+        l128 = ftlm_lnz(F, L, op, R, M, L.FTLM_C128, betas)
+        dlnz = {"M": M, "R": R, "betas": betas, "lnZ_c64": [float(v) for v in l64],
+                "lnZ_c128": [float(v) for v in l128],
+                "abs_diff": [float(abs(p - q)) for p, q in zip(l64, l128)],
+                "seconds": round(time.perf_counter() - t1, 1)}
+        log(f"dlnZ (M={M}): {dlnz['abs_diff']}")
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    if cpu:
+        log(f"cpu baseline: {cpu['value']:.3e} sample-steps/s ({cpu['sample']})")
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "sample-steps/s", "n_gpus": 1, "steps": K,
+        "warmup": W, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64 (fp32 H*v, fp64 dots/alpha/beta)", "data": "synthetic",
+        "config": {"workload": "4x4 (8,8) FTLM, batched block Lanczos, random-phase starts", "dim": dim,
+                   "model": MODEL, "R": R, "setup_s": round(t_setup, 2)},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "plain_fp64": plain,
+        "batched_vs_plain": None if plain is None else round(value / plain["value"], 3),
+        "dlnZ": dlnz,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,39 @@
+"""MI355X-native FTLM hot path for the 2D extended Hubbard model.
+
+Mirrors the reference package `ftlm` (python/ftlm/__init__.py): HubbardParams, nsites and
+version_string, plus the hot-path API (FockBasis, apply_extended_hubbard, Lanczos, FTLM) backed by
+the gfx950 library libftlm_hip.so.  Unlike the reference there is no silent fallback: importing a
+hot-path symbol without the built library raises.
+"""
+
+from . import _lib
+from .api import (
+    FockBasis,
+    FtlmParams,
+    HubbardOperator,
+    HubbardParams,
+    KrylovSession,
+    LanczosExtrema,
+    RectLattice,
+    RitzData,
+    apply_extended_hubbard,
+    build_hubbard_geometry,
+    device_count,
+    ftlm_log_partition_complex,
+    ftlm_log_partition_real,
+    ftlm_ritz,
+    ftlm_thermo,
+    lanczos_extrema,
+    lanczos_tridiagonal,
+    nsites,
+    quadrature_ritz,
+    tridiagonal_extrema,
+    version_string,
+)
+
+__all__ = [
+    "HubbardParams", "nsites", "version_string", "RectLattice", "build_hubbard_geometry", "FockBasis",
+    "HubbardOperator", "apply_extended_hubbard", "LanczosExtrema", "lanczos_tridiagonal", "lanczos_extrema",
+    "tridiagonal_extrema", "FtlmParams", "ftlm_log_partition_complex", "ftlm_log_partition_real", "RitzData",
+    "ftlm_ritz", "ftlm_thermo", "quadrature_ritz", "device_count", "KrylovSession", "_lib",
+]

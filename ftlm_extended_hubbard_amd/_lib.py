@@ -1,0 +1,116 @@
+"""ctypes binding of libftlm_hip.so (the C ABI declared in include/ftlm_hip.h).
+
+Loading fails loudly: there is no CPU fallback anywhere in this package.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libftlm_hip.so")
+
+FTLM_OK, FTLM_EINVAL, FTLM_ERANGE, FTLM_EHIP, FTLM_ENOMEM, FTLM_EUNSUPPORTED = range(6)
+FTLM_C128, FTLM_C64 = 0, 1
+FTLM_START_GAUSS_MT, FTLM_START_PHASE = 0, 1
+FTLM_QUAD_REFERENCE, FTLM_QUAD_CONVERGED = 0, 1
+
+
+class FtlmHipError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[ftlm_hip code {code}] {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    _fields_ = [("Lx", C.c_int), ("Ly", C.c_int), ("t", C.c_double), ("tp", C.c_double), ("U", C.c_double),
+                ("V", C.c_double), ("phi_x", C.c_double), ("phi_y", C.c_double)]
+
+
+class Hop(C.Structure):
+    _fields_ = [("from_", C.c_int), ("to", C.c_int), ("spin", C.c_int), ("re", C.c_double), ("im", C.c_double)]
+
+
+class Pair(C.Structure):
+    _fields_ = [("i", C.c_int), ("j", C.c_int)]
+
+
+class FtlmOpts(C.Structure):
+    _fields_ = [("R", C.c_int), ("M", C.c_int), ("seed", C.c_uint), ("start", C.c_int), ("dtype", C.c_int),
+                ("quad", C.c_int), ("clamp_M_to_dim", C.c_int), ("real_variant", C.c_int)]
+
+
+# C struct layout (ftlm_hop_t is 32 bytes: three ints, 4 bytes of padding, two doubles)
+HOP_DTYPE = np.dtype([("from", np.int32), ("to", np.int32), ("spin", np.int32), ("re", np.float64),
+                      ("im", np.float64)], align=True)
+PAIR_DTYPE = np.dtype([("i", np.int32), ("j", np.int32)], align=True)
+assert HOP_DTYPE.itemsize == C.sizeof(Hop) and PAIR_DTYPE.itemsize == C.sizeof(Pair)
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libftlm_hip.so once; raise if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m ftlm_extended_hubbard_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    vp, i64p, dp, ip = C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.ftlm_hip_version.restype = C.c_char_p
+    L.ftlm_hip_last_error.restype = C.c_char_p
+    L.ftlm_hip_device_count.argtypes = [ip]
+    L.ftlm_hip_nsites.argtypes = [C.POINTER(Params)]
+    L.ftlm_hip_build_geometry.argtypes = [C.POINTER(Params), vp, C.c_int, ip, vp, C.c_int, ip]
+    L.ftlm_hip_sector_create.argtypes = [C.c_int, C.c_int, C.c_int, vp, C.c_int, vp, C.c_int, C.c_double,
+                                         C.c_double, C.POINTER(vp)]
+    L.ftlm_hip_op_destroy.argtypes = [vp]
+    L.ftlm_hip_op_dim.argtypes = [vp, i64p, i64p, i64p]
+    L.ftlm_hip_sector_masks.argtypes = [vp, C.c_int64, C.c_int64, vp, vp]
+    L.ftlm_hip_index_of.argtypes = [vp, vp, vp, C.c_int64, vp]
+    L.ftlm_hip_apply.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
+    L.ftlm_hip_apply_host.argtypes = [vp, vp, vp]
+    L.ftlm_hip_lanczos.argtypes = [vp, C.c_int, C.c_uint, vp, vp, vp, ip, ip]
+    L.ftlm_hip_lanczos_extrema.argtypes = [vp, C.c_int, C.c_uint, dp, dp, ip]
+    L.ftlm_hip_tridiagonal_extrema.argtypes = [vp, C.c_int, vp, C.c_int, dp, dp]
+    L.ftlm_hip_ftlm_ritz.argtypes = [vp, C.c_int, C.POINTER(FtlmOpts), vp, vp, vp, vp, vp, vp]
+    L.ftlm_hip_ftlm_thermo.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, C.c_int,
+                                       vp, vp, vp, vp, vp]
+    L.ftlm_hip_ftlm_log_partition.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp]
+    L.ftlm_hip_quadrature_ritz.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]
+    L.ftlm_hip_krylov_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(vp)]
+    L.ftlm_hip_krylov_run.argtypes = [vp, C.c_int, vp]
+    L.ftlm_hip_krylov_result.argtypes = [vp, vp, vp, vp, vp]
+    L.ftlm_hip_krylov_bytes.argtypes = [vp, i64p]
+    L.ftlm_hip_krylov_destroy.argtypes = [vp]
+    _lib = L
+    return L
+
+
+def check(rc: int) -> None:
+    if rc != FTLM_OK:
+        msg = lib().ftlm_hip_last_error().decode(errors="replace")
+        if rc in (FTLM_EINVAL, FTLM_EUNSUPPORTED):
+            raise ValueError(f"[ftlm_hip code {rc}] {msg}")
+        if rc == FTLM_ERANGE:
+            raise IndexError(f"[ftlm_hip code {rc}] {msg}")
+        if rc == FTLM_ENOMEM:
+            raise MemoryError(f"[ftlm_hip code {rc}] {msg}")
+        raise FtlmHipError(rc, msg)
+
+
+def ptr(a: np.ndarray | None) -> int | None:
+    return None if a is None else a.ctypes.data
+
+
+def exported_symbols() -> list[str]:
+    """Names of every entry point declared in include/ftlm_hip.h."""
+    import re
+
+    hdr = os.path.join(os.path.dirname(PKG), "include", "ftlm_hip.h")
+    with open(hdr) as f:
+        return sorted(set(re.findall(r"\b(ftlm_hip_[a-z_0-9]+)\s*\(", f.read())))

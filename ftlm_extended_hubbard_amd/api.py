@@ -1,0 +1,403 @@
+"""Python mirror of the reference's FTLM hot-path API, running on the MI355X through libftlm_hip.so.
+
+Names, argument meaning and error behaviour follow the reference C++ API:
+    HubbardParams, nsites, version_string        include/ftlm/hubbard_params.hpp, python/bindings.cpp
+    RectLattice, build_hubbard_geometry          include/ftlm/lattice.hpp:25-40
+    FockBasis(n_sites, n_up, n_down)             include/ftlm/fock_basis.hpp:11-35
+    apply_extended_hubbard(p, basis, hops, pairs, x)   include/ftlm/hubbard_hamiltonian.hpp:13-16
+    lanczos_tridiagonal / lanczos_extrema / tridiagonal_extrema   include/ftlm/lanczos.hpp:45-63
+    FtlmParams, ftlm_log_partition_complex / _real               include/ftlm/ftlm_thermo.hpp:22-50
+Where the reference takes an operator callback (std::function apply_h) this API takes a device
+operator (HubbardOperator / KBlockOperator): the operator itself lives on the GPU.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+
+
+# ---------------------------------------------------------------- params / geometry
+
+@dataclass
+class HubbardParams:
+    """include/ftlm/hubbard_params.hpp:6-15 (pybind11 defaults of python/bindings.cpp:13-28)."""
+
+    Lx: int = 2
+    Ly: int = 2
+    t: float = 1.0
+    tp: float = 0.0
+    U: float = 0.0
+    V: float = 0.0
+    phi_x: float = 0.0
+    phi_y: float = 0.0
+
+    def _c(self) -> L.Params:
+        return L.Params(self.Lx, self.Ly, self.t, self.tp, self.U, self.V, self.phi_x, self.phi_y)
+
+
+def nsites(params: HubbardParams) -> int:
+    return L.lib().ftlm_hip_nsites(C.byref(params._c()))
+
+
+def version_string() -> str:
+    return L.lib().ftlm_hip_version().decode()
+
+
+@dataclass
+class RectLattice:
+    Lx: int = 1
+    Ly: int = 1
+
+    def n_sites(self) -> int:
+        return self.Lx * self.Ly
+
+    def site_index(self, x: int, y: int) -> int:
+        return y * self.Lx + x
+
+    def coords(self, site: int) -> tuple[int, int]:
+        return site % self.Lx, site // self.Lx
+
+
+def build_hubbard_geometry(p: HubbardParams, lat: RectLattice | None = None):
+    """Directed spinful hops and NN pairs, same entries and order as src/lattice.cpp:38-110.
+    Returns (hops, pairs) as numpy structured arrays (fields from,to,spin,re,im / i,j)."""
+    if lat is not None and (lat.Lx, lat.Ly) != (p.Lx, p.Ly):
+        p = HubbardParams(lat.Lx, lat.Ly, p.t, p.tp, p.U, p.V, p.phi_x, p.phi_y)
+    n = max(p.Lx * p.Ly, 1)
+    cap_h, cap_p = 64 * n + 64, 8 * n + 8
+    hops = np.zeros(cap_h, L.HOP_DTYPE)
+    pairs = np.zeros(cap_p, L.PAIR_DTYPE)
+    nh, np_ = C.c_int(0), C.c_int(0)
+    L.check(L.lib().ftlm_hip_build_geometry(C.byref(p._c()), hops.ctypes.data, cap_h, C.byref(nh),
+                                            pairs.ctypes.data, cap_p, C.byref(np_)))
+    return hops[: nh.value].copy(), pairs[: np_.value].copy()
+
+
+# ---------------------------------------------------------------- operators
+
+class _Op:
+    """Owns one ftlm_op_t handle."""
+
+    _h: C.c_void_p
+
+    def __init__(self):
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and L._lib is not None:
+            L._lib.ftlm_hip_op_destroy(h)
+            self._h = C.c_void_p()
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def dim(self) -> int:
+        d = C.c_int64(0)
+        L.check(L.lib().ftlm_hip_op_dim(self._h, C.byref(d), None, None))
+        return d.value
+
+    # ---- H * v
+    def apply(self, x: np.ndarray) -> np.ndarray:
+        """y = H x for host complex128 vector(s) x of shape (dim,) (copies through the device)."""
+        x = np.ascontiguousarray(x, dtype=np.complex128)
+        if x.shape != (self.dim(),):
+            raise ValueError(f"x must have shape ({self.dim()},)")
+        y = np.empty_like(x)
+        L.check(L.lib().ftlm_hip_apply_host(self._h, x.ctypes.data, y.ctypes.data))
+        return y
+
+    def apply_device(self, x, y, R: int = 1, dtype: int = L.FTLM_C128, stream: int | None = None) -> None:
+        """y = H x on device buffers (torch tensors or raw pointers), layout v[elem*R + r]."""
+        xp = x.data_ptr() if hasattr(x, "data_ptr") else int(x)
+        yp = y.data_ptr() if hasattr(y, "data_ptr") else int(y)
+        if stream is None:
+            try:
+                import torch
+
+                stream = torch.cuda.current_stream().cuda_stream
+            except Exception:  # noqa: BLE001
+                stream = 0
+        L.check(L.lib().ftlm_hip_apply(self._h, C.c_void_p(xp), C.c_void_p(yp), R, dtype, C.c_void_p(stream)))
+
+    # ---- Lanczos
+    def lanczos(self, max_steps: int, seed: int = 1, q0: np.ndarray | None = None):
+        """lanczos_tridiagonal (src/lanczos.cpp:125-209): (used, alpha, beta)."""
+        m = max(max_steps, 1)
+        a, b = np.zeros(m), np.zeros(m)
+        nb, used = C.c_int(0), C.c_int(0)
+        q = None if q0 is None else np.ascontiguousarray(q0, np.complex128)
+        L.check(L.lib().ftlm_hip_lanczos(self._h, max_steps, seed, L.ptr(q), a.ctypes.data, b.ctypes.data,
+                                         C.byref(nb), C.byref(used)))
+        return used.value, a[: used.value].copy(), b[: nb.value].copy()
+
+
+class FockBasis:
+    """(N_up, N_dn) sector basis, enumerated and ranked on the device (include/ftlm/fock_basis.hpp)."""
+
+    def __init__(self, n_sites: int, n_up: int, n_down: int):
+        self._n, self._nu, self._nd = n_sites, n_up, n_down
+        self._op = HubbardOperator._create(n_sites, n_up, n_down, None, None, 0.0, 0.0)
+        d, du, dd = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        L.check(L.lib().ftlm_hip_op_dim(self._op.handle, C.byref(d), C.byref(du), C.byref(dd)))
+        self._dim, self._du, self._dd = d.value, du.value, dd.value
+        self._masks = None
+
+    def n_sites(self) -> int:
+        return self._n
+
+    def n_up(self) -> int:
+        return self._nu
+
+    def n_down(self) -> int:
+        return self._nd
+
+    def dim(self) -> int:
+        return self._dim
+
+    def masks(self) -> tuple[np.ndarray, np.ndarray]:
+        if self._masks is None:
+            up = np.zeros(self._dim, np.uint64)
+            dn = np.zeros(self._dim, np.uint64)
+            L.check(L.lib().ftlm_hip_sector_masks(self._op.handle, 0, self._dim, up.ctypes.data, dn.ctypes.data))
+            self._masks = (up, dn)
+        return self._masks
+
+    def up_mask(self, idx: int) -> int:
+        if not 0 <= idx < self._dim:
+            raise IndexError("FockBasis index out of range")
+        return int(self.masks()[0][idx])
+
+    def down_mask(self, idx: int) -> int:
+        if not 0 <= idx < self._dim:
+            raise IndexError("FockBasis index out of range")
+        return int(self.masks()[1][idx])
+
+    def index_of(self, up, down):
+        """Index of (up, down) or -1; scalars or arrays (ranked on the device)."""
+        scalar = np.isscalar(up)
+        u = np.atleast_1d(np.asarray(up, dtype=np.uint64))
+        d = np.atleast_1d(np.asarray(down, dtype=np.uint64))
+        out = np.zeros(len(u), np.int64)
+        L.check(L.lib().ftlm_hip_index_of(self._op.handle, u.ctypes.data, d.ctypes.data, len(u), out.ctypes.data))
+        return int(out[0]) if scalar else out
+
+
+class HubbardOperator(_Op):
+    """Extended-Hubbard Hamiltonian of one (N_up, N_dn) sector resident on the device."""
+
+    @classmethod
+    def _create(cls, n_sites, n_up, n_dn, hops, pairs, U, V):
+        self = cls()
+        hops = np.zeros(0, L.HOP_DTYPE) if hops is None else np.ascontiguousarray(hops, L.HOP_DTYPE)
+        pairs = np.zeros(0, L.PAIR_DTYPE) if pairs is None else np.ascontiguousarray(pairs, L.PAIR_DTYPE)
+        L.check(L.lib().ftlm_hip_sector_create(n_sites, n_up, n_dn, hops.ctypes.data if len(hops) else None,
+                                               len(hops), pairs.ctypes.data if len(pairs) else None, len(pairs),
+                                               float(U), float(V), C.byref(self._h)))
+        return self
+
+    @classmethod
+    def from_params(cls, p: HubbardParams, n_up: int, n_dn: int) -> "HubbardOperator":
+        hops, pairs = build_hubbard_geometry(p)
+        return cls._create(p.Lx * p.Ly, n_up, n_dn, hops, pairs, p.U, p.V)
+
+
+_APPLY_CACHE: dict = {}
+
+
+def apply_extended_hubbard(p: HubbardParams, basis: FockBasis, hops, pairs, x: np.ndarray) -> np.ndarray:
+    """Drop-in of apply_extended_hubbard (src/hubbard_hamiltonian.cpp:51-108): returns y = H x.
+    The device operator for (basis, hops, pairs, U, V) is built once and cached."""
+    hops = np.ascontiguousarray(hops, L.HOP_DTYPE)
+    pairs = np.ascontiguousarray(pairs, L.PAIR_DTYPE)
+    key = (basis.n_sites(), basis.n_up(), basis.n_down(), hops.tobytes(), pairs.tobytes(), float(p.U), float(p.V))
+    op = _APPLY_CACHE.get(key)
+    if op is None:
+        if len(_APPLY_CACHE) > 16:
+            _APPLY_CACHE.clear()
+        op = HubbardOperator._create(basis.n_sites(), basis.n_up(), basis.n_down(), hops, pairs, p.U, p.V)
+        _APPLY_CACHE[key] = op
+    return op.apply(x)
+
+
+# ---------------------------------------------------------------- Lanczos / FTLM
+
+@dataclass
+class LanczosExtrema:
+    min_eval: float = 0.0
+    max_eval: float = 0.0
+    steps_used: int = 0
+
+
+def lanczos_tridiagonal(op: _Op, max_steps: int, seed: int, q0: np.ndarray | None = None):
+    """(used, alpha, beta) as src/lanczos.cpp:125-209; q0 is the optional supplied start vector."""
+    return op.lanczos(max_steps, seed, q0)
+
+
+def lanczos_extrema(op: _Op, max_steps: int, seed: int = 1) -> LanczosExtrema:
+    lo, hi, used = C.c_double(0), C.c_double(0), C.c_int(0)
+    L.check(L.lib().ftlm_hip_lanczos_extrema(op.handle, max_steps, seed, C.byref(lo), C.byref(hi), C.byref(used)))
+    return LanczosExtrema(lo.value, hi.value, used.value)
+
+
+def tridiagonal_extrema(alpha, beta, steps: int) -> tuple[float, float]:
+    a = np.ascontiguousarray(alpha[:steps], np.float64)
+    b = np.ascontiguousarray(beta, np.float64) if len(beta) else np.zeros(1)
+    lo, hi = C.c_double(0), C.c_double(0)
+    L.check(L.lib().ftlm_hip_tridiagonal_extrema(a.ctypes.data if steps > 0 else None, steps, b.ctypes.data,
+                                                 len(beta), C.byref(lo), C.byref(hi)))
+    return lo.value, hi.value
+
+
+@dataclass
+class FtlmParams:
+    """include/ftlm/ftlm_thermo.hpp:22-32, plus the device-path options."""
+
+    n_random: int = 16
+    lanczos_steps: int = 96
+    seed: int = 1
+    start: int = L.FTLM_START_GAUSS_MT
+    dtype: int = L.FTLM_C128
+    quad: int = L.FTLM_QUAD_REFERENCE
+    clamp_M_to_dim: bool = False
+
+    def _c(self, real: bool = False) -> L.FtlmOpts:
+        return L.FtlmOpts(self.n_random, self.lanczos_steps, self.seed, self.start, self.dtype, self.quad,
+                          int(self.clamp_M_to_dim), int(real))
+
+
+def _log_partition(op: _Op, betas, par: FtlmParams, real: bool):
+    scalar = np.isscalar(betas)
+    b = np.atleast_1d(np.asarray(betas, np.float64))
+    out = np.zeros(len(b))
+    opts = par._c(real)
+    L.check(L.lib().ftlm_hip_ftlm_log_partition(op.handle, b.ctypes.data, len(b), C.byref(opts), out.ctypes.data))
+    return float(out[0]) if scalar else out
+
+
+def ftlm_log_partition_complex(op: _Op, beta, par: FtlmParams = FtlmParams()):
+    """src/ftlm_thermo.cpp:213-256; `beta` may be an array (the Lanczos data are reused)."""
+    return _log_partition(op, beta, par, False)
+
+
+def ftlm_log_partition_real(op: _Op, beta, par: FtlmParams = FtlmParams()):
+    """src/ftlm_thermo.cpp:150-211 with the Re(H) apply of benchmarks/bench_ftlm_nmu_rect.cpp:151-159."""
+    return _log_partition(op, beta, par, True)
+
+
+@dataclass
+class RitzData:
+    ritz: np.ndarray     # [n_ops, R, M]
+    weight: np.ndarray   # [n_ops, R, M]
+    used: np.ndarray     # [n_ops, R]
+    dims: np.ndarray     # [n_ops]
+    alpha: np.ndarray | None = None
+    beta: np.ndarray | None = None
+
+
+def ftlm_ritz(ops, par: FtlmParams, real: bool = False, keep_tridiagonals: bool = False) -> RitzData:
+    n = len(ops)
+    R, M = max(1, par.n_random), max(1, par.lanczos_steps)
+    rz, wt = np.zeros((n, R, M)), np.zeros((n, R, M))
+    us = np.zeros((n, R), np.int32)
+    al = np.zeros((n, R, M)) if keep_tridiagonals else None
+    be = np.zeros((n, R, M)) if keep_tridiagonals else None
+    arr = (C.c_void_p * n)(*[o.handle.value for o in ops])
+    opts = par._c(real)
+    L.check(L.lib().ftlm_hip_ftlm_ritz(arr, n, C.byref(opts), rz.ctypes.data, wt.ctypes.data, us.ctypes.data,
+                                       L.ptr(al), L.ptr(be), None))
+    dims = np.array([o.dim() for o in ops], np.int64)
+    return RitzData(rz, wt, us, dims, al, be)
+
+
+def ftlm_thermo(rd: RitzData, betas, N=None, mus=None):
+    """Per-operator lnZ(beta), E(beta) and, with N and mus, the grand-canonical lnZ, <N>, E."""
+    n, R, M = rd.ritz.shape
+    b = np.ascontiguousarray(np.atleast_1d(betas), np.float64)
+    nb = len(b)
+    lnz, e = np.zeros((n, nb)), np.zeros((n, nb))
+    nmu = 0 if mus is None else len(mus)
+    mu = None if mus is None else np.ascontiguousarray(mus, np.float64)
+    Nn = None if N is None else np.ascontiguousarray(N, np.int32)
+    lg, ng, eg = (np.zeros((nb, max(nmu, 1))) for _ in range(3))
+    L.check(L.lib().ftlm_hip_ftlm_thermo(rd.ritz.ctypes.data, rd.weight.ctypes.data, rd.used.ctypes.data, n, R, M,
+                                         rd.dims.ctypes.data, L.ptr(Nn), b.ctypes.data, nb, L.ptr(mu), nmu,
+                                         lnz.ctypes.data, e.ctypes.data, lg.ctypes.data, ng.ctypes.data,
+                                         eg.ctypes.data))
+    out = {"lnZ": lnz, "E": e}
+    if nmu:
+        out.update({"lnZ_gc": lg, "N_gc": ng, "E_gc": eg})
+    return out
+
+
+class KrylovSession:
+    """R device-resident Lanczos recurrences advanced step by step (LanczosComplexWorkspace,
+    include/ftlm/lanczos.hpp:13-35, holding R samples)."""
+
+    def __init__(self, op: _Op, R: int, max_steps: int, dtype: int = L.FTLM_C128,
+                 start: int = L.FTLM_START_PHASE, seed: int = 1):
+        self._op = op  # keeps the operator alive
+        self.R, self.M = R, max_steps
+        self._h = C.c_void_p()
+        L.check(L.lib().ftlm_hip_krylov_create(op.handle, R, max_steps, dtype, start, seed, C.byref(self._h)))
+
+    def run(self, n_steps: int, stream: int | None = None) -> None:
+        if stream is None:
+            try:
+                import torch
+
+                stream = torch.cuda.current_stream().cuda_stream
+            except Exception:  # noqa: BLE001
+                stream = 0
+        L.check(L.lib().ftlm_hip_krylov_run(self._h, n_steps, C.c_void_p(stream)))
+
+    def result(self):
+        a, b = np.zeros((self.M, self.R)), np.zeros((self.M, self.R))
+        used, alive = np.zeros(self.R, np.int32), np.zeros(self.R, np.int32)
+        L.check(L.lib().ftlm_hip_krylov_result(self._h, a.ctypes.data, b.ctypes.data, used.ctypes.data,
+                                               alive.ctypes.data))
+        return a, b, used, alive
+
+    def nbytes(self) -> int:
+        v = C.c_int64(0)
+        L.check(L.lib().ftlm_hip_krylov_bytes(self._h, C.byref(v)))
+        return v.value
+
+    def close(self) -> None:
+        if self._h is not None and self._h.value and L._lib is not None:
+            L._lib.ftlm_hip_krylov_destroy(self._h)
+        self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+def quadrature_ritz(alpha, beta, quad: int = L.FTLM_QUAD_REFERENCE):
+    a = np.ascontiguousarray(alpha, np.float64)
+    b = np.ascontiguousarray(beta if len(beta) else np.zeros(1), np.float64)
+    n = len(a)
+    lam, w = np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    L.check(L.lib().ftlm_hip_quadrature_ritz(a.ctypes.data, n, b.ctypes.data, len(beta), quad, lam.ctypes.data,
+                                             w.ctypes.data))
+    return lam[:n], w[:n]
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    L.check(L.lib().ftlm_hip_device_count(C.byref(c)))
+    return c.value
+
+
+__all__ = [
+    "HubbardParams", "nsites", "version_string", "RectLattice", "build_hubbard_geometry", "FockBasis",
+    "HubbardOperator", "apply_extended_hubbard", "LanczosExtrema", "lanczos_tridiagonal", "lanczos_extrema",
+    "tridiagonal_extrema", "FtlmParams", "ftlm_log_partition_complex", "ftlm_log_partition_real", "RitzData",
+    "ftlm_ritz", "ftlm_thermo", "quadrature_ritz", "device_count", "KrylovSession",
+]

@@ -1,0 +1,180 @@
+// common.hpp -- shared host/device definitions of libftlm_hip.so (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ftlm_hip.h"
+
+namespace ftlm_hip {
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define FTLM_HIP_CHECK(expr)                                                                    \
+  do {                                                                                          \
+    hipError_t _e = (expr);                                                                     \
+    if (_e != hipSuccess)                                                                       \
+      throw ::ftlm_hip::Error(FTLM_EHIP, std::string(#expr " failed: ") + hipGetErrorString(_e) + \
+                                             " (" __FILE__ ":" + std::to_string(__LINE__) + ")"); \
+  } while (0)
+
+#define FTLM_REQUIRE(cond, code, msg)                       \
+  do {                                                      \
+    if (!(cond)) throw ::ftlm_hip::Error((code), (msg));    \
+  } while (0)
+
+// Runs `body`, converting exceptions into status codes + last-error text.
+template <class F>
+int guarded(F&& body) {
+  try {
+    body();
+    return FTLM_OK;
+  } catch (const Error& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_last_error("out of host memory");
+    return FTLM_ENOMEM;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return FTLM_EHIP;
+  }
+}
+
+// ------------------------------------------------------------------ device buffers
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  explicit DevBuf(size_t count) { alloc(count); }
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void alloc(size_t count) {
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      throw Error(e == hipErrorOutOfMemory ? FTLM_ENOMEM : FTLM_EHIP,
+                  "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed: " + hipGetErrorString(e));
+    }
+    n = count;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void upload(const T* h, size_t count, hipStream_t s = nullptr) {
+    if (count) FTLM_HIP_CHECK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+  }
+  void upload(const std::vector<T>& v, hipStream_t s = nullptr) { upload(v.data(), v.size(), s); }
+};
+
+// ------------------------------------------------------------------ combinatorics
+constexpr int kMaxSites = 32;
+constexpr int kTargetBits = 20;  // pull-hop entry: target index bits
+constexpr uint32_t kTargetMask = (1u << kTargetBits) - 1u;
+constexpr int kBondBits = 11;    // bond index bits
+constexpr uint32_t kBondMask = (1u << kBondBits) - 1u;
+
+// Binomial table C[a][b], a,b <= 32 (values < 2^63).
+struct Binom {
+  int64_t c[kMaxSites + 2][kMaxSites + 2];
+};
+
+Binom host_binomials();
+int64_t binom(int a, int b);
+
+// ------------------------------------------------------------------ operator (sector / k-block)
+enum OpKind { kSector = 0, kKBlock = 1 };
+
+// Device view of a full (N_up, N_dn) sector with its per-spin pull-hop tables.
+// Hop entry (uint32): bits 0..19 target string rank, 20..30 bond id, 31 own-spin JW parity.
+struct SectorView {
+  int64_t dim;
+  int Dup, Ddn;
+  int n_sites;
+  double U, V;
+  const uint32_t* up_mask;   // [Dup]
+  const uint32_t* dn_mask;   // [Ddn]
+  const int32_t* off_up;     // [Dup + 1]
+  const uint32_t* ent_up;    // [nnz_up]
+  const int32_t* off_dn;     // [Ddn + 1]
+  const uint32_t* ent_dn;    // [nnz_dn]
+  const double2* coef_up;    // [nbond_up] summed directed-bond coefficient
+  const double2* coef_dn;    // [nbond_dn]
+  const uint32_t* xmask_up;  // [nbond_up] down-string sites inside the JW string of an up bond
+  const uint32_t* xmask_dn;  // [nbond_dn] up-string sites inside the JW string of a down bond
+  const int32_t* nuu;        // [Dup]  sum over NN pairs of u_i u_j
+  const int32_t* ndd;        // [Ddn]  sum over NN pairs of d_i d_j
+  const uint4* planes;       // [Dup]  bit planes of nb_u(s) = sum of u over pair partners of s
+};
+
+// Device view of an orbit-representative momentum block (filled by orbit.hip).
+struct KBlockView;
+
+struct OpBase {
+  int kind = kSector;
+  int device = 0;
+  int64_t dim = 0;
+  virtual ~OpBase() = default;
+};
+
+}  // namespace ftlm_hip
+
+// The opaque C handle.
+struct ftlm_op : ftlm_hip::OpBase {};
+
+namespace ftlm_hip {
+
+struct Sector : ftlm_op {
+  int n_sites = 0, n_up = 0, n_dn = 0;
+  int Dup = 0, Ddn = 0;
+  double U = 0.0, V = 0.0;
+  int nbond[2] = {0, 0};
+  int64_t nnz[2] = {0, 0};
+  bool real_h = true;  // every summed bond coefficient has zero imaginary part
+  DevBuf<uint32_t> up_mask, dn_mask;
+  DevBuf<int32_t> off[2];
+  DevBuf<uint32_t> ent[2];
+  DevBuf<double2> coef[2];
+  DevBuf<uint32_t> xmask[2];
+  DevBuf<int32_t> nuu, ndd;
+  DevBuf<uint4> planes;
+  SectorView view() const;
+};
+
+// ------------------------------------------------------------------ launches
+inline unsigned grid_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
+
+void check_launch(const char* what);
+
+// Block-vector H*V dispatch (hv.hip).  xs: optional per-vector input scale [R] on the device
+// (y = H (xs .* x)); dotp: optional partial sums of <xs.*x | y> per block, layout
+// [nblocks][R] double2; returns the number of partial blocks written.
+// force_real: use only the real parts of the bond coefficients (the reference's
+// ftlm_log_partition_real applies Re(H) to the real and imaginary parts separately).
+int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
+             hipStream_t s, bool force_real = false);
+int hv_partial_blocks(const ftlm_op* op);
+
+}  // namespace ftlm_hip

@@ -1,0 +1,684 @@
+// lanczos.hip -- device-resident block Lanczos recurrences and the FTLM drivers.
+//
+// Replaces lanczos_tridiagonal (src/lanczos.cpp:125-209), lanczos_extrema (src/lanczos.cpp:211-231)
+// and the sample loop of ftlm_log_partition_complex / _real (src/ftlm_thermo.cpp:150-256).
+//
+// R independent recurrences (one per random start vector) advance together as a block so that
+// each H*V launch shares its decode work across the R vectors.  Per step and vector r:
+//     w = H q;  z = <q|w>;  alpha = Re z;  w -= z q;  w -= beta_prev q_prev;
+//     if |w|^2 < 1e-24: stop r;  beta = |w|;  q_prev = q;  q = w / beta
+// exactly as the reference (no reorthogonalisation; complex z is subtracted).  Dots and norms
+// are FP64 (per-block partials, then a fixed-order reduction, so results are deterministic).
+// Normalisation is lazy: buffer A holds w and a per-vector scale sa = 1/beta, applied inside the
+// H*V and update kernels, which saves one read+write of the block per step.  Three buffers
+// rotate (A, B, C) -> (C, A, B); the scalar bookkeeping (alpha, beta, scales, stop flags) stays
+// on the device, so the whole step sequence can be recorded into a hipGraph.
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "common.hpp"
+
+namespace ftlm_hip {
+
+void quadrature_device(const double* d_alpha, const double* d_beta, const int* d_n, const int* d_nb, int stride,
+                       int nsamp, int max_n, int quad, int cap_fixed, double tol, int with_weights, double* d_ritz,
+                       double* d_weight, int out_stride, hipStream_t st);
+void thermo_device(const double* d_ritz, const double* d_weight, const int* d_used, int n_ops, int R, int M,
+                   const int64_t* d_dims, const int* d_N, const double* d_betas, int nb, const double* d_mus, int nmu,
+                   double* d_lnZ, double* d_E, double* d_lnZgc, double* d_Ngc, double* d_Egc, hipStream_t st);
+
+template <typename T>
+struct V2Of;
+template <>
+struct V2Of<double> {
+  using t = double2;
+};
+template <>
+struct V2Of<float> {
+  using t = float2;
+};
+
+constexpr int kRedBlock = 256;
+constexpr int kUpdBlock = 256;
+constexpr int kUpdMaxGrid = 4096;
+
+__device__ __forceinline__ double wsum(double v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Block r: z_r = sum_b dotp[b][r]; alpha[k][r] = Re z_r for live vectors.
+__global__ __launch_bounds__(kRedBlock) void k_reduce_alpha(const double2* __restrict__ dotp, int nblk, int R,
+                                                            const int* __restrict__ alive, double2* __restrict__ z,
+                                                            double* __restrict__ alpha, int k) {
+  const int r = blockIdx.x;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock) {
+    const double2 v = dotp[static_cast<int64_t>(i) * R + r];
+    a += v.x;
+    b += v.y;
+  }
+  __shared__ double sa[kRedBlock / 64], sb[kRedBlock / 64];
+  a = wsum(a);
+  b = wsum(b);
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = a;
+    sb[threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0;
+    for (int w = 0; w < kRedBlock / 64; ++w) {
+      x += sa[w];
+      y += sb[w];
+    }
+    if (alive[r]) {
+      z[r] = make_double2(x, y);
+      alpha[static_cast<int64_t>(k) * R + r] = x;
+    } else {
+      z[r] = make_double2(0.0, 0.0);
+    }
+  }
+}
+
+// C_r <- C_r - z_r (sa_r A_r) - bprev_r (sb_r B_r);  partial |C_r|^2.
+template <typename T, int RB>
+__global__ __launch_bounds__(kUpdBlock) void k_update(int64_t dim, int R, int r0, typename V2Of<T>::t* __restrict__ C,
+                                                      const typename V2Of<T>::t* __restrict__ A,
+                                                      const typename V2Of<T>::t* __restrict__ B,
+                                                      const double2* __restrict__ z, const double* __restrict__ sa,
+                                                      const double* __restrict__ sb, const double* __restrict__ bprev,
+                                                      const int* __restrict__ alive, double* __restrict__ normp,
+                                                      const int use_prev) {
+  using V2 = typename V2Of<T>::t;
+  double zr[RB], zi[RB], qa[RB], qb[RB], nrm[RB];
+  bool live[RB];
+#pragma unroll
+  for (int r = 0; r < RB; ++r) {
+    live[r] = alive[r0 + r] != 0;
+    zr[r] = z[r0 + r].x;
+    zi[r] = z[r0 + r].y;
+    qa[r] = sa[r0 + r];
+    qb[r] = bprev[r0 + r] * sb[r0 + r];
+    nrm[r] = 0.0;
+  }
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * kUpdBlock + threadIdx.x; e < dim;
+       e += static_cast<int64_t>(gridDim.x) * kUpdBlock) {
+    const int64_t o = e * R + r0;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const V2 c = C[o + r], a = A[o + r];
+      double wr = static_cast<double>(c.x), wi = static_cast<double>(c.y);
+      const double ar = qa[r] * static_cast<double>(a.x), ai = qa[r] * static_cast<double>(a.y);
+      // w -= z * q  (complex product, then subtract), then w -= beta_prev * q_prev
+      wr -= zr[r] * ar - zi[r] * ai;
+      wi -= zr[r] * ai + zi[r] * ar;
+      if (use_prev) {  // the reference skips this term at k = 0 (src/lanczos.cpp:186-190)
+        const V2 b = B[o + r];
+        wr -= qb[r] * static_cast<double>(b.x);
+        wi -= qb[r] * static_cast<double>(b.y);
+      }
+      V2 out;
+      out.x = static_cast<T>(live[r] ? wr : 0.0);
+      out.y = static_cast<T>(live[r] ? wi : 0.0);
+      C[o + r] = out;
+      const double sr = static_cast<double>(out.x), si = static_cast<double>(out.y);
+      nrm[r] += sr * sr + si * si;
+    }
+  }
+  __shared__ double red[kUpdBlock / 64][RB];
+#pragma unroll
+  for (int r = 0; r < RB; ++r) {
+    const double v = wsum(nrm[r]);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][r] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < RB) {
+    double s = 0.0;
+    for (int w = 0; w < kUpdBlock / 64; ++w) s += red[w][threadIdx.x];
+    normp[static_cast<int64_t>(blockIdx.x) * R + r0 + threadIdx.x] = s;
+  }
+}
+
+// Block r: nw2 = sum_b normp[b][r]; stop or record beta and rotate the scales.
+__global__ __launch_bounds__(kRedBlock) void k_reduce_beta(const double* __restrict__ normp, int nblk, int R,
+                                                           int* __restrict__ alive, int* __restrict__ used,
+                                                           double* __restrict__ beta, double* __restrict__ sa,
+                                                           double* __restrict__ sb, double* __restrict__ bprev, int k) {
+  const int r = blockIdx.x;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock) a += normp[static_cast<int64_t>(i) * R + r];
+  __shared__ double sh[kRedBlock / 64];
+  a = wsum(a);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double nw2 = 0.0;
+    for (int w = 0; w < kRedBlock / 64; ++w) nw2 += sh[w];
+    if (alive[r]) {
+      used[r] = k + 1;
+      if (nw2 < 1e-24) {
+        alive[r] = 0;
+        sa[r] = 0.0;
+        sb[r] = 0.0;
+        bprev[r] = 0.0;
+      } else {
+        const double b = sqrt(nw2);
+        beta[static_cast<int64_t>(k) * R + r] = b;
+        sb[r] = sa[r];
+        sa[r] = 1.0 / b;
+        bprev[r] = b;
+      }
+    }
+  }
+}
+
+// sa_r = 1/|q_r| (or, when |q_r| < 1e-18, q_r[0] = 1 and sa_r = 1: src/lanczos.cpp:170-175).
+__global__ void k_start_norm(const double* __restrict__ normp, int nblk, int R, double* sa) {
+  const int r = blockIdx.x;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock) a += normp[static_cast<int64_t>(i) * R + r];
+  __shared__ double sh[kRedBlock / 64];
+  a = wsum(a);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int w = 0; w < kRedBlock / 64; ++w) s += sh[w];
+    const double n0 = sqrt(s);
+    sa[r] = n0 < 1e-18 ? -1.0 : 1.0 / n0;  // -1 flags the unit-vector fallback
+  }
+}
+
+template <typename T>
+__global__ void k_norm_partial(int64_t dim, int R, const typename V2Of<T>::t* __restrict__ q, double* normp) {
+  // one partial per (block, r); R <= 32
+  double acc[32];
+  for (int r = 0; r < R; ++r) acc[r] = 0.0;
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * kUpdBlock + threadIdx.x; e < dim;
+       e += static_cast<int64_t>(gridDim.x) * kUpdBlock)
+    for (int r = 0; r < R; ++r) {
+      const auto v = q[e * R + r];
+      acc[r] += static_cast<double>(v.x) * static_cast<double>(v.x) + static_cast<double>(v.y) * static_cast<double>(v.y);
+    }
+  __shared__ double red[kUpdBlock / 64];
+  for (int r = 0; r < R; ++r) {
+    const double v = wsum(acc[r]);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0.0;
+      for (int w = 0; w < kUpdBlock / 64; ++w) s += red[w];
+      normp[static_cast<int64_t>(blockIdx.x) * R + r] = s;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename T>
+__global__ void k_start_fallback(typename V2Of<T>::t* q, int R, double* sa) {
+  const int r = threadIdx.x;
+  if (r < R && sa[r] < 0.0) {
+    q[r].x = static_cast<T>(1.0);
+    q[r].y = static_cast<T>(0.0);
+    sa[r] = 1.0;
+  }
+}
+
+// Random-phase start vectors e^{i 2 pi u}, u from a counter-based hash of (seed, sample, index).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void k_phase_start(int64_t dim, int R, uint64_t seed, const uint32_t* sample_ids, typename V2Of<T>::t* q) {
+  for (int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; e < dim;
+       e += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    for (int r = 0; r < R; ++r) {
+      const uint64_t h = mix64(mix64(seed * 0x9e3779b97f4a7c15ull + sample_ids[r]) + static_cast<uint64_t>(e));
+      const double th = 6.283185307179586 * static_cast<double>(h >> 11) * 0x1.0p-53;
+      double s, c;
+      sincos(th, &s, &c);
+      q[e * R + r].x = static_cast<T>(c);
+      q[e * R + r].y = static_cast<T>(s);
+    }
+}
+
+// ---------------------------------------------------------------- host engine
+
+struct Krylov {
+  const ftlm_op* op = nullptr;
+  int R = 1, M = 1, dtype = FTLM_C128;
+  int64_t dim = 0;
+  size_t vbytes = 0;  // bytes of one block of R vectors
+  DevBuf<unsigned char> buf[3];
+  DevBuf<double> sa, sb, bprev, alpha, beta;
+  DevBuf<double2> z, dotp;
+  DevBuf<double> normp;
+  DevBuf<int> alive, used;
+  int nblk_hv = 0, nblk_upd = 0;
+  bool force_real = false;
+
+  void init(const ftlm_op* o, int R_, int M_, int dt) {
+    op = o;
+    R = R_;
+    M = M_;
+    dtype = dt;
+    dim = o->dim;
+    vbytes = static_cast<size_t>(dim) * R * (dt == FTLM_C64 ? sizeof(float2) : sizeof(double2));
+    for (auto& b : buf) b.alloc(vbytes);
+    sa.alloc(R);
+    sb.alloc(R);
+    bprev.alloc(R);
+    alpha.alloc(static_cast<size_t>(M) * R);
+    beta.alloc(static_cast<size_t>(M) * R);
+    z.alloc(R);
+    alive.alloc(R);
+    used.alloc(R);
+    nblk_hv = hv_partial_blocks(o);
+    nblk_upd = static_cast<int>(std::min<int64_t>(kUpdMaxGrid, (dim + kUpdBlock - 1) / kUpdBlock));
+    dotp.alloc(static_cast<size_t>(nblk_hv) * R);
+    normp.alloc(static_cast<size_t>(std::max(nblk_hv, nblk_upd)) * R);
+  }
+
+  void launch_update(void* C, const void* A, const void* B, int use_prev, hipStream_t st) {
+    const dim3 g(nblk_upd), b(kUpdBlock);
+    for (int r0 = 0; r0 < R;) {
+      int rb = 1;
+      const int maxrb = dtype == FTLM_C64 ? 16 : 8;
+      while (rb * 2 <= maxrb && r0 + rb * 2 <= R) rb *= 2;
+#define FTLM_UPD(TT, RBV)                                                                                     \
+  hipLaunchKernelGGL((k_update<TT, RBV>), g, b, 0, st, dim, R, r0, static_cast<V2Of<TT>::t*>(C),              \
+                     static_cast<const V2Of<TT>::t*>(A), static_cast<const V2Of<TT>::t*>(B), z.p, sa.p, sb.p, \
+                     bprev.p, alive.p, normp.p, use_prev)
+      if (dtype == FTLM_C64) {
+        switch (rb) {
+          case 16: FTLM_UPD(float, 16); break;
+          case 8: FTLM_UPD(float, 8); break;
+          case 4: FTLM_UPD(float, 4); break;
+          case 2: FTLM_UPD(float, 2); break;
+          default: FTLM_UPD(float, 1); break;
+        }
+      } else {
+        switch (rb) {
+          case 8: FTLM_UPD(double, 8); break;
+          case 4: FTLM_UPD(double, 4); break;
+          case 2: FTLM_UPD(double, 2); break;
+          default: FTLM_UPD(double, 1); break;
+        }
+      }
+#undef FTLM_UPD
+      r0 += rb;
+    }
+    check_launch("k_update");
+  }
+
+  // Prepare the start block in buf[0]: normalise lazily and reset the recurrence state.
+  void start(hipStream_t st) {
+    std::vector<int> one(static_cast<size_t>(R), 1), zero(static_cast<size_t>(R), 0);
+    std::vector<double> dz(static_cast<size_t>(R), 0.0);
+    alive.upload(one, st);
+    used.upload(zero, st);
+    sb.upload(dz, st);
+    bprev.upload(dz, st);
+    FTLM_HIP_CHECK(hipMemsetAsync(alpha.p, 0, alpha.n * sizeof(double), st));
+    FTLM_HIP_CHECK(hipMemsetAsync(beta.p, 0, beta.n * sizeof(double), st));
+    const dim3 g(nblk_upd), b(kUpdBlock);
+    if (dtype == FTLM_C64)
+      hipLaunchKernelGGL(k_norm_partial<float>, g, b, 0, st, dim, R, static_cast<const float2*>((void*)buf[0].p), normp.p);
+    else
+      hipLaunchKernelGGL(k_norm_partial<double>, g, b, 0, st, dim, R, static_cast<const double2*>((void*)buf[0].p),
+                         normp.p);
+    hipLaunchKernelGGL(k_start_norm, dim3(R), dim3(kRedBlock), 0, st, normp.p, nblk_upd, R, sa.p);
+    if (dtype == FTLM_C64)
+      hipLaunchKernelGGL(k_start_fallback<float>, dim3(1), dim3(64), 0, st, static_cast<float2*>((void*)buf[0].p), R, sa.p);
+    else
+      hipLaunchKernelGGL(k_start_fallback<double>, dim3(1), dim3(64), 0, st, static_cast<double2*>((void*)buf[0].p), R,
+                         sa.p);
+    check_launch("start");
+  }
+
+  // Host-computed start norms (bit-identical to the reference's sequential sum).
+  void start_host_norm(const std::vector<double>& inv_norm, hipStream_t st) {
+    std::vector<int> one(static_cast<size_t>(R), 1), zero(static_cast<size_t>(R), 0);
+    std::vector<double> dz(static_cast<size_t>(R), 0.0);
+    alive.upload(one, st);
+    used.upload(zero, st);
+    sb.upload(dz, st);
+    bprev.upload(dz, st);
+    sa.upload(inv_norm, st);
+    FTLM_HIP_CHECK(hipMemsetAsync(alpha.p, 0, alpha.n * sizeof(double), st));
+    FTLM_HIP_CHECK(hipMemsetAsync(beta.p, 0, beta.n * sizeof(double), st));
+  }
+
+  // One Lanczos step k with (A, B, C) = buffers (a, b, c).
+  void step(int k, int a, int b, int c, hipStream_t st) {
+    hv_apply(op, buf[a].p, buf[c].p, R, dtype, sa.p, dotp.p, st, force_real);
+    hipLaunchKernelGGL(k_reduce_alpha, dim3(R), dim3(kRedBlock), 0, st, dotp.p, nblk_hv, R, alive.p, z.p, alpha.p, k);
+    launch_update(buf[c].p, buf[a].p, buf[b].p, k > 0 ? 1 : 0, st);
+    hipLaunchKernelGGL(k_reduce_beta, dim3(R), dim3(kRedBlock), 0, st, normp.p, nblk_upd, R, alive.p, used.p, beta.p,
+                       sa.p, sb.p, bprev.p, k);
+    check_launch("lanczos step");
+  }
+
+  // Steps [k0, k1); buffer roles rotate (A,B,C) -> (C,A,B) every step, starting from A = buf[0].
+  void run(int k0, int k1, hipStream_t st) {
+    for (int k = k0; k < k1; ++k) {
+      const int a = (3 - k % 3) % 3;  // k=0 -> 0, k=1 -> 2, k=2 -> 1
+      const int b = (a + 1) % 3, c = (a + 2) % 3;
+      step(k, a, b, c, st);
+    }
+  }
+};
+
+// Reference start vector: mt19937(seed) + normal(0,1), (re, im) per index.
+void gauss_mt(int64_t dim, unsigned seed, std::vector<std::complex<double>>* q) {
+  std::mt19937 rng(seed);
+  std::normal_distribution<double> g(0.0, 1.0);
+  q->resize(static_cast<size_t>(dim));
+  for (int64_t i = 0; i < dim; ++i) {
+    const double re = g(rng);
+    const double im = g(rng);
+    (*q)[static_cast<size_t>(i)] = std::complex<double>(re, im);
+  }
+}
+
+double host_norm(const std::vector<std::complex<double>>& q) {
+  double s = 0.0;
+  for (const auto& z : q) s += std::norm(z);
+  return std::sqrt(s);
+}
+
+// Fill buf[0] with R start vectors.  seeds[r] per sample.
+void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const double* q0, hipStream_t st) {
+  const int R = K.R;
+  const int64_t dim = K.dim;
+  if (start == FTLM_START_PHASE && q0 == nullptr) {
+    std::vector<uint32_t> ids(static_cast<size_t>(R));
+    for (int r = 0; r < R; ++r) ids[r] = seeds[r];
+    DevBuf<uint32_t> dids(static_cast<size_t>(R));
+    dids.upload(ids, st);
+    const unsigned g = static_cast<unsigned>(std::min<int64_t>(8192, (dim + 255) / 256));
+    if (K.dtype == FTLM_C64)
+      hipLaunchKernelGGL(k_phase_start<float>, dim3(g), dim3(256), 0, st, dim, R, 0x5eedull, dids.p,
+                         static_cast<float2*>((void*)K.buf[0].p));
+    else
+      hipLaunchKernelGGL(k_phase_start<double>, dim3(g), dim3(256), 0, st, dim, R, 0x5eedull, dids.p,
+                         static_cast<double2*>((void*)K.buf[0].p));
+    check_launch("k_phase_start");
+    K.start(st);
+    FTLM_HIP_CHECK(hipStreamSynchronize(st));
+    return;
+  }
+  // host-generated (reference mt19937 Gaussian or caller-supplied), interleaved [dim][R]
+  std::vector<double> inv(static_cast<size_t>(R));
+  const size_t es = K.dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2);
+  std::vector<unsigned char> host(static_cast<size_t>(dim) * R * es);
+  std::vector<std::complex<double>> q;
+  bool fallback = false;
+  for (int r = 0; r < R; ++r) {
+    if (q0 != nullptr) {
+      q.assign(reinterpret_cast<const std::complex<double>*>(q0) + static_cast<int64_t>(r) * dim,
+               reinterpret_cast<const std::complex<double>*>(q0) + static_cast<int64_t>(r + 1) * dim);
+    } else {
+      gauss_mt(dim, seeds[r], &q);
+    }
+    const double n0 = host_norm(q);
+    if (n0 < 1e-18) {
+      q[0] = std::complex<double>(1.0, 0.0);
+      inv[r] = 1.0;
+      fallback = true;
+    } else {
+      inv[r] = 1.0 / n0;
+    }
+    for (int64_t i = 0; i < dim; ++i) {
+      if (K.dtype == FTLM_C64) {
+        float2 v = make_float2(static_cast<float>(q[i].real()), static_cast<float>(q[i].imag()));
+        std::memcpy(&host[(static_cast<size_t>(i) * R + r) * es], &v, es);
+      } else {
+        double2 v = make_double2(q[i].real(), q[i].imag());
+        std::memcpy(&host[(static_cast<size_t>(i) * R + r) * es], &v, es);
+      }
+    }
+  }
+  (void)fallback;
+  FTLM_HIP_CHECK(hipMemcpyAsync(K.buf[0].p, host.data(), host.size(), hipMemcpyHostToDevice, st));
+  if (K.dtype == FTLM_C64) {
+    // complex64 storage: recompute the norm of the stored (rounded) vectors on the device
+    K.start(st);
+  } else {
+    K.start_host_norm(inv, st);
+  }
+  FTLM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void check_opts(const ftlm_ftlm_opts_t* o) {
+  FTLM_REQUIRE(o != nullptr, FTLM_EINVAL, "null options");
+  FTLM_REQUIRE(o->dtype == FTLM_C128 || o->dtype == FTLM_C64, FTLM_EINVAL, "bad dtype");
+  FTLM_REQUIRE(o->start == FTLM_START_GAUSS_MT || o->start == FTLM_START_PHASE, FTLM_EINVAL, "bad start");
+  FTLM_REQUIRE(o->quad == FTLM_QUAD_REFERENCE || o->quad == FTLM_QUAD_CONVERGED, FTLM_EINVAL, "bad quad");
+}
+
+
+}  // namespace ftlm_hip
+
+struct ftlm_krylov {
+  ftlm_hip::Krylov K;
+  int steps_done = 0;
+};
+
+using namespace ftlm_hip;
+
+extern "C" {
+
+int ftlm_hip_krylov_create(const ftlm_op_t* op, int R, int max_steps, int dtype, int start, unsigned seed,
+                           ftlm_krylov_t** out) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && out != nullptr, FTLM_EINVAL, "null argument");
+    *out = nullptr;
+    FTLM_REQUIRE(R >= 1 && R <= 32 && max_steps >= 1, FTLM_EINVAL, "R in [1,32], max_steps >= 1");
+    FTLM_REQUIRE(dtype == FTLM_C128 || dtype == FTLM_C64, FTLM_EINVAL, "bad dtype");
+    FTLM_REQUIRE(start == FTLM_START_GAUSS_MT || start == FTLM_START_PHASE, FTLM_EINVAL, "bad start");
+    FTLM_HIP_CHECK(hipSetDevice(op->device));
+    auto s = std::make_unique<ftlm_krylov>();
+    s->K.init(op, R, max_steps, dtype);
+    std::vector<unsigned> seeds(static_cast<size_t>(R));
+    for (int r = 0; r < R; ++r) seeds[r] = seed + static_cast<unsigned>(r) * 100003u;
+    fill_start(s->K, start, seeds, nullptr, nullptr);
+    *out = s.release();
+  });
+}
+
+int ftlm_hip_krylov_run(ftlm_krylov_t* k, int n_steps, void* stream) {
+  return guarded([&] {
+    FTLM_REQUIRE(k != nullptr && n_steps >= 0, FTLM_EINVAL, "bad argument");
+    const int k1 = std::min(k->K.M, k->steps_done + n_steps);
+    k->K.run(k->steps_done, k1, static_cast<hipStream_t>(stream));
+    k->steps_done = k1;
+  });
+}
+
+int ftlm_hip_krylov_result(ftlm_krylov_t* k, double* alpha, double* beta, int* used, int* alive) {
+  return guarded([&] {
+    FTLM_REQUIRE(k != nullptr, FTLM_EINVAL, "null session");
+    FTLM_HIP_CHECK(hipDeviceSynchronize());
+    const size_t n = static_cast<size_t>(k->K.M) * k->K.R;
+    if (alpha) FTLM_HIP_CHECK(hipMemcpy(alpha, k->K.alpha.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (beta) FTLM_HIP_CHECK(hipMemcpy(beta, k->K.beta.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    if (used) FTLM_HIP_CHECK(hipMemcpy(used, k->K.used.p, k->K.R * sizeof(int), hipMemcpyDeviceToHost));
+    if (alive) FTLM_HIP_CHECK(hipMemcpy(alive, k->K.alive.p, k->K.R * sizeof(int), hipMemcpyDeviceToHost));
+  });
+}
+
+int ftlm_hip_krylov_bytes(const ftlm_krylov_t* k, int64_t* bytes) {
+  return guarded([&] {
+    FTLM_REQUIRE(k != nullptr && bytes != nullptr, FTLM_EINVAL, "null argument");
+    *bytes = static_cast<int64_t>(3 * k->K.vbytes + k->K.dotp.n * sizeof(double2) + k->K.normp.n * sizeof(double));
+  });
+}
+
+int ftlm_hip_krylov_destroy(ftlm_krylov_t* k) {
+  return guarded([&] { delete k; });
+}
+
+int ftlm_hip_lanczos(const ftlm_op_t* op, int max_steps, unsigned seed, const double* q0, double* alpha, double* beta,
+                     int* n_beta, int* used) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && alpha != nullptr && beta != nullptr && n_beta && used, FTLM_EINVAL, "null argument");
+    *used = 0;
+    *n_beta = 0;
+    if (op->dim <= 0 || max_steps <= 0) return;
+    FTLM_HIP_CHECK(hipSetDevice(op->device));
+    Krylov K;
+    K.init(op, 1, max_steps, FTLM_C128);
+    fill_start(K, FTLM_START_GAUSS_MT, {seed}, q0, nullptr);
+    K.run(0, max_steps, nullptr);
+    int u = 0;
+    FTLM_HIP_CHECK(hipMemcpy(&u, K.used.p, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<double> a(static_cast<size_t>(max_steps)), b(static_cast<size_t>(max_steps));
+    FTLM_HIP_CHECK(hipMemcpy(a.data(), K.alpha.p, sizeof(double) * max_steps, hipMemcpyDeviceToHost));
+    FTLM_HIP_CHECK(hipMemcpy(b.data(), K.beta.p, sizeof(double) * max_steps, hipMemcpyDeviceToHost));
+    int alive = 0;
+    FTLM_HIP_CHECK(hipMemcpy(&alive, K.alive.p, sizeof(int), hipMemcpyDeviceToHost));
+    *used = u;
+    *n_beta = alive ? u : u - 1;  // a breakdown step records alpha but not beta
+    std::copy(a.begin(), a.begin() + u, alpha);
+    std::copy(b.begin(), b.begin() + *n_beta, beta);
+  });
+}
+
+int ftlm_hip_lanczos_extrema(const ftlm_op_t* op, int max_steps, unsigned seed, double* min_eval, double* max_eval,
+                             int* steps_used) {
+  std::vector<double> a(static_cast<size_t>(std::max(max_steps, 1))), b(static_cast<size_t>(std::max(max_steps, 1)));
+  int nb = 0, u = 0;
+  int rc = ftlm_hip_lanczos(op, max_steps, seed, nullptr, a.data(), b.data(), &nb, &u);
+  if (rc != FTLM_OK) return rc;
+  if (min_eval) *min_eval = 0.0;
+  if (max_eval) *max_eval = 0.0;
+  if (steps_used) *steps_used = u;
+  if (u == 0) return FTLM_OK;
+  double lo = 0.0, hi = 0.0;
+  rc = ftlm_hip_tridiagonal_extrema(a.data(), u, b.data(), u >= 2 ? u - 1 : 0, &lo, &hi);
+  if (rc != FTLM_OK) return rc;
+  if (min_eval) *min_eval = lo;
+  if (max_eval) *max_eval = hi;
+  return FTLM_OK;
+}
+
+int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, double* ritz, double* weight,
+                       int* used, double* alpha_out, double* beta_out, void* stream) {
+  return guarded([&] {
+    check_opts(opts);
+    FTLM_REQUIRE(ops != nullptr && n_ops >= 1 && ritz && weight && used, FTLM_EINVAL, "null argument");
+    const int R = std::max(1, opts->R), M = std::max(1, opts->M);
+    FTLM_REQUIRE(R <= 32, FTLM_EUNSUPPORTED, "at most 32 samples per operator block");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    for (int io = 0; io < n_ops; ++io) {
+      const ftlm_op* op0 = ops[io];
+      FTLM_REQUIRE(op0 != nullptr, FTLM_EINVAL, "null operator");
+      FTLM_HIP_CHECK(hipSetDevice(op0->device));
+      const ftlm_op* op = op0;
+      const int64_t dim = op->dim;
+      double* rz = ritz + static_cast<int64_t>(io) * R * M;
+      double* wt = weight + static_cast<int64_t>(io) * R * M;
+      int* us = used + static_cast<int64_t>(io) * R;
+      std::fill(rz, rz + static_cast<int64_t>(R) * M, 0.0);
+      std::fill(wt, wt + static_cast<int64_t>(R) * M, 0.0);
+      if (dim <= 0) {
+        std::fill(us, us + R, 0);
+        continue;
+      }
+      int Mop = M;
+      if (opts->clamp_M_to_dim) Mop = static_cast<int>(std::min<int64_t>(M, dim));
+      if (dim == 1) {
+        // src/ftlm_thermo.cpp:155-160 / 219-224: a single state, lnZ = -beta H00
+        DevBuf<double2> x(1), y(1);
+        const double2 one = make_double2(1.0, 0.0);
+        x.upload(&one, 1, st);
+        hv_apply(op, x.p, y.p, 1, FTLM_C128, nullptr, nullptr, st, opts->real_variant != 0);
+        double2 h;
+        FTLM_HIP_CHECK(hipMemcpyAsync(&h, y.p, sizeof(double2), hipMemcpyDeviceToHost, st));
+        FTLM_HIP_CHECK(hipStreamSynchronize(st));
+        for (int r = 0; r < R; ++r) {
+          rz[static_cast<int64_t>(r) * M] = h.x;
+          wt[static_cast<int64_t>(r) * M] = 1.0;
+          us[r] = 1;
+        }
+        continue;
+      }
+      Krylov K;
+      K.init(op, R, Mop, opts->dtype);
+      K.force_real = opts->real_variant != 0;
+      std::vector<unsigned> seeds(static_cast<size_t>(R));
+      for (int r = 0; r < R; ++r) seeds[r] = opts->seed + static_cast<unsigned>(r) * 100003u;
+      fill_start(K, opts->start, seeds, nullptr, st);
+      K.run(0, Mop, st);
+      // quadrature per sample: alpha/beta are [Mop][R] (stride R)
+      DevBuf<int> nb(static_cast<size_t>(R));
+      std::vector<int> hu(static_cast<size_t>(R)), ha(static_cast<size_t>(R));
+      FTLM_HIP_CHECK(hipMemcpyAsync(hu.data(), K.used.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+      FTLM_HIP_CHECK(hipMemcpyAsync(ha.data(), K.alive.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+      FTLM_HIP_CHECK(hipStreamSynchronize(st));
+      std::vector<int> hnb(static_cast<size_t>(R));
+      for (int r = 0; r < R; ++r) hnb[r] = ha[r] ? hu[r] : hu[r] - 1;
+      nb.upload(hnb, st);
+      DevBuf<double> dr(static_cast<size_t>(R) * M), dw(static_cast<size_t>(R) * M);
+      FTLM_HIP_CHECK(hipMemsetAsync(dr.p, 0, dr.n * sizeof(double), st));
+      FTLM_HIP_CHECK(hipMemsetAsync(dw.p, 0, dw.n * sizeof(double), st));
+      quadrature_device(K.alpha.p, K.beta.p, K.used.p, nb.p, R, R, Mop, opts->quad, 0, 1e-14, 1, dr.p, dw.p, M, st);
+      FTLM_HIP_CHECK(hipMemcpyAsync(rz, dr.p, sizeof(double) * R * M, hipMemcpyDeviceToHost, st));
+      FTLM_HIP_CHECK(hipMemcpyAsync(wt, dw.p, sizeof(double) * R * M, hipMemcpyDeviceToHost, st));
+      std::copy(hu.begin(), hu.end(), us);
+      if (alpha_out || beta_out) {
+        std::vector<double> a(static_cast<size_t>(Mop) * R), b(static_cast<size_t>(Mop) * R);
+        FTLM_HIP_CHECK(hipMemcpyAsync(a.data(), K.alpha.p, a.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        FTLM_HIP_CHECK(hipMemcpyAsync(b.data(), K.beta.p, b.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+        FTLM_HIP_CHECK(hipStreamSynchronize(st));
+        for (int r = 0; r < R; ++r)
+          for (int k = 0; k < M; ++k) {
+            const int64_t o = (static_cast<int64_t>(io) * R + r) * M + k;
+            if (alpha_out) alpha_out[o] = k < Mop ? a[static_cast<size_t>(k) * R + r] : 0.0;
+            if (beta_out) beta_out[o] = k < Mop ? b[static_cast<size_t>(k) * R + r] : 0.0;
+          }
+      }
+      FTLM_HIP_CHECK(hipStreamSynchronize(st));
+    }
+  });
+}
+
+int ftlm_hip_ftlm_log_partition(const ftlm_op_t* op, const double* betas, int nb, const ftlm_ftlm_opts_t* opts,
+                                double* lnZ) {
+  return guarded([&] {
+    check_opts(opts);
+    FTLM_REQUIRE(op != nullptr && betas != nullptr && lnZ != nullptr && nb >= 1, FTLM_EINVAL, "null argument");
+    const int R = std::max(1, opts->R), M = std::max(1, opts->M);
+    if (op->dim <= 0) {
+      for (int i = 0; i < nb; ++i) lnZ[i] = -INFINITY;
+      return;
+    }
+    std::vector<double> rz(static_cast<size_t>(R) * M), wt(static_cast<size_t>(R) * M);
+    std::vector<int> us(static_cast<size_t>(R));
+    ftlm_op_t* ops[1] = {const_cast<ftlm_op_t*>(op)};
+    int rc = ftlm_hip_ftlm_ritz(ops, 1, opts, rz.data(), wt.data(), us.data(), nullptr, nullptr, nullptr);
+    if (rc != FTLM_OK) throw Error(rc, ftlm_hip_last_error());
+    const int64_t dim = op->dim;
+    rc = ftlm_hip_ftlm_thermo(rz.data(), wt.data(), us.data(), 1, R, M, &dim, nullptr, betas, nb, nullptr, 0, lnZ,
+                              nullptr, nullptr, nullptr, nullptr);
+    if (rc != FTLM_OK) throw Error(rc, ftlm_hip_last_error());
+    if (dim == 1) {
+      // the reference returns -beta * H00 without the ln(dim) term (ln 1 = 0 anyway)
+      for (int i = 0; i < nb; ++i) lnZ[i] = -betas[i] * rz[0];
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,504 @@
+// sector.hip -- lattice geometry, (N_up, N_dn) sector enumeration and ranking, pull-hop tables.
+//
+// Replaces, on the device:
+//   FockBasis ctor / enumerate_bit_combinations      src/fock_basis.cpp:7-42
+//   FockBasis::index_of (hash map)                   src/fock_basis.cpp:44-51
+//   the per-element hop scan + fermions_between_modes of apply_extended_hubbard
+//                                                    src/hubbard_hamiltonian.cpp:24-108
+// by a combinatorial-number-system unrank/rank and per-spin tables of "pull" hops (for each
+// string: the strings that hop INTO it, with the bond id and the own-spin Jordan-Wigner parity).
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <numeric>
+
+#include "common.hpp"
+
+namespace ftlm_hip {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+void check_launch(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) throw Error(FTLM_EHIP, std::string(what) + " launch failed: " + hipGetErrorString(e));
+}
+
+Binom host_binomials() {
+  Binom b;
+  std::memset(&b, 0, sizeof(b));
+  for (int a = 0; a <= kMaxSites + 1; ++a) {
+    b.c[a][0] = 1;
+    for (int k = 1; k <= a; ++k) b.c[a][k] = b.c[a - 1][k - 1] + (k <= a - 1 ? b.c[a - 1][k] : 0);
+  }
+  return b;
+}
+
+int64_t binom(int a, int b) {
+  static const Binom t = host_binomials();
+  if (a < 0 || b < 0 || b > a || a > kMaxSites + 1) return 0;
+  return t.c[a][b];
+}
+
+// Binomials for device-side rank/unrank.
+__constant__ int64_t c_binom[kMaxSites + 2][kMaxSites + 2];
+
+namespace {
+
+std::once_flag g_binom_once[64];
+
+void ensure_device_binomials(int dev) {
+  std::call_once(g_binom_once[dev & 63], [] {
+    const Binom b = host_binomials();
+    FTLM_HIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_binom), &b.c[0][0], sizeof(b.c)));
+  });
+}
+
+// Lexicographic rank of a k-subset of [0, n) given as a bit mask (positions ascending):
+//   rank = sum_i [ C(n-1-p_{i-1}, k-i+1) - C(n-p_i, k-i+1) ],  p_0 = -1
+// (hockey-stick sum of the subsets that agree on the first i-1 positions and have a smaller
+// i-th position).  Matches the enumeration order of src/fock_basis.cpp:7-15.
+__device__ __forceinline__ int64_t lex_rank(uint32_t m, int n, int k) {
+  int64_t r = 0;
+  int prev = -1, i = 1;
+  while (m) {
+    const int p = __ffs(m) - 1;
+    m &= m - 1;
+    r += c_binom[n - 1 - prev][k - i + 1] - c_binom[n - p][k - i + 1];
+    prev = p;
+    ++i;
+  }
+  return r;
+}
+
+// Inverse of lex_rank.
+__device__ __forceinline__ uint32_t lex_unrank(int64_t r, int n, int k) {
+  uint32_t m = 0;
+  for (int p = 0; p < n && k > 0; ++p) {
+    const int64_t c = c_binom[n - 1 - p][k - 1];  // subsets whose next position is p
+    if (r < c) {
+      m |= 1u << p;
+      --k;
+    } else {
+      r -= c;
+    }
+  }
+  return m;
+}
+
+__global__ void k_unrank(uint32_t* out, int D, int n, int k) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D) out[i] = lex_unrank(i, n, k);
+}
+
+struct BondDev {
+  uint8_t from, to;
+  uint32_t own;    // own-spin sites strictly inside the JW string
+};
+
+// Count / fill pull-hop entries: string m receives a hop along bond (from -> to) when `to` is
+// occupied and `from` is empty in m; the source string is m ^ (1<<from) ^ (1<<to).
+__global__ void k_hop_count(const uint32_t* masks, int D, const BondDev* bonds, int nb, int32_t* cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D) return;
+  const uint32_t m = masks[i];
+  int c = 0;
+  for (int b = 0; b < nb; ++b) c += ((m >> bonds[b].to) & 1u) & (((m >> bonds[b].from) & 1u) ^ 1u);
+  cnt[i] = c;
+}
+
+__global__ void k_hop_fill(const uint32_t* masks, int D, const BondDev* bonds, int nb, const int32_t* off,
+                           uint32_t* ent, int n, int k) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D) return;
+  const uint32_t m = masks[i];
+  int w = off[i];
+  for (int b = 0; b < nb; ++b) {
+    const BondDev bd = bonds[b];
+    if (!((m >> bd.to) & 1u) || ((m >> bd.from) & 1u)) continue;
+    const uint32_t src = m ^ (1u << bd.from) ^ (1u << bd.to);
+    const uint32_t t = static_cast<uint32_t>(lex_rank(src, n, k));
+    const uint32_t par = __popc(m & bd.own) & 1u;
+    ent[w++] = t | (static_cast<uint32_t>(b) << kTargetBits) | (par << 31);
+  }
+}
+
+// Per-string pieces of the diagonal  V * sum_pairs (u_i + d_i)(u_j + d_j):
+//   same-spin part sum_pairs s_i s_j, and bit planes of nb(x) = sum over pair partners of s.
+__global__ void k_diag_tables(const uint32_t* masks, int D, const int2* pairs, int np, int32_t* same,
+                              uint4* planes) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D) return;
+  const uint32_t m = masks[i];
+  int s = 0;
+  uint8_t nb[kMaxSites];
+  for (int q = 0; q < kMaxSites; ++q) nb[q] = 0;
+  for (int p = 0; p < np; ++p) {
+    const int a = pairs[p].x, b = pairs[p].y;
+    const uint32_t oa = (m >> a) & 1u, ob = (m >> b) & 1u;
+    s += static_cast<int>(oa & ob);
+    nb[a] += static_cast<uint8_t>(ob);
+    nb[b] += static_cast<uint8_t>(oa);
+  }
+  same[i] = s;
+  if (planes) {
+    uint4 pl = make_uint4(0, 0, 0, 0);
+    for (int q = 0; q < kMaxSites; ++q) {
+      pl.x |= static_cast<uint32_t>(nb[q] & 1) << q;
+      pl.y |= static_cast<uint32_t>((nb[q] >> 1) & 1) << q;
+      pl.z |= static_cast<uint32_t>((nb[q] >> 2) & 1) << q;
+      pl.w |= static_cast<uint32_t>((nb[q] >> 3) & 1) << q;
+    }
+    planes[i] = pl;
+  }
+}
+
+// index_of with the reference's key semantics: key = (up << n) | dn, decoded back into (u, d).
+__global__ void k_index_of(const uint64_t* up, const uint64_t* dn, int64_t cnt, int n, int nu, int nd, int Ddn,
+                           int64_t* out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const uint64_t key = (up[i] << n) | dn[i];
+  const uint64_t lowmask = (n >= 64) ? ~0ull : ((1ull << n) - 1ull);
+  const uint64_t u = key >> n, d = key & lowmask;
+  int64_t r = -1;
+  if (u <= lowmask && __popcll(u) == nu && __popcll(d) == nd)
+    r = lex_rank(static_cast<uint32_t>(u), n, nu) * Ddn + lex_rank(static_cast<uint32_t>(d), n, nd);
+  out[i] = r;
+}
+
+struct BondHost {
+  int from, to;
+  std::complex<double> coeff;
+  uint32_t own, other;
+};
+
+// Merge the directed hop list per spin: equal (from, to, spin) entries are summed in list order.
+void merge_bonds(const ftlm_hop_t* hops, int n_hops, int n_sites, std::vector<BondHost> out[2]) {
+  std::map<std::pair<int, int>, int> idx[2];
+  for (int h = 0; h < n_hops; ++h) {
+    const ftlm_hop_t& e = hops[h];
+    FTLM_REQUIRE(e.spin == 0 || e.spin == 1, FTLM_EINVAL, "hop spin must be 0 or 1");
+    FTLM_REQUIRE(e.from >= 0 && e.from < n_sites && e.to >= 0 && e.to < n_sites, FTLM_EINVAL,
+                 "hop site outside [0, n_sites)");
+    if (e.from == e.to) continue;  // number operators are not hops; the reference never emits them
+    auto key = std::make_pair(e.from, e.to);
+    auto it = idx[e.spin].find(key);
+    if (it == idx[e.spin].end()) {
+      const int lo = std::min(e.from, e.to), hi = std::max(e.from, e.to);
+      uint32_t inner = 0, other = 0;
+      for (int s = lo + 1; s <= hi - 1; ++s) inner |= 1u << s;
+      if (e.spin == 0) {
+        for (int s = lo; s <= hi - 1; ++s) other |= 1u << s;  // down modes 2s+1 in (2lo, 2hi)
+      } else {
+        for (int s = lo + 1; s <= hi; ++s) other |= 1u << s;  // up modes 2s in (2lo+1, 2hi+1)
+      }
+      idx[e.spin][key] = static_cast<int>(out[e.spin].size());
+      out[e.spin].push_back(BondHost{e.from, e.to, std::complex<double>(e.re, e.im), inner, other});
+    } else {
+      out[e.spin][static_cast<size_t>(it->second)].coeff += std::complex<double>(e.re, e.im);
+    }
+  }
+}
+
+int current_device() {
+  int d = 0;
+  FTLM_HIP_CHECK(hipGetDevice(&d));
+  return d;
+}
+
+}  // namespace
+
+SectorView Sector::view() const {
+  SectorView v;
+  v.dim = dim;
+  v.Dup = Dup;
+  v.Ddn = Ddn;
+  v.n_sites = n_sites;
+  v.U = U;
+  v.V = V;
+  v.up_mask = up_mask.p;
+  v.dn_mask = dn_mask.p;
+  v.off_up = off[0].p;
+  v.ent_up = ent[0].p;
+  v.off_dn = off[1].p;
+  v.ent_dn = ent[1].p;
+  v.coef_up = coef[0].p;
+  v.coef_dn = coef[1].p;
+  v.xmask_up = xmask[0].p;
+  v.xmask_dn = xmask[1].p;
+  v.nuu = nuu.p;
+  v.ndd = ndd.p;
+  v.planes = planes.p;
+  return v;
+}
+
+namespace {
+
+void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair_t* pairs, int n_pairs) {
+  const int n = S->n_sites;
+  S->device = current_device();
+  ensure_device_binomials(S->device);
+  const int64_t Du = binom(n, S->n_up), Dd = binom(n, S->n_dn);
+  FTLM_REQUIRE(Du < (1 << kTargetBits) && Dd < (1 << kTargetBits), FTLM_EUNSUPPORTED,
+               "per-spin string count must be < 2^20 on the device path");
+  S->Dup = static_cast<int>(Du);
+  S->Ddn = static_cast<int>(Dd);
+  S->dim = Du * Dd;
+
+  // strings (K1: device unrank)
+  S->up_mask.alloc(static_cast<size_t>(Du));
+  S->dn_mask.alloc(static_cast<size_t>(Dd));
+  hipLaunchKernelGGL(k_unrank, dim3(grid_for(Du, 256)), dim3(256), 0, 0, S->up_mask.p, S->Dup, n, S->n_up);
+  hipLaunchKernelGGL(k_unrank, dim3(grid_for(Dd, 256)), dim3(256), 0, 0, S->dn_mask.p, S->Ddn, n, S->n_dn);
+  check_launch("k_unrank");
+
+  // bonds
+  std::vector<BondHost> bonds[2];
+  merge_bonds(hops, n_hops, n, bonds);
+  S->real_h = true;
+  for (int sp = 0; sp < 2; ++sp) {
+    const int nb = static_cast<int>(bonds[sp].size());
+    FTLM_REQUIRE(nb <= static_cast<int>(kBondMask), FTLM_EUNSUPPORTED, "too many distinct directed bonds");
+    S->nbond[sp] = nb;
+    std::vector<BondDev> bd(static_cast<size_t>(std::max(nb, 1)));
+    std::vector<double2> cf(static_cast<size_t>(std::max(nb, 1)));
+    std::vector<uint32_t> xm(static_cast<size_t>(std::max(nb, 1)));
+    for (int b = 0; b < nb; ++b) {
+      const BondHost& h = bonds[sp][static_cast<size_t>(b)];
+      bd[b] = BondDev{static_cast<uint8_t>(h.from), static_cast<uint8_t>(h.to), h.own};
+      cf[b] = make_double2(h.coeff.real(), h.coeff.imag());
+      xm[b] = h.other;
+      if (h.coeff.imag() != 0.0) S->real_h = false;
+    }
+    S->coef[sp].alloc(cf.size());
+    S->coef[sp].upload(cf);
+    S->xmask[sp].alloc(xm.size());
+    S->xmask[sp].upload(xm);
+    DevBuf<BondDev> dbonds(bd.size());
+    dbonds.upload(bd);
+
+    const int D = sp == 0 ? S->Dup : S->Ddn;
+    const int k = sp == 0 ? S->n_up : S->n_dn;
+    const uint32_t* masks = sp == 0 ? S->up_mask.p : S->dn_mask.p;
+    DevBuf<int32_t> cnt(static_cast<size_t>(D));
+    hipLaunchKernelGGL(k_hop_count, dim3(grid_for(D, 256)), dim3(256), 0, 0, masks, D, dbonds.p, nb, cnt.p);
+    check_launch("k_hop_count");
+    std::vector<int32_t> hc(static_cast<size_t>(D)), ho(static_cast<size_t>(D) + 1, 0);
+    FTLM_HIP_CHECK(hipMemcpy(hc.data(), cnt.p, sizeof(int32_t) * D, hipMemcpyDeviceToHost));
+    int64_t acc = 0;
+    for (int i = 0; i < D; ++i) {
+      ho[i] = static_cast<int32_t>(acc);
+      acc += hc[i];
+    }
+    FTLM_REQUIRE(acc < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "hop table too large");
+    ho[D] = static_cast<int32_t>(acc);
+    S->nnz[sp] = acc;
+    S->off[sp].alloc(ho.size());
+    S->off[sp].upload(ho);
+    S->ent[sp].alloc(static_cast<size_t>(std::max<int64_t>(acc, 1)));
+    hipLaunchKernelGGL(k_hop_fill, dim3(grid_for(D, 256)), dim3(256), 0, 0, masks, D, dbonds.p, nb, S->off[sp].p,
+                       S->ent[sp].p, n, k);
+    check_launch("k_hop_fill");
+    FTLM_HIP_CHECK(hipDeviceSynchronize());
+  }
+
+  // diagonal tables
+  std::vector<int2> pr(static_cast<size_t>(std::max(n_pairs, 1)));
+  for (int p = 0; p < n_pairs; ++p) {
+    FTLM_REQUIRE(pairs[p].i >= 0 && pairs[p].i < n && pairs[p].j >= 0 && pairs[p].j < n, FTLM_EINVAL,
+                 "pair site outside [0, n_sites)");
+    pr[p] = make_int2(pairs[p].i, pairs[p].j);
+  }
+  {
+    std::vector<int> deg(static_cast<size_t>(n), 0);
+    for (int p = 0; p < n_pairs; ++p) {
+      ++deg[pairs[p].i];
+      ++deg[pairs[p].j];
+    }
+    for (int s = 0; s < n; ++s)
+      FTLM_REQUIRE(deg[s] <= 15, FTLM_EUNSUPPORTED, "a site appears in more than 15 NN pairs");
+  }
+  DevBuf<int2> dpairs(pr.size());
+  dpairs.upload(pr);
+  S->nuu.alloc(static_cast<size_t>(S->Dup));
+  S->ndd.alloc(static_cast<size_t>(S->Ddn));
+  S->planes.alloc(static_cast<size_t>(S->Dup));
+  hipLaunchKernelGGL(k_diag_tables, dim3(grid_for(S->Dup, 128)), dim3(128), 0, 0, S->up_mask.p, S->Dup, dpairs.p,
+                     n_pairs, S->nuu.p, S->planes.p);
+  hipLaunchKernelGGL(k_diag_tables, dim3(grid_for(S->Ddn, 128)), dim3(128), 0, 0, S->dn_mask.p, S->Ddn, dpairs.p,
+                     n_pairs, S->ndd.p, static_cast<uint4*>(nullptr));
+  check_launch("k_diag_tables");
+  FTLM_HIP_CHECK(hipDeviceSynchronize());
+}
+
+void geometry(const ftlm_params_t& p, std::vector<ftlm_hop_t>* hops, std::vector<ftlm_pair_t>* pairs) {
+  // build_hubbard_geometry (src/lattice.cpp:38-110): per site (row-major), east then north
+  // neighbour with their NN pair, then the four diagonals (+,+) (+,-) (-,+) (-,-); each bond adds
+  // both directions for both spins; Peierls phase exp(i(wx phi_x + wy phi_y)) on wrapped bonds.
+  hops->clear();
+  pairs->clear();
+  auto bond = [&](int a, int b, double wphase, double ts) {
+    if (std::abs(ts) < 1e-15) return;
+    const std::complex<double> ph = std::polar(1.0, wphase);
+    const std::complex<double> f = -ts * ph, r = -ts * std::conj(ph);
+    for (int s = 0; s < 2; ++s) {
+      hops->push_back(ftlm_hop_t{a, b, s, f.real(), f.imag()});
+      hops->push_back(ftlm_hop_t{b, a, s, r.real(), r.imag()});
+    }
+  };
+  auto pair = [&](int a, int b) {
+    if (a == b) return;
+    pairs->push_back(ftlm_pair_t{std::min(a, b), std::max(a, b)});
+  };
+  const int Lx = p.Lx, Ly = p.Ly;
+  for (int y = 0; y < Ly; ++y)
+    for (int x = 0; x < Lx; ++x) {
+      const int i = y * Lx + x;
+      if (Lx > 1) {
+        const int wx = (x + 1 >= Lx) ? 1 : 0;
+        const int j = y * Lx + (x + 1) % Lx;
+        if (i != j) {
+          pair(i, j);
+          bond(i, j, static_cast<double>(wx) * p.phi_x + 0.0 * p.phi_y, p.t);
+        }
+      }
+      if (Ly > 1) {
+        const int wy = (y + 1 >= Ly) ? 1 : 0;
+        const int j = ((y + 1) % Ly) * Lx + x;
+        if (i != j) {
+          pair(i, j);
+          bond(i, j, 0.0 * p.phi_x + static_cast<double>(wy) * p.phi_y, p.t);
+        }
+      }
+      static const int dirs[4][2] = {{1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+      for (const auto& dv : dirs) {
+        int nx = x + dv[0], ny = y + dv[1], wx = 0, wy = 0;
+        if (nx >= Lx) { nx -= Lx; wx = 1; } else if (nx < 0) { nx += Lx; wx = -1; }
+        if (ny >= Ly) { ny -= Ly; wy = 1; } else if (ny < 0) { ny += Ly; wy = -1; }
+        const int j = ny * Lx + nx;
+        if (i != j) bond(i, j, static_cast<double>(wx) * p.phi_x + static_cast<double>(wy) * p.phi_y, p.tp);
+      }
+    }
+}
+
+}  // namespace
+
+}  // namespace ftlm_hip
+
+using namespace ftlm_hip;
+
+extern "C" {
+
+const char* ftlm_hip_version(void) { return "0.1.0-dev+mi355x"; }
+
+const char* ftlm_hip_last_error(void) { return g_last_error.c_str(); }
+
+int ftlm_hip_device_count(int* count) {
+  return guarded([&] {
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    *count = (e == hipSuccess) ? c : 0;
+  });
+}
+
+int ftlm_hip_nsites(const ftlm_params_t* p) { return p ? p->Lx * p->Ly : 0; }
+
+int ftlm_hip_build_geometry(const ftlm_params_t* p, ftlm_hop_t* hops, int hop_cap, int* n_hops, ftlm_pair_t* pairs,
+                            int pair_cap, int* n_pairs) {
+  return guarded([&] {
+    FTLM_REQUIRE(p != nullptr && n_hops != nullptr && n_pairs != nullptr, FTLM_EINVAL, "null argument");
+    FTLM_REQUIRE(p->Lx >= 1 && p->Ly >= 1, FTLM_EINVAL, "Lx, Ly must be >= 1");
+    std::vector<ftlm_hop_t> h;
+    std::vector<ftlm_pair_t> q;
+    geometry(*p, &h, &q);
+    *n_hops = static_cast<int>(h.size());
+    *n_pairs = static_cast<int>(q.size());
+    FTLM_REQUIRE(static_cast<int>(h.size()) <= hop_cap && static_cast<int>(q.size()) <= pair_cap, FTLM_ERANGE,
+                 "geometry buffers too small");
+    if (!h.empty()) std::memcpy(hops, h.data(), h.size() * sizeof(ftlm_hop_t));
+    if (!q.empty()) std::memcpy(pairs, q.data(), q.size() * sizeof(ftlm_pair_t));
+  });
+}
+
+int ftlm_hip_sector_create(int n_sites, int n_up, int n_dn, const ftlm_hop_t* hops, int n_hops,
+                           const ftlm_pair_t* pairs, int n_pairs, double U, double V, ftlm_op_t** out) {
+  return guarded([&] {
+    FTLM_REQUIRE(out != nullptr, FTLM_EINVAL, "null output handle");
+    *out = nullptr;
+    FTLM_REQUIRE(n_sites >= 1 && n_sites <= kMaxSites, FTLM_EINVAL,
+                 "FockBasis: require 1 <= n_sites <= 32 (bit packing)");
+    FTLM_REQUIRE(n_up >= 0 && n_dn >= 0 && n_up <= n_sites && n_dn <= n_sites, FTLM_EINVAL,
+                 "FockBasis: particle numbers out of range");
+    FTLM_REQUIRE(n_hops >= 0 && n_pairs >= 0 && (n_hops == 0 || hops) && (n_pairs == 0 || pairs), FTLM_EINVAL,
+                 "null hop / pair list");
+    auto S = std::make_unique<Sector>();
+    S->kind = kSector;
+    S->n_sites = n_sites;
+    S->n_up = n_up;
+    S->n_dn = n_dn;
+    S->U = U;
+    S->V = V;
+    build_sector(S.get(), hops, n_hops, pairs, n_pairs);
+    *out = S.release();
+  });
+}
+
+int ftlm_hip_op_destroy(ftlm_op_t* op) {
+  return guarded([&] { delete op; });
+}
+
+int ftlm_hip_op_dim(const ftlm_op_t* op, int64_t* dim, int64_t* dim_up, int64_t* dim_dn) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr, FTLM_EINVAL, "null operator");
+    if (dim) *dim = op->dim;
+    const Sector* S = op->kind == kSector ? static_cast<const Sector*>(op) : nullptr;
+    if (dim_up) *dim_up = S ? S->Dup : 0;
+    if (dim_dn) *dim_dn = S ? S->Ddn : 0;
+  });
+}
+
+int ftlm_hip_sector_masks(const ftlm_op_t* op, int64_t first, int64_t count, uint64_t* up, uint64_t* dn) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && op->kind == kSector, FTLM_EINVAL, "not a full-sector operator");
+    const Sector* S = static_cast<const Sector*>(op);
+    FTLM_REQUIRE(first >= 0 && count >= 0 && first + count <= S->dim, FTLM_ERANGE, "FockBasis index out of range");
+    if (count == 0) return;
+    std::vector<uint32_t> hu(static_cast<size_t>(S->Dup)), hd(static_cast<size_t>(S->Ddn));
+    FTLM_HIP_CHECK(hipMemcpy(hu.data(), S->up_mask.p, sizeof(uint32_t) * S->Dup, hipMemcpyDeviceToHost));
+    FTLM_HIP_CHECK(hipMemcpy(hd.data(), S->dn_mask.p, sizeof(uint32_t) * S->Ddn, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < count; ++i) {
+      const int64_t g = first + i;
+      if (up) up[i] = hu[static_cast<size_t>(g / S->Ddn)];
+      if (dn) dn[i] = hd[static_cast<size_t>(g % S->Ddn)];
+    }
+  });
+}
+
+int ftlm_hip_index_of(const ftlm_op_t* op, const uint64_t* up, const uint64_t* dn, int64_t count, int64_t* out) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && op->kind == kSector, FTLM_EINVAL, "not a full-sector operator");
+    const Sector* S = static_cast<const Sector*>(op);
+    if (count <= 0) return;
+    FTLM_HIP_CHECK(hipSetDevice(S->device));
+    ensure_device_binomials(S->device);
+    DevBuf<uint64_t> du(static_cast<size_t>(count)), dd(static_cast<size_t>(count));
+    DevBuf<int64_t> dr(static_cast<size_t>(count));
+    du.upload(up, static_cast<size_t>(count));
+    dd.upload(dn, static_cast<size_t>(count));
+    hipLaunchKernelGGL(k_index_of, dim3(grid_for(count, 256)), dim3(256), 0, 0, du.p, dd.p, count, S->n_sites,
+                       S->n_up, S->n_dn, S->Ddn, dr.p);
+    check_launch("k_index_of");
+    FTLM_HIP_CHECK(hipMemcpy(out, dr.p, sizeof(int64_t) * count, hipMemcpyDeviceToHost));
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,511 @@
+// tridiag.hip -- Gauss-quadrature eigensolvers for Lanczos tridiagonals and the FTLM Ritz-weight
+// thermodynamic sums, on the device.
+//
+// k_jacobi: the reference's max-pivot single-rotation Jacobi (src/ftlm_thermo.cpp:17-80 with
+//   max(3000,120n) rotations at 1e-14; src/lanczos.cpp:38-102 with 120 rotations at 1e-15 for
+//   the extrema).  One workgroup per tridiagonal.  The pivot search keeps, per row i, the largest
+//   |A[i][j]| (j > i) and its first column; a rotation on (p, q) changes rows p, q entirely and
+//   other rows only in columns p and q, so each row is rescanned only when its maximum may have
+//   dropped.  The global pivot is the first strict maximum in row-major order, exactly as the
+//   reference's nested scan.  Only row 0 of the eigenvector matrix is accumulated: it is all the
+//   quadrature weights need, and each row of V is rotated independently, so the values equal the
+//   reference's V[0][k].
+// k_ql: implicit-shift QL (Wilkinson shift) run to convergence, one thread per tridiagonal;
+//   the converged alternative (FTLM_QUAD_CONVERGED).
+// k_thermo_op / k_thermo_gc: log-sum-exp Ritz sums for lnZ(beta) and E(beta) per operator and
+//   the grand-canonical (beta, mu) mixture (src/ftlm_thermo.cpp:120-145, 195-210;
+//   benchmarks/bench_ftlm_nmu_rect.cpp:186-219).
+
+#include <cmath>
+
+#include "common.hpp"
+
+namespace ftlm_hip {
+
+constexpr int kJacBlock = 256;
+
+struct RowMax {
+  double v;
+  int col;
+};
+
+// Rescan row i of the full row-major matrix: max over j > i, first column on ties.
+__device__ __forceinline__ RowMax scan_row(const double* A, int n, int i) {
+  RowMax r{0.0, -1};
+  for (int j = i + 1; j < n; ++j) {
+    const double v = fabs(A[static_cast<int64_t>(i) * n + j]);
+    if (v > r.v) {
+      r.v = v;
+      r.col = j;
+    }
+  }
+  return r;
+}
+
+// One tridiagonal per workgroup.  alpha/beta are strided: entry i of sample s at [i*stride + s].
+// n[s] = size (used steps), nb_avail = number of valid beta entries (n-1 or fewer).
+__global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__ alpha, const double* __restrict__ beta,
+                                                      const int* __restrict__ nvec, const int* __restrict__ nbvec,
+                                                      int stride, int cap_fixed, double tol, int with_weights,
+                                                      double* __restrict__ scratch, int64_t scratch_per,
+                                                      double* __restrict__ ritz, double* __restrict__ weight,
+                                                      int out_stride) {
+#pragma clang fp contract(off)
+  const int s = blockIdx.x;
+  const int n = nvec[s];
+  const int nb = nbvec ? nbvec[s] : (n - 1);
+  double* A = scratch + scratch_per * s;
+  extern __shared__ unsigned char smem_raw[];
+  double* v0 = reinterpret_cast<double*>(smem_raw);                 // [n]
+  RowMax* rm = reinterpret_cast<RowMax*>(v0 + n);                    // [n]
+  __shared__ double s_val[kJacBlock / 64];
+  __shared__ int s_row[kJacBlock / 64];
+  __shared__ double s_c, s_s, s_big;
+  __shared__ int s_p, s_q;
+  const int tid = threadIdx.x;
+  if (n <= 0) return;
+
+  for (int64_t k = tid; k < static_cast<int64_t>(n) * n; k += blockDim.x) A[k] = 0.0;
+  __syncthreads();
+  for (int i = tid; i < n; i += blockDim.x) {
+    A[static_cast<int64_t>(i) * n + i] = alpha[static_cast<int64_t>(i) * stride + s];
+    if (i < n - 1 && i < nb) {
+      const double b = beta[static_cast<int64_t>(i) * stride + s];
+      A[static_cast<int64_t>(i) * n + i + 1] = b;
+      A[static_cast<int64_t>(i + 1) * n + i] = b;
+    }
+    v0[i] = (i == 0) ? 1.0 : 0.0;
+  }
+  __syncthreads();
+  const int cap = cap_fixed > 0 ? cap_fixed : max(3000, 120 * n);
+  if (n > 1) {
+    for (int i = tid; i < n; i += blockDim.x) rm[i] = scan_row(A, n, i);
+    __syncthreads();
+    for (int it = 0; it < cap; ++it) {
+      // ---- pivot: largest row maximum, smallest row on ties
+      double bv = -1.0;
+      int br = n;
+      for (int i = tid; i < n; i += blockDim.x) {
+        const double v = rm[i].col >= 0 ? rm[i].v : 0.0;
+        if (v > bv || (v == bv && i < br)) {
+          bv = v;
+          br = i;
+        }
+      }
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        const double ov = __shfl_xor(bv, o, 64);
+        const int orow = __shfl_xor(br, o, 64);
+        if (ov > bv || (ov == bv && orow < br)) {
+          bv = ov;
+          br = orow;
+        }
+      }
+      if ((tid & 63) == 0) {
+        s_val[tid >> 6] = bv;
+        s_row[tid >> 6] = br;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        double v = s_val[0];
+        int r = s_row[0];
+        for (int w = 1; w < kJacBlock / 64; ++w)
+          if (s_val[w] > v || (s_val[w] == v && s_row[w] < r)) {
+            v = s_val[w];
+            r = s_row[w];
+          }
+        // the reference's scan starts from (p, q) = (0, 1), max 0: a zero maximum keeps (0, 1)
+        int p = 0, q = 1;
+        if (v > 0.0 && r < n && rm[r].col >= 0) {
+          p = r;
+          q = rm[r].col;
+        } else {
+          v = 0.0;
+        }
+        s_big = v;
+        s_p = p;
+        s_q = q;
+        if (v >= tol) {
+          const double app = A[static_cast<int64_t>(p) * n + p];
+          const double aqq = A[static_cast<int64_t>(q) * n + q];
+          const double apq = A[static_cast<int64_t>(p) * n + q];
+          const double phi = 0.5 * atan2(2.0 * apq, aqq - app);
+          const double c = cos(phi), sn = sin(phi);
+          s_c = c;
+          s_s = sn;
+          const double npp = c * c * app - 2.0 * sn * c * apq + sn * sn * aqq;
+          const double nqq = sn * sn * app + 2.0 * sn * c * apq + c * c * aqq;
+          // diagonal / pivot entries are written after the off-diagonal sweep below reads rows
+          // p and q, so stash them
+          A[static_cast<int64_t>(p) * n + q] = npp;  // temporarily hold npp in (p,q)
+          A[static_cast<int64_t>(q) * n + p] = nqq;  // and nqq in (q,p)
+          const double vp = v0[p], vq = v0[q];
+          v0[p] = c * vp - sn * vq;
+          v0[q] = sn * vp + c * vq;
+        }
+      }
+      __syncthreads();
+      if (s_big < tol) break;
+      const int p = s_p, q = s_q;
+      const double c = s_c, sn = s_s;
+      for (int k = tid; k < n; k += blockDim.x) {
+        if (k == p || k == q) continue;
+        const double apk = A[static_cast<int64_t>(p) * n + k];
+        const double aqk = A[static_cast<int64_t>(q) * n + k];
+        const double rp = c * apk - sn * aqk;
+        const double rq = c * aqk + sn * apk;
+        A[static_cast<int64_t>(p) * n + k] = rp;
+        A[static_cast<int64_t>(k) * n + p] = rp;
+        A[static_cast<int64_t>(q) * n + k] = rq;
+        A[static_cast<int64_t>(k) * n + q] = rq;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        const double npp = A[static_cast<int64_t>(p) * n + q];
+        const double nqq = A[static_cast<int64_t>(q) * n + p];
+        A[static_cast<int64_t>(p) * n + q] = 0.0;
+        A[static_cast<int64_t>(q) * n + p] = 0.0;
+        A[static_cast<int64_t>(p) * n + p] = npp;
+        A[static_cast<int64_t>(q) * n + q] = nqq;
+      }
+      __syncthreads();
+      // ---- row maxima: rows p, q fully; rows i < q see new values in columns p (if i < p), q
+      for (int i = tid; i < n; i += blockDim.x) {
+        if (i == p || i == q) {
+          rm[i] = scan_row(A, n, i);
+          continue;
+        }
+        if (i > q && i > p) continue;
+        RowMax r = rm[i];
+        if (r.col == p || r.col == q) {
+          rm[i] = scan_row(A, n, i);
+          continue;
+        }
+        const int c1 = p < q ? p : q, c2 = p < q ? q : p;
+        if (c1 > i) {
+          const double v = fabs(A[static_cast<int64_t>(i) * n + c1]);
+          if (v > r.v || (v == r.v && r.col >= 0 && c1 < r.col) || (r.col < 0 && v > 0.0)) {
+            r.v = v;
+            r.col = c1;
+          }
+        }
+        if (c2 > i) {
+          const double v = fabs(A[static_cast<int64_t>(i) * n + c2]);
+          if (v > r.v || (v == r.v && r.col >= 0 && c2 < r.col) || (r.col < 0 && v > 0.0)) {
+            r.v = v;
+            r.col = c2;
+          }
+        }
+        rm[i] = r;
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = tid; k < n; k += blockDim.x) {
+    ritz[static_cast<int64_t>(s) * out_stride + k] = A[static_cast<int64_t>(k) * n + k];
+    if (with_weights) weight[static_cast<int64_t>(s) * out_stride + k] = v0[k] * v0[k];
+  }
+}
+
+// Converged eigen-decomposition of a symmetric tridiagonal (implicit QL, Wilkinson shift),
+// returning eigenvalues and squared first components of the eigenvectors.
+__global__ void k_ql(const double* __restrict__ alpha, const double* __restrict__ beta, const int* __restrict__ nvec,
+                     const int* __restrict__ nbvec, int stride, int nsamp, double* __restrict__ work,
+                     int64_t work_per, double* __restrict__ ritz, double* __restrict__ weight, int out_stride,
+                     int* __restrict__ status) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsamp) return;
+  const int n = nvec[s];
+  const int nb = nbvec ? nbvec[s] : n - 1;
+  double* d = ritz + static_cast<int64_t>(s) * out_stride;
+  double* z = weight + static_cast<int64_t>(s) * out_stride;
+  double* e = work + work_per * s;
+  for (int i = 0; i < n; ++i) {
+    d[i] = alpha[static_cast<int64_t>(i) * stride + s];
+    e[i] = (i < n - 1 && i < nb) ? beta[static_cast<int64_t>(i) * stride + s] : 0.0;
+    z[i] = (i == 0) ? 1.0 : 0.0;
+  }
+  int bad = 0;
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    while (true) {
+      int m = l;
+      for (; m < n - 1; ++m) {
+        const double dd = fabs(d[m]) + fabs(d[m + 1]);
+        if (fabs(e[m]) <= 1e-300 + 2.220446049250313e-16 * dd) break;
+      }
+      if (m == l) break;
+      if (++iter > 200) {
+        bad = 1;
+        break;
+      }
+      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double r = hypot(g, 1.0);
+      g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+      double sn = 1.0, c = 1.0, p = 0.0;
+      int i = m - 1;
+      bool deflated = false;
+      for (; i >= l; --i) {
+        double f = sn * e[i];
+        const double b = c * e[i];
+        r = hypot(f, g);
+        e[i + 1] = r;
+        if (r == 0.0) {
+          d[i + 1] -= p;
+          e[m] = 0.0;
+          deflated = true;
+          break;
+        }
+        sn = f / r;
+        c = g / r;
+        g = d[i + 1] - p;
+        r = (d[i] - g) * sn + 2.0 * c * b;
+        p = sn * r;
+        d[i + 1] = g + p;
+        g = c * r - b;
+        f = z[i + 1];
+        z[i + 1] = sn * z[i] + c * f;
+        z[i] = c * z[i] - sn * f;
+      }
+      if (deflated) continue;
+      d[l] -= p;
+      e[l] = g;
+      e[m] = 0.0;
+    }
+  }
+  for (int i = 0; i < n; ++i) z[i] = z[i] * z[i];
+  if (bad) atomicAdd(status, 1);
+}
+
+// ln sum_k w_k exp(-beta lambda_k) over k < n, skipping w <= 0 (src/ftlm_thermo.cpp:120-145).
+__device__ double log_quad(const double* lam, const double* w, int n, double bt, double* e_num) {
+  if (n <= 0) return -INFINITY;
+  if (n == 1) {
+    if (e_num) *e_num = lam[0];
+    return -bt * lam[0];
+  }
+  double mx = -INFINITY;
+  for (int k = 0; k < n; ++k)
+    if (w[k] > 0.0) mx = fmax(mx, -bt * lam[k] + log(w[k]));
+  if (!isfinite(mx)) return -INFINITY;
+  double s = 0.0, se = 0.0;
+  for (int k = 0; k < n; ++k) {
+    if (w[k] <= 0.0) continue;
+    const double t = exp(-bt * lam[k] + log(w[k]) - mx);
+    s += t;
+    se += t * lam[k];
+  }
+  if (e_num) *e_num = se / s;  // energy of this sample's quadrature
+  return mx + log(s);
+}
+
+// Per operator and beta: lnZ = ln dim + log-mean-exp_r L_r; E = sum_r e^{L_r} E_r / sum_r e^{L_r}.
+__global__ void k_thermo_op(const double* ritz, const double* weight, const int* used, int n_ops, int R, int M,
+                            const int64_t* dims, const double* betas, int nb, double* lnZ, double* E) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_ops * nb) return;
+  const int op = t / nb, ib = t % nb;
+  const double bt = betas[ib];
+  double mx = -INFINITY;
+  int cnt = 0;
+  for (int r = 0; r < R; ++r) {
+    const int n = used[op * R + r];
+    if (n <= 0) continue;
+    const int64_t o = (static_cast<int64_t>(op) * R + r) * M;
+    const double L = log_quad(ritz + o, weight + o, n, bt, nullptr);
+    if (cnt == 0 || L > mx) mx = L;
+    ++cnt;
+  }
+  double lz = -INFINITY, en = NAN;
+  if (cnt > 0 && dims[op] > 0) {
+    double s = 0.0, se = 0.0;
+    for (int r = 0; r < R; ++r) {
+      const int n = used[op * R + r];
+      if (n <= 0) continue;
+      const int64_t o = (static_cast<int64_t>(op) * R + r) * M;
+      double er = 0.0;
+      const double L = log_quad(ritz + o, weight + o, n, bt, &er);
+      const double wgt = exp(L - mx);
+      s += wgt;
+      se += wgt * er;
+    }
+    lz = log(static_cast<double>(dims[op])) + (mx + log(s / static_cast<double>(cnt)));
+    en = se / s;
+  }
+  if (lnZ) lnZ[t] = lz;
+  if (E) E[t] = en;
+}
+
+// Grand canonical mixture over operators (sectors): weight_s = exp(lnZ_s + beta mu N_s).
+__global__ void k_thermo_gc(const double* lnZ, const double* E, const int* N, int n_ops, const double* betas, int nb,
+                            const double* mus, int nmu, double* lnZ_gc, double* N_gc, double* E_gc) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nb * nmu) return;
+  const int ib = t / nmu, im = t % nmu;
+  const double bt = betas[ib], mu = mus[im];
+  double mx = -INFINITY;
+  for (int s = 0; s < n_ops; ++s) {
+    const double l = lnZ[s * nb + ib];
+    if (!isfinite(l)) continue;
+    mx = fmax(mx, l + bt * mu * static_cast<double>(N[s]));
+  }
+  double z = 0.0, zn = 0.0, ze = 0.0;
+  if (isfinite(mx)) {
+    for (int s = 0; s < n_ops; ++s) {
+      const double l = lnZ[s * nb + ib];
+      if (!isfinite(l)) continue;
+      const double w = exp(l + bt * mu * static_cast<double>(N[s]) - mx);
+      z += w;
+      zn += static_cast<double>(N[s]) * w;
+      if (E) ze += E[s * nb + ib] * w;
+    }
+  }
+  if (lnZ_gc) lnZ_gc[t] = isfinite(mx) ? mx + log(z) : -INFINITY;
+  if (N_gc) N_gc[t] = z > 0.0 ? zn / z : 0.0;
+  if (E_gc) E_gc[t] = z > 0.0 ? ze / z : NAN;
+}
+
+// ---------------------------------------------------------------- host drivers
+
+// Diagonalise nsamp tridiagonals stored strided in device memory.  Outputs (device) ritz and
+// weight [nsamp][out_stride].
+void quadrature_device(const double* d_alpha, const double* d_beta, const int* d_n, const int* d_nb, int stride,
+                       int nsamp, int max_n, int quad, int cap_fixed, double tol, int with_weights, double* d_ritz,
+                       double* d_weight, int out_stride, hipStream_t st) {
+  if (nsamp <= 0 || max_n <= 0) return;
+  if (quad == FTLM_QUAD_CONVERGED) {
+    DevBuf<double> work(static_cast<size_t>(nsamp) * max_n);
+    DevBuf<int> status(1);
+    FTLM_HIP_CHECK(hipMemsetAsync(status.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_ql, dim3(grid_for(nsamp, 64)), dim3(64), 0, st, d_alpha, d_beta, d_n, d_nb, stride, nsamp,
+                       work.p, static_cast<int64_t>(max_n), d_ritz, d_weight, out_stride, status.p);
+    check_launch("k_ql");
+    int bad = 0;
+    FTLM_HIP_CHECK(hipMemcpyAsync(&bad, status.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    FTLM_HIP_CHECK(hipStreamSynchronize(st));
+    FTLM_REQUIRE(bad == 0, FTLM_EHIP, "tridiagonal QL did not converge");
+    return;
+  }
+  const int64_t per = static_cast<int64_t>(max_n) * max_n;
+  DevBuf<double> scratch(static_cast<size_t>(per) * nsamp);
+  const size_t shmem = static_cast<size_t>(max_n) * (sizeof(double) + sizeof(RowMax));
+  FTLM_REQUIRE(shmem <= 60 * 1024, FTLM_EUNSUPPORTED, "tridiagonal too large for the Jacobi kernel");
+  hipLaunchKernelGGL(k_jacobi, dim3(nsamp), dim3(kJacBlock), shmem, st, d_alpha, d_beta, d_n, d_nb, stride, cap_fixed,
+                     tol, with_weights, scratch.p, per, d_ritz, d_weight, out_stride);
+  check_launch("k_jacobi");
+  FTLM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+void thermo_device(const double* d_ritz, const double* d_weight, const int* d_used, int n_ops, int R, int M,
+                   const int64_t* d_dims, const int* d_N, const double* d_betas, int nb, const double* d_mus, int nmu,
+                   double* d_lnZ, double* d_E, double* d_lnZgc, double* d_Ngc, double* d_Egc, hipStream_t st) {
+  hipLaunchKernelGGL(k_thermo_op, dim3(grid_for(static_cast<int64_t>(n_ops) * nb, 64)), dim3(64), 0, st, d_ritz,
+                     d_weight, d_used, n_ops, R, M, d_dims, d_betas, nb, d_lnZ, d_E);
+  check_launch("k_thermo_op");
+  if (nmu > 0 && d_N != nullptr) {
+    hipLaunchKernelGGL(k_thermo_gc, dim3(grid_for(static_cast<int64_t>(nb) * nmu, 64)), dim3(64), 0, st, d_lnZ, d_E,
+                       d_N, n_ops, d_betas, nb, d_mus, nmu, d_lnZgc, d_Ngc, d_Egc);
+    check_launch("k_thermo_gc");
+  }
+}
+
+}  // namespace ftlm_hip
+
+using namespace ftlm_hip;
+
+extern "C" {
+
+int ftlm_hip_tridiagonal_extrema(const double* alpha, int n, const double* beta, int nb, double* lo, double* hi) {
+  return guarded([&] {
+    FTLM_REQUIRE(lo != nullptr && hi != nullptr, FTLM_EINVAL, "null output");
+    if (n <= 0) {
+      *lo = 0.0;
+      *hi = 0.0;
+      return;
+    }
+    FTLM_REQUIRE(alpha != nullptr && (nb <= 0 || beta != nullptr), FTLM_EINVAL, "null alpha / beta");
+    DevBuf<double> da(static_cast<size_t>(n)), db(static_cast<size_t>(std::max(n, 1))), dr(static_cast<size_t>(n));
+    std::vector<double> hb(static_cast<size_t>(std::max(n, 1)), 0.0);
+    for (int i = 0; i < std::min(nb, n); ++i) hb[i] = beta[i];
+    da.upload(alpha, static_cast<size_t>(n));
+    db.upload(hb);
+    DevBuf<int> dn(1), dnb(1);
+    const int nbv = std::max(0, nb);
+    dn.upload(&n, 1);
+    dnb.upload(&nbv, 1);
+    quadrature_device(da.p, db.p, dn.p, dnb.p, 1, 1, n, FTLM_QUAD_REFERENCE, 120, 1e-15, 0, dr.p, nullptr, n, nullptr);
+    std::vector<double> r(static_cast<size_t>(n));
+    FTLM_HIP_CHECK(hipMemcpy(r.data(), dr.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    double a = r[0], b = r[0];
+    for (double v : r) {
+      a = std::min(a, v);
+      b = std::max(b, v);
+    }
+    *lo = a;
+    *hi = b;
+  });
+}
+
+int ftlm_hip_quadrature_ritz(const double* alpha, int n, const double* beta, int nb, int quad, double* ritz,
+                             double* weight) {
+  return guarded([&] {
+    FTLM_REQUIRE(n >= 0 && ritz != nullptr && weight != nullptr, FTLM_EINVAL, "bad arguments");
+    if (n == 0) return;
+    if (n == 1) {
+      ritz[0] = alpha[0];
+      weight[0] = 1.0;
+      return;
+    }
+    DevBuf<double> da(static_cast<size_t>(n)), db(static_cast<size_t>(n)), dr(static_cast<size_t>(n)),
+        dw(static_cast<size_t>(n));
+    std::vector<double> hb(static_cast<size_t>(n), 0.0);
+    for (int i = 0; i < std::min(nb, n); ++i) hb[i] = beta[i];
+    da.upload(alpha, static_cast<size_t>(n));
+    db.upload(hb);
+    DevBuf<int> dn(1), dnb(1);
+    const int nbv = std::max(0, nb);
+    dn.upload(&n, 1);
+    dnb.upload(&nbv, 1);
+    quadrature_device(da.p, db.p, dn.p, dnb.p, 1, 1, n, quad, 0, 1e-14, 1, dr.p, dw.p, n, nullptr);
+    FTLM_HIP_CHECK(hipMemcpy(ritz, dr.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    FTLM_HIP_CHECK(hipMemcpy(weight, dw.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int ftlm_hip_ftlm_thermo(const double* ritz, const double* weight, const int* used, int n_ops, int R, int M,
+                         const int64_t* dims, const int* N, const double* betas, int nb, const double* mus, int nmu,
+                         double* lnZ_op, double* E_op, double* lnZ_gc, double* N_gc, double* E_gc) {
+  return guarded([&] {
+    FTLM_REQUIRE(n_ops >= 1 && R >= 1 && M >= 1 && nb >= 1 && nmu >= 0, FTLM_EINVAL, "bad sizes");
+    FTLM_REQUIRE(ritz && weight && used && dims && betas, FTLM_EINVAL, "null input");
+    const size_t rk = static_cast<size_t>(n_ops) * R * M;
+    DevBuf<double> dr(rk), dw(rk), dbt(static_cast<size_t>(nb)), dmu(static_cast<size_t>(std::max(nmu, 1)));
+    DevBuf<int> du(static_cast<size_t>(n_ops) * R), dN(static_cast<size_t>(n_ops));
+    DevBuf<int64_t> dd(static_cast<size_t>(n_ops));
+    dr.upload(ritz, rk);
+    dw.upload(weight, rk);
+    du.upload(used, static_cast<size_t>(n_ops) * R);
+    dd.upload(dims, static_cast<size_t>(n_ops));
+    dbt.upload(betas, static_cast<size_t>(nb));
+    if (nmu > 0) {
+      FTLM_REQUIRE(N != nullptr && mus != nullptr, FTLM_EINVAL, "mu grid needs N and mus");
+      dmu.upload(mus, static_cast<size_t>(nmu));
+      dN.upload(N, static_cast<size_t>(n_ops));
+    }
+    const size_t ob = static_cast<size_t>(n_ops) * nb, gb = static_cast<size_t>(nb) * std::max(nmu, 1);
+    DevBuf<double> dl(ob), de(ob), dlg(gb), dng(gb), deg(gb);
+    thermo_device(dr.p, dw.p, du.p, n_ops, R, M, dd.p, nmu > 0 ? dN.p : nullptr, dbt.p, nb, dmu.p, nmu, dl.p, de.p,
+                  dlg.p, dng.p, deg.p, nullptr);
+    FTLM_HIP_CHECK(hipDeviceSynchronize());
+    if (lnZ_op) FTLM_HIP_CHECK(hipMemcpy(lnZ_op, dl.p, ob * sizeof(double), hipMemcpyDeviceToHost));
+    if (E_op) FTLM_HIP_CHECK(hipMemcpy(E_op, de.p, ob * sizeof(double), hipMemcpyDeviceToHost));
+    if (nmu > 0) {
+      const size_t g = static_cast<size_t>(nb) * nmu * sizeof(double);
+      if (lnZ_gc) FTLM_HIP_CHECK(hipMemcpy(lnZ_gc, dlg.p, g, hipMemcpyDeviceToHost));
+      if (N_gc) FTLM_HIP_CHECK(hipMemcpy(N_gc, dng.p, g, hipMemcpyDeviceToHost));
+      if (E_gc) FTLM_HIP_CHECK(hipMemcpy(E_gc, deg.p, g, hipMemcpyDeviceToHost));
+    }
+  });
+}
+
+}  // extern "C"

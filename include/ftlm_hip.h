@@ -1,0 +1,193 @@
+/* ftlm_hip.h -- C ABI of the MI355X (gfx950) FTLM hot-path library libftlm_hip.so.
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes.  No exceptions cross this
+ * boundary; the C++ header-compatible layer (ftlm_extended_hubbard_amd/csrc/host) maps status
+ * codes back onto the exceptions the reference throws.  Every entry point names the reference
+ * interface it replaces (paths relative to the reference checkout).
+ *
+ * Vector layout.  A block of R complex vectors over a sector of dimension dim = D_up * D_dn is
+ * stored element-major with the vector index innermost:  v[(iu * D_dn + id) * R + r], where
+ * iu / id are the lexicographic ranks of the up / down strings.  For R = 1 this is exactly the
+ * reference's x[iu * D_dn + id] (src/fock_basis.cpp:33-41).  Complex values are interleaved
+ * (re, im) pairs of double (FTLM_C128) or float (FTLM_C64).
+ *
+ * Streams are hipStream_t passed as void* (NULL = the default stream).  Device pointers are
+ * hipMalloc'd memory owned by the caller.  All entry points are thread-safe for distinct handles.
+ */
+#ifndef FTLM_HIP_H
+#define FTLM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference exception types in parentheses) ---- */
+#define FTLM_OK 0
+#define FTLM_EINVAL 1       /* std::invalid_argument (src/fock_basis.cpp:19-24) */
+#define FTLM_ERANGE 2       /* std::out_of_range (FockBasis::up_mask .at(), fock_basis.hpp:20-21) */
+#define FTLM_EHIP 3         /* std::runtime_error: HIP runtime failure / no device */
+#define FTLM_ENOMEM 4       /* std::bad_alloc */
+#define FTLM_EUNSUPPORTED 5 /* std::invalid_argument: outside this library's limits */
+
+/* ---- Krylov storage precision ---- */
+#define FTLM_C128 0 /* complex<double> */
+#define FTLM_C64 1  /* complex<float> storage; dots, norms, alpha/beta accumulate in FP64 */
+
+/* ---- start vectors ---- */
+#define FTLM_START_GAUSS_MT 0 /* reference: std::mt19937(seed) + normal_distribution, (re,im) per
+                                 index, src/lanczos.cpp:140-175 (generated on the host) */
+#define FTLM_START_PHASE 1    /* random phase e^{i theta}, theta ~ U[0,2pi), counter-based device RNG */
+
+/* ---- Gauss-quadrature eigensolver ---- */
+#define FTLM_QUAD_REFERENCE 0 /* capped max-pivot Jacobi, max(3000,120n) rotations at 1e-14
+                                 (src/ftlm_thermo.cpp:17-146) */
+#define FTLM_QUAD_CONVERGED 1 /* implicit-shift QL on the tridiagonal, run to convergence */
+
+/* HubbardParams (include/ftlm/hubbard_params.hpp:6-15). */
+typedef struct ftlm_params {
+  int Lx, Ly;
+  double t, tp, U, V, phi_x, phi_y;
+} ftlm_params_t;
+
+/* SpinfulHopping: H += coeff * c^dag_{to,spin} c_{from,spin} (include/ftlm/lattice.hpp:11-16). */
+typedef struct ftlm_hop {
+  int from, to, spin;
+  double re, im;
+} ftlm_hop_t;
+
+/* NearestPair (include/ftlm/lattice.hpp:19-22). */
+typedef struct ftlm_pair {
+  int i, j;
+} ftlm_pair_t;
+
+/* A Hermitian operator resident on the device: a full (N_up, N_down) sector or an orbit k-block. */
+typedef struct ftlm_op ftlm_op_t;
+
+/* ------------------------------------------------------------------ library */
+const char* ftlm_hip_version(void);    /* version_string (src/version.cpp:5) */
+const char* ftlm_hip_last_error(void); /* message of this thread's last failing call */
+int ftlm_hip_device_count(int* count);
+
+/* nsites (src/hubbard_params.cpp:5). */
+int ftlm_hip_nsites(const ftlm_params_t* p);
+
+/* build_hubbard_geometry (src/lattice.cpp:38-110): same entries, same order, including the
+ * duplicate bonds of small lattices.  n_hops / n_pairs receive the required sizes; returns
+ * FTLM_ERANGE when a capacity is too small. */
+int ftlm_hip_build_geometry(const ftlm_params_t* p, ftlm_hop_t* hops, int hop_cap, int* n_hops,
+                            ftlm_pair_t* pairs, int pair_cap, int* n_pairs);
+
+/* ------------------------------------------------------------------ full sectors */
+/* FockBasis(n_sites, n_up, n_down) (src/fock_basis.cpp:17-42) + the operator data of
+ * apply_extended_hubbard (src/hubbard_hamiltonian.cpp:51-108): enumerates the sector on the
+ * device and builds the per-spin pull-hop tables.  Limits: n_sites <= 32, each spin's string
+ * count < 2^20, at most 2047 distinct directed bonds per spin. */
+int ftlm_hip_sector_create(int n_sites, int n_up, int n_dn, const ftlm_hop_t* hops, int n_hops,
+                           const ftlm_pair_t* pairs, int n_pairs, double U, double V, ftlm_op_t** out);
+int ftlm_hip_op_destroy(ftlm_op_t* op);
+
+/* FockBasis::dim / n_up / n_down; for k-blocks dim_up = dim_dn = 0. */
+int ftlm_hip_op_dim(const ftlm_op_t* op, int64_t* dim, int64_t* dim_up, int64_t* dim_dn);
+
+/* FockBasis::up_mask / down_mask (include/ftlm/fock_basis.hpp:20-21) for indices
+ * [first, first+count), copied to host arrays.  FTLM_ERANGE outside [0, dim). */
+int ftlm_hip_sector_masks(const ftlm_op_t* op, int64_t first, int64_t count, uint64_t* up, uint64_t* dn);
+
+/* FockBasis::index_of (src/fock_basis.cpp:44-51), batched, ranked on the device by the
+ * combinatorial number system; -1 for states outside the sector.  Host arrays. */
+int ftlm_hip_index_of(const ftlm_op_t* op, const uint64_t* up, const uint64_t* dn, int64_t count,
+                      int64_t* out);
+
+/* ------------------------------------------------------------------ H * v */
+/* y = H x for a block of R vectors (layout above), device pointers, asynchronous on `stream`.
+ * Replaces apply_extended_hubbard (src/hubbard_hamiltonian.cpp:51-108) and apply_orbit_k_block
+ * (src/orbit_k_block_basis.cpp:151-220).  y is overwritten.  R in [1, 32]. */
+int ftlm_hip_apply(const ftlm_op_t* op, const void* x, void* y, int R, int dtype, void* stream);
+
+/* Drop-in for apply_extended_hubbard(p, basis, hops, pairs, x, y) on host complex<double> arrays
+ * of length dim: copies in, applies, copies out (synchronous). */
+int ftlm_hip_apply_host(const ftlm_op_t* op, const double* x, double* y);
+
+/* ------------------------------------------------------------------ Lanczos */
+/* lanczos_tridiagonal (src/lanczos.cpp:125-209): 3-vector Hermitian Lanczos without
+ * reorthogonalisation, device resident, FP64 dots/norms.  q0 (host complex<double>, length dim)
+ * may be NULL: then the reference start vector mt19937(seed)+normal is drawn.  alpha and beta
+ * must hold max_steps entries; *n_beta = len(beta); *used = steps used (0 on failure). */
+int ftlm_hip_lanczos(const ftlm_op_t* op, int max_steps, unsigned seed, const double* q0, double* alpha,
+                     double* beta, int* n_beta, int* used);
+
+/* lanczos_extrema (src/lanczos.cpp:211-231): Ritz min/max from the 120-rotation capped Jacobi. */
+int ftlm_hip_lanczos_extrema(const ftlm_op_t* op, int max_steps, unsigned seed, double* min_eval,
+                             double* max_eval, int* steps_used);
+
+/* tridiagonal_extrema (src/lanczos.cpp:106-123), evaluated on the device. */
+int ftlm_hip_tridiagonal_extrema(const double* alpha, int n, const double* beta, int nb, double* lo,
+                                 double* hi);
+
+/* ------------------------------------------------------------------ FTLM */
+typedef struct ftlm_ftlm_opts {
+  int R;              /* random start vectors per operator (FtlmParams::n_random) */
+  int M;              /* Lanczos steps (FtlmParams::lanczos_steps) */
+  unsigned seed;      /* FtlmParams::seed; sample r uses seed + r*100003 (src/ftlm_thermo.cpp:189) */
+  int start;          /* FTLM_START_* */
+  int dtype;          /* FTLM_C128 / FTLM_C64 Krylov storage */
+  int quad;           /* FTLM_QUAD_* */
+  int clamp_M_to_dim; /* 1: M_op = min(M, dim) */
+  int real_variant;   /* 1: ftlm_log_partition_real semantics (Re/Im applied separately,
+                         src/ftlm_thermo.cpp:150-211) -- identical for real H */
+} ftlm_ftlm_opts_t;
+
+/* Runs R independent Lanczos recurrences per operator, all on the device, and diagonalises every
+ * tridiagonal with the chosen quadrature eigensolver.  Outputs (host):
+ *   ritz[n_ops][R][M]   Ritz values (first used entries valid)
+ *   weight[n_ops][R][M] squared first eigenvector components
+ *   used[n_ops][R]      Lanczos steps used (0: sample skipped)
+ * alpha/beta[n_ops][R][M] may be NULL.  Operators are processed in order; each one's R samples
+ * run as one block recurrence (block H*V). */
+int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, double* ritz,
+                       double* weight, int* used, double* alpha, double* beta, void* stream);
+
+/* Per-operator FTLM thermodynamics from Ritz data, on the device:
+ *   lnZ_op[n_ops][nb] = ln dim + log-mean-exp_r ln sum_k w e^{-beta lambda}
+ *                      (ftlm_log_partition_complex, src/ftlm_thermo.cpp:226-255)
+ *   E_op[n_ops][nb]   = sum_rk w lambda e^{-beta lambda} / sum_rk w e^{-beta lambda}
+ * and the grand-canonical mixture over operators with particle numbers N[n_ops]
+ * (benchmarks/bench_ftlm_nmu_rect.cpp:186-219), for every (beta, mu):
+ *   lnZ_gc[nb][nmu], N_gc[nb][nmu] (<N>), E_gc[nb][nmu] (<H>, mu excluded).
+ * Any output pointer may be NULL. */
+int ftlm_hip_ftlm_thermo(const double* ritz, const double* weight, const int* used, int n_ops, int R, int M,
+                         const int64_t* dims, const int* N, const double* betas, int nb, const double* mus,
+                         int nmu, double* lnZ_op, double* E_op, double* lnZ_gc, double* N_gc, double* E_gc);
+
+/* ftlm_log_partition_complex / _real (src/ftlm_thermo.cpp:150-256) for a list of betas (the
+ * Lanczos data are beta independent).  opts->start = FTLM_START_GAUSS_MT reproduces the
+ * reference start vectors.  dim == 1 returns -beta*H00; dim <= 0 returns -inf. */
+int ftlm_hip_ftlm_log_partition(const ftlm_op_t* op, const double* betas, int nb, const ftlm_ftlm_opts_t* opts,
+                                double* lnZ);
+
+/* Capped-Jacobi Gauss quadrature of one tridiagonal (log_lanczos_tridiagonal_quadrature_exp,
+ * src/ftlm_thermo.cpp:83-146), on the device: Ritz values and weights of n entries. */
+int ftlm_hip_quadrature_ritz(const double* alpha, int n, const double* beta, int nb, int quad, double* ritz,
+                             double* weight);
+
+/* ------------------------------------------------------------------ step-level Krylov sessions */
+/* The three-vector workspace of the reference (LanczosComplexWorkspace, include/ftlm/lanczos.hpp:13-35)
+ * as a device object holding R recurrences: create draws the start block (FTLM_START_*; sample r
+ * uses seed + r*100003), run advances n_steps (up to max_steps in total) asynchronously on
+ * `stream`, result copies alpha/beta [max_steps][R], used [R] and alive [R] to the host
+ * (synchronising); bytes reports the session's device memory footprint. */
+typedef struct ftlm_krylov ftlm_krylov_t;
+int ftlm_hip_krylov_create(const ftlm_op_t* op, int R, int max_steps, int dtype, int start, unsigned seed,
+                           ftlm_krylov_t** out);
+int ftlm_hip_krylov_run(ftlm_krylov_t* k, int n_steps, void* stream);
+int ftlm_hip_krylov_result(ftlm_krylov_t* k, double* alpha, double* beta, int* used, int* alive);
+int ftlm_hip_krylov_bytes(const ftlm_krylov_t* k, int64_t* bytes);
+int ftlm_hip_krylov_destroy(ftlm_krylov_t* k);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FTLM_HIP_H */

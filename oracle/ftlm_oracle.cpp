@@ -703,6 +703,65 @@ int orc_apply_cols(const double* params8, int nu, int nd, const double* x, doubl
   return 0;
 }
 
+// Selected rows of y = H x with the reference's push semantics: for each requested row r, every
+// column c that can reach r (r itself and the states one listed hop away from r) is pushed
+// through the full hop loop of src/hubbard_hamiltonian.cpp:58-107 and only the contributions that
+// land on r are kept, in column order.  Lets a test check rows of a 4x4 (8,8) product without
+// the ~60 min full push.  Ranks use the closed form (no hash map).
+int orc_apply_rows(const double* params8, int nu, int nd, const double* x, long nrows, const long* rows, double* out) {
+  const Model m = model_from(params8);
+  Basis b;
+  if (!make_basis(m.Lx * m.Ly, nu, nd, &b, false)) return 1;
+  std::vector<Hop> hops;
+  std::vector<Pair> pairs;
+  geometry(m, &hops, &pairs);
+  Ranker ru, rd;
+  ru.init(b.n, nu);
+  rd.init(b.n, nd);
+  const long D = static_cast<long>(b.dns.size());
+  const cplx* xc = reinterpret_cast<const cplx*>(x);
+  for (long q = 0; q < nrows; ++q) {
+    const long r = rows[q];
+    if (r < 0 || r >= b.dim()) return 2;
+    const uint64_t ur = b.up(r), dr = b.dn(r);
+    std::vector<long> cols{r};
+    for (const Hop& h : hops) {  // states one hop away (the list holds both directions)
+      const uint64_t s = h.spin == 0 ? ur : dr;
+      if (!((s >> h.from) & 1) || ((s >> h.to) & 1)) continue;
+      const uint64_t flip = (uint64_t{1} << h.from) ^ (uint64_t{1} << h.to);
+      const uint64_t u2 = h.spin == 0 ? ur ^ flip : ur, d2 = h.spin == 1 ? dr ^ flip : dr;
+      cols.push_back(ru.rank(u2) * D + rd.rank(d2));
+    }
+    std::sort(cols.begin(), cols.end());
+    cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+    cplx acc(0.0, 0.0);
+    std::vector<cplx> ytmp(1);
+    for (long c : cols) {
+      // push column c, keep what lands on row r
+      const uint64_t u = b.up(c), d = b.dn(c);
+      const cplx vc = xc[c];
+      if (std::norm(vc) < 1e-28) continue;
+      if (c == r) {
+        cplx dg(static_cast<double>(__builtin_popcountll(u & d)) * m.U, 0.0);
+        for (const Pair& pr : pairs) dg += m.V * static_cast<double>(occ(u, d, pr.i) * occ(u, d, pr.j));
+        acc += dg * vc;
+      }
+      for (const Hop& h : hops) {
+        const uint64_t src = h.spin == 0 ? u : d;
+        if (!((src >> h.from) & 1) || ((src >> h.to) & 1)) continue;
+        const uint64_t flip = (uint64_t{1} << h.from) ^ (uint64_t{1} << h.to);
+        const uint64_t u2 = h.spin == 0 ? u ^ flip : u, d2 = h.spin == 1 ? d ^ flip : d;
+        if (ru.rank(u2) * D + rd.rank(d2) != r) continue;
+        const int nb = between(2 * h.from + h.spin, 2 * h.to + h.spin, u, d);
+        acc += (((nb & 1) ? -1.0 : 1.0) * h.coeff) * vc;
+      }
+    }
+    out[2 * q] = acc.real();
+    out[2 * q + 1] = acc.imag();
+  }
+  return 0;
+}
+
 int orc_rank_closed_form(int n, int k, const uint64_t* masks, long cnt, long* out) {
   Ranker r;
   r.init(n, k);

@@ -147,6 +147,7 @@ class Oracle(_Common):
         L.orc_apply.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp]
         L.orc_apply_cols.argtypes = [_dp, C.c_int, C.c_int, _dp, _dp, C.c_long, C.c_long]
         L.orc_rank_closed_form.argtypes = [C.c_int, C.c_int, _up, C.c_long, _lp]
+        L.orc_apply_rows.argtypes = [_dp, C.c_int, C.c_int, C.c_void_p, C.c_long, _lp, _dp]
         L.orc_gauss_start.argtypes = [C.c_long, C.c_uint, _dp]
         L.orc_lanczos.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_uint, C.c_void_p, _dp, _dp,
                                   C.POINTER(C.c_int)]
@@ -214,6 +215,14 @@ class Oracle(_Common):
         nb = C.c_int(0)
         used = self.lib.orc_lanczos(p8, nu, nd, M, 0, q0.ctypes.data, a, b, C.byref(nb))
         return used, a[:used].copy(), b[: nb.value].copy()
+
+    def apply_rows(self, p8, nu, nd, x, rows):
+        """Rows `rows` of H x (reference push semantics restricted to the columns reaching them)."""
+        rows = np.ascontiguousarray(rows, np.int64)
+        out = np.zeros(2 * len(rows))
+        assert x.dtype == np.complex128 and x.flags.c_contiguous
+        assert self.lib.orc_apply_rows(p8, nu, nd, x.ctypes.data, len(rows), rows, out) == 0
+        return out.view(np.complex128)
 
     def apply_cols(self, p8, nu, nd, x, y, c0, c1):
         assert self.lib.orc_apply_cols(p8, nu, nd, x.view(np.float64), y.view(np.float64), c0, c1) == 0

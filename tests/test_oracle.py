@@ -333,3 +333,15 @@ def test_translation_sign_closed_form(oracle):
                     c += 1
                 parity += c - 1
         assert oracle.translation_sign(u, d, 4, 2, dx, dy) == (-1.0 if parity % 2 else 1.0)
+
+
+def test_apply_rows_equals_full_push(oracle):
+    # the row-restricted push used for 4x4 (8,8) checks reproduces the full push bit for bit
+    p8 = params8(4, 3, 1.0, -0.25, 8.0, 1.0, 0.1, 0.0)
+    d = oracle.basis_dim(12, 3, 2)
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(d) + 1j * rng.standard_normal(d)
+    y = oracle.apply(p8, 3, 2, x)
+    rows = rng.integers(0, d, 300)
+    got = oracle.apply_rows(p8, 3, 2, x, rows)
+    assert np.array_equal(got.view(np.float64), y[rows].view(np.float64))
