@@ -59,6 +59,13 @@ def lib() -> C.CDLL:
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with `python -m ftlm_extended_hubbard_amd.build`")
+    # PyTorch-ROCm ships its own libamdhip64.so.7 / libhsa-runtime64.so.1.  Two HIP runtimes in
+    # one process cannot share the device, so load torch's first when it is installed: the
+    # dynamic linker then binds this library to that same runtime (identical SONAMEs).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     vp, i64p, dp, ip = C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_double), C.POINTER(C.c_int)
     L.ftlm_hip_version.restype = C.c_char_p
