@@ -127,7 +127,16 @@ struct SectorView {
   const int32_t* nuu;        // [Dup]  sum over NN pairs of u_i u_j
   const int32_t* ndd;        // [Ddn]  sum over NN pairs of d_i d_j
   const uint4* planes;       // [Dup]  bit planes of nb_u(s) = sum of u over pair partners of s
+  // slot-major copies of the hop tables: entT[k * D + i] is entry k of string i (k < cnt[i]),
+  // so a wavefront reading slot k of 64 consecutive strings touches one contiguous 256 B run
+  const uint32_t* entT_up;   // [maxcnt_up][Dup]
+  const int32_t* cnt_up;     // [Dup]
+  const uint32_t* entT_dn;   // [maxcnt_dn][Ddn]
+  const int32_t* cnt_dn;     // [Ddn]
+  int nbond_up, nbond_dn;
 };
+
+constexpr int kMaxBondsLds = 512;  // per-spin directed bonds held in LDS by the H*V kernels
 
 // Device view of an orbit-representative momentum block (filled by orbit.hip).
 struct KBlockView;
@@ -160,6 +169,9 @@ struct Sector : ftlm_op {
   DevBuf<uint32_t> xmask[2];
   DevBuf<int32_t> nuu, ndd;
   DevBuf<uint4> planes;
+  DevBuf<uint32_t> entT[2];
+  DevBuf<int32_t> cnt[2];
+  int maxcnt[2] = {0, 0};
   SectorView view() const;
 };
 

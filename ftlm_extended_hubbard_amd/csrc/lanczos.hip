@@ -88,62 +88,71 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_alpha(const double2* __res
   }
 }
 
-// C_r <- C_r - z_r (sa_r A_r) - bprev_r (sb_r B_r);  partial |C_r|^2.
-template <typename T, int RB>
-__global__ __launch_bounds__(kUpdBlock) void k_update(int64_t dim, int R, int r0, typename V2Of<T>::t* __restrict__ C,
-                                                      const typename V2Of<T>::t* __restrict__ A,
-                                                      const typename V2Of<T>::t* __restrict__ B,
-                                                      const double2* __restrict__ z, const double* __restrict__ sa,
-                                                      const double* __restrict__ sb, const double* __restrict__ bprev,
-                                                      const int* __restrict__ alive, double* __restrict__ normp,
-                                                      const int use_prev) {
+// Flat form of the update over the n = dim*R complex values of the block: consecutive threads
+// touch consecutive values (fully coalesced).  The block size is a multiple of R and so is the
+// grid stride, so each thread always sees the same vector r = tid % R and keeps one norm partial.
+template <typename T>
+__global__ __launch_bounds__(kUpdBlock) void k_update_flat(int64_t n, int R, typename V2Of<T>::t* __restrict__ C,
+                                                           const typename V2Of<T>::t* __restrict__ A,
+                                                           const typename V2Of<T>::t* __restrict__ B,
+                                                           const double2* __restrict__ z, const double* __restrict__ sa,
+                                                           const double* __restrict__ sb,
+                                                           const double* __restrict__ bprev,
+                                                           const int* __restrict__ alive, double* __restrict__ normp,
+                                                           const int use_prev) {
   using V2 = typename V2Of<T>::t;
-  double zr[RB], zi[RB], qa[RB], qb[RB], nrm[RB];
-  bool live[RB];
-#pragma unroll
-  for (int r = 0; r < RB; ++r) {
-    live[r] = alive[r0 + r] != 0;
-    zr[r] = z[r0 + r].x;
-    zi[r] = z[r0 + r].y;
-    qa[r] = sa[r0 + r];
-    qb[r] = bprev[r0 + r] * sb[r0 + r];
-    nrm[r] = 0.0;
-  }
-  for (int64_t e = static_cast<int64_t>(blockIdx.x) * kUpdBlock + threadIdx.x; e < dim;
-       e += static_cast<int64_t>(gridDim.x) * kUpdBlock) {
-    const int64_t o = e * R + r0;
-#pragma unroll
-    for (int r = 0; r < RB; ++r) {
-      const V2 c = C[o + r], a = A[o + r];
-      double wr = static_cast<double>(c.x), wi = static_cast<double>(c.y);
-      const double ar = qa[r] * static_cast<double>(a.x), ai = qa[r] * static_cast<double>(a.y);
-      // w -= z * q  (complex product, then subtract), then w -= beta_prev * q_prev
-      wr -= zr[r] * ar - zi[r] * ai;
-      wi -= zr[r] * ai + zi[r] * ar;
-      if (use_prev) {  // the reference skips this term at k = 0 (src/lanczos.cpp:186-190)
-        const V2 b = B[o + r];
-        wr -= qb[r] * static_cast<double>(b.x);
-        wi -= qb[r] * static_cast<double>(b.y);
-      }
-      V2 out;
-      out.x = static_cast<T>(live[r] ? wr : 0.0);
-      out.y = static_cast<T>(live[r] ? wi : 0.0);
-      C[o + r] = out;
-      const double sr = static_cast<double>(out.x), si = static_cast<double>(out.y);
-      nrm[r] += sr * sr + si * si;
+  const int nthr = blockDim.x;  // multiple of R
+  const int r = threadIdx.x % R;
+  const bool live = alive[r] != 0;
+  const double zr = z[r].x, zi = z[r].y, qa = sa[r], qb = bprev[r] * sb[r];
+  double nrm = 0.0;
+  for (int64_t f = static_cast<int64_t>(blockIdx.x) * nthr + threadIdx.x; f < n;
+       f += static_cast<int64_t>(gridDim.x) * nthr) {
+    const V2 c = C[f], a = A[f];
+    double wr = static_cast<double>(c.x), wi = static_cast<double>(c.y);
+    const double ar = qa * static_cast<double>(a.x), ai = qa * static_cast<double>(a.y);
+    wr -= zr * ar - zi * ai;  // w -= z * q
+    wi -= zr * ai + zi * ar;
+    if (use_prev) {  // w -= beta_prev * q_prev (skipped at k = 0, src/lanczos.cpp:186-190)
+      const V2 b = B[f];
+      wr -= qb * static_cast<double>(b.x);
+      wi -= qb * static_cast<double>(b.y);
     }
+    V2 out;
+    out.x = static_cast<T>(live ? wr : 0.0);
+    out.y = static_cast<T>(live ? wi : 0.0);
+    C[f] = out;
+    const double sr = static_cast<double>(out.x), si = static_cast<double>(out.y);
+    nrm += sr * sr + si * si;
   }
-  __shared__ double red[kUpdBlock / 64][RB];
-#pragma unroll
-  for (int r = 0; r < RB; ++r) {
-    const double v = wsum(nrm[r]);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][r] = v;
-  }
+  __shared__ double red[kUpdBlock];
+  red[threadIdx.x] = nrm;
   __syncthreads();
-  if (threadIdx.x < RB) {
+  if (threadIdx.x < R) {
     double s = 0.0;
-    for (int w = 0; w < kUpdBlock / 64; ++w) s += red[w][threadIdx.x];
-    normp[static_cast<int64_t>(blockIdx.x) * R + r0 + threadIdx.x] = s;
+    for (int t = threadIdx.x; t < nthr; t += R) s += red[t];
+    normp[static_cast<int64_t>(blockIdx.x) * R + threadIdx.x] = s;
+  }
+}
+
+// |q_r|^2 partials of the start block, same flat layout.
+template <typename T>
+__global__ __launch_bounds__(kUpdBlock) void k_norm_flat(int64_t n, int R, const typename V2Of<T>::t* __restrict__ q,
+                                                         double* __restrict__ normp) {
+  const int nthr = blockDim.x;
+  double nrm = 0.0;
+  for (int64_t f = static_cast<int64_t>(blockIdx.x) * nthr + threadIdx.x; f < n;
+       f += static_cast<int64_t>(gridDim.x) * nthr) {
+    const auto v = q[f];
+    nrm += static_cast<double>(v.x) * static_cast<double>(v.x) + static_cast<double>(v.y) * static_cast<double>(v.y);
+  }
+  __shared__ double red[kUpdBlock];
+  red[threadIdx.x] = nrm;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nthr; t += R) s += red[t];
+    normp[static_cast<int64_t>(blockIdx.x) * R + threadIdx.x] = s;
   }
 }
 
@@ -194,31 +203,6 @@ __global__ void k_start_norm(const double* __restrict__ normp, int nblk, int R, 
     for (int w = 0; w < kRedBlock / 64; ++w) s += sh[w];
     const double n0 = sqrt(s);
     sa[r] = n0 < 1e-18 ? -1.0 : 1.0 / n0;  // -1 flags the unit-vector fallback
-  }
-}
-
-template <typename T>
-__global__ void k_norm_partial(int64_t dim, int R, const typename V2Of<T>::t* __restrict__ q, double* normp) {
-  // one partial per (block, r); R <= 32
-  double acc[32];
-  for (int r = 0; r < R; ++r) acc[r] = 0.0;
-  for (int64_t e = static_cast<int64_t>(blockIdx.x) * kUpdBlock + threadIdx.x; e < dim;
-       e += static_cast<int64_t>(gridDim.x) * kUpdBlock)
-    for (int r = 0; r < R; ++r) {
-      const auto v = q[e * R + r];
-      acc[r] += static_cast<double>(v.x) * static_cast<double>(v.x) + static_cast<double>(v.y) * static_cast<double>(v.y);
-    }
-  __shared__ double red[kUpdBlock / 64];
-  for (int r = 0; r < R; ++r) {
-    const double v = wsum(acc[r]);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double s = 0.0;
-      for (int w = 0; w < kUpdBlock / 64; ++w) s += red[w];
-      normp[static_cast<int64_t>(blockIdx.x) * R + r] = s;
-    }
-    __syncthreads();
   }
 }
 
@@ -285,41 +269,26 @@ struct Krylov {
     alive.alloc(R);
     used.alloc(R);
     nblk_hv = hv_partial_blocks(o);
-    nblk_upd = static_cast<int>(std::min<int64_t>(kUpdMaxGrid, (dim + kUpdBlock - 1) / kUpdBlock));
+    nblk_upd = static_cast<int>(std::min<int64_t>(kUpdMaxGrid, (dim * R + upd_threads() - 1) / upd_threads()));
     dotp.alloc(static_cast<size_t>(nblk_hv) * R);
     normp.alloc(static_cast<size_t>(std::max(nblk_hv, nblk_upd)) * R);
   }
 
+  // block size: the largest multiple of R not above kUpdBlock (each thread keeps one vector r)
+  int upd_threads() const { return (kUpdBlock / R) * R; }
+
   void launch_update(void* C, const void* A, const void* B, int use_prev, hipStream_t st) {
-    const dim3 g(nblk_upd), b(kUpdBlock);
-    for (int r0 = 0; r0 < R;) {
-      int rb = 1;
-      const int maxrb = dtype == FTLM_C64 ? 16 : 8;
-      while (rb * 2 <= maxrb && r0 + rb * 2 <= R) rb *= 2;
-#define FTLM_UPD(TT, RBV)                                                                                     \
-  hipLaunchKernelGGL((k_update<TT, RBV>), g, b, 0, st, dim, R, r0, static_cast<V2Of<TT>::t*>(C),              \
-                     static_cast<const V2Of<TT>::t*>(A), static_cast<const V2Of<TT>::t*>(B), z.p, sa.p, sb.p, \
-                     bprev.p, alive.p, normp.p, use_prev)
-      if (dtype == FTLM_C64) {
-        switch (rb) {
-          case 16: FTLM_UPD(float, 16); break;
-          case 8: FTLM_UPD(float, 8); break;
-          case 4: FTLM_UPD(float, 4); break;
-          case 2: FTLM_UPD(float, 2); break;
-          default: FTLM_UPD(float, 1); break;
-        }
-      } else {
-        switch (rb) {
-          case 8: FTLM_UPD(double, 8); break;
-          case 4: FTLM_UPD(double, 4); break;
-          case 2: FTLM_UPD(double, 2); break;
-          default: FTLM_UPD(double, 1); break;
-        }
-      }
-#undef FTLM_UPD
-      r0 += rb;
-    }
-    check_launch("k_update");
+    const dim3 g(nblk_upd), b(upd_threads());
+    const int64_t n = dim * R;
+    if (dtype == FTLM_C64)
+      hipLaunchKernelGGL(k_update_flat<float>, g, b, 0, st, n, R, static_cast<float2*>(C),
+                         static_cast<const float2*>(A), static_cast<const float2*>(B), z.p, sa.p, sb.p, bprev.p,
+                         alive.p, normp.p, use_prev);
+    else
+      hipLaunchKernelGGL(k_update_flat<double>, g, b, 0, st, n, R, static_cast<double2*>(C),
+                         static_cast<const double2*>(A), static_cast<const double2*>(B), z.p, sa.p, sb.p, bprev.p,
+                         alive.p, normp.p, use_prev);
+    check_launch("k_update_flat");
   }
 
   // Prepare the start block in buf[0]: normalise lazily and reset the recurrence state.
@@ -332,11 +301,12 @@ struct Krylov {
     bprev.upload(dz, st);
     FTLM_HIP_CHECK(hipMemsetAsync(alpha.p, 0, alpha.n * sizeof(double), st));
     FTLM_HIP_CHECK(hipMemsetAsync(beta.p, 0, beta.n * sizeof(double), st));
-    const dim3 g(nblk_upd), b(kUpdBlock);
+    const dim3 g(nblk_upd), b(upd_threads());
     if (dtype == FTLM_C64)
-      hipLaunchKernelGGL(k_norm_partial<float>, g, b, 0, st, dim, R, static_cast<const float2*>((void*)buf[0].p), normp.p);
+      hipLaunchKernelGGL(k_norm_flat<float>, g, b, 0, st, dim * R, R, static_cast<const float2*>((void*)buf[0].p),
+                         normp.p);
     else
-      hipLaunchKernelGGL(k_norm_partial<double>, g, b, 0, st, dim, R, static_cast<const double2*>((void*)buf[0].p),
+      hipLaunchKernelGGL(k_norm_flat<double>, g, b, 0, st, dim * R, R, static_cast<const double2*>((void*)buf[0].p),
                          normp.p);
     hipLaunchKernelGGL(k_start_norm, dim3(R), dim3(kRedBlock), 0, st, normp.p, nblk_upd, R, sa.p);
     if (dtype == FTLM_C64)
@@ -362,8 +332,8 @@ struct Krylov {
 
   // One Lanczos step k with (A, B, C) = buffers (a, b, c).
   void step(int k, int a, int b, int c, hipStream_t st) {
-    hv_apply(op, buf[a].p, buf[c].p, R, dtype, sa.p, dotp.p, st, force_real);
-    hipLaunchKernelGGL(k_reduce_alpha, dim3(R), dim3(kRedBlock), 0, st, dotp.p, nblk_hv, R, alive.p, z.p, alpha.p, k);
+    const int nb = hv_apply(op, buf[a].p, buf[c].p, R, dtype, sa.p, dotp.p, st, force_real);
+    hipLaunchKernelGGL(k_reduce_alpha, dim3(R), dim3(kRedBlock), 0, st, dotp.p, nb, R, alive.p, z.p, alpha.p, k);
     launch_update(buf[c].p, buf[a].p, buf[b].p, k > 0 ? 1 : 0, st);
     hipLaunchKernelGGL(k_reduce_beta, dim3(R), dim3(kRedBlock), 0, st, normp.p, nblk_upd, R, alive.p, used.p, beta.p,
                        sa.p, sb.p, bprev.p, k);

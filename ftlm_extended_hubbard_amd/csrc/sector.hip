@@ -131,6 +131,15 @@ __global__ void k_hop_fill(const uint32_t* masks, int D, const BondDev* bonds, i
   }
 }
 
+// CSR -> slot-major copy of a hop table (entT[k * D + i] = entry k of string i).
+__global__ void k_hop_transpose(const int32_t* off, const uint32_t* ent, int D, uint32_t* entT, int32_t* cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= D) return;
+  const int o = off[i], c = off[i + 1] - o;
+  cnt[i] = c;
+  for (int k = 0; k < c; ++k) entT[static_cast<int64_t>(k) * D + i] = ent[o + k];
+}
+
 // Per-string pieces of the diagonal  V * sum_pairs (u_i + d_i)(u_j + d_j):
 //   same-spin part sum_pairs s_i s_j, and bit planes of nb(x) = sum over pair partners of s.
 __global__ void k_diag_tables(const uint32_t* masks, int D, const int2* pairs, int np, int32_t* same,
@@ -238,6 +247,12 @@ SectorView Sector::view() const {
   v.nuu = nuu.p;
   v.ndd = ndd.p;
   v.planes = planes.p;
+  v.entT_up = entT[0].p;
+  v.cnt_up = cnt[0].p;
+  v.entT_dn = entT[1].p;
+  v.cnt_dn = cnt[1].p;
+  v.nbond_up = nbond[0];
+  v.nbond_dn = nbond[1];
   return v;
 }
 
@@ -267,7 +282,8 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
   S->real_h = true;
   for (int sp = 0; sp < 2; ++sp) {
     const int nb = static_cast<int>(bonds[sp].size());
-    FTLM_REQUIRE(nb <= static_cast<int>(kBondMask), FTLM_EUNSUPPORTED, "too many distinct directed bonds");
+    FTLM_REQUIRE(nb <= kMaxBondsLds && nb <= static_cast<int>(kBondMask), FTLM_EUNSUPPORTED,
+                 "more than 512 distinct directed bonds per spin");
     S->nbond[sp] = nb;
     std::vector<BondDev> bd(static_cast<size_t>(std::max(nb, 1)));
     std::vector<double2> cf(static_cast<size_t>(std::max(nb, 1)));
@@ -308,6 +324,14 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
     hipLaunchKernelGGL(k_hop_fill, dim3(grid_for(D, 256)), dim3(256), 0, 0, masks, D, dbonds.p, nb, S->off[sp].p,
                        S->ent[sp].p, n, k);
     check_launch("k_hop_fill");
+    int mc = 0;
+    for (int i = 0; i < D; ++i) mc = std::max(mc, hc[i]);
+    S->maxcnt[sp] = mc;
+    S->entT[sp].alloc(static_cast<size_t>(std::max(mc, 1)) * D);
+    S->cnt[sp].alloc(static_cast<size_t>(D));
+    hipLaunchKernelGGL(k_hop_transpose, dim3(grid_for(D, 256)), dim3(256), 0, 0, S->off[sp].p, S->ent[sp].p, D,
+                       S->entT[sp].p, S->cnt[sp].p);
+    check_launch("k_hop_transpose");
     FTLM_HIP_CHECK(hipDeviceSynchronize());
   }
 
