@@ -69,6 +69,24 @@ def kernel_roofline(F, L, op, R, dtype, torch, reps=5):
     return t, B
 
 
+def profile_hv(F, L, torch, R):
+    """Workload of the committed rocprofv3 summaries: 3 H*V launches (R complex64 vectors) on the
+    4x4 (8,8) sector, and one copy of 4 GiB (the FETCH_SIZE / WRITE_SIZE calibration kernel)."""
+    op = F.HubbardOperator.from_params(F.HubbardParams(**MODEL), NU, ND)
+    dim = op.dim()
+    X = torch.randn(dim * R, dtype=torch.complex64, device="cuda")
+    Y = torch.empty_like(X)
+    for _ in range(3):
+        op.apply_device(X, Y, R=R, dtype=L.FTLM_C64)
+    torch.cuda.synchronize()
+    del X, Y
+    a = torch.empty(1 << 30, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    log("profile workload done: 3 x H*V (R=%d, c64) + 4 GiB copy" % R)
+
+
 def traffic_from_profile():
     path = os.path.join(ROOT, "profiles", "hv_traffic_4x4_c64_r16.json")
     if not os.path.exists(path):
@@ -129,10 +147,21 @@ def cpu_baseline(budget_s=12.0):
 
 
 def ftlm_lnz(F, L, op, R, M, dtype, betas):
+    """lnZ(beta) from one R-sample FTLM run, with the converged (QL) Gauss rule and with the
+    reference's capped Jacobi applied to the same tridiagonals."""
+    import numpy as np
+
     par = F.FtlmParams(n_random=R, lanczos_steps=M, seed=1, start=L.FTLM_START_PHASE, dtype=dtype,
-                       quad=L.FTLM_QUAD_REFERENCE)
-    rd = F.ftlm_ritz([op], par)
-    return F.ftlm_thermo(rd, betas)["lnZ"][0]
+                       quad=L.FTLM_QUAD_CONVERGED)
+    rd = F.ftlm_ritz([op], par, keep_tridiagonals=True)
+    conv = F.ftlm_thermo(rd, betas)["lnZ"][0]
+    cap = F.RitzData(np.zeros_like(rd.ritz), np.zeros_like(rd.weight), rd.used, rd.dims)
+    for r in range(R):
+        n = int(rd.used[0, r])
+        lam, w = F.quadrature_ritz(rd.alpha[0, r, :n], rd.beta[0, r, : n - 1], L.FTLM_QUAD_REFERENCE)
+        cap.ritz[0, r, :n], cap.weight[0, r, :n] = lam, w
+    capped = F.ftlm_thermo(cap, betas)["lnZ"][0]
+    return conv, capped
 
 
 def main():
@@ -145,6 +174,8 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--profile-hv", action="store_true",
+                    help="only run 3 block H*V (R vectors, complex64) and one 4 GiB device copy, for rocprofv3")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -159,6 +190,9 @@ def main():
 
     assert torch.cuda.is_available(), "no GPU"
     K, W, R = args.steps, args.warmup, args.R
+    if args.profile_hv:
+        profile_hv(F, L, torch, R)
+        return
     t0 = time.perf_counter()
     op = F.HubbardOperator.from_params(F.HubbardParams(**MODEL), NU, ND)
     torch.cuda.synchronize()
@@ -193,13 +227,13 @@ def main():
     # ---- roofline of the dominant kernel (block H*V)
     t_k, B = kernel_roofline(F, L, op, R, L.FTLM_C64, torch)
     achieved = B / t_k / 1e9
-    log(f"k_hv_sector c64 R={R}: {t_k * 1e3:.3f} ms/launch, {achieved:.0f} GB/s algorithmic")
+    log(f"H*V c64 R={R}: {t_k * 1e3:.3f} ms, {achieved:.0f} GB/s algorithmic")
     t_k1, B1 = kernel_roofline(F, L, op, 1, L.FTLM_C128, torch)
-    log(f"k_hv_sector c128 R=1: {t_k1 * 1e3:.3f} ms/launch, {B1 / t_k1 / 1e9:.0f} GB/s algorithmic")
+    log(f"H*V c128 R=1: {t_k1 * 1e3:.3f} ms, {B1 / t_k1 / 1e9:.0f} GB/s algorithmic")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
-                "kernel": f"k_hv_sector<float,{min(R, 16)},false>", "launch_ms": round(t_k * 1e3, 4),
-                "algorithmic_bytes_per_launch": B,
+                "kernel": f"H*V = k_hv_dn<float,{min(R, 16)},false> + k_hv_up<float,{min(R, 16)},false>",
+                "launch_ms": round(t_k * 1e3, 4), "algorithmic_bytes_per_launch": B,
                 "plain_c128_r1": {"launch_ms": round(t_k1 * 1e3, 4), "achieved": round(B1 / t_k1 / 1e9, 1)}}
 
     # ---- |dlnZ| of the batched complex64 FTLM against the FP64 path, same start vectors
@@ -208,13 +242,16 @@ def main():
         betas = [0.5, 2.0, 8.0]
         M = args.validate_M
         t1 = time.perf_counter()
-        l64 = ftlm_lnz(F, L, op, R, M, L.FTLM_C64, betas)
-        l128 = ftlm_lnz(F, L, op, R, M, L.FTLM_C128, betas)
-        dlnz = {"M": M, "R": R, "betas": betas, "lnZ_c64": [float(v) for v in l64],
-                "lnZ_c128": [float(v) for v in l128],
-                "abs_diff": [float(abs(p - q)) for p, q in zip(l64, l128)],
+        c64, k64 = ftlm_lnz(F, L, op, R, M, L.FTLM_C64, betas)
+        c128, k128 = ftlm_lnz(F, L, op, R, M, L.FTLM_C128, betas)
+        dlnz = {"M": M, "R": R, "betas": betas, "quadrature": "converged QL (gate); capped Jacobi reported",
+                "lnZ_c64": [float(v) for v in c64], "lnZ_fp64": [float(v) for v in c128],
+                "abs_diff": [float(abs(p - q)) for p, q in zip(c64, c128)],
+                "capped_lnZ_c64": [float(v) for v in k64], "capped_lnZ_fp64": [float(v) for v in k128],
+                "capped_abs_diff": [float(abs(p - q)) for p, q in zip(k64, k128)],
+                "pass_1e-6": bool(max(abs(p - q) for p, q in zip(c64, c128)) <= 1e-6),
                 "seconds": round(time.perf_counter() - t1, 1)}
-        log(f"dlnZ (M={M}): {dlnz['abs_diff']}")
+        log(f"dlnZ c64 vs fp64 (M={M}, converged): {dlnz['abs_diff']}; capped: {dlnz['capped_abs_diff']}")
 
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:

@@ -127,13 +127,16 @@ struct SectorView {
   const int32_t* nuu;        // [Dup]  sum over NN pairs of u_i u_j
   const int32_t* ndd;        // [Ddn]  sum over NN pairs of d_i d_j
   const uint4* planes;       // [Dup]  bit planes of nb_u(s) = sum of u over pair partners of s
-  // slot-major copies of the hop tables: entT[k * D + i] is entry k of string i (k < cnt[i]),
-  // so a wavefront reading slot k of 64 consecutive strings touches one contiguous 256 B run
-  const uint32_t* entT_up;   // [maxcnt_up][Dup]
+  // slot-major copies of the hop tables: entT[k * stride + i] is entry k of string i
+  // (k < cnt[i], zero beyond; stride = D rounded up to 64), so slot k of consecutive strings is
+  // one contiguous run
+  const uint32_t* entT_up;   // [maxcnt_up][stride_up]
   const int32_t* cnt_up;     // [Dup]
-  const uint32_t* entT_dn;   // [maxcnt_dn][Ddn]
+  const uint32_t* entT_dn;   // [maxcnt_dn][stride_dn]
   const int32_t* cnt_dn;     // [Ddn]
   int nbond_up, nbond_dn;
+  int entT_stride_up, entT_stride_dn;
+  int maxcnt_up, maxcnt_dn;
 };
 
 constexpr int kMaxBondsLds = 512;  // per-spin directed bonds held in LDS by the H*V kernels
@@ -172,6 +175,10 @@ struct Sector : ftlm_op {
   DevBuf<uint32_t> entT[2];
   DevBuf<int32_t> cnt[2];
   int maxcnt[2] = {0, 0};
+  int entT_stride[2] = {0, 0};
+  // per-XCD tile counters of the two H*V passes ([0..7] rows pass, [8..15] column pass); each
+  // pass zeroes the other pass's counters, so applies on one operator must be stream-ordered
+  DevBuf<int32_t> tile_ctr;
   SectorView view() const;
 };
 

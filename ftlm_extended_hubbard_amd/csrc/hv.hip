@@ -5,29 +5,35 @@
 // sources, so no atomics and no hashing are needed:
 //
 //   y[iu,id] = diag(u,d) x[iu,id]
-//            + sum_{up bonds b into u}   s * c_b * x[iu', id]   (coalesced: the whole wavefront
-//                                                               reads one contiguous row segment)
-//            + sum_{down bonds b into d} s * c_b * x[iu, id']   (gather inside row iu)
+//            + sum_{down bonds b into d} s * c_b * x[iu, id']   (sources inside row iu)
+//            + sum_{up bonds b into u}   s * c_b * x[iu', id]   (sources inside column id)
 //
 // with the interleaved Jordan-Wigner sign s = (-1)^(own-spin parity from the table
 //                                                 + popcount(other string & bond mask)).
 // The sign depends only on modes strictly between the bond ends, so evaluating it on the row
 // (target) state equals the reference's evaluation on the column state.
-//
 // diag(u,d) = U popcount(u&d) + V (sum_pairs u_i u_j + sum_pairs d_i d_j + sum_s d_s nb_u(s)),
 // the last term from four bit planes of nb_u (number of occupied pair partners of s).
 //
-// A launch handles RB vectors [r0, r0+RB) of a block of RT vectors laid out v[elem*RT + r];
-// decode work (tables, signs, addresses) is shared by the RB vectors.
-//
-// Two kernels:
-//  k_hv_rows  (D_dn >= 128): a workgroup owns a tile of 256 consecutive ids of ONE row iu, so
-//             everything that depends on u is wavefront-uniform: the up-bond loop runs on scalar
-//             loads, and the down-bond coefficients are pre-signed with the row's cross-spin JW
-//             parity into an LDS table once per row.  Down-bond entries are read slot-major
-//             (one contiguous 256 B run per wavefront and slot).  Tiles are dealt to the 8 XCDs
-//             in contiguous row ranges so that the blocks gathering inside a row share one L2.
-//  k_hv_flat  (small sectors): one thread per flattened element, no row uniformity assumed.
+// Cache structure.  Every source element is read by ~34 outputs of its row (down bonds) and ~34
+// of its column (up bonds).  One row of R vectors is D_dn * R * sizeof(complex) bytes (206 KB for
+// one complex128 vector, 1.65 MB for 16 complex64 vectors), so an XCD's 4 MB L2 can hold the
+// few rows it is working on but not the ~34 other rows an up bond reaches.  Large sectors
+// therefore use two passes:
+//   k_hv_dn  diagonal + down bonds, tiles = (row, chunk of ids) in row-major order.  u is
+//            uniform per tile: the down-bond coefficients are pre-signed with the row's
+//            cross-spin parity in LDS.
+//   k_hv_up  up bonds, y += ... ; tiles = (stripe of W adjacent ids, chunk of rows) in
+//            stripe-major order; a stripe of all rows (D_up * W * R * s) is kept below ~3 MB.
+// Workgroups are dealt to the 8 XCDs round-robin (blockIdx % 8).  Each XCD owns a contiguous
+// share of the tiles and hands them out through an atomic counter, so the tiles in flight on an
+// XCD always form one contiguous window (a few rows, or one stripe) that its L2 holds, no
+// matter how unevenly its workgroups progress.  Hop tables are slot-major, so the table slice
+// of the window is contiguous too.
+// The R vectors of an element are spread over L lanes of 16 bytes, so a wavefront load reads
+// whole 128-byte lines.  Per-hop addressing is a 32-bit offset from a per-row (per-column) base
+// pointer; complex updates are four explicit FMAs.  Small sectors (D_dn < 128) use one flat
+// single-pass kernel.
 
 #include <cstdlib>
 
@@ -53,73 +59,85 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 constexpr int kHvBlock = 256;
-constexpr int kHvMaxGrid = 8192;  // bounds the dot-partial array
-constexpr int kXcd = 8;           // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kHvMaxGrid = 8192;   // total dot-partial blocks of one apply
+constexpr int kPassMaxGrid = 4096; // per pass
+constexpr int kXcd = 8;            // MI355X: 8 XCDs, workgroups dealt round-robin
 
-// acc += c * v for RB vectors (real or complex coefficient).
-template <typename T, int RB, bool REAL>
-__device__ __forceinline__ void axpy_block(T (&ar)[RB], T (&ai)[RB], const typename Cx<T>::v2* __restrict__ src,
-                                           T cr, T ci) {
-#pragma unroll
-  for (int r = 0; r < RB; ++r) {
-    const typename Cx<T>::v2 v = src[r];
-    if (REAL) {
-      ar[r] += cr * v.x;
-      ai[r] += cr * v.y;
-    } else {
-      ar[r] += cr * v.x - ci * v.y;
-      ai[r] += cr * v.y + ci * v.x;
-    }
-  }
-}
-
-// Store y = xs * acc and accumulate <xs x_own | y> (FP64).
+// Lane layout of an element's RB vectors: L lanes of 16 bytes (VPL vectors each).
 template <typename T, int RB>
-__device__ __forceinline__ void finish_element(const T (&ar)[RB], const T (&ai)[RB], const typename Cx<T>::v2 (&xo)[RB],
-                                               typename Cx<T>::v2* __restrict__ ye, const double* __restrict__ xs,
-                                               int r0, bool want_dot, double (&dre)[RB], double (&dim)[RB]) {
+struct Split {
+  static constexpr int BYTES = RB * static_cast<int>(sizeof(typename Cx<T>::v2));
+  static constexpr int L = BYTES >= 32 ? BYTES / 16 : 1;
+  static constexpr int VPL = RB / L;
+  static constexpr int EPB = kHvBlock / L;  // elements per pass of a workgroup
+  static_assert(RB % L == 0 && kHvBlock % L == 0, "bad vector split");
+};
+
+template <typename T>
+__device__ __forceinline__ T fmaT(T a, T b, T c) {
+  return fma(a, b, c);
+}
+
+// acc += (cr + i ci) * v for N vectors.
+template <typename T, int N, bool REAL>
+__device__ __forceinline__ void axpy_block(T (&ar)[N], T (&ai)[N], const typename Cx<T>::v2* __restrict__ src, T cr,
+                                           T ci) {
 #pragma unroll
-  for (int r = 0; r < RB; ++r) {
-    T yr = ar[r], yi = ai[r];
-    double sx = 1.0;
-    if (xs) {
-      sx = xs[r0 + r];
-      yr = static_cast<T>(static_cast<double>(yr) * sx);
-      yi = static_cast<T>(static_cast<double>(yi) * sx);
-    }
-    typename Cx<T>::v2 o;
-    o.x = yr;
-    o.y = yi;
-    ye[r] = o;
-    if (want_dot) {
-      const double qr = sx * static_cast<double>(xo[r].x), qi = sx * static_cast<double>(xo[r].y);
-      dre[r] += qr * static_cast<double>(yr) + qi * static_cast<double>(yi);
-      dim[r] += qr * static_cast<double>(yi) - qi * static_cast<double>(yr);
+  for (int r = 0; r < N; ++r) {
+    const typename Cx<T>::v2 v = src[r];
+    ar[r] = fmaT(cr, v.x, ar[r]);
+    ai[r] = fmaT(cr, v.y, ai[r]);
+    if (!REAL) {
+      ar[r] = fmaT(-ci, v.y, ar[r]);
+      ai[r] = fmaT(ci, v.x, ai[r]);
     }
   }
 }
 
-template <int RB>
-__device__ __forceinline__ void block_dot_store(const double (&dre)[RB], const double (&dim)[RB], double2* dotp, int RT,
-                                                int r0) {
-  __shared__ double red[kHvBlock / 64][RB][2];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+// acc += (cr + i ci) * v for N vectors already in registers.
+template <typename T, int N, bool REAL>
+__device__ __forceinline__ void axpy_regs(T (&ar)[N], T (&ai)[N], const typename Cx<T>::v2 (&v)[N], T cr, T ci) {
 #pragma unroll
-  for (int r = 0; r < RB; ++r) {
-    const double a = wave_sum(dre[r]), b = wave_sum(dim[r]);
-    if (lane == 0) {
-      red[wv][r][0] = a;
-      red[wv][r][1] = b;
+  for (int r = 0; r < N; ++r) {
+    ar[r] = fmaT(cr, v[r].x, ar[r]);
+    ai[r] = fmaT(cr, v[r].y, ai[r]);
+    if (!REAL) {
+      ar[r] = fmaT(-ci, v[r].y, ar[r]);
+      ai[r] = fmaT(ci, v[r].x, ai[r]);
     }
+  }
+}
+
+constexpr int kUnroll = 8;  // hops whose gathers are issued together
+
+// Hop entries of a tile are staged in LDS (16-byte loads, then LDS reads shared by the L lanes
+// of an element) when a tile is at most 64 elements wide; wider single-vector tiles read them
+// straight from the L2-resident table, which costs fewer vector-memory instructions.
+template <int EPB>
+struct StageEntries {
+  static constexpr bool value = EPB <= 64;
+};
+
+// Block reduction of per-lane dot partials; lane t holds vectors [r0 + (t % L) * VPL, ... + VPL).
+template <int L, int VPL, int RB>
+__device__ __forceinline__ void store_dot(const double (&dre)[VPL], const double (&dim)[VPL], double2* dotp,
+                                          int64_t slot, int RT, int r0) {
+  __shared__ double red[kHvBlock][VPL][2];
+  __syncthreads();
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) {
+    red[threadIdx.x][v][0] = dre[v];
+    red[threadIdx.x][v][1] = dim[v];
   }
   __syncthreads();
   if (threadIdx.x < RB) {
+    const int s2 = threadIdx.x / VPL, v2 = threadIdx.x % VPL;
     double a = 0.0, b = 0.0;
-    for (int w = 0; w < kHvBlock / 64; ++w) {
-      a += red[w][threadIdx.x][0];
-      b += red[w][threadIdx.x][1];
+    for (int j = s2; j < kHvBlock; j += L) {
+      a += red[j][v2][0];
+      b += red[j][v2][1];
     }
-    dotp[static_cast<int64_t>(blockIdx.x) * RT + r0 + threadIdx.x] = make_double2(a, b);
+    dotp[slot * RT + r0 + threadIdx.x] = make_double2(a, b);
   }
 }
 
@@ -129,81 +147,270 @@ __device__ __forceinline__ double diag_value(const SectorView& S, uint32_t u, ui
   return S.U * static_cast<double>(__popc(u & d)) + S.V * static_cast<double>(S.nuu[iu] + S.ndd[id] + cross);
 }
 
-// ------------------------------------------------------------------ row-tiled kernel
-template <typename T, int RB, bool REAL>
-__global__ __launch_bounds__(kHvBlock) void k_hv_rows(const SectorView S, const typename Cx<T>::v2* __restrict__ x,
-                                                      typename Cx<T>::v2* __restrict__ y, const int RT, const int r0,
-                                                      const double* __restrict__ xs, double2* __restrict__ dotp,
-                                                      const int nchunk, const int64_t ntiles) {
-  using V2 = typename Cx<T>::v2;
-  __shared__ T s_cr[kMaxBondsLds], s_ci[kMaxBondsLds];
-  double dre[RB], dimg[RB];
-#pragma unroll
-  for (int r = 0; r < RB; ++r) dre[r] = dimg[r] = 0.0;
+// ------------------------------------------------------------------ per-XCD tile queues
+// Each pass owns 9 counters: [0..7] next tile of each XCD's share, [8] finished workgroups.  A
+// workgroup (on XCD blockIdx % 8) repeatedly takes the next tile of its XCD's contiguous share,
+// so the tiles in flight on one XCD always form one contiguous window whatever the relative
+// speed of its workgroups.  The last workgroup to finish returns the counters to zero.
+__device__ __forceinline__ int next_tile(int* ctr, int xcd, int* s_tile) {
+  __syncthreads();
+  if (threadIdx.x == 0) *s_tile = atomicAdd(&ctr[xcd], 1);
+  __syncthreads();
+  return *s_tile;
+}
 
-  const int xcd = blockIdx.x % kXcd, lb = blockIdx.x / kXcd, nbx = gridDim.x / kXcd;
-  const int64_t per = (ntiles + kXcd - 1) / kXcd;
-  const int64_t t_begin = xcd * per, t_end = min(ntiles, t_begin + per);
-  int cur_row = -1;
+__device__ __forceinline__ void queue_done(int* ctr) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&ctr[kXcd], 1) == static_cast<int>(gridDim.x) - 1) {
+      for (int i = 0; i < kXcd; ++i) atomicExch(&ctr[i], 0);
+      atomicExch(&ctr[kXcd], 0);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ rows pass: diag + down bonds
+// Tiles: (row iu, chunk of EPB ids), row-major; XCD x owns tiles [x*per, (x+1)*per).
+template <typename T, int RB, bool REAL>
+__global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const typename Cx<T>::v2* __restrict__ x,
+                                                    typename Cx<T>::v2* __restrict__ y, const int RT, const int r0,
+                                                    const double* __restrict__ xs, double2* __restrict__ dotp,
+                                                    int* __restrict__ ctr, const int nchunk, const int ntiles) {
+  using V2 = typename Cx<T>::v2;
+  using SP = Split<T, RB>;
+  constexpr int L = SP::L, VPL = SP::VPL, EPB = SP::EPB;
+  constexpr bool kStage = StageEntries<EPB>::value;
+  __shared__ T s_cr[kMaxBondsLds], s_ci[kMaxBondsLds];
+  __shared__ int s_tile;
+  extern __shared__ uint4 s_ent4[];  // this tile's entries, [maxcnt][EPB] uint32
+  const uint32_t* s_ent = reinterpret_cast<const uint32_t*>(s_ent4);
+  const int xcd = blockIdx.x % kXcd;
+  const int sub = static_cast<int>(threadIdx.x) % L, el = static_cast<int>(threadIdx.x) / L;
+  const int rr = r0 + sub * VPL;
+  const int per = (ntiles + kXcd - 1) / kXcd;
+  const int t0 = min(ntiles, xcd * per), nmine = min(ntiles, t0 + per) - t0;
   const int Ddn = S.Ddn;
-  for (int64_t tile = t_begin + lb; tile < t_end; tile += nbx) {
-    const int iu = static_cast<int>(tile / nchunk);
-    const int chunk = static_cast<int>(tile - static_cast<int64_t>(iu) * nchunk);
+  const int64_t row_stride = static_cast<int64_t>(Ddn) * RT;
+  const int mc = S.maxcnt_dn, estride = S.entT_stride_dn;
+  double sxv[VPL], dre[VPL], dimg[VPL];
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) {
+    sxv[v] = xs ? xs[rr + v] : 1.0;
+    dre[v] = dimg[v] = 0.0;
+  }
+  int cur_row = -1, cur_chunk = -1;
+  for (int local = next_tile(ctr, xcd, &s_tile); local < nmine; local = next_tile(ctr, xcd, &s_tile)) {
+    const int tile = t0 + local;
+    const int iu = tile / nchunk;
+    const int chunk = tile - iu * nchunk;
     const uint32_t u = S.up_mask[iu];
-    if (iu != cur_row) {  // uniform: every thread of the block sees the same tile
-      __syncthreads();
+    // uniform branches: the whole block works on one tile
+    if (kStage && chunk != cur_chunk) {  // the chunk's entries: one 16-byte load per 4 strings and slot
+      const uint4* src4 = reinterpret_cast<const uint4*>(S.entT_dn + chunk * EPB);
+      for (int i = threadIdx.x; i < mc * (EPB / 4); i += kHvBlock) {
+        const int k = i / (EPB / 4), q = i - k * (EPB / 4);
+        s_ent4[i] = src4[(static_cast<int64_t>(k) * estride) / 4 + q];
+      }
+      cur_chunk = chunk;
+    }
+    if (iu != cur_row) {
       for (int b = threadIdx.x; b < S.nbond_dn; b += kHvBlock) {
         const double2 c = S.coef_dn[b];
         const bool neg = (__popc(u & S.xmask_dn[b]) & 1u) != 0;
         s_cr[b] = static_cast<T>(neg ? -c.x : c.x);
         s_ci[b] = static_cast<T>(neg ? -c.y : c.y);
       }
-      __syncthreads();
       cur_row = iu;
     }
-    const int id = chunk * kHvBlock + static_cast<int>(threadIdx.x);
+    __syncthreads();
+    const int id = chunk * EPB + el;
     if (id >= Ddn) continue;
     const uint32_t d = S.dn_mask[id];
     const double dg = diag_value(S, u, d, iu, id);
-    const int64_t e = static_cast<int64_t>(iu) * Ddn + id;
-
-    T ar[RB], ai[RB];
-    V2 xo[RB];
-    const V2* xe = x + e * RT + r0;
+    const V2* xrow = x + iu * row_stride + rr;
+    T ar[VPL], ai[VPL];
+    V2 xo[VPL];
+    const V2* xe = xrow + id * RT;
 #pragma unroll
-    for (int r = 0; r < RB; ++r) {
-      xo[r] = xe[r];
-      ar[r] = static_cast<T>(dg) * xo[r].x;
-      ai[r] = static_cast<T>(dg) * xo[r].y;
+    for (int v = 0; v < VPL; ++v) {
+      xo[v] = xe[v];
+      ar[v] = static_cast<T>(dg) * xo[v].x;
+      ai[v] = static_cast<T>(dg) * xo[v].y;
     }
-
-    // up-spin bonds: the entry list, bond data and source row are wavefront-uniform
-    const int ku0 = S.off_up[iu], ku1 = S.off_up[iu + 1];
-    for (int k = ku0; k < ku1; ++k) {
-      const uint32_t ent = S.ent_up[k];
-      const uint32_t t = ent & kTargetMask;
-      const uint32_t b = (ent >> kTargetBits) & kBondMask;
-      const uint32_t par = (ent >> 31) ^ (__popc(d & S.xmask_up[b]) & 1u);
-      const double2 c = S.coef_up[b];
-      const T cr = static_cast<T>(par ? -c.x : c.x);
-      const T ci = static_cast<T>(par ? -c.y : c.y);
-      axpy_block<T, RB, REAL>(ar, ai, x + (static_cast<int64_t>(t) * Ddn + id) * RT + r0, cr, ci);
-    }
-    // down-spin bonds: per-lane entries, slot-major table, row-signed coefficients in LDS
     const int kd = S.cnt_dn[id];
-    const int64_t rowbase = static_cast<int64_t>(iu) * Ddn;
-    for (int k = 0; k < kd; ++k) {
-      const uint32_t ent = S.entT_dn[static_cast<int64_t>(k) * Ddn + id];
-      const uint32_t t = ent & kTargetMask;
-      const uint32_t b = (ent >> kTargetBits) & kBondMask;
-      const bool neg = (ent >> 31) != 0;
-      const T cr = neg ? -s_cr[b] : s_cr[b];
-      const T ci = neg ? -s_ci[b] : s_ci[b];
-      axpy_block<T, RB, REAL>(ar, ai, x + (rowbase + t) * RT + r0, cr, ci);
+    // staged: slot k of this element at s_ent[k * EPB + el]; otherwise straight from the table
+    const uint32_t* ent = kStage ? s_ent + el : S.entT_dn + id;
+    const int eld = kStage ? EPB : estride;
+    // kUnroll sources are loaded before any is consumed (independent gathers in flight);
+    // accumulation stays in entry order.
+    for (int k0 = 0; k0 < kd; k0 += kUnroll) {
+      uint32_t en[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) en[j] = (k0 + j < kd) ? ent[(k0 + j) * eld] : 0u;
+      V2 src[kUnroll][VPL];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) {
+        const V2* sp = xrow + (en[j] & kTargetMask) * RT;
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          if (k0 + j < kd) {
+            src[j][v] = sp[v];
+          } else {
+            src[j][v].x = static_cast<T>(0);
+            src[j][v].y = static_cast<T>(0);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) {
+        if (k0 + j >= kd) break;
+        const uint32_t b = (en[j] >> kTargetBits) & kBondMask;
+        const bool neg = (en[j] >> 31) != 0;
+        axpy_regs<T, VPL, REAL>(ar, ai, src[j], neg ? -s_cr[b] : s_cr[b], neg ? -s_ci[b] : s_ci[b]);
+      }
     }
-    finish_element<T, RB>(ar, ai, xo, y + e * RT + r0, xs, r0, dotp != nullptr, dre, dimg);
+    V2* ye = y + iu * row_stride + rr + id * RT;
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+      V2 o;
+      o.x = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
+      o.y = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
+      ye[v] = o;
+      if (dotp) {
+        const double qr = sxv[v] * static_cast<double>(xo[v].x), qi = sxv[v] * static_cast<double>(xo[v].y);
+        dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+        dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+      }
+    }
   }
-  if (dotp) block_dot_store<RB>(dre, dimg, dotp, RT, r0);
+  queue_done(ctr);
+  if (dotp) store_dot<L, VPL, RB>(dre, dimg, dotp, blockIdx.x, RT, r0);
+}
+
+// ------------------------------------------------------------------ column pass: up bonds, y +=
+// Tiles: (stripe of W ids, chunk of EPB/W rows), stripe-major; element e of a tile ->
+// (row chunk*EPB/W + e/W, id stripe*W + e%W).  While an XCD works through one stripe (all rows x
+// W ids <= ~3 MB) every up bond of the stripe reads from that XCD's L2.
+template <typename T, int RB, bool REAL>
+__global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const typename Cx<T>::v2* __restrict__ x,
+                                                    typename Cx<T>::v2* __restrict__ y, const int RT, const int r0,
+                                                    const double* __restrict__ xs, double2* __restrict__ dotp,
+                                                    const int64_t slot0, int* __restrict__ ctr, const int wlog,
+                                                    const int tps, const int ntiles) {
+  using V2 = typename Cx<T>::v2;
+  using SP = Split<T, RB>;
+  constexpr int L = SP::L, VPL = SP::VPL, EPB = SP::EPB;
+  constexpr bool kStage = StageEntries<EPB>::value;
+  __shared__ T s_c0[kMaxBondsLds], s_c1[kMaxBondsLds];
+  __shared__ uint32_t s_xm[kMaxBondsLds];
+  __shared__ int s_tile;
+  for (int b = threadIdx.x; b < S.nbond_up; b += kHvBlock) {
+    const double2 c = S.coef_up[b];
+    s_c0[b] = static_cast<T>(c.x);
+    s_c1[b] = static_cast<T>(c.y);
+    s_xm[b] = S.xmask_up[b];
+  }
+  const int xcd = blockIdx.x % kXcd;
+  const int sub = static_cast<int>(threadIdx.x) % L, el = static_cast<int>(threadIdx.x) / L;
+  const int rr = r0 + sub * VPL;
+  const int W = 1 << wlog;
+  const int rows_per_tile = EPB >> wlog;
+  const int per = (ntiles + kXcd - 1) / kXcd;
+  const int t0 = min(ntiles, xcd * per), nmine = min(ntiles, t0 + per) - t0;
+  const int Ddn = S.Ddn, Dup = S.Dup;
+  const int64_t row_stride = static_cast<int64_t>(Ddn) * RT;
+  double sxv[VPL], dre[VPL], dimg[VPL];
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) {
+    sxv[v] = xs ? xs[rr + v] : 1.0;
+    dre[v] = dimg[v] = 0.0;
+  }
+  extern __shared__ uint4 s_ent4[];  // this tile's entries, [maxcnt][rows_per_tile] uint32
+  uint32_t* s_ent = reinterpret_cast<uint32_t*>(s_ent4);
+  const int mc = S.maxcnt_up, estride = S.entT_stride_up;
+  int cur_rc = -1;
+  for (int local = next_tile(ctr, xcd, &s_tile); local < nmine; local = next_tile(ctr, xcd, &s_tile)) {
+    const int tile = t0 + local;
+    const int stripe = tile / tps;
+    const int rc = tile - stripe * tps;
+    if (kStage && rc != cur_rc) {  // uniform: stage the row chunk's entries
+      const int rbase = rc * rows_per_tile;
+      if ((rows_per_tile & 3) == 0) {
+        const int q4 = rows_per_tile / 4;
+        const uint4* src4 = reinterpret_cast<const uint4*>(S.entT_up + rbase);
+        for (int i = threadIdx.x; i < mc * q4; i += kHvBlock) {
+          const int k = i / q4, q = i - k * q4;
+          s_ent4[i] = src4[(static_cast<int64_t>(k) * estride) / 4 + q];
+        }
+      } else {
+        for (int i = threadIdx.x; i < mc * rows_per_tile; i += kHvBlock) {
+          const int k = i / rows_per_tile, q = i - k * rows_per_tile;
+          s_ent[i] = S.entT_up[static_cast<int64_t>(k) * estride + rbase + q];
+        }
+      }
+      cur_rc = rc;
+      __syncthreads();
+    }
+    const int id = (stripe << wlog) + (el & (W - 1));
+    const int rl = el >> wlog;
+    const int iu = rc * rows_per_tile + rl;
+    if (id >= Ddn || iu >= Dup) continue;
+    const uint32_t d = S.dn_mask[id];
+    const V2* xcol = x + static_cast<int64_t>(id) * RT + rr;
+    T ar[VPL], ai[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) ar[v] = ai[v] = static_cast<T>(0);
+    const int ku = S.cnt_up[iu];
+    const uint32_t* ent = kStage ? s_ent + rl : S.entT_up + iu;
+    const int eld = kStage ? rows_per_tile : estride;
+    for (int k0 = 0; k0 < ku; k0 += kUnroll) {
+      uint32_t en[kUnroll];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) en[j] = (k0 + j < ku) ? ent[(k0 + j) * eld] : 0u;
+      V2 src[kUnroll][VPL];
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) {
+        const V2* sp = xcol + (en[j] & kTargetMask) * row_stride;
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          if (k0 + j < ku) {
+            src[j][v] = sp[v];
+          } else {
+            src[j][v].x = static_cast<T>(0);
+            src[j][v].y = static_cast<T>(0);
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < kUnroll; ++j) {
+        if (k0 + j >= ku) break;
+        const uint32_t b = (en[j] >> kTargetBits) & kBondMask;
+        const bool neg = ((en[j] >> 31) ^ (__popc(d & s_xm[b]) & 1u)) != 0;
+        axpy_regs<T, VPL, REAL>(ar, ai, src[j], neg ? -s_c0[b] : s_c0[b], neg ? -s_c1[b] : s_c1[b]);
+      }
+    }
+    const int64_t eo = iu * row_stride + static_cast<int64_t>(id) * RT + rr;
+    V2* ye = y + eo;
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+      const V2 prev = ye[v];
+      const T ur = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
+      const T ui = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
+      V2 o;
+      o.x = prev.x + ur;
+      o.y = prev.y + ui;
+      ye[v] = o;
+      if (dotp) {
+        const V2 q = x[eo + v];
+        const double qr = sxv[v] * static_cast<double>(q.x), qi = sxv[v] * static_cast<double>(q.y);
+        dre[v] += qr * static_cast<double>(ur) + qi * static_cast<double>(ui);
+        dimg[v] += qr * static_cast<double>(ui) - qi * static_cast<double>(ur);
+      }
+    }
+  }
+  queue_done(ctr);
+  if (dotp) store_dot<L, VPL, RB>(dre, dimg, dotp, slot0 + blockIdx.x, RT, r0);
 }
 
 // ------------------------------------------------------------------ flat kernel (small sectors)
@@ -232,15 +439,6 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_flat(const SectorView S, const 
       ar[r] = static_cast<T>(dg) * xo[r].x;
       ai[r] = static_cast<T>(dg) * xo[r].y;
     }
-    for (int k = S.off_up[iu]; k < S.off_up[iu + 1]; ++k) {
-      const uint32_t ent = S.ent_up[k];
-      const uint32_t t = ent & kTargetMask;
-      const uint32_t b = (ent >> kTargetBits) & kBondMask;
-      const uint32_t par = (ent >> 31) ^ (__popc(d & S.xmask_up[b]) & 1u);
-      const double2 c = S.coef_up[b];
-      axpy_block<T, RB, REAL>(ar, ai, x + (static_cast<int64_t>(t) * S.Ddn + id) * RT + r0,
-                              static_cast<T>(par ? -c.x : c.x), static_cast<T>(par ? -c.y : c.y));
-    }
     const int64_t rowbase = static_cast<int64_t>(iu) * S.Ddn;
     for (int k = S.off_dn[id]; k < S.off_dn[id + 1]; ++k) {
       const uint32_t ent = S.ent_dn[k];
@@ -251,63 +449,147 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_flat(const SectorView S, const 
       axpy_block<T, RB, REAL>(ar, ai, x + (rowbase + t) * RT + r0, static_cast<T>(par ? -c.x : c.x),
                               static_cast<T>(par ? -c.y : c.y));
     }
-    finish_element<T, RB>(ar, ai, xo, y + e * RT + r0, xs, r0, dotp != nullptr, dre, dimg);
+    V2* ye = y + e * RT + r0;
+    T ur[RB], ui[RB];
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const double sx = xs ? xs[r0 + r] : 1.0;
+      ur[r] = static_cast<T>(static_cast<double>(ar[r]) * sx);
+      ui[r] = static_cast<T>(static_cast<double>(ai[r]) * sx);
+      ar[r] = ai[r] = static_cast<T>(0);
+    }
+    // up bonds accumulate separately and are added last, as the column sweep does
+    for (int k = S.off_up[iu]; k < S.off_up[iu + 1]; ++k) {
+      const uint32_t ent = S.ent_up[k];
+      const uint32_t t = ent & kTargetMask;
+      const uint32_t b = (ent >> kTargetBits) & kBondMask;
+      const uint32_t par = (ent >> 31) ^ (__popc(d & S.xmask_up[b]) & 1u);
+      const double2 c = S.coef_up[b];
+      axpy_block<T, RB, REAL>(ar, ai, x + (static_cast<int64_t>(t) * S.Ddn + id) * RT + r0,
+                              static_cast<T>(par ? -c.x : c.x), static_cast<T>(par ? -c.y : c.y));
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const double sx = xs ? xs[r0 + r] : 1.0;
+      V2 o;
+      o.x = ur[r] + static_cast<T>(static_cast<double>(ar[r]) * sx);
+      o.y = ui[r] + static_cast<T>(static_cast<double>(ai[r]) * sx);
+      ye[r] = o;
+      if (dotp) {
+        const double qr = sx * static_cast<double>(xo[r].x), qi = sx * static_cast<double>(xo[r].y);
+        dre[r] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+        dimg[r] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+      }
+    }
   }
-  if (dotp) block_dot_store<RB>(dre, dimg, dotp, RT, r0);
+  if (dotp) {
+    __shared__ double red[kHvBlock / 64][RB][2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const double a = wave_sum(dre[r]), b = wave_sum(dimg[r]);
+      if (lane == 0) {
+        red[wv][r][0] = a;
+        red[wv][r][1] = b;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < RB) {
+      double a = 0.0, b = 0.0;
+      for (int w = 0; w < kHvBlock / 64; ++w) {
+        a += red[w][threadIdx.x][0];
+        b += red[w][threadIdx.x][1];
+      }
+      dotp[static_cast<int64_t>(blockIdx.x) * RT + r0 + threadIdx.x] = make_double2(a, b);
+    }
+  }
 }
 
+// ------------------------------------------------------------------ host dispatch
 namespace {
-
-bool use_rows(const SectorView& S) { return S.Ddn >= 128; }
 
 int env_int(const char* name, int dflt) {
   const char* v = std::getenv(name);
   return v ? std::atoi(v) : dflt;
 }
 
-// Rows kernel: blocks per XCD sized so that the rows being gathered concurrently on one XCD
-// (about bpx tiles of 256 elements x RT vectors) stay near the 4 MB L2 of that XCD.
-int grid_size(const SectorView& S, int RT, int elem_bytes) {
-  if (use_rows(S)) {
-    const int nchunk = (S.Ddn + kHvBlock - 1) / kHvBlock;
-    const int64_t ntiles = static_cast<int64_t>(S.Dup) * nchunk;
-    const int64_t per = (ntiles + kXcd - 1) / kXcd;
-    const int64_t window = static_cast<int64_t>(env_int("FTLM_HV_WINDOW_KB", 4096)) * 1024;
-    int64_t bpx = window / (static_cast<int64_t>(kHvBlock) * RT * elem_bytes);
-    bpx = std::max<int64_t>(64, std::min<int64_t>(bpx, kHvMaxGrid / kXcd));
-    bpx = env_int("FTLM_HV_BPX", static_cast<int>(bpx));
-    bpx = std::max<int64_t>(1, std::min<int64_t>(bpx, kHvMaxGrid / kXcd));
-    return static_cast<int>(kXcd * std::min<int64_t>(per, bpx));
+struct Plan {
+  bool two = false;
+  int grid_flat = 0;
+  int grid_dn = 0, nchunk = 1, ntiles_dn = 0;
+  int grid_up = 0, wlog = 0, tps = 1, ntiles_up = 0;
+};
+
+Plan make_plan(const Sector* Sx, int RT, int esize, int L) {
+  const SectorView S = Sx->view();
+  Plan p;
+  p.two = S.Ddn >= 128;
+  if (!p.two) {
+    const int64_t tiles = (S.dim + kHvBlock - 1) / kHvBlock;
+    p.grid_flat = static_cast<int>(std::min<int64_t>(tiles, kHvMaxGrid));
+    return p;
   }
-  const int64_t tiles = (S.dim + kHvBlock - 1) / kHvBlock;
-  return static_cast<int>(std::min<int64_t>(tiles, kHvMaxGrid));
+  const int EPB = kHvBlock / L;
+  const int64_t budget = static_cast<int64_t>(env_int("FTLM_HV_L2_KB", 3072)) * 1024;
+  // workgroups per XCD: enough to fill its 32 CUs several times over
+  const int bpx = std::max(1, std::min(kPassMaxGrid / kXcd, env_int("FTLM_HV_BPX", 256)));
+  p.nchunk = (S.Ddn + EPB - 1) / EPB;
+  const int64_t ntd = static_cast<int64_t>(S.Dup) * p.nchunk;
+  FTLM_REQUIRE(ntd < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many row tiles");
+  p.ntiles_dn = static_cast<int>(ntd);
+  p.grid_dn = kXcd * static_cast<int>(std::min<int64_t>(bpx, (ntd + kXcd - 1) / kXcd));
+  // column pass: widest power-of-two stripe W <= EPB with D_up * W * RT * esize <= budget
+  const int64_t col_bytes = static_cast<int64_t>(S.Dup) * RT * esize;
+  int wl = 0;
+  while ((1 << (wl + 1)) <= EPB && col_bytes * (int64_t{1} << (wl + 1)) <= budget) ++wl;
+  wl = std::max(0, std::min(wl, env_int("FTLM_HV_WLOG", wl)));
+  while ((1 << wl) > EPB) --wl;
+  p.wlog = wl;
+  const int rows_per_tile = EPB >> wl;
+  p.tps = (S.Dup + rows_per_tile - 1) / rows_per_tile;
+  const int64_t ntu = static_cast<int64_t>((S.Ddn + (1 << wl) - 1) >> wl) * p.tps;
+  FTLM_REQUIRE(ntu < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many column tiles");
+  p.ntiles_up = static_cast<int>(ntu);
+  p.grid_up = kXcd * static_cast<int>(std::min<int64_t>(bpx, (ntu + kXcd - 1) / kXcd));
+  return p;
 }
 
 template <typename T, int RB, bool REAL>
-void launch(const SectorView& S, const void* x, void* y, int RT, int r0, const double* xs, double2* dotp,
-            hipStream_t st) {
+int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const double* xs, double2* dotp,
+           hipStream_t st) {
   using V2 = typename Cx<T>::v2;
-  const dim3 g(grid_size(S, RT, sizeof(V2))), b(kHvBlock);
+  using SP = Split<T, RB>;
+  const SectorView S = Sx->view();
+  const Plan p = make_plan(Sx, RT, sizeof(V2), SP::L);
   const V2* xp = static_cast<const V2*>(x);
   V2* yp = static_cast<V2*>(y);
-  if (use_rows(S)) {
-    const int nchunk = (S.Ddn + kHvBlock - 1) / kHvBlock;
-    const int64_t ntiles = static_cast<int64_t>(S.Dup) * nchunk;
-    hipLaunchKernelGGL((k_hv_rows<T, RB, REAL>), g, b, 0, st, S, xp, yp, RT, r0, xs, dotp, nchunk, ntiles);
-  } else {
-    hipLaunchKernelGGL((k_hv_flat<T, RB, REAL>), g, b, 0, st, S, xp, yp, RT, r0, xs, dotp);
+  if (!p.two) {
+    hipLaunchKernelGGL((k_hv_flat<T, RB, REAL>), dim3(p.grid_flat), dim3(kHvBlock), 0, st, S, xp, yp, RT, r0, xs,
+                       dotp);
+    return p.grid_flat;
   }
+  const bool stage = StageEntries<SP::EPB>::value;
+  const size_t lds_dn = stage ? static_cast<size_t>(std::max(S.maxcnt_dn, 1)) * SP::EPB * sizeof(uint32_t) : 16;
+  const size_t lds_up =
+      stage ? static_cast<size_t>(std::max(S.maxcnt_up, 1)) * (SP::EPB >> p.wlog) * sizeof(uint32_t) : 16;
+  FTLM_REQUIRE(lds_dn <= 48 * 1024 && lds_up <= 48 * 1024, FTLM_EUNSUPPORTED,
+               "hop table of one H*V tile exceeds 48 KB of LDS");
+  hipLaunchKernelGGL((k_hv_dn<T, RB, REAL>), dim3(p.grid_dn), dim3(kHvBlock), lds_dn, st, S, xp, yp, RT, r0, xs,
+                     dotp, Sx->tile_ctr.p, p.nchunk, p.ntiles_dn);
+  hipLaunchKernelGGL((k_hv_up<T, RB, REAL>), dim3(p.grid_up), dim3(kHvBlock), lds_up, st, S, xp, yp, RT, r0, xs,
+                     dotp, static_cast<int64_t>(p.grid_dn), Sx->tile_ctr.p + 16, p.wlog, p.tps, p.ntiles_up);
+  return p.grid_dn + p.grid_up;
 }
 
 template <typename T, bool REAL>
-void launch_chunk(const SectorView& S, const void* x, void* y, int RT, int r0, int rb, const double* xs,
-                  double2* dotp, hipStream_t st) {
+int launch_chunk(const Sector* S, const void* x, void* y, int RT, int r0, int rb, const double* xs, double2* dotp,
+                 hipStream_t st) {
   switch (rb) {
-    case 16: launch<T, 16, REAL>(S, x, y, RT, r0, xs, dotp, st); break;
-    case 8: launch<T, 8, REAL>(S, x, y, RT, r0, xs, dotp, st); break;
-    case 4: launch<T, 4, REAL>(S, x, y, RT, r0, xs, dotp, st); break;
-    case 2: launch<T, 2, REAL>(S, x, y, RT, r0, xs, dotp, st); break;
-    default: launch<T, 1, REAL>(S, x, y, RT, r0, xs, dotp, st); break;
+    case 16: return launch<T, 16, REAL>(S, x, y, RT, r0, xs, dotp, st);
+    case 8: return launch<T, 8, REAL>(S, x, y, RT, r0, xs, dotp, st);
+    case 4: return launch<T, 4, REAL>(S, x, y, RT, r0, xs, dotp, st);
+    case 2: return launch<T, 2, REAL>(S, x, y, RT, r0, xs, dotp, st);
+    default: return launch<T, 1, REAL>(S, x, y, RT, r0, xs, dotp, st);
   }
 }
 
@@ -320,25 +602,35 @@ int hv_partial_blocks(const ftlm_op* op) {
 }
 
 // Block apply over all RT vectors in chunks of RB (<= 8 for complex128, <= 16 for complex64).
+// Chunks are either all two-pass or all flat; the number of dot-partial blocks written is the
+// maximum over chunks, and blocks a chunk does not launch are zero-filled first.
 int hv_apply_sector(const Sector* S, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
                     hipStream_t st, bool force_real) {
   const bool real = S->real_h || force_real;
-  const SectorView v = S->view();
   const int maxrb = dtype == FTLM_C64 ? 16 : 8;
+  int nblk = 0;
+  std::vector<std::pair<int, int>> chunks;
   for (int r0 = 0; r0 < R;) {
     int rb = 1;
     while (rb * 2 <= maxrb && r0 + rb * 2 <= R) rb *= 2;
-    if (dtype == FTLM_C64) {
-      if (real) launch_chunk<float, true>(v, x, y, R, r0, rb, xs, dotp, st);
-      else launch_chunk<float, false>(v, x, y, R, r0, rb, xs, dotp, st);
-    } else {
-      if (real) launch_chunk<double, true>(v, x, y, R, r0, rb, xs, dotp, st);
-      else launch_chunk<double, false>(v, x, y, R, r0, rb, xs, dotp, st);
-    }
+    chunks.emplace_back(r0, rb);
     r0 += rb;
   }
+  if (dotp && chunks.size() > 1)
+    FTLM_HIP_CHECK(hipMemsetAsync(dotp, 0, sizeof(double2) * static_cast<size_t>(kHvMaxGrid) * R, st));
+  for (const auto& c : chunks) {
+    int nb;
+    if (dtype == FTLM_C64) {
+      nb = real ? launch_chunk<float, true>(S, x, y, R, c.first, c.second, xs, dotp, st)
+                : launch_chunk<float, false>(S, x, y, R, c.first, c.second, xs, dotp, st);
+    } else {
+      nb = real ? launch_chunk<double, true>(S, x, y, R, c.first, c.second, xs, dotp, st)
+                : launch_chunk<double, false>(S, x, y, R, c.first, c.second, xs, dotp, st);
+    }
+    nblk = std::max(nblk, nb);
+  }
   check_launch("k_hv_sector");
-  return grid_size(v, R, dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2));
+  return nblk;
 }
 
 int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,

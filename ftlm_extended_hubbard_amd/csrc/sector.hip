@@ -132,12 +132,15 @@ __global__ void k_hop_fill(const uint32_t* masks, int D, const BondDev* bonds, i
 }
 
 // CSR -> slot-major copy of a hop table (entT[k * D + i] = entry k of string i).
-__global__ void k_hop_transpose(const int32_t* off, const uint32_t* ent, int D, uint32_t* entT, int32_t* cnt) {
+// Slot rows are padded to a multiple of 256 strings (zero entries) so that any aligned chunk of up
+// to 256 strings of one slot lies inside the table and loads in 16-byte pieces.
+__global__ void k_hop_transpose(const int32_t* off, const uint32_t* ent, int D, int Dpad, uint32_t* entT,
+                                int32_t* cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= D) return;
   const int o = off[i], c = off[i + 1] - o;
   cnt[i] = c;
-  for (int k = 0; k < c; ++k) entT[static_cast<int64_t>(k) * D + i] = ent[o + k];
+  for (int k = 0; k < c; ++k) entT[static_cast<int64_t>(k) * Dpad + i] = ent[o + k];
 }
 
 // Per-string pieces of the diagonal  V * sum_pairs (u_i + d_i)(u_j + d_j):
@@ -253,6 +256,10 @@ SectorView Sector::view() const {
   v.cnt_dn = cnt[1].p;
   v.nbond_up = nbond[0];
   v.nbond_dn = nbond[1];
+  v.entT_stride_up = entT_stride[0];
+  v.entT_stride_dn = entT_stride[1];
+  v.maxcnt_up = maxcnt[0];
+  v.maxcnt_dn = maxcnt[1];
   return v;
 }
 
@@ -327,9 +334,12 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
     int mc = 0;
     for (int i = 0; i < D; ++i) mc = std::max(mc, hc[i]);
     S->maxcnt[sp] = mc;
-    S->entT[sp].alloc(static_cast<size_t>(std::max(mc, 1)) * D);
+    const int Dpad = (D + 255) / 256 * 256;
+    S->entT_stride[sp] = Dpad;
+    S->entT[sp].alloc(static_cast<size_t>(std::max(mc, 1)) * Dpad);
+    FTLM_HIP_CHECK(hipMemset(S->entT[sp].p, 0, S->entT[sp].n * sizeof(uint32_t)));
     S->cnt[sp].alloc(static_cast<size_t>(D));
-    hipLaunchKernelGGL(k_hop_transpose, dim3(grid_for(D, 256)), dim3(256), 0, 0, S->off[sp].p, S->ent[sp].p, D,
+    hipLaunchKernelGGL(k_hop_transpose, dim3(grid_for(D, 256)), dim3(256), 0, 0, S->off[sp].p, S->ent[sp].p, D, Dpad,
                        S->entT[sp].p, S->cnt[sp].p);
     check_launch("k_hop_transpose");
     FTLM_HIP_CHECK(hipDeviceSynchronize());
@@ -351,6 +361,8 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
     for (int s = 0; s < n; ++s)
       FTLM_REQUIRE(deg[s] <= 15, FTLM_EUNSUPPORTED, "a site appears in more than 15 NN pairs");
   }
+  S->tile_ctr.alloc(32);
+  FTLM_HIP_CHECK(hipMemset(S->tile_ctr.p, 0, 32 * sizeof(int32_t)));
   DevBuf<int2> dpairs(pr.size());
   dpairs.upload(pr);
   S->nuu.alloc(static_cast<size_t>(S->Dup));
