@@ -44,29 +44,24 @@ def time_steps(sess, K, torch):
 
 
 def kernel_roofline(F, L, op, R, dtype, torch, reps=5):
-    """Average duration of the H*V kernel (one launch for R <= 16 at complex64) via HIP events on
-    the launch stream; achieved = algorithmic bytes (read x, write y once) / duration."""
+    """Average duration of the two H*V kernels (row pass k_hv_dn, column pass k_hv_up), each
+    bracketed by HIP events recorded by the library on the launch stream.  Algorithmic bytes of
+    one H*V: read x once and write y once, B = R * dim * 2 * s."""
     dim = op.dim()
     ctype = torch.complex64 if dtype == L.FTLM_C64 else torch.complex128
     s = 8 if dtype == L.FTLM_C64 else 16
     X = torch.randn(dim * R, dtype=ctype, device="cuda")
     Y = torch.empty_like(X)
     st = torch.cuda.current_stream()
-    op.apply_device(X, Y, R=R, dtype=dtype, stream=st.cuda_stream)  # warm
-    torch.cuda.synchronize()
-    ts = []
+    op.apply_pass_times(X, Y, R=R, dtype=dtype, stream=st.cuda_stream)  # warm
+    rows, cols = [], []
     for _ in range(reps):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        op.apply_device(X, Y, R=R, dtype=dtype, stream=st.cuda_stream)
-        e1.record(st)
-        torch.cuda.synchronize()
-        ts.append(e0.elapsed_time(e1) / 1e3)
+        a, b = op.apply_pass_times(X, Y, R=R, dtype=dtype, stream=st.cuda_stream)
+        rows.append(a / 1e3)
+        cols.append(b / 1e3)
     del X, Y
     torch.cuda.empty_cache()
-    t = sum(ts) / len(ts)
-    B = R * dim * 2 * s
-    return t, B
+    return sum(rows) / reps, sum(cols) / reps, R * dim * 2 * s
 
 
 def profile_hv(F, L, torch, R):
@@ -88,14 +83,16 @@ def profile_hv(F, L, torch, R):
 
 
 def traffic_from_profile():
-    path = os.path.join(ROOT, "profiles", "hv_traffic_4x4_c64_r16.json")
+    """HBM bytes per H*V (R=16 complex64) from the committed counter summary: the line
+    'hbm_bytes_per_hv = <int>' (FETCH_SIZE + WRITE_SIZE of both passes, copy-calibrated)."""
+    path = os.path.join(ROOT, "profiles", "hv_traffic_4x4_c64_r16.md")
     if not os.path.exists(path):
         return None
-    try:
-        with open(path) as f:
-            return float(json.load(f)["hbm_bytes_per_launch"])
-    except Exception:  # noqa: BLE001
-        return None
+    with open(path) as f:
+        for line in f:
+            if line.strip().startswith("hbm_bytes_per_hv ="):
+                return int(line.split("=")[1].strip().strip("`"))
+    return None
 
 
 def cpu_baseline(budget_s=12.0):
@@ -225,16 +222,23 @@ def main():
         log(f"plain c128 R=1: {plain['ms_per_step']:.2f} ms/step, {plain['value']:.1f} sample-steps/s")
 
     # ---- roofline of the dominant kernel (block H*V)
-    t_k, B = kernel_roofline(F, L, op, R, L.FTLM_C64, torch)
+    tr, tc, B = kernel_roofline(F, L, op, R, L.FTLM_C64, torch)
+    t_k = tr + tc
     achieved = B / t_k / 1e9
-    log(f"H*V c64 R={R}: {t_k * 1e3:.3f} ms, {achieved:.0f} GB/s algorithmic")
-    t_k1, B1 = kernel_roofline(F, L, op, 1, L.FTLM_C128, torch)
-    log(f"H*V c128 R=1: {t_k1 * 1e3:.3f} ms, {B1 / t_k1 / 1e9:.0f} GB/s algorithmic")
+    log(f"H*V c64 R={R}: rows {tr * 1e3:.3f} ms + columns {tc * 1e3:.3f} ms, {achieved:.0f} GB/s algorithmic")
+    tr1, tc1, B1 = kernel_roofline(F, L, op, 1, L.FTLM_C128, torch)
+    log(f"H*V c128 R=1: rows {tr1 * 1e3:.3f} ms + columns {tc1 * 1e3:.3f} ms, "
+        f"{B1 / (tr1 + tc1) / 1e9:.0f} GB/s algorithmic")
+    rb = min(R, 16)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
-                "kernel": f"H*V = k_hv_dn<float,{min(R, 16)},false> + k_hv_up<float,{min(R, 16)},false>",
-                "launch_ms": round(t_k * 1e3, 4), "algorithmic_bytes_per_launch": B,
-                "plain_c128_r1": {"launch_ms": round(t_k1 * 1e3, 4), "achieved": round(B1 / t_k1 / 1e9, 1)}}
+                "kernel": f"H*V = k_hv_dn<float,{rb},false> then k_hv_up<float,{rb},false>",
+                "launch_ms": round(t_k * 1e3, 4),
+                "pass_ms": {"k_hv_dn": round(tr * 1e3, 4), "k_hv_up": round(tc * 1e3, 4)},
+                "algorithmic_bytes_per_launch": B,
+                "plain_c128_r1": {"launch_ms": round((tr1 + tc1) * 1e3, 4),
+                                  "pass_ms": {"k_hv_dn": round(tr1 * 1e3, 4), "k_hv_up": round(tc1 * 1e3, 4)},
+                                  "achieved": round(B1 / (tr1 + tc1) / 1e9, 1)}}
 
     # ---- |dlnZ| of the batched complex64 FTLM against the FP64 path, same start vectors
     dlnz = None
@@ -260,9 +264,10 @@ def main():
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "sample-steps/s", "n_gpus": 1, "steps": K,
         "warmup": W, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64 (fp32 H*v, fp64 dots/alpha/beta)", "data": "synthetic",
-        "config": {"workload": "4x4 (8,8) FTLM, batched block Lanczos, random-phase starts", "dim": dim,
-                   "model": MODEL, "R": R, "setup_s": round(t_setup, 2)},
+        "vs_baseline": None, "dtype": "c64", "data": "synthetic",
+        "config": {"workload": "4x4 (8,8) FTLM Lanczos steps, R random-phase vectors per block, complex64 "
+                               "vectors with fp64 dots/alpha/beta", "lattice": "4x4", "sector": [NU, ND],
+                   "dim": dim, "hamiltonian": MODEL, "R": R, "setup_s": round(t_setup, 2)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "plain_fp64": plain,

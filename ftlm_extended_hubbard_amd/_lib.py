@@ -80,6 +80,7 @@ def lib() -> C.CDLL:
     L.ftlm_hip_sector_masks.argtypes = [vp, C.c_int64, C.c_int64, vp, vp]
     L.ftlm_hip_index_of.argtypes = [vp, vp, vp, C.c_int64, vp]
     L.ftlm_hip_apply.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
+    L.ftlm_hip_apply_pass_times.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp]
     L.ftlm_hip_apply_host.argtypes = [vp, vp, vp]
     L.ftlm_hip_lanczos.argtypes = [vp, C.c_int, C.c_uint, vp, vp, vp, ip, ip]
     L.ftlm_hip_lanczos_extrema.argtypes = [vp, C.c_int, C.c_uint, dp, dp, ip]

@@ -127,6 +127,20 @@ class _Op:
                 stream = 0
         L.check(L.lib().ftlm_hip_apply(self._h, C.c_void_p(xp), C.c_void_p(yp), R, dtype, C.c_void_p(stream)))
 
+    def apply_pass_times(self, x, y, R: int = 1, dtype: int = L.FTLM_C128, stream: int | None = None):
+        """apply_device with the row pass (diagonal + down bonds) and the column pass (up bonds)
+        timed by HIP events on the launch stream; synchronises and returns (ms_row, ms_col)."""
+        xp = x.data_ptr() if hasattr(x, "data_ptr") else int(x)
+        yp = y.data_ptr() if hasattr(y, "data_ptr") else int(y)
+        if stream is None:
+            import torch
+
+            stream = torch.cuda.current_stream().cuda_stream
+        a, b = C.c_double(0.0), C.c_double(0.0)
+        L.check(L.lib().ftlm_hip_apply_pass_times(self._h, C.c_void_p(xp), C.c_void_p(yp), R, dtype,
+                                                  C.c_void_p(stream), C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     # ---- Lanczos
     def lanczos(self, max_steps: int, seed: int = 1, q0: np.ndarray | None = None):
         """lanczos_tridiagonal (src/lanczos.cpp:125-209): (used, alpha, beta)."""

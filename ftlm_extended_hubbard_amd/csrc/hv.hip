@@ -35,6 +35,7 @@
 // pointer; complex updates are four explicit FMAs.  Small sectors (D_dn < 128) use one flat
 // single-pass kernel.
 
+#include <array>
 #include <cstdlib>
 
 #include "common.hpp"
@@ -513,6 +514,18 @@ int env_int(const char* name, int dflt) {
   return v ? std::atoi(v) : dflt;
 }
 
+// Per-pass timing (ftlm_hip_apply_pass_times): while set, launch() records an event before the
+// row pass, between the passes and after the column pass (a flat launch counts as a row pass).
+thread_local std::vector<std::array<hipEvent_t, 3>>* t_pass_events = nullptr;
+
+void record_pass_event(int slot, hipStream_t st) {
+  if (!t_pass_events) return;
+  if (slot == 0) t_pass_events->push_back({nullptr, nullptr, nullptr});
+  hipEvent_t& e = t_pass_events->back()[slot];
+  FTLM_HIP_CHECK(hipEventCreate(&e));
+  FTLM_HIP_CHECK(hipEventRecord(e, st));
+}
+
 struct Plan {
   bool two = false;
   int grid_flat = 0;
@@ -564,8 +577,11 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
   const V2* xp = static_cast<const V2*>(x);
   V2* yp = static_cast<V2*>(y);
   if (!p.two) {
+    record_pass_event(0, st);
     hipLaunchKernelGGL((k_hv_flat<T, RB, REAL>), dim3(p.grid_flat), dim3(kHvBlock), 0, st, S, xp, yp, RT, r0, xs,
                        dotp);
+    record_pass_event(1, st);
+    record_pass_event(2, st);
     return p.grid_flat;
   }
   const bool stage = StageEntries<SP::EPB>::value;
@@ -574,10 +590,13 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
       stage ? static_cast<size_t>(std::max(S.maxcnt_up, 1)) * (SP::EPB >> p.wlog) * sizeof(uint32_t) : 16;
   FTLM_REQUIRE(lds_dn <= 48 * 1024 && lds_up <= 48 * 1024, FTLM_EUNSUPPORTED,
                "hop table of one H*V tile exceeds 48 KB of LDS");
+  record_pass_event(0, st);
   hipLaunchKernelGGL((k_hv_dn<T, RB, REAL>), dim3(p.grid_dn), dim3(kHvBlock), lds_dn, st, S, xp, yp, RT, r0, xs,
                      dotp, Sx->tile_ctr.p, p.nchunk, p.ntiles_dn);
+  record_pass_event(1, st);
   hipLaunchKernelGGL((k_hv_up<T, RB, REAL>), dim3(p.grid_up), dim3(kHvBlock), lds_up, st, S, xp, yp, RT, r0, xs,
                      dotp, static_cast<int64_t>(p.grid_dn), Sx->tile_ctr.p + 16, p.wlog, p.tps, p.ntiles_up);
+  record_pass_event(2, st);
   return p.grid_dn + p.grid_up;
 }
 
@@ -657,6 +676,42 @@ int ftlm_hip_apply(const ftlm_op_t* op, const void* x, void* y, int R, int dtype
     FTLM_REQUIRE(x != y, FTLM_EINVAL, "x and y must not alias");
     hv_apply(op, x, y, R, dtype, nullptr, nullptr, static_cast<hipStream_t>(stream));
   });
+}
+
+int ftlm_hip_apply_pass_times(const ftlm_op_t* op, const void* x, void* y, int R, int dtype, void* stream,
+                              double* ms_row, double* ms_col) {
+  std::vector<std::array<hipEvent_t, 3>> ev;
+  const int rc = guarded([&] {
+    FTLM_REQUIRE(op != nullptr && x != nullptr && y != nullptr, FTLM_EINVAL, "null argument");
+    FTLM_REQUIRE(R >= 1 && R <= 32, FTLM_EINVAL, "R must be in [1, 32]");
+    FTLM_REQUIRE(dtype == FTLM_C128 || dtype == FTLM_C64, FTLM_EINVAL, "dtype must be FTLM_C128 or FTLM_C64");
+    FTLM_REQUIRE(x != y, FTLM_EINVAL, "x and y must not alias");
+    FTLM_REQUIRE(op->kind == kSector, FTLM_EUNSUPPORTED, "pass timing is defined for full-sector operators");
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    t_pass_events = &ev;
+    try {
+      hv_apply(op, x, y, R, dtype, nullptr, nullptr, st);
+    } catch (...) {
+      t_pass_events = nullptr;
+      throw;
+    }
+    t_pass_events = nullptr;
+    FTLM_HIP_CHECK(hipStreamSynchronize(st));
+    double a = 0.0, b = 0.0;
+    for (const auto& e : ev) {
+      float t0 = 0.f, t1 = 0.f;
+      FTLM_HIP_CHECK(hipEventElapsedTime(&t0, e[0], e[1]));
+      FTLM_HIP_CHECK(hipEventElapsedTime(&t1, e[1], e[2]));
+      a += t0;
+      b += t1;
+    }
+    if (ms_row) *ms_row = a;
+    if (ms_col) *ms_col = b;
+  });
+  for (auto& e : ev)
+    for (hipEvent_t h : e)
+      if (h) (void)hipEventDestroy(h);
+  return rc;
 }
 
 int ftlm_hip_apply_host(const ftlm_op_t* op, const double* x, double* y) {

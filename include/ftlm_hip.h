@@ -106,6 +106,13 @@ int ftlm_hip_index_of(const ftlm_op_t* op, const uint64_t* up, const uint64_t* d
  * (src/orbit_k_block_basis.cpp:151-220).  y is overwritten.  R in [1, 32]. */
 int ftlm_hip_apply(const ftlm_op_t* op, const void* x, void* y, int R, int dtype, void* stream);
 
+/* ftlm_hip_apply with the row pass (diagonal + down bonds) and the column pass (up bonds) timed
+ * separately by HIP events recorded on `stream`; synchronises the stream.  No counterpart in the
+ * reference: instrumentation for roofline reports.  Full-sector operators only; a single-pass
+ * (small) sector reports its whole time as *ms_row. */
+int ftlm_hip_apply_pass_times(const ftlm_op_t* op, const void* x, void* y, int R, int dtype, void* stream,
+                              double* ms_row, double* ms_col);
+
 /* Drop-in for apply_extended_hubbard(p, basis, hops, pairs, x, y) on host complex<double> arrays
  * of length dim: copies in, applies, copies out (synchronous). */
 int ftlm_hip_apply_host(const ftlm_op_t* op, const double* x, double* y);
