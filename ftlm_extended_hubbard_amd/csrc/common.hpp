@@ -91,10 +91,20 @@ struct DevBuf {
 
 // ------------------------------------------------------------------ combinatorics
 constexpr int kMaxSites = 32;
-constexpr int kTargetBits = 20;  // pull-hop entry: target index bits
+// Pull-hop entry (uint32): bits 0..19 source string rank, 20..28 bond id, 29 own-spin JW parity,
+// 30..31 zero.  entry >> kTargetBits is therefore a 10-bit index (parity * 512 + bond) into a
+// table of pre-signed coefficients.  Bond kZeroBond has coefficient 0: slot-major tables pad
+// every string's hop list with kPadEntry up to a multiple of kSlotPad slots, so an unrolled
+// group of hops never needs a bounds test (a padded hop reads string 0 and adds 0 * x).
+constexpr int kTargetBits = 20;
 constexpr uint32_t kTargetMask = (1u << kTargetBits) - 1u;
-constexpr int kBondBits = 11;    // bond index bits
+constexpr int kBondBits = 9;
 constexpr uint32_t kBondMask = (1u << kBondBits) - 1u;
+constexpr int kParityBit = kTargetBits + kBondBits;  // 29
+constexpr uint32_t kZeroBond = kBondMask;             // 511
+constexpr uint32_t kPadEntry = kZeroBond << kTargetBits;
+constexpr int kCoefSlots = 2 << kBondBits;            // 1024 pre-signed coefficients
+constexpr int kSlotPad = 8;
 
 // Binomial table C[a][b], a,b <= 32 (values < 2^63).
 struct Binom {
@@ -128,8 +138,8 @@ struct SectorView {
   const int32_t* ndd;        // [Ddn]  sum over NN pairs of d_i d_j
   const uint4* planes;       // [Dup]  bit planes of nb_u(s) = sum of u over pair partners of s
   // slot-major copies of the hop tables: entT[k * stride + i] is entry k of string i
-  // (k < cnt[i], zero beyond; stride = D rounded up to 64), so slot k of consecutive strings is
-  // one contiguous run
+  // (k < cnt[i], kPadEntry beyond; stride = D rounded up to 256), so slot k of consecutive
+  // strings is one contiguous run; maxcnt_* = slots per table (max count rounded up to kSlotPad)
   const uint32_t* entT_up;   // [maxcnt_up][stride_up]
   const int32_t* cnt_up;     // [Dup]
   const uint32_t* entT_dn;   // [maxcnt_dn][stride_dn]
@@ -139,7 +149,7 @@ struct SectorView {
   int maxcnt_up, maxcnt_dn;
 };
 
-constexpr int kMaxBondsLds = 512;  // per-spin directed bonds held in LDS by the H*V kernels
+constexpr int kMaxBondsLds = 512;  // LDS slots per spin for bond data (ids < kZeroBond are bonds)
 
 // Device view of an orbit-representative momentum block (filled by orbit.hip).
 struct KBlockView;

@@ -173,6 +173,10 @@ __device__ __forceinline__ void queue_done(int* ctr) {
 
 // ------------------------------------------------------------------ rows pass: diag + down bonds
 // Tiles: (row iu, chunk of EPB ids), row-major; XCD x owns tiles [x*per, (x+1)*per).
+// Per hop: one entry read, one 24-bit multiply-add for the source offset from the (uniform) row
+// base, one LDS read of the pre-signed coefficient (index = entry >> 20), and the complex FMAs.
+// Hop lists are padded to multiples of kUnroll with zero-coefficient entries, so a group of
+// kUnroll gathers is issued without bounds tests.
 template <typename T, int RB, bool REAL>
 __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const typename Cx<T>::v2* __restrict__ x,
                                                     typename Cx<T>::v2* __restrict__ y, const int RT, const int r0,
@@ -182,7 +186,8 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
   using SP = Split<T, RB>;
   constexpr int L = SP::L, VPL = SP::VPL, EPB = SP::EPB;
   constexpr bool kStage = StageEntries<EPB>::value;
-  __shared__ T s_cr[kMaxBondsLds], s_ci[kMaxBondsLds];
+  static_assert(kUnroll == kSlotPad, "hop groups must match the slot padding");
+  __shared__ V2 s_c[kCoefSlots];  // row-signed coefficients [parity * 512 + bond]
   __shared__ int s_tile;
   extern __shared__ uint4 s_ent4[];  // this tile's entries, [maxcnt][EPB] uint32
   const uint32_t* s_ent = reinterpret_cast<const uint32_t*>(s_ent4);
@@ -193,6 +198,8 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
   const int t0 = min(ntiles, xcd * per), nmine = min(ntiles, t0 + per) - t0;
   const int Ddn = S.Ddn;
   const int64_t row_stride = static_cast<int64_t>(Ddn) * RT;
+  const uint32_t es = static_cast<uint32_t>(RT * sizeof(V2));      // bytes per element
+  const uint32_t lane_b = static_cast<uint32_t>(rr * sizeof(V2));  // this lane's bytes in an element
   const int mc = S.maxcnt_dn, estride = S.entT_stride_dn;
   double sxv[VPL], dre[VPL], dimg[VPL];
 #pragma unroll
@@ -202,7 +209,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
   }
   int cur_row = -1, cur_chunk = -1;
   for (int local = next_tile(ctr, xcd, &s_tile); local < nmine; local = next_tile(ctr, xcd, &s_tile)) {
-    const int tile = t0 + local;
+    const int tile = __builtin_amdgcn_readfirstlane(t0 + local);
     const int iu = tile / nchunk;
     const int chunk = tile - iu * nchunk;
     const uint32_t u = S.up_mask[iu];
@@ -216,11 +223,19 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
       cur_chunk = chunk;
     }
     if (iu != cur_row) {
-      for (int b = threadIdx.x; b < S.nbond_dn; b += kHvBlock) {
-        const double2 c = S.coef_dn[b];
-        const bool neg = (__popc(u & S.xmask_dn[b]) & 1u) != 0;
-        s_cr[b] = static_cast<T>(neg ? -c.x : c.x);
-        s_ci[b] = static_cast<T>(neg ? -c.y : c.y);
+      for (int b = threadIdx.x; b < kMaxBondsLds; b += kHvBlock) {
+        V2 c;
+        c.x = c.y = static_cast<T>(0);
+        if (b < S.nbond_dn) {
+          const double2 cc = S.coef_dn[b];
+          const bool neg = (__popc(u & S.xmask_dn[b]) & 1u) != 0;
+          c.x = static_cast<T>(neg ? -cc.x : cc.x);
+          c.y = static_cast<T>(neg ? -cc.y : cc.y);
+        }
+        s_c[b] = c;
+        c.x = -c.x;
+        c.y = -c.y;
+        s_c[b + kMaxBondsLds] = c;
       }
       cur_row = iu;
     }
@@ -229,10 +244,10 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
     if (id >= Ddn) continue;
     const uint32_t d = S.dn_mask[id];
     const double dg = diag_value(S, u, d, iu, id);
-    const V2* xrow = x + iu * row_stride + rr;
+    const char* xb = reinterpret_cast<const char*>(x + static_cast<int64_t>(iu) * row_stride);  // uniform
     T ar[VPL], ai[VPL];
     V2 xo[VPL];
-    const V2* xe = xrow + id * RT;
+    const V2* xe = reinterpret_cast<const V2*>(xb + (__umul24(static_cast<uint32_t>(id), es) + lane_b));
 #pragma unroll
     for (int v = 0; v < VPL; ++v) {
       xo[v] = xe[v];
@@ -248,30 +263,22 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
     for (int k0 = 0; k0 < kd; k0 += kUnroll) {
       uint32_t en[kUnroll];
 #pragma unroll
-      for (int j = 0; j < kUnroll; ++j) en[j] = (k0 + j < kd) ? ent[(k0 + j) * eld] : 0u;
+      for (int j = 0; j < kUnroll; ++j) en[j] = ent[(k0 + j) * eld];
       V2 src[kUnroll][VPL];
 #pragma unroll
       for (int j = 0; j < kUnroll; ++j) {
-        const V2* sp = xrow + (en[j] & kTargetMask) * RT;
+        const V2* sp = reinterpret_cast<const V2*>(xb + (__umul24(en[j] & kTargetMask, es) + lane_b));
 #pragma unroll
-        for (int v = 0; v < VPL; ++v) {
-          if (k0 + j < kd) {
-            src[j][v] = sp[v];
-          } else {
-            src[j][v].x = static_cast<T>(0);
-            src[j][v].y = static_cast<T>(0);
-          }
-        }
+        for (int v = 0; v < VPL; ++v) src[j][v] = sp[v];
       }
 #pragma unroll
       for (int j = 0; j < kUnroll; ++j) {
-        if (k0 + j >= kd) break;
-        const uint32_t b = (en[j] >> kTargetBits) & kBondMask;
-        const bool neg = (en[j] >> 31) != 0;
-        axpy_regs<T, VPL, REAL>(ar, ai, src[j], neg ? -s_cr[b] : s_cr[b], neg ? -s_ci[b] : s_ci[b]);
+        const V2 c = s_c[en[j] >> kTargetBits];
+        axpy_regs<T, VPL, REAL>(ar, ai, src[j], c.x, c.y);
       }
     }
-    V2* ye = y + iu * row_stride + rr + id * RT;
+    V2* ye = reinterpret_cast<V2*>(reinterpret_cast<char*>(y + static_cast<int64_t>(iu) * row_stride) +
+                                   (__umul24(static_cast<uint32_t>(id), es) + lane_b));
 #pragma unroll
     for (int v = 0; v < VPL; ++v) {
       V2 o;
@@ -289,6 +296,40 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
   if (dotp) store_dot<L, VPL, RB>(dre, dimg, dotp, blockIdx.x, RT, r0);
 }
 
+// Up-bond accumulation of one element over its padded hop list.  ONE_ID: every lane of the
+// block works on the same id (stripes one id wide), so the cross-spin sign popcount(d & mask_b)
+// is already folded into s_c; otherwise each lane applies it from s_xm.
+template <typename T, int VPL, bool REAL, bool ONE_ID>
+__device__ __forceinline__ void up_hops(T (&ar)[VPL], T (&ai)[VPL], const typename Cx<T>::v2* __restrict__ xcol,
+                                        const uint32_t* __restrict__ ent, int eld, int ku, uint32_t rs24,
+                                        const typename Cx<T>::v2* s_c, const uint32_t* s_xm, uint32_t d) {
+  using V2 = typename Cx<T>::v2;
+  for (int k0 = 0; k0 < ku; k0 += kUnroll) {
+    uint32_t en[kUnroll];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) en[j] = ent[(k0 + j) * eld];
+    V2 src[kUnroll][VPL];
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      // 20-bit row index times a 24-bit row stride: a 48-bit element offset (mul_u24 + mul_hi_u24)
+      const uint64_t off = static_cast<uint64_t>(en[j] & kTargetMask) * static_cast<uint64_t>(rs24 & 0xffffffu);
+      const V2* sp = xcol + off;
+#pragma unroll
+      for (int v = 0; v < VPL; ++v) src[j][v] = sp[v];
+    }
+#pragma unroll
+    for (int j = 0; j < kUnroll; ++j) {
+      const uint32_t ci = en[j] >> kTargetBits;
+      V2 c = s_c[ci];
+      if (!ONE_ID && (__popc(d & s_xm[ci & kBondMask]) & 1u)) {
+        c.x = -c.x;
+        c.y = -c.y;
+      }
+      axpy_regs<T, VPL, REAL>(ar, ai, src[j], c.x, c.y);
+    }
+  }
+}
+
 // ------------------------------------------------------------------ column pass: up bonds, y +=
 // Tiles: (stripe of W ids, chunk of EPB/W rows), stripe-major; element e of a tile ->
 // (row chunk*EPB/W + e/W, id stripe*W + e%W).  While an XCD works through one stripe (all rows x
@@ -303,15 +344,30 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
   using SP = Split<T, RB>;
   constexpr int L = SP::L, VPL = SP::VPL, EPB = SP::EPB;
   constexpr bool kStage = StageEntries<EPB>::value;
-  __shared__ T s_c0[kMaxBondsLds], s_c1[kMaxBondsLds];
+  __shared__ V2 s_c[kCoefSlots];  // [parity * 512 + bond]; signed by d when stripes are one id wide
   __shared__ uint32_t s_xm[kMaxBondsLds];
   __shared__ int s_tile;
-  for (int b = threadIdx.x; b < S.nbond_up; b += kHvBlock) {
-    const double2 c = S.coef_up[b];
-    s_c0[b] = static_cast<T>(c.x);
-    s_c1[b] = static_cast<T>(c.y);
-    s_xm[b] = S.xmask_up[b];
-  }
+  const bool one_id = wlog == 0;
+  auto fill_coef = [&](uint32_t dm) {
+    for (int b = threadIdx.x; b < kMaxBondsLds; b += kHvBlock) {
+      V2 c;
+      c.x = c.y = static_cast<T>(0);
+      uint32_t xm = 0;
+      if (b < S.nbond_up) {
+        const double2 cc = S.coef_up[b];
+        xm = S.xmask_up[b];
+        const bool neg = (__popc(dm & xm) & 1u) != 0;
+        c.x = static_cast<T>(neg ? -cc.x : cc.x);
+        c.y = static_cast<T>(neg ? -cc.y : cc.y);
+      }
+      s_c[b] = c;
+      c.x = -c.x;
+      c.y = -c.y;
+      s_c[b + kMaxBondsLds] = c;
+      s_xm[b] = xm;
+    }
+  };
+  if (!one_id) fill_coef(0u);
   const int xcd = blockIdx.x % kXcd;
   const int sub = static_cast<int>(threadIdx.x) % L, el = static_cast<int>(threadIdx.x) / L;
   const int rr = r0 + sub * VPL;
@@ -321,6 +377,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
   const int t0 = min(ntiles, xcd * per), nmine = min(ntiles, t0 + per) - t0;
   const int Ddn = S.Ddn, Dup = S.Dup;
   const int64_t row_stride = static_cast<int64_t>(Ddn) * RT;
+  const uint32_t rs24 = static_cast<uint32_t>(row_stride);  // < 2^24, checked by the host
   double sxv[VPL], dre[VPL], dimg[VPL];
 #pragma unroll
   for (int v = 0; v < VPL; ++v) {
@@ -330,11 +387,12 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
   extern __shared__ uint4 s_ent4[];  // this tile's entries, [maxcnt][rows_per_tile] uint32
   uint32_t* s_ent = reinterpret_cast<uint32_t*>(s_ent4);
   const int mc = S.maxcnt_up, estride = S.entT_stride_up;
-  int cur_rc = -1;
+  int cur_rc = -1, cur_stripe = -1;
   for (int local = next_tile(ctr, xcd, &s_tile); local < nmine; local = next_tile(ctr, xcd, &s_tile)) {
-    const int tile = t0 + local;
+    const int tile = __builtin_amdgcn_readfirstlane(t0 + local);
     const int stripe = tile / tps;
     const int rc = tile - stripe * tps;
+    bool changed = false;
     if (kStage && rc != cur_rc) {  // uniform: stage the row chunk's entries
       const int rbase = rc * rows_per_tile;
       if ((rows_per_tile & 3) == 0) {
@@ -351,8 +409,14 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
         }
       }
       cur_rc = rc;
-      __syncthreads();
+      changed = true;
     }
+    if (one_id && stripe != cur_stripe) {  // uniform: sign the coefficients by this id's string
+      fill_coef(S.dn_mask[stripe]);
+      cur_stripe = stripe;
+      changed = true;
+    }
+    if (changed) __syncthreads();
     const int id = (stripe << wlog) + (el & (W - 1));
     const int rl = el >> wlog;
     const int iu = rc * rows_per_tile + rl;
@@ -365,32 +429,10 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
     const int ku = S.cnt_up[iu];
     const uint32_t* ent = kStage ? s_ent + rl : S.entT_up + iu;
     const int eld = kStage ? rows_per_tile : estride;
-    for (int k0 = 0; k0 < ku; k0 += kUnroll) {
-      uint32_t en[kUnroll];
-#pragma unroll
-      for (int j = 0; j < kUnroll; ++j) en[j] = (k0 + j < ku) ? ent[(k0 + j) * eld] : 0u;
-      V2 src[kUnroll][VPL];
-#pragma unroll
-      for (int j = 0; j < kUnroll; ++j) {
-        const V2* sp = xcol + (en[j] & kTargetMask) * row_stride;
-#pragma unroll
-        for (int v = 0; v < VPL; ++v) {
-          if (k0 + j < ku) {
-            src[j][v] = sp[v];
-          } else {
-            src[j][v].x = static_cast<T>(0);
-            src[j][v].y = static_cast<T>(0);
-          }
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < kUnroll; ++j) {
-        if (k0 + j >= ku) break;
-        const uint32_t b = (en[j] >> kTargetBits) & kBondMask;
-        const bool neg = ((en[j] >> 31) ^ (__popc(d & s_xm[b]) & 1u)) != 0;
-        axpy_regs<T, VPL, REAL>(ar, ai, src[j], neg ? -s_c0[b] : s_c0[b], neg ? -s_c1[b] : s_c1[b]);
-      }
-    }
+    if (one_id)
+      up_hops<T, VPL, REAL, true>(ar, ai, xcol, ent, eld, ku, rs24, s_c, s_xm, d);
+    else
+      up_hops<T, VPL, REAL, false>(ar, ai, xcol, ent, eld, ku, rs24, s_c, s_xm, d);
     const int64_t eo = iu * row_stride + static_cast<int64_t>(id) * RT + rr;
     V2* ye = y + eo;
 #pragma unroll
@@ -445,7 +487,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_flat(const SectorView S, const 
       const uint32_t ent = S.ent_dn[k];
       const uint32_t t = ent & kTargetMask;
       const uint32_t b = (ent >> kTargetBits) & kBondMask;
-      const uint32_t par = (ent >> 31) ^ (__popc(u & S.xmask_dn[b]) & 1u);
+      const uint32_t par = ((ent >> kParityBit) & 1u) ^ (__popc(u & S.xmask_dn[b]) & 1u);
       const double2 c = S.coef_dn[b];
       axpy_block<T, RB, REAL>(ar, ai, x + (rowbase + t) * RT + r0, static_cast<T>(par ? -c.x : c.x),
                               static_cast<T>(par ? -c.y : c.y));
@@ -464,7 +506,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_flat(const SectorView S, const 
       const uint32_t ent = S.ent_up[k];
       const uint32_t t = ent & kTargetMask;
       const uint32_t b = (ent >> kTargetBits) & kBondMask;
-      const uint32_t par = (ent >> 31) ^ (__popc(d & S.xmask_up[b]) & 1u);
+      const uint32_t par = ((ent >> kParityBit) & 1u) ^ (__popc(d & S.xmask_up[b]) & 1u);
       const double2 c = S.coef_up[b];
       axpy_block<T, RB, REAL>(ar, ai, x + (static_cast<int64_t>(t) * S.Ddn + id) * RT + r0,
                               static_cast<T>(par ? -c.x : c.x), static_cast<T>(par ? -c.y : c.y));
@@ -590,6 +632,8 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
       stage ? static_cast<size_t>(std::max(S.maxcnt_up, 1)) * (SP::EPB >> p.wlog) * sizeof(uint32_t) : 16;
   FTLM_REQUIRE(lds_dn <= 48 * 1024 && lds_up <= 48 * 1024, FTLM_EUNSUPPORTED,
                "hop table of one H*V tile exceeds 48 KB of LDS");
+  FTLM_REQUIRE(static_cast<int64_t>(S.Ddn) * RT < (int64_t{1} << 24), FTLM_EUNSUPPORTED,
+               "D_dn * R must stay below 2^24 for the column pass's 24-bit source offsets");
   record_pass_event(0, st);
   hipLaunchKernelGGL((k_hv_dn<T, RB, REAL>), dim3(p.grid_dn), dim3(kHvBlock), lds_dn, st, S, xp, yp, RT, r0, xs,
                      dotp, Sx->tile_ctr.p, p.nchunk, p.ntiles_dn);

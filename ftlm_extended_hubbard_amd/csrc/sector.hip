@@ -127,13 +127,14 @@ __global__ void k_hop_fill(const uint32_t* masks, int D, const BondDev* bonds, i
     const uint32_t src = m ^ (1u << bd.from) ^ (1u << bd.to);
     const uint32_t t = static_cast<uint32_t>(lex_rank(src, n, k));
     const uint32_t par = __popc(m & bd.own) & 1u;
-    ent[w++] = t | (static_cast<uint32_t>(b) << kTargetBits) | (par << 31);
+    ent[w++] = t | (static_cast<uint32_t>(b) << kTargetBits) | (par << kParityBit);
   }
 }
 
-// CSR -> slot-major copy of a hop table (entT[k * D + i] = entry k of string i).
-// Slot rows are padded to a multiple of 256 strings (zero entries) so that any aligned chunk of up
-// to 256 strings of one slot lies inside the table and loads in 16-byte pieces.
+// CSR -> slot-major copy of a hop table (entT[k * Dpad + i] = entry k of string i); the table is
+// pre-filled with kPadEntry.  Slot rows are padded to a multiple of 256 strings so that any
+// aligned chunk of up to 256 strings of one slot lies inside the table and loads in 16-byte
+// pieces.
 __global__ void k_hop_transpose(const int32_t* off, const uint32_t* ent, int D, int Dpad, uint32_t* entT,
                                 int32_t* cnt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -289,8 +290,8 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
   S->real_h = true;
   for (int sp = 0; sp < 2; ++sp) {
     const int nb = static_cast<int>(bonds[sp].size());
-    FTLM_REQUIRE(nb <= kMaxBondsLds && nb <= static_cast<int>(kBondMask), FTLM_EUNSUPPORTED,
-                 "more than 512 distinct directed bonds per spin");
+    FTLM_REQUIRE(nb < static_cast<int>(kZeroBond), FTLM_EUNSUPPORTED,
+                 "more than 510 distinct directed bonds per spin");
     S->nbond[sp] = nb;
     std::vector<BondDev> bd(static_cast<size_t>(std::max(nb, 1)));
     std::vector<double2> cf(static_cast<size_t>(std::max(nb, 1)));
@@ -333,11 +334,13 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
     check_launch("k_hop_fill");
     int mc = 0;
     for (int i = 0; i < D; ++i) mc = std::max(mc, hc[i]);
+    mc = std::max(kSlotPad, (mc + kSlotPad - 1) / kSlotPad * kSlotPad);
     S->maxcnt[sp] = mc;
     const int Dpad = (D + 255) / 256 * 256;
     S->entT_stride[sp] = Dpad;
-    S->entT[sp].alloc(static_cast<size_t>(std::max(mc, 1)) * Dpad);
-    FTLM_HIP_CHECK(hipMemset(S->entT[sp].p, 0, S->entT[sp].n * sizeof(uint32_t)));
+    S->entT[sp].alloc(static_cast<size_t>(mc) * Dpad);
+    FTLM_HIP_CHECK(hipMemsetD32(reinterpret_cast<hipDeviceptr_t>(S->entT[sp].p), static_cast<int>(kPadEntry),
+                                S->entT[sp].n));
     S->cnt[sp].alloc(static_cast<size_t>(D));
     hipLaunchKernelGGL(k_hop_transpose, dim3(grid_for(D, 256)), dim3(256), 0, 0, S->off[sp].p, S->ent[sp].p, D, Dpad,
                        S->entT[sp].p, S->cnt[sp].p);
