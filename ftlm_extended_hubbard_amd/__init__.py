@@ -12,12 +12,16 @@ from .api import (
     FtlmParams,
     HubbardOperator,
     HubbardParams,
+    KBlockOperator,
     KrylovSession,
     LanczosExtrema,
+    MomentumSector,
+    OrbitKRepEntries,
     RectLattice,
     RitzData,
     apply_extended_hubbard,
     build_hubbard_geometry,
+    canonical_states,
     device_count,
     ftlm_log_partition_complex,
     ftlm_log_partition_real,
@@ -26,7 +30,10 @@ from .api import (
     lanczos_extrema,
     lanczos_tridiagonal,
     nsites,
+    pack_raw_state,
     quadrature_ritz,
+    translate_states,
+    translation_bloch_phase,
     tridiagonal_extrema,
     version_string,
 )
@@ -35,5 +42,6 @@ __all__ = [
     "HubbardParams", "nsites", "version_string", "RectLattice", "build_hubbard_geometry", "FockBasis",
     "HubbardOperator", "apply_extended_hubbard", "LanczosExtrema", "lanczos_tridiagonal", "lanczos_extrema",
     "tridiagonal_extrema", "FtlmParams", "ftlm_log_partition_complex", "ftlm_log_partition_real", "RitzData",
-    "ftlm_ritz", "ftlm_thermo", "quadrature_ritz", "device_count", "KrylovSession", "_lib",
+    "ftlm_ritz", "ftlm_thermo", "quadrature_ritz", "device_count", "KrylovSession", "MomentumSector", "KBlockOperator", "OrbitKRepEntries", "pack_raw_state",
+    "translate_states", "canonical_states", "translation_bloch_phase", "_lib",
 ]

@@ -95,6 +95,15 @@ def lib() -> C.CDLL:
     L.ftlm_hip_krylov_result.argtypes = [vp, vp, vp, vp, vp]
     L.ftlm_hip_krylov_bytes.argtypes = [vp, i64p]
     L.ftlm_hip_krylov_destroy.argtypes = [vp]
+    L.ftlm_hip_translate_states.argtypes = [C.c_int, C.c_int, vp, C.c_int64, C.c_int, C.c_int, vp, vp]
+    L.ftlm_hip_canonical_states.argtypes = [C.c_int, C.c_int, vp, C.c_int64, vp, vp]
+    L.ftlm_hip_bloch_phase.argtypes = [C.c_int] * 6 + [dp, dp]
+    L.ftlm_hip_kblock_create.argtypes = [C.c_int] * 6 + [C.c_double, vp, C.c_int, vp, C.c_int, C.c_double,
+                                                         C.c_double, vp, C.POINTER(vp)]
+    L.ftlm_hip_kblock_info.argtypes = [vp, ip, ip, i64p, i64p]
+    L.ftlm_hip_kblock_entries.argtypes = [vp, C.c_int64, C.c_int64, vp, vp, vp, vp]
+    L.ftlm_hip_kblock_index_of_rep.argtypes = [vp, vp, C.c_int64, vp]
+    L.ftlm_hip_kblock_map.argtypes = [vp, vp, C.c_int64, vp, vp, vp, vp]
     _lib = L
     return L
 

@@ -223,6 +223,113 @@ class HubbardOperator(_Op):
         return cls._create(p.Lx * p.Ly, n_up, n_dn, hops, pairs, p.U, p.V)
 
 
+# ---------------------------------------------------------------- translations and k-blocks
+
+def pack_raw_state(up, dn):
+    """(up << 16) | dn (include/ftlm/symmetry/packed_raw_state.hpp:12-14), elementwise."""
+    return ((np.asarray(up, np.uint32) << np.uint32(16)) | np.asarray(dn, np.uint32)).astype(np.uint32)
+
+
+def translate_states(Lx: int, Ly: int, states, dx: int, dy: int):
+    """translate_occupation_bits_rect on both strings and fermionic_translation_sign_rect
+    (occup_bits_translate.hpp:10-25, fermionic_translation.hpp:32-54): (translated, sign)."""
+    st = np.ascontiguousarray(states, np.uint32)
+    out, sg = np.zeros(len(st), np.uint32), np.zeros(len(st))
+    L.check(L.lib().ftlm_hip_translate_states(Lx, Ly, st.ctypes.data, len(st), dx, dy, out.ctypes.data,
+                                              sg.ctypes.data))
+    return out, sg
+
+
+def canonical_states(Lx: int, Ly: int, states):
+    """canonical_raw_state_rect and |translation_stabilizer_rect| (orbit.hpp:93-155)."""
+    st = np.ascontiguousarray(states, np.uint32)
+    out, ss = np.zeros(len(st), np.uint32), np.zeros(len(st), np.int32)
+    L.check(L.lib().ftlm_hip_canonical_states(Lx, Ly, st.ctypes.data, len(st), out.ctypes.data, ss.ctypes.data))
+    return out, ss
+
+
+@dataclass(frozen=True)
+class MomentumSector:
+    """MomentumSector (include/ftlm/symmetry/momentum_sector.hpp:20-33)."""
+    kx: int = 0
+    ky: int = 0
+    lx: int = 4
+    ly: int = 4
+
+    def flat_index(self) -> int:
+        return self.ky * self.lx + self.kx
+
+
+def translation_bloch_phase(K: MomentumSector, dx: int, dy: int) -> complex:
+    """exp(-2 pi i (kx dx / lx + ky dy / ly)) (momentum_sector.hpp:57-63)."""
+    re, im = C.c_double(0.0), C.c_double(0.0)
+    L.check(L.lib().ftlm_hip_bloch_phase(K.lx, K.ly, K.kx, K.ky, dx, dy, C.byref(re), C.byref(im)))
+    return complex(re.value, im.value)
+
+
+@dataclass
+class OrbitKRepEntries:
+    rep: np.ndarray          # packed representatives, sorted
+    orbit_size: np.ndarray
+    stabilizer_size: np.ndarray
+    A_k: np.ndarray          # complex
+
+
+class KBlockOperator(_Op):
+    """OrbitKBlockBasis (src/orbit_k_block_basis.cpp:70-149) with the operator of
+    apply_orbit_k_block (:151-220), resident on the device.  Blocks of one sector and
+    Hamiltonian share one orbit graph when created with `share=`."""
+
+    @classmethod
+    def _create(cls, Lx, Ly, n_up, n_dn, K: MomentumSector, hops, pairs, U, V, tol=1e-12, share=None):
+        self = cls()
+        hops = np.ascontiguousarray(hops, L.HOP_DTYPE)
+        pairs = np.ascontiguousarray(pairs, L.PAIR_DTYPE)
+        L.check(L.lib().ftlm_hip_kblock_create(Lx, Ly, n_up, n_dn, K.kx, K.ky, float(tol),
+                                               hops.ctypes.data if len(hops) else None, len(hops),
+                                               pairs.ctypes.data if len(pairs) else None, len(pairs), float(U),
+                                               float(V), None if share is None else share._h, C.byref(self._h)))
+        self.K = K
+        self._share = share  # keeps the shared graph's owner alive for readability; the graph is refcounted
+        return self
+
+    @classmethod
+    def from_params(cls, p: HubbardParams, n_up: int, n_dn: int, K: MomentumSector, tol: float = 1e-12,
+                    share: "KBlockOperator | None" = None) -> "KBlockOperator":
+        hops, pairs = build_hubbard_geometry(p)
+        K = MomentumSector(K.kx, K.ky, p.Lx, p.Ly)
+        return cls._create(p.Lx, p.Ly, n_up, n_dn, K, hops, pairs, p.U, p.V, tol, share)
+
+    def info(self):
+        kx, ky, no, nnz = C.c_int(0), C.c_int(0), C.c_int64(0), C.c_int64(0)
+        L.check(L.lib().ftlm_hip_kblock_info(self._h, C.byref(kx), C.byref(ky), C.byref(no), C.byref(nnz)))
+        return {"kx": kx.value, "ky": ky.value, "n_orbits": no.value, "nnz": nnz.value}
+
+    def entries(self) -> OrbitKRepEntries:
+        n = self.dim()
+        rep, osz, ssz = np.zeros(n, np.uint32), np.zeros(n, np.int32), np.zeros(n, np.int32)
+        A = np.zeros(2 * n)
+        if n:
+            L.check(L.lib().ftlm_hip_kblock_entries(self._h, 0, n, rep.ctypes.data, osz.ctypes.data, ssz.ctypes.data,
+                                                    A.ctypes.data))
+        return OrbitKRepEntries(rep, osz, ssz, A[0::2] + 1j * A[1::2])
+
+    def reduced_index_of_rep(self, reps) -> np.ndarray:
+        r = np.ascontiguousarray(np.atleast_1d(reps), np.uint32)
+        out = np.zeros(len(r), np.int64)
+        L.check(L.lib().ftlm_hip_kblock_index_of_rep(self._h, r.ctypes.data, len(r), out.ctypes.data))
+        return out
+
+    def map_raw_to_rep(self, states):
+        """(index, dx, dy, eta) per packed state; index -1 where the reference returns false."""
+        st = np.ascontiguousarray(np.atleast_1d(states), np.uint32)
+        n = len(st)
+        idx, dx, dy, eta = np.zeros(n, np.int64), np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
+        L.check(L.lib().ftlm_hip_kblock_map(self._h, st.ctypes.data, n, idx.ctypes.data, dx.ctypes.data,
+                                            dy.ctypes.data, eta.ctypes.data))
+        return idx, dx, dy, eta
+
+
 _APPLY_CACHE: dict = {}
 
 

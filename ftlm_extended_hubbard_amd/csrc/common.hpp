@@ -192,6 +192,18 @@ struct Sector : ftlm_op {
   SectorView view() const;
 };
 
+// Interleaved complex element types of the two vector dtypes.
+template <typename T>
+struct Cx;
+template <>
+struct Cx<double> {
+  using v2 = double2;
+};
+template <>
+struct Cx<float> {
+  using v2 = float2;
+};
+
 // ------------------------------------------------------------------ launches
 inline unsigned grid_for(int64_t n, int block) { return static_cast<unsigned>((n + block - 1) / block); }
 

@@ -42,17 +42,6 @@
 
 namespace ftlm_hip {
 
-template <typename T>
-struct Cx;
-template <>
-struct Cx<double> {
-  using v2 = double2;
-};
-template <>
-struct Cx<float> {
-  using v2 = float2;
-};
-
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -703,6 +692,7 @@ int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const 
              hipStream_t st, bool force_real) {
   if (op->kind == kSector)
     return hv_apply_sector(static_cast<const Sector*>(op), x, y, R, dtype, xs, dotp, st, force_real);
+  FTLM_REQUIRE(!force_real, FTLM_EUNSUPPORTED, "the real-part apply is defined for full-sector operators only");
   return hv_apply_kblock(op, x, y, R, dtype, xs, dotp, st);
 }
 

@@ -1,14 +1,948 @@
-// orbit.hip -- translation-orbit momentum blocks (k-blocks) on the device.
-// (filled in below the sector path; until then the k-block dispatch reports unsupported)
+// orbit.hip -- translation orbits and momentum (k) blocks on the device (gfx950).
+//
+// Replaces, on the device:
+//   translate_occupation_bits_rect        include/ftlm/symmetry/occup_bits_translate.hpp:10-25
+//   fermionic_translation_sign_rect       include/ftlm/symmetry/fermionic_translation.hpp:32-54
+//   canonical_raw_state_rect / translation_stabilizer_rect   include/ftlm/symmetry/orbit.hpp:93-155
+//   translation_bloch_phase               include/ftlm/symmetry/momentum_sector.hpp:57-63
+//   OrbitKBlockBasis::build / reduced_index_of_rep / map_raw_to_rep
+//                                         src/orbit_k_block_basis.cpp:56-149
+//   apply_orbit_k_block                   src/orbit_k_block_basis.cpp:151-220
+//
+// Orbit graph (shared by every momentum of one sector and Hamiltonian).  States are scanned in
+// numeric (u, d) order, so the orbit representatives (minimal pack (u << 16) | d over the
+// orbit) come out sorted by pack, which is the reference's row order of every k-block.  For
+// each representative we keep its stabiliser as a 16-bit mask over translations
+// t = dy * Lx + dx and the fermionic sign of each stabiliser element.  The Hamiltonian is
+// stored once, K-independent, in pull form: the reference's push form (column beta, hop
+// rep(beta) -> s', s' = T_R rep(alpha)) is generated per column and stably sorted by target
+// orbit alpha, so row alpha lists its (beta, hop) terms in the reference's accumulation order.
+// An entry holds beta, the translation R, the sign jw(hop) * eta(alpha, R) and the merged
+// bond; the diagonal is an entry of its own at column alpha.
+//
+// A momentum K then only needs A_K(alpha) = sum_{S in Stab} eta_S chi*_K(S) per orbit (the
+// rows with |A_K| > tol, the map orbit -> row, sqrt|A_K|) and its 16 Bloch phases:
+//   y[a] = sum_entries c_bond * sign * sqrt(|A_a| / |A_b|) * chi*_K(R) * x[b]
+// with the unconjugated phase of translation_bloch_phase, as the reference applies it.
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
+#include <memory>
+#include <tuple>
+#include <vector>
 
 #include "common.hpp"
 
 namespace ftlm_hip {
 
+constexpr int kMaxOrbitSites = 16;   // RawState packing (u << 16) | d
+constexpr uint32_t kColMask = 0xFFFFFFu;
+constexpr uint32_t kDiagFlag = 1u << 29;
+
+// Lattice translation tables (kernel argument): tsite[t][s] = image of site s under
+// translation t = dy * lx + dx.
+struct LatDev {
+  int lx, ly, n, nt;
+  uint8_t tsite[kMaxOrbitSites][kMaxOrbitSites];
+};
+
+LatDev make_lat(int lx, int ly) {
+  LatDev L{};
+  L.lx = lx;
+  L.ly = ly;
+  L.n = lx * ly;
+  L.nt = lx * ly;
+  for (int dy = 0; dy < ly; ++dy)
+    for (int dx = 0; dx < lx; ++dx)
+      for (int s = 0; s < L.n; ++s) {
+        const int x = s % lx, y = s / lx;
+        L.tsite[dy * lx + dx][s] = static_cast<uint8_t>(((y + dy) % ly) * lx + (x + dx) % lx);
+      }
+  return L;
+}
+
+__host__ __device__ __forceinline__ uint32_t tr_bits(uint32_t m, const LatDev& L, int t) {
+  uint32_t out = 0;
+  while (m) {
+    const int s = __builtin_ctz(m);
+    m &= m - 1;
+    out |= 1u << L.tsite[t][s];
+  }
+  return out;
+}
+
+__host__ __device__ __forceinline__ uint32_t pack_state(uint32_t u, uint32_t d) { return (u << 16) | d; }
+
+// Parity of the inversions of the translated mode list, modes listed by site ascending with up
+// (2s) before down (2s+1): the interleaved mode order itself.
+__host__ __device__ __forceinline__ int tr_parity(uint32_t u, uint32_t d, const LatDev& L, int t) {
+  uint64_t seen = 0;
+  int inv = 0;
+  for (int s = 0; s < L.n; ++s) {
+    const int ts = L.tsite[t][s];
+    if ((u >> s) & 1u) {
+      const int m = 2 * ts;
+      inv += __builtin_popcountll(seen >> (m + 1));
+      seen |= uint64_t{1} << m;
+    }
+    if ((d >> s) & 1u) {
+      const int m = 2 * ts + 1;
+      inv += __builtin_popcountll(seen >> (m + 1));
+      seen |= uint64_t{1} << m;
+    }
+  }
+  return inv & 1;
+}
+
+// Canonical pack (minimum over the orbit), stabiliser mask and odd-sign mask of the stabiliser.
+__host__ __device__ __forceinline__ uint32_t canon_state(uint32_t u, uint32_t d, const LatDev& L, uint32_t* stab,
+                                                         uint32_t* eta) {
+  const uint32_t self = pack_state(u, d);
+  uint32_t best = 0xFFFFFFFFu, st = 0, et = 0;
+  for (int t = 0; t < L.nt; ++t) {
+    const uint32_t p = pack_state(tr_bits(u, L, t), tr_bits(d, L, t));
+    best = p < best ? p : best;
+    if (p == self) {
+      st |= 1u << t;
+      if (stab && tr_parity(u, d, L, t)) et |= 1u << t;
+    }
+  }
+  if (stab) *stab = st;
+  if (eta) *eta = et;
+  return best;
+}
+
+// First translation R (dx outer, dy inner) with T_R rep = s; -1 if none.
+__host__ __device__ __forceinline__ int find_translation(uint32_t rep, uint32_t s, const LatDev& L) {
+  const uint32_t ru = rep >> 16, rd = rep & 0xFFFFu, su = s >> 16, sd = s & 0xFFFFu;
+  for (int dx = 0; dx < L.lx; ++dx)
+    for (int dy = 0; dy < L.ly; ++dy) {
+      const int t = dy * L.lx + dx;
+      if (tr_bits(ru, L, t) == su && tr_bits(rd, L, t) == sd) return t;
+    }
+  return -1;
+}
+
+__device__ __forceinline__ int64_t lower_bound_u32(const uint32_t* a, int64_t n, uint32_t key) {
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (a[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// Occupied modes strictly between modes ma and mb, interleaved order m = 2 * site + spin
+// (fermions_between_modes, src/hubbard_hamiltonian.cpp:24-43).
+__device__ __forceinline__ uint32_t interleave16(uint32_t x) {
+  x &= 0xFFFFu;
+  x = (x | (x << 8)) & 0x00FF00FFu;
+  x = (x | (x << 4)) & 0x0F0F0F0Fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  x = (x | (x << 1)) & 0x55555555u;
+  return x;
+}
+
+__device__ __forceinline__ int modes_between(int ma, int mb, uint32_t u, uint32_t d) {
+  const int lo = ma < mb ? ma : mb, hi = ma < mb ? mb : ma;
+  const uint32_t occ = interleave16(u) | (interleave16(d) << 1);
+  const uint32_t below_hi = hi >= 32 ? 0xFFFFFFFFu : ((1u << hi) - 1u);
+  const uint32_t upto_lo = (2u << lo) - 1u;
+  return __popc(occ & below_hi & ~upto_lo);
+}
+
+struct OrbBond {
+  int from, to, spin;
+};
+
+// ------------------------------------------------------------------ graph kernels
+__global__ void k_orb_flag(const uint32_t* upN, int Du, const uint32_t* dnN, int Dd, LatDev L, uint8_t* flag) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= static_cast<int64_t>(Du) * Dd) return;
+  const uint32_t u = upN[i / Dd], d = dnN[i % Dd];
+  flag[i] = canon_state(u, d, L, nullptr, nullptr) == pack_state(u, d) ? 1 : 0;
+}
+
+__global__ void k_orb_reps(const int32_t* idx, int64_t nrep, const uint32_t* upN, const uint32_t* dnN, int Dd,
+                           LatDev L, const int2* pairs, int npairs, double U, double V, uint32_t* pack,
+                           uint16_t* stab, uint16_t* eta, double* diag) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nrep) return;
+  const int64_t i = idx[g];
+  const uint32_t u = upN[i / Dd], d = dnN[i % Dd];
+  uint32_t st = 0, et = 0;
+  canon_state(u, d, L, &st, &et);
+  pack[g] = pack_state(u, d);
+  stab[g] = static_cast<uint16_t>(st);
+  eta[g] = static_cast<uint16_t>(et);
+  // diagonal in the reference's order: U * n_double, then + V * o_i o_j pair by pair
+  double dg = static_cast<double>(__popc(u & d)) * U;
+  for (int p = 0; p < npairs; ++p) {
+    const int oi = static_cast<int>(((u >> pairs[p].x) & 1u) + ((d >> pairs[p].x) & 1u));
+    const int oj = static_cast<int>(((u >> pairs[p].y) & 1u) + ((d >> pairs[p].y) & 1u));
+    dg += V * static_cast<double>(oi * oj);
+  }
+  diag[g] = dg;
+}
+
+__device__ __forceinline__ bool hop_applies(const OrbBond& b, uint32_t u, uint32_t d) {
+  const uint32_t s = b.spin == 0 ? u : d;
+  return ((s >> b.from) & 1u) && !((s >> b.to) & 1u);
+}
+
+__global__ void k_orb_count(const uint32_t* pack, int64_t nrep, const OrbBond* bonds, int nb, int32_t* cnt) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nrep) return;
+  const uint32_t u = pack[g] >> 16, d = pack[g] & 0xFFFFu;
+  int c = 1;  // diagonal
+  for (int b = 0; b < nb; ++b) c += hop_applies(bonds[b], u, d) ? 1 : 0;
+  cnt[g] = c;
+}
+
+// Push-form entries of column g: key = target orbit, value = {g | R << 24 | sign << 28, bond}.
+__global__ void k_orb_fill(const uint32_t* pack, int64_t nrep, const OrbBond* bonds, int nb, const int32_t* off,
+                           LatDev L, uint32_t* keys, uint2* vals, int* bad) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nrep) return;
+  const uint32_t u = pack[g] >> 16, d = pack[g] & 0xFFFFu;
+  int64_t w = off[g];
+  keys[w] = static_cast<uint32_t>(g);
+  vals[w] = make_uint2(static_cast<uint32_t>(g) | kDiagFlag, 0u);
+  ++w;
+  for (int b = 0; b < nb; ++b) {
+    const OrbBond bd = bonds[b];
+    if (!hop_applies(bd, u, d)) continue;
+    const uint32_t flip = (1u << bd.from) ^ (1u << bd.to);
+    const uint32_t u2 = bd.spin == 0 ? u ^ flip : u, d2 = bd.spin == 1 ? d ^ flip : d;
+    const int jw = modes_between(2 * bd.from + bd.spin, 2 * bd.to + bd.spin, u, d) & 1;
+    const uint32_t sp = pack_state(u2, d2);
+    const uint32_t cp = canon_state(u2, d2, L, nullptr, nullptr);
+    const int64_t a = lower_bound_u32(pack, nrep, cp);
+    int t = -1;
+    if (a < nrep && pack[a] == cp) t = find_translation(cp, sp, L);
+    if (t < 0) {  // cannot happen for a closed sector; reported to the host
+      atomicExch(bad, 1);
+      keys[w] = static_cast<uint32_t>(g);
+      vals[w] = make_uint2(static_cast<uint32_t>(g) | kDiagFlag, 0xFFFFFFFFu);
+      ++w;
+      continue;
+    }
+    const int eta = tr_parity(cp >> 16, cp & 0xFFFFu, L, t);
+    keys[w] = static_cast<uint32_t>(a);
+    vals[w] = make_uint2(static_cast<uint32_t>(g) | (static_cast<uint32_t>(t) << 24) |
+                             (static_cast<uint32_t>(jw ^ eta) << 28),
+                         static_cast<uint32_t>(b));
+    ++w;
+  }
+}
+
+__global__ void k_csr_rows(const uint32_t* keys, int64_t nnz, int32_t* row_off) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nnz) return;
+  if (i == 0 || keys[i - 1] != keys[i]) row_off[keys[i]] = static_cast<int32_t>(i);
+}
+
+// ------------------------------------------------------------------ per-momentum kernels
+struct Phases {
+  double2 chi[kMaxOrbitSites];  // translation_bloch_phase(K, dx, dy) at t = dy * lx + dx
+};
+
+// A_K = sum over the stabiliser (t ascending) of eta_t * chi_t, as the reference accumulates it
+// ((eta, 0) * chi is exact, so the sum is a plain ordered sum of +-chi).
+__global__ void k_kblock_A(const uint16_t* stab, const uint16_t* eta, int64_t nrep, int nt, Phases ph, double tol,
+                           double2* A, uint8_t* member) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nrep) return;
+  double re = 0.0, im = 0.0;
+  const uint32_t st = stab[g], et = eta[g];
+  for (int t = 0; t < nt; ++t) {
+    if (!((st >> t) & 1u)) continue;
+    const double2 c = ph.chi[t];
+    if ((et >> t) & 1u) {
+      re = __dadd_rn(re, -c.x);
+      im = __dadd_rn(im, -c.y);
+    } else {
+      re = __dadd_rn(re, c.x);
+      im = __dadd_rn(im, c.y);
+    }
+  }
+  A[g] = make_double2(re, im);
+  member[g] = hypot(re, im) > tol ? 1 : 0;
+}
+
+__global__ void k_kblock_maps(const int32_t* rows, int64_t dk, const double2* A, int32_t* map, double* sq,
+                              double* isq) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= dk) return;
+  const int32_t g = rows[r];
+  const double a = hypot(A[g].x, A[g].y);
+  map[g] = static_cast<int32_t>(r);
+  sq[g] = sqrt(a);
+  isq[g] = 1.0 / sqrt(a);
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f64(double* p, int64_t n, double v) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// Batched map_raw_to_rep / reduced_index_of_rep / translation primitives.
+__global__ void k_kblock_map_states(const uint32_t* states, int64_t count, const uint32_t* pack, int64_t nrep,
+                                    const int32_t* map, LatDev L, int64_t* index, int* dx, int* dy, double* eta) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t s = states[i];
+  const uint32_t cp = canon_state(s >> 16, s & 0xFFFFu, L, nullptr, nullptr);
+  const int64_t g = lower_bound_u32(pack, nrep, cp);
+  int64_t r = -1;
+  int t = -1;
+  if (g < nrep && pack[g] == cp && map[g] >= 0) {
+    t = find_translation(cp, s, L);
+    if (t >= 0) r = map[g];
+  }
+  index[i] = r;
+  dx[i] = r >= 0 ? t % L.lx : 0;
+  dy[i] = r >= 0 ? t / L.lx : 0;
+  eta[i] = r >= 0 ? (tr_parity(cp >> 16, cp & 0xFFFFu, L, t) ? -1.0 : 1.0) : 0.0;
+}
+
+__global__ void k_kblock_index_of_rep(const uint32_t* reps, int64_t count, const uint32_t* pack, int64_t nrep,
+                                      const int32_t* map, int64_t* out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int64_t g = lower_bound_u32(pack, nrep, reps[i]);
+  out[i] = (g < nrep && pack[g] == reps[i]) ? map[g] : -1;
+}
+
+__global__ void k_kblock_entries(const int32_t* rows, int64_t first, int64_t count, const uint32_t* pack,
+                                 const uint16_t* stab, const double2* A, int nt, uint32_t* rep, int* osz, int* ssz,
+                                 double2* aout) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int32_t g = rows[first + i];
+  const int s = __popc(static_cast<uint32_t>(stab[g]));
+  rep[i] = pack[g];
+  ssz[i] = s;
+  osz[i] = nt / s;
+  aout[i] = A[g];
+}
+
+__global__ void k_translate_states(const uint32_t* states, int64_t count, LatDev L, int t, uint32_t* out,
+                                   double* sign) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t u = states[i] >> 16, d = states[i] & 0xFFFFu;
+  if (out) out[i] = pack_state(tr_bits(u, L, t), tr_bits(d, L, t));
+  if (sign) sign[i] = tr_parity(u, d, L, t) ? -1.0 : 1.0;
+}
+
+__global__ void k_canonical_states(const uint32_t* states, int64_t count, LatDev L, uint32_t* canon,
+                                   int* stab_size) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  uint32_t st = 0, et = 0;
+  const uint32_t c = canon_state(states[i] >> 16, states[i] & 0xFFFFu, L, &st, &et);
+  if (canon) canon[i] = c;
+  if (stab_size) stab_size[i] = __popc(st);
+}
+
+// ------------------------------------------------------------------ apply
+// One thread per (row, group of RB vectors); row r of block K gathers its CSR terms.
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void k_kblock_apply(
+    const int32_t* __restrict__ rows, int64_t dk, const int32_t* __restrict__ row_off, const uint2* __restrict__ ent,
+    const int32_t* __restrict__ map, const double* __restrict__ sq, const double* __restrict__ isq,
+    const double* __restrict__ diag, const double2* __restrict__ coef, int nbond, Phases ph,
+    const typename Cx<T>::v2* __restrict__ x, typename Cx<T>::v2* __restrict__ y, int RT, int r0,
+    const double* __restrict__ xs, double2* __restrict__ dotp) {
+  using V2 = typename Cx<T>::v2;
+  __shared__ double2 s_coef[kMaxBondsLds];
+  __shared__ double2 s_chi[kMaxOrbitSites];
+  for (int b = threadIdx.x; b < nbond; b += blockDim.x) s_coef[b] = coef[b];
+  if (threadIdx.x < kMaxOrbitSites) s_chi[threadIdx.x] = ph.chi[threadIdx.x];
+  __syncthreads();
+  double dre[RB], dimg[RB];
+#pragma unroll
+  for (int v = 0; v < RB; ++v) dre[v] = dimg[v] = 0.0;
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < dk;
+       r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t g = rows[r];
+    const double sa = sq[g];
+    T ar[RB], ai[RB];
+#pragma unroll
+    for (int v = 0; v < RB; ++v) ar[v] = ai[v] = static_cast<T>(0);
+    for (int32_t e = row_off[g]; e < row_off[g + 1]; ++e) {
+      const uint2 en = ent[e];
+      const int32_t col = static_cast<int32_t>(en.x & kColMask);
+      const int32_t c = map[col];
+      if (c < 0) continue;
+      double cr, ci;
+      if (en.x & kDiagFlag) {
+        cr = diag[g];
+        ci = 0.0;
+      } else {
+        // ((c_bond * (sign * nrm)) * chi) as the reference multiplies it
+        const double nrm = sa * isq[col];
+        const double w = ((en.x >> 28) & 1u) ? -nrm : nrm;
+        const double2 cb = s_coef[en.y];
+        const double hr = cb.x * w, hi = cb.y * w;
+        const double2 ch = s_chi[(en.x >> 24) & 15u];
+        cr = hr * ch.x - hi * ch.y;
+        ci = hr * ch.y + hi * ch.x;
+      }
+      const V2* src = x + static_cast<int64_t>(c) * RT + r0;
+      const T tr = static_cast<T>(cr), ti = static_cast<T>(ci);
+#pragma unroll
+      for (int v = 0; v < RB; ++v) {
+        const V2 s = src[v];
+        ar[v] = fma(tr, s.x, ar[v]);
+        ai[v] = fma(tr, s.y, ai[v]);
+        ar[v] = fma(-ti, s.y, ar[v]);
+        ai[v] = fma(ti, s.x, ai[v]);
+      }
+    }
+    V2* dst = y + r * RT + r0;
+    const V2* own = x + r * RT + r0;
+#pragma unroll
+    for (int v = 0; v < RB; ++v) {
+      const double sx = xs ? xs[r0 + v] : 1.0;
+      V2 o;
+      o.x = static_cast<T>(static_cast<double>(ar[v]) * sx);
+      o.y = static_cast<T>(static_cast<double>(ai[v]) * sx);
+      dst[v] = o;
+      if (dotp) {
+        const V2 q = own[v];
+        const double qr = sx * static_cast<double>(q.x), qi = sx * static_cast<double>(q.y);
+        dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+        dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+      }
+    }
+  }
+  if (dotp) {
+    __shared__ double red[256 / 64][RB][2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int v = 0; v < RB; ++v) {
+      double a = dre[v], b = dimg[v];
+#pragma unroll
+      for (int o = 32; o >= 1; o >>= 1) {
+        a += __shfl_xor(a, o, 64);
+        b += __shfl_xor(b, o, 64);
+      }
+      if (lane == 0) {
+        red[wv][v][0] = a;
+        red[wv][v][1] = b;
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < RB) {
+      double a = 0.0, b = 0.0;
+      for (int w = 0; w < 256 / 64; ++w) {
+        a += red[w][threadIdx.x][0];
+        b += red[w][threadIdx.x][1];
+      }
+      dotp[static_cast<int64_t>(blockIdx.x) * RT + r0 + threadIdx.x] = make_double2(a, b);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ host objects
+namespace {
+
+std::vector<uint32_t> strings_numeric(int n, int k) {
+  std::vector<uint32_t> v;
+  if (k == 0) {
+    v.push_back(0);
+    return v;
+  }
+  if (k > n) return v;
+  uint32_t m = (k >= 32) ? 0xFFFFFFFFu : ((1u << k) - 1u);
+  const uint64_t lim = uint64_t{1} << n;
+  while (m < lim) {
+    v.push_back(m);
+    const uint32_t c = m & (0u - m), r = m + c;  // next mask with the same popcount (Gosper)
+    if (r == 0) break;
+    m = (((r ^ m) >> 2) / c) | r;
+  }
+  return v;
+}
+
+template <class F>
+void cub_call(F&& f, hipStream_t st) {
+  size_t bytes = 0;
+  FTLM_HIP_CHECK(f(nullptr, bytes, st));
+  DevBuf<uint8_t> tmp(std::max<size_t>(bytes, 1));
+  FTLM_HIP_CHECK(f(tmp.p, bytes, st));
+  FTLM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+}  // namespace
+
+struct OrbitGraph {
+  int lx = 0, ly = 0, n_up = 0, n_dn = 0;
+  double U = 0.0, V = 0.0;
+  std::vector<ftlm_hop_t> hops;
+  std::vector<ftlm_pair_t> pairs;
+  LatDev L{};
+  int64_t nrep = 0, nnz = 0;
+  int nbond = 0;
+  bool real_h = true;
+  DevBuf<uint32_t> pack;
+  DevBuf<uint16_t> stab, eta;
+  DevBuf<double> diag;
+  DevBuf<int32_t> row_off;
+  DevBuf<uint2> ent;
+  DevBuf<double2> coef;
+
+  bool same_model(int lx_, int ly_, int nu, int nd, double U_, double V_, const ftlm_hop_t* h, int nh,
+                  const ftlm_pair_t* p, int np) const {
+    if (lx_ != lx || ly_ != ly || nu != n_up || nd != n_dn || U_ != U || V_ != V) return false;
+    if (nh != static_cast<int>(hops.size()) || np != static_cast<int>(pairs.size())) return false;
+    for (int i = 0; i < nh; ++i)
+      if (h[i].from != hops[i].from || h[i].to != hops[i].to || h[i].spin != hops[i].spin || h[i].re != hops[i].re ||
+          h[i].im != hops[i].im)
+        return false;
+    for (int i = 0; i < np; ++i)
+      if (p[i].i != pairs[i].i || p[i].j != pairs[i].j) return false;
+    return true;
+  }
+};
+
+struct KBlock : ftlm_op {
+  std::shared_ptr<OrbitGraph> G;
+  int kx = 0, ky = 0;
+  double tol = 1e-12;
+  Phases ph{};
+  DevBuf<int32_t> rows, map;
+  DevBuf<double> sq, isq;
+  DevBuf<double2> A;  // [nrep]
+};
+
+std::shared_ptr<OrbitGraph> build_orbit_graph(int lx, int ly, int n_up, int n_dn, const ftlm_hop_t* hops, int n_hops,
+                                              const ftlm_pair_t* pairs, int n_pairs, double U, double V) {
+  const int n = lx * ly;
+  auto G = std::make_shared<OrbitGraph>();
+  G->lx = lx;
+  G->ly = ly;
+  G->n_up = n_up;
+  G->n_dn = n_dn;
+  G->U = U;
+  G->V = V;
+  G->hops.assign(hops, hops + n_hops);
+  G->pairs.assign(pairs, pairs + n_pairs);
+  G->L = make_lat(lx, ly);
+  const hipStream_t st = nullptr;
+
+  // merged bonds (same (from, to, spin) summed in list order), in first-occurrence order
+  std::vector<OrbBond> bonds;
+  std::vector<double2> cf;
+  std::map<std::tuple<int, int, int>, int> seen;
+  for (int h = 0; h < n_hops; ++h) {
+    const ftlm_hop_t& e = hops[h];
+    FTLM_REQUIRE(e.spin == 0 || e.spin == 1, FTLM_EINVAL, "hop spin must be 0 or 1");
+    FTLM_REQUIRE(e.from >= 0 && e.from < n && e.to >= 0 && e.to < n, FTLM_EINVAL, "hop site outside [0, n_sites)");
+    if (e.from == e.to) continue;
+    const auto key = std::make_tuple(e.from, e.to, e.spin);
+    auto it = seen.find(key);
+    if (it == seen.end()) {
+      seen.emplace(key, static_cast<int>(bonds.size()));
+      bonds.push_back(OrbBond{e.from, e.to, e.spin});
+      cf.push_back(make_double2(e.re, e.im));
+    } else {
+      cf[it->second].x += e.re;
+      cf[it->second].y += e.im;
+    }
+  }
+  FTLM_REQUIRE(static_cast<int>(bonds.size()) <= kMaxBondsLds, FTLM_EUNSUPPORTED,
+               "more than 512 distinct directed bonds");
+  for (const double2& c : cf)
+    if (c.y != 0.0) G->real_h = false;
+  G->nbond = static_cast<int>(bonds.size());
+  G->coef.alloc(std::max<size_t>(cf.size(), 1));
+  if (!cf.empty()) G->coef.upload(cf);
+  DevBuf<OrbBond> dbonds(std::max<size_t>(bonds.size(), 1));
+  if (!bonds.empty()) dbonds.upload(bonds.data(), bonds.size());
+  std::vector<int2> pr(static_cast<size_t>(std::max(n_pairs, 1)));
+  for (int p = 0; p < n_pairs; ++p) {
+    FTLM_REQUIRE(pairs[p].i >= 0 && pairs[p].i < n && pairs[p].j >= 0 && pairs[p].j < n, FTLM_EINVAL,
+                 "pair site outside [0, n_sites)");
+    pr[p] = make_int2(pairs[p].i, pairs[p].j);
+  }
+  DevBuf<int2> dpairs(pr.size());
+  dpairs.upload(pr);
+
+  // 1. representatives, in pack order
+  const std::vector<uint32_t> upN = strings_numeric(n, n_up), dnN = strings_numeric(n, n_dn);
+  const int Du = static_cast<int>(upN.size()), Dd = static_cast<int>(dnN.size());
+  const int64_t dim = static_cast<int64_t>(Du) * Dd;
+  FTLM_REQUIRE(dim < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "sector too large for the orbit scan");
+  DevBuf<uint32_t> dup(upN.size()), ddn(dnN.size());
+  dup.upload(upN);
+  ddn.upload(dnN);
+  DevBuf<uint8_t> flag(static_cast<size_t>(dim));
+  hipLaunchKernelGGL(k_orb_flag, dim3(grid_for(dim, 256)), dim3(256), 0, st, dup.p, Du, ddn.p, Dd, G->L, flag.p);
+  check_launch("k_orb_flag");
+  DevBuf<int32_t> idx(static_cast<size_t>(dim));
+  DevBuf<int32_t> nsel(1);
+  {
+    hipcub::CountingInputIterator<int32_t> it(0);
+    const int ndim = static_cast<int>(dim);
+    cub_call([&](void* t, size_t& b, hipStream_t s) {
+      return hipcub::DeviceSelect::Flagged(t, b, it, flag.p, idx.p, nsel.p, ndim, s);
+    }, st);
+  }
+  int32_t nr = 0;
+  FTLM_HIP_CHECK(hipMemcpy(&nr, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+  flag.release();
+  G->nrep = nr;
+  FTLM_REQUIRE(nr < static_cast<int32_t>(kColMask), FTLM_EUNSUPPORTED, "more than 2^24 orbits");
+  G->pack.alloc(std::max<int64_t>(nr, 1));
+  G->stab.alloc(std::max<int64_t>(nr, 1));
+  G->eta.alloc(std::max<int64_t>(nr, 1));
+  G->diag.alloc(std::max<int64_t>(nr, 1));
+  if (nr > 0) {
+    hipLaunchKernelGGL(k_orb_reps, dim3(grid_for(nr, 256)), dim3(256), 0, st, idx.p, static_cast<int64_t>(nr),
+                       dup.p, ddn.p, Dd, G->L, dpairs.p, n_pairs, U, V, G->pack.p, G->stab.p, G->eta.p, G->diag.p);
+    check_launch("k_orb_reps");
+  }
+  idx.release();
+
+  // 2. push entries, stably sorted by target orbit
+  DevBuf<int32_t> cnt(std::max<int64_t>(nr, 1)), off(static_cast<size_t>(nr) + 1);
+  if (nr > 0) {
+    hipLaunchKernelGGL(k_orb_count, dim3(grid_for(nr, 256)), dim3(256), 0, st, G->pack.p, static_cast<int64_t>(nr),
+                       dbonds.p, G->nbond, cnt.p);
+    check_launch("k_orb_count");
+  }
+  std::vector<int32_t> hc(static_cast<size_t>(nr));
+  if (nr > 0) FTLM_HIP_CHECK(hipMemcpy(hc.data(), cnt.p, sizeof(int32_t) * nr, hipMemcpyDeviceToHost));
+  std::vector<int32_t> ho(static_cast<size_t>(nr) + 1, 0);
+  int64_t acc = 0;
+  for (int32_t g = 0; g < nr; ++g) {
+    ho[g] = static_cast<int32_t>(acc);
+    acc += hc[g];
+    FTLM_REQUIRE(acc < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "orbit hop table too large");
+  }
+  ho[nr] = static_cast<int32_t>(acc);
+  G->nnz = acc;
+  off.upload(ho);
+  DevBuf<uint32_t> k0(std::max<int64_t>(acc, 1)), k1(std::max<int64_t>(acc, 1));
+  DevBuf<uint2> v0(std::max<int64_t>(acc, 1)), v1(std::max<int64_t>(acc, 1));
+  DevBuf<int> bad(1);
+  FTLM_HIP_CHECK(hipMemset(bad.p, 0, sizeof(int)));
+  if (nr > 0) {
+    hipLaunchKernelGGL(k_orb_fill, dim3(grid_for(nr, 256)), dim3(256), 0, st, G->pack.p, static_cast<int64_t>(nr),
+                       dbonds.p, G->nbond, off.p, G->L, k0.p, v0.p, bad.p);
+    check_launch("k_orb_fill");
+  }
+  int hbad = 0;
+  FTLM_HIP_CHECK(hipMemcpy(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost));
+  FTLM_REQUIRE(hbad == 0, FTLM_EHIP, "orbit graph: a hop left the orbit list");
+  int end_bit = 1;
+  while ((int64_t{1} << end_bit) < std::max<int64_t>(nr, 2)) ++end_bit;
+  hipcub::DoubleBuffer<uint32_t> K(k0.p, k1.p);
+  hipcub::DoubleBuffer<uint2> Vv(v0.p, v1.p);
+  if (acc > 0) {
+    const int nn = static_cast<int>(acc);
+    cub_call([&](void* t, size_t& b, hipStream_t s) {
+      return hipcub::DeviceRadixSort::SortPairs(t, b, K, Vv, nn, 0, end_bit, s);
+    }, st);
+  }
+  G->row_off.alloc(static_cast<size_t>(nr) + 1);
+  if (acc > 0) {
+    hipLaunchKernelGGL(k_csr_rows, dim3(grid_for(acc, 256)), dim3(256), 0, st, K.Current(), acc, G->row_off.p);
+    check_launch("k_csr_rows");
+  }
+  const int32_t last = static_cast<int32_t>(acc);
+  FTLM_HIP_CHECK(hipMemcpy(G->row_off.p + nr, &last, sizeof(int32_t), hipMemcpyHostToDevice));
+  // keep the sorted values; release the rest
+  if (Vv.Current() == v0.p) {
+    G->ent = std::move(v0);
+  } else {
+    G->ent = std::move(v1);
+  }
+  FTLM_HIP_CHECK(hipDeviceSynchronize());
+  return G;
+}
+
+void build_kblock(KBlock* B, double tol) {
+  const OrbitGraph& G = *B->G;
+  const int64_t nr = G.nrep;
+  const hipStream_t st = nullptr;
+  // translation_bloch_phase (include/ftlm/symmetry/momentum_sector.hpp:57-63), host libm
+  const double pi = std::acos(-1.0);
+  for (int dy = 0; dy < G.ly; ++dy)
+    for (int dx = 0; dx < G.lx; ++dx) {
+      const double th = -2.0 * pi *
+                        (static_cast<double>(B->kx * dx) / static_cast<double>(G.lx) +
+                         static_cast<double>(B->ky * dy) / static_cast<double>(G.ly));
+      B->ph.chi[dy * G.lx + dx] = make_double2(std::cos(th), std::sin(th));
+    }
+  B->A.alloc(std::max<int64_t>(nr, 1));
+  B->map.alloc(std::max<int64_t>(nr, 1));
+  B->sq.alloc(std::max<int64_t>(nr, 1));
+  B->isq.alloc(std::max<int64_t>(nr, 1));
+  DevBuf<uint8_t> member(std::max<int64_t>(nr, 1));
+  DevBuf<int32_t> rows(std::max<int64_t>(nr, 1)), nsel(1);
+  int32_t dk = 0;
+  if (nr > 0) {
+    hipLaunchKernelGGL(k_kblock_A, dim3(grid_for(nr, 256)), dim3(256), 0, st, G.stab.p, G.eta.p, nr, G.L.nt, B->ph,
+                       tol, B->A.p, member.p);
+    check_launch("k_kblock_A");
+    hipcub::CountingInputIterator<int32_t> it(0);
+    const int nn = static_cast<int>(nr);
+    cub_call([&](void* t, size_t& b, hipStream_t s) {
+      return hipcub::DeviceSelect::Flagged(t, b, it, member.p, rows.p, nsel.p, nn, s);
+    }, st);
+    FTLM_HIP_CHECK(hipMemcpy(&dk, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(nr, 256)), dim3(256), 0, st, B->map.p, nr, -1);
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(nr, 256)), dim3(256), 0, st, B->sq.p, nr, 0.0);
+    hipLaunchKernelGGL(k_fill_f64, dim3(grid_for(nr, 256)), dim3(256), 0, st, B->isq.p, nr, 0.0);
+    check_launch("k_fill");
+  }
+  B->rows.alloc(std::max<int32_t>(dk, 1));
+  if (dk > 0) {
+    FTLM_HIP_CHECK(hipMemcpy(B->rows.p, rows.p, sizeof(int32_t) * dk, hipMemcpyDeviceToDevice));
+    hipLaunchKernelGGL(k_kblock_maps, dim3(grid_for(dk, 256)), dim3(256), 0, st, B->rows.p,
+                       static_cast<int64_t>(dk), B->A.p, B->map.p, B->sq.p, B->isq.p);
+    check_launch("k_kblock_maps");
+  }
+  FTLM_HIP_CHECK(hipDeviceSynchronize());
+  B->dim = dk;
+}
+
+namespace {
+
+template <typename T, int RB>
+void launch_kblock(const KBlock* B, const void* x, void* y, int RT, int r0, const double* xs, double2* dotp,
+                   hipStream_t st, int grid) {
+  using V2 = typename Cx<T>::v2;
+  const OrbitGraph& G = *B->G;
+  hipLaunchKernelGGL((k_kblock_apply<T, RB>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, G.row_off.p, G.ent.p,
+                     B->map.p, B->sq.p, B->isq.p, G.diag.p, G.coef.p, G.nbond, B->ph, static_cast<const V2*>(x),
+                     static_cast<V2*>(y), RT, r0, xs, dotp);
+}
+
+template <typename T>
+void launch_kblock_chunk(const KBlock* B, const void* x, void* y, int RT, int r0, int rb, const double* xs,
+                         double2* dotp, hipStream_t st, int grid) {
+  switch (rb) {
+    case 8: launch_kblock<T, 8>(B, x, y, RT, r0, xs, dotp, st, grid); break;
+    case 4: launch_kblock<T, 4>(B, x, y, RT, r0, xs, dotp, st, grid); break;
+    case 2: launch_kblock<T, 2>(B, x, y, RT, r0, xs, dotp, st, grid); break;
+    default: launch_kblock<T, 1>(B, x, y, RT, r0, xs, dotp, st, grid); break;
+  }
+}
+
+}  // namespace
+
+constexpr int kKbMaxGrid = 4096;
+
 int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
                     hipStream_t st) {
-  (void)op; (void)x; (void)y; (void)R; (void)dtype; (void)xs; (void)dotp; (void)st;
-  throw Error(FTLM_EUNSUPPORTED, "k-block operators are not available in this build");
+  const KBlock* B = static_cast<const KBlock*>(op);
+  if (B->dim == 0) return 0;
+  const int grid = static_cast<int>(std::min<int64_t>(kKbMaxGrid, (B->dim + 255) / 256));
+  std::vector<std::pair<int, int>> chunks;
+  for (int r0 = 0; r0 < R;) {
+    int rb = 1;
+    while (rb * 2 <= 8 && r0 + rb * 2 <= R) rb *= 2;
+    chunks.emplace_back(r0, rb);
+    r0 += rb;
+  }
+  for (const auto& c : chunks) {
+    if (dtype == FTLM_C64)
+      launch_kblock_chunk<float>(B, x, y, R, c.first, c.second, xs, dotp, st, grid);
+    else
+      launch_kblock_chunk<double>(B, x, y, R, c.first, c.second, xs, dotp, st, grid);
+  }
+  check_launch("k_kblock_apply");
+  return grid;
 }
 
 }  // namespace ftlm_hip
+
+using namespace ftlm_hip;
+
+namespace {
+
+void check_lattice(int Lx, int Ly) {
+  FTLM_REQUIRE(Lx >= 1 && Ly >= 1, FTLM_EINVAL, "lattice extents must be positive");
+  FTLM_REQUIRE(Lx * Ly <= kMaxOrbitSites, FTLM_EINVAL, "OrbitKBlockBasis::build: n_sites must be <= 16 (RawState bit width)");
+}
+
+const KBlock* as_kblock(const ftlm_op_t* op) {
+  FTLM_REQUIRE(op != nullptr && op->kind == kKBlock, FTLM_EINVAL, "not a k-block operator");
+  return static_cast<const KBlock*>(op);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ftlm_hip_kblock_create(int Lx, int Ly, int n_up, int n_dn, int kx, int ky, double tol, const ftlm_hop_t* hops,
+                           int n_hops, const ftlm_pair_t* pairs, int n_pairs, double U, double V,
+                           const ftlm_op_t* share, ftlm_op_t** out) {
+  return guarded([&] {
+    FTLM_REQUIRE(out != nullptr, FTLM_EINVAL, "null output handle");
+    *out = nullptr;
+    check_lattice(Lx, Ly);
+    const int n = Lx * Ly;
+    FTLM_REQUIRE(n_up >= 0 && n_up <= n && n_dn >= 0 && n_dn <= n, FTLM_EINVAL,
+                 "FockBasis: particle numbers must be in [0, n_sites]");
+    FTLM_REQUIRE(kx >= 0 && kx < Lx && ky >= 0 && ky < Ly, FTLM_EINVAL, "momentum (kx, ky) outside the lattice");
+    FTLM_REQUIRE(n_hops >= 0 && (n_hops == 0 || hops != nullptr), FTLM_EINVAL, "null hop list");
+    FTLM_REQUIRE(n_pairs >= 0 && (n_pairs == 0 || pairs != nullptr), FTLM_EINVAL, "null pair list");
+    int dev = 0;
+    FTLM_HIP_CHECK(hipGetDevice(&dev));
+    auto B = std::make_unique<KBlock>();
+    B->kind = kKBlock;
+    B->device = dev;
+    B->kx = kx;
+    B->ky = ky;
+    B->tol = tol;
+    if (share != nullptr) {
+      const KBlock* S = as_kblock(share);
+      FTLM_REQUIRE(S->G->same_model(Lx, Ly, n_up, n_dn, U, V, hops, n_hops, pairs, n_pairs), FTLM_EINVAL,
+                   "share: the operator belongs to another sector or Hamiltonian");
+      FTLM_REQUIRE(S->device == dev, FTLM_EINVAL, "share: the operator lives on another device");
+      B->G = S->G;
+    } else {
+      B->G = build_orbit_graph(Lx, Ly, n_up, n_dn, hops, n_hops, pairs, n_pairs, U, V);
+    }
+    build_kblock(B.get(), tol);
+    *out = B.release();
+  });
+}
+
+int ftlm_hip_kblock_info(const ftlm_op_t* op, int* kx, int* ky, int64_t* n_orbits, int64_t* nnz) {
+  return guarded([&] {
+    const KBlock* B = as_kblock(op);
+    if (kx) *kx = B->kx;
+    if (ky) *ky = B->ky;
+    if (n_orbits) *n_orbits = B->G->nrep;
+    if (nnz) *nnz = B->G->nnz;
+  });
+}
+
+int ftlm_hip_kblock_entries(const ftlm_op_t* op, int64_t first, int64_t count, uint32_t* rep, int* orbit_size,
+                            int* stab_size, double* A) {
+  return guarded([&] {
+    const KBlock* B = as_kblock(op);
+    FTLM_REQUIRE(first >= 0 && count >= 0 && first + count <= B->dim, FTLM_ERANGE, "entry range outside the block");
+    if (count == 0) return;
+    FTLM_HIP_CHECK(hipSetDevice(B->device));
+    const OrbitGraph& G = *B->G;
+    const size_t n = static_cast<size_t>(count);
+    DevBuf<uint32_t> dr(n);
+    DevBuf<int> dos(n), dss(n);
+    DevBuf<double2> da(n);
+    hipLaunchKernelGGL(k_kblock_entries, dim3(grid_for(count, 256)), dim3(256), 0, 0, B->rows.p, first, count,
+                       G.pack.p, G.stab.p, B->A.p, G.L.nt, dr.p, dos.p, dss.p, da.p);
+    check_launch("k_kblock_entries");
+    if (rep) FTLM_HIP_CHECK(hipMemcpy(rep, dr.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (orbit_size) FTLM_HIP_CHECK(hipMemcpy(orbit_size, dos.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (stab_size) FTLM_HIP_CHECK(hipMemcpy(stab_size, dss.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (A) FTLM_HIP_CHECK(hipMemcpy(A, da.p, sizeof(double2) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int ftlm_hip_kblock_index_of_rep(const ftlm_op_t* op, const uint32_t* reps, int64_t count, int64_t* out) {
+  return guarded([&] {
+    const KBlock* B = as_kblock(op);
+    FTLM_REQUIRE(count >= 0 && (count == 0 || (reps && out)), FTLM_EINVAL, "null argument");
+    if (count == 0) return;
+    FTLM_HIP_CHECK(hipSetDevice(B->device));
+    DevBuf<uint32_t> dr(static_cast<size_t>(count));
+    DevBuf<int64_t> dout(static_cast<size_t>(count));
+    dr.upload(reps, static_cast<size_t>(count));
+    hipLaunchKernelGGL(k_kblock_index_of_rep, dim3(grid_for(count, 256)), dim3(256), 0, 0, dr.p, count, B->G->pack.p,
+                       B->G->nrep, B->map.p, dout.p);
+    check_launch("k_kblock_index_of_rep");
+    FTLM_HIP_CHECK(hipMemcpy(out, dout.p, sizeof(int64_t) * count, hipMemcpyDeviceToHost));
+  });
+}
+
+int ftlm_hip_kblock_map(const ftlm_op_t* op, const uint32_t* states, int64_t count, int64_t* index, int* dx, int* dy,
+                        double* eta) {
+  return guarded([&] {
+    const KBlock* B = as_kblock(op);
+    FTLM_REQUIRE(count >= 0 && (count == 0 || (states && index && dx && dy && eta)), FTLM_EINVAL, "null argument");
+    if (count == 0) return;
+    FTLM_HIP_CHECK(hipSetDevice(B->device));
+    const size_t n = static_cast<size_t>(count);
+    DevBuf<uint32_t> ds(n);
+    DevBuf<int64_t> di(n);
+    DevBuf<int> ddx(n), ddy(n);
+    DevBuf<double> de(n);
+    ds.upload(states, n);
+    hipLaunchKernelGGL(k_kblock_map_states, dim3(grid_for(count, 256)), dim3(256), 0, 0, ds.p, count, B->G->pack.p,
+                       B->G->nrep, B->map.p, B->G->L, di.p, ddx.p, ddy.p, de.p);
+    check_launch("k_kblock_map_states");
+    FTLM_HIP_CHECK(hipMemcpy(index, di.p, sizeof(int64_t) * n, hipMemcpyDeviceToHost));
+    FTLM_HIP_CHECK(hipMemcpy(dx, ddx.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    FTLM_HIP_CHECK(hipMemcpy(dy, ddy.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    FTLM_HIP_CHECK(hipMemcpy(eta, de.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int ftlm_hip_translate_states(int Lx, int Ly, const uint32_t* states, int64_t count, int dx, int dy, uint32_t* out,
+                              double* sign) {
+  return guarded([&] {
+    check_lattice(Lx, Ly);
+    FTLM_REQUIRE(count >= 0 && (count == 0 || states != nullptr), FTLM_EINVAL, "null argument");
+    if (count == 0) return;
+    const int tx = ((dx % Lx) + Lx) % Lx, ty = ((dy % Ly) + Ly) % Ly;
+    const LatDev L = make_lat(Lx, Ly);
+    const size_t n = static_cast<size_t>(count);
+    DevBuf<uint32_t> ds(n), dout(n);
+    DevBuf<double> dsg(n);
+    ds.upload(states, n);
+    hipLaunchKernelGGL(k_translate_states, dim3(grid_for(count, 256)), dim3(256), 0, 0, ds.p, count, L,
+                       ty * Lx + tx, dout.p, dsg.p);
+    check_launch("k_translate_states");
+    if (out) FTLM_HIP_CHECK(hipMemcpy(out, dout.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (sign) FTLM_HIP_CHECK(hipMemcpy(sign, dsg.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int ftlm_hip_canonical_states(int Lx, int Ly, const uint32_t* states, int64_t count, uint32_t* canon,
+                              int* stab_size) {
+  return guarded([&] {
+    check_lattice(Lx, Ly);
+    FTLM_REQUIRE(count >= 0 && (count == 0 || states != nullptr), FTLM_EINVAL, "null argument");
+    if (count == 0) return;
+    const LatDev L = make_lat(Lx, Ly);
+    const size_t n = static_cast<size_t>(count);
+    DevBuf<uint32_t> ds(n), dc(n);
+    DevBuf<int> dss(n);
+    ds.upload(states, n);
+    hipLaunchKernelGGL(k_canonical_states, dim3(grid_for(count, 256)), dim3(256), 0, 0, ds.p, count, L, dc.p, dss.p);
+    check_launch("k_canonical_states");
+    if (canon) FTLM_HIP_CHECK(hipMemcpy(canon, dc.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost));
+    if (stab_size) FTLM_HIP_CHECK(hipMemcpy(stab_size, dss.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int ftlm_hip_bloch_phase(int Lx, int Ly, int kx, int ky, int dx, int dy, double* re, double* im) {
+  return guarded([&] {
+    FTLM_REQUIRE(Lx >= 1 && Ly >= 1 && re && im, FTLM_EINVAL, "bad argument");
+    const double pi = std::acos(-1.0);
+    const double th = -2.0 * pi *
+                      (static_cast<double>(kx * dx) / static_cast<double>(Lx) +
+                       static_cast<double>(ky * dy) / static_cast<double>(Ly));
+    *re = std::cos(th);
+    *im = std::sin(th);
+  });
+}
+
+}  // extern "C"

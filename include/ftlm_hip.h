@@ -193,6 +193,55 @@ int ftlm_hip_krylov_result(ftlm_krylov_t* k, double* alpha, double* beta, int* u
 int ftlm_hip_krylov_bytes(const ftlm_krylov_t* k, int64_t* bytes);
 int ftlm_hip_krylov_destroy(ftlm_krylov_t* k);
 
+/* ------------------------------------------------------------------ translations and k-blocks */
+/* States are packed RawStates, (up << 16) | dn (include/ftlm/symmetry/packed_raw_state.hpp:12-14);
+ * lattices have at most 16 sites; translation t = (dx, dy). */
+
+/* translate_occupation_bits_rect (include/ftlm/symmetry/occup_bits_translate.hpp:10-25) on both
+ * strings and fermionic_translation_sign_rect (include/ftlm/symmetry/fermionic_translation.hpp:32-54),
+ * batched on the device; out and/or sign may be NULL. */
+int ftlm_hip_translate_states(int Lx, int Ly, const uint32_t* states, int64_t count, int dx, int dy, uint32_t* out,
+                              double* sign);
+
+/* canonical_raw_state_rect (minimal pack over the orbit) and the size of
+ * translation_stabilizer_rect (include/ftlm/symmetry/orbit.hpp:93-155), batched on the device. */
+int ftlm_hip_canonical_states(int Lx, int Ly, const uint32_t* states, int64_t count, uint32_t* canon,
+                              int* stab_size);
+
+/* translation_bloch_phase(K, dx, dy) = exp(-2 pi i (kx dx / Lx + ky dy / Ly))
+ * (include/ftlm/symmetry/momentum_sector.hpp:57-63). */
+int ftlm_hip_bloch_phase(int Lx, int Ly, int kx, int ky, int dx, int dy, double* re, double* im);
+
+/* OrbitKBlockBasis::build(FockBasis(Lx*Ly, n_up, n_dn), MomentumSector{kx, ky, Lx, Ly}, tol)
+ * (src/orbit_k_block_basis.cpp:70-117) plus the operator data of apply_orbit_k_block
+ * (src/orbit_k_block_basis.cpp:151-220): hops and pairs as build_hubbard_geometry returns them.
+ * The orbit graph (representatives and the K-independent pull-form hop table) is built on the
+ * device; pass another k-block of the same sector and Hamiltonian as `share` (or NULL) to reuse
+ * its graph.  The result is an operator for ftlm_hip_apply / _lanczos / _ftlm_* of dimension
+ * OrbitKBlockBasis::dim(). */
+int ftlm_hip_kblock_create(int Lx, int Ly, int n_up, int n_dn, int kx, int ky, double tol, const ftlm_hop_t* hops,
+                           int n_hops, const ftlm_pair_t* pairs, int n_pairs, double U, double V,
+                           const ftlm_op_t* share, ftlm_op_t** out);
+
+/* Momentum, number of translation orbits of the sector and stored hop terms of a k-block. */
+int ftlm_hip_kblock_info(const ftlm_op_t* op, int* kx, int* ky, int64_t* n_orbits, int64_t* nnz);
+
+/* OrbitKBlockBasis::entries() rows [first, first+count): rep (packed), orbit_size,
+ * stabilizer_size, A_k as (re, im) pairs (include/ftlm/symmetry/orbit_k_block_basis.hpp:35-42);
+ * any output may be NULL. */
+int ftlm_hip_kblock_entries(const ftlm_op_t* op, int64_t first, int64_t count, uint32_t* rep, int* orbit_size,
+                            int* stab_size, double* A);
+
+/* OrbitKBlockBasis::reduced_index_of_rep (src/orbit_k_block_basis.cpp:119-125), batched; -1 when
+ * the representative is not a row of the block. */
+int ftlm_hip_kblock_index_of_rep(const ftlm_op_t* op, const uint32_t* reps, int64_t count, int64_t* out);
+
+/* OrbitKBlockBasis::map_raw_to_rep (src/orbit_k_block_basis.cpp:127-149), batched: row, (dx, dy)
+ * of the first translation (dx outer, dy inner) with T rep = s, and eta; index -1 (dx = dy = 0,
+ * eta = 0) where the reference returns false. */
+int ftlm_hip_kblock_map(const ftlm_op_t* op, const uint32_t* states, int64_t count, int64_t* index, int* dx, int* dy,
+                        double* eta);
+
 #ifdef __cplusplus
 }
 #endif

@@ -975,6 +975,11 @@ double orc_translation_sign(uint32_t u, uint32_t d, int lx, int ly, int dx, int 
 
 uint32_t orc_canonical(uint32_t u, uint32_t d, int lx, int ly) { return canonical(u, d, lx, ly); }
 
+// Packed image of (u, d) under translation (dx, dy) (occup_bits_translate.hpp:10-25).
+uint32_t orc_translate(uint32_t u, uint32_t d, int lx, int ly, int dx, int dy) {
+  return pack(tr_bits(u, lx, ly, dx, dy), tr_bits(d, lx, ly, dx, dy));
+}
+
 // Lanczos on the orbit block with ftlm_log_partition_complex semantics (one beta).
 double orc_orbit_ftlm_log_partition(const double* params8, int nu, int nd, int kx, int ky, double bt,
                                     int R, int M, unsigned seed) {

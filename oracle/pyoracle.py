@@ -171,6 +171,8 @@ class Oracle(_Common):
         L.orc_translation_sign.restype = C.c_double
         L.orc_canonical.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_int]
         L.orc_canonical.restype = C.c_uint32
+        L.orc_translate.argtypes = [C.c_uint32, C.c_uint32] + [C.c_int] * 4
+        L.orc_translate.restype = C.c_uint32
         L.orc_orbit_ftlm_log_partition.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                                    C.c_int, C.c_int, C.c_uint]
         L.orc_orbit_ftlm_log_partition.restype = C.c_double
@@ -263,6 +265,9 @@ class Oracle(_Common):
 
     def canonical(self, u, d, lx, ly):
         return self.lib.orc_canonical(u, d, lx, ly)
+
+    def translate(self, u, d, lx, ly, dx, dy):
+        return self.lib.orc_translate(u, d, lx, ly, dx, dy)
 
     def exact_log_partition(self, H, beta):
         H = np.ascontiguousarray(H, np.complex128)

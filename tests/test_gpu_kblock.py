@@ -1,0 +1,202 @@
+"""GPU parity of the translation primitives and the orbit k-blocks (through the C ABI) against the
+reference's golden vectors, the CPU oracle and identities the symmetry reduction must satisfy.
+
+Tolerances: packed states, orbit/stabiliser sizes, row indices, translations and signs bit-exact;
+A_k within 1e-14; complex128 H_K x within 1e-12 relative; block spectra within 1e-9 of the full
+sector's; lnZ within 1e-9 absolute.
+"""
+
+import sys
+
+import numpy as np
+import pytest
+
+from pyoracle import params8
+
+pytestmark = pytest.mark.gpu
+
+api = pytest.importorskip("ftlm_extended_hubbard_amd.api")
+from ftlm_extended_hubbard_amd import _lib as L  # noqa: E402
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    if api.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def model(lx, ly, phi=0.0):
+    # the orbit fixtures' Hamiltonian (tests/golden/make_golden.py)
+    return api.HubbardParams(lx, ly, 1.0, -0.25, 8.0, 1.0, phi, 0.0)
+
+
+def kblock(lx, ly, nu, nd, kx, ky, share=None, phi=0.0):
+    return api.KBlockOperator.from_params(model(lx, ly, phi), nu, nd, api.MomentumSector(kx, ky, lx, ly),
+                                          share=share)
+
+
+def sector_states(oracle, lx, ly, nu, nd):
+    up, dn = oracle.basis_masks(lx * ly, nu, nd)
+    return api.pack_raw_state(up, dn)
+
+
+def test_translate_and_canonical_vs_oracle(oracle):
+    rng = np.random.default_rng(5)
+    for (lx, ly, nu, nd) in [(4, 4, 8, 8), (4, 2, 3, 2), (3, 2, 3, 3), (2, 2, 2, 1), (4, 3, 6, 5)]:
+        say("translate", lx, ly, nu, nd)
+        st = sector_states(oracle, lx, ly, nu, nd)
+        st = st[rng.choice(len(st), size=min(len(st), 300), replace=False)]
+        for dx in range(lx):
+            for dy in range(ly):
+                out, sg = api.translate_states(lx, ly, st, dx, dy)
+                want = [oracle.translate(int(s) >> 16, int(s) & 0xFFFF, lx, ly, dx, dy) for s in st]
+                wsg = [oracle.translation_sign(int(s) >> 16, int(s) & 0xFFFF, lx, ly, dx, dy) for s in st]
+                assert np.array_equal(out, np.array(want, np.uint32)), (lx, ly, dx, dy)
+                assert np.array_equal(sg, np.array(wsg)), (lx, ly, dx, dy)
+        can, ss = api.canonical_states(lx, ly, st)
+        want = [oracle.canonical(int(s) >> 16, int(s) & 0xFFFF, lx, ly) for s in st]
+        assert np.array_equal(can, np.array(want, np.uint32))
+        # |orbit| * |stabiliser| = |G|: the stabiliser counts the translations fixing the state
+        fixed = np.zeros(len(st), np.int32)
+        for dx in range(lx):
+            for dy in range(ly):
+                fixed += (api.translate_states(lx, ly, st, dx, dy)[0] == st)
+        assert np.array_equal(ss, fixed)
+
+
+def test_bloch_phase():
+    K = api.MomentumSector(1, 2, 4, 4)
+    assert abs(api.translation_bloch_phase(K, 0, 0) - 1.0) == 0.0
+    z = api.translation_bloch_phase(K, 1, 1)
+    assert abs(z - np.exp(-2j * np.pi * (1 / 4 + 2 / 4))) < 1e-15
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd", [(2, 2, 2, 2), (3, 2, 3, 3), (4, 2, 4, 4), (4, 2, 2, 3)])
+def test_kblock_basis_golden(golden, lx, ly, nu, nd):
+    say("kblock basis", lx, ly, nu, nd)
+    dims = golden[f"orbit/{lx}x{ly}/{nu}_{nd}/dims"]
+    first = None
+    for ky in range(ly):
+        for kx in range(lx):
+            B = kblock(lx, ly, nu, nd, kx, ky, share=first)
+            first = first or B
+            assert B.dim() == dims[ky, kx], (kx, ky)
+            key = f"orbit/{lx}x{ly}/{nu}_{nd}/k{kx}{ky}"
+            if key + "/reps" not in golden.files:
+                continue
+            e = B.entries()
+            assert np.array_equal(e.rep, golden[key + "/reps"])
+            assert np.array_equal(e.orbit_size, golden[key + "/orbit_size"])
+            assert np.array_equal(e.stabilizer_size, golden[key + "/stab_size"])
+            assert np.max(np.abs(e.A_k - golden[key + "/A"])) < 1e-14
+            assert np.array_equal(B.reduced_index_of_rep(e.rep), np.arange(B.dim()))
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd,kx,ky", [(3, 2, 3, 3, 1, 0), (2, 2, 2, 2, 0, 0), (4, 2, 4, 4, 2, 1),
+                                               (4, 2, 2, 3, 1, 0)])
+def test_kblock_map_apply_lnz_golden(golden, oracle, lx, ly, nu, nd, kx, ky):
+    say("kblock apply", lx, ly, nu, nd, kx, ky)
+    key = f"orbit_apply/{lx}x{ly}/{nu}_{nd}/k{kx}{ky}"
+    B = kblock(lx, ly, nu, nd, kx, ky)
+    st = sector_states(oracle, lx, ly, nu, nd)
+    idx, dx, dy, eta = B.map_raw_to_rep(st)
+    want = golden[key + "/map"]
+    assert np.array_equal(idx, want[:, 0])
+    ok = want[:, 0] >= 0
+    assert np.array_equal(dx[ok], want[ok, 1]) and np.array_equal(dy[ok], want[ok, 2])
+    assert np.array_equal(eta[ok].astype(np.int32), want[ok, 3])
+    y = B.apply(golden[key + "/x"])
+    assert rel(y, golden[key + "/y"]) < 1e-12
+    par = api.FtlmParams(n_random=2, lanczos_steps=12, seed=3)
+    lnz = api.ftlm_log_partition_complex(B, 1.5, par)
+    w = float(golden[key + "/lnZ_b1.5_R2_M12_s3"][0])
+    assert abs(lnz - w) < 1e-9 * max(1.0, abs(w)), (lnz, w)
+
+
+def test_kblock_apply_vs_oracle_flux(oracle):
+    """Nonzero flux breaks translation symmetry; the block operator is still the reference's
+    push-form matrix (not Hermitian), which the pull table must reproduce."""
+    lx, ly, nu, nd, kx, ky = 4, 2, 2, 3, 1, 1
+    say("kblock flux", lx, ly, nu, nd, kx, ky)
+    B = kblock(lx, ly, nu, nd, kx, ky, phi=0.3)
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal(B.dim()) + 1j * rng.standard_normal(B.dim())
+    y = B.apply(x)
+    want = oracle.orbit_apply(params8(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.3, 0.0), nu, nd, kx, ky, x)
+    assert rel(y, want) < 1e-12
+
+
+def test_kblock_block_vectors_and_c64(oracle):
+    import torch
+
+    lx, ly, nu, nd = 4, 2, 4, 4
+    say("kblock R/c64", lx, ly, nu, nd)
+    B = kblock(lx, ly, nu, nd, 1, 0)
+    d = B.dim()
+    rng = np.random.default_rng(2)
+    for R in (1, 3, 8, 16):
+        X = rng.standard_normal((d, R)) + 1j * rng.standard_normal((d, R))
+        xd = torch.tensor(X, dtype=torch.complex128, device="cuda").contiguous()
+        yd = torch.empty_like(xd)
+        B.apply_device(xd, yd, R=R, dtype=L.FTLM_C128)
+        Y = yd.cpu().numpy()
+        for r in range(R):
+            assert np.array_equal(Y[:, r], B.apply(X[:, r])), (R, r)
+        x64 = xd.to(torch.complex64)
+        y64 = torch.empty_like(x64)
+        B.apply_device(x64, y64, R=R, dtype=L.FTLM_C64)
+        assert rel(y64.cpu().numpy().astype(np.complex128), Y) < 1e-5
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd", [(3, 2, 3, 3), (4, 2, 2, 3)])
+def test_kblock_spectra_union_equals_full_sector(oracle, lx, ly, nu, nd):
+    """At zero flux H commutes with translations: the block spectra together are the sector's."""
+    say("kblock spectra", lx, ly, nu, nd)
+    p = model(lx, ly)
+    full = np.linalg.eigvalsh(oracle.dense_hamiltonian(params8(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0), nu, nd))
+    ev, first = [], None
+    for ky in range(ly):
+        for kx in range(lx):
+            B = api.KBlockOperator.from_params(p, nu, nd, api.MomentumSector(kx, ky, lx, ly), share=first)
+            first = first or B
+            d = B.dim()
+            H = np.zeros((d, d), np.complex128)
+            for j in range(d):
+                e = np.zeros(d, np.complex128)
+                e[j] = 1.0
+                H[:, j] = B.apply(e)
+            assert np.max(np.abs(H - H.conj().T)) < 1e-12
+            ev.extend(np.linalg.eigvalsh(H))
+    ev = np.sort(np.array(ev))
+    assert len(ev) == len(full)
+    assert np.max(np.abs(ev - full)) < 1e-9
+
+
+def test_kblock_dims_sum_4x4():
+    """Sum over the 16 momenta of the block dimensions is the sector dimension (every orbit
+    survives at |G|/|Stab| momenta), at 4x4 (4,4) (dim 3,312,400)."""
+    say("kblock dims 4x4 (4,4)")
+    p = model(4, 4)
+    tot, first = 0, None
+    for ky in range(4):
+        for kx in range(4):
+            B = api.KBlockOperator.from_params(p, 4, 4, api.MomentumSector(kx, ky, 4, 4), share=first)
+            first = first or B
+            tot += B.dim()
+            if (kx, ky) == (1, 2):
+                # Hermiticity of a large block on random vectors: <x, H y> = <H x, y>
+                rng = np.random.default_rng(4)
+                x = rng.standard_normal(B.dim()) + 1j * rng.standard_normal(B.dim())
+                y = rng.standard_normal(B.dim()) + 1j * rng.standard_normal(B.dim())
+                a, b = np.vdot(x, B.apply(y)), np.vdot(B.apply(x), y)
+                assert abs(a - b) < 1e-10 * abs(a)
+    assert tot == 1820 * 1820

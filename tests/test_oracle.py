@@ -345,3 +345,27 @@ def test_apply_rows_equals_full_push(oracle):
     rows = rng.integers(0, d, 300)
     got = oracle.apply_rows(p8, 3, 2, x, rows)
     assert np.array_equal(got.view(np.float64), y[rows].view(np.float64))
+
+
+def test_oracle_translate_group_action(oracle):
+    """orc_translate is a Z_lx x Z_ly action: composition adds shifts, full cycles are the
+    identity, occupation counts are kept, and the canonical state is the orbit's minimum."""
+    rng = np.random.default_rng(3)
+    for (lx, ly) in [(4, 4), (4, 2), (3, 2)]:
+        n = lx * ly
+        for _ in range(40):
+            u = int(rng.integers(0, 1 << n))
+            d = int(rng.integers(0, 1 << n))
+            base = (u << 16) | d
+            assert oracle.translate(u, d, lx, ly, lx, 0) == base
+            assert oracle.translate(u, d, lx, ly, 0, ly) == base
+            imgs = []
+            for dx in range(lx):
+                for dy in range(ly):
+                    t = oracle.translate(u, d, lx, ly, dx, dy)
+                    assert bin(t >> 16).count("1") == bin(u).count("1")
+                    assert bin(t & 0xFFFF).count("1") == bin(d).count("1")
+                    t2 = oracle.translate(t >> 16, t & 0xFFFF, lx, ly, 1, 2)
+                    assert t2 == oracle.translate(u, d, lx, ly, dx + 1, dy + 2)
+                    imgs.append(t)
+            assert oracle.canonical(u, d, lx, ly) == min(imgs)
