@@ -6,6 +6,11 @@ the gfx950 library libftlm_hip.so.  Unlike the reference there is no silent fall
 hot-path symbol without the built library raises.
 """
 
+try:  # torch first: its bundled HIP runtime must be the one every library of this package binds to
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 from . import _lib
 from .api import (
     FockBasis,

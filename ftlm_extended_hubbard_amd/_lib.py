@@ -44,6 +44,9 @@ class FtlmOpts(C.Structure):
 
 
 # C struct layout (ftlm_hop_t is 32 bytes: three ints, 4 bytes of padding, two doubles)
+# void (*)(const double* x, double* y, void* ctx) -- ftlm_host_apply_fn
+HOST_APPLY_FN = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+
 HOP_DTYPE = np.dtype([("from", np.int32), ("to", np.int32), ("spin", np.int32), ("re", np.float64),
                       ("im", np.float64)], align=True)
 PAIR_DTYPE = np.dtype([("i", np.int32), ("j", np.int32)], align=True)
@@ -95,6 +98,7 @@ def lib() -> C.CDLL:
     L.ftlm_hip_krylov_result.argtypes = [vp, vp, vp, vp, vp]
     L.ftlm_hip_krylov_bytes.argtypes = [vp, i64p]
     L.ftlm_hip_krylov_destroy.argtypes = [vp]
+    L.ftlm_hip_callback_op_create.argtypes = [C.c_int64, HOST_APPLY_FN, vp, C.POINTER(vp)]
     L.ftlm_hip_translate_states.argtypes = [C.c_int, C.c_int, vp, C.c_int64, C.c_int, C.c_int, vp, vp]
     L.ftlm_hip_canonical_states.argtypes = [C.c_int, C.c_int, vp, C.c_int64, vp, vp]
     L.ftlm_hip_bloch_phase.argtypes = [C.c_int] * 6 + [dp, dp]

@@ -1,9 +1,13 @@
-"""Build libftlm_hip.so (gfx950) in-tree with hipcc.
+"""Build the package's native libraries in-tree.
 
     python -m ftlm_extended_hubbard_amd.build
 
-Every HIP source under csrc/ is compiled with hipcc for gfx950 only, with packed-FP32 VALU
-instructions disabled (-packed-fp32-ops), and linked into ftlm_extended_hubbard_amd/libftlm_hip.so.
+- libftlm_hip.so: every HIP source under csrc/, compiled with hipcc for gfx950 only, with
+  packed-FP32 VALU instructions disabled (-packed-fp32-ops); exports the C ABI of
+  include/ftlm_hip.h.
+- libftlm.so: the reference's C++ API (csrc/host/include/ftlm/*.hpp) over that C ABI.
+- _ftlm_core*.so: the pybind11 module over libftlm.so.
+- tests/cpp/_bin/*: the C++ API tests (tests/cpp/*.cpp).
 """
 
 from __future__ import annotations
@@ -56,10 +60,59 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+HOST = os.path.join(CSRC, "host")
+HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wextra", f"-I{os.path.join(ROOT, 'include')}",
+              f"-I{os.path.join(HOST, 'include')}"]
+CXX = os.environ.get("CXX", "g++")
+
+
+def _run(cmd: list[str], what: str) -> None:
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+
+
+def build_host(verbose: bool = False) -> None:
+    """libftlm.so, the _ftlm_core extension and the C++ test programs (needs libftlm_hip.so)."""
+    lib = os.path.join(PKG, "libftlm.so")
+    _run([CXX, *HOST_FLAGS, "-shared", os.path.join(HOST, "ftlm_host.cpp"), "-o", lib + ".tmp", f"-L{PKG}",
+          "-lftlm_hip", "-Wl,-rpath,$ORIGIN"], "libftlm.so")
+    os.replace(lib + ".tmp", lib)
+    import sysconfig
+
+    try:
+        import pybind11
+
+        suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+        mod = os.path.join(PKG, "_ftlm_core" + suffix)
+        inc = [f"-I{pybind11.get_include()}", f"-I{sysconfig.get_paths()['include']}"]
+        _run([CXX, *HOST_FLAGS, *inc, "-shared", os.path.join(HOST, "bindings.cpp"), "-o", mod + ".tmp", f"-L{PKG}",
+              "-lftlm", "-lftlm_hip", "-Wl,-rpath,$ORIGIN"], "_ftlm_core")
+        os.replace(mod + ".tmp", mod)
+    except ImportError:
+        if verbose:
+            print("pybind11 not importable: _ftlm_core not built")
+    tdir = os.path.join(ROOT, "tests", "cpp")
+    bdir = os.path.join(tdir, "_bin")
+    os.makedirs(bdir, exist_ok=True)
+    srcs = sorted(glob.glob(os.path.join(tdir, "test_*.cpp")))
+
+    def one(src: str) -> None:
+        out = os.path.join(bdir, os.path.splitext(os.path.basename(src))[0])
+        _run([CXX, *HOST_FLAGS, src, "-o", out, f"-L{PKG}", "-lftlm", "-lftlm_hip",
+              "-Wl,-rpath,$ORIGIN/../../../ftlm_extended_hubbard_amd"], os.path.basename(src))
+
+    with cf.ThreadPoolExecutor(max_workers=max(1, min(4, len(srcs)))) as ex:
+        list(ex.map(one, srcs))
+    if verbose:
+        print("built", lib, "and", len(srcs), "C++ tests")
+
+
 def clean() -> None:
     shutil.rmtree(BUILD, ignore_errors=True)
 
 
 if __name__ == "__main__":
     build(verbose=True)
+    build_host(verbose=True)
     sys.exit(0)

@@ -115,7 +115,7 @@ Binom host_binomials();
 int64_t binom(int a, int b);
 
 // ------------------------------------------------------------------ operator (sector / k-block)
-enum OpKind { kSector = 0, kKBlock = 1 };
+enum OpKind { kSector = 0, kKBlock = 1, kCallback = 2 };
 
 // Device view of a full (N_up, N_dn) sector with its per-spin pull-hop tables.
 // Hop entry (uint32): bits 0..19 target string rank, 20..30 bond id, 31 own-spin JW parity.

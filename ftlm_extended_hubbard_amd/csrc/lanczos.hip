@@ -544,13 +544,14 @@ int ftlm_hip_lanczos_extrema(const ftlm_op_t* op, int max_steps, unsigned seed, 
   return FTLM_OK;
 }
 
+constexpr int kMaxSamplesPerBlock = 32;  // recurrences of one Krylov block (R of ftlm_hip_apply)
+
 int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, double* ritz, double* weight,
                        int* used, double* alpha_out, double* beta_out, void* stream) {
   return guarded([&] {
     check_opts(opts);
     FTLM_REQUIRE(ops != nullptr && n_ops >= 1 && ritz && weight && used, FTLM_EINVAL, "null argument");
     const int R = std::max(1, opts->R), M = std::max(1, opts->M);
-    FTLM_REQUIRE(R <= 32, FTLM_EUNSUPPORTED, "at most 32 samples per operator block");
     hipStream_t st = static_cast<hipStream_t>(stream);
     for (int io = 0; io < n_ops; ++io) {
       const ftlm_op* op0 = ops[io];
@@ -585,42 +586,49 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
         }
         continue;
       }
-      Krylov K;
-      K.init(op, R, Mop, opts->dtype);
-      K.force_real = opts->real_variant != 0;
-      std::vector<unsigned> seeds(static_cast<size_t>(R));
-      for (int r = 0; r < R; ++r) seeds[r] = opts->seed + static_cast<unsigned>(r) * 100003u;
-      fill_start(K, opts->start, seeds, nullptr, st);
-      K.run(0, Mop, st);
-      // quadrature per sample: alpha/beta are [Mop][R] (stride R)
-      DevBuf<int> nb(static_cast<size_t>(R));
-      std::vector<int> hu(static_cast<size_t>(R)), ha(static_cast<size_t>(R));
-      FTLM_HIP_CHECK(hipMemcpyAsync(hu.data(), K.used.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
-      FTLM_HIP_CHECK(hipMemcpyAsync(ha.data(), K.alive.p, sizeof(int) * R, hipMemcpyDeviceToHost, st));
-      FTLM_HIP_CHECK(hipStreamSynchronize(st));
-      std::vector<int> hnb(static_cast<size_t>(R));
-      for (int r = 0; r < R; ++r) hnb[r] = ha[r] ? hu[r] : hu[r] - 1;
-      nb.upload(hnb, st);
-      DevBuf<double> dr(static_cast<size_t>(R) * M), dw(static_cast<size_t>(R) * M);
-      FTLM_HIP_CHECK(hipMemsetAsync(dr.p, 0, dr.n * sizeof(double), st));
-      FTLM_HIP_CHECK(hipMemsetAsync(dw.p, 0, dw.n * sizeof(double), st));
-      quadrature_device(K.alpha.p, K.beta.p, K.used.p, nb.p, R, R, Mop, opts->quad, 0, 1e-14, 1, dr.p, dw.p, M, st);
-      FTLM_HIP_CHECK(hipMemcpyAsync(rz, dr.p, sizeof(double) * R * M, hipMemcpyDeviceToHost, st));
-      FTLM_HIP_CHECK(hipMemcpyAsync(wt, dw.p, sizeof(double) * R * M, hipMemcpyDeviceToHost, st));
-      std::copy(hu.begin(), hu.end(), us);
-      if (alpha_out || beta_out) {
-        std::vector<double> a(static_cast<size_t>(Mop) * R), b(static_cast<size_t>(Mop) * R);
-        FTLM_HIP_CHECK(hipMemcpyAsync(a.data(), K.alpha.p, a.size() * sizeof(double), hipMemcpyDeviceToHost, st));
-        FTLM_HIP_CHECK(hipMemcpyAsync(b.data(), K.beta.p, b.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+      // samples run in groups of at most kMaxSamplesPerBlock recurrences; sample r keeps its seed
+      for (int g0 = 0; g0 < R; g0 += kMaxSamplesPerBlock) {
+        const int Rg = std::min(kMaxSamplesPerBlock, R - g0);
+        Krylov K;
+        K.init(op, Rg, Mop, opts->dtype);
+        K.force_real = opts->real_variant != 0;
+        std::vector<unsigned> seeds(static_cast<size_t>(Rg));
+        for (int r = 0; r < Rg; ++r) seeds[r] = opts->seed + static_cast<unsigned>(g0 + r) * 100003u;
+        fill_start(K, opts->start, seeds, nullptr, st);
+        K.run(0, Mop, st);
+        // quadrature per sample: alpha/beta are [Mop][Rg] (stride Rg)
+        DevBuf<int> nb(static_cast<size_t>(Rg));
+        std::vector<int> hu(static_cast<size_t>(Rg)), ha(static_cast<size_t>(Rg));
+        FTLM_HIP_CHECK(hipMemcpyAsync(hu.data(), K.used.p, sizeof(int) * Rg, hipMemcpyDeviceToHost, st));
+        FTLM_HIP_CHECK(hipMemcpyAsync(ha.data(), K.alive.p, sizeof(int) * Rg, hipMemcpyDeviceToHost, st));
         FTLM_HIP_CHECK(hipStreamSynchronize(st));
-        for (int r = 0; r < R; ++r)
-          for (int k = 0; k < M; ++k) {
-            const int64_t o = (static_cast<int64_t>(io) * R + r) * M + k;
-            if (alpha_out) alpha_out[o] = k < Mop ? a[static_cast<size_t>(k) * R + r] : 0.0;
-            if (beta_out) beta_out[o] = k < Mop ? b[static_cast<size_t>(k) * R + r] : 0.0;
-          }
+        std::vector<int> hnb(static_cast<size_t>(Rg));
+        for (int r = 0; r < Rg; ++r) hnb[r] = ha[r] ? hu[r] : hu[r] - 1;
+        nb.upload(hnb, st);
+        DevBuf<double> dr(static_cast<size_t>(Rg) * M), dw(static_cast<size_t>(Rg) * M);
+        FTLM_HIP_CHECK(hipMemsetAsync(dr.p, 0, dr.n * sizeof(double), st));
+        FTLM_HIP_CHECK(hipMemsetAsync(dw.p, 0, dw.n * sizeof(double), st));
+        quadrature_device(K.alpha.p, K.beta.p, K.used.p, nb.p, Rg, Rg, Mop, opts->quad, 0, 1e-14, 1, dr.p, dw.p, M,
+                          st);
+        FTLM_HIP_CHECK(hipMemcpyAsync(rz + static_cast<int64_t>(g0) * M, dr.p, sizeof(double) * Rg * M,
+                                      hipMemcpyDeviceToHost, st));
+        FTLM_HIP_CHECK(hipMemcpyAsync(wt + static_cast<int64_t>(g0) * M, dw.p, sizeof(double) * Rg * M,
+                                      hipMemcpyDeviceToHost, st));
+        std::copy(hu.begin(), hu.end(), us + g0);
+        if (alpha_out || beta_out) {
+          std::vector<double> a(static_cast<size_t>(Mop) * Rg), b(static_cast<size_t>(Mop) * Rg);
+          FTLM_HIP_CHECK(hipMemcpyAsync(a.data(), K.alpha.p, a.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+          FTLM_HIP_CHECK(hipMemcpyAsync(b.data(), K.beta.p, b.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+          FTLM_HIP_CHECK(hipStreamSynchronize(st));
+          for (int r = 0; r < Rg; ++r)
+            for (int k = 0; k < M; ++k) {
+              const int64_t o = (static_cast<int64_t>(io) * R + g0 + r) * M + k;
+              if (alpha_out) alpha_out[o] = k < Mop ? a[static_cast<size_t>(k) * Rg + r] : 0.0;
+              if (beta_out) beta_out[o] = k < Mop ? b[static_cast<size_t>(k) * Rg + r] : 0.0;
+            }
+        }
+        FTLM_HIP_CHECK(hipStreamSynchronize(st));
       }
-      FTLM_HIP_CHECK(hipStreamSynchronize(st));
     }
   });
 }

@@ -856,6 +856,18 @@ int ftlm_hip_kblock_entries(const ftlm_op_t* op, int64_t first, int64_t count, u
   });
 }
 
+int ftlm_hip_kblock_orbit_index(const ftlm_op_t* op, int64_t first, int64_t count, int64_t* out) {
+  return guarded([&] {
+    const KBlock* B = as_kblock(op);
+    FTLM_REQUIRE(first >= 0 && count >= 0 && first + count <= B->dim, FTLM_ERANGE, "entry range outside the block");
+    FTLM_REQUIRE(count == 0 || out != nullptr, FTLM_EINVAL, "null output");
+    if (count == 0) return;
+    std::vector<int32_t> rows(static_cast<size_t>(count));
+    FTLM_HIP_CHECK(hipMemcpy(rows.data(), B->rows.p + first, sizeof(int32_t) * count, hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i < count; ++i) out[i] = rows[static_cast<size_t>(i)];
+  });
+}
+
 int ftlm_hip_kblock_index_of_rep(const ftlm_op_t* op, const uint32_t* reps, int64_t count, int64_t* out) {
   return guarded([&] {
     const KBlock* B = as_kblock(op);

@@ -522,6 +522,18 @@ int ftlm_hip_sector_masks(const ftlm_op_t* op, int64_t first, int64_t count, uin
   });
 }
 
+int ftlm_hip_sector_strings(const ftlm_op_t* op, int spin, uint32_t* out) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && op->kind == kSector, FTLM_EINVAL, "not a full-sector operator");
+    FTLM_REQUIRE(spin == 0 || spin == 1, FTLM_EINVAL, "spin must be 0 or 1");
+    FTLM_REQUIRE(out != nullptr, FTLM_EINVAL, "null output");
+    const Sector* S = static_cast<const Sector*>(op);
+    const int D = spin == 0 ? S->Dup : S->Ddn;
+    FTLM_HIP_CHECK(hipMemcpy(out, spin == 0 ? S->up_mask.p : S->dn_mask.p, sizeof(uint32_t) * D,
+                             hipMemcpyDeviceToHost));
+  });
+}
+
 int ftlm_hip_index_of(const ftlm_op_t* op, const uint64_t* up, const uint64_t* dn, int64_t count, int64_t* out) {
   return guarded([&] {
     FTLM_REQUIRE(op != nullptr && op->kind == kSector, FTLM_EINVAL, "not a full-sector operator");

@@ -95,6 +95,11 @@ int ftlm_hip_op_dim(const ftlm_op_t* op, int64_t* dim, int64_t* dim_up, int64_t*
  * [first, first+count), copied to host arrays.  FTLM_ERANGE outside [0, dim). */
 int ftlm_hip_sector_masks(const ftlm_op_t* op, int64_t first, int64_t count, uint64_t* up, uint64_t* dn);
 
+/* The per-spin string tables behind FockBasis::up_mask / down_mask (src/fock_basis.cpp:26-41):
+ * spin 0 writes the D_up up strings, spin 1 the D_dn down strings, in lexicographic order (the
+ * state index is i_up * D_dn + i_dn). */
+int ftlm_hip_sector_strings(const ftlm_op_t* op, int spin, uint32_t* out);
+
 /* FockBasis::index_of (src/fock_basis.cpp:44-51), batched, ranked on the device by the
  * combinatorial number system; -1 for states outside the sector.  Host arrays. */
 int ftlm_hip_index_of(const ftlm_op_t* op, const uint64_t* up, const uint64_t* dn, int64_t count,
@@ -193,6 +198,15 @@ int ftlm_hip_krylov_result(ftlm_krylov_t* k, double* alpha, double* beta, int* u
 int ftlm_hip_krylov_bytes(const ftlm_krylov_t* k, int64_t* bytes);
 int ftlm_hip_krylov_destroy(ftlm_krylov_t* k);
 
+/* ------------------------------------------------------------------ host-callback operators */
+/* The operator argument of lanczos_tridiagonal / lanczos_extrema / ftlm_log_partition_complex
+ * (std::function over host complex<double> arrays, include/ftlm/lanczos.hpp:47-63,
+ * include/ftlm/ftlm_thermo.hpp:41-50) as an operator handle: `apply(x, y, ctx)` computes y = H x
+ * on interleaved (re, im) host arrays of length dim.  The Lanczos / FTLM recurrences then run on
+ * the device; each H*v copies one vector to the host and back.  complex128 only. */
+typedef void (*ftlm_host_apply_fn)(const double* x, double* y, void* ctx);
+int ftlm_hip_callback_op_create(int64_t dim, ftlm_host_apply_fn apply, void* ctx, ftlm_op_t** out);
+
 /* ------------------------------------------------------------------ translations and k-blocks */
 /* States are packed RawStates, (up << 16) | dn (include/ftlm/symmetry/packed_raw_state.hpp:12-14);
  * lattices have at most 16 sites; translation t = (dx, dy). */
@@ -231,6 +245,10 @@ int ftlm_hip_kblock_info(const ftlm_op_t* op, int* kx, int* ky, int64_t* n_orbit
  * any output may be NULL. */
 int ftlm_hip_kblock_entries(const ftlm_op_t* op, int64_t first, int64_t count, uint32_t* rep, int* orbit_size,
                             int* stab_size, double* A);
+
+/* OrbitKRepEntry::orbit_index of rows [first, first+count): the row's position in the sector's
+ * pack-sorted orbit list (the reference indexes its own MomentumSectorMap, whose order differs). */
+int ftlm_hip_kblock_orbit_index(const ftlm_op_t* op, int64_t first, int64_t count, int64_t* out);
 
 /* OrbitKBlockBasis::reduced_index_of_rep (src/orbit_k_block_basis.cpp:119-125), batched; -1 when
  * the representative is not a row of the block. */

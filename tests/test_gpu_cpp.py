@@ -1,0 +1,51 @@
+"""GPU: the drop-in C++ API.  Runs the C++ test programs of tests/cpp (written against the
+reference's headers) and checks the pybind11 module against the golden vectors."""
+
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "_bin")
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("prog", ["test_pipeline", "test_thermo", "test_orbit"])
+def test_cpp_program(prog):
+    path = os.path.join(BIN, prog)
+    if not os.path.exists(path):
+        pytest.fail(f"{path} missing: run __graft_entry__.build()")
+    say("cpp", prog)
+    r = subprocess.run([path], capture_output=True, text=True, timeout=100)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0, r.stderr
+    assert f"{prog} ok" in r.stdout
+
+
+def test_pybind_hot_path(golden):
+    from ftlm_extended_hubbard_amd import _ftlm_core as c
+
+    say("pybind apply/lanczos c1_2x2")
+    m = golden["geom/c1_2x2/model"]
+    p = c.HubbardParams(int(m[0]), int(m[1]), *map(float, m[2:]))
+    hops, pairs = c.build_hubbard_geometry(p, c.RectLattice(p.Lx, p.Ly))
+    b = c.FockBasis(p.Lx * p.Ly, 2, 2)
+    assert b.dim() == 36
+    x = golden["apply/c1_2x2/2_2/x"]
+    y = c.apply_extended_hubbard(p, b, hops, pairs, x)
+    want = golden["apply/c1_2x2/2_2/y"]
+    assert np.max(np.abs(y - want)) < 1e-12 * np.max(np.abs(want))
+    op = c.SectorOperator(p, b, hops, pairs)
+    used, a, bt = op.lanczos_tridiagonal(36, 7)
+    wa = golden["lanczos/c1_2x2/2_2/s7/alpha"]
+    n = min(12, len(wa))
+    assert np.max(np.abs(np.array(a[:n]) - wa[:n])) < 1e-10

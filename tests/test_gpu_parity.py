@@ -300,3 +300,16 @@ def test_ftlm_long_recurrence(golden, oracle, M):
     for g, bt, gap in zip(got, betas, (8.5e-4, 3.0e-3, 0.26)):
         want = golden[f"ftlm/c2_4x2/4_4/M{M}_beta{bt}"][0]
         assert abs(g - want) < (1e-8 if M == 80 else 10 * gap), (M, bt, g, want)
+
+
+def test_ftlm_more_than_one_sample_block(golden):
+    """R above one Krylov block (32 recurrences) runs in groups; sample r keeps seed + r*100003, so
+    the tail group equals a separate run started at that seed."""
+    say("ftlm R=40 groups")
+    op = api.HubbardOperator.from_params(hp(golden, "c2_4x2"), 3, 2)
+    par = api.FtlmParams(n_random=40, lanczos_steps=12, seed=5)
+    rd = api.ftlm_ritz([op], par)
+    tail = api.ftlm_ritz([op], api.FtlmParams(n_random=8, lanczos_steps=12, seed=5 + 32 * 100003))
+    assert np.array_equal(rd.ritz[0, 32:], tail.ritz[0]) and np.array_equal(rd.weight[0, 32:], tail.weight[0])
+    head = api.ftlm_ritz([op], api.FtlmParams(n_random=32, lanczos_steps=12, seed=5))
+    assert np.array_equal(rd.ritz[0, :32], head.ritz[0])
