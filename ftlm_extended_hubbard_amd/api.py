@@ -413,6 +413,19 @@ def ftlm_log_partition_real(op: _Op, beta, par: FtlmParams = FtlmParams()):
     return _log_partition(op, beta, par, True)
 
 
+def exact_log_partition(op: _Op, betas, return_evals: bool = False):
+    """exact_log_partition_small_complex (benchmarks/bench_ftlm_nmu_4x4_k.cpp:249-336) on the
+    device for an operator of at most 1024 states: lnZ per beta (and the 2*dim Jacobi diagonal of
+    the real embedding when return_evals)."""
+    scalar = np.isscalar(betas)
+    b = np.atleast_1d(np.asarray(betas, np.float64))
+    out = np.zeros(len(b))
+    ev = np.zeros(max(2 * op.dim(), 2)) if return_evals else None
+    L.check(L.lib().ftlm_hip_exact_log_partition(op.handle, b.ctypes.data, len(b), out.ctypes.data, L.ptr(ev)))
+    res = float(out[0]) if scalar else out
+    return (res, ev[: 2 * op.dim()]) if return_evals else res
+
+
 @dataclass
 class RitzData:
     ritz: np.ndarray     # [n_ops, R, M]

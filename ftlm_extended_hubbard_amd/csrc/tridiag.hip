@@ -49,7 +49,7 @@ __global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__
                                                       int stride, int cap_fixed, double tol, int with_weights,
                                                       double* __restrict__ scratch, int64_t scratch_per,
                                                       double* __restrict__ ritz, double* __restrict__ weight,
-                                                      int out_stride) {
+                                                      int out_stride, const double* __restrict__ dense) {
 #pragma clang fp contract(off)
   const int s = blockIdx.x;
   const int n = nvec[s];
@@ -65,16 +65,21 @@ __global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__
   const int tid = threadIdx.x;
   if (n <= 0) return;
 
-  for (int64_t k = tid; k < static_cast<int64_t>(n) * n; k += blockDim.x) A[k] = 0.0;
-  __syncthreads();
-  for (int i = tid; i < n; i += blockDim.x) {
-    A[static_cast<int64_t>(i) * n + i] = alpha[static_cast<int64_t>(i) * stride + s];
-    if (i < n - 1 && i < nb) {
-      const double b = beta[static_cast<int64_t>(i) * stride + s];
-      A[static_cast<int64_t>(i) * n + i + 1] = b;
-      A[static_cast<int64_t>(i + 1) * n + i] = b;
+  if (dense) {  // a full symmetric n x n matrix per workgroup (row-major, n = nvec[s])
+    for (int64_t k = tid; k < static_cast<int64_t>(n) * n; k += blockDim.x) A[k] = dense[scratch_per * s + k];
+    for (int i = tid; i < n; i += blockDim.x) v0[i] = (i == 0) ? 1.0 : 0.0;
+  } else {
+    for (int64_t k = tid; k < static_cast<int64_t>(n) * n; k += blockDim.x) A[k] = 0.0;
+    __syncthreads();
+    for (int i = tid; i < n; i += blockDim.x) {
+      A[static_cast<int64_t>(i) * n + i] = alpha[static_cast<int64_t>(i) * stride + s];
+      if (i < n - 1 && i < nb) {
+        const double b = beta[static_cast<int64_t>(i) * stride + s];
+        A[static_cast<int64_t>(i) * n + i + 1] = b;
+        A[static_cast<int64_t>(i + 1) * n + i] = b;
+      }
+      v0[i] = (i == 0) ? 1.0 : 0.0;
     }
-    v0[i] = (i == 0) ? 1.0 : 0.0;
   }
   __syncthreads();
   const int cap = cap_fixed > 0 ? cap_fixed : max(3000, 120 * n);
@@ -365,6 +370,39 @@ __global__ void k_thermo_gc(const double* lnZ, const double* E, const int* N, in
   if (E_gc) E_gc[t] = z > 0.0 ? ze / z : NAN;
 }
 
+// Real 2d x 2d embedding of a complex d x d block, [[Re, -Im], [Im, Re]]
+// (benchmarks/bench_ftlm_nmu_4x4_k.cpp:298-311); H is row-major complex.
+__global__ void k_embed_real(const double2* __restrict__ H, int d, double* __restrict__ R) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<int64_t>(d) * d) return;
+  const int i = static_cast<int>(t / d), j = static_cast<int>(t % d);
+  const double2 h = H[t];
+  const int64_t m = 2 * d;
+  R[i * m + j] = h.x;
+  R[i * m + j + d] = -h.y;
+  R[(i + d) * m + j] = h.y;
+  R[(i + d) * m + j + d] = h.x;
+}
+
+__global__ void k_identity_block(double2* X, int d) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= static_cast<int64_t>(d) * d) return;
+  X[t] = make_double2((t / d) == (t % d) ? 1.0 : 0.0, 0.0);
+}
+
+// ln Z = ln(1/2) + max_t + ln sum_i exp(t_i - max_t), t_i = -beta lambda_i over the 2d
+// eigenvalues of the embedding (every eigenvalue of H twice).
+__global__ void k_ed_lnz(const double* lam, int m, const double* betas, int nb, double* lnZ) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const double bt = betas[b];
+  double mx = -INFINITY;
+  for (int i = 0; i < m; ++i) mx = fmax(mx, -bt * lam[i]);
+  double sum = 0.0;
+  for (int i = 0; i < m; ++i) sum += exp(-bt * lam[i] - mx);
+  lnZ[b] = log(0.5) + mx + log(sum);
+}
+
 // ---------------------------------------------------------------- host drivers
 
 // Diagonalise nsamp tridiagonals stored strided in device memory.  Outputs (device) ritz and
@@ -391,7 +429,7 @@ void quadrature_device(const double* d_alpha, const double* d_beta, const int* d
   const size_t shmem = static_cast<size_t>(max_n) * (sizeof(double) + sizeof(RowMax));
   FTLM_REQUIRE(shmem <= 60 * 1024, FTLM_EUNSUPPORTED, "tridiagonal too large for the Jacobi kernel");
   hipLaunchKernelGGL(k_jacobi, dim3(nsamp), dim3(kJacBlock), shmem, st, d_alpha, d_beta, d_n, d_nb, stride, cap_fixed,
-                     tol, with_weights, scratch.p, per, d_ritz, d_weight, out_stride);
+                     tol, with_weights, scratch.p, per, d_ritz, d_weight, out_stride, nullptr);
   check_launch("k_jacobi");
   FTLM_HIP_CHECK(hipStreamSynchronize(st));
 }
@@ -414,6 +452,51 @@ void thermo_device(const double* d_ritz, const double* d_weight, const int* d_us
 using namespace ftlm_hip;
 
 extern "C" {
+
+int ftlm_hip_exact_log_partition(const ftlm_op_t* op, const double* betas, int nb, double* lnZ, double* evals) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && betas != nullptr && lnZ != nullptr && nb >= 1, FTLM_EINVAL, "bad arguments");
+    FTLM_HIP_CHECK(hipSetDevice(op->device));
+    const int64_t d64 = op->dim;
+    if (d64 <= 0) {
+      for (int b = 0; b < nb; ++b) lnZ[b] = -INFINITY;
+      return;
+    }
+    FTLM_REQUIRE(d64 <= 1024, FTLM_EUNSUPPORTED, "exact diagonalisation is for blocks of at most 1024 states");
+    const int d = static_cast<int>(d64);
+    // dense H = H * I: the block apply of the d unit vectors, row-major in the vector-block layout
+    DevBuf<double2> X(static_cast<size_t>(d) * d), Y(static_cast<size_t>(d) * d);
+    hipLaunchKernelGGL(k_identity_block, dim3(grid_for(static_cast<int64_t>(d) * d, 256)), dim3(256), 0, nullptr, X.p, d);
+    check_launch("k_identity_block");
+    hv_apply(op, X.p, Y.p, d, FTLM_C128, nullptr, nullptr, nullptr);
+    if (d == 1) {  // a single state: ln Z = -beta H00 (bench_ftlm_nmu_4x4_k.cpp:254-260)
+      double2 h;
+      FTLM_HIP_CHECK(hipMemcpy(&h, Y.p, sizeof(double2), hipMemcpyDeviceToHost));
+      for (int b = 0; b < nb; ++b) lnZ[b] = -betas[b] * h.x;
+      if (evals) evals[0] = evals[1] = h.x;
+      return;
+    }
+    const int m = 2 * d;
+    DevBuf<double> Rm(static_cast<size_t>(m) * m), scratch(static_cast<size_t>(m) * m), lam(static_cast<size_t>(m));
+    hipLaunchKernelGGL(k_embed_real, dim3(grid_for(static_cast<int64_t>(d) * d, 256)), dim3(256), 0, nullptr, Y.p, d,
+                       Rm.p);
+    check_launch("k_embed_real");
+    DevBuf<int> dn(1);
+    dn.upload(&m, 1);
+    const size_t shmem = static_cast<size_t>(m) * (sizeof(double) + sizeof(RowMax));
+    // jacobi_symmetric_all(R, max(5000, 150 m), 1e-13) (bench_ftlm_nmu_4x4_k.cpp:182-230, 324)
+    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(kJacBlock), shmem, nullptr, nullptr, nullptr, dn.p, nullptr, 1,
+                       std::max(5000, 150 * m), 1e-13, 0, scratch.p, static_cast<int64_t>(m) * m, lam.p, nullptr, m,
+                       Rm.p);
+    check_launch("k_jacobi(dense)");
+    DevBuf<double> dbt(static_cast<size_t>(nb)), dl(static_cast<size_t>(nb));
+    dbt.upload(betas, static_cast<size_t>(nb));
+    hipLaunchKernelGGL(k_ed_lnz, dim3(grid_for(nb, 64)), dim3(64), 0, nullptr, lam.p, m, dbt.p, nb, dl.p);
+    check_launch("k_ed_lnz");
+    FTLM_HIP_CHECK(hipMemcpy(lnZ, dl.p, sizeof(double) * nb, hipMemcpyDeviceToHost));
+    if (evals) FTLM_HIP_CHECK(hipMemcpy(evals, lam.p, sizeof(double) * m, hipMemcpyDeviceToHost));
+  });
+}
 
 int ftlm_hip_tridiagonal_extrema(const double* alpha, int n, const double* beta, int nb, double* lo, double* hi) {
   return guarded([&] {

@@ -198,6 +198,13 @@ int ftlm_hip_krylov_result(ftlm_krylov_t* k, double* alpha, double* beta, int* u
 int ftlm_hip_krylov_bytes(const ftlm_krylov_t* k, int64_t* bytes);
 int ftlm_hip_krylov_destroy(ftlm_krylov_t* k);
 
+/* exact_log_partition_small_complex (benchmarks/bench_ftlm_nmu_4x4_k.cpp:249-336) on the device:
+ * dense H of the operator (dim <= 1024), its real 2d x 2d embedding, the reference's capped
+ * max-pivot Jacobi (max(5000, 150 m) rotations, 1e-13), ln Z = ln(1/2) + log-sum-exp(-beta lambda)
+ * for each of nb betas.  dim <= 0 gives -inf, dim == 1 gives -beta H00.  evals (optional, 2*dim)
+ * receives the Jacobi diagonal: every eigenvalue of H twice, unsorted. */
+int ftlm_hip_exact_log_partition(const ftlm_op_t* op, const double* betas, int nb, double* lnZ, double* evals);
+
 /* ------------------------------------------------------------------ host-callback operators */
 /* The operator argument of lanczos_tridiagonal / lanczos_extrema / ftlm_log_partition_complex
  * (std::function over host complex<double> arrays, include/ftlm/lanczos.hpp:47-63,

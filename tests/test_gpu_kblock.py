@@ -200,3 +200,55 @@ def test_kblock_dims_sum_4x4():
                 a, b = np.vdot(x, B.apply(y)), np.vdot(B.apply(x), y)
                 assert abs(a - b) < 1e-10 * abs(a)
     assert tot == 1820 * 1820
+
+
+def dense_of(B):
+    d = B.dim()
+    H = np.zeros((d, d), np.complex128)
+    for j in range(d):
+        e = np.zeros(d, np.complex128)
+        e[j] = 1.0
+        H[:, j] = B.apply(e)
+    return H
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd,kx,ky", [(2, 2, 2, 2, 0, 0), (2, 2, 2, 2, 1, 1), (3, 2, 3, 3, 1, 0),
+                                               (3, 2, 2, 1, 2, 1)])
+def test_small_block_ed_vs_oracle(oracle, lx, ly, nu, nd, kx, ky):
+    """exact_log_partition_small_complex: device ED of a k-block against the oracle's restatement
+    on the same dense block (the capped Jacobi is deterministic given its input)."""
+    say("ed kblock", lx, ly, nu, nd, kx, ky)
+    B = kblock(lx, ly, nu, nd, kx, ky)
+    H = dense_of(B)
+    betas = np.array([0.1, 1.0, 5.0, 20.0])
+    got, ev = api.exact_log_partition(B, betas, return_evals=True)
+    want = np.array([oracle.exact_log_partition(H, b) for b in betas])
+    w = np.linalg.eigvalsh(H)
+    exact = np.array([np.max(-b * w) + np.log(np.sum(np.exp(-b * w - np.max(-b * w)))) for b in betas])
+    # Beyond ~40 rows the reference's cap of max(5000, 150 m) rotations stops the Jacobi before it
+    # converges, and the capped diagonal then depends on ulp-level differences of atan2/cos/sin
+    # between the device and the host: 1e-9 relative; both stay equally close to the exact lnZ.
+    assert np.max(np.abs(got - want) / np.maximum(1.0, np.abs(want))) < 1e-9, (got, want)
+    assert np.max(np.abs(np.abs(got - exact) - np.abs(want - exact))) < 1e-9 * np.max(np.abs(exact))
+    if 2 * B.dim() <= 40:  # converged well inside the rotation cap: the exact spectrum, twice
+        assert np.max(np.abs(got - want)) < 1e-10
+        assert np.max(np.abs(np.sort(ev) - np.sort(np.concatenate([w, w])))) < 1e-10
+        for b, g in zip(betas, got):
+            mx = np.max(-b * w)
+            assert abs(g - (mx + np.log(np.sum(np.exp(-b * w - mx))))) < 1e-10
+
+
+def test_small_sector_ed(oracle, golden):
+    """Full sectors: dim 1 (-beta H00) and the 2x2 (2,2) sector of the 2x2 configuration."""
+    say("ed sectors")
+    m = golden["geom/c1_2x2/model"]
+    p = api.HubbardParams(int(m[0]), int(m[1]), *map(float, m[2:]))
+    op = api.HubbardOperator.from_params(p, 4, 4)
+    assert op.dim() == 1
+    h00 = op.apply(np.ones(1, np.complex128))[0].real
+    assert api.exact_log_partition(op, 2.0) == -2.0 * h00
+    op = api.HubbardOperator.from_params(p, 2, 2)
+    H = oracle.dense_hamiltonian(params8(*m), 2, 2)
+    got = api.exact_log_partition(op, np.array([0.5, 3.0]))
+    want = [oracle.exact_log_partition(H, b) for b in (0.5, 3.0)]
+    assert np.max(np.abs(got - want)) < 1e-9
