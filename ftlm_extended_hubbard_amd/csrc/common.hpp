@@ -104,7 +104,10 @@ constexpr int kParityBit = kTargetBits + kBondBits;  // 29
 constexpr uint32_t kZeroBond = kBondMask;             // 511
 constexpr uint32_t kPadEntry = kZeroBond << kTargetBits;
 constexpr int kCoefSlots = 2 << kBondBits;            // 1024 pre-signed coefficients
-constexpr int kSlotPad = 8;
+#ifndef FTLM_SLOT_PAD
+#define FTLM_SLOT_PAD 8
+#endif
+constexpr int kSlotPad = FTLM_SLOT_PAD;
 
 // Binomial table C[a][b], a,b <= 32 (values < 2^63).
 struct Binom {

@@ -98,7 +98,10 @@ __device__ __forceinline__ void axpy_regs(T (&ar)[N], T (&ai)[N], const typename
   }
 }
 
-constexpr int kUnroll = 8;  // hops whose gathers are issued together
+#ifndef FTLM_HV_UNROLL
+#define FTLM_HV_UNROLL 8
+#endif
+constexpr int kUnroll = FTLM_HV_UNROLL;  // hops whose gathers are issued together (divides kSlotPad)
 
 // Hop entries of a tile are staged in LDS (16-byte loads, then LDS reads shared by the L lanes
 // of an element) when a tile is at most 64 elements wide; wider single-vector tiles read them
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
   using SP = Split<T, RB>;
   constexpr int L = SP::L, VPL = SP::VPL, EPB = SP::EPB;
   constexpr bool kStage = StageEntries<EPB>::value;
-  static_assert(kUnroll == kSlotPad, "hop groups must match the slot padding");
+  static_assert(kSlotPad % kUnroll == 0, "hop groups must divide the slot padding");
   __shared__ V2 s_c[kCoefSlots];  // row-signed coefficients [parity * 512 + bond]
   __shared__ int s_tile;
   extern __shared__ uint4 s_ent4[];  // this tile's entries, [maxcnt][EPB] uint32
