@@ -82,6 +82,85 @@ def profile_hv(F, L, torch, R):
     log("profile workload done: 3 x H*V (R=%d, c64) + 4 GiB copy" % R)
 
 
+def kblock_workload(F, L, torch, steps, validate=True):
+    """4x4 (8,8) momentum blocks (t=1, t'=-0.25, U=8, V=1, phi=0; the 16 momenta): orbit-graph and
+    block setup times, block apply times, a k-block Lanczos step at R=32 complex64, and two exact
+    identities at full size: sum_K dim_K = dim and min_K E0_K = E0 of the full sector."""
+    import numpy as np
+
+    p = F.HubbardParams(4, 4, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0)
+    out = {"metric": "4x4 (8,8) k-block FTLM sample-steps/s (R=32 complex64, fp64 reductions)",
+           "unit": "sample-steps/s", "n_gpus": 1, "higher_is_better": True, "data": "synthetic",
+           "config": {"workload": "4x4 (8,8) momentum blocks, 16 K", "hamiltonian": dict(
+               Lx=4, Ly=4, t=1.0, tp=-0.25, U=8.0, V=1.0, phi_x=0.0, phi_y=0.0)}}
+    t0 = time.perf_counter()
+    first = F.KBlockOperator.from_params(p, NU, ND, F.MomentumSector(0, 0, 4, 4))
+    torch.cuda.synchronize()
+    out["graph_and_first_block_s"] = round(time.perf_counter() - t0, 3)
+    info = first.info()
+    log(f"orbit graph: {info['n_orbits']} orbits, {info['nnz']} hop terms, {out['graph_and_first_block_s']} s")
+    blocks = [first]
+    t0 = time.perf_counter()
+    for ky in range(4):
+        for kx in range(4):
+            if (kx, ky) != (0, 0):
+                blocks.append(F.KBlockOperator.from_params(p, NU, ND, F.MomentumSector(kx, ky, 4, 4), share=first))
+    torch.cuda.synchronize()
+    out["other_15_blocks_s"] = round(time.perf_counter() - t0, 3)
+    dims = [b.dim() for b in blocks]
+    out["n_orbits"], out["nnz"], out["dims"] = info["n_orbits"], info["nnz"], dims
+    out["sum_dims_equals_dim"] = bool(sum(dims) == 12870 * 12870)
+    log(f"block dims {min(dims)}..{max(dims)}, sum {sum(dims)} (sector {12870 * 12870})")
+    # apply timings on the largest block
+    B = blocks[int(np.argmax(dims))]
+    d = B.dim()
+    ap = {}
+    for R, dt, ct, s in [(1, L.FTLM_C128, torch.complex128, 16), (8, L.FTLM_C128, torch.complex128, 16),
+                         (16, L.FTLM_C64, torch.complex64, 8)]:
+        X = torch.randn(d * R, dtype=ct, device="cuda")
+        Y = torch.empty_like(X)
+        B.apply_device(X, Y, R=R, dtype=dt)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(3):
+            B.apply_device(X, Y, R=R, dtype=dt)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 3
+        ap[f"R{R}_{'c64' if dt == L.FTLM_C64 else 'c128'}"] = {"ms": round(ms, 3),
+                                                              "GB_s_algorithmic": round(R * d * 2 * s / ms / 1e6, 1)}
+        log(f"k-block apply dim={d} R={R} dtype={dt}: {ms:.3f} ms")
+        del X, Y
+    torch.cuda.empty_cache()
+    out["apply"] = ap
+    # Lanczos steps, R=32 complex64 random phases, on the largest block
+    sess = F.KrylovSession(B, R=32, max_steps=steps + 1, dtype=L.FTLM_C64, start=L.FTLM_START_PHASE, seed=1)
+    sess.run(1)
+    secs = time_steps(sess, steps, torch)
+    sess.close()
+    out["value"] = round(steps * 32 / secs, 3)
+    out["ms_per_step"] = round(secs / steps * 1e3, 3)
+    out["steps"] = steps
+    log(f"k-block Lanczos R=32 c64: {out['ms_per_step']} ms/step, {out['value']} sample-steps/s")
+    if not validate:
+        return out
+    # ground energy: full sector vs the momentum blocks (converged Ritz values, M=200)
+    par = F.FtlmParams(n_random=1, lanczos_steps=200, seed=3, start=L.FTLM_START_PHASE, dtype=L.FTLM_C128,
+                       quad=L.FTLM_QUAD_CONVERGED)
+    t0 = time.perf_counter()
+    full = F.HubbardOperator.from_params(p, NU, ND)
+    rd = F.ftlm_ritz([full], par)
+    e_full = float(np.min(rd.ritz[0, 0, : rd.used[0, 0]]))
+    del full
+    rk = F.ftlm_ritz(blocks, par)
+    e_k = [float(np.min(rk.ritz[i, 0, : rk.used[i, 0]])) for i in range(len(blocks))]
+    out["E0"] = {"full_sector": e_full, "min_over_K": min(e_k), "per_K": e_k,
+                 "abs_diff": abs(e_full - min(e_k)), "seconds": round(time.perf_counter() - t0, 1)}
+    log(f"E0 full {e_full:.12f} vs min_K {min(e_k):.12f}")
+    return out
+
+
 def traffic_from_profile():
     """HBM bytes per H*V (R=16 complex64) from the committed counter summary: the line
     'hbm_bytes_per_hv = <int>' (FETCH_SIZE + WRITE_SIZE of both passes, copy-calibrated)."""
@@ -171,6 +250,8 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--workload", choices=["sector", "kblock"], default="sector",
+                    help="sector: the 4x4 (8,8) full-sector line (default); kblock: the momentum-block line")
     ap.add_argument("--profile-hv", action="store_true",
                     help="only run 3 block H*V (R vectors, complex64) and one 4 GiB device copy, for rocprofv3")
     args = ap.parse_args()
@@ -189,6 +270,9 @@ def main():
     K, W, R = args.steps, args.warmup, args.R
     if args.profile_hv:
         profile_hv(F, L, torch, R)
+        return
+    if args.workload == "kblock":
+        print(json.dumps(kblock_workload(F, L, torch, K, validate=not args.no_validate)), flush=True)
         return
     t0 = time.perf_counter()
     op = F.HubbardOperator.from_params(F.HubbardParams(**MODEL), NU, ND)

@@ -42,6 +42,7 @@ namespace ftlm_hip {
 constexpr int kMaxOrbitSites = 16;   // RawState packing (u << 16) | d
 constexpr uint32_t kColMask = 0xFFFFFFu;
 constexpr uint32_t kDiagFlag = 1u << 29;
+constexpr int kStabShift = 9;  // entry .y: bond (bits 0..8), |Stab_row| - 1 (9..12), |Stab_col| - 1 (13..16)
 
 // Lattice translation tables (kernel argument): tsite[t][s] = image of site s under
 // translation t = dy * lx + dx.
@@ -204,8 +205,8 @@ __global__ void k_orb_count(const uint32_t* pack, int64_t nrep, const OrbBond* b
 }
 
 // Push-form entries of column g: key = target orbit, value = {g | R << 24 | sign << 28, bond}.
-__global__ void k_orb_fill(const uint32_t* pack, int64_t nrep, const OrbBond* bonds, int nb, const int32_t* off,
-                           LatDev L, uint32_t* keys, uint2* vals, int* bad) {
+__global__ void k_orb_fill(const uint32_t* pack, const uint16_t* stab, int64_t nrep, const OrbBond* bonds, int nb,
+                           const int32_t* off, LatDev L, uint32_t* keys, uint2* vals, int* bad) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= nrep) return;
   const uint32_t u = pack[g] >> 16, d = pack[g] & 0xFFFFu;
@@ -232,10 +233,13 @@ __global__ void k_orb_fill(const uint32_t* pack, int64_t nrep, const OrbBond* bo
       continue;
     }
     const int eta = tr_parity(cp >> 16, cp & 0xFFFFu, L, t);
+    // |Stab| of the target (row) and source (column) orbits, for sqrt(|A_a| / |A_b|): a surviving
+    // orbit has |A_K| = |Stab| at every momentum (orthogonality of the stabiliser characters)
+    const uint32_t sa = __popc(static_cast<uint32_t>(stab[a])) - 1u, sb = __popc(static_cast<uint32_t>(stab[g])) - 1u;
     keys[w] = static_cast<uint32_t>(a);
     vals[w] = make_uint2(static_cast<uint32_t>(g) | (static_cast<uint32_t>(t) << 24) |
                              (static_cast<uint32_t>(jw ^ eta) << 28),
-                         static_cast<uint32_t>(b));
+                         static_cast<uint32_t>(b) | (sa << kStabShift) | (sb << (kStabShift + 4)));
     ++w;
   }
 }
@@ -283,6 +287,12 @@ __global__ void k_kblock_maps(const int32_t* rows, int64_t dk, const double2* A,
   map[g] = static_cast<int32_t>(r);
   sq[g] = sqrt(a);
   isq[g] = 1.0 / sqrt(a);
+}
+
+// Column of every stored term in this block's row numbering (-1: the source orbit is not a row).
+__global__ void k_kblock_cols(const uint2* ent, int64_t nnz, const int32_t* map, int32_t* col) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < nnz) col[e] = map[ent[e].x & kColMask];
 }
 
 __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
@@ -356,19 +366,31 @@ __global__ void k_canonical_states(const uint32_t* states, int64_t count, LatDev
 }
 
 // ------------------------------------------------------------------ apply
-// One thread per (row, group of RB vectors); row r of block K gathers its CSR terms.
+// One thread per (block row, group of RB vectors).  The row's terms are read in groups of
+// kKbGroup with their block columns (resolved per momentum) and gathered together; a term whose
+// source orbit is not a row of this block, and the padding of the last group, get coefficient 0
+// and read the row's own element.  Coefficient of a hop term, in the reference's order:
+//   ((c_bond * (sign * sqrt(|Stab_row| / |Stab_col|))) * chi*_K(R)).
+template <int RB>
+struct KbGroup {
+  static constexpr int value = RB <= 2 ? 8 : (RB <= 4 ? 4 : 2);
+};
+
 template <typename T, int RB>
 __global__ __launch_bounds__(256) void k_kblock_apply(
     const int32_t* __restrict__ rows, int64_t dk, const int32_t* __restrict__ row_off, const uint2* __restrict__ ent,
-    const int32_t* __restrict__ map, const double* __restrict__ sq, const double* __restrict__ isq,
-    const double* __restrict__ diag, const double2* __restrict__ coef, int nbond, Phases ph,
-    const typename Cx<T>::v2* __restrict__ x, typename Cx<T>::v2* __restrict__ y, int RT, int r0,
+    const int32_t* __restrict__ colk, const double* __restrict__ diag, const double2* __restrict__ coef, int nbond,
+    Phases ph, const typename Cx<T>::v2* __restrict__ x, typename Cx<T>::v2* __restrict__ y, int RT, int r0,
     const double* __restrict__ xs, double2* __restrict__ dotp) {
   using V2 = typename Cx<T>::v2;
+  constexpr int G = KbGroup<RB>::value;
   __shared__ double2 s_coef[kMaxBondsLds];
   __shared__ double2 s_chi[kMaxOrbitSites];
-  for (int b = threadIdx.x; b < nbond; b += blockDim.x) s_coef[b] = coef[b];
+  __shared__ double s_nrm[kMaxOrbitSites * kMaxOrbitSites];
+  for (int b = threadIdx.x; b < kMaxBondsLds; b += blockDim.x) s_coef[b] = b < nbond ? coef[b] : make_double2(0.0, 0.0);
   if (threadIdx.x < kMaxOrbitSites) s_chi[threadIdx.x] = ph.chi[threadIdx.x];
+  for (int i = threadIdx.x; i < kMaxOrbitSites * kMaxOrbitSites; i += blockDim.x)
+    s_nrm[i] = sqrt(static_cast<double>((i & 15) + 1) / static_cast<double>((i >> 4) + 1));
   __syncthreads();
   double dre[RB], dimg[RB];
 #pragma unroll
@@ -376,38 +398,52 @@ __global__ __launch_bounds__(256) void k_kblock_apply(
   for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < dk;
        r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int32_t g = rows[r];
-    const double sa = sq[g];
+    const int32_t e0 = row_off[g], e1 = row_off[g + 1];
+    const double dg = diag[g];
     T ar[RB], ai[RB];
 #pragma unroll
     for (int v = 0; v < RB; ++v) ar[v] = ai[v] = static_cast<T>(0);
-    for (int32_t e = row_off[g]; e < row_off[g + 1]; ++e) {
-      const uint2 en = ent[e];
-      const int32_t col = static_cast<int32_t>(en.x & kColMask);
-      const int32_t c = map[col];
-      if (c < 0) continue;
-      double cr, ci;
-      if (en.x & kDiagFlag) {
-        cr = diag[g];
-        ci = 0.0;
-      } else {
-        // ((c_bond * (sign * nrm)) * chi) as the reference multiplies it
-        const double nrm = sa * isq[col];
-        const double w = ((en.x >> 28) & 1u) ? -nrm : nrm;
-        const double2 cb = s_coef[en.y];
-        const double hr = cb.x * w, hi = cb.y * w;
-        const double2 ch = s_chi[(en.x >> 24) & 15u];
-        cr = hr * ch.x - hi * ch.y;
-        ci = hr * ch.y + hi * ch.x;
-      }
-      const V2* src = x + static_cast<int64_t>(c) * RT + r0;
-      const T tr = static_cast<T>(cr), ti = static_cast<T>(ci);
+    for (int32_t e = e0; e < e1; e += G) {
+      uint2 en[G];
+      int32_t cc[G];
 #pragma unroll
-      for (int v = 0; v < RB; ++v) {
-        const V2 s = src[v];
-        ar[v] = fma(tr, s.x, ar[v]);
-        ai[v] = fma(tr, s.y, ai[v]);
-        ar[v] = fma(-ti, s.y, ar[v]);
-        ai[v] = fma(ti, s.x, ai[v]);
+      for (int j = 0; j < G; ++j) {
+        const int32_t ej = min(e + j, e1 - 1);
+        en[j] = ent[ej];
+        const int32_t c = colk[ej];
+        cc[j] = (e + j < e1 && c >= 0) ? c : -1;
+      }
+      V2 src[G][RB];
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const V2* sp = x + static_cast<int64_t>(cc[j] >= 0 ? cc[j] : r) * RT + r0;
+#pragma unroll
+        for (int v = 0; v < RB; ++v) src[j][v] = sp[v];
+      }
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        double cr, ci;
+        if (en[j].x & kDiagFlag) {
+          cr = dg;
+          ci = 0.0;
+        } else {
+          const double nrm = s_nrm[(en[j].y >> kStabShift) & 0xFFu];
+          const double w = ((en[j].x >> 28) & 1u) ? -nrm : nrm;
+          const double2 cb = s_coef[en[j].y & kBondMask];
+          const double hr = cb.x * w, hi = cb.y * w;
+          const double2 ch = s_chi[(en[j].x >> 24) & 15u];
+          cr = hr * ch.x - hi * ch.y;
+          ci = hr * ch.y + hi * ch.x;
+        }
+        if (cc[j] < 0) cr = ci = 0.0;
+        const T tr = static_cast<T>(cr), ti = static_cast<T>(ci);
+#pragma unroll
+        for (int v = 0; v < RB; ++v) {
+          ar[v] = fma(tr, src[j][v].x, ar[v]);
+          ai[v] = fma(tr, src[j][v].y, ai[v]);
+          ar[v] = fma(-ti, src[j][v].y, ar[v]);
+          ai[v] = fma(ti, src[j][v].x, ai[v]);
+        }
       }
     }
     V2* dst = y + r * RT + r0;
@@ -449,6 +485,123 @@ __global__ __launch_bounds__(256) void k_kblock_apply(
       for (int w = 0; w < 256 / 64; ++w) {
         a += red[w][threadIdx.x][0];
         b += red[w][threadIdx.x][1];
+      }
+      dotp[static_cast<int64_t>(blockIdx.x) * RT + r0 + threadIdx.x] = make_double2(a, b);
+    }
+  }
+}
+
+// Wide blocks: the RB vectors of a row are spread over L lanes of 16 bytes (VPL vectors each), as
+// in the full-sector kernels, so a term's gather is one contiguous RB * s run read by L lanes; a
+// workgroup covers 256 / L rows and every lane keeps 8 terms in flight.
+template <typename T, int RB>
+__global__ __launch_bounds__(256) void k_kblock_apply_split(
+    const int32_t* __restrict__ rows, int64_t dk, const int32_t* __restrict__ row_off, const uint2* __restrict__ ent,
+    const int32_t* __restrict__ colk, const double* __restrict__ diag, const double2* __restrict__ coef, int nbond,
+    Phases ph, const typename Cx<T>::v2* __restrict__ x, typename Cx<T>::v2* __restrict__ y, int RT, int r0,
+    const double* __restrict__ xs, double2* __restrict__ dotp) {
+  using V2 = typename Cx<T>::v2;
+  constexpr int BYTES = RB * static_cast<int>(sizeof(V2));
+  constexpr int L = BYTES >= 32 ? BYTES / 16 : 1;
+  constexpr int VPL = RB / L;
+  constexpr int RPB = 256 / L;  // rows per workgroup pass
+  constexpr int G = 8;
+  __shared__ double2 s_coef[kMaxBondsLds];
+  __shared__ double2 s_chi[kMaxOrbitSites];
+  __shared__ double s_nrm[kMaxOrbitSites * kMaxOrbitSites];
+  for (int b = threadIdx.x; b < kMaxBondsLds; b += blockDim.x) s_coef[b] = b < nbond ? coef[b] : make_double2(0.0, 0.0);
+  if (threadIdx.x < kMaxOrbitSites) s_chi[threadIdx.x] = ph.chi[threadIdx.x];
+  for (int i = threadIdx.x; i < kMaxOrbitSites * kMaxOrbitSites; i += blockDim.x)
+    s_nrm[i] = sqrt(static_cast<double>((i & 15) + 1) / static_cast<double>((i >> 4) + 1));
+  __syncthreads();
+  const int sub = static_cast<int>(threadIdx.x) % L, rl = static_cast<int>(threadIdx.x) / L;
+  const int rr = r0 + sub * VPL;
+  double dre[VPL], dimg[VPL], sxv[VPL];
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) {
+    dre[v] = dimg[v] = 0.0;
+    sxv[v] = xs ? xs[rr + v] : 1.0;
+  }
+  for (int64_t r = static_cast<int64_t>(blockIdx.x) * RPB + rl; r < dk; r += static_cast<int64_t>(gridDim.x) * RPB) {
+    const int32_t g = rows[r];
+    const int32_t e0 = row_off[g], e1 = row_off[g + 1];
+    const double dg = diag[g];
+    T ar[VPL], ai[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) ar[v] = ai[v] = static_cast<T>(0);
+    for (int32_t e = e0; e < e1; e += G) {
+      uint2 en[G];
+      int32_t cc[G];
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const int32_t ej = min(e + j, e1 - 1);
+        en[j] = ent[ej];
+        const int32_t c = colk[ej];
+        cc[j] = (e + j < e1 && c >= 0) ? c : -1;
+      }
+      V2 src[G][VPL];
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const V2* sp = x + static_cast<int64_t>(cc[j] >= 0 ? cc[j] : r) * RT + rr;
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) src[j][v] = sp[v];
+      }
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        double cr, ci;
+        if (en[j].x & kDiagFlag) {
+          cr = dg;
+          ci = 0.0;
+        } else {
+          const double nrm = s_nrm[(en[j].y >> kStabShift) & 0xFFu];
+          const double w = ((en[j].x >> 28) & 1u) ? -nrm : nrm;
+          const double2 cb = s_coef[en[j].y & kBondMask];
+          const double hr = cb.x * w, hi = cb.y * w;
+          const double2 ch = s_chi[(en[j].x >> 24) & 15u];
+          cr = hr * ch.x - hi * ch.y;
+          ci = hr * ch.y + hi * ch.x;
+        }
+        if (cc[j] < 0) cr = ci = 0.0;
+        const T tr = static_cast<T>(cr), ti = static_cast<T>(ci);
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          ar[v] = fma(tr, src[j][v].x, ar[v]);
+          ai[v] = fma(tr, src[j][v].y, ai[v]);
+          ar[v] = fma(-ti, src[j][v].y, ar[v]);
+          ai[v] = fma(ti, src[j][v].x, ai[v]);
+        }
+      }
+    }
+    V2* dst = y + r * RT + rr;
+    const V2* own = x + r * RT + rr;
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+      V2 o;
+      o.x = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
+      o.y = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
+      dst[v] = o;
+      if (dotp) {
+        const V2 q = own[v];
+        const double qr = sxv[v] * static_cast<double>(q.x), qi = sxv[v] * static_cast<double>(q.y);
+        dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+        dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+      }
+    }
+  }
+  if (dotp) {  // lane t holds vectors rr .. rr + VPL of its row slot; sum over the L-strided lanes
+    __shared__ double red[256][VPL][2];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+      red[threadIdx.x][v][0] = dre[v];
+      red[threadIdx.x][v][1] = dimg[v];
+    }
+    __syncthreads();
+    if (threadIdx.x < RB) {
+      const int s2 = threadIdx.x / VPL, v2 = threadIdx.x % VPL;
+      double a = 0.0, b = 0.0;
+      for (int j = s2; j < 256; j += L) {
+        a += red[j][v2][0];
+        b += red[j][v2][1];
       }
       dotp[static_cast<int64_t>(blockIdx.x) * RT + r0 + threadIdx.x] = make_double2(a, b);
     }
@@ -523,6 +676,7 @@ struct KBlock : ftlm_op {
   double tol = 1e-12;
   Phases ph{};
   DevBuf<int32_t> rows, map;
+  DevBuf<int32_t> colk;  // [nnz] block column of every stored term, -1 when not a row here
   DevBuf<double> sq, isq;
   DevBuf<double2> A;  // [nrep]
 };
@@ -640,8 +794,8 @@ std::shared_ptr<OrbitGraph> build_orbit_graph(int lx, int ly, int n_up, int n_dn
   DevBuf<int> bad(1);
   FTLM_HIP_CHECK(hipMemset(bad.p, 0, sizeof(int)));
   if (nr > 0) {
-    hipLaunchKernelGGL(k_orb_fill, dim3(grid_for(nr, 256)), dim3(256), 0, st, G->pack.p, static_cast<int64_t>(nr),
-                       dbonds.p, G->nbond, off.p, G->L, k0.p, v0.p, bad.p);
+    hipLaunchKernelGGL(k_orb_fill, dim3(grid_for(nr, 256)), dim3(256), 0, st, G->pack.p, G->stab.p,
+                       static_cast<int64_t>(nr), dbonds.p, G->nbond, off.p, G->L, k0.p, v0.p, bad.p);
     check_launch("k_orb_fill");
   }
   int hbad = 0;
@@ -715,6 +869,12 @@ void build_kblock(KBlock* B, double tol) {
     hipLaunchKernelGGL(k_kblock_maps, dim3(grid_for(dk, 256)), dim3(256), 0, st, B->rows.p,
                        static_cast<int64_t>(dk), B->A.p, B->map.p, B->sq.p, B->isq.p);
     check_launch("k_kblock_maps");
+    B->colk.alloc(static_cast<size_t>(std::max<int64_t>(G.nnz, 1)));
+    if (G.nnz > 0) {
+      hipLaunchKernelGGL(k_kblock_cols, dim3(grid_for(G.nnz, 256)), dim3(256), 0, st, G.ent.p, G.nnz, B->map.p,
+                         B->colk.p);
+      check_launch("k_kblock_cols");
+    }
   }
   FTLM_HIP_CHECK(hipDeviceSynchronize());
   B->dim = dk;
@@ -727,15 +887,22 @@ void launch_kblock(const KBlock* B, const void* x, void* y, int RT, int r0, cons
                    hipStream_t st, int grid) {
   using V2 = typename Cx<T>::v2;
   const OrbitGraph& G = *B->G;
-  hipLaunchKernelGGL((k_kblock_apply<T, RB>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, G.row_off.p, G.ent.p,
-                     B->map.p, B->sq.p, B->isq.p, G.diag.p, G.coef.p, G.nbond, B->ph, static_cast<const V2*>(x),
-                     static_cast<V2*>(y), RT, r0, xs, dotp);
+  if (RB * sizeof(V2) >= 32) {
+    hipLaunchKernelGGL((k_kblock_apply_split<T, RB>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, G.row_off.p,
+                       G.ent.p, B->colk.p, G.diag.p, G.coef.p, G.nbond, B->ph, static_cast<const V2*>(x),
+                       static_cast<V2*>(y), RT, r0, xs, dotp);
+  } else {
+    hipLaunchKernelGGL((k_kblock_apply<T, RB>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, G.row_off.p,
+                       G.ent.p, B->colk.p, G.diag.p, G.coef.p, G.nbond, B->ph, static_cast<const V2*>(x),
+                       static_cast<V2*>(y), RT, r0, xs, dotp);
+  }
 }
 
 template <typename T>
 void launch_kblock_chunk(const KBlock* B, const void* x, void* y, int RT, int r0, int rb, const double* xs,
                          double2* dotp, hipStream_t st, int grid) {
   switch (rb) {
+    case 16: launch_kblock<T, 16>(B, x, y, RT, r0, xs, dotp, st, grid); break;
     case 8: launch_kblock<T, 8>(B, x, y, RT, r0, xs, dotp, st, grid); break;
     case 4: launch_kblock<T, 4>(B, x, y, RT, r0, xs, dotp, st, grid); break;
     case 2: launch_kblock<T, 2>(B, x, y, RT, r0, xs, dotp, st, grid); break;
@@ -752,10 +919,11 @@ int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype,
   const KBlock* B = static_cast<const KBlock*>(op);
   if (B->dim == 0) return 0;
   const int grid = static_cast<int>(std::min<int64_t>(kKbMaxGrid, (B->dim + 255) / 256));
+  const int maxrb = dtype == FTLM_C64 ? 16 : 8;
   std::vector<std::pair<int, int>> chunks;
   for (int r0 = 0; r0 < R;) {
     int rb = 1;
-    while (rb * 2 <= 8 && r0 + rb * 2 <= R) rb *= 2;
+    while (rb * 2 <= maxrb && r0 + rb * 2 <= R) rb *= 2;
     chunks.emplace_back(r0, rb);
     r0 += rb;
   }
