@@ -161,6 +161,93 @@ def kblock_workload(F, L, torch, steps, validate=True):
     return out
 
 
+def gc4x2_workload(F, L, torch, validate=True, cpu=True):
+    """Grand-canonical FTLM on the 4x2 lattice (t=1, t'=-0.25, U=8, V=1, phi=0): all 81 (N_up, N_dn)
+    sectors (dims 1..4900, sum 65,536), R=64 random-phase starts per sector, M=min(dim, 80),
+    50 temperatures log-spaced in [0.05, 5] and 41 chemical potentials in [-2, 10].  The batched
+    complex64 recurrences run in one persistent launch; the plain complex128 per-sector path runs
+    beside it with the same starts."""
+    import numpy as np
+
+    p = F.HubbardParams(4, 2, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0)
+    R, M = 64, 80
+    t0 = time.perf_counter()
+    ops, N = [], []
+    for nu in range(9):
+        for nd in range(9):
+            ops.append(F.HubbardOperator.from_params(p, nu, nd))
+            N.append(nu + nd)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t0
+    dims = np.array([o.dim() for o in ops])
+    sample_steps = int(sum(R * min(int(d), M) for d in dims if d > 1))
+    Ts = np.logspace(np.log10(0.05), np.log10(5.0), 50)
+    betas, mus = 1.0 / Ts, np.linspace(-2.0, 10.0, 41)
+
+    def run(dtype, path, reps=3):
+        par = F.FtlmParams(n_random=R, lanczos_steps=M, seed=1, start=L.FTLM_START_PHASE, dtype=dtype,
+                           quad=L.FTLM_QUAD_CONVERGED, clamp_M_to_dim=True, path=path)
+        F.ftlm_ritz(ops, par)  # warm
+        ts = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            rd = F.ftlm_ritz(ops, par)
+            ts.append(time.perf_counter() - t1)
+        t2 = time.perf_counter()
+        th = F.ftlm_thermo(rd, betas, N=N, mus=mus)
+        return rd, th, min(ts), time.perf_counter() - t2
+
+    rd64, th64, t64, tth = run(L.FTLM_C64, L.FTLM_PATH_PERSISTENT)
+    log(f"4x2 grand canonical: persistent c64 {t64 * 1e3:.1f} ms for {sample_steps} sample-steps "
+        f"({sample_steps / t64:.0f}/s); thermo over 50 T x 41 mu {tth * 1e3:.1f} ms")
+    rdp, thp, tp128, _ = run(L.FTLM_C128, L.FTLM_PATH_PERSISTENT)
+    log(f"4x2 grand canonical: persistent c128 {tp128 * 1e3:.1f} ms ({sample_steps / tp128:.0f}/s)")
+    rd128, th128, t128, _ = run(L.FTLM_C128, L.FTLM_PATH_PER_OP, reps=1)
+    log(f"4x2 grand canonical: per-sector c128 {t128 * 1e3:.1f} ms ({sample_steps / t128:.0f}/s)")
+    out = {"metric": "4x2 grand-canonical FTLM sample-steps/s (81 sectors x R=64, M=min(dim,80), complex64)",
+           "value": round(sample_steps / t64, 1), "unit": "sample-steps/s", "n_gpus": 1, "higher_is_better": True,
+           "data": "synthetic", "dtype": "c64",
+           "config": {"workload": "4x2 all 81 (N_up, N_dn) sectors, random-phase starts, 50 T x 41 mu",
+                      "hamiltonian": dict(Lx=4, Ly=2, t=1.0, tp=-0.25, U=8.0, V=1.0, phi_x=0.0, phi_y=0.0),
+                      "R": R, "M": M, "setup_s": round(t_setup, 3)},
+           "sample_steps": sample_steps, "persistent_c64_s": round(t64, 4), "thermo_grid_s": round(tth, 4),
+           "persistent_c128": {"s": round(tp128, 4), "value": round(sample_steps / tp128, 1)},
+           "plain_c128_per_sector": {"s": round(t128, 4), "value": round(sample_steps / t128, 1)},
+           "batched_vs_plain": round(t128 / t64, 2)}
+    if validate:
+        dN = np.abs(th64["N_gc"] - th128["N_gc"])
+        dl = np.abs(th64["lnZ"] - th128["lnZ"])[dims > 1]
+        dlp = np.abs(thp["lnZ"] - th128["lnZ"])[dims > 1]
+        out["c64_vs_c128"] = {"max_abs_dN_gc": float(dN.max()), "max_abs_dlnZ_sector": float(dl.max())}
+        out["persistent_c128_vs_per_sector_c128"] = {"max_abs_dlnZ_sector": float(dlp.max())}
+        log(f"c64 vs c128 (same starts): max |d<N>| {dN.max():.2e}, max |dlnZ| {dl.max():.2e}; "
+            f"persistent c128 vs per-sector: max |dlnZ| {dlp.max():.2e}")
+    if cpu:
+        out["cpu_baseline"] = gc_cpu_baseline(R, M)
+    return out
+
+
+def gc_cpu_baseline(R, M, budget_s=12.0):
+    """The oracle's plain FTLM (reference arithmetic: complex128 push-form H*v, mt19937 starts, capped
+    Jacobi quadrature) on the largest 4x2 sector, one thread, timed per sample-step."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle, params8
+
+    orc = Oracle()
+    p8 = params8(4, 2, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0)
+    done, t_used = 0, 0.0
+    while t_used < budget_s:
+        t0 = time.perf_counter()
+        orc.ftlm_log_partition(p8, 4, 4, 1.0, 1, M, 7 + done)
+        t_used += time.perf_counter() - t0
+        done += 1
+    rate = done * M / t_used
+    return {"value": rate, "unit": "sample-steps/s", "cores": 1, "kind": "port",
+            "sample": f"oracle ftlm_log_partition_complex on 4x2 (4,4) (dim 4900), {done} samples x M={M} "
+                      f"in {t_used:.1f} s, 1 thread; per sample-step of the largest sector"}
+
+
 def traffic_from_profile():
     """HBM bytes per H*V (R=16 complex64) from the committed counter summary: the line
     'hbm_bytes_per_hv = <int>' (FETCH_SIZE + WRITE_SIZE of both passes, copy-calibrated)."""
@@ -250,8 +337,9 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["sector", "kblock"], default="sector",
-                    help="sector: the 4x4 (8,8) full-sector line (default); kblock: the momentum-block line")
+    ap.add_argument("--workload", choices=["sector", "kblock", "gc4x2"], default="sector",
+                    help="sector: the 4x4 (8,8) full-sector line (default); kblock: the 4x4 (8,8) momentum-block "
+                         "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch)")
     ap.add_argument("--profile-hv", action="store_true",
                     help="only run 3 block H*V (R vectors, complex64) and one 4 GiB device copy, for rocprofv3")
     args = ap.parse_args()
@@ -270,6 +358,9 @@ def main():
     K, W, R = args.steps, args.warmup, args.R
     if args.profile_hv:
         profile_hv(F, L, torch, R)
+        return
+    if args.workload == "gc4x2":
+        print(json.dumps(gc4x2_workload(F, L, torch, validate=not args.no_validate, cpu=not args.no_cpu)), flush=True)
         return
     if args.workload == "kblock":
         print(json.dumps(kblock_workload(F, L, torch, K, validate=not args.no_validate)), flush=True)

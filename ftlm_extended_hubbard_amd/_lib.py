@@ -17,6 +17,7 @@ FTLM_OK, FTLM_EINVAL, FTLM_ERANGE, FTLM_EHIP, FTLM_ENOMEM, FTLM_EUNSUPPORTED = r
 FTLM_C128, FTLM_C64 = 0, 1
 FTLM_START_GAUSS_MT, FTLM_START_PHASE = 0, 1
 FTLM_QUAD_REFERENCE, FTLM_QUAD_CONVERGED = 0, 1
+FTLM_PATH_AUTO, FTLM_PATH_PER_OP, FTLM_PATH_PERSISTENT = 0, 1, 2
 
 
 class FtlmHipError(RuntimeError):
@@ -40,7 +41,7 @@ class Pair(C.Structure):
 
 class FtlmOpts(C.Structure):
     _fields_ = [("R", C.c_int), ("M", C.c_int), ("seed", C.c_uint), ("start", C.c_int), ("dtype", C.c_int),
-                ("quad", C.c_int), ("clamp_M_to_dim", C.c_int), ("real_variant", C.c_int)]
+                ("quad", C.c_int), ("clamp_M_to_dim", C.c_int), ("real_variant", C.c_int), ("path", C.c_int)]
 
 
 # C struct layout (ftlm_hop_t is 32 bytes: three ints, 4 bytes of padding, two doubles)

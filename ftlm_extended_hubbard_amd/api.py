@@ -388,10 +388,11 @@ class FtlmParams:
     dtype: int = L.FTLM_C128
     quad: int = L.FTLM_QUAD_REFERENCE
     clamp_M_to_dim: bool = False
+    path: int = L.FTLM_PATH_AUTO
 
     def _c(self, real: bool = False) -> L.FtlmOpts:
         return L.FtlmOpts(self.n_random, self.lanczos_steps, self.seed, self.start, self.dtype, self.quad,
-                          int(self.clamp_M_to_dim), int(real))
+                          int(self.clamp_M_to_dim), int(real), int(self.path))
 
 
 def _log_partition(op: _Op, betas, par: FtlmParams, real: bool):

@@ -448,6 +448,126 @@ struct ftlm_krylov {
 
 using namespace ftlm_hip;
 
+constexpr int kMaxSamplesPerBlock = 32;  // recurrences of one Krylov block (R of ftlm_hip_apply)
+
+namespace ftlm_hip {
+// every (sector, sample) recurrence in one persistent launch (batched.hip)
+void lanczos_batched(const std::vector<const Sector*>& sectors, const std::vector<int>& Ms, int R, unsigned seed,
+                     int start, int dtype, bool force_real, const std::vector<std::vector<double2>>* q0_host,
+                     double* d_alpha, double* d_beta, int* d_used, int* d_alive, hipStream_t st);
+}  // namespace ftlm_hip
+
+namespace {
+
+// ftlm_hip_ftlm_ritz for a list of small full sectors through one persistent launch.
+void ritz_persistent(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, double* ritz, double* weight,
+                     int* used, double* alpha_out, double* beta_out, hipStream_t st) {
+  const int R = std::max(1, opts->R), M = std::max(1, opts->M);
+  std::vector<const Sector*> secs;
+  std::vector<int> Ms, which;  // which[s] = operator index of batch sector s
+  for (int io = 0; io < n_ops; ++io) {
+    const ftlm_op* op = ops[io];
+    const int64_t dim = op->dim;
+    double* rz = ritz + static_cast<int64_t>(io) * R * M;
+    double* wt = weight + static_cast<int64_t>(io) * R * M;
+    int* us = used + static_cast<int64_t>(io) * R;
+    std::fill(rz, rz + static_cast<int64_t>(R) * M, 0.0);
+    std::fill(wt, wt + static_cast<int64_t>(R) * M, 0.0);
+    std::fill(us, us + R, 0);
+    if (alpha_out) std::fill(alpha_out + static_cast<int64_t>(io) * R * M, alpha_out + static_cast<int64_t>(io + 1) * R * M, 0.0);
+    if (beta_out) std::fill(beta_out + static_cast<int64_t>(io) * R * M, beta_out + static_cast<int64_t>(io + 1) * R * M, 0.0);
+    if (dim <= 0) continue;
+    if (dim == 1) {  // a single state: ln Z = -beta H00 (src/ftlm_thermo.cpp:155-160 / 219-224)
+      DevBuf<double2> x(1), y(1);
+      const double2 one = make_double2(1.0, 0.0);
+      x.upload(&one, 1, st);
+      hv_apply(op, x.p, y.p, 1, FTLM_C128, nullptr, nullptr, st, opts->real_variant != 0);
+      double2 h;
+      FTLM_HIP_CHECK(hipMemcpyAsync(&h, y.p, sizeof(double2), hipMemcpyDeviceToHost, st));
+      FTLM_HIP_CHECK(hipStreamSynchronize(st));
+      for (int r = 0; r < R; ++r) {
+        rz[static_cast<int64_t>(r) * M] = h.x;
+        wt[static_cast<int64_t>(r) * M] = 1.0;
+        us[r] = 1;
+      }
+      continue;
+    }
+    secs.push_back(static_cast<const Sector*>(op));
+    Ms.push_back(opts->clamp_M_to_dim ? static_cast<int>(std::min<int64_t>(M, dim)) : M);
+    which.push_back(io);
+  }
+  if (secs.empty()) return;
+  const int ns = static_cast<int>(secs.size()), n_out = ns * R;
+  std::vector<std::vector<double2>> q0;
+  if (opts->start == FTLM_START_GAUSS_MT) {  // the reference's mt19937 Gaussian starts, per sample
+    q0.resize(static_cast<size_t>(ns));
+    std::vector<std::complex<double>> q;
+    for (int s = 0; s < ns; ++s) {
+      const int64_t dim = secs[s]->dim;
+      q0[s].resize(static_cast<size_t>(R) * dim);
+      for (int r = 0; r < R; ++r) {
+        gauss_mt(dim, opts->seed + static_cast<unsigned>(r) * 100003u, &q);
+        for (int64_t i = 0; i < dim; ++i)
+          q0[s][static_cast<size_t>(r) * dim + i] = make_double2(q[i].real(), q[i].imag());
+      }
+    }
+  }
+  DevBuf<double> da(static_cast<size_t>(M) * n_out), db(static_cast<size_t>(M) * n_out);
+  DevBuf<int> du(static_cast<size_t>(n_out)), dal(static_cast<size_t>(n_out));
+  FTLM_HIP_CHECK(hipMemsetAsync(da.p, 0, da.n * sizeof(double), st));
+  FTLM_HIP_CHECK(hipMemsetAsync(db.p, 0, db.n * sizeof(double), st));
+  FTLM_HIP_CHECK(hipMemsetAsync(du.p, 0, du.n * sizeof(int), st));
+  FTLM_HIP_CHECK(hipMemsetAsync(dal.p, 0, dal.n * sizeof(int), st));
+  lanczos_batched(secs, Ms, R, opts->seed, opts->start, opts->dtype, opts->real_variant != 0,
+                  opts->start == FTLM_START_GAUSS_MT ? &q0 : nullptr, da.p, db.p, du.p, dal.p, st);
+  std::vector<int> hu(static_cast<size_t>(n_out)), ha(static_cast<size_t>(n_out)), hnb(static_cast<size_t>(n_out));
+  FTLM_HIP_CHECK(hipMemcpy(hu.data(), du.p, sizeof(int) * n_out, hipMemcpyDeviceToHost));
+  FTLM_HIP_CHECK(hipMemcpy(ha.data(), dal.p, sizeof(int) * n_out, hipMemcpyDeviceToHost));
+  for (int i = 0; i < n_out; ++i) hnb[i] = ha[i] ? hu[i] : hu[i] - 1;
+  DevBuf<int> dnb(static_cast<size_t>(n_out));
+  dnb.upload(hnb, st);
+  DevBuf<double> dr(static_cast<size_t>(n_out) * M), dw(static_cast<size_t>(n_out) * M);
+  FTLM_HIP_CHECK(hipMemsetAsync(dr.p, 0, dr.n * sizeof(double), st));
+  FTLM_HIP_CHECK(hipMemsetAsync(dw.p, 0, dw.n * sizeof(double), st));
+  quadrature_device(da.p, db.p, du.p, dnb.p, n_out, n_out, M, opts->quad, 0, 1e-14, 1, dr.p, dw.p, M, st);
+  std::vector<double> hr(static_cast<size_t>(n_out) * M), hw(static_cast<size_t>(n_out) * M);
+  FTLM_HIP_CHECK(hipMemcpy(hr.data(), dr.p, sizeof(double) * hr.size(), hipMemcpyDeviceToHost));
+  FTLM_HIP_CHECK(hipMemcpy(hw.data(), dw.p, sizeof(double) * hw.size(), hipMemcpyDeviceToHost));
+  std::vector<double> ha_(alpha_out ? static_cast<size_t>(M) * n_out : 0), hb_(beta_out ? static_cast<size_t>(M) * n_out : 0);
+  if (alpha_out) FTLM_HIP_CHECK(hipMemcpy(ha_.data(), da.p, sizeof(double) * ha_.size(), hipMemcpyDeviceToHost));
+  if (beta_out) FTLM_HIP_CHECK(hipMemcpy(hb_.data(), db.p, sizeof(double) * hb_.size(), hipMemcpyDeviceToHost));
+  for (int s = 0; s < ns; ++s) {
+    const int io = which[s];
+    for (int r = 0; r < R; ++r) {
+      const int o = s * R + r;
+      const int64_t dst = (static_cast<int64_t>(io) * R + r) * M;
+      std::copy(hr.begin() + static_cast<int64_t>(o) * M, hr.begin() + static_cast<int64_t>(o + 1) * M, ritz + dst);
+      std::copy(hw.begin() + static_cast<int64_t>(o) * M, hw.begin() + static_cast<int64_t>(o + 1) * M, weight + dst);
+      used[static_cast<int64_t>(io) * R + r] = hu[o];
+      for (int k = 0; k < M; ++k) {
+        if (alpha_out) alpha_out[dst + k] = ha_[static_cast<size_t>(k) * n_out + o];
+        if (beta_out) beta_out[dst + k] = hb_[static_cast<size_t>(k) * n_out + o];
+      }
+    }
+  }
+}
+
+bool persistent_eligible(ftlm_op_t* const* ops, int n_ops, int path) {
+  if (path == FTLM_PATH_PER_OP) return false;
+  const int64_t lim = path == FTLM_PATH_PERSISTENT ? 262144 : 16384;
+  for (int i = 0; i < n_ops; ++i) {
+    if (ops[i] == nullptr || ops[i]->kind != kSector || ops[i]->dim > lim) {
+      FTLM_REQUIRE(path != FTLM_PATH_PERSISTENT, FTLM_EUNSUPPORTED,
+                   "the persistent path takes full sectors of at most 262144 states");
+      return false;
+    }
+  }
+  return n_ops > 0;
+}
+
+}  // namespace
+
+
 extern "C" {
 
 int ftlm_hip_krylov_create(const ftlm_op_t* op, int R, int max_steps, int dtype, int start, unsigned seed,
@@ -544,15 +664,19 @@ int ftlm_hip_lanczos_extrema(const ftlm_op_t* op, int max_steps, unsigned seed, 
   return FTLM_OK;
 }
 
-constexpr int kMaxSamplesPerBlock = 32;  // recurrences of one Krylov block (R of ftlm_hip_apply)
-
 int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, double* ritz, double* weight,
                        int* used, double* alpha_out, double* beta_out, void* stream) {
   return guarded([&] {
     check_opts(opts);
     FTLM_REQUIRE(ops != nullptr && n_ops >= 1 && ritz && weight && used, FTLM_EINVAL, "null argument");
+    FTLM_REQUIRE(opts->path >= FTLM_PATH_AUTO && opts->path <= FTLM_PATH_PERSISTENT, FTLM_EINVAL, "bad path");
     const int R = std::max(1, opts->R), M = std::max(1, opts->M);
     hipStream_t st = static_cast<hipStream_t>(stream);
+    if (persistent_eligible(ops, n_ops, opts->path)) {
+      FTLM_HIP_CHECK(hipSetDevice(ops[0]->device));
+      ritz_persistent(ops, n_ops, opts, ritz, weight, used, alpha_out, beta_out, st);
+      return;
+    }
     for (int io = 0; io < n_ops; ++io) {
       const ftlm_op* op0 = ops[io];
       FTLM_REQUIRE(op0 != nullptr, FTLM_EINVAL, "null operator");

@@ -16,6 +16,7 @@
 //   the grand-canonical (beta, mu) mixture (src/ftlm_thermo.cpp:120-145, 195-210;
 //   benchmarks/bench_ftlm_nmu_rect.cpp:186-219).
 
+#include <algorithm>
 #include <cmath>
 
 #include "common.hpp"
@@ -215,20 +216,26 @@ __global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__
 // Converged eigen-decomposition of a symmetric tridiagonal (implicit QL, Wilkinson shift),
 // returning eigenvalues and squared first components of the eigenvectors.
 __global__ void k_ql(const double* __restrict__ alpha, const double* __restrict__ beta, const int* __restrict__ nvec,
-                     const int* __restrict__ nbvec, int stride, int nsamp, double* __restrict__ work,
-                     int64_t work_per, double* __restrict__ ritz, double* __restrict__ weight, int out_stride,
-                     int* __restrict__ status) {
+                     const int* __restrict__ nbvec, int stride, int nsamp, int max_n, double* __restrict__ ritz,
+                     double* __restrict__ weight, int out_stride, int* __restrict__ status) {
+  // d, e and the first eigenvector row z of every thread's tridiagonal live in LDS, interleaved
+  // by thread ([i * blockDim + tid]), so the sequential QL sweeps never wait on global memory
+  extern __shared__ double ql_lds[];
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nt = blockDim.x, t = threadIdx.x;
+  double* dl = ql_lds;
+  double* el = ql_lds + static_cast<int64_t>(max_n) * nt;
+  double* zl = ql_lds + 2 * static_cast<int64_t>(max_n) * nt;
+#define QD(i) dl[(i) * nt + t]
+#define QE(i) el[(i) * nt + t]
+#define QZ(i) zl[(i) * nt + t]
   if (s >= nsamp) return;
   const int n = nvec[s];
   const int nb = nbvec ? nbvec[s] : n - 1;
-  double* d = ritz + static_cast<int64_t>(s) * out_stride;
-  double* z = weight + static_cast<int64_t>(s) * out_stride;
-  double* e = work + work_per * s;
   for (int i = 0; i < n; ++i) {
-    d[i] = alpha[static_cast<int64_t>(i) * stride + s];
-    e[i] = (i < n - 1 && i < nb) ? beta[static_cast<int64_t>(i) * stride + s] : 0.0;
-    z[i] = (i == 0) ? 1.0 : 0.0;
+    QD(i) = alpha[static_cast<int64_t>(i) * stride + s];
+    QE(i) = (i < n - 1 && i < nb) ? beta[static_cast<int64_t>(i) * stride + s] : 0.0;
+    QZ(i) = (i == 0) ? 1.0 : 0.0;
   }
   int bad = 0;
   for (int l = 0; l < n; ++l) {
@@ -236,49 +243,57 @@ __global__ void k_ql(const double* __restrict__ alpha, const double* __restrict_
     while (true) {
       int m = l;
       for (; m < n - 1; ++m) {
-        const double dd = fabs(d[m]) + fabs(d[m + 1]);
-        if (fabs(e[m]) <= 1e-300 + 2.220446049250313e-16 * dd) break;
+        const double dd = fabs(QD(m)) + fabs(QD(m + 1));
+        if (fabs(QE(m)) <= 1e-300 + 2.220446049250313e-16 * dd) break;
       }
       if (m == l) break;
       if (++iter > 200) {
         bad = 1;
         break;
       }
-      double g = (d[l + 1] - d[l]) / (2.0 * e[l]);
+      double g = (QD(l + 1) - QD(l)) / (2.0 * QE(l));
       double r = hypot(g, 1.0);
-      g = d[m] - d[l] + e[l] / (g + (g >= 0.0 ? r : -r));
+      g = QD(m) - QD(l) + QE(l) / (g + (g >= 0.0 ? r : -r));
       double sn = 1.0, c = 1.0, p = 0.0;
       int i = m - 1;
       bool deflated = false;
       for (; i >= l; --i) {
-        double f = sn * e[i];
-        const double b = c * e[i];
+        double f = sn * QE(i);
+        const double b = c * QE(i);
         r = hypot(f, g);
-        e[i + 1] = r;
+        QE(i + 1) = r;
         if (r == 0.0) {
-          d[i + 1] -= p;
-          e[m] = 0.0;
+          QD(i + 1) -= p;
+          QE(m) = 0.0;
           deflated = true;
           break;
         }
         sn = f / r;
         c = g / r;
-        g = d[i + 1] - p;
-        r = (d[i] - g) * sn + 2.0 * c * b;
+        g = QD(i + 1) - p;
+        r = (QD(i) - g) * sn + 2.0 * c * b;
         p = sn * r;
-        d[i + 1] = g + p;
+        QD(i + 1) = g + p;
         g = c * r - b;
-        f = z[i + 1];
-        z[i + 1] = sn * z[i] + c * f;
-        z[i] = c * z[i] - sn * f;
+        f = QZ(i + 1);
+        QZ(i + 1) = sn * QZ(i) + c * f;
+        QZ(i) = c * QZ(i) - sn * f;
       }
       if (deflated) continue;
-      d[l] -= p;
-      e[l] = g;
-      e[m] = 0.0;
+      QD(l) -= p;
+      QE(l) = g;
+      QE(m) = 0.0;
     }
   }
-  for (int i = 0; i < n; ++i) z[i] = z[i] * z[i];
+  double* d = ritz + static_cast<int64_t>(s) * out_stride;
+  double* z = weight + static_cast<int64_t>(s) * out_stride;
+  for (int i = 0; i < n; ++i) {
+    d[i] = QD(i);
+    z[i] = QZ(i) * QZ(i);
+  }
+#undef QD
+#undef QE
+#undef QZ
   if (bad) atomicAdd(status, 1);
 }
 
@@ -412,11 +427,19 @@ void quadrature_device(const double* d_alpha, const double* d_beta, const int* d
                        double* d_weight, int out_stride, hipStream_t st) {
   if (nsamp <= 0 || max_n <= 0) return;
   if (quad == FTLM_QUAD_CONVERGED) {
-    DevBuf<double> work(static_cast<size_t>(nsamp) * max_n);
     DevBuf<int> status(1);
     FTLM_HIP_CHECK(hipMemsetAsync(status.p, 0, sizeof(int), st));
-    hipLaunchKernelGGL(k_ql, dim3(grid_for(nsamp, 64)), dim3(64), 0, st, d_alpha, d_beta, d_n, d_nb, stride, nsamp,
-                       work.p, static_cast<int64_t>(max_n), d_ritz, d_weight, out_stride, status.p);
+    // threads per workgroup: as many tridiagonals as fit 3 * max_n doubles each in the LDS budget
+    int dev = 0, lds_max = 0;
+    FTLM_HIP_CHECK(hipGetDevice(&dev));
+    FTLM_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const size_t budget = std::min<size_t>(120 * 1024, static_cast<size_t>(lds_max));
+    int nt = 64;
+    while (nt > 1 && static_cast<size_t>(nt) * 3 * max_n * sizeof(double) > budget) nt /= 2;
+    const size_t shmem = static_cast<size_t>(nt) * 3 * max_n * sizeof(double);
+    FTLM_REQUIRE(shmem <= budget, FTLM_EUNSUPPORTED, "tridiagonal too large for the QL kernel");
+    hipLaunchKernelGGL(k_ql, dim3(grid_for(nsamp, nt)), dim3(nt), shmem, st, d_alpha, d_beta, d_n, d_nb, stride, nsamp,
+                       max_n, d_ritz, d_weight, out_stride, status.p);
     check_launch("k_ql");
     int bad = 0;
     FTLM_HIP_CHECK(hipMemcpyAsync(&bad, status.p, sizeof(int), hipMemcpyDeviceToHost, st));

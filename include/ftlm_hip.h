@@ -149,7 +149,13 @@ typedef struct ftlm_ftlm_opts {
   int clamp_M_to_dim; /* 1: M_op = min(M, dim) */
   int real_variant;   /* 1: ftlm_log_partition_real semantics (Re/Im applied separately,
                          src/ftlm_thermo.cpp:150-211) -- identical for real H */
+  int path;           /* FTLM_PATH_*: how the recurrences of several operators are scheduled */
 } ftlm_ftlm_opts_t;
+
+#define FTLM_PATH_AUTO 0       /* persistent launch when every operator is a sector of dim <= 16384 */
+#define FTLM_PATH_PER_OP 1     /* per operator, R recurrences per block H*V */
+#define FTLM_PATH_PERSISTENT 2 /* every (sector, sample) recurrence in one persistent launch (sectors
+                                  of dim <= 262144) */
 
 /* Runs R independent Lanczos recurrences per operator, all on the device, and diagonalises every
  * tridiagonal with the chosen quadrature eigensolver.  Outputs (host):

@@ -313,3 +313,22 @@ def test_ftlm_more_than_one_sample_block(golden):
     assert np.array_equal(rd.ritz[0, 32:], tail.ritz[0]) and np.array_equal(rd.weight[0, 32:], tail.weight[0])
     head = api.ftlm_ritz([op], api.FtlmParams(n_random=32, lanczos_steps=12, seed=5))
     assert np.array_equal(rd.ritz[0, :32], head.ritz[0])
+
+
+def test_persistent_path_matches_per_operator(golden):
+    """All 81 sectors of the 4x2 lattice in one persistent launch give the per-operator path's
+    Ritz data (complex128, the reference's Gaussian starts), and complex64 storage stays within
+    1e-6 of it in lnZ."""
+    say("persistent vs per-op, 81 sectors")
+    p = hp(golden, "c2_4x2")
+    ops = [api.HubbardOperator.from_params(p, nu, nd) for nu in range(9) for nd in range(9)]
+    kw = dict(n_random=3, lanczos_steps=24, seed=11, clamp_M_to_dim=True)
+    a = api.ftlm_ritz(ops, api.FtlmParams(path=L.FTLM_PATH_PERSISTENT, **kw), keep_tridiagonals=True)
+    b = api.ftlm_ritz(ops, api.FtlmParams(path=L.FTLM_PATH_PER_OP, **kw), keep_tridiagonals=True)
+    assert np.array_equal(a.used, b.used)
+    assert np.max(np.abs(a.alpha[:, :, :6] - b.alpha[:, :, :6])) < 1e-10
+    betas = np.array([0.5, 2.0])
+    la, lb = api.ftlm_thermo(a, betas)["lnZ"], api.ftlm_thermo(b, betas)["lnZ"]
+    assert np.max(np.abs(la - lb)) < 1e-8
+    c = api.ftlm_ritz(ops, api.FtlmParams(path=L.FTLM_PATH_PERSISTENT, dtype=L.FTLM_C64, **kw))
+    assert np.max(np.abs(api.ftlm_thermo(c, betas)["lnZ"] - lb)) < 1e-5
