@@ -248,6 +248,74 @@ def gc_cpu_baseline(R, M, budget_s=12.0):
                       f"in {t_used:.1f} s, 1 thread; per sample-step of the largest sector"}
 
 
+def grad_workload(F, L, torch, validate=True):
+    """Differentiated FTLM: d lnZ(beta) / d(t, t', U, V, mu, phi_x), beta in {0.5, 1, 2, 4, 8}, on the
+    4x3 (6,6) sector (dim 853,776; t=1, t'=-0.25, U=8, V=1, mu=U/2+2V, phi_x=0.05) with R=32 and
+    M=100 (forward-mode Lanczos, converged Gauss rule), then on the 4x4 (8,8) K=(0,0) momentum block
+    (phi=0, R=8).  Central differences (h=1e-5, same seeds) check the gradients."""
+    import numpy as np
+
+    betas = np.array([0.5, 1.0, 2.0, 4.0, 8.0])
+    U, V = 8.0, 1.0
+    base = dict(Lx=4, Ly=3, t=1.0, tp=-0.25, U=U, V=V, phi_x=0.05, phi_y=0.0)
+    mu = U / 2 + 2 * V
+    par = F.FtlmParams(n_random=32, lanczos_steps=100, seed=5, start=L.FTLM_START_PHASE,
+                       quad=L.FTLM_QUAD_CONVERGED)
+    names = ("t", "tp", "U", "V", "phi_x")
+    p = F.HubbardParams(**base)
+    F.ftlm_log_partition_gradient(p, 6, 6, betas[:1], F.FtlmParams(n_random=2, lanczos_steps=4), params=names)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lnz, g = F.ftlm_log_partition_gradient(p, 6, 6, betas, par, params=names + ("mu",), mu=mu)
+    t_grad = time.perf_counter() - t0
+    op = F.HubbardOperator.from_params(p, 6, 6)
+    t0 = time.perf_counter()
+    plain = F.ftlm_log_partition_complex(op, betas, par)
+    t_plain = time.perf_counter() - t0
+    log(f"4x3 (6,6) gradient (5 parameters + mu): {t_grad:.2f} s; lnZ alone {t_plain:.2f} s")
+    out = {"metric": "differentiated FTLM: d lnZ / d(t, t', U, V, mu, phi_x), 4x3 (6,6), R=32, M=100",
+           "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic", "dtype": "c128",
+           "config": {"workload": "4x3 (6,6) FTLM gradient", "hamiltonian": base, "mu": mu, "R": 32, "M": 100,
+                      "betas": betas.tolist()},
+           "value": round(t_grad, 3), "lnZ_only_s": round(t_plain, 3),
+           "lnZ": lnz.tolist(), "grad": {k: v.tolist() for k, v in g.items()},
+           "observables": {  # -(1/beta) d lnZ / d theta
+               "double_occupancy": (-g["U"] / betas).tolist(), "nn_density": (-g["V"] / betas).tolist(),
+               "kinetic_t": (-g["t"] / betas * base["t"]).tolist(), "N": (g["mu"] / betas).tolist()}}
+    if validate:
+        h, fd_err = 1e-5, {}
+        for name in names:
+            vals = []
+            for sgn in (+1, -1):
+                q = dict(base)
+                q[name] += sgn * h
+                o = F.HubbardOperator.from_params(F.HubbardParams(**q), 6, 6)
+                vals.append(F.ftlm_log_partition_complex(o, betas, par))
+                del o
+            fd = (vals[0] - vals[1]) / (2 * h)
+            fd_err[name] = float(np.max(np.abs(fd - g[name]) / np.maximum(np.abs(fd), 1e-3)))
+        out["fd_max_rel_err"] = fd_err
+        log(f"central differences, max relative error: {fd_err}")
+    # the 4x4 (8,8) K = (0,0) block, phi = 0
+    pk = F.HubbardParams(4, 4, 1.0, -0.25, U, V, 0.0, 0.0)
+    K = F.MomentumSector(0, 0, 4, 4)
+    kn = ("t", "tp", "U", "V")
+    t0 = time.perf_counter()
+    B = F.KBlockOperator.from_params(pk, NU, ND, K)
+    dB = [F.derivative_operator(pk, NU, ND, q, K) for q in kn]
+    torch.cuda.synchronize()
+    t_build = time.perf_counter() - t0
+    pk_par = F.FtlmParams(n_random=8, lanczos_steps=100, seed=5, start=L.FTLM_START_PHASE, quad=L.FTLM_QUAD_CONVERGED)
+    t0 = time.perf_counter()
+    lk, gk = F.ftlm_gradient(B, dB, betas, pk_par)
+    t_k = time.perf_counter() - t0
+    log(f"4x4 (8,8) K=(0,0) block (dim {B.dim()}): gradient (t, t', U, V) R=8 M=100 {t_k:.2f} s, "
+        f"operators {t_build:.2f} s")
+    out["kblock_4x4_K00"] = {"dim": B.dim(), "R": 8, "M": 100, "seconds": round(t_k, 3), "build_s": round(t_build, 3),
+                             "lnZ": lk.tolist(), "grad": {n: gk[i].tolist() for i, n in enumerate(kn)}}
+    return out
+
+
 def traffic_from_profile():
     """HBM bytes per H*V (R=16 complex64) from the committed counter summary: the line
     'hbm_bytes_per_hv = <int>' (FETCH_SIZE + WRITE_SIZE of both passes, copy-calibrated)."""
@@ -337,9 +405,10 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["sector", "kblock", "gc4x2"], default="sector",
+    ap.add_argument("--workload", choices=["sector", "kblock", "gc4x2", "grad"], default="sector",
                     help="sector: the 4x4 (8,8) full-sector line (default); kblock: the 4x4 (8,8) momentum-block "
-                         "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch)")
+                         "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch); grad: "
+                         "the differentiated FTLM line (4x3 (6,6) and the 4x4 (8,8) K=(0,0) block)")
     ap.add_argument("--profile-hv", action="store_true",
                     help="only run 3 block H*V (R vectors, complex64) and one 4 GiB device copy, for rocprofv3")
     args = ap.parse_args()
@@ -358,6 +427,9 @@ def main():
     K, W, R = args.steps, args.warmup, args.R
     if args.profile_hv:
         profile_hv(F, L, torch, R)
+        return
+    if args.workload == "grad":
+        print(json.dumps(grad_workload(F, L, torch, validate=not args.no_validate)), flush=True)
         return
     if args.workload == "gc4x2":
         print(json.dumps(gc4x2_workload(F, L, torch, validate=not args.no_validate, cpu=not args.no_cpu)), flush=True)

@@ -28,8 +28,12 @@ from .api import (
     build_hubbard_geometry,
     canonical_states,
     device_count,
+    derivative_operator,
     exact_log_partition,
+    ftlm_gradient,
     ftlm_log_partition_complex,
+    ftlm_log_partition_gradient,
+    geometry_derivative,
     ftlm_log_partition_real,
     ftlm_ritz,
     ftlm_thermo,
@@ -49,5 +53,6 @@ __all__ = [
     "HubbardOperator", "apply_extended_hubbard", "LanczosExtrema", "lanczos_tridiagonal", "lanczos_extrema",
     "tridiagonal_extrema", "FtlmParams", "ftlm_log_partition_complex", "ftlm_log_partition_real", "RitzData",
     "ftlm_ritz", "ftlm_thermo", "quadrature_ritz", "device_count", "KrylovSession", "MomentumSector", "KBlockOperator", "OrbitKRepEntries", "pack_raw_state",
-    "translate_states", "canonical_states", "translation_bloch_phase", "exact_log_partition", "_lib",
+    "translate_states", "canonical_states", "translation_bloch_phase", "exact_log_partition", "geometry_derivative", "derivative_operator",
+    "ftlm_gradient", "ftlm_log_partition_gradient", "_lib",
 ]

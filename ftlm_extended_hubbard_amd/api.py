@@ -427,6 +427,73 @@ def exact_log_partition(op: _Op, betas, return_evals: bool = False):
     return (res, ev[: 2 * op.dim()]) if return_evals else res
 
 
+# ---------------------------------------------------------------- derivatives
+
+GRAD_PARAMS = ("t", "tp", "U", "V", "phi_x", "phi_y")
+
+
+def geometry_derivative(p: HubbardParams, which: str):
+    """d(coefficient)/d(theta) of every hop of build_hubbard_geometry, theta in t, tp, phi_x, phi_y."""
+    code = {"t": 0, "tp": 1, "phi_x": 2, "phi_y": 3}[which]
+    n = max(p.Lx * p.Ly, 1)
+    cap = 64 * n + 64
+    hops = np.zeros(cap, L.HOP_DTYPE)
+    nh = C.c_int(0)
+    L.check(L.lib().ftlm_hip_build_geometry_derivative(C.byref(p._c()), code, hops.ctypes.data, cap, C.byref(nh)))
+    return hops[: nh.value].copy()
+
+
+def derivative_operator(p: HubbardParams, n_up: int, n_dn: int, which: str,
+                        K: "MomentumSector | None" = None) -> "_Op":
+    """dH/dtheta on the (n_up, n_dn) sector, or on its momentum-K block, as an operator: derivative
+    hops for t, tp, phi_x, phi_y; the double-occupancy (U) or nearest-neighbour density (V) term for
+    U, V.  dH commutes with the lattice translations, so its K block is the block of dH."""
+    n = p.Lx * p.Ly
+    no_hops = np.zeros(0, L.HOP_DTYPE)
+    if which == "U":
+        hops, pairs, U, V = no_hops, None, 1.0, 0.0
+    elif which == "V":
+        hops, (_, pairs), U, V = no_hops, build_hubbard_geometry(p), 0.0, 1.0
+    else:
+        hops, pairs, U, V = geometry_derivative(p, which), None, 0.0, 0.0
+    if K is None:
+        return HubbardOperator._create(n, n_up, n_dn, hops if len(hops) else None, pairs, U, V)
+    K = MomentumSector(K.kx, K.ky, p.Lx, p.Ly)
+    pairs = np.zeros(0, L.PAIR_DTYPE) if pairs is None else pairs
+    return KBlockOperator._create(p.Lx, p.Ly, n_up, n_dn, K, hops, pairs, U, V)
+
+
+def ftlm_gradient(op: _Op, dops, betas, par: FtlmParams):
+    """lnZ(beta) of the FTLM estimator with the converged Gauss rule and its exact derivatives
+    along the operators dops (dH/dtheta): returns (lnZ [nb], grad [len(dops)][nb])."""
+    b = np.ascontiguousarray(np.atleast_1d(betas), np.float64)
+    nb, npar = len(b), len(dops)
+    lnz, g = np.zeros(nb), np.zeros((max(npar, 1), nb))
+    arr = (C.c_void_p * max(npar, 1))(*[d.handle.value for d in dops]) if npar else None
+    L.check(L.lib().ftlm_hip_ftlm_gradient(op.handle, arr, npar, C.byref(par._c()), b.ctypes.data, nb,
+                                           lnz.ctypes.data, g.ctypes.data))
+    return lnz, g[:npar]
+
+
+def ftlm_log_partition_gradient(p: HubbardParams, n_up: int, n_dn: int, betas, par: FtlmParams,
+                                params=("t", "tp", "U", "V", "phi_x"), mu: float | None = None,
+                                K: "MomentumSector | None" = None):
+    """lnZ(beta) of the sector (or of its momentum-K block) and d lnZ / d theta for theta in params
+    (any of t, tp, U, V, phi_x, phi_y, mu).  With mu, H - mu N is used: lnZ gains beta mu N and
+    d lnZ / d mu = beta N."""
+    op = HubbardOperator.from_params(p, n_up, n_dn) if K is None else KBlockOperator.from_params(p, n_up, n_dn, K)
+    hop = [q for q in params if q != "mu"]
+    dops = [derivative_operator(p, n_up, n_dn, q, K) for q in hop]
+    b = np.atleast_1d(np.asarray(betas, np.float64))
+    lnz, g = ftlm_gradient(op, dops, b, par)
+    out = {q: g[i] for i, q in enumerate(hop)}
+    if mu is not None:
+        lnz = lnz + b * mu * (n_up + n_dn)
+    if "mu" in params:
+        out["mu"] = b * (n_up + n_dn)
+    return lnz, out
+
+
 @dataclass
 class RitzData:
     ritz: np.ndarray     # [n_ops, R, M]

@@ -451,6 +451,56 @@ int ftlm_hip_device_count(int* count) {
 
 int ftlm_hip_nsites(const ftlm_params_t* p) { return p ? p->Lx * p->Ly : 0; }
 
+// d(coefficient)/d(theta) of every entry build_hubbard_geometry emits, same traversal and order
+// (so duplicate bonds merge the same way).  theta: 0 = t, 1 = t', 2 = phi_x, 3 = phi_y.  A bond
+// whose own amplitude is zero is still emitted for d/dt and d/dt' (the operator it multiplies).
+void geometry_derivative(const ftlm_params_t& p, int which, std::vector<ftlm_hop_t>* hops) {
+  hops->clear();
+  auto bond = [&](int a, int b, int wx, int wy, double ts, bool nn) {
+    const double wphase = static_cast<double>(wx) * p.phi_x + static_cast<double>(wy) * p.phi_y;
+    const std::complex<double> ph = std::polar(1.0, wphase);
+    std::complex<double> f, r;  // d/dtheta of -ts * ph and of -ts * conj(ph)
+    if (which == 0 || which == 1) {
+      if ((which == 0) != nn) return;
+      f = -ph;
+      r = -std::conj(ph);
+    } else {
+      const double w = which == 2 ? wx : wy;
+      if (std::abs(ts) < 1e-15 || w == 0.0) return;
+      const std::complex<double> I(0.0, 1.0);
+      f = -ts * (I * w) * ph;
+      r = -ts * (-I * w) * std::conj(ph);
+    }
+    for (int s = 0; s < 2; ++s) {
+      hops->push_back(ftlm_hop_t{a, b, s, f.real(), f.imag()});
+      hops->push_back(ftlm_hop_t{b, a, s, r.real(), r.imag()});
+    }
+  };
+  const int Lx = p.Lx, Ly = p.Ly;
+  for (int y = 0; y < Ly; ++y)
+    for (int x = 0; x < Lx; ++x) {
+      const int i = y * Lx + x;
+      if (Lx > 1) {
+        const int wx = (x + 1 >= Lx) ? 1 : 0;
+        const int j = y * Lx + (x + 1) % Lx;
+        if (i != j && !(std::abs(p.t) < 1e-15 && which >= 2)) bond(i, j, wx, 0, p.t, true);
+      }
+      if (Ly > 1) {
+        const int wy = (y + 1 >= Ly) ? 1 : 0;
+        const int j = ((y + 1) % Ly) * Lx + x;
+        if (i != j && !(std::abs(p.t) < 1e-15 && which >= 2)) bond(i, j, 0, wy, p.t, true);
+      }
+      static const int dirs[4][2] = {{1, 1}, {1, -1}, {-1, 1}, {-1, -1}};
+      for (const auto& dv : dirs) {
+        int nx = x + dv[0], ny = y + dv[1], wx = 0, wy = 0;
+        if (nx >= Lx) { nx -= Lx; wx = 1; } else if (nx < 0) { nx += Lx; wx = -1; }
+        if (ny >= Ly) { ny -= Ly; wy = 1; } else if (ny < 0) { ny += Ly; wy = -1; }
+        const int j = ny * Lx + nx;
+        if (i != j) bond(i, j, wx, wy, p.tp, false);
+      }
+    }
+}
+
 int ftlm_hip_build_geometry(const ftlm_params_t* p, ftlm_hop_t* hops, int hop_cap, int* n_hops, ftlm_pair_t* pairs,
                             int pair_cap, int* n_pairs) {
   return guarded([&] {
@@ -465,6 +515,20 @@ int ftlm_hip_build_geometry(const ftlm_params_t* p, ftlm_hop_t* hops, int hop_ca
                  "geometry buffers too small");
     if (!h.empty()) std::memcpy(hops, h.data(), h.size() * sizeof(ftlm_hop_t));
     if (!q.empty()) std::memcpy(pairs, q.data(), q.size() * sizeof(ftlm_pair_t));
+  });
+}
+
+int ftlm_hip_build_geometry_derivative(const ftlm_params_t* p, int which, ftlm_hop_t* hops, int hop_cap,
+                                       int* n_hops) {
+  return guarded([&] {
+    FTLM_REQUIRE(p != nullptr && n_hops != nullptr, FTLM_EINVAL, "null argument");
+    FTLM_REQUIRE(which >= 0 && which <= 3, FTLM_EINVAL, "which must be 0 (t), 1 (t'), 2 (phi_x) or 3 (phi_y)");
+    FTLM_REQUIRE(p->Lx >= 1 && p->Ly >= 1 && p->Lx * p->Ly <= kMaxSites, FTLM_EINVAL, "bad lattice");
+    std::vector<ftlm_hop_t> h;
+    geometry_derivative(*p, which, &h);
+    *n_hops = static_cast<int>(h.size());
+    FTLM_REQUIRE(static_cast<int>(h.size()) <= hop_cap, FTLM_ERANGE, "hop buffer too small");
+    if (!h.empty()) std::memcpy(hops, h.data(), h.size() * sizeof(ftlm_hop_t));
   });
 }
 

@@ -211,6 +211,23 @@ int ftlm_hip_krylov_destroy(ftlm_krylov_t* k);
  * receives the Jacobi diagonal: every eigenvalue of H twice, unsorted. */
 int ftlm_hip_exact_log_partition(const ftlm_op_t* op, const double* betas, int nb, double* lnZ, double* evals);
 
+/* ------------------------------------------------------------------ derivatives */
+/* d(coefficient)/d(theta) of every entry of build_hubbard_geometry (same order):
+ * which = 0 (t), 1 (t'), 2 (phi_x), 3 (phi_y).  With U' and V' set to 0 this gives the hop part
+ * of dH/dtheta; dH/dU and dH/dV are the sector operators with no hops and (U, V) = (1, 0) / (0, 1).
+ * No counterpart in the reference (its parameters enter src/lattice.cpp:38-110). */
+int ftlm_hip_build_geometry_derivative(const ftlm_params_t* p, int which, ftlm_hop_t* hops, int hop_cap,
+                                       int* n_hops);
+
+/* lnZ(beta) of the FTLM estimator (ftlm_log_partition_complex, src/ftlm_thermo.cpp:213-256, with
+ * the converged Gauss rule) and its exact derivatives dlnZ/dtheta_p for npar parameters whose
+ * dH/dtheta_p are given as operators of the same size (forward-mode differentiation of the
+ * Lanczos recurrence and of the quadrature).  opts: R <= 32 samples (seed + r*100003, Gaussian or
+ * random-phase starts), M <= 512 steps, complex128.  grad is [npar][nb].  No counterpart in the
+ * reference, which has no derivatives. */
+int ftlm_hip_ftlm_gradient(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
+                           const double* betas, int nb, double* lnZ, double* grad);
+
 /* ------------------------------------------------------------------ host-callback operators */
 /* The operator argument of lanczos_tridiagonal / lanczos_extrema / ftlm_log_partition_complex
  * (std::function over host complex<double> arrays, include/ftlm/lanczos.hpp:47-63,

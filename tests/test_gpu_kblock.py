@@ -252,3 +252,29 @@ def test_small_sector_ed(oracle, golden):
     got = api.exact_log_partition(op, np.array([0.5, 3.0]))
     want = [oracle.exact_log_partition(H, b) for b in (0.5, 3.0)]
     assert np.max(np.abs(got - want)) < 1e-9
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd,kx,ky,phi", [(4, 2, 3, 3, 1, 0, 0.05), (4, 2, 4, 4, 0, 0, 0.0)])
+def test_kblock_gradient_vs_central_differences(lx, ly, nu, nd, kx, ky, phi):
+    """d lnZ_K / d(t, t', U, V, phi_x) of one momentum block's FTLM estimator (same seeds,
+    converged Gauss rule) against central differences of the block's lnZ with h = 1e-5."""
+    say("kblock gradient FD", lx, ly, nu, nd, kx, ky, phi)
+    base = dict(Lx=lx, Ly=ly, t=1.0, tp=-0.25, U=8.0, V=1.0, phi_x=phi, phi_y=0.0)
+    K = api.MomentumSector(kx, ky, lx, ly)
+    betas = np.array([0.5, 2.0, 8.0])
+    par = api.FtlmParams(n_random=4, lanczos_steps=24, seed=9, quad=L.FTLM_QUAD_CONVERGED, clamp_M_to_dim=True)
+    names = ("t", "tp", "U", "V", "phi_x")
+    lnz, g = api.ftlm_log_partition_gradient(api.HubbardParams(**base), nu, nd, betas, par, params=names, K=K)
+    B = api.KBlockOperator.from_params(api.HubbardParams(**base), nu, nd, K)
+    assert np.max(np.abs(lnz - api.ftlm_log_partition_complex(B, betas, par))) < 1e-9
+    h = 1e-5
+    for name in names:
+        vals = []
+        for sgn in (+1, -1):
+            q = dict(base)
+            q[name] += sgn * h
+            vals.append(api.ftlm_log_partition_complex(
+                api.KBlockOperator.from_params(api.HubbardParams(**q), nu, nd, K), betas, par))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        err = np.abs(fd - g[name]) / np.maximum(np.abs(fd), 1e-3)
+        assert np.max(err) < 1e-6, (name, fd, g[name])

@@ -332,3 +332,31 @@ def test_persistent_path_matches_per_operator(golden):
     assert np.max(np.abs(la - lb)) < 1e-8
     c = api.ftlm_ritz(ops, api.FtlmParams(path=L.FTLM_PATH_PERSISTENT, dtype=L.FTLM_C64, **kw))
     assert np.max(np.abs(api.ftlm_thermo(c, betas)["lnZ"] - lb)) < 1e-5
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd,phi", [(4, 2, 3, 2, 0.05), (4, 2, 4, 4, 0.0), (2, 2, 2, 2, 0.3)])
+def test_gradient_vs_central_differences(lx, ly, nu, nd, phi):
+    """d lnZ / d(t, t', U, V, phi_x) of the FTLM estimator (same seeds, converged Gauss rule)
+    against central differences with h = 1e-5, and d lnZ / d mu = beta N."""
+    say("gradient FD", lx, ly, nu, nd, phi)
+    base = dict(Lx=lx, Ly=ly, t=1.0, tp=-0.25, U=8.0, V=1.0, phi_x=phi, phi_y=0.0)
+    betas = np.array([0.5, 2.0, 8.0])
+    par = api.FtlmParams(n_random=4, lanczos_steps=24, seed=9, quad=L.FTLM_QUAD_CONVERGED, clamp_M_to_dim=True)
+    names = ("t", "tp", "U", "V", "phi_x")
+    lnz, g = api.ftlm_log_partition_gradient(api.HubbardParams(**base), nu, nd, betas, par, params=names + ("mu",),
+                                             mu=0.7)
+    op0 = api.HubbardOperator.from_params(api.HubbardParams(**base), nu, nd)
+    ref = api.ftlm_log_partition_complex(op0, betas, par) + betas * 0.7 * (nu + nd)
+    assert np.max(np.abs(lnz - ref)) < 1e-9
+    h = 1e-5
+    for name in names:
+        vals = []
+        for sgn in (+1, -1):
+            q = dict(base)
+            q[name] += sgn * h
+            op = api.HubbardOperator.from_params(api.HubbardParams(**q), nu, nd)
+            vals.append(api.ftlm_log_partition_complex(op, betas, par))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        err = np.abs(fd - g[name]) / np.maximum(np.abs(fd), 1e-3)
+        assert np.max(err) < 1e-6, (name, fd, g[name])
+    assert np.allclose(g["mu"], betas * (nu + nd))
