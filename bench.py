@@ -251,11 +251,14 @@ def gc_cpu_baseline(R, M, budget_s=12.0):
 def grad_workload(F, L, torch, validate=True):
     """Differentiated FTLM: d lnZ(beta) / d(t, t', U, V, mu, phi_x), beta in {0.5, 1, 2, 4, 8}, on the
     4x3 (6,6) sector (dim 853,776; t=1, t'=-0.25, U=8, V=1, mu=U/2+2V, phi_x=0.05) with R=32 and
-    M=100 (forward-mode Lanczos, converged Gauss rule), then on the 4x4 (8,8) K=(0,0) momentum block
-    (phi=0, R=8).  Central differences (h=1e-5, same seeds) check the gradients."""
+    M=100 (converged Gauss rule): the full Jacobian by forward-mode Lanczos, and one weighted row
+    sum_b w_b dlnZ(beta_b)/dtheta by the adjoint sweep (checkpoints every round(sqrt(2M)) steps,
+    term-split parameter adjoints), checked against each other and against central differences
+    (h=1e-5, same seeds).  Then the 4x4 (8,8) K=(0,0) momentum block (phi=0, R=32) both ways."""
     import numpy as np
 
     betas = np.array([0.5, 1.0, 2.0, 4.0, 8.0])
+    wts = np.ones_like(betas)
     U, V = 8.0, 1.0
     base = dict(Lx=4, Ly=3, t=1.0, tp=-0.25, U=U, V=V, phi_x=0.05, phi_y=0.0)
     mu = U / 2 + 2 * V
@@ -263,21 +266,29 @@ def grad_workload(F, L, torch, validate=True):
                        quad=L.FTLM_QUAD_CONVERGED)
     names = ("t", "tp", "U", "V", "phi_x")
     p = F.HubbardParams(**base)
-    F.ftlm_log_partition_gradient(p, 6, 6, betas[:1], F.FtlmParams(n_random=2, lanczos_steps=4), params=names)
+    small = F.FtlmParams(n_random=2, lanczos_steps=4)
+    F.ftlm_log_partition_gradient(p, 6, 6, betas[:1], small, params=names)
+    F.ftlm_log_partition_vjp(p, 6, 6, betas[:1], [1.0], small, params=names)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     lnz, g = F.ftlm_log_partition_gradient(p, 6, 6, betas, par, params=names + ("mu",), mu=mu)
     t_grad = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    lnz_v, gv = F.ftlm_log_partition_vjp(p, 6, 6, betas, wts, par, params=names + ("mu",), mu=mu)
+    t_vjp = time.perf_counter() - t0
     op = F.HubbardOperator.from_params(p, 6, 6)
     t0 = time.perf_counter()
     plain = F.ftlm_log_partition_complex(op, betas, par)
     t_plain = time.perf_counter() - t0
-    log(f"4x3 (6,6) gradient (5 parameters + mu): {t_grad:.2f} s; lnZ alone {t_plain:.2f} s")
+    vjp_err = max(abs(gv[k] - float(np.dot(wts, g[k]))) / max(abs(float(np.dot(wts, g[k]))), 1e-3) for k in gv)
+    log(f"4x3 (6,6): Jacobian (5 parameters + mu, 5 betas) forward mode {t_grad:.2f} s; one weighted row by "
+        f"the adjoint sweep {t_vjp:.2f} s (max rel. diff. to forward mode {vjp_err:.1e}); lnZ alone {t_plain:.2f} s")
     out = {"metric": "differentiated FTLM: d lnZ / d(t, t', U, V, mu, phi_x), 4x3 (6,6), R=32, M=100",
            "unit": "s", "higher_is_better": False, "n_gpus": 1, "data": "synthetic", "dtype": "c128",
            "config": {"workload": "4x3 (6,6) FTLM gradient", "hamiltonian": base, "mu": mu, "R": 32, "M": 100,
                       "betas": betas.tolist()},
-           "value": round(t_grad, 3), "lnZ_only_s": round(t_plain, 3),
+           "value": round(t_grad, 3), "vjp_s": round(t_vjp, 3), "lnZ_only_s": round(t_plain, 3),
+           "vjp_vs_forward_max_rel": vjp_err, "lnZ_diff_plain": float(np.max(np.abs(lnz - mu * 12 * betas - plain))),
            "lnZ": lnz.tolist(), "grad": {k: v.tolist() for k, v in g.items()},
            "observables": {  # -(1/beta) d lnZ / d theta
                "double_occupancy": (-g["U"] / betas).tolist(), "nn_density": (-g["V"] / betas).tolist(),
@@ -296,7 +307,8 @@ def grad_workload(F, L, torch, validate=True):
             fd_err[name] = float(np.max(np.abs(fd - g[name]) / np.maximum(np.abs(fd), 1e-3)))
         out["fd_max_rel_err"] = fd_err
         log(f"central differences, max relative error: {fd_err}")
-    # the 4x4 (8,8) K = (0,0) block, phi = 0
+    # the 4x4 (8,8) K = (0,0) block at phi = 0 (with flux the reference's block operator is not
+    # Hermitian: its Peierls phases sit on the wrapped bonds, which translations move)
     pk = F.HubbardParams(4, 4, 1.0, -0.25, U, V, 0.0, 0.0)
     K = F.MomentumSector(0, 0, 4, 4)
     kn = ("t", "tp", "U", "V")
@@ -305,14 +317,20 @@ def grad_workload(F, L, torch, validate=True):
     dB = [F.derivative_operator(pk, NU, ND, q, K) for q in kn]
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
-    pk_par = F.FtlmParams(n_random=8, lanczos_steps=100, seed=5, start=L.FTLM_START_PHASE, quad=L.FTLM_QUAD_CONVERGED)
+    pk_par = F.FtlmParams(n_random=32, lanczos_steps=100, seed=5, start=L.FTLM_START_PHASE,
+                          quad=L.FTLM_QUAD_CONVERGED)
     t0 = time.perf_counter()
     lk, gk = F.ftlm_gradient(B, dB, betas, pk_par)
     t_k = time.perf_counter() - t0
-    log(f"4x4 (8,8) K=(0,0) block (dim {B.dim()}): gradient (t, t', U, V) R=8 M=100 {t_k:.2f} s, "
-        f"operators {t_build:.2f} s")
-    out["kblock_4x4_K00"] = {"dim": B.dim(), "R": 8, "M": 100, "seconds": round(t_k, 3), "build_s": round(t_build, 3),
-                             "lnZ": lk.tolist(), "grad": {n: gk[i].tolist() for i, n in enumerate(kn)}}
+    t0 = time.perf_counter()
+    lkv, gkv = F.ftlm_vjp(B, dB, betas, wts, pk_par)
+    t_kv = time.perf_counter() - t0
+    kerr = float(np.max(np.abs(gkv - gk @ wts) / np.maximum(np.abs(gk @ wts), 1e-3)))
+    log(f"4x4 (8,8) K=(0,0) block (dim {B.dim()}), R=32 M=100: Jacobian (t, t', U, V) forward mode {t_k:.2f} s, "
+        f"adjoint row {t_kv:.2f} s (max rel. diff. {kerr:.1e}); operators {t_build:.2f} s")
+    out["kblock_4x4_K00"] = {"dim": B.dim(), "R": 32, "M": 100, "seconds": round(t_k, 3), "vjp_s": round(t_kv, 3),
+                             "vjp_vs_forward_max_rel": kerr, "build_s": round(t_build, 3), "lnZ": lk.tolist(),
+                             "grad": {n: gk[i].tolist() for i, n in enumerate(kn)}}
     return out
 
 

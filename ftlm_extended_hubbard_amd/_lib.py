@@ -18,6 +18,7 @@ FTLM_C128, FTLM_C64 = 0, 1
 FTLM_START_GAUSS_MT, FTLM_START_PHASE = 0, 1
 FTLM_QUAD_REFERENCE, FTLM_QUAD_CONVERGED = 0, 1
 FTLM_PATH_AUTO, FTLM_PATH_PER_OP, FTLM_PATH_PERSISTENT = 0, 1, 2
+FTLM_TS_AUTO, FTLM_TS_GENERIC, FTLM_TS_FUSED = 0, 1, 2
 
 
 class FtlmHipError(RuntimeError):
@@ -101,6 +102,7 @@ def lib() -> C.CDLL:
     L.ftlm_hip_krylov_destroy.argtypes = [vp]
     L.ftlm_hip_build_geometry_derivative.argtypes = [C.POINTER(Params), C.c_int, vp, C.c_int, ip]
     L.ftlm_hip_ftlm_gradient.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp, C.c_int, vp, vp]
+    L.ftlm_hip_ftlm_vjp.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]
     L.ftlm_hip_exact_log_partition.argtypes = [vp, vp, C.c_int, vp, vp]
     L.ftlm_hip_callback_op_create.argtypes = [C.c_int64, HOST_APPLY_FN, vp, C.POINTER(vp)]
     L.ftlm_hip_translate_states.argtypes = [C.c_int, C.c_int, vp, C.c_int64, C.c_int, C.c_int, vp, vp]

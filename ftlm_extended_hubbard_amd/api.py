@@ -475,6 +475,39 @@ def ftlm_gradient(op: _Op, dops, betas, par: FtlmParams):
     return lnz, g[:npar]
 
 
+def ftlm_vjp(op: _Op, dops, betas, weights, par: FtlmParams, checkpoint: int = 0, mode: int = L.FTLM_TS_AUTO):
+    """Reverse mode: (lnZ [nb], g [len(dops)]) with g_p = sum_b weights[b] dlnZ(beta_b)/dtheta_p,
+    dH/dtheta_p given by the operators dops; Krylov vectors are recomputed from checkpoints every
+    `checkpoint` steps (0: round(sqrt(2M)))."""
+    b = np.ascontiguousarray(np.atleast_1d(betas), np.float64)
+    gw = np.ascontiguousarray(np.broadcast_to(np.asarray(weights, np.float64), b.shape))
+    nb, npar = len(b), len(dops)
+    lnz, g = np.zeros(nb), np.zeros(max(npar, 1))
+    arr = (C.c_void_p * max(npar, 1))(*[d.handle.value for d in dops]) if npar else None
+    L.check(L.lib().ftlm_hip_ftlm_vjp(op.handle, arr, npar, C.byref(par._c()), b.ctypes.data, nb, gw.ctypes.data,
+                                      int(checkpoint), int(mode), lnz.ctypes.data, g.ctypes.data))
+    return lnz, g[:npar]
+
+
+def ftlm_log_partition_vjp(p: HubbardParams, n_up: int, n_dn: int, betas, weights, par: FtlmParams,
+                           params=("t", "tp", "U", "V", "phi_x"), mu: float | None = None,
+                           K: "MomentumSector | None" = None, checkpoint: int = 0, mode: int = L.FTLM_TS_AUTO):
+    """lnZ(beta) of the sector (or of its momentum-K block) and sum_b weights[b] dlnZ(beta_b)/dtheta
+    for theta in params (any of t, tp, U, V, phi_x, phi_y, mu), by the adjoint Lanczos sweep."""
+    op = HubbardOperator.from_params(p, n_up, n_dn) if K is None else KBlockOperator.from_params(p, n_up, n_dn, K)
+    hop = [q for q in params if q != "mu"]
+    dops = [derivative_operator(p, n_up, n_dn, q, K) for q in hop]
+    b = np.atleast_1d(np.asarray(betas, np.float64))
+    w = np.broadcast_to(np.asarray(weights, np.float64), b.shape)
+    lnz, g = ftlm_vjp(op, dops, b, w, par, checkpoint, mode)
+    out = {q: float(g[i]) for i, q in enumerate(hop)}
+    if mu is not None:
+        lnz = lnz + b * mu * (n_up + n_dn)
+    if "mu" in params:
+        out["mu"] = float(np.sum(w * b) * (n_up + n_dn))
+    return lnz, out
+
+
 def ftlm_log_partition_gradient(p: HubbardParams, n_up: int, n_dn: int, betas, par: FtlmParams,
                                 params=("t", "tp", "U", "V", "phi_x"), mu: float | None = None,
                                 K: "MomentumSector | None" = None):

@@ -6,7 +6,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <stdexcept>
+#include <complex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "ftlm_hip.h"
@@ -192,6 +194,11 @@ struct Sector : ftlm_op {
   // per-XCD tile counters of the two H*V passes ([0..7] rows pass, [8..15] column pass); each
   // pass zeroes the other pass's counters, so applies on one operator must be stream-ordered
   DevBuf<int32_t> tile_ctr;
+  // host copies: directed bond (from, to) and summed coefficient per spin, in bond-id order, and
+  // the NN pair list as sorted (min, max) sites (term-split derivative tables are built from them)
+  std::vector<std::pair<int, int>> bond_key[2];
+  std::vector<std::complex<double>> bond_coef[2];
+  std::vector<std::pair<int, int>> pair_list;
   SectorView view() const;
 };
 
@@ -220,5 +227,7 @@ void check_launch(const char* what);
 int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
              hipStream_t s, bool force_real = false);
 int hv_partial_blocks(const ftlm_op* op);
+// True when a momentum-block operator is Hermitian (translation-invariant hop coefficients).
+bool kblock_hermitian(const ftlm_op* op);
 
 }  // namespace ftlm_hip

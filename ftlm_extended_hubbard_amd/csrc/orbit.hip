@@ -649,6 +649,7 @@ struct OrbitGraph {
   int64_t nrep = 0, nnz = 0;
   int nbond = 0;
   bool real_h = true;
+  bool hermitian = true;  // the merged hop coefficients are translation invariant
   DevBuf<uint32_t> pack;
   DevBuf<uint16_t> stab, eta;
   DevBuf<double> diag;
@@ -681,6 +682,32 @@ struct KBlock : ftlm_op {
   DevBuf<double2> A;  // [nrep]
 };
 
+// True when the merged hop coefficients (equal (from, to, spin) summed) are invariant under the
+// lattice translations (site = y * lx + x); the block operator is Hermitian exactly then.  The
+// reference's Peierls phases sit on the wrapped bonds only, so a nonzero flux breaks this: its
+// block operator then has |H - H^dag| of the order of the flux and complex eigenvalues.
+bool hops_translation_invariant(int lx, int ly, const ftlm_hop_t* hops, int n_hops) {
+  std::map<std::tuple<int, int, int>, std::complex<double>> m;
+  double scale = 0.0;
+  for (int h = 0; h < n_hops; ++h) {
+    if (hops[h].from == hops[h].to) continue;
+    m[std::make_tuple(hops[h].from, hops[h].to, hops[h].spin)] += std::complex<double>(hops[h].re, hops[h].im);
+  }
+  for (const auto& kv : m) scale = std::max(scale, std::abs(kv.second));
+  const double tol = 1e-12 * std::max(scale, 1.0);
+  auto tr = [&](int site, int dx, int dy) { return ((site / lx + dy) % ly) * lx + (site % lx + dx) % lx; };
+  const int gen[2][2] = {{1, 0}, {0, 1}};
+  for (const auto& g : gen)
+    for (const auto& kv : m) {
+      const auto key = std::make_tuple(tr(std::get<0>(kv.first), g[0], g[1]), tr(std::get<1>(kv.first), g[0], g[1]),
+                                       std::get<2>(kv.first));
+      const auto it = m.find(key);
+      const std::complex<double> c = it == m.end() ? std::complex<double>(0.0, 0.0) : it->second;
+      if (std::abs(c - kv.second) > tol) return false;
+    }
+  return true;
+}
+
 std::shared_ptr<OrbitGraph> build_orbit_graph(int lx, int ly, int n_up, int n_dn, const ftlm_hop_t* hops, int n_hops,
                                               const ftlm_pair_t* pairs, int n_pairs, double U, double V) {
   const int n = lx * ly;
@@ -694,6 +721,7 @@ std::shared_ptr<OrbitGraph> build_orbit_graph(int lx, int ly, int n_up, int n_dn
   G->hops.assign(hops, hops + n_hops);
   G->pairs.assign(pairs, pairs + n_pairs);
   G->L = make_lat(lx, ly);
+  G->hermitian = hops_translation_invariant(lx, ly, hops, n_hops);
   const hipStream_t st = nullptr;
 
   // merged bonds (same (from, to, spin) summed in list order), in first-occurrence order
@@ -913,6 +941,8 @@ void launch_kblock_chunk(const KBlock* B, const void* x, void* y, int RT, int r0
 }  // namespace
 
 constexpr int kKbMaxGrid = 4096;
+
+bool kblock_hermitian(const ftlm_op* op) { return static_cast<const KBlock*>(op)->G->hermitian; }
 
 int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
                     hipStream_t st) {

@@ -301,6 +301,8 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
       bd[b] = BondDev{static_cast<uint8_t>(h.from), static_cast<uint8_t>(h.to), h.own};
       cf[b] = make_double2(h.coeff.real(), h.coeff.imag());
       xm[b] = h.other;
+      S->bond_key[sp].emplace_back(h.from, h.to);
+      S->bond_coef[sp].push_back(h.coeff);
       if (h.coeff.imag() != 0.0) S->real_h = false;
     }
     S->coef[sp].alloc(cf.size());
@@ -354,7 +356,9 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
     FTLM_REQUIRE(pairs[p].i >= 0 && pairs[p].i < n && pairs[p].j >= 0 && pairs[p].j < n, FTLM_EINVAL,
                  "pair site outside [0, n_sites)");
     pr[p] = make_int2(pairs[p].i, pairs[p].j);
+    S->pair_list.emplace_back(std::min(pairs[p].i, pairs[p].j), std::max(pairs[p].i, pairs[p].j));
   }
+  std::sort(S->pair_list.begin(), S->pair_list.end());
   {
     std::vector<int> deg(static_cast<size_t>(n), 0);
     for (int p = 0; p < n_pairs; ++p) {

@@ -83,7 +83,7 @@ int ftlm_hip_build_geometry(const ftlm_params_t* p, ftlm_hop_t* hops, int hop_ca
 /* FockBasis(n_sites, n_up, n_down) (src/fock_basis.cpp:17-42) + the operator data of
  * apply_extended_hubbard (src/hubbard_hamiltonian.cpp:51-108): enumerates the sector on the
  * device and builds the per-spin pull-hop tables.  Limits: n_sites <= 32, each spin's string
- * count < 2^20, at most 2047 distinct directed bonds per spin. */
+ * count < 2^20, at most 510 distinct directed bonds per spin. */
 int ftlm_hip_sector_create(int n_sites, int n_up, int n_dn, const ftlm_hop_t* hops, int n_hops,
                            const ftlm_pair_t* pairs, int n_pairs, double U, double V, ftlm_op_t** out);
 int ftlm_hip_op_destroy(ftlm_op_t* op);
@@ -227,6 +227,23 @@ int ftlm_hip_build_geometry_derivative(const ftlm_params_t* p, int which, ftlm_h
  * reference, which has no derivatives. */
 int ftlm_hip_ftlm_gradient(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
                            const double* betas, int nb, double* lnZ, double* grad);
+
+/* Vector-Jacobian product of the same estimator (reverse mode): lnZ(beta_b) for nb betas and
+ * grad_p = sum_b gw[b] dlnZ(beta_b)/dtheta_p, from one forward Lanczos pass that keeps
+ * (q_{k-1}, q_k) every ckpt steps (0: round(sqrt(2M))), the quadrature's derivatives, and one
+ * adjoint pass that recomputes each segment's Krylov vectors from its checkpoint.  The parameter
+ * adjoints sum_k Re <w-bar_k| dH_p |q_k> come from one term-split pass over a full sector's own
+ * hop lists (all derivative tables and the U / V terms at once) when every dH_p is a hop table
+ * over H's bonds plus U / V terms (mode FTLM_TS_AUTO or FTLM_TS_FUSED), otherwise from one apply
+ * of each dH_p (FTLM_TS_GENERIC, and any operator kind).  The backward sweep applies H as H^dag:
+ * momentum blocks whose hop coefficients are not translation invariant (flux on wrapped bonds)
+ * are refused with FTLM_EUNSUPPORTED; callback operators must be Hermitian.  grad is [npar].  No
+ * counterpart in the reference. */
+#define FTLM_TS_AUTO 0
+#define FTLM_TS_GENERIC 1
+#define FTLM_TS_FUSED 2
+int ftlm_hip_ftlm_vjp(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
+                      const double* betas, int nb, const double* gw, int ckpt, int mode, double* lnZ, double* grad);
 
 /* ------------------------------------------------------------------ host-callback operators */
 /* The operator argument of lanczos_tridiagonal / lanczos_extrema / ftlm_log_partition_complex

@@ -360,3 +360,49 @@ def test_gradient_vs_central_differences(lx, ly, nu, nd, phi):
         err = np.abs(fd - g[name]) / np.maximum(np.abs(fd), 1e-3)
         assert np.max(err) < 1e-6, (name, fd, g[name])
     assert np.allclose(g["mu"], betas * (nu + nd))
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd,phi,R,M", [(4, 2, 3, 2, 0.05, 4, 24), (4, 3, 3, 3, 0.05, 4, 30),
+                                               (2, 2, 2, 1, 0.3, 3, 6)])
+def test_vjp_matches_forward_mode(lx, ly, nu, nd, phi, R, M):
+    """The adjoint sweep (term-split and per-operator parameter adjoints, several checkpoint
+    intervals) against the forward-mode derivatives of the same estimator:
+    sum_b w_b dlnZ(beta_b)/dtheta within 1e-9 relative.  4x2 uses the single-pass H*V, 4x3 (3,3)
+    the two-pass one; the 2x2 (2,1) sector (dim 24, M=6 unclamped) runs every step of a short chain."""
+    say("vjp vs forward", lx, ly, nu, nd, phi, R, M)
+    base = api.HubbardParams(lx, ly, 1.0, -0.25, 8.0, 1.0, phi, 0.0)
+    betas = np.array([0.5, 2.0, 8.0])
+    w = np.array([0.3, -1.1, 0.7])
+    par = api.FtlmParams(n_random=R, lanczos_steps=M, seed=11, quad=L.FTLM_QUAD_CONVERGED)
+    names = ("t", "tp", "U", "V", "phi_x")
+    lnz_f, g = api.ftlm_log_partition_gradient(base, nu, nd, betas, par, params=names)
+    want = {k: float(np.dot(w, g[k])) for k in names}
+    for ck, mode in [(0, L.FTLM_TS_FUSED), (1, L.FTLM_TS_FUSED), (3, L.FTLM_TS_GENERIC), (M, L.FTLM_TS_AUTO)]:
+        lnz, got = api.ftlm_log_partition_vjp(base, nu, nd, betas, w, par, params=names, checkpoint=ck, mode=mode)
+        assert np.max(np.abs(lnz - lnz_f)) < 1e-10
+        for k in names:
+            assert abs(got[k] - want[k]) <= 1e-9 * max(abs(want[k]), 1e-3), (ck, mode, k, got[k], want[k])
+
+
+def test_vjp_breakdown_vs_central_differences():
+    """A chain that breaks down (the 2-site (1,0) sector, dim 2, M=5 unclamped): the adjoint and
+    forward-mode derivatives against central differences (h=1e-5) of the estimator."""
+    say("vjp breakdown")
+    base = dict(Lx=2, Ly=1, t=1.0, tp=0.0, U=8.0, V=1.0, phi_x=0.2, phi_y=0.0)
+    betas = np.array([0.5, 3.0])
+    w = np.array([1.0, 0.25])
+    par = api.FtlmParams(n_random=3, lanczos_steps=5, seed=4, quad=L.FTLM_QUAD_CONVERGED)
+    names = ("t", "phi_x")
+    lnz, got = api.ftlm_log_partition_vjp(api.HubbardParams(**base), 1, 0, betas, w, par, params=names)
+    _, g = api.ftlm_log_partition_gradient(api.HubbardParams(**base), 1, 0, betas, par, params=names)
+    h = 1e-5
+    for name in names:
+        vals = []
+        for sgn in (+1, -1):
+            q = dict(base)
+            q[name] += sgn * h
+            op = api.HubbardOperator.from_params(api.HubbardParams(**q), 1, 0)
+            vals.append(np.dot(w, api.ftlm_log_partition_complex(op, betas, par)))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(got[name] - fd) <= 1e-6 * max(abs(fd), 1e-3), (name, got[name], fd)
+        assert abs(np.dot(w, g[name]) - fd) <= 1e-6 * max(abs(fd), 1e-3), (name, g[name], fd)
