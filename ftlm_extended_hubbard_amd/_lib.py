@@ -103,6 +103,10 @@ def lib() -> C.CDLL:
     L.ftlm_hip_build_geometry_derivative.argtypes = [C.POINTER(Params), C.c_int, vp, C.c_int, ip]
     L.ftlm_hip_ftlm_gradient.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp, C.c_int, vp, vp]
     L.ftlm_hip_ftlm_vjp.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]
+    L.ftlm_hip_ftlm_tape_forward.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp, C.c_int, C.c_int, C.c_int, vp,
+                                             C.POINTER(vp)]
+    L.ftlm_hip_ftlm_tape_backward.argtypes = [vp, vp, vp]
+    L.ftlm_hip_ftlm_tape_destroy.argtypes = [vp]
     L.ftlm_hip_exact_log_partition.argtypes = [vp, vp, C.c_int, vp, vp]
     L.ftlm_hip_callback_op_create.argtypes = [C.c_int64, HOST_APPLY_FN, vp, C.POINTER(vp)]
     L.ftlm_hip_translate_states.argtypes = [C.c_int, C.c_int, vp, C.c_int64, C.c_int, C.c_int, vp, vp]

@@ -489,6 +489,42 @@ def ftlm_vjp(op: _Op, dops, betas, weights, par: FtlmParams, checkpoint: int = 0
     return lnz, g[:npar]
 
 
+class FtlmTape:
+    """The forward pass of the adjoint method (ftlm_hip_ftlm_tape_forward): lnZ(betas) now, then
+    backward(weights) -> g_p = sum_b weights[b] dlnZ(beta_b)/dtheta_p as often as needed.  Holds
+    the operators and the device checkpoints until closed."""
+
+    def __init__(self, op: _Op, dops, betas, par: FtlmParams, checkpoint: int = 0, mode: int = L.FTLM_TS_AUTO):
+        self.betas = np.ascontiguousarray(np.atleast_1d(betas), np.float64)
+        self._ops = (op, list(dops))
+        self._h = C.c_void_p()
+        self.lnZ = np.zeros(len(self.betas))
+        npar = len(dops)
+        arr = (C.c_void_p * max(npar, 1))(*[d.handle.value for d in dops]) if npar else None
+        L.check(L.lib().ftlm_hip_ftlm_tape_forward(op.handle, arr, npar, C.byref(par._c()), self.betas.ctypes.data,
+                                                   len(self.betas), int(checkpoint), int(mode),
+                                                   self.lnZ.ctypes.data, C.byref(self._h)))
+
+    def backward(self, weights) -> np.ndarray:
+        if not self._h:
+            raise ValueError("the tape is closed")
+        w = np.ascontiguousarray(np.broadcast_to(np.asarray(weights, np.float64), self.betas.shape))
+        g = np.zeros(max(len(self._ops[1]), 1))
+        L.check(L.lib().ftlm_hip_ftlm_tape_backward(self._h, w.ctypes.data, g.ctypes.data))
+        return g[: len(self._ops[1])]
+
+    def close(self) -> None:
+        if self._h:
+            L.lib().ftlm_hip_ftlm_tape_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def ftlm_log_partition_vjp(p: HubbardParams, n_up: int, n_dn: int, betas, weights, par: FtlmParams,
                            params=("t", "tp", "U", "V", "phi_x"), mu: float | None = None,
                            K: "MomentumSector | None" = None, checkpoint: int = 0, mode: int = L.FTLM_TS_AUTO):

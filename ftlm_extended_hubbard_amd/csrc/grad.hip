@@ -674,6 +674,365 @@ bool build_termsplit(const ftlm_op* op, ftlm_op* const* dops, int npar, TermSpli
 
 using namespace ftlm_hip;
 
+// The saved state of one forward Lanczos pass for repeated adjoint sweeps.
+struct ftlm_tape {
+  const ftlm_op* op = nullptr;
+  std::vector<ftlm_op*> dops;
+  ftlm_hip::TermSplit ts;
+  int R = 0, M = 0, s = 1, nb = 0, kmax = 0;
+  int64_t dim = 0;
+  std::vector<ftlm_hip::DevBuf<double2>> ck;  // ck[2c], ck[2c+1] = q_{cs-1}, q_{cs}
+  std::vector<double2> zs;                    // [M][R] z_k (dead vectors 0)
+  std::vector<double> bs;                     // [M][R] beta_k
+  std::vector<int> used;                      // steps taken per vector
+  std::vector<double> L, G;                   // L [R][nb]; dL/d(alpha_k, beta_k): [nb][R][M][2]
+};
+
+namespace ftlm_hip {
+namespace {
+
+// Forward pass of the adjoint method: the recurrence of ftlm_hip_ftlm_gradient without tangents,
+// keeping (q_{k-1}, q_k) every s steps, then the quadrature and its derivatives per sample.
+void tape_forward(ftlm_tape* T, const ftlm_op* op, ftlm_op* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
+                  const double* betas, int nb, int ckpt, int mode, double* lnZ) {
+  FTLM_REQUIRE(op != nullptr && opts != nullptr && betas != nullptr && lnZ != nullptr && nb >= 1, FTLM_EINVAL,
+               "null argument");
+  FTLM_REQUIRE(npar >= 0 && npar <= kGMaxPar && (npar == 0 || dops != nullptr), FTLM_EINVAL,
+               "bad derivative operators");
+  FTLM_REQUIRE(mode == FTLM_TS_AUTO || mode == FTLM_TS_GENERIC || mode == FTLM_TS_FUSED, FTLM_EINVAL, "bad mode");
+  FTLM_REQUIRE(ckpt >= 0, FTLM_EINVAL, "negative checkpoint interval");
+  FTLM_REQUIRE(opts->start == FTLM_START_GAUSS_MT || opts->start == FTLM_START_PHASE, FTLM_EINVAL, "bad start");
+  for (int p = 0; p < npar; ++p)
+    FTLM_REQUIRE(dops[p] != nullptr && dops[p]->dim == op->dim, FTLM_EINVAL, "derivative operator of another size");
+  FTLM_HIP_CHECK(hipSetDevice(op->device));
+  const int64_t dim = op->dim;
+  const int R = std::max(1, opts->R);
+  int M = std::max(1, opts->M);
+  if (opts->clamp_M_to_dim) M = static_cast<int>(std::min<int64_t>(M, std::max<int64_t>(dim, 1)));
+  FTLM_REQUIRE(R <= 32, FTLM_EUNSUPPORTED, "at most 32 samples per gradient call");
+  FTLM_REQUIRE(M <= kGMaxM, FTLM_EUNSUPPORTED, "at most 512 Lanczos steps per gradient call");
+  FTLM_REQUIRE(dim > 1, FTLM_EUNSUPPORTED, "the adjoint sweep needs dim > 1 (a one-state operator has lnZ = -beta H00)");
+  // the adjoint sweep applies H in place of H^dag
+  FTLM_REQUIRE(op->kind != kKBlock || kblock_hermitian(op), FTLM_EUNSUPPORTED,
+               "this momentum-block operator is not Hermitian (its hop coefficients are not translation "
+               "invariant, e.g. a Peierls flux on the wrapped bonds); use ftlm_hip_ftlm_gradient");
+  if (mode != FTLM_TS_GENERIC && npar > 0) build_termsplit(op, dops, npar, &T->ts);
+  FTLM_REQUIRE(mode != FTLM_TS_FUSED || T->ts.fused || npar == 0, FTLM_EUNSUPPORTED,
+               "these derivative operators have no term-split form over this operator");
+  T->op = op;
+  T->dops.assign(dops, dops + npar);
+  T->R = R;
+  T->M = M;
+  T->nb = nb;
+  T->dim = dim;
+  const int s = std::min(M, ckpt > 0 ? ckpt : std::max(1, static_cast<int>(std::lround(std::sqrt(2.0 * M)))));
+  T->s = s;
+  const int nck = (M + s - 1) / s;
+  const hipStream_t st = nullptr;
+  const int64_t n = dim * R;
+  const size_t vb = static_cast<size_t>(n);
+  T->ck.resize(2 * static_cast<size_t>(nck));
+  for (auto& b : T->ck) b.alloc(vb);
+  DevBuf<double2> b0(vb), b1(vb), b2(vb);
+  double2* qp = b0.p;
+  double2* q = b1.p;
+  double2* w = b2.p;
+  const int bsz = (kGBlock / R) * R;
+  const int gblk = static_cast<int>(std::min<int64_t>(kGMaxGrid, (n + bsz - 1) / bsz));
+  DevBuf<double2> part(static_cast<size_t>(gblk) * R), dres(static_cast<size_t>(R));
+  std::vector<double2> hres(static_cast<size_t>(R));
+  auto collect = [&] {
+    hipLaunchKernelGGL(k_gsum, dim3(R), dim3(kGBlock), 0, st, part.p, gblk, R, dres.p);
+    check_launch("k_gsum");
+    FTLM_HIP_CHECK(hipMemcpyAsync(hres.data(), dres.p, sizeof(double2) * R, hipMemcpyDeviceToHost, st));
+    FTLM_HIP_CHECK(hipStreamSynchronize(st));
+  };
+  FTLM_HIP_CHECK(hipMemsetAsync(qp, 0, vb * sizeof(double2), st));
+  std::vector<unsigned> seeds(static_cast<size_t>(R));
+  for (int r = 0; r < R; ++r) seeds[r] = opts->seed + static_cast<unsigned>(r) * 100003u;
+  if (opts->start == FTLM_START_PHASE) {
+    DevBuf<uint32_t> ids(static_cast<size_t>(R));
+    std::vector<uint32_t> hid(seeds.begin(), seeds.end());
+    ids.upload(hid, st);
+    hipLaunchKernelGGL(k_gphase, dim3(grid_n(dim)), dim3(kGBlock), 0, st, dim, R, ids.p, q);
+    check_launch("k_gphase");
+    FTLM_HIP_CHECK(hipStreamSynchronize(st));
+  } else {
+    std::vector<double2> h(vb);
+    std::vector<std::complex<double>> g;
+    for (int r = 0; r < R; ++r) {
+      gauss_mt(dim, seeds[r], &g);
+      for (int64_t i = 0; i < dim; ++i) h[static_cast<size_t>(i) * R + r] = make_double2(g[i].real(), g[i].imag());
+    }
+    FTLM_HIP_CHECK(hipMemcpy(q, h.data(), vb * sizeof(double2), hipMemcpyHostToDevice));
+  }
+  hipLaunchKernelGGL(k_gstep_dots, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, q, q, GradVecs{}, part.p);
+  check_launch("k_gstep_dots");
+  collect();
+  for (int r = 0; r < R; ++r) FTLM_REQUIRE(std::sqrt(hres[r].x) >= 1e-18, FTLM_EUNSUPPORTED, "zero start vector");
+  hipLaunchKernelGGL(k_gscale, dim3(gblk), dim3(kGBlock), 0, st, n, R, q, dres.p);
+  check_launch("k_gscale");
+  T->zs.assign(static_cast<size_t>(M) * R, make_double2(0.0, 0.0));
+  T->bs.assign(static_cast<size_t>(M) * R, 0.0);
+  T->used.assign(static_cast<size_t>(R), 0);
+  std::vector<int> alive(static_cast<size_t>(R), 1);
+  std::vector<double> bprev(static_cast<size_t>(R), 0.0), bet(static_cast<size_t>(R), 0.0);
+  std::vector<double2> z(static_cast<size_t>(R));
+  DevBuf<double2> dz(static_cast<size_t>(R));
+  DevBuf<double> dbp(static_cast<size_t>(R)), dbt(static_cast<size_t>(R));
+  DevBuf<int> dalive(static_cast<size_t>(R));
+  for (int k = 0; k < M; ++k) {
+    if (k % s == 0) {
+      const int c = k / s;
+      FTLM_HIP_CHECK(hipMemcpyAsync(T->ck[2 * c].p, qp, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
+      FTLM_HIP_CHECK(hipMemcpyAsync(T->ck[2 * c + 1].p, q, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    }
+    hv_apply(op, q, w, R, FTLM_C128, nullptr, nullptr, st);
+    hipLaunchKernelGGL(k_gstep_dots, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, q, w, GradVecs{}, part.p);
+    check_launch("k_gstep_dots");
+    collect();
+    for (int r = 0; r < R; ++r) {
+      z[r] = alive[r] ? hres[r] : make_double2(0.0, 0.0);
+      T->zs[static_cast<size_t>(k) * R + r] = z[r];
+    }
+    dz.upload(z, st);
+    dbp.upload(bprev, st);
+    dalive.upload(alive, st);
+    hipLaunchKernelGGL(k_gstep_update, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, q, qp, w, GradVecs{}, dz.p, dbp.p,
+                       dalive.p, k > 0 ? 1 : 0, part.p);
+    check_launch("k_gstep_update");
+    collect();
+    std::fill(bet.begin(), bet.end(), 0.0);
+    for (int r = 0; r < R; ++r) {
+      if (!alive[r]) continue;
+      T->used[r] = k + 1;
+      const double nw2 = hres[r].x;
+      if (nw2 < 1e-24) {
+        alive[r] = 0;
+        continue;
+      }
+      const double b = std::sqrt(nw2);
+      T->bs[static_cast<size_t>(k) * R + r] = b;
+      bet[r] = b;
+      bprev[r] = b;
+    }
+    bool any = false;
+    for (int r = 0; r < R; ++r) any = any || alive[r];
+    if (!any || k + 1 == M) break;
+    dbt.upload(bet, st);
+    dalive.upload(alive, st);
+    hipLaunchKernelGGL(k_gstep_norm, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, w, GradVecs{}, dbt.p, dalive.p);
+    check_launch("k_gstep_norm");
+    double2* t = qp;
+    qp = q;
+    q = w;
+    w = t;
+  }
+  T->kmax = 0;
+  for (int r = 0; r < R; ++r) T->kmax = std::max(T->kmax, T->used[r]);
+  // quadrature: L_r(beta_b) and dL_r/d(alpha_k, beta_k)
+  std::vector<double> ha(static_cast<size_t>(R) * M, 0.0), hb(static_cast<size_t>(R) * M, 0.0);
+  for (int r = 0; r < R; ++r)
+    for (int k = 0; k < T->used[r]; ++k) {
+      ha[static_cast<size_t>(r) * M + k] = T->zs[static_cast<size_t>(k) * R + r].x;
+      hb[static_cast<size_t>(r) * M + k] = T->bs[static_cast<size_t>(k) * R + r];
+    }
+  DevBuf<double> da(ha.size()), db(hb.size()), dbeta(static_cast<size_t>(nb));
+  da.upload(ha, st);
+  db.upload(hb, st);
+  dbeta.upload(betas, static_cast<size_t>(nb), st);
+  DevBuf<int> dn(static_cast<size_t>(R));
+  dn.upload(T->used, st);
+  DevBuf<double> Uw(static_cast<size_t>(R) * M * M), Wm(static_cast<size_t>(R) * M * M);
+  DevBuf<double> Ls(static_cast<size_t>(R) * nb), G(static_cast<size_t>(nb) * R * M * 2);
+  DevBuf<int> status(1);
+  FTLM_HIP_CHECK(hipMemsetAsync(status.p, 0, sizeof(int), st));
+  FTLM_HIP_CHECK(hipMemsetAsync(G.p, 0, G.n * sizeof(double), st));
+  hipLaunchKernelGGL(k_grad_tridiag, dim3(R), dim3(kGBlock), 0, st, da.p, db.p, static_cast<const double*>(nullptr),
+                     static_cast<const double*>(nullptr), dn.p, R, M, 0, dbeta.p, nb, Uw.p, Wm.p, Ls.p,
+                     static_cast<double*>(nullptr), G.p, status.p);
+  check_launch("k_grad_tridiag");
+  int bad = 0;
+  FTLM_HIP_CHECK(hipMemcpy(&bad, status.p, sizeof(int), hipMemcpyDeviceToHost));
+  FTLM_REQUIRE(bad == 0, FTLM_EHIP, "tridiagonal QL did not converge");
+  T->L.resize(Ls.n);
+  T->G.resize(G.n);
+  FTLM_HIP_CHECK(hipMemcpy(T->L.data(), Ls.p, T->L.size() * sizeof(double), hipMemcpyDeviceToHost));
+  FTLM_HIP_CHECK(hipMemcpy(T->G.data(), G.p, T->G.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int b = 0; b < nb; ++b) {  // lnZ_b = ln dim + log-mean-exp_r L_r(b)
+    double mx = -INFINITY;
+    for (int r = 0; r < R; ++r) mx = std::max(mx, T->L[static_cast<size_t>(r) * nb + b]);
+    double zsum = 0.0;
+    for (int r = 0; r < R; ++r) zsum += std::exp(T->L[static_cast<size_t>(r) * nb + b] - mx);
+    lnZ[b] = std::log(static_cast<double>(dim)) + mx + std::log(zsum / R);
+  }
+}
+
+// Adjoint sweep for the objective sum_b gw[b] lnZ(beta_b): grad[p] = sum_k Re <w-bar_k| dH_p |q_k>.
+void tape_backward(const ftlm_tape* T, const double* gw, double* grad) {
+  const int npar = static_cast<int>(T->dops.size());
+  FTLM_REQUIRE(gw != nullptr && (npar == 0 || grad != nullptr), FTLM_EINVAL, "null argument");
+  if (npar == 0) return;
+  const ftlm_op* op = T->op;
+  FTLM_HIP_CHECK(hipSetDevice(op->device));
+  const int R = T->R, M = T->M, s = T->s, nb = T->nb, kmax = T->kmax;
+  const std::vector<int>& used = T->used;
+  const std::vector<double>& bs = T->bs;
+  const std::vector<double2>& zs = T->zs;
+  const TermSplit& ts = T->ts;
+  const hipStream_t st = nullptr;
+  const int64_t n = T->dim * R;
+  const size_t vb = static_cast<size_t>(n);
+  // d(sum_b gw_b lnZ_b)/dL_r(b) = gw_b softmax_r(L(b)): adjoints of alpha_k (abar) and beta_k (bbar)
+  std::vector<double> abar(static_cast<size_t>(R) * M, 0.0), bbar(static_cast<size_t>(R) * M, 0.0);
+  for (int b = 0; b < nb; ++b) {
+    double mx = -INFINITY;
+    for (int r = 0; r < R; ++r) mx = std::max(mx, T->L[static_cast<size_t>(r) * nb + b]);
+    double zsum = 0.0;
+    for (int r = 0; r < R; ++r) zsum += std::exp(T->L[static_cast<size_t>(r) * nb + b] - mx);
+    for (int r = 0; r < R; ++r) {
+      const double wgt = gw[b] * std::exp(T->L[static_cast<size_t>(r) * nb + b] - mx) / zsum;
+      for (int k = 0; k < used[r]; ++k) {
+        const double* g = &T->G[((static_cast<size_t>(b) * R + r) * M + k) * 2];
+        abar[static_cast<size_t>(r) * M + k] += wgt * g[0];
+        bbar[static_cast<size_t>(r) * M + k] += wgt * g[1];
+      }
+    }
+  }
+  // seg[i] = q_{j0-1+i} of the segment being reversed
+  std::vector<DevBuf<double2>> seg(static_cast<size_t>(s) + 2);
+  for (auto& b : seg) b.alloc(vb);
+  DevBuf<double2> A(vb), P(vb), WA(vb), Y(vb), tmp(ts.fused ? 0 : vb);
+  const int bsz = (kGBlock / R) * R;
+  const int gblk = static_cast<int>(std::min<int64_t>(kGMaxGrid, (n + bsz - 1) / bsz));
+  const int gbil = static_cast<int>(std::min<int64_t>(kGMaxGrid, (n + kGBlock - 1) / kGBlock));
+  DevBuf<double2> part(std::max(static_cast<size_t>(gblk) * 2 * R, static_cast<size_t>(gbil) * kTsQ));
+  DevBuf<double2> dres(static_cast<size_t>(2) * R);
+  std::vector<double2> hres(static_cast<size_t>(2) * R);
+  auto collect = [&](int nq) {
+    hipLaunchKernelGGL(k_gsum, dim3(nq * R), dim3(kGBlock), 0, st, part.p, gblk, nq * R, dres.p);
+    check_launch("k_gsum");
+    FTLM_HIP_CHECK(hipMemcpyAsync(hres.data(), dres.p, sizeof(double2) * nq * R, hipMemcpyDeviceToHost, st));
+    FTLM_HIP_CHECK(hipStreamSynchronize(st));
+  };
+  const int nacc = ts.fused ? kTsQ : npar;
+  DevBuf<double> tacc(static_cast<size_t>(nacc));
+  FTLM_HIP_CHECK(hipMemsetAsync(tacc.p, 0, tacc.n * sizeof(double), st));
+  FTLM_HIP_CHECK(hipMemsetAsync(A.p, 0, vb * sizeof(double2), st));
+  FTLM_HIP_CHECK(hipMemsetAsync(P.p, 0, vb * sizeof(double2), st));
+  std::vector<double> rho(static_cast<size_t>(R), 0.0), bacc(static_cast<size_t>(R), 0.0);
+  std::vector<double> c1(static_cast<size_t>(R)), c2(static_cast<size_t>(R)), bk(static_cast<size_t>(R)),
+      bm(static_cast<size_t>(R)), scale(static_cast<size_t>(R));
+  std::vector<double2> zk(static_cast<size_t>(R)), zb(static_cast<size_t>(R));
+  std::vector<int> live(static_cast<size_t>(R));
+  DevBuf<double> d_c1(static_cast<size_t>(R)), d_c2(static_cast<size_t>(R)), d_bk(static_cast<size_t>(R)),
+      d_bm(static_cast<size_t>(R)), dbt(static_cast<size_t>(R));
+  DevBuf<double2> d_zb(static_cast<size_t>(R)), d_zk(static_cast<size_t>(R));
+  DevBuf<int> dalive(static_cast<size_t>(R));
+  const Sector* Hs = ts.fused ? static_cast<const Sector*>(op) : nullptr;
+  const size_t bil_lds =
+      ts.fused ? static_cast<size_t>(ts.ntab) * 2 * ts.nbp * sizeof(double2) + 2 * static_cast<size_t>(ts.nbp) * 4 : 0;
+  auto beta_at = [&](int k, int r) { return k >= 0 ? bs[static_cast<size_t>(k) * R + r] : 0.0; };
+  for (int c = (kmax - 1) / s; c >= 0; --c) {
+    const int j0 = c * s, j1 = std::min(j0 + s, kmax);
+    FTLM_HIP_CHECK(hipMemcpyAsync(seg[0].p, T->ck[2 * c].p, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    FTLM_HIP_CHECK(hipMemcpyAsync(seg[1].p, T->ck[2 * c + 1].p, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    // recompute q_{j0+1} .. q_{j1} with the stored scalars and the forward pass's kernels; at a
+    // vector's last step the unnormalised w is kept
+    for (int j = j0; j < j1; ++j) {
+      double2* X = seg[static_cast<size_t>(j - j0 + 2)].p;
+      const double2* qj = seg[static_cast<size_t>(j - j0 + 1)].p;
+      const double2* qjm = seg[static_cast<size_t>(j - j0)].p;
+      hv_apply(op, qj, X, R, FTLM_C128, nullptr, nullptr, st);
+      for (int r = 0; r < R; ++r) {
+        live[r] = j < used[r] ? 1 : 0;
+        zk[r] = zs[static_cast<size_t>(j) * R + r];
+        bm[r] = beta_at(j - 1, r);
+        scale[r] = j + 1 < used[r] ? bs[static_cast<size_t>(j) * R + r] : 1.0;
+      }
+      d_zk.upload(zk, st);
+      d_bm.upload(bm, st);
+      dalive.upload(live, st);
+      hipLaunchKernelGGL(k_gstep_update, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, qj, qjm, X, GradVecs{}, d_zk.p,
+                         d_bm.p, dalive.p, j > 0 ? 1 : 0, part.p);
+      dbt.upload(scale, st);
+      hipLaunchKernelGGL(k_gstep_norm, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, X, GradVecs{}, dbt.p, dalive.p);
+      check_launch("recompute");
+    }
+    for (int k = j1 - 1; k >= j0; --k) {
+      const double2* qm = seg[static_cast<size_t>(k - j0)].p;
+      const double2* qk = seg[static_cast<size_t>(k - j0 + 1)].p;
+      const double2* X = seg[static_cast<size_t>(k - j0 + 2)].p;
+      for (int r = 0; r < R; ++r) {
+        const bool inner = k + 1 < used[r];  // q_{k+1} and beta_k feed the quadrature
+        const double b = bs[static_cast<size_t>(k) * R + r];
+        c1[r] = inner ? 1.0 / b : 0.0;
+        c2[r] = inner ? bbar[static_cast<size_t>(r) * M + k] + bacc[r] - rho[r] / b : 0.0;
+      }
+      d_c1.upload(c1, st);
+      d_c2.upload(c2, st);
+      hipLaunchKernelGGL(k_adj_wb, dim3(gblk), dim3(kGBlock), 0, st, n, R, A.p, X, qk, qm, d_c1.p, d_c2.p, WA.p,
+                         part.p);
+      check_launch("k_adj_wb");
+      collect(2);
+      for (int r = 0; r < R; ++r) {
+        const double2 cc = hres[r];
+        zb[r] = k < used[r] ? make_double2(abar[static_cast<size_t>(r) * M + k] - cc.x, -cc.y) : make_double2(0.0, 0.0);
+        bacc[r] = -hres[static_cast<size_t>(R) + r].x;  // adjoint of beta_{k-1} from this step
+      }
+      d_zb.upload(zb, st);
+      hipLaunchKernelGGL(k_adj_wa, dim3(gblk), dim3(kGBlock), 0, st, n, R, WA.p, qk, d_zb.p);
+      check_launch("k_adj_wa");
+      // parameter adjoints: Re <w_a-bar | dH_p q_k>
+      if (ts.fused) {
+        hipLaunchKernelGGL(k_bil_sector, dim3(gbil), dim3(kGBlock), bil_lds, st, Hs->view(), WA.p, qk, R, ts.dc.p,
+                           ts.ntab, ts.nbp, part.p);
+        hipLaunchKernelGGL(k_tsum, dim3(kTsQ), dim3(kGBlock), 0, st, part.p, static_cast<int64_t>(gbil), kTsQ, tacc.p);
+        check_launch("k_bil_sector");
+      } else {
+        for (int p = 0; p < npar; ++p) {
+          hv_apply(T->dops[p], qk, tmp.p, R, FTLM_C128, nullptr, nullptr, st);
+          hipLaunchKernelGGL(k_gstep_dots, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, WA.p, tmp.p, GradVecs{},
+                             part.p);
+          hipLaunchKernelGGL(k_tsum, dim3(1), dim3(kGBlock), 0, st, part.p, static_cast<int64_t>(gblk) * R, 1,
+                             tacc.p + p);
+        }
+        check_launch("derivative dots");
+      }
+      if (k == 0) break;  // the start vector does not depend on the parameters
+      hv_apply(op, WA.p, Y.p, R, FTLM_C128, nullptr, nullptr, st);
+      for (int r = 0; r < R; ++r) {
+        zk[r] = zs[static_cast<size_t>(k) * R + r];
+        bk[r] = k + 1 < used[r] ? bs[static_cast<size_t>(k) * R + r] : (k < used[r] ? 1.0 : 0.0);
+        bm[r] = beta_at(k - 1, r);
+      }
+      d_zk.upload(zk, st);
+      d_bk.upload(bk, st);
+      d_bm.upload(bm, st);
+      hipLaunchKernelGGL(k_adj_qbar, dim3(gblk), dim3(kGBlock), 0, st, n, R, P.p, A.p, WA.p, Y.p, X, qk, qm, d_zb.p,
+                         d_zk.p, d_bk.p, d_bm.p, part.p);
+      check_launch("k_adj_qbar");
+      collect(1);
+      for (int r = 0; r < R; ++r) rho[r] = hres[r].x;
+      std::swap(A.p, P.p);
+    }
+  }
+  std::vector<double> hacc(static_cast<size_t>(nacc));
+  FTLM_HIP_CHECK(hipMemcpy(hacc.data(), tacc.p, hacc.size() * sizeof(double), hipMemcpyDeviceToHost));
+  for (int p = 0; p < npar; ++p) {
+    if (!ts.fused) {
+      grad[p] = hacc[p];
+      continue;
+    }
+    double g = ts.dU[p] * hacc[kTsTables] + ts.dV[p] * hacc[kTsTables + 1];
+    if (ts.tab[p] >= 0) g += hacc[ts.tab[p]];
+    grad[p] = g;
+  }
+}
+
+}  // namespace
+}  // namespace ftlm_hip
+
 extern "C" {
 
 int ftlm_hip_ftlm_gradient(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
@@ -869,312 +1228,42 @@ int ftlm_hip_ftlm_gradient(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar
   });
 }
 
+
+int ftlm_hip_ftlm_tape_forward(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
+                               const double* betas, int nb, int ckpt, int mode, double* lnZ, ftlm_tape_t** out) {
+  return guarded([&] {
+    FTLM_REQUIRE(out != nullptr, FTLM_EINVAL, "null output handle");
+    *out = nullptr;
+    auto T = std::make_unique<ftlm_tape>();
+    tape_forward(T.get(), op, dops, npar, opts, betas, nb, ckpt, mode, lnZ);
+    *out = T.release();
+  });
+}
+
+int ftlm_hip_ftlm_tape_backward(const ftlm_tape_t* tape, const double* gw, double* grad) {
+  return guarded([&] {
+    FTLM_REQUIRE(tape != nullptr, FTLM_EINVAL, "null tape");
+    tape_backward(tape, gw, grad);
+  });
+}
+
+int ftlm_hip_ftlm_tape_destroy(ftlm_tape_t* tape) {
+  return guarded([&] { delete tape; });
+}
+
 int ftlm_hip_ftlm_vjp(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
                       const double* betas, int nb, const double* gw, int ckpt, int mode, double* lnZ, double* grad) {
   return guarded([&] {
-    FTLM_REQUIRE(op != nullptr && opts != nullptr && betas != nullptr && gw != nullptr && lnZ != nullptr && nb >= 1,
-                 FTLM_EINVAL, "null argument");
-    FTLM_REQUIRE(npar >= 0 && npar <= kGMaxPar && (npar == 0 || (dops != nullptr && grad != nullptr)), FTLM_EINVAL,
-                 "bad derivative operators");
-    FTLM_REQUIRE(mode == FTLM_TS_AUTO || mode == FTLM_TS_GENERIC || mode == FTLM_TS_FUSED, FTLM_EINVAL, "bad mode");
-    FTLM_REQUIRE(ckpt >= 0, FTLM_EINVAL, "negative checkpoint interval");
-    FTLM_REQUIRE(opts->start == FTLM_START_GAUSS_MT || opts->start == FTLM_START_PHASE, FTLM_EINVAL, "bad start");
-    for (int p = 0; p < npar; ++p)
-      FTLM_REQUIRE(dops[p] != nullptr && dops[p]->dim == op->dim, FTLM_EINVAL, "derivative operator of another size");
-    FTLM_HIP_CHECK(hipSetDevice(op->device));
-    const int64_t dim = op->dim;
-    const int R = std::max(1, opts->R);
-    int M = std::max(1, opts->M);
-    if (opts->clamp_M_to_dim) M = static_cast<int>(std::min<int64_t>(M, std::max<int64_t>(dim, 1)));
-    FTLM_REQUIRE(R <= 32, FTLM_EUNSUPPORTED, "at most 32 samples per gradient call");
-    FTLM_REQUIRE(M <= kGMaxM, FTLM_EUNSUPPORTED, "at most 512 Lanczos steps per gradient call");
-    if (dim <= 0) {
+    FTLM_REQUIRE(op != nullptr && gw != nullptr && lnZ != nullptr && (npar == 0 || grad != nullptr), FTLM_EINVAL,
+                 "null argument");
+    if (op->dim <= 0) {
       for (int b = 0; b < nb; ++b) lnZ[b] = -INFINITY;
       for (int i = 0; i < npar; ++i) grad[i] = 0.0;
       return;
     }
-    FTLM_REQUIRE(dim > 1, FTLM_EUNSUPPORTED, "a one-state operator has the closed form -beta H00");
-    // the adjoint sweep applies H in place of H^dag
-    FTLM_REQUIRE(op->kind != kKBlock || kblock_hermitian(op), FTLM_EUNSUPPORTED,
-                 "this momentum-block operator is not Hermitian (its hop coefficients are not translation "
-                 "invariant, e.g. a Peierls flux on the wrapped bonds); use ftlm_hip_ftlm_gradient");
-    TermSplit ts;
-    if (mode != FTLM_TS_GENERIC && npar > 0) build_termsplit(op, dops, npar, &ts);
-    FTLM_REQUIRE(mode != FTLM_TS_FUSED || ts.fused || npar == 0, FTLM_EUNSUPPORTED,
-                 "these derivative operators have no term-split form over this operator");
-    const hipStream_t st = nullptr;
-    const int64_t n = dim * R;
-    const size_t vb = static_cast<size_t>(n);
-    const int s = std::min(M, ckpt > 0 ? ckpt : std::max(1, static_cast<int>(std::lround(std::sqrt(2.0 * M)))));
-    const int nck = (M + s - 1) / s;
-    // seg[i] = q_{j0-1+i} of the segment being reversed; ck[2c], ck[2c+1] = q_{cs-1}, q_{cs}
-    std::vector<DevBuf<double2>> seg(static_cast<size_t>(s) + 2), ck(2 * static_cast<size_t>(nck));
-    for (auto& b : seg) b.alloc(vb);
-    for (auto& b : ck) b.alloc(vb);
-    DevBuf<double2> A(vb), P(vb), WA(vb), Y(vb), tmp(ts.fused || npar == 0 ? 0 : vb);
-    const int bsz = (kGBlock / R) * R;
-    const int gblk = static_cast<int>(std::min<int64_t>(kGMaxGrid, (n + bsz - 1) / bsz));
-    const int gbil = static_cast<int>(std::min<int64_t>(kGMaxGrid, (n + kGBlock - 1) / kGBlock));
-    DevBuf<double2> part(std::max(static_cast<size_t>(gblk) * 2 * R, static_cast<size_t>(gbil) * kTsQ));
-    DevBuf<double2> dres(static_cast<size_t>(2) * R);
-    std::vector<double2> hres(static_cast<size_t>(2) * R);
-    auto collect = [&](int nq) {
-      hipLaunchKernelGGL(k_gsum, dim3(nq * R), dim3(kGBlock), 0, st, part.p, gblk, nq * R, dres.p);
-      check_launch("k_gsum");
-      FTLM_HIP_CHECK(hipMemcpyAsync(hres.data(), dres.p, sizeof(double2) * nq * R, hipMemcpyDeviceToHost, st));
-      FTLM_HIP_CHECK(hipStreamSynchronize(st));
-    };
-    // ---- forward pass: the recurrence of ftlm_hip_ftlm_gradient without tangents, keeping
-    // (q_{k-1}, q_k) every s steps
-    double2* qp = seg[0].p;
-    double2* q = seg[1].p;
-    double2* w = seg[2 % seg.size()].p;
-    FTLM_HIP_CHECK(hipMemsetAsync(qp, 0, vb * sizeof(double2), st));
-    std::vector<unsigned> seeds(static_cast<size_t>(R));
-    for (int r = 0; r < R; ++r) seeds[r] = opts->seed + static_cast<unsigned>(r) * 100003u;
-    if (opts->start == FTLM_START_PHASE) {
-      DevBuf<uint32_t> ids(static_cast<size_t>(R));
-      std::vector<uint32_t> hid(seeds.begin(), seeds.end());
-      ids.upload(hid, st);
-      hipLaunchKernelGGL(k_gphase, dim3(grid_n(dim)), dim3(kGBlock), 0, st, dim, R, ids.p, q);
-      check_launch("k_gphase");
-      FTLM_HIP_CHECK(hipStreamSynchronize(st));
-    } else {
-      std::vector<double2> h(vb);
-      std::vector<std::complex<double>> g;
-      for (int r = 0; r < R; ++r) {
-        gauss_mt(dim, seeds[r], &g);
-        for (int64_t i = 0; i < dim; ++i) h[static_cast<size_t>(i) * R + r] = make_double2(g[i].real(), g[i].imag());
-      }
-      FTLM_HIP_CHECK(hipMemcpy(q, h.data(), vb * sizeof(double2), hipMemcpyHostToDevice));
-    }
-    hipLaunchKernelGGL(k_gstep_dots, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, q, q, GradVecs{}, part.p);
-    check_launch("k_gstep_dots");
-    collect(1);
-    for (int r = 0; r < R; ++r) FTLM_REQUIRE(std::sqrt(hres[r].x) >= 1e-18, FTLM_EUNSUPPORTED, "zero start vector");
-    hipLaunchKernelGGL(k_gscale, dim3(gblk), dim3(kGBlock), 0, st, n, R, q, dres.p);
-    check_launch("k_gscale");
-    // per step and vector: z_k (dead vectors 0), beta_k; used[r] = steps taken by vector r
-    std::vector<double2> zs(static_cast<size_t>(M) * R, make_double2(0.0, 0.0));
-    std::vector<double> bs(static_cast<size_t>(M) * R, 0.0);
-    std::vector<int> used(static_cast<size_t>(R), 0), alive(static_cast<size_t>(R), 1);
-    std::vector<double> bprev(static_cast<size_t>(R), 0.0), bet(static_cast<size_t>(R), 0.0);
-    DevBuf<double2> dz(static_cast<size_t>(R));
-    DevBuf<double> dbp(static_cast<size_t>(R)), dbt(static_cast<size_t>(R));
-    DevBuf<int> dalive(static_cast<size_t>(R));
-    for (int k = 0; k < M; ++k) {
-      if (k % s == 0) {
-        const int c = k / s;
-        FTLM_HIP_CHECK(hipMemcpyAsync(ck[2 * c].p, qp, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
-        FTLM_HIP_CHECK(hipMemcpyAsync(ck[2 * c + 1].p, q, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
-      }
-      hv_apply(op, q, w, R, FTLM_C128, nullptr, nullptr, st);
-      hipLaunchKernelGGL(k_gstep_dots, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, q, w, GradVecs{}, part.p);
-      check_launch("k_gstep_dots");
-      collect(1);
-      std::vector<double2> z(static_cast<size_t>(R));
-      for (int r = 0; r < R; ++r) {
-        z[r] = alive[r] ? hres[r] : make_double2(0.0, 0.0);
-        zs[static_cast<size_t>(k) * R + r] = z[r];
-      }
-      dz.upload(z, st);
-      dbp.upload(bprev, st);
-      dalive.upload(alive, st);
-      hipLaunchKernelGGL(k_gstep_update, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, q, qp, w, GradVecs{}, dz.p, dbp.p,
-                         dalive.p, k > 0 ? 1 : 0, part.p);
-      check_launch("k_gstep_update");
-      collect(1);
-      std::fill(bet.begin(), bet.end(), 0.0);
-      for (int r = 0; r < R; ++r) {
-        if (!alive[r]) continue;
-        used[r] = k + 1;
-        const double nw2 = hres[r].x;
-        if (nw2 < 1e-24) {
-          alive[r] = 0;
-          continue;
-        }
-        const double b = std::sqrt(nw2);
-        bs[static_cast<size_t>(k) * R + r] = b;
-        bet[r] = b;
-        bprev[r] = b;
-      }
-      bool any = false;
-      for (int r = 0; r < R; ++r) any = any || alive[r];
-      if (!any || k + 1 == M) break;
-      dbt.upload(bet, st);
-      dalive.upload(alive, st);
-      hipLaunchKernelGGL(k_gstep_norm, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, w, GradVecs{}, dbt.p, dalive.p);
-      check_launch("k_gstep_norm");
-      double2* t = qp;
-      qp = q;
-      q = w;
-      w = t;
-    }
-    // ---- quadrature: L_r(beta_b) and dL_r/d(alpha_k, beta_k)
-    int kmax = 0;
-    for (int r = 0; r < R; ++r) kmax = std::max(kmax, used[r]);
-    std::vector<double> ha(static_cast<size_t>(R) * M, 0.0), hb(static_cast<size_t>(R) * M, 0.0);
-    for (int r = 0; r < R; ++r)
-      for (int k = 0; k < used[r]; ++k) {
-        ha[static_cast<size_t>(r) * M + k] = zs[static_cast<size_t>(k) * R + r].x;
-        hb[static_cast<size_t>(r) * M + k] = bs[static_cast<size_t>(k) * R + r];
-      }
-    {
-      DevBuf<double> da(ha.size()), db(hb.size()), dbeta(static_cast<size_t>(nb));
-      da.upload(ha, st);
-      db.upload(hb, st);
-      dbeta.upload(betas, static_cast<size_t>(nb), st);
-      DevBuf<int> dn(static_cast<size_t>(R));
-      dn.upload(used, st);
-      DevBuf<double> Uw(static_cast<size_t>(R) * M * M), Wm(static_cast<size_t>(R) * M * M);
-      DevBuf<double> Ls(static_cast<size_t>(R) * nb), G(static_cast<size_t>(nb) * R * M * 2);
-      DevBuf<int> status(1);
-      FTLM_HIP_CHECK(hipMemsetAsync(status.p, 0, sizeof(int), st));
-      FTLM_HIP_CHECK(hipMemsetAsync(G.p, 0, G.n * sizeof(double), st));
-      hipLaunchKernelGGL(k_grad_tridiag, dim3(R), dim3(kGBlock), 0, st, da.p, db.p, static_cast<const double*>(nullptr),
-                         static_cast<const double*>(nullptr), dn.p, R, M, 0, dbeta.p, nb, Uw.p, Wm.p, Ls.p,
-                         static_cast<double*>(nullptr), G.p, status.p);
-      check_launch("k_grad_tridiag");
-      int bad = 0;
-      FTLM_HIP_CHECK(hipMemcpy(&bad, status.p, sizeof(int), hipMemcpyDeviceToHost));
-      FTLM_REQUIRE(bad == 0, FTLM_EHIP, "tridiagonal QL did not converge");
-      std::vector<double> hL(Ls.n), hG(G.n);
-      FTLM_HIP_CHECK(hipMemcpy(hL.data(), Ls.p, hL.size() * sizeof(double), hipMemcpyDeviceToHost));
-      FTLM_HIP_CHECK(hipMemcpy(hG.data(), G.p, hG.size() * sizeof(double), hipMemcpyDeviceToHost));
-      // lnZ_b = ln dim + log-mean-exp_r L_r(b); d(sum_b gw_b lnZ_b)/dL_r(b) = gw_b softmax_r(b)
-      std::fill(ha.begin(), ha.end(), 0.0);  // reused: abar[r][k]
-      std::fill(hb.begin(), hb.end(), 0.0);  // reused: bbar[r][k] (from the quadrature)
-      for (int b = 0; b < nb; ++b) {
-        double mx = -INFINITY;
-        for (int r = 0; r < R; ++r) mx = std::max(mx, hL[static_cast<size_t>(r) * nb + b]);
-        double zsum = 0.0;
-        for (int r = 0; r < R; ++r) zsum += std::exp(hL[static_cast<size_t>(r) * nb + b] - mx);
-        lnZ[b] = std::log(static_cast<double>(dim)) + mx + std::log(zsum / R);
-        for (int r = 0; r < R; ++r) {
-          const double wgt = gw[b] * std::exp(hL[static_cast<size_t>(r) * nb + b] - mx) / zsum;
-          for (int k = 0; k < used[r]; ++k) {
-            const double* g = &hG[((static_cast<size_t>(b) * R + r) * M + k) * 2];
-            ha[static_cast<size_t>(r) * M + k] += wgt * g[0];
-            hb[static_cast<size_t>(r) * M + k] += wgt * g[1];
-          }
-        }
-      }
-    }
-    if (npar == 0) return;
-    // ---- backward pass over segments [j0, j1), latest first
-    const int nacc = ts.fused ? kTsQ : npar;
-    DevBuf<double> tacc(static_cast<size_t>(nacc));
-    FTLM_HIP_CHECK(hipMemsetAsync(tacc.p, 0, tacc.n * sizeof(double), st));
-    FTLM_HIP_CHECK(hipMemsetAsync(A.p, 0, vb * sizeof(double2), st));
-    FTLM_HIP_CHECK(hipMemsetAsync(P.p, 0, vb * sizeof(double2), st));
-    std::vector<double> rho(static_cast<size_t>(R), 0.0), bacc(static_cast<size_t>(R), 0.0);
-    std::vector<double> c1(static_cast<size_t>(R)), c2(static_cast<size_t>(R)), bk(static_cast<size_t>(R)),
-        bm(static_cast<size_t>(R)), scale(static_cast<size_t>(R));
-    std::vector<double2> zk(static_cast<size_t>(R)), zb(static_cast<size_t>(R));
-    std::vector<int> live(static_cast<size_t>(R));
-    DevBuf<double> d_c1(static_cast<size_t>(R)), d_c2(static_cast<size_t>(R)), d_bk(static_cast<size_t>(R)),
-        d_bm(static_cast<size_t>(R));
-    DevBuf<double2> d_zb(static_cast<size_t>(R)), d_zk(static_cast<size_t>(R));
-    const Sector* Hs = ts.fused ? static_cast<const Sector*>(op) : nullptr;
-    const size_t bil_lds =
-        ts.fused ? static_cast<size_t>(ts.ntab) * 2 * ts.nbp * sizeof(double2) + 2 * static_cast<size_t>(ts.nbp) * 4 : 0;
-    auto beta_at = [&](int k, int r) { return k >= 0 ? bs[static_cast<size_t>(k) * R + r] : 0.0; };
-    for (int c = (kmax - 1) / s; c >= 0; --c) {
-      const int j0 = c * s, j1 = std::min(j0 + s, kmax);
-      FTLM_HIP_CHECK(hipMemcpyAsync(seg[0].p, ck[2 * c].p, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
-      FTLM_HIP_CHECK(hipMemcpyAsync(seg[1].p, ck[2 * c + 1].p, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
-      // recompute q_{j0+1} .. q_{j1} with the stored scalars and the forward pass's kernels; at a
-      // vector's last step the unnormalised w is kept
-      for (int j = j0; j < j1; ++j) {
-        double2* X = seg[static_cast<size_t>(j - j0 + 2)].p;
-        const double2* qj = seg[static_cast<size_t>(j - j0 + 1)].p;
-        const double2* qjm = seg[static_cast<size_t>(j - j0)].p;
-        hv_apply(op, qj, X, R, FTLM_C128, nullptr, nullptr, st);
-        for (int r = 0; r < R; ++r) {
-          live[r] = j < used[r] ? 1 : 0;
-          zk[r] = zs[static_cast<size_t>(j) * R + r];
-          bm[r] = beta_at(j - 1, r);
-          scale[r] = j + 1 < used[r] ? bs[static_cast<size_t>(j) * R + r] : 1.0;
-        }
-        dz.upload(zk, st);
-        dbp.upload(bm, st);
-        dalive.upload(live, st);
-        hipLaunchKernelGGL(k_gstep_update, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, qj, qjm, X, GradVecs{}, dz.p,
-                           dbp.p, dalive.p, j > 0 ? 1 : 0, part.p);
-        dbt.upload(scale, st);
-        hipLaunchKernelGGL(k_gstep_norm, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, X, GradVecs{}, dbt.p, dalive.p);
-        check_launch("recompute");
-      }
-      for (int k = j1 - 1; k >= j0; --k) {
-        const double2* qm = seg[static_cast<size_t>(k - j0)].p;
-        const double2* qk = seg[static_cast<size_t>(k - j0 + 1)].p;
-        const double2* X = seg[static_cast<size_t>(k - j0 + 2)].p;
-        for (int r = 0; r < R; ++r) {
-          const bool inner = k + 1 < used[r];  // q_{k+1} and beta_k feed the quadrature
-          const double b = bs[static_cast<size_t>(k) * R + r];
-          c1[r] = inner ? 1.0 / b : 0.0;
-          c2[r] = inner ? hb[static_cast<size_t>(r) * M + k] + bacc[r] - rho[r] / b : 0.0;
-        }
-        d_c1.upload(c1, st);
-        d_c2.upload(c2, st);
-        hipLaunchKernelGGL(k_adj_wb, dim3(gblk), dim3(kGBlock), 0, st, n, R, A.p, X, qk, qm, d_c1.p, d_c2.p, WA.p,
-                           part.p);
-        check_launch("k_adj_wb");
-        collect(2);
-        for (int r = 0; r < R; ++r) {
-          const double2 cc = hres[r];
-          zb[r] = k < used[r] ? make_double2(ha[static_cast<size_t>(r) * M + k] - cc.x, -cc.y) : make_double2(0.0, 0.0);
-          bacc[r] = -hres[static_cast<size_t>(R) + r].x;  // adjoint of beta_{k-1} from this step
-        }
-        d_zb.upload(zb, st);
-        hipLaunchKernelGGL(k_adj_wa, dim3(gblk), dim3(kGBlock), 0, st, n, R, WA.p, qk, d_zb.p);
-        check_launch("k_adj_wa");
-        // parameter adjoints: Re <w_a-bar | dH_p q_k>
-        if (ts.fused) {
-          hipLaunchKernelGGL(k_bil_sector, dim3(gbil), dim3(kGBlock), bil_lds, st, Hs->view(), WA.p, qk, R, ts.dc.p,
-                             ts.ntab, ts.nbp, part.p);
-          hipLaunchKernelGGL(k_tsum, dim3(kTsQ), dim3(kGBlock), 0, st, part.p, static_cast<int64_t>(gbil), kTsQ,
-                             tacc.p);
-          check_launch("k_bil_sector");
-        } else {
-          for (int p = 0; p < npar; ++p) {
-            hv_apply(dops[p], qk, tmp.p, R, FTLM_C128, nullptr, nullptr, st);
-            hipLaunchKernelGGL(k_gstep_dots, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, WA.p, tmp.p, GradVecs{},
-                               part.p);
-            hipLaunchKernelGGL(k_tsum, dim3(1), dim3(kGBlock), 0, st, part.p, static_cast<int64_t>(gblk) * R, 1,
-                               tacc.p + p);
-          }
-          check_launch("derivative dots");
-        }
-        if (k == 0) break;  // the start vector does not depend on the parameters
-        hv_apply(op, WA.p, Y.p, R, FTLM_C128, nullptr, nullptr, st);
-        for (int r = 0; r < R; ++r) {
-          zk[r] = zs[static_cast<size_t>(k) * R + r];
-          bk[r] = k + 1 < used[r] ? bs[static_cast<size_t>(k) * R + r] : (k < used[r] ? 1.0 : 0.0);
-          bm[r] = beta_at(k - 1, r);
-        }
-        d_zk.upload(zk, st);
-        d_bk.upload(bk, st);
-        d_bm.upload(bm, st);
-        hipLaunchKernelGGL(k_adj_qbar, dim3(gblk), dim3(kGBlock), 0, st, n, R, P.p, A.p, WA.p, Y.p, X, qk, qm, d_zb.p,
-                           d_zk.p, d_bk.p, d_bm.p, part.p);
-        check_launch("k_adj_qbar");
-        collect(1);
-        for (int r = 0; r < R; ++r) rho[r] = hres[r].x;
-        std::swap(A.p, P.p);
-      }
-    }
-    std::vector<double> hacc(static_cast<size_t>(nacc));
-    FTLM_HIP_CHECK(hipMemcpy(hacc.data(), tacc.p, hacc.size() * sizeof(double), hipMemcpyDeviceToHost));
-    for (int p = 0; p < npar; ++p) {
-      if (!ts.fused) {
-        grad[p] = hacc[p];
-        continue;
-      }
-      double g = ts.dU[p] * hacc[kTsTables] + ts.dV[p] * hacc[kTsTables + 1];
-      if (ts.tab[p] >= 0) g += hacc[ts.tab[p]];
-      grad[p] = g;
-    }
+    ftlm_tape T;
+    tape_forward(&T, op, dops, npar, opts, betas, nb, ckpt, mode, lnZ);
+    tape_backward(&T, gw, grad);
   });
 }
 

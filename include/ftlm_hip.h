@@ -245,6 +245,17 @@ int ftlm_hip_ftlm_gradient(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar
 int ftlm_hip_ftlm_vjp(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
                       const double* betas, int nb, const double* gw, int ckpt, int mode, double* lnZ, double* grad);
 
+/* The two halves of ftlm_hip_ftlm_vjp, for callers that choose the weights after the forward pass
+ * (reverse-mode autodiff) or need several weighted rows (a Jacobian).  tape_forward runs the
+ * forward pass, returns lnZ and keeps the checkpoints, scalars and quadrature derivatives on the
+ * tape; tape_backward(gw) runs one adjoint sweep and may be called any number of times.  op and
+ * dops must stay alive until tape_destroy.  No counterpart in the reference. */
+typedef struct ftlm_tape ftlm_tape_t;
+int ftlm_hip_ftlm_tape_forward(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
+                               const double* betas, int nb, int ckpt, int mode, double* lnZ, ftlm_tape_t** out);
+int ftlm_hip_ftlm_tape_backward(const ftlm_tape_t* tape, const double* gw, double* grad);
+int ftlm_hip_ftlm_tape_destroy(ftlm_tape_t* tape);
+
 /* ------------------------------------------------------------------ host-callback operators */
 /* The operator argument of lanczos_tridiagonal / lanczos_extrema / ftlm_log_partition_complex
  * (std::function over host complex<double> arrays, include/ftlm/lanczos.hpp:47-63,

@@ -110,3 +110,15 @@ def test_geometry_derivative(model):
         d = _merged(api.geometry_derivative(p, which))
         for k in set(fp) | set(fm) | set(d):
             assert abs((fp[k] - fm[k]) / (2 * h) - d[k]) < 1e-8, (which, k)
+
+
+def test_autograd_module_surface():
+    """The torch.autograd wrapper imports without a device and checks theta's shape first."""
+    import torch
+
+    from ftlm_extended_hubbard_amd import autograd as AG
+
+    assert AG.THETA == ("t", "tp", "U", "V", "phi_x", "phi_y", "mu")
+    with pytest.raises(ValueError, match="7 entries"):
+        AG.ftlm_log_partition(torch.zeros(3, dtype=torch.float64), AG.FtlmModel(2, 2, 1, 1), [1.0],
+                              AG.api.FtlmParams())

@@ -406,3 +406,33 @@ def test_vjp_breakdown_vs_central_differences():
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(got[name] - fd) <= 1e-6 * max(abs(fd), 1e-3), (name, got[name], fd)
         assert abs(np.dot(w, g[name]) - fd) <= 1e-6 * max(abs(fd), 1e-3), (name, g[name], fd)
+
+
+def test_autograd_function_matches_forward_mode():
+    """torch.autograd through the tape: backward of sum_b w_b lnZ_b and the full Jacobian
+    (torch.autograd.functional.jacobian: one adjoint sweep per beta on one forward tape) against
+    the forward-mode derivatives; d/dmu = beta N."""
+    import torch
+
+    from ftlm_extended_hubbard_amd import autograd as AG
+
+    say("autograd")
+    model = AG.FtlmModel(4, 2, 3, 2)
+    betas = np.array([0.5, 2.0, 8.0])
+    par = api.FtlmParams(n_random=4, lanczos_steps=24, seed=11, quad=L.FTLM_QUAD_CONVERGED)
+    th0 = [1.0, -0.25, 8.0, 1.0, 0.05, 0.02, 0.7]
+    theta = torch.tensor(th0, dtype=torch.float64, requires_grad=True)
+    lnz = AG.ftlm_log_partition(theta, model, betas, par)
+    p = api.HubbardParams(4, 2, *th0[:6])
+    names = ("t", "tp", "U", "V", "phi_x", "phi_y")
+    lnz_f, g = api.ftlm_log_partition_gradient(p, 3, 2, betas, par, params=names + ("mu",), mu=th0[6])
+    assert np.max(np.abs(lnz.detach().numpy() - lnz_f)) < 1e-10
+    w = torch.tensor([0.3, -1.1, 0.7], dtype=torch.float64)
+    (lnz * w).sum().backward()
+    for i, k in enumerate(names + ("mu",)):
+        want = float(np.dot(w.numpy(), g[k]))
+        assert abs(theta.grad[i].item() - want) <= 1e-9 * max(abs(want), 1e-3), (k, theta.grad[i].item(), want)
+    jac = torch.autograd.functional.jacobian(lambda th: AG.ftlm_log_partition(th, model, betas, par),
+                                             theta.detach().clone())
+    for i, k in enumerate(names + ("mu",)):
+        assert np.allclose(jac[:, i].numpy(), g[k], rtol=1e-9, atol=1e-12), (k, jac[:, i], g[k])
