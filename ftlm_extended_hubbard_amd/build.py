@@ -104,8 +104,17 @@ def build_host(verbose: bool = False) -> None:
 
     with cf.ThreadPoolExecutor(max_workers=max(1, min(4, len(srcs)))) as ex:
         list(ex.map(one, srcs))
+    # the grand-canonical n(mu) drivers (C ABI only)
+    bin_dir = os.path.join(PKG, "bin")
+    os.makedirs(bin_dir, exist_ok=True)
+    drv = os.path.join(HOST, "drivers", "ftlm_nmu.cpp")
+    for name, k in (("ftlm_nmu_rect", 0), ("ftlm_nmu_rect_k", 1)):
+        out = os.path.join(bin_dir, name)
+        _run([CXX, *HOST_FLAGS, f"-DFTLM_NMU_K={k}", drv, "-o", out + ".tmp", f"-L{PKG}", "-lftlm_hip",
+              "-Wl,-rpath,$ORIGIN/.."], name)
+        os.replace(out + ".tmp", out)
     if verbose:
-        print("built", lib, "and", len(srcs), "C++ tests")
+        print("built", lib, "and", len(srcs), "C++ tests, drivers in", bin_dir)
 
 
 def clean() -> None:

@@ -1,0 +1,397 @@
+// ftlm_nmu.cpp -- grand-canonical n(mu) at fixed beta on the device: the sector / momentum-block
+// orchestration of benchmarks/bench_ftlm_nmu_rect.cpp (built as ftlm_nmu_rect) and
+// benchmarks/bench_ftlm_nmu_4x4_k.cpp (built as ftlm_nmu_rect_k, FTLM_NMU_K=1), over the C ABI.
+//
+// Kept as the reference has them (they are the drivers' interface): the --key=value flags and
+// their defaults, the per-sector seed (every sector uses --seed) and per-block seed
+// seed + idx*257 + ky*17 + kx (bench_ftlm_nmu_4x4_k.cpp:1077), the ED cutoff for small blocks
+// (:587, 1072), the grand-canonical bookkeeping (bench_ftlm_nmu_rect.cpp:186-219) and the TSV /
+// METRIC output (:181-184, 232-244).  Every sector or block runs on the device:
+//   nonmom   ftlm_log_partition_real of every (N_up, N_dn) sector, all sectors in one
+//            ftlm_hip_ftlm_ritz call (one persistent launch when every sector is small);
+//   k-orbit  every momentum block of every sector as an orbit-representative operator
+//            (apply_orbit_k_block), FTLM (complex) above the ED cutoff, device dense ED below;
+//   ed       every block by device dense ED.
+// The reference's k-gram solver (Gram blocks, LAPACK zheev) is not built; k-orbit is the default
+// of ftlm_nmu_rect_k.  Device options beyond the reference: --start=gauss|phase,
+// --quad=capped|converged, --dtype=c128|c64 (defaults reproduce the reference).
+#include <sys/resource.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "ftlm_hip.h"
+
+#ifndef FTLM_NMU_K
+#define FTLM_NMU_K 0
+#endif
+
+namespace {
+
+using clock_type = std::chrono::high_resolution_clock;
+
+const char* find_arg(int argc, char** argv, const char* key) {
+  const std::string prefix = std::string("--") + key + "=";
+  for (int i = 1; i < argc; ++i)
+    if (std::strncmp(argv[i], prefix.c_str(), prefix.size()) == 0) return argv[i] + prefix.size();
+  return nullptr;
+}
+double arg_double(int argc, char** argv, const char* key, double dflt) {
+  const char* v = find_arg(argc, argv, key);
+  return v ? std::strtod(v, nullptr) : dflt;
+}
+int arg_int(int argc, char** argv, const char* key, int dflt) {
+  const char* v = find_arg(argc, argv, key);
+  return v ? std::atoi(v) : dflt;
+}
+bool has_arg(int argc, char** argv, const char* key) { return find_arg(argc, argv, key) != nullptr; }
+
+double peak_rss_bytes() {
+  struct rusage ru {};
+  if (getrusage(RUSAGE_SELF, &ru) != 0) return 0.0;
+  return static_cast<double>(ru.ru_maxrss) * 1024.0;
+}
+
+[[noreturn]] void fail(const char* what) {
+  std::cerr << "ftlm_nmu: " << what << ": " << ftlm_hip_last_error() << "\n";
+  std::exit(3);
+}
+void ok(int rc, const char* what) {
+  if (rc != FTLM_OK) fail(what);
+}
+
+double logsumexp2(double a, double b) {
+  if (!std::isfinite(a)) return b;
+  if (!std::isfinite(b)) return a;
+  const double m = std::max(a, b);
+  return m + std::log(std::exp(a - m) + std::exp(b - m));
+}
+
+struct Model {
+  int Lx = 4, Ly = 2;
+  ftlm_params_t p{};
+  std::vector<ftlm_hop_t> hops;
+  std::vector<ftlm_pair_t> pairs;
+};
+
+Model make_model(int argc, char** argv, int Lx, int Ly) {
+  Model m;
+  m.Lx = Lx;
+  m.Ly = Ly;
+  m.p.Lx = Lx;
+  m.p.Ly = Ly;
+  m.p.t = arg_double(argc, argv, "t", 1.0);
+  m.p.tp = arg_double(argc, argv, "tp", -0.35);
+  m.p.U = arg_double(argc, argv, "U", 5.75);
+  m.p.V = arg_double(argc, argv, "V", 0.9);
+  m.p.phi_x = 0.0;
+  m.p.phi_y = 0.0;
+  int nh = 0, np = 0;
+  ftlm_hip_build_geometry(&m.p, nullptr, 0, &nh, nullptr, 0, &np);  // sizes (FTLM_ERANGE expected)
+  m.hops.resize(static_cast<size_t>(std::max(nh, 1)));
+  m.pairs.resize(static_cast<size_t>(std::max(np, 1)));
+  ok(ftlm_hip_build_geometry(&m.p, m.hops.data(), nh, &nh, m.pairs.data(), np, &np), "build_hubbard_geometry");
+  m.hops.resize(static_cast<size_t>(nh));
+  m.pairs.resize(static_cast<size_t>(np));
+  return m;
+}
+
+// Device options: defaults give the reference's estimator (mt19937 Gaussian starts, capped
+// Jacobi quadrature, complex128 vectors).
+ftlm_ftlm_opts_t make_opts(int argc, char** argv, int R, int M, unsigned seed, int real_variant) {
+  ftlm_ftlm_opts_t o{};
+  o.R = R;
+  o.M = M;
+  o.seed = seed;
+  const char* st = find_arg(argc, argv, "start");
+  const char* qd = find_arg(argc, argv, "quad");
+  const char* dt = find_arg(argc, argv, "dtype");
+  o.start = (st && std::strcmp(st, "phase") == 0) ? FTLM_START_PHASE : FTLM_START_GAUSS_MT;
+  o.quad = (qd && std::strcmp(qd, "converged") == 0) ? FTLM_QUAD_CONVERGED : FTLM_QUAD_REFERENCE;
+  o.dtype = (dt && std::strcmp(dt, "c64") == 0) ? FTLM_C64 : FTLM_C128;
+  o.clamp_M_to_dim = 0;
+  o.real_variant = real_variant;
+  o.path = FTLM_PATH_AUTO;
+  return o;
+}
+
+int sector_index(int nu, int nd, int n_sites) { return nu * (n_sites + 1) + nd; }
+
+// n(mu) lines of bench_ftlm_nmu_rect.cpp:186-219.
+void write_nmu(std::ostream& out, const std::vector<double>& logZ, const std::vector<int>& Nelec, int n_sites,
+               double beta, double mu_min, double mu_max, int n_mu) {
+  const int nsec = static_cast<int>(logZ.size());
+  for (int k = 0; k < n_mu; ++k) {
+    const double tmu = static_cast<double>(k) / static_cast<double>(n_mu - 1);
+    const double mu = mu_min + tmu * (mu_max - mu_min);
+    double mx = -std::numeric_limits<double>::infinity();
+    for (int i = 0; i < nsec; ++i) {
+      if (!std::isfinite(logZ[i])) continue;
+      mx = std::max(mx, logZ[i] + beta * mu * static_cast<double>(Nelec[i]));
+    }
+    if (!std::isfinite(mx)) {
+      out << mu << "\t0\n";
+      continue;
+    }
+    double sum_w = 0.0, sum_Nw = 0.0;
+    for (int i = 0; i < nsec; ++i) {
+      if (!std::isfinite(logZ[i])) continue;
+      const double w = std::exp(logZ[i] + beta * mu * static_cast<double>(Nelec[i]) - mx);
+      sum_w += w;
+      sum_Nw += static_cast<double>(Nelec[i]) * w;
+    }
+    const double n_avg = (sum_w > 0.0) ? (sum_Nw / sum_w) / static_cast<double>(n_sites) : 0.0;
+    out << mu << "\t" << n_avg << "\n";
+  }
+}
+
+struct Common {
+  int Lx, Ly;
+  double beta, mu_min, mu_max;
+  int n_mu, n_rand, lz_steps;
+  unsigned seed;
+  const char* out_path;
+  int no_monitor;
+};
+
+Common parse_common(int argc, char** argv, int Ly_dflt, int nrand_dflt, int lz_dflt) {
+  Common c;
+  c.Lx = arg_int(argc, argv, "Lx", 4);
+  c.Ly = arg_int(argc, argv, "Ly", Ly_dflt);
+  c.beta = arg_double(argc, argv, "beta", 20.0);
+  c.mu_min = arg_double(argc, argv, "mu-min", -5.0);
+  c.mu_max = arg_double(argc, argv, "mu-max", 25.0);
+  c.n_mu = std::max(2, arg_int(argc, argv, "n-mu", 221));
+  c.n_rand = arg_int(argc, argv, "ftlm-random", nrand_dflt);
+  c.lz_steps = has_arg(argc, argv, "lanczos-steps") ? arg_int(argc, argv, "lanczos-steps", lz_dflt) : lz_dflt;
+  c.seed = static_cast<unsigned>(arg_int(argc, argv, "seed", 7));
+  c.out_path = find_arg(argc, argv, "out");
+  c.no_monitor = arg_int(argc, argv, "no-monitor", 0);
+  return c;
+}
+
+// Non-momentum solver: ftlm_log_partition_real of every sector (bench_ftlm_nmu_rect.cpp:92-246).
+int run_nonmom(int argc, char** argv, const char* metric_kind) {
+  const Common c = parse_common(argc, argv, 2, 12, 72);
+  const auto t_wall0 = clock_type::now();
+  const Model m = make_model(argc, argv, c.Lx, c.Ly);
+  const int n_sites = c.Lx * c.Ly;
+  const int nsec = (n_sites + 1) * (n_sites + 1);
+  std::vector<double> logZ(static_cast<size_t>(nsec), -std::numeric_limits<double>::infinity());
+  std::vector<int> Nelec(static_cast<size_t>(nsec), 0);
+  std::cout << "=== FTLM grand-canonical n(mu)  " << c.Lx << "x" << c.Ly << "  beta=" << c.beta << " ===\n";
+  std::cout << "t=" << m.p.t << " tp=" << m.p.tp << " U=" << m.p.U << " V=" << m.p.V << "\n";
+  std::cout << "ftlm_random=" << c.n_rand << " lanczos_steps=" << c.lz_steps << "  sectors=" << nsec << "\n";
+  if (!c.no_monitor) std::cout << "[monitor] wall time + peak RSS reported at exit (disable with --no-monitor=1)\n";
+
+  const auto t0 = clock_type::now();
+  // every non-empty sector as a device operator, then one FTLM call over all of them
+  std::vector<ftlm_op_t*> ops;
+  std::vector<int> idx_of_op, N_of_op;
+  std::vector<int64_t> dims;
+  for (int nu = 0; nu <= n_sites; ++nu)
+    for (int nd = 0; nd <= n_sites; ++nd) {
+      const int idx = sector_index(nu, nd, n_sites);
+      Nelec[static_cast<size_t>(idx)] = nu + nd;
+      ftlm_op_t* op = nullptr;
+      ok(ftlm_hip_sector_create(n_sites, nu, nd, m.hops.data(), static_cast<int>(m.hops.size()), m.pairs.data(),
+                                static_cast<int>(m.pairs.size()), m.p.U, m.p.V, &op),
+         "sector_create");
+      int64_t dim = 0;
+      ok(ftlm_hip_op_dim(op, &dim, nullptr, nullptr), "op_dim");
+      if (dim <= 0) {
+        ftlm_hip_op_destroy(op);
+        continue;
+      }
+      ops.push_back(op);
+      idx_of_op.push_back(idx);
+      N_of_op.push_back(nu + nd);
+      dims.push_back(dim);
+    }
+  const ftlm_ftlm_opts_t o = make_opts(argc, argv, c.n_rand, c.lz_steps, c.seed, 1);
+  const int nops = static_cast<int>(ops.size());
+  const size_t per = static_cast<size_t>(c.n_rand) * c.lz_steps;
+  std::vector<double> ritz(per * nops), weight(per * nops), lz(static_cast<size_t>(nops));
+  std::vector<int> used(static_cast<size_t>(c.n_rand) * nops);
+  ok(ftlm_hip_ftlm_ritz(ops.data(), nops, &o, ritz.data(), weight.data(), used.data(), nullptr, nullptr, nullptr),
+     "ftlm_ritz");
+  ok(ftlm_hip_ftlm_thermo(ritz.data(), weight.data(), used.data(), nops, c.n_rand, c.lz_steps, dims.data(),
+                          N_of_op.data(), &c.beta, 1, nullptr, 0, lz.data(), nullptr, nullptr, nullptr, nullptr),
+     "ftlm_thermo");
+  for (int i = 0; i < nops; ++i) {
+    logZ[static_cast<size_t>(idx_of_op[i])] = lz[i];
+    const int nu = idx_of_op[i] / (n_sites + 1), nd = idx_of_op[i] % (n_sites + 1);
+    std::cerr << "sector (" << nu << "," << nd << ") dim=" << dims[i] << " logZ=" << lz[i] << "\n";
+    ftlm_hip_op_destroy(ops[i]);
+  }
+  const double t_sectors = std::chrono::duration<double>(clock_type::now() - t0).count();
+  std::cout << "[timing] all sectors logZ: " << t_sectors << " s\n";
+
+  std::ostream* out = &std::cout;
+  std::ofstream file;
+  if (c.out_path) {
+    file.open(c.out_path);
+    if (!file) {
+      std::cerr << "ftlm_nmu_rect: cannot open " << c.out_path << "\n";
+      return 2;
+    }
+    out = &file;
+  }
+  *out << "# FTLM grand canonical  Lx=" << c.Lx << " Ly=" << c.Ly << "  beta=" << c.beta << "  t=" << m.p.t
+       << " tp=" << m.p.tp << " U=" << m.p.U << " V=" << m.p.V << "\n";
+  *out << "# ftlm_random=" << c.n_rand << " lanczos_steps=" << c.lz_steps << "  sector_time_s=" << t_sectors << "\n";
+  write_nmu(*out, logZ, Nelec, n_sites, c.beta, c.mu_min, c.mu_max, c.n_mu);
+  if (c.out_path) std::cout << "wrote " << c.out_path << "\n";
+  const double t_wall = std::chrono::duration<double>(clock_type::now() - t_wall0).count();
+  const double peak_b = peak_rss_bytes();
+  const double peak_mib = peak_b / (1024.0 * 1024.0);
+  if (!c.no_monitor) {
+    std::cout << "[monitor] wall_time_s=" << t_wall << "  wall_sector_logZ_s=" << t_sectors
+              << "  peak_rss_mib=" << peak_mib << "  peak_rss_bytes=" << peak_b << "\n";
+    if (c.out_path && file.is_open()) {
+      *out << "# wall_time_s=" << t_wall << "  peak_rss_bytes=" << peak_b << "  peak_rss_mib=" << peak_mib << "\n";
+      file.flush();
+    }
+  }
+  std::cout << "METRIC kind=" << metric_kind << " Lx=" << c.Lx << " Ly=" << c.Ly << " beta=" << c.beta
+            << " wall_time_s=" << t_wall << " wall_sector_logZ_s=" << t_sectors << " peak_rss_bytes=" << peak_b
+            << " peak_rss_mib=" << peak_mib << " n_mu=" << c.n_mu;
+  if (c.no_monitor) std::cout << " no_monitor=1";
+  std::cout << "\n";
+  return 0;
+}
+
+#if FTLM_NMU_K
+// Momentum-block solvers (bench_ftlm_nmu_4x4_k.cpp:546-1262): k-orbit and ed.
+int run_k(int argc, char** argv, bool ed_only) {
+  const int Lx = arg_int(argc, argv, "Lx", 4), Ly = arg_int(argc, argv, "Ly", 3);
+  if (Lx <= 0 || Ly <= 0 || Lx * Ly > 16) {
+    std::cerr << "ftlm_nmu_rect_k: requires 1 <= Lx*Ly <= 16\n";
+    return 2;
+  }
+  const int n_sites = Lx * Ly;
+  // momentum-Lanczos depth default: max(20, 2 * 72 / N)
+  const Common c = parse_common(argc, argv, 3, 8, std::max(20, (2 * 72) / std::max(1, n_sites)));
+  const bool log_k_dims = arg_int(argc, argv, "no-log-k-dims", 0) == 0;
+  int ed_cutoff = arg_int(argc, argv, "ed-cutoff", 64);
+  if (ed_only && !has_arg(argc, argv, "ed-cutoff")) ed_cutoff = std::numeric_limits<int>::max() / 4;
+  const auto t_wall0 = clock_type::now();
+  const Model m = make_model(argc, argv, Lx, Ly);
+  const int nsec = (n_sites + 1) * (n_sites + 1);
+  std::vector<double> logZ(static_cast<size_t>(nsec), -std::numeric_limits<double>::infinity());
+  std::vector<int> Nelec(static_cast<size_t>(nsec), 0);
+  std::cout << "=== FTLM grand-canonical n(mu) " << Lx << "x" << Ly << " with momentum blocks ===\n";
+  std::cout << "[solver] " << (ed_only ? "ed" : "k-orbit");
+  if (ed_only) std::cout << "  ed_cutoff=" << ed_cutoff;
+  std::cout << "\n";
+  std::cout << "beta=" << c.beta << "  t=" << m.p.t << " tp=" << m.p.tp << " U=" << m.p.U << " V=" << m.p.V << "\n";
+  std::cout << "ftlm_random=" << c.n_rand << " lanczos_steps=" << c.lz_steps << "  sectors=" << nsec << "\n";
+  if (!c.no_monitor) std::cout << "[monitor] wall time + peak RSS reported at exit (disable with --no-monitor=1)\n";
+
+  const auto t0 = clock_type::now();
+  int64_t blocks_ed = 0, blocks_ftlm = 0;
+  for (int nu = 0; nu <= n_sites; ++nu)
+    for (int nd = 0; nd <= n_sites; ++nd) {
+      const int idx = sector_index(nu, nd, n_sites);
+      Nelec[static_cast<size_t>(idx)] = nu + nd;
+      double logZ_sector = -std::numeric_limits<double>::infinity();
+      int64_t dim_k_total = 0;
+      int n_ed = 0, n_ftlm = 0;
+      ftlm_op_t* first = nullptr;  // the sector's first block owns the orbit graph the others share
+      std::vector<ftlm_op_t*> blocks;
+      for (int ky = 0; ky < Ly; ++ky)
+        for (int kx = 0; kx < Lx; ++kx) {
+          ftlm_op_t* op = nullptr;
+          ok(ftlm_hip_kblock_create(Lx, Ly, nu, nd, kx, ky, 1e-12, m.hops.data(), static_cast<int>(m.hops.size()),
+                                    m.pairs.data(), static_cast<int>(m.pairs.size()), m.p.U, m.p.V, first, &op),
+             "kblock_create");
+          if (!first) first = op;
+          blocks.push_back(op);
+          int64_t dk = 0;
+          ok(ftlm_hip_op_dim(op, &dk, nullptr, nullptr), "op_dim");
+          dim_k_total += dk;
+          if (log_k_dims) std::cerr << "  k=(" << kx << "," << ky << ") dim=" << dk << " [orbit-direct]\n";
+          if (dk <= 0) continue;
+          double lz_k = -std::numeric_limits<double>::infinity();
+          if (dk <= ed_cutoff) {
+            ++n_ed;
+            ok(ftlm_hip_exact_log_partition(op, &c.beta, 1, &lz_k, nullptr), "exact_log_partition");
+          } else {
+            ++n_ftlm;
+            const unsigned seed_k = c.seed + static_cast<unsigned>(idx * 257 + ky * 17 + kx);
+            const ftlm_ftlm_opts_t o = make_opts(argc, argv, c.n_rand, c.lz_steps, seed_k, 0);
+            ok(ftlm_hip_ftlm_log_partition(op, &c.beta, 1, &o, &lz_k), "ftlm_log_partition");
+          }
+          logZ_sector = logsumexp2(logZ_sector, lz_k);
+        }
+      for (auto it = blocks.rbegin(); it != blocks.rend(); ++it) ftlm_hip_op_destroy(*it);
+      blocks_ed += n_ed;
+      blocks_ftlm += n_ftlm;
+      logZ[static_cast<size_t>(idx)] = logZ_sector;
+      std::cerr << "sector (" << nu << "," << nd << ") dim_k_total=" << dim_k_total << " ed_blocks=" << n_ed
+                << " ftlm_blocks=" << n_ftlm << " logZ=" << logZ_sector << "\n";
+    }
+  const double t_sectors = std::chrono::duration<double>(clock_type::now() - t0).count();
+  std::cout << "[timing] all sectors logZ: " << t_sectors << " s\n";
+
+  std::ostream* out = &std::cout;
+  std::ofstream file;
+  if (c.out_path) {
+    file.open(c.out_path);
+    if (!file) {
+      std::cerr << "ftlm_nmu_rect_k: cannot open " << c.out_path << "\n";
+      return 2;
+    }
+    out = &file;
+  }
+  *out << "# FTLM grand canonical (rect k-block) Lx=" << Lx << " Ly=" << Ly << " beta=" << c.beta << " t=" << m.p.t
+       << " tp=" << m.p.tp << " U=" << m.p.U << " V=" << m.p.V << "\n";
+  *out << "# ftlm_random=" << c.n_rand << " lanczos_steps=" << c.lz_steps << " sector_time_s=" << t_sectors << "\n";
+  write_nmu(*out, logZ, Nelec, n_sites, c.beta, c.mu_min, c.mu_max, c.n_mu);
+  if (c.out_path) std::cout << "wrote " << c.out_path << "\n";
+  const double t_wall = std::chrono::duration<double>(clock_type::now() - t_wall0).count();
+  const double peak_b = peak_rss_bytes();
+  const double peak_mib = peak_b / (1024.0 * 1024.0);
+  if (!c.no_monitor) {
+    std::cout << "[monitor] wall_time_s=" << t_wall << "  wall_sector_logZ_s=" << t_sectors
+              << "  peak_rss_mib=" << peak_mib << "  peak_rss_bytes=" << peak_b << "\n";
+    if (c.out_path && file.is_open()) {
+      *out << "# wall_time_s=" << t_wall << " peak_rss_bytes=" << peak_b << " peak_rss_mib=" << peak_mib << "\n";
+      file.flush();
+    }
+  }
+  std::cout << "METRIC kind=FTLM_nmu_rect_k solver=" << (ed_only ? "ed" : "k-orbit") << " wall_time_s=" << t_wall
+            << " wall_sector_logZ_s=" << t_sectors << " peak_rss_bytes=" << peak_b << " peak_rss_mib=" << peak_mib
+            << " n_mu=" << c.n_mu << " ed_blocks=" << blocks_ed << " ftlm_blocks=" << blocks_ftlm << "\n";
+  return 0;
+}
+#endif
+
+}  // namespace
+
+int main(int argc, char** argv) {
+#if FTLM_NMU_K
+  const char* solver = find_arg(argc, argv, "solver");
+  if (solver && std::strcmp(solver, "nonmom") == 0) return run_nonmom(argc, argv, "FTLM_nmu_rect solver=nonmom");
+  if (solver && std::strcmp(solver, "ed") == 0) return run_k(argc, argv, true);
+  if (solver && std::strcmp(solver, "k-gram") == 0) {
+    std::cerr << "ftlm_nmu_rect_k: --solver=k-gram (Gram blocks, LAPACK zheev) is not built; use k-orbit\n";
+    return 2;
+  }
+  if (solver && std::strcmp(solver, "k-orbit") != 0) {
+    std::cerr << "ftlm_nmu_rect_k: unknown --solver=" << solver << " (supported: k-orbit, nonmom, ed)\n";
+    return 2;
+  }
+  return run_k(argc, argv, false);
+#else
+  return run_nonmom(argc, argv, "FTLM_nmu_rect");
+#endif
+}
