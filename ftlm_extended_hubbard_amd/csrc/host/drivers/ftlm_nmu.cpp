@@ -305,16 +305,24 @@ int run_k(int argc, char** argv, bool ed_only) {
       double logZ_sector = -std::numeric_limits<double>::infinity();
       int64_t dim_k_total = 0;
       int n_ed = 0, n_ftlm = 0;
-      ftlm_op_t* first = nullptr;  // the sector's first block owns the orbit graph the others share
-      std::vector<ftlm_op_t*> blocks;
+      // the sector's first block is kept until the sector is done: later blocks share its orbit
+      // graph; every other block is released as soon as its logZ is known
+      ftlm_op_t* first = nullptr;
       for (int ky = 0; ky < Ly; ++ky)
         for (int kx = 0; kx < Lx; ++kx) {
           ftlm_op_t* op = nullptr;
           ok(ftlm_hip_kblock_create(Lx, Ly, nu, nd, kx, ky, 1e-12, m.hops.data(), static_cast<int>(m.hops.size()),
                                     m.pairs.data(), static_cast<int>(m.pairs.size()), m.p.U, m.p.V, first, &op),
              "kblock_create");
-          if (!first) first = op;
-          blocks.push_back(op);
+          const bool keep = first == nullptr;
+          if (keep) first = op;
+          struct Release {
+            ftlm_op_t* op;
+            bool keep;
+            ~Release() {
+              if (!keep) ftlm_hip_op_destroy(op);
+            }
+          } release{op, keep};
           int64_t dk = 0;
           ok(ftlm_hip_op_dim(op, &dk, nullptr, nullptr), "op_dim");
           dim_k_total += dk;
@@ -332,7 +340,7 @@ int run_k(int argc, char** argv, bool ed_only) {
           }
           logZ_sector = logsumexp2(logZ_sector, lz_k);
         }
-      for (auto it = blocks.rbegin(); it != blocks.rend(); ++it) ftlm_hip_op_destroy(*it);
+      if (first) ftlm_hip_op_destroy(first);
       blocks_ed += n_ed;
       blocks_ftlm += n_ftlm;
       logZ[static_cast<size_t>(idx)] = logZ_sector;
