@@ -293,6 +293,18 @@ def grad_workload(F, L, torch, validate=True):
            "observables": {  # -(1/beta) d lnZ / d theta
                "double_occupancy": (-g["U"] / betas).tolist(), "nn_density": (-g["V"] / betas).tolist(),
                "kinetic_t": (-g["t"] / betas * base["t"]).tolist(), "N": (g["mu"] / betas).tolist()}}
+    # FTLM estimates of <dH/dtheta> (the standard observable estimator over the same samples) next
+    # to -(1/beta) dlnZ/dtheta: equal for exact thermodynamics, apart here by the FTLM sampling error
+    t0 = time.perf_counter()
+    _, ob = F.ftlm_log_partition_observables(p, 6, 6, betas, par, params=("t", "tp", "U", "V"))
+    t_obs = time.perf_counter() - t0
+    out["ftlm_observables"] = {
+        "seconds": round(t_obs, 3),
+        "estimator": {k: v.tolist() for k, v in ob.items()},
+        "from_gradient": {k: (-g[k] / betas).tolist() for k in ob},
+        "max_rel_diff": {k: float(np.max(np.abs(ob[k] + g[k] / betas) / np.maximum(np.abs(ob[k]), 1e-12)))
+                         for k in ob}}
+    log(f"FTLM <dH/dtheta> vs -(1/beta) dlnZ/dtheta, max rel. diff.: {out['ftlm_observables']['max_rel_diff']}")
     if validate:
         h, fd_err = 1e-5, {}
         for name in names:

@@ -36,6 +36,8 @@ from .api import (
     ftlm_log_partition_vjp,
     ftlm_vjp,
     FtlmTape,
+    ftlm_observables,
+    ftlm_log_partition_observables,
     geometry_derivative,
     ftlm_log_partition_real,
     ftlm_ritz,
@@ -57,5 +59,5 @@ __all__ = [
     "tridiagonal_extrema", "FtlmParams", "ftlm_log_partition_complex", "ftlm_log_partition_real", "RitzData",
     "ftlm_ritz", "ftlm_thermo", "quadrature_ritz", "device_count", "KrylovSession", "MomentumSector", "KBlockOperator", "OrbitKRepEntries", "pack_raw_state",
     "translate_states", "canonical_states", "translation_bloch_phase", "exact_log_partition", "geometry_derivative", "derivative_operator",
-    "ftlm_gradient", "ftlm_log_partition_gradient", "ftlm_vjp", "ftlm_log_partition_vjp", "FtlmTape", "_lib",
+    "ftlm_gradient", "ftlm_log_partition_gradient", "ftlm_vjp", "ftlm_log_partition_vjp", "FtlmTape", "ftlm_observables", "ftlm_log_partition_observables", "_lib",
 ]

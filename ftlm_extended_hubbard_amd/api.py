@@ -489,6 +489,29 @@ def ftlm_vjp(op: _Op, dops, betas, weights, par: FtlmParams, checkpoint: int = 0
     return lnz, g[:npar]
 
 
+def ftlm_observables(op: _Op, aops, betas, par: FtlmParams):
+    """FTLM estimates <A_p>(beta) over the samples of ftlm_gradient (same seeds and steps, converged
+    Gauss rule): (lnZ [nb], obs [len(aops)][nb])."""
+    b = np.ascontiguousarray(np.atleast_1d(betas), np.float64)
+    nb, n = len(b), len(aops)
+    lnz, obs = np.zeros(nb), np.zeros((n, nb))
+    arr = (C.c_void_p * n)(*[a.handle.value for a in aops])
+    L.check(L.lib().ftlm_hip_ftlm_observables(op.handle, arr, n, C.byref(par._c()), b.ctypes.data, nb,
+                                              lnz.ctypes.data, obs.ctypes.data))
+    return lnz, obs
+
+
+def ftlm_log_partition_observables(p: HubbardParams, n_up: int, n_dn: int, betas, par: FtlmParams,
+                                   params=("t", "tp", "U", "V"), K: "MomentumSector | None" = None):
+    """lnZ(beta) and the FTLM estimates of <dH/dtheta> for theta in params (t, tp, U, V, phi_x,
+    phi_y): <D> for U, the NN pair count for V, the hop parts for t and t'.  At exact
+    thermodynamics these are -(1/beta) dlnZ/dtheta."""
+    op = HubbardOperator.from_params(p, n_up, n_dn) if K is None else KBlockOperator.from_params(p, n_up, n_dn, K)
+    aops = [derivative_operator(p, n_up, n_dn, q, K) for q in params]
+    lnz, obs = ftlm_observables(op, aops, betas, par)
+    return lnz, {q: obs[i] for i, q in enumerate(params)}
+
+
 class FtlmTape:
     """The forward pass of the adjoint method (ftlm_hip_ftlm_tape_forward): lnZ(betas) now, then
     backward(weights) -> g_p = sum_b weights[b] dlnZ(beta_b)/dtheta_p as often as needed.  Holds

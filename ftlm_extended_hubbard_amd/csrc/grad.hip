@@ -211,30 +211,14 @@ __global__ __launch_bounds__(kGBlock) void k_gscale(int64_t n, int R, double2* _
   }
 }
 
-// One workgroup per sample s (gout, when given, receives dL_s/dalpha_k and dL_s/dbeta_k for
-// every beta): converged eigen-decomposition of T_s by implicit QL, then for every
-// beta L_s(beta) and dL_s/dtheta_p.  Thread 0 runs the scalar QL recurrence in LDS and records each
-// sweep's plane rotations; all threads then apply the sweep to their rows of the eigenvector matrix
-// (U[i * n + a]: component a of eigenvector i, so a sweep's loads are coalesced over a).
-// alpha/beta: [s][Mmax]; dalpha/dbeta: [p][s][Mmax].  Outputs L [s][nb], dL [p][s][nb].
-__global__ __launch_bounds__(kGBlock) void k_grad_tridiag(
-    const double* __restrict__ alpha, const double* __restrict__ beta, const double* __restrict__ dalpha,
-    const double* __restrict__ dbeta, const int* __restrict__ nvec, int nsamp, int Mmax, int npar,
-    const double* __restrict__ betas, int nb, double* __restrict__ Uw, double* __restrict__ W,
-    double* __restrict__ L, double* __restrict__ dL, double* __restrict__ gout, int* __restrict__ status) {
-  const int s = blockIdx.x;
-  const int n = nvec[s];
-  if (n <= 0) return;  // uniform over the workgroup
+// Converged implicit-QL eigen-decomposition of the tridiagonal (d, e) of length n by one
+// workgroup: thread 0 runs the scalar recurrence in LDS and records each sweep's plane rotations in
+// (rc, rs); every thread then applies the sweep to its rows of U (U[i * n + a]: component a of
+// eigenvector i, so a sweep's loads are coalesced over a).  d, e, rc, rs, ctl live in LDS; U
+// (n * n) in global memory.  Eigenvalues are left in d.  Returns nonzero (on every thread) when an
+// eigenvalue needs more than 200 sweeps.
+__device__ int ql_vectors(double* d, double* e, double* rc, double* rs, int* ctl, double* U, int n) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  double* U = Uw + static_cast<int64_t>(s) * Mmax * Mmax;
-  double* Wm = W + static_cast<int64_t>(s) * Mmax * Mmax;
-  __shared__ double d[kGMaxM], e[kGMaxM], rc[kGMaxM], rs[kGMaxM], ex[kGMaxM], g0[kGMaxM], g1[kGMaxM];
-  __shared__ int ctl[3];
-  __shared__ double lz_sh;
-  for (int i = tid; i < n; i += nt) {
-    d[i] = alpha[static_cast<int64_t>(s) * Mmax + i];
-    e[i] = i < n - 1 ? beta[static_cast<int64_t>(s) * Mmax + i] : 0.0;
-  }
   for (int64_t f = tid; f < static_cast<int64_t>(n) * n; f += nt) U[f] = (f / n == f % n) ? 1.0 : 0.0;
   __syncthreads();
   int bad = 0, iter = 0;
@@ -309,6 +293,33 @@ __global__ __launch_bounds__(kGBlock) void k_grad_tridiag(
       if (flag != 0) break;
     }
   }
+  return bad;
+}
+
+// One workgroup per sample s (gout, when given, receives dL_s/dalpha_k and dL_s/dbeta_k for
+// every beta): eigen-decomposition of T_s (ql_vectors), then for every beta L_s(beta) and
+// dL_s/dtheta_p.  alpha/beta: [s][Mmax]; dalpha/dbeta: [p][s][Mmax].  Outputs L [s][nb],
+// dL [p][s][nb].
+__global__ __launch_bounds__(kGBlock) void k_grad_tridiag(
+    const double* __restrict__ alpha, const double* __restrict__ beta, const double* __restrict__ dalpha,
+    const double* __restrict__ dbeta, const int* __restrict__ nvec, int nsamp, int Mmax, int npar,
+    const double* __restrict__ betas, int nb, double* __restrict__ Uw, double* __restrict__ W,
+    double* __restrict__ L, double* __restrict__ dL, double* __restrict__ gout, int* __restrict__ status) {
+  const int s = blockIdx.x;
+  const int n = nvec[s];
+  if (n <= 0) return;  // uniform over the workgroup
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double* U = Uw + static_cast<int64_t>(s) * Mmax * Mmax;
+  double* Wm = W + static_cast<int64_t>(s) * Mmax * Mmax;
+  __shared__ double d[kGMaxM], e[kGMaxM], rc[kGMaxM], rs[kGMaxM], ex[kGMaxM], g0[kGMaxM], g1[kGMaxM];
+  __shared__ int ctl[3];
+  __shared__ double lz_sh;
+  for (int i = tid; i < n; i += nt) {
+    d[i] = alpha[static_cast<int64_t>(s) * Mmax + i];
+    e[i] = i < n - 1 ? beta[static_cast<int64_t>(s) * Mmax + i] : 0.0;
+  }
+  __syncthreads();
+  const int bad = ql_vectors(d, e, rc, rs, ctl, U, n);
   if (bad) {
     if (tid == 0) atomicAdd(status, 1);
     return;  // uniform: every thread saw flag 2
@@ -398,6 +409,103 @@ __global__ void k_gphase(int64_t dim, int R, const uint32_t* ids, double2* q) {
       sincos(th, &s, &c);
       q[e * R + r] = make_double2(c, s);
     }
+}
+
+// FTLM estimator of observables, per sample s and beta: with T_s = U diag(lambda) U^T, u_j = U[j][0]
+// (= <r|psi_j>) and a_k = <q_k|A r> (so <psi_j|A|r> = sum_k U[j][k] a_k),
+//     num_s = Re sum_j e^{-beta lambda_j} u_j <psi_j|A|r> / sum_j e^{-beta lambda_j} u_j^2,
+// and L_s as in k_grad_tridiag; <A> = sum_s softmax(L)_s num_s.  One workgroup per sample.
+// a: [p][s][Mmax] complex.  Outputs L [s][nb], num [p][s][nb].
+__global__ __launch_bounds__(kGBlock) void k_obs_tridiag(const double* __restrict__ alpha,
+                                                         const double* __restrict__ beta,
+                                                         const int* __restrict__ nvec, int nsamp, int Mmax,
+                                                         int nobs, const double2* __restrict__ a,
+                                                         const double* __restrict__ betas, int nb,
+                                                         double* __restrict__ Uw, double* __restrict__ L,
+                                                         double* __restrict__ num, int* __restrict__ status) {
+  const int s = blockIdx.x;
+  const int n = nvec[s];
+  if (n <= 0) return;  // uniform over the workgroup
+  const int tid = threadIdx.x, nt = blockDim.x;
+  double* U = Uw + static_cast<int64_t>(s) * Mmax * Mmax;
+  __shared__ double d[kGMaxM], e[kGMaxM], rc[kGMaxM], rs[kGMaxM], pa[kGMaxM];
+  __shared__ int ctl[3];
+  __shared__ double red[kGBlock];
+  __shared__ double lz_sh;
+  for (int i = tid; i < n; i += nt) {
+    d[i] = alpha[static_cast<int64_t>(s) * Mmax + i];
+    e[i] = i < n - 1 ? beta[static_cast<int64_t>(s) * Mmax + i] : 0.0;
+  }
+  __syncthreads();
+  if (ql_vectors(d, e, rc, rs, ctl, U, n)) {
+    if (tid == 0) atomicAdd(status, 1);
+    return;  // uniform
+  }
+  for (int p = 0; p < nobs; ++p) {
+    // pa[j] = u_j Re <psi_j|A|r>
+    const double2* ap = a + (static_cast<int64_t>(p) * nsamp + s) * Mmax;
+    for (int j = tid; j < n; j += nt) {
+      double v = 0.0;
+      for (int k = 0; k < n; ++k) v += U[static_cast<int64_t>(j) * n + k] * ap[k].x;
+      pa[j] = U[static_cast<int64_t>(j) * n] * v;
+    }
+    __syncthreads();
+    for (int ib = 0; ib < nb; ++ib) {
+      const double bt = betas[ib];
+      if (tid == 0) {  // L = ln sum_j u_j^2 e^{-bt lambda_j}, shifted by the largest exponent
+        double mx = -INFINITY;
+        for (int i = 0; i < n; ++i) {
+          const double u = U[static_cast<int64_t>(i) * n];
+          if (u != 0.0) mx = fmax(mx, -bt * d[i] + log(u * u));
+        }
+        double z = 0.0;
+        for (int i = 0; i < n; ++i) {
+          const double u = U[static_cast<int64_t>(i) * n];
+          if (u != 0.0) z += exp(-bt * d[i] + log(u * u) - mx);
+        }
+        lz_sh = mx + log(z);
+        if (p == 0) L[static_cast<int64_t>(s) * nb + ib] = lz_sh;
+      }
+      __syncthreads();
+      const double lz = lz_sh;
+      double acc = 0.0;
+      for (int j = tid; j < n; j += nt) acc += exp(-bt * d[j] - lz) * pa[j];
+      red[tid] = acc;
+      __syncthreads();
+      for (int h = nt / 2; h > 0; h >>= 1) {
+        if (tid < h) red[tid] += red[tid + h];
+        __syncthreads();
+      }
+      if (tid == 0) num[(static_cast<int64_t>(p) * nsamp + s) * nb + ib] = red[0];
+      __syncthreads();
+    }
+  }
+}
+
+// Block partials of <q | b_p> for p < np (b_p through GradVecs::tmp), [block][np][R].
+__global__ __launch_bounds__(kGBlock) void k_mdot(int64_t n, int R, int np, const double2* __restrict__ q, GradVecs V,
+                                                  double2* __restrict__ part) {
+  const int bsz = (kGBlock / R) * R;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * bsz;
+  const int64_t f0 = static_cast<int>(threadIdx.x) < bsz ? static_cast<int64_t>(blockIdx.x) * bsz + threadIdx.x : n;
+  double xr[kGMaxPar], xi[kGMaxPar];
+#pragma unroll
+  for (int p = 0; p < kGMaxPar; ++p) xr[p] = xi[p] = 0.0;
+  for (int64_t f = f0; f < n; f += stride) {
+    const double2 a = q[f];
+#pragma unroll
+    for (int p = 0; p < kGMaxPar; ++p)
+      if (p < np) {
+        const double2 b = V.tmp[p][f];
+        xr[p] += a.x * b.x + a.y * b.y;
+        xi[p] += a.x * b.y - a.y * b.x;
+      }
+  }
+  __shared__ double rr[kGBlock], ri[kGBlock];
+  double2* o = part + static_cast<int64_t>(blockIdx.x) * np * R;
+#pragma unroll
+  for (int p = 0; p < kGMaxPar; ++p)
+    if (p < np) reduce_store(xr[p], xi[p], R, bsz, rr, ri, o + static_cast<int64_t>(p) * R);
 }
 
 // lnZ(beta) = ln dim + log-mean-exp_s L_s; dlnZ_p = sum_s softmax(L)_s dL_s,p.
@@ -693,8 +801,14 @@ namespace {
 
 // Forward pass of the adjoint method: the recurrence of ftlm_hip_ftlm_gradient without tangents,
 // keeping (q_{k-1}, q_k) every s steps, then the quadrature and its derivatives per sample.
+// Optional observables of the forward pass: a[(p * M + k) * R + r] = <q_k|A_p q_0> per step.
+struct ObsAcc {
+  std::vector<const ftlm_op*> aops;
+  std::vector<double2> a;
+};
+
 void tape_forward(ftlm_tape* T, const ftlm_op* op, ftlm_op* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
-                  const double* betas, int nb, int ckpt, int mode, double* lnZ) {
+                  const double* betas, int nb, int ckpt, int mode, double* lnZ, ObsAcc* obs = nullptr) {
   FTLM_REQUIRE(op != nullptr && opts != nullptr && betas != nullptr && lnZ != nullptr && nb >= 1, FTLM_EINVAL,
                "null argument");
   FTLM_REQUIRE(npar >= 0 && npar <= kGMaxPar && (npar == 0 || dops != nullptr), FTLM_EINVAL,
@@ -772,6 +886,21 @@ void tape_forward(ftlm_tape* T, const ftlm_op* op, ftlm_op* const* dops, int npa
   for (int r = 0; r < R; ++r) FTLM_REQUIRE(std::sqrt(hres[r].x) >= 1e-18, FTLM_EUNSUPPORTED, "zero start vector");
   hipLaunchKernelGGL(k_gscale, dim3(gblk), dim3(kGBlock), 0, st, n, R, q, dres.p);
   check_launch("k_gscale");
+  // observables: b_p = A_p q_0, then <q_k|b_p> at every step
+  const int nobs = obs ? static_cast<int>(obs->aops.size()) : 0;
+  std::vector<DevBuf<double2>> bvec(static_cast<size_t>(nobs));
+  GradVecs BV{};
+  DevBuf<double2> opart(nobs ? static_cast<size_t>(gblk) * nobs * R : 0), ores(static_cast<size_t>(nobs) * R);
+  std::vector<double2> hobs(static_cast<size_t>(nobs) * R);
+  if (nobs) {
+    obs->a.assign(static_cast<size_t>(nobs) * M * R, make_double2(0.0, 0.0));
+    for (int p = 0; p < nobs; ++p) {
+      bvec[p].alloc(vb);
+      hv_apply(obs->aops[p], q, bvec[p].p, R, FTLM_C128, nullptr, nullptr, st);
+      BV.tmp[p] = bvec[p].p;
+    }
+    check_launch("observable start");
+  }
   T->zs.assign(static_cast<size_t>(M) * R, make_double2(0.0, 0.0));
   T->bs.assign(static_cast<size_t>(M) * R, 0.0);
   T->used.assign(static_cast<size_t>(R), 0);
@@ -786,6 +915,17 @@ void tape_forward(ftlm_tape* T, const ftlm_op* op, ftlm_op* const* dops, int npa
       const int c = k / s;
       FTLM_HIP_CHECK(hipMemcpyAsync(T->ck[2 * c].p, qp, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
       FTLM_HIP_CHECK(hipMemcpyAsync(T->ck[2 * c + 1].p, q, vb * sizeof(double2), hipMemcpyDeviceToDevice, st));
+    }
+    if (nobs) {
+      hipLaunchKernelGGL(k_mdot, dim3(gblk), dim3(kGBlock), 0, st, n, R, nobs, q, BV, opart.p);
+      hipLaunchKernelGGL(k_gsum, dim3(nobs * R), dim3(kGBlock), 0, st, opart.p, gblk, nobs * R, ores.p);
+      check_launch("k_mdot");
+      FTLM_HIP_CHECK(hipMemcpyAsync(hobs.data(), ores.p, sizeof(double2) * nobs * R, hipMemcpyDeviceToHost, st));
+      FTLM_HIP_CHECK(hipStreamSynchronize(st));
+      for (int p = 0; p < nobs; ++p)
+        for (int r = 0; r < R; ++r)
+          obs->a[(static_cast<size_t>(p) * M + k) * R + r] =
+              alive[r] ? hobs[static_cast<size_t>(p) * R + r] : make_double2(0.0, 0.0);
     }
     hv_apply(op, q, w, R, FTLM_C128, nullptr, nullptr, st);
     hipLaunchKernelGGL(k_gstep_dots, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, q, w, GradVecs{}, part.p);
@@ -1249,6 +1389,64 @@ int ftlm_hip_ftlm_tape_backward(const ftlm_tape_t* tape, const double* gw, doubl
 
 int ftlm_hip_ftlm_tape_destroy(ftlm_tape_t* tape) {
   return guarded([&] { delete tape; });
+}
+
+int ftlm_hip_ftlm_observables(const ftlm_op_t* op, ftlm_op_t* const* aops, int nobs, const ftlm_ftlm_opts_t* opts,
+                              const double* betas, int nb, double* lnZ, double* obs) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && aops != nullptr && obs != nullptr && lnZ != nullptr && betas != nullptr && nb >= 1,
+                 FTLM_EINVAL, "null argument");
+    FTLM_REQUIRE(nobs >= 1 && nobs <= kGMaxPar, FTLM_EINVAL, "between 1 and 16 observables");
+    for (int p = 0; p < nobs; ++p)
+      FTLM_REQUIRE(aops[p] != nullptr && aops[p]->dim == op->dim, FTLM_EINVAL, "observable of another size");
+    ftlm_tape T;
+    ObsAcc acc;
+    acc.aops.assign(aops, aops + nobs);
+    tape_forward(&T, op, nullptr, 0, opts, betas, nb, std::max(1, opts->M), FTLM_TS_GENERIC, lnZ, &acc);
+    const int R = T.R, M = T.M;
+    const hipStream_t st = nullptr;
+    std::vector<double> ha(static_cast<size_t>(R) * M, 0.0), hb(static_cast<size_t>(R) * M, 0.0);
+    std::vector<double2> hA(static_cast<size_t>(nobs) * R * M, make_double2(0.0, 0.0));
+    for (int r = 0; r < R; ++r)
+      for (int k = 0; k < T.used[r]; ++k) {
+        ha[static_cast<size_t>(r) * M + k] = T.zs[static_cast<size_t>(k) * R + r].x;
+        hb[static_cast<size_t>(r) * M + k] = T.bs[static_cast<size_t>(k) * R + r];
+        for (int p = 0; p < nobs; ++p)
+          hA[(static_cast<size_t>(p) * R + r) * M + k] = acc.a[(static_cast<size_t>(p) * M + k) * R + r];
+      }
+    DevBuf<double> da(ha.size()), db(hb.size()), dbeta(static_cast<size_t>(nb)), Uw(static_cast<size_t>(R) * M * M);
+    DevBuf<double> Ls(static_cast<size_t>(R) * nb), num(static_cast<size_t>(nobs) * R * nb);
+    DevBuf<double2> dA(hA.size());
+    DevBuf<int> dn(static_cast<size_t>(R)), status(1);
+    da.upload(ha, st);
+    db.upload(hb, st);
+    dA.upload(hA, st);
+    dbeta.upload(betas, static_cast<size_t>(nb), st);
+    dn.upload(T.used, st);
+    FTLM_HIP_CHECK(hipMemsetAsync(status.p, 0, sizeof(int), st));
+    hipLaunchKernelGGL(k_obs_tridiag, dim3(R), dim3(kGBlock), 0, st, da.p, db.p, dn.p, R, M, nobs, dA.p, dbeta.p, nb,
+                       Uw.p, Ls.p, num.p, status.p);
+    check_launch("k_obs_tridiag");
+    int bad = 0;
+    FTLM_HIP_CHECK(hipMemcpy(&bad, status.p, sizeof(int), hipMemcpyDeviceToHost));
+    FTLM_REQUIRE(bad == 0, FTLM_EHIP, "tridiagonal QL did not converge");
+    std::vector<double> hL(Ls.n), hN(num.n);
+    FTLM_HIP_CHECK(hipMemcpy(hL.data(), Ls.p, hL.size() * sizeof(double), hipMemcpyDeviceToHost));
+    FTLM_HIP_CHECK(hipMemcpy(hN.data(), num.p, hN.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int b = 0; b < nb; ++b) {  // <A>(beta) = sum_r softmax_r(L_r(beta)) num_r
+      double mx = -INFINITY;
+      for (int r = 0; r < R; ++r) mx = std::max(mx, hL[static_cast<size_t>(r) * nb + b]);
+      double zsum = 0.0;
+      for (int r = 0; r < R; ++r) zsum += std::exp(hL[static_cast<size_t>(r) * nb + b] - mx);
+      for (int p = 0; p < nobs; ++p) {
+        double v = 0.0;
+        for (int r = 0; r < R; ++r)
+          v += std::exp(hL[static_cast<size_t>(r) * nb + b] - mx) / zsum *
+               hN[(static_cast<size_t>(p) * R + r) * nb + b];
+        obs[static_cast<size_t>(p) * nb + b] = v;
+      }
+    }
+  });
 }
 
 int ftlm_hip_ftlm_vjp(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,

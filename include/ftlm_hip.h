@@ -245,6 +245,16 @@ int ftlm_hip_ftlm_gradient(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar
 int ftlm_hip_ftlm_vjp(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar, const ftlm_ftlm_opts_t* opts,
                       const double* betas, int nb, const double* gw, int ckpt, int mode, double* lnZ, double* grad);
 
+/* FTLM estimator of observables A_p (Hermitian operators of the same size, e.g. the derivative
+ * operators: double occupancy, NN pair count, the hop part):
+ *   <A>(beta) = sum_r sum_j e^{-beta lambda_j} <r|psi_j><psi_j|A|r> / sum_r sum_j e^{-beta lambda_j} |<r|psi_j>|^2
+ * over the same R samples, M steps and converged Gauss rule as ftlm_hip_ftlm_gradient, with
+ * <psi_j|A|r> = sum_k U_kj <q_k|A r> (one apply of A_p to the start block and one dot per step).
+ * Also returns lnZ.  dim > 1, 1 <= nobs <= 16; obs is [nobs][nb].  No counterpart in the
+ * reference, which estimates only lnZ (for kinetic energy, double occupancy, NN correlations, N). */
+int ftlm_hip_ftlm_observables(const ftlm_op_t* op, ftlm_op_t* const* aops, int nobs, const ftlm_ftlm_opts_t* opts,
+                              const double* betas, int nb, double* lnZ, double* obs);
+
 /* The two halves of ftlm_hip_ftlm_vjp, for callers that choose the weights after the forward pass
  * (reverse-mode autodiff) or need several weighted rows (a Jacobian).  tape_forward runs the
  * forward pass, returns lnZ and keeps the checkpoints, scalars and quadrature derivatives on the

@@ -436,3 +436,40 @@ def test_autograd_function_matches_forward_mode():
                                              theta.detach().clone())
     for i, k in enumerate(names + ("mu",)):
         assert np.allclose(jac[:, i].numpy(), g[k], rtol=1e-9, atol=1e-12), (k, jac[:, i], g[k])
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd,phi", [(2, 2, 2, 1, 0.3), (3, 2, 1, 1, 0.0)])
+def test_observables_full_krylov_vs_dense(oracle, lx, ly, nu, nd, phi):
+    """With a complete Krylov space (M = dim) the FTLM observable estimator is
+    sum_r <r|e^{-beta H} A|r> / sum_r <r|e^{-beta H}|r> over the reference's start vectors
+    (mt19937 Gaussians, seed + r*100003): against dense numpy, for A = dH/dtheta."""
+    say("observables full Krylov", lx, ly, nu, nd, phi)
+    p = api.HubbardParams(lx, ly, 1.0, -0.25, 8.0, 1.0, phi, 0.0)
+    names = ("t", "tp", "U", "V", "phi_x")
+    op = api.HubbardOperator.from_params(p, nu, nd)
+    dim = op.dim()
+    betas = np.array([0.3, 1.0, 4.0])
+    par = api.FtlmParams(n_random=3, lanczos_steps=dim, seed=5, quad=L.FTLM_QUAD_CONVERGED)
+    lnz, obs = api.ftlm_log_partition_observables(p, nu, nd, betas, par, params=names)
+    H = np.zeros((dim, dim), np.complex128)
+    E = np.eye(dim, dtype=np.complex128)
+    for j in range(dim):
+        H[:, j] = op.apply(E[:, j])
+    w, V = np.linalg.eigh(H)
+    starts = []
+    for r in range(3):
+        q = oracle.gauss_start(dim, 5 + r * 100003)
+        starts.append(q / np.linalg.norm(q))
+    for i, name in enumerate(names):
+        dop = api.derivative_operator(p, nu, nd, name)
+        A = np.zeros((dim, dim), np.complex128)
+        for j in range(dim):
+            A[:, j] = dop.apply(E[:, j])
+        for ib, b in enumerate(betas):
+            eb = (V * np.exp(-b * (w - w[0]))) @ V.conj().T
+            Z = sum(np.vdot(q, eb @ q).real for q in starts)
+            NA = sum(np.vdot(q, eb @ (A @ q)).real for q in starts)
+            assert abs(obs[name][ib] - NA / Z) <= 1e-8 * max(1.0, abs(NA / Z)), (name, b, obs[name][ib], NA / Z)
+            if i == 0:
+                want = np.log(dim) + np.log(Z / 3) - b * w[0]
+                assert abs(lnz[ib] - want) <= 1e-9 * max(1.0, abs(want)), (b, lnz[ib], want)
