@@ -346,6 +346,38 @@ def grad_workload(F, L, torch, validate=True):
     return out
 
 
+def graph_workload(F, L, torch):
+    """Eager vs hipGraph-captured per-operator Lanczos (FTLM_PATH_PER_OP vs FTLM_PATH_GRAPH):
+    M=100 steps of R=16 complex64 random-phase vectors on sectors of growing size (t=1,
+    t'=-0.25, U=8, V=1, phi_x=0.05), converged Gauss rule; wall time of ftlm_ritz per operator
+    (median of 3)."""
+    import numpy as np
+
+    cases = [(4, 2, 2, 2), (4, 3, 2, 2), (4, 3, 3, 3), (4, 3, 4, 4), (4, 3, 6, 6)]
+    rows = []
+    for lx, ly, nu, nd in cases:
+        p = F.HubbardParams(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.05, 0.0)
+        op = F.HubbardOperator.from_params(p, nu, nd)
+        res = {}
+        for name, path in (("eager", L.FTLM_PATH_PER_OP), ("graph", L.FTLM_PATH_GRAPH)):
+            par = F.FtlmParams(n_random=16, lanczos_steps=100, seed=3, start=L.FTLM_START_PHASE, dtype=L.FTLM_C64,
+                               quad=L.FTLM_QUAD_CONVERGED, path=path)
+            F.ftlm_ritz([op], par)
+            ts = []
+            for _ in range(3):
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                F.ftlm_ritz([op], par)
+                ts.append(time.perf_counter() - t0)
+            res[name] = float(np.median(ts))
+        rows.append({"sector": f"{lx}x{ly} ({nu},{nd})", "dim": op.dim(), "eager_ms": round(1e3 * res["eager"], 2),
+                     "graph_ms": round(1e3 * res["graph"], 2), "speedup": round(res["eager"] / res["graph"], 3)})
+        log(f"{rows[-1]}")
+    return {"metric": "per-operator FTLM Lanczos, eager vs hipGraph-captured, R=16 c64, M=100", "unit": "ms",
+            "higher_is_better": False, "n_gpus": 1, "data": "synthetic", "dtype": "c64",
+            "config": {"workload": "sector ladder 4x2 (2,2) .. 4x3 (6,6)"}, "rows": rows}
+
+
 def traffic_from_profile():
     """HBM bytes per H*V (R=16 complex64) from the committed counter summary: the line
     'hbm_bytes_per_hv = <int>' (FETCH_SIZE + WRITE_SIZE of both passes, copy-calibrated)."""
@@ -435,7 +467,7 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["sector", "kblock", "gc4x2", "grad"], default="sector",
+    ap.add_argument("--workload", choices=["sector", "kblock", "gc4x2", "grad", "graph"], default="sector",
                     help="sector: the 4x4 (8,8) full-sector line (default); kblock: the 4x4 (8,8) momentum-block "
                          "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch); grad: "
                          "the differentiated FTLM line (4x3 (6,6) and the 4x4 (8,8) K=(0,0) block)")
@@ -457,6 +489,9 @@ def main():
     K, W, R = args.steps, args.warmup, args.R
     if args.profile_hv:
         profile_hv(F, L, torch, R)
+        return
+    if args.workload == "graph":
+        print(json.dumps(graph_workload(F, L, torch)), flush=True)
         return
     if args.workload == "grad":
         print(json.dumps(grad_workload(F, L, torch, validate=not args.no_validate)), flush=True)

@@ -17,7 +17,7 @@ FTLM_OK, FTLM_EINVAL, FTLM_ERANGE, FTLM_EHIP, FTLM_ENOMEM, FTLM_EUNSUPPORTED = r
 FTLM_C128, FTLM_C64 = 0, 1
 FTLM_START_GAUSS_MT, FTLM_START_PHASE = 0, 1
 FTLM_QUAD_REFERENCE, FTLM_QUAD_CONVERGED = 0, 1
-FTLM_PATH_AUTO, FTLM_PATH_PER_OP, FTLM_PATH_PERSISTENT = 0, 1, 2
+FTLM_PATH_AUTO, FTLM_PATH_PER_OP, FTLM_PATH_PERSISTENT, FTLM_PATH_GRAPH = 0, 1, 2, 3
 FTLM_TS_AUTO, FTLM_TS_GENERIC, FTLM_TS_FUSED = 0, 1, 2
 
 
@@ -97,6 +97,7 @@ def lib() -> C.CDLL:
     L.ftlm_hip_quadrature_ritz.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]
     L.ftlm_hip_krylov_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(vp)]
     L.ftlm_hip_krylov_run.argtypes = [vp, C.c_int, vp]
+    L.ftlm_hip_krylov_run_graph.argtypes = L.ftlm_hip_krylov_run.argtypes
     L.ftlm_hip_krylov_result.argtypes = [vp, vp, vp, vp, vp]
     L.ftlm_hip_krylov_bytes.argtypes = [vp, i64p]
     L.ftlm_hip_krylov_destroy.argtypes = [vp]

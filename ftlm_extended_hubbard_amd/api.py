@@ -642,7 +642,8 @@ class KrylovSession:
         self._h = C.c_void_p()
         L.check(L.lib().ftlm_hip_krylov_create(op.handle, R, max_steps, dtype, start, seed, C.byref(self._h)))
 
-    def run(self, n_steps: int, stream: int | None = None) -> None:
+    def run(self, n_steps: int, stream: int | None = None, graph: bool = False) -> None:
+        """Advance n_steps on `stream`; graph=True captures them as one hipGraph first."""
         if stream is None:
             try:
                 import torch
@@ -650,7 +651,8 @@ class KrylovSession:
                 stream = torch.cuda.current_stream().cuda_stream
             except Exception:  # noqa: BLE001
                 stream = 0
-        L.check(L.lib().ftlm_hip_krylov_run(self._h, n_steps, C.c_void_p(stream)))
+        fn = L.lib().ftlm_hip_krylov_run_graph if graph else L.lib().ftlm_hip_krylov_run
+        L.check(fn(self._h, n_steps, C.c_void_p(stream)))
 
     def result(self):
         a, b = np.zeros((self.M, self.R)), np.zeros((self.M, self.R))

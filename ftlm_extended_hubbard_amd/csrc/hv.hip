@@ -540,6 +540,13 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_flat(const SectorView S, const 
   }
 }
 
+// dst[0 .. n) = 0 (a kernel rather than a memset node, so that applies can be graph-captured)
+__global__ void k_zero_d2(double2* __restrict__ dst, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = make_double2(0.0, 0.0);
+}
+
 // ------------------------------------------------------------------ host dispatch
 namespace {
 
@@ -671,8 +678,11 @@ int hv_apply_sector(const Sector* S, const void* x, void* y, int R, int dtype, c
     chunks.emplace_back(r0, rb);
     r0 += rb;
   }
-  if (dotp && chunks.size() > 1)
-    FTLM_HIP_CHECK(hipMemsetAsync(dotp, 0, sizeof(double2) * static_cast<size_t>(kHvMaxGrid) * R, st));
+  if (dotp && chunks.size() > 1) {
+    const int64_t nz = static_cast<int64_t>(kHvMaxGrid) * R;
+    hipLaunchKernelGGL(k_zero_d2, dim3(static_cast<unsigned>(std::min<int64_t>(1024, (nz + 255) / 256))), dim3(256),
+                       0, st, dotp, nz);
+  }
   for (const auto& c : chunks) {
     int nb;
     if (dtype == FTLM_C64) {

@@ -348,6 +348,49 @@ struct Krylov {
       step(k, a, b, c, st);
     }
   }
+
+  // The same steps captured as one hipGraph on a private stream (ordered after `st` and before
+  // anything later on `st`) and launched once.  Every step is kernel launches only: the H*V
+  // tile counters reset themselves and partial sums are cleared by a kernel, never a memset node.
+  void run_graph(int k0, int k1, hipStream_t st) {
+    FTLM_REQUIRE(op->kind != kCallback, FTLM_EUNSUPPORTED, "host-callback operators cannot be graph-captured");
+    if (k1 <= k0) return;
+    hipStream_t cs = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ex = nullptr;
+    auto cleanup = [&] {
+      if (ex) (void)hipGraphExecDestroy(ex);
+      if (g) (void)hipGraphDestroy(g);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (cs) (void)hipStreamDestroy(cs);
+    };
+    try {
+      FTLM_HIP_CHECK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+      FTLM_HIP_CHECK(hipEventCreateWithFlags(&e0, hipEventDisableTiming));
+      FTLM_HIP_CHECK(hipEventCreateWithFlags(&e1, hipEventDisableTiming));
+      FTLM_HIP_CHECK(hipEventRecord(e0, st));
+      FTLM_HIP_CHECK(hipStreamWaitEvent(cs, e0, 0));
+      FTLM_HIP_CHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      try {
+        run(k0, k1, cs);
+      } catch (...) {
+        (void)hipStreamEndCapture(cs, &g);
+        throw;
+      }
+      FTLM_HIP_CHECK(hipStreamEndCapture(cs, &g));
+      FTLM_HIP_CHECK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+      FTLM_HIP_CHECK(hipGraphLaunch(ex, cs));
+      FTLM_HIP_CHECK(hipEventRecord(e1, cs));
+      FTLM_HIP_CHECK(hipStreamWaitEvent(st, e1, 0));
+      FTLM_HIP_CHECK(hipStreamSynchronize(cs));
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  }
 };
 
 // Reference start vector: mt19937(seed) + normal(0,1), (re, im) per index.
@@ -553,7 +596,7 @@ void ritz_persistent(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* o
 }
 
 bool persistent_eligible(ftlm_op_t* const* ops, int n_ops, int path) {
-  if (path == FTLM_PATH_PER_OP) return false;
+  if (path == FTLM_PATH_PER_OP || path == FTLM_PATH_GRAPH) return false;
   const int64_t lim = path == FTLM_PATH_PERSISTENT ? 262144 : 16384;
   for (int i = 0; i < n_ops; ++i) {
     if (ops[i] == nullptr || ops[i]->kind != kSector || ops[i]->dim > lim) {
@@ -593,6 +636,16 @@ int ftlm_hip_krylov_run(ftlm_krylov_t* k, int n_steps, void* stream) {
     FTLM_REQUIRE(k != nullptr && n_steps >= 0, FTLM_EINVAL, "bad argument");
     const int k1 = std::min(k->K.M, k->steps_done + n_steps);
     k->K.run(k->steps_done, k1, static_cast<hipStream_t>(stream));
+    k->steps_done = k1;
+  });
+}
+
+int ftlm_hip_krylov_run_graph(ftlm_krylov_t* k, int n_steps, void* stream) {
+  return guarded([&] {
+    FTLM_REQUIRE(k != nullptr && n_steps >= 0, FTLM_EINVAL, "bad argument");
+    FTLM_HIP_CHECK(hipSetDevice(k->K.op->device));
+    const int k1 = std::min(k->K.M, k->steps_done + n_steps);
+    k->K.run_graph(k->steps_done, k1, static_cast<hipStream_t>(stream));
     k->steps_done = k1;
   });
 }
@@ -669,7 +722,7 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
   return guarded([&] {
     check_opts(opts);
     FTLM_REQUIRE(ops != nullptr && n_ops >= 1 && ritz && weight && used, FTLM_EINVAL, "null argument");
-    FTLM_REQUIRE(opts->path >= FTLM_PATH_AUTO && opts->path <= FTLM_PATH_PERSISTENT, FTLM_EINVAL, "bad path");
+    FTLM_REQUIRE(opts->path >= FTLM_PATH_AUTO && opts->path <= FTLM_PATH_GRAPH, FTLM_EINVAL, "bad path");
     const int R = std::max(1, opts->R), M = std::max(1, opts->M);
     hipStream_t st = static_cast<hipStream_t>(stream);
     if (persistent_eligible(ops, n_ops, opts->path)) {
@@ -719,7 +772,10 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
         std::vector<unsigned> seeds(static_cast<size_t>(Rg));
         for (int r = 0; r < Rg; ++r) seeds[r] = opts->seed + static_cast<unsigned>(g0 + r) * 100003u;
         fill_start(K, opts->start, seeds, nullptr, st);
-        K.run(0, Mop, st);
+        if (opts->path == FTLM_PATH_GRAPH && op->kind != kCallback)
+          K.run_graph(0, Mop, st);
+        else
+          K.run(0, Mop, st);
         // quadrature per sample: alpha/beta are [Mop][Rg] (stride Rg)
         DevBuf<int> nb(static_cast<size_t>(Rg));
         std::vector<int> hu(static_cast<size_t>(Rg)), ha(static_cast<size_t>(Rg));

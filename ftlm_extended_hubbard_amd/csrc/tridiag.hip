@@ -255,29 +255,46 @@ __global__ void k_ql(const double* __restrict__ alpha, const double* __restrict_
       double r = hypot(g, 1.0);
       g = QD(m) - QD(l) + QE(l) / (g + (g >= 0.0 ? r : -r));
       double sn = 1.0, c = 1.0, p = 0.0;
-      int i = m - 1;
       bool deflated = false;
-      for (; i >= l; --i) {
-        double f = sn * QE(i);
-        const double b = c * QE(i);
-        r = hypot(f, g);
+      // d[i+1] and z[i+1] are carried in registers (no rotation of this sweep has changed d[i+1]
+      // since it was read as d[i]; z[i+1] is the previous rotation's z[i]); e, d, z at i-1 are
+      // loaded while rotation i runs.  Entries are bounded (tridiagonals of Lanczos on a
+      // Hamiltonian), so r = sqrt(f^2 + g^2) needs no scaling.
+      double d_n1 = QD(m), z_n1 = QZ(m);
+      double e_i = QE(m - 1), d_i = QD(m - 1), z_i = QZ(m - 1);
+      for (int i = m - 1; i >= l; --i) {
+        double e_p = 0.0, d_p = 0.0, z_p = 0.0;
+        if (i > l) {
+          e_p = QE(i - 1);
+          d_p = QD(i - 1);
+          z_p = QZ(i - 1);
+        }
+        const double f = sn * e_i;
+        const double b = c * e_i;
+        r = sqrt(f * f + g * g);
         QE(i + 1) = r;
         if (r == 0.0) {
-          QD(i + 1) -= p;
+          QD(i + 1) = d_n1 - p;
           QE(m) = 0.0;
           deflated = true;
           break;
         }
-        sn = f / r;
-        c = g / r;
-        g = QD(i + 1) - p;
-        r = (QD(i) - g) * sn + 2.0 * c * b;
+        const double inv = 1.0 / r;
+        sn = f * inv;
+        c = g * inv;
+        g = d_n1 - p;
+        r = (d_i - g) * sn + 2.0 * c * b;
         p = sn * r;
         QD(i + 1) = g + p;
         g = c * r - b;
-        f = QZ(i + 1);
-        QZ(i + 1) = sn * QZ(i) + c * f;
-        QZ(i) = c * QZ(i) - sn * f;
+        QZ(i + 1) = sn * z_i + c * z_n1;
+        const double zi = c * z_i - sn * z_n1;
+        QZ(i) = zi;
+        d_n1 = d_i;
+        z_n1 = zi;
+        e_i = e_p;
+        d_i = d_p;
+        z_i = z_p;
       }
       if (deflated) continue;
       QD(l) -= p;

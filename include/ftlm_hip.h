@@ -156,6 +156,8 @@ typedef struct ftlm_ftlm_opts {
 #define FTLM_PATH_PER_OP 1     /* per operator, R recurrences per block H*V */
 #define FTLM_PATH_PERSISTENT 2 /* every (sector, sample) recurrence in one persistent launch (sectors
                                   of dim <= 262144) */
+#define FTLM_PATH_GRAPH 3      /* per operator, the whole M-step recurrence captured as one hipGraph
+                                  and launched once (host-callback operators run eagerly) */
 
 /* Runs R independent Lanczos recurrences per operator, all on the device, and diagonalises every
  * tridiagonal with the chosen quadrature eigensolver.  Outputs (host):
@@ -200,6 +202,10 @@ typedef struct ftlm_krylov ftlm_krylov_t;
 int ftlm_hip_krylov_create(const ftlm_op_t* op, int R, int max_steps, int dtype, int start, unsigned seed,
                            ftlm_krylov_t** out);
 int ftlm_hip_krylov_run(ftlm_krylov_t* k, int n_steps, void* stream);
+/* The same n_steps captured as one hipGraph (kernel nodes only) and launched once on a private
+ * stream ordered after `stream`; results are identical to ftlm_hip_krylov_run.  No counterpart in
+ * the reference. */
+int ftlm_hip_krylov_run_graph(ftlm_krylov_t* k, int n_steps, void* stream);
 int ftlm_hip_krylov_result(ftlm_krylov_t* k, double* alpha, double* beta, int* used, int* alive);
 int ftlm_hip_krylov_bytes(const ftlm_krylov_t* k, int64_t* bytes);
 int ftlm_hip_krylov_destroy(ftlm_krylov_t* k);

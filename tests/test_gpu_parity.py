@@ -473,3 +473,60 @@ def test_observables_full_krylov_vs_dense(oracle, lx, ly, nu, nd, phi):
             if i == 0:
                 want = np.log(dim) + np.log(Z / 3) - b * w[0]
                 assert abs(lnz[ib] - want) <= 1e-9 * max(1.0, abs(want)), (b, lnz[ib], want)
+
+
+def _graph_case(kind):
+    if kind == "sector_two_pass":  # D_dn = 220 >= 128: row and column passes
+        return api.HubbardOperator.from_params(api.HubbardParams(4, 3, 1.0, -0.25, 8.0, 1.0, 0.05, 0.0), 3, 3)
+    if kind == "sector_flat":
+        return api.HubbardOperator.from_params(api.HubbardParams(4, 2, 1.0, -0.25, 8.0, 1.0, 0.1, 0.0), 3, 2)
+    p = api.HubbardParams(4, 2, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0)
+    return api.KBlockOperator.from_params(p, 3, 3, api.MomentumSector(1, 0, 4, 2))
+
+
+@pytest.mark.parametrize("kind", ["sector_two_pass", "sector_flat", "kblock"])
+@pytest.mark.parametrize("dtype", [L.FTLM_C128, L.FTLM_C64])
+def test_graph_path_identical_to_eager(kind, dtype):
+    """FTLM_PATH_GRAPH (each operator's recurrence captured as one hipGraph, launched once) against
+    the eager per-operator path.  R = 20 spans several vector chunks (their partial sums are
+    cleared by a kernel inside the graph).  Three graph runs, each followed by an eager run, all
+    checked.  The single-pass sector and block applies assign work statically: bit-identical.  The
+    two-pass sector apply takes tiles from an atomic queue, so its per-block dot partials are summed
+    in a run-dependent order (eager runs differ from each other the same way): early alpha/beta to
+    1e-12 and lnZ (converged rule) to 1e-9."""
+    say("graph vs eager", kind, dtype)
+    op = _graph_case(kind)
+    kw = dict(n_random=20, lanczos_steps=30, seed=3, start=L.FTLM_START_PHASE, dtype=dtype)
+    exact = kind != "sector_two_pass"
+    ref = api.ftlm_ritz([op], api.FtlmParams(path=L.FTLM_PATH_PER_OP, **kw), keep_tridiagonals=True)
+    betas = np.array([0.5, 2.0])
+    pc = dict(kw, quad=L.FTLM_QUAD_CONVERGED)
+    ref_lnz = api.ftlm_log_partition_complex(op, betas, api.FtlmParams(path=L.FTLM_PATH_PER_OP, **pc))
+    for rep in range(3):
+        g = api.ftlm_ritz([op], api.FtlmParams(path=L.FTLM_PATH_GRAPH, **kw), keep_tridiagonals=True)
+        e = api.ftlm_ritz([op], api.FtlmParams(path=L.FTLM_PATH_PER_OP, **kw), keep_tridiagonals=True)
+        for got in (g, e):
+            assert np.array_equal(got.used, ref.used)
+            if exact:
+                assert np.array_equal(got.alpha, ref.alpha) and np.array_equal(got.beta, ref.beta), rep
+                assert np.array_equal(got.ritz, ref.ritz) and np.array_equal(got.weight, ref.weight), rep
+            else:
+                assert np.allclose(got.alpha[..., :10], ref.alpha[..., :10], rtol=1e-12, atol=1e-12), rep
+                assert np.allclose(got.beta[..., :10], ref.beta[..., :10], rtol=1e-12, atol=1e-12), rep
+        lg = api.ftlm_log_partition_complex(op, betas, api.FtlmParams(path=L.FTLM_PATH_GRAPH, **pc))
+        assert np.max(np.abs(lg - ref_lnz)) <= 1e-9 * np.max(np.abs(ref_lnz)), (rep, lg, ref_lnz)
+
+
+def test_krylov_session_graph_steps():
+    """KrylovSession: 10 eager steps then 20 graph-captured steps equal 30 eager steps (a block
+    operator: statically scheduled, so bit-identical)."""
+    say("krylov session graph")
+    op = _graph_case("kblock")
+    a = api.KrylovSession(op, 6, 30, L.FTLM_C64, L.FTLM_START_PHASE, 5)
+    a.run(30)
+    b = api.KrylovSession(op, 6, 30, L.FTLM_C64, L.FTLM_START_PHASE, 5)
+    b.run(10)
+    b.run(20, graph=True)
+    ra, rb = a.result(), b.result()
+    for x, y in zip(ra, rb):
+        assert np.array_equal(x, y)
