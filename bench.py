@@ -346,6 +346,81 @@ def grad_workload(F, L, torch, validate=True):
     return out
 
 
+def kblock_all_workload(F, L, torch):
+    """All 16 momentum blocks of 4x4 (8,8) (t=1, t'=-0.25, U=8, V=1, phi=0), R=32 random-phase
+    vectors per block, M=120, converged Gauss rule: complex64 vectors (fp64 dots/alpha/beta) and
+    complex128 with the same starts; the block-summed lnZ(beta) for beta in [0.1, 20] against each
+    other and against an independent full-sector complex128 run (R=16, M=120), with the sampling
+    standard error of both estimates."""
+    import numpy as np
+
+    p = F.HubbardParams(4, 4, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0)
+    betas = np.geomspace(0.1, 20.0, 12)
+    first = F.KBlockOperator.from_params(p, NU, ND, F.MomentumSector(0, 0, 4, 4))
+    blocks = [first] + [F.KBlockOperator.from_params(p, NU, ND, F.MomentumSector(kx, ky, 4, 4), share=first)
+                        for ky in range(4) for kx in range(4) if (kx, ky) != (0, 0)]
+    dims = np.array([b.dim() for b in blocks], np.int64)
+    out = {"metric": "4x4 (8,8) all 16 momentum blocks FTLM, R=32, M=120", "unit": "s", "higher_is_better": False,
+           "n_gpus": 1, "data": "synthetic", "config": {"workload": "4x4 (8,8), 16 K blocks", "R": 32, "M": 120,
+                                                        "betas": betas.tolist()}, "dims": dims.tolist()}
+
+    def lnz_stats(rd, R):
+        # per-sample L_r(beta) = ln sum_k w e^{-beta lambda}; Z_op = dim mean_r e^{L_r}
+        lnz_ops, var_ops = [], []
+        for io in range(rd.ritz.shape[0]):
+            Ls = []
+            for r in range(R):
+                u = rd.used[io, r]
+                lam, w = rd.ritz[io, r, :u], rd.weight[io, r, :u]
+                x = -np.outer(betas, lam) + np.log(np.maximum(w, 1e-300))
+                mx = x.max(axis=1)
+                Ls.append(mx + np.log(np.exp(x - mx[:, None]).sum(axis=1)))
+            Ls = np.array(Ls)  # [R][nb]
+            m = Ls.max(axis=0)
+            e = np.exp(Ls - m)
+            lnz_ops.append(np.log(rd.dims[io]) + m + np.log(e.mean(axis=0)))
+            var_ops.append((e.std(axis=0, ddof=1) / e.mean(axis=0)) ** 2 / R)  # relative variance of Z_op
+        lnz_ops = np.array(lnz_ops)
+        tot = np.logaddexp.reduce(lnz_ops, axis=0)
+        wgt = np.exp(lnz_ops - tot)  # share of each op in Z
+        rel_se = np.sqrt((wgt ** 2 * np.array(var_ops)).sum(axis=0))
+        return tot, rel_se
+
+    res = {}
+    for name, dt in (("c64", L.FTLM_C64), ("c128", L.FTLM_C128)):
+        par = F.FtlmParams(n_random=32, lanczos_steps=120, seed=11, start=L.FTLM_START_PHASE, dtype=dt,
+                           quad=L.FTLM_QUAD_CONVERGED, path=L.FTLM_PATH_PER_OP)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rd = F.ftlm_ritz(blocks, par)
+        sec = time.perf_counter() - t0
+        tot, se = lnz_stats(rd, 32)
+        res[name] = (tot, se, sec)
+        log(f"16 blocks x R=32 x M=120 {name}: {sec:.1f} s")
+    del blocks, first
+    torch.cuda.empty_cache()
+    full = F.HubbardOperator.from_params(p, NU, ND)
+    parf = F.FtlmParams(n_random=16, lanczos_steps=120, seed=29, start=L.FTLM_START_PHASE, dtype=L.FTLM_C128,
+                        quad=L.FTLM_QUAD_CONVERGED, path=L.FTLM_PATH_PER_OP)
+    t0 = time.perf_counter()
+    rdf = F.ftlm_ritz([full], parf)
+    secf = time.perf_counter() - t0
+    totf, sef = lnz_stats(rdf, 16)
+    log(f"full sector R=16 M=120 c128: {secf:.1f} s")
+    d64 = np.abs(res["c64"][0] - res["c128"][0])
+    diff = res["c128"][0] - totf
+    sig = np.sqrt(res["c128"][1] ** 2 + sef ** 2)
+    out.update({"value": round(res["c64"][2], 3), "c128_s": round(res["c128"][2], 3), "full_sector_c128_s": round(secf, 3),
+                "lnZ_blocks_c64": res["c64"][0].tolist(), "lnZ_blocks_c128": res["c128"][0].tolist(),
+                "lnZ_full_c128": totf.tolist(), "max_abs_dlnZ_c64_vs_c128": float(d64.max()),
+                "dlnZ_blocks_minus_full": diff.tolist(), "sampling_se": sig.tolist(),
+                "max_abs_dlnZ_over_se": float(np.max(np.abs(diff) / sig))})
+    log(f"block-summed lnZ c64 vs c128 (same starts): max |dlnZ| {d64.max():.2e}")
+    log(f"block-summed vs full sector (independent samples): dlnZ {np.round(diff, 5).tolist()}")
+    log(f"  sampling standard error {np.round(sig, 5).tolist()}")
+    return out
+
+
 def graph_workload(F, L, torch):
     """Eager vs hipGraph-captured per-operator Lanczos (FTLM_PATH_PER_OP vs FTLM_PATH_GRAPH):
     M=100 steps of R=16 complex64 random-phase vectors on sectors of growing size (t=1,
@@ -467,7 +542,7 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["sector", "kblock", "gc4x2", "grad", "graph"], default="sector",
+    ap.add_argument("--workload", choices=["sector", "kblock", "kblock_all", "gc4x2", "grad", "graph"], default="sector",
                     help="sector: the 4x4 (8,8) full-sector line (default); kblock: the 4x4 (8,8) momentum-block "
                          "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch); grad: "
                          "the differentiated FTLM line (4x3 (6,6) and the 4x4 (8,8) K=(0,0) block)")
@@ -489,6 +564,9 @@ def main():
     K, W, R = args.steps, args.warmup, args.R
     if args.profile_hv:
         profile_hv(F, L, torch, R)
+        return
+    if args.workload == "kblock_all":
+        print(json.dumps(kblock_all_workload(F, L, torch)), flush=True)
         return
     if args.workload == "graph":
         print(json.dumps(graph_workload(F, L, torch)), flush=True)
