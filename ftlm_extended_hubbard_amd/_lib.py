@@ -104,6 +104,7 @@ def lib() -> C.CDLL:
     L.ftlm_hip_build_geometry_derivative.argtypes = [C.POINTER(Params), C.c_int, vp, C.c_int, ip]
     L.ftlm_hip_ftlm_gradient.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp, C.c_int, vp, vp]
     L.ftlm_hip_ftlm_vjp.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]
+    L.ftlm_hip_dense_jacobi.argtypes = [vp, C.c_int, C.c_int, C.c_double, vp, vp, vp]
     L.ftlm_hip_ftlm_observables.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp, C.c_int, vp, vp]
     L.ftlm_hip_ftlm_tape_forward.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp, C.c_int, C.c_int, C.c_int, vp,
                                              C.POINTER(vp)]

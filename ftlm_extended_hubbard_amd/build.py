@@ -113,6 +113,15 @@ def build_host(verbose: bool = False) -> None:
         _run([CXX, *HOST_FLAGS, f"-DFTLM_NMU_K={k}", drv, "-o", out + ".tmp", f"-L{PKG}", "-lftlm_hip",
               "-Wl,-rpath,$ORIGIN/.."], name)
         os.replace(out + ".tmp", out)
+    # the small benchmark drivers (C ABI plus HIP runtime device buffers)
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    bench = os.path.join(HOST, "drivers", "ftlm_bench.cpp")
+    for name, k in (("ftlm_bench_2x2", 1), ("ftlm_bench_4x2_ftlm", 2), ("ftlm_bench_2x2_nmu", 3)):
+        out = os.path.join(bin_dir, name)
+        _run([CXX, *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", f"-I{rocm}/include", f"-DFTLM_BENCH={k}", bench, "-o",
+              out + ".tmp", f"-L{PKG}", "-lftlm_hip", f"-L{rocm}/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN/..",
+              f"-Wl,-rpath,{rocm}/lib"], name)
+        os.replace(out + ".tmp", out)
     if verbose:
         print("built", lib, "and", len(srcs), "C++ tests, drivers in", bin_dir)
 

@@ -538,6 +538,28 @@ int ftlm_hip_exact_log_partition(const ftlm_op_t* op, const double* betas, int n
   });
 }
 
+int ftlm_hip_dense_jacobi(const double* A, int n, int max_rotations, double tol, double* diag, double* lo,
+                          double* hi) {
+  return guarded([&] {
+    FTLM_REQUIRE(A != nullptr && n >= 1 && max_rotations >= 1, FTLM_EINVAL, "bad arguments");
+    const size_t shmem = static_cast<size_t>(n) * (sizeof(double) + sizeof(RowMax));
+    FTLM_REQUIRE(shmem <= 60 * 1024 && n <= 2048, FTLM_EUNSUPPORTED, "matrix too large for the Jacobi kernel");
+    const int64_t nn = static_cast<int64_t>(n) * n;
+    DevBuf<double> dA(static_cast<size_t>(nn)), scratch(static_cast<size_t>(nn)), lam(static_cast<size_t>(n));
+    dA.upload(A, static_cast<size_t>(nn));
+    DevBuf<int> dn(1);
+    dn.upload(&n, 1);
+    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(kJacBlock), shmem, nullptr, nullptr, nullptr, dn.p, nullptr, 1,
+                       max_rotations, tol, 0, scratch.p, nn, lam.p, nullptr, n, dA.p);
+    check_launch("k_jacobi(dense)");
+    std::vector<double> d(static_cast<size_t>(n));
+    FTLM_HIP_CHECK(hipMemcpy(d.data(), lam.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (diag) std::copy(d.begin(), d.end(), diag);
+    if (lo) *lo = *std::min_element(d.begin(), d.end());
+    if (hi) *hi = *std::max_element(d.begin(), d.end());
+  });
+}
+
 int ftlm_hip_tridiagonal_extrema(const double* alpha, int n, const double* beta, int nb, double* lo, double* hi) {
   return guarded([&] {
     FTLM_REQUIRE(lo != nullptr && hi != nullptr, FTLM_EINVAL, "null output");

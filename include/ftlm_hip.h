@@ -134,6 +134,15 @@ int ftlm_hip_lanczos(const ftlm_op_t* op, int max_steps, unsigned seed, const do
 int ftlm_hip_lanczos_extrema(const ftlm_op_t* op, int max_steps, unsigned seed, double* min_eval,
                              double* max_eval, int* steps_used);
 
+/* The capped max-pivot Jacobi of the reference's dense checks -- jacobi_min_max
+ * (benchmarks/bench_2x2.cpp:32-95, 1600 rotations at 1e-15) and sorted_eigenvalues_jacobi
+ * (benchmarks/bench_2x2_nmu.cpp:75-126, 12000 rotations at 1e-14) -- on the device for a real
+ * symmetric n x n host matrix (row-major): at most max_rotations rotations, stopping when the
+ * largest off-diagonal magnitude is below tol.  diag (n, unsorted, may be NULL) receives the final
+ * diagonal, lo / hi (may be NULL) its extremes. */
+int ftlm_hip_dense_jacobi(const double* A, int n, int max_rotations, double tol, double* diag, double* lo,
+                          double* hi);
+
 /* tridiagonal_extrema (src/lanczos.cpp:106-123), evaluated on the device. */
 int ftlm_hip_tridiagonal_extrema(const double* alpha, int n, const double* beta, int nb, double* lo,
                                  double* hi);

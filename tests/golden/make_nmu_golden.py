@@ -4,6 +4,8 @@ Run where /root/reference exists (after `make -C oracle ref refbench`):
     python tests/golden/make_nmu_golden.py
 - "rect": the reference's own bench_ftlm_nmu_rect (benchmarks/bench_ftlm_nmu_rect.cpp, built from
   its sources into oracle/_ref/) on small lattices: the n(mu) table and every sector's logZ.
+- "bench": the printed numbers of the reference's bench_2x2, bench_4x2_ftlm and bench_2x2_nmu
+  (built the same way).
 - "rect_k": the momentum-block solver of benchmarks/bench_ftlm_nmu_4x4_k.cpp with
   --solver=k-orbit on a 3x2 lattice.  That driver needs LAPACK and is not built here; its
   per-block rule (:1072-1079: ED below the cutoff, else ftlm_log_partition_complex on
@@ -111,8 +113,41 @@ def run_rect_k(c):
     return {"args": args, "mu": mu, "n": nn, "sector_logZ": logZ, "blocks": blocks}
 
 
+NUM = r"(-?[0-9.]+(?:e[-+]?[0-9]+)?)"
+BENCH_CASES = {
+    "bench_2x2": ["--matvec-iters=0"],
+    "bench_4x2_ftlm": ["--samples=8", "--lanczos-steps=64"],
+    "bench_2x2_nmu": ["--beta=5", "--n-mu=31"],
+}
+
+
+def run_bench(name, args):
+    """The printed numbers of the reference's small drivers (bench_2x2, bench_4x2_ftlm, bench_2x2_nmu)."""
+    r = subprocess.run([os.path.join(ROOT, "oracle", "_ref", name), *args], capture_output=True, text=True,
+                       check=True)
+    txt = r.stdout
+    out = {"args": args}
+    if name == "bench_2x2":
+        m = re.search(r"E_min=" + NUM + r"  E_max=" + NUM, txt)
+        out["dense_min"], out["dense_max"] = float(m.group(1)), float(m.group(2))
+        m = re.search(r"used=(\d+) .*E_min≈" + NUM + r"  E_max≈" + NUM, txt)
+        out["lz_used"], out["lz_min"], out["lz_max"] = int(m.group(1)), float(m.group(2)), float(m.group(3))
+    elif name == "bench_4x2_ftlm":
+        m = re.search(r"avg Ritz E_min=" + NUM + r"  E_max=" + NUM, txt)
+        out["avg_min"], out["avg_max"] = float(m.group(1)), float(m.group(2))
+        out["sum_steps"] = int(re.search(r"sum_lanczos_steps_used=(\d+)", txt).group(1))
+        out["matvec_count"] = int(re.search(r"matvec_count=(\d+)", txt).group(1))
+    else:
+        rows = [line.split("\t") for line in txt.splitlines() if "\t" in line and not line.startswith("#")]
+        out["mu"] = [float(a) for a, _ in rows]
+        out["n"] = [float(b) for _, b in rows]
+        out["levels"] = int(re.search(r"exact diagonalization: (\d+) eigenlevels", txt).group(1))
+    return out
+
+
 def main():
-    out = {"rect": [run_rect(a) for a in RECT_CASES], "rect_k": [run_rect_k(K_CASE)]}
+    out = {"rect": [run_rect(a) for a in RECT_CASES], "rect_k": [run_rect_k(K_CASE)],
+           "bench": {k: run_bench(k, v) for k, v in BENCH_CASES.items()}}
     path = os.path.join(HERE, "nmu.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
