@@ -230,4 +230,20 @@ int hv_partial_blocks(const ftlm_op* op);
 // True when a momentum-block operator is Hermitian (translation-invariant hop coefficients).
 bool kblock_hermitian(const ftlm_op* op);
 
+// Derivative operators of a momentum block as term-split data (orbit.hip): up to four hop tables
+// over the block's own bond ids ([j][512] complex) and up to four diagonal arrays (each
+// derivative block's own orbit diagonal); tab[p] / dg[p] name parameter p's table / array or -1.
+struct KbTermSplit {
+  int ntab = 0, ndg = 0;
+  std::vector<int> tab, dg;
+  const double* diag[4] = {nullptr, nullptr, nullptr, nullptr};
+  DevBuf<double2> dc;
+};
+bool kblock_termsplit(const ftlm_op* op, ftlm_op* const* dops, int npar, KbTermSplit* T);
+// Block partial sums [blocks][8] of the term-split bilinear forms Re <a| . |b> (R vectors);
+// kblock_bilinear_blocks gives the number of blocks.
+int kblock_bilinear_blocks(const ftlm_op* op, int R);
+void kblock_bilinear(const ftlm_op* op, const KbTermSplit& T, const double2* a, const double2* b, int R,
+                     double2* part, hipStream_t st);
+
 }  // namespace ftlm_hip

@@ -668,33 +668,38 @@ __global__ __launch_bounds__(kGBlock) void k_bil_sector(const SectorView S, cons
 #pragma unroll
     for (int j = 0; j < kTsTables; ++j) tr[j] = ti[j] = 0.0;
     const int64_t rowbase = static_cast<int64_t>(iu) * S.Ddn;
-    for (int k = S.off_dn[id]; k < S.off_dn[id + 1]; ++k) {  // down bonds: sources in row iu
-      const uint32_t en = S.ent_dn[k];
-      const uint32_t t = en & kTargetMask, bd = (en >> kTargetBits) & kBondMask;
-      const bool neg = ((((en >> kParityBit) & 1u) ^ (__popc(u & s_xm[nbp + bd]) & 1u)) != 0);
-      double2 src = b[(rowbase + t) * R + r];
-      if (neg) src = make_double2(-src.x, -src.y);
+    constexpr int G = 8;  // hops whose loads are issued together
+    // spin 1 (down bonds, sources in row iu), then spin 0 (up bonds, sources in column id)
 #pragma unroll
-      for (int j = 0; j < kTsTables; ++j)
-        if (j < ntab) {
-          const double2 c = s_dc[(j * 2 + 1) * nbp + bd];
-          tr[j] += c.x * src.x - c.y * src.y;
-          ti[j] += c.x * src.y + c.y * src.x;
-        }
-    }
-    for (int k = S.off_up[iu]; k < S.off_up[iu + 1]; ++k) {  // up bonds: sources in column id
-      const uint32_t en = S.ent_up[k];
-      const uint32_t t = en & kTargetMask, bd = (en >> kTargetBits) & kBondMask;
-      const bool neg = ((((en >> kParityBit) & 1u) ^ (__popc(d & s_xm[bd]) & 1u)) != 0);
-      double2 src = b[(static_cast<int64_t>(t) * S.Ddn + id) * R + r];
-      if (neg) src = make_double2(-src.x, -src.y);
+    for (int sp = 1; sp >= 0; --sp) {
+      const int k0 = sp ? S.off_dn[id] : S.off_up[iu], k1 = sp ? S.off_dn[id + 1] : S.off_up[iu + 1];
+      const uint32_t* E = sp ? S.ent_dn : S.ent_up;
+      const uint32_t other = sp ? u : d;
+      for (int k = k0; k < k1; k += G) {
+        uint32_t en[G];
+        double2 src[G];
 #pragma unroll
-      for (int j = 0; j < kTsTables; ++j)
-        if (j < ntab) {
-          const double2 c = s_dc[(j * 2) * nbp + bd];
-          tr[j] += c.x * src.x - c.y * src.y;
-          ti[j] += c.x * src.y + c.y * src.x;
+        for (int j = 0; j < G; ++j) en[j] = E[min(k + j, k1 - 1)];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const int64_t t = en[j] & kTargetMask;
+          src[j] = b[(sp ? rowbase + t : t * S.Ddn + id) * R + r];
         }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          if (k + j >= k1) continue;
+          const uint32_t bd = (en[j] >> kTargetBits) & kBondMask;
+          const bool neg = ((((en[j] >> kParityBit) & 1u) ^ (__popc(other & s_xm[sp * nbp + bd]) & 1u)) != 0);
+          const double2 x = neg ? make_double2(-src[j].x, -src[j].y) : src[j];
+#pragma unroll
+          for (int q = 0; q < kTsTables; ++q)
+            if (q < ntab) {
+              const double2 c = s_dc[(q * 2 + sp) * nbp + bd];
+              tr[q] += c.x * x.x - c.y * x.y;
+              ti[q] += c.x * x.y + c.y * x.x;
+            }
+        }
+      }
     }
 #pragma unroll
     for (int j = 0; j < kTsTables; ++j) acc[j] += av.x * tr[j] + av.y * ti[j];
@@ -723,14 +728,21 @@ unsigned grid_n(int64_t n) { return static_cast<unsigned>(std::min<int64_t>(kGMa
 // sector, every derivative operator is a sector of the same (n_sites, n_up, n_dn) whose bonds are
 // bonds of H, at most kTsTables of them have hops, and V terms use H's NN pair list.
 struct TermSplit {
-  bool fused = false;
+  bool fused = false;   // a term-split form exists (full sector or momentum block)
+  bool kblock = false;  // the momentum-block form (kb) rather than the sector tables
   int ntab = 0, nbp = 1;
   std::vector<int> tab;
   std::vector<double> dU, dV;
   DevBuf<double2> dc;  // [ntab][spin][nbp]
+  KbTermSplit kb;
 };
 
 bool build_termsplit(const ftlm_op* op, ftlm_op* const* dops, int npar, TermSplit* T) {
+  if (op->kind == kKBlock) {
+    if (!kblock_termsplit(op, dops, npar, &T->kb)) return false;
+    T->kblock = T->fused = true;
+    return true;
+  }
   if (op->kind != kSector) return false;
   const Sector* H = static_cast<const Sector*>(op);
   std::map<std::pair<int, int>, int> id[2];
@@ -1045,8 +1057,11 @@ void tape_backward(const ftlm_tape* T, const double* gw, double* grad) {
   DevBuf<double2> A(vb), P(vb), WA(vb), Y(vb), tmp(ts.fused ? 0 : vb);
   const int bsz = (kGBlock / R) * R;
   const int gblk = static_cast<int>(std::min<int64_t>(kGMaxGrid, (n + bsz - 1) / bsz));
-  const int gbil = static_cast<int>(std::min<int64_t>(kGMaxGrid, (n + kGBlock - 1) / kGBlock));
-  DevBuf<double2> part(std::max(static_cast<size_t>(gblk) * 2 * R, static_cast<size_t>(gbil) * kTsQ));
+  const int gbil = ts.kblock ? kblock_bilinear_blocks(op, R)
+                             : static_cast<int>(std::min<int64_t>(kGMaxGrid, (n + kGBlock - 1) / kGBlock));
+  constexpr int kKbQ = 8;  // the momentum-block form's partial sums per block (4 tables + 4 diagonals)
+  const int nq_bil = ts.kblock ? kKbQ : kTsQ;
+  DevBuf<double2> part(std::max(static_cast<size_t>(gblk) * 2 * R, static_cast<size_t>(gbil) * nq_bil));
   DevBuf<double2> dres(static_cast<size_t>(2) * R);
   std::vector<double2> hres(static_cast<size_t>(2) * R);
   auto collect = [&](int nq) {
@@ -1055,7 +1070,7 @@ void tape_backward(const ftlm_tape* T, const double* gw, double* grad) {
     FTLM_HIP_CHECK(hipMemcpyAsync(hres.data(), dres.p, sizeof(double2) * nq * R, hipMemcpyDeviceToHost, st));
     FTLM_HIP_CHECK(hipStreamSynchronize(st));
   };
-  const int nacc = ts.fused ? kTsQ : npar;
+  const int nacc = ts.fused ? nq_bil : npar;
   DevBuf<double> tacc(static_cast<size_t>(nacc));
   FTLM_HIP_CHECK(hipMemsetAsync(tacc.p, 0, tacc.n * sizeof(double), st));
   FTLM_HIP_CHECK(hipMemsetAsync(A.p, 0, vb * sizeof(double2), st));
@@ -1069,9 +1084,9 @@ void tape_backward(const ftlm_tape* T, const double* gw, double* grad) {
       d_bm(static_cast<size_t>(R)), dbt(static_cast<size_t>(R));
   DevBuf<double2> d_zb(static_cast<size_t>(R)), d_zk(static_cast<size_t>(R));
   DevBuf<int> dalive(static_cast<size_t>(R));
-  const Sector* Hs = ts.fused ? static_cast<const Sector*>(op) : nullptr;
+  const Sector* Hs = ts.fused && !ts.kblock ? static_cast<const Sector*>(op) : nullptr;
   const size_t bil_lds =
-      ts.fused ? static_cast<size_t>(ts.ntab) * 2 * ts.nbp * sizeof(double2) + 2 * static_cast<size_t>(ts.nbp) * 4 : 0;
+      Hs ? static_cast<size_t>(ts.ntab) * 2 * ts.nbp * sizeof(double2) + 2 * static_cast<size_t>(ts.nbp) * 4 : 0;
   auto beta_at = [&](int k, int r) { return k >= 0 ? bs[static_cast<size_t>(k) * R + r] : 0.0; };
   for (int c = (kmax - 1) / s; c >= 0; --c) {
     const int j0 = c * s, j1 = std::min(j0 + s, kmax);
@@ -1124,7 +1139,11 @@ void tape_backward(const ftlm_tape* T, const double* gw, double* grad) {
       hipLaunchKernelGGL(k_adj_wa, dim3(gblk), dim3(kGBlock), 0, st, n, R, WA.p, qk, d_zb.p);
       check_launch("k_adj_wa");
       // parameter adjoints: Re <w_a-bar | dH_p q_k>
-      if (ts.fused) {
+      if (ts.kblock) {
+        kblock_bilinear(op, ts.kb, WA.p, qk, R, part.p, st);
+        hipLaunchKernelGGL(k_tsum, dim3(kKbQ), dim3(kGBlock), 0, st, part.p, static_cast<int64_t>(gbil), kKbQ, tacc.p);
+        check_launch("k_tsum");
+      } else if (ts.fused) {
         hipLaunchKernelGGL(k_bil_sector, dim3(gbil), dim3(kGBlock), bil_lds, st, Hs->view(), WA.p, qk, R, ts.dc.p,
                            ts.ntab, ts.nbp, part.p);
         hipLaunchKernelGGL(k_tsum, dim3(kTsQ), dim3(kGBlock), 0, st, part.p, static_cast<int64_t>(gbil), kTsQ, tacc.p);
@@ -1162,6 +1181,13 @@ void tape_backward(const ftlm_tape* T, const double* gw, double* grad) {
   for (int p = 0; p < npar; ++p) {
     if (!ts.fused) {
       grad[p] = hacc[p];
+      continue;
+    }
+    if (ts.kblock) {
+      double g = 0.0;
+      if (ts.kb.tab[p] >= 0) g += hacc[ts.kb.tab[p]];
+      if (ts.kb.dg[p] >= 0) g += hacc[4 + ts.kb.dg[p]];
+      grad[p] = g;
       continue;
     }
     double g = ts.dU[p] * hacc[kTsTables] + ts.dV[p] * hacc[kTsTables + 1];

@@ -608,6 +608,108 @@ __global__ __launch_bounds__(256) void k_kblock_apply_split(
   }
 }
 
+
+// Term-split bilinear forms of a momentum block (derivative operators dH_p over the same block):
+// per hop table j (derivative coefficients over this graph's bond ids, [j][kMaxBondsLds])
+// Re <a| dH_j |b>, and per diagonal array k (a derivative operator's own orbit diagonal)
+// Re <a| diag_k |b>, summed over the R vectors.  Terms are the apply's: coefficient x sign x
+// sqrt(|Stab_row| / |Stab_col|) x Bloch phase, the column resolved per K.  One thread per
+// (row, vector).  partials: part[block * kKbQ + j] (hop tables) and [block * kKbQ + 4 + k].
+constexpr int kKbTables = 4;
+constexpr int kKbQ = 2 * kKbTables;
+
+struct KbDiags {
+  const double* d[kKbTables];
+};
+
+__global__ __launch_bounds__(256) void k_bil_kblock(const int32_t* __restrict__ rows, int64_t dk,
+                                                    const int32_t* __restrict__ row_off,
+                                                    const uint2* __restrict__ ent, const int32_t* __restrict__ colk,
+                                                    Phases ph, const double2* __restrict__ dc, int ntab, KbDiags dg,
+                                                    int ndg, const double2* __restrict__ a,
+                                                    const double2* __restrict__ b, int R, double2* __restrict__ part) {
+  __shared__ double2 s_dc[kKbTables][kMaxBondsLds];
+  __shared__ double2 s_chi[kMaxOrbitSites];
+  __shared__ double s_nrm[kMaxOrbitSites * kMaxOrbitSites];
+  for (int i = threadIdx.x; i < kKbTables * kMaxBondsLds; i += blockDim.x)
+    s_dc[i / kMaxBondsLds][i % kMaxBondsLds] = i / kMaxBondsLds < ntab ? dc[i] : make_double2(0.0, 0.0);
+  if (threadIdx.x < kMaxOrbitSites) s_chi[threadIdx.x] = ph.chi[threadIdx.x];
+  for (int i = threadIdx.x; i < kMaxOrbitSites * kMaxOrbitSites; i += blockDim.x)
+    s_nrm[i] = sqrt(static_cast<double>((i & 15) + 1) / static_cast<double>((i >> 4) + 1));
+  __syncthreads();
+  double acc[kKbQ];
+#pragma unroll
+  for (int j = 0; j < kKbQ; ++j) acc[j] = 0.0;
+  const int64_t n = dk * R;
+  for (int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; f < n;
+       f += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t r = f / R;
+    const int v = static_cast<int>(f - r * R);
+    const int32_t g = rows[r];
+    const double2 av = a[f];
+    double tr[kKbTables], ti[kKbTables];
+#pragma unroll
+    for (int j = 0; j < kKbTables; ++j) tr[j] = ti[j] = 0.0;
+    double dre = 0.0;  // Re conj(a) b over the diagonal entry
+    const int32_t e0 = row_off[g], e1 = row_off[g + 1];
+    constexpr int G = 8;  // terms whose loads are issued together
+    for (int32_t e = e0; e < e1; e += G) {
+      uint2 en[G];
+      int32_t cc[G];
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        const int32_t ej = min(e + j, e1 - 1);
+        en[j] = ent[ej];
+        const int32_t c = colk[ej];
+        cc[j] = (e + j < e1 && c >= 0) ? c : -1;
+      }
+      double2 src[G];
+#pragma unroll
+      for (int j = 0; j < G; ++j) src[j] = b[(cc[j] >= 0 ? static_cast<int64_t>(cc[j]) : r) * R + v];
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        if (cc[j] < 0) continue;
+        if (en[j].x & kDiagFlag) {
+          dre += av.x * src[j].x + av.y * src[j].y;
+          continue;
+        }
+        const double nrm = s_nrm[(en[j].y >> kStabShift) & 0xFFu];
+        const double w = ((en[j].x >> 28) & 1u) ? -nrm : nrm;
+        const double2 ch = s_chi[(en[j].x >> 24) & 15u];
+        const uint32_t bd = en[j].y & kBondMask;
+#pragma unroll
+        for (int t = 0; t < kKbTables; ++t)
+          if (t < ntab) {
+            const double2 cb = s_dc[t][bd];
+            const double hr = cb.x * w, hi = cb.y * w;
+            const double cr = hr * ch.x - hi * ch.y, ci = hr * ch.y + hi * ch.x;
+            tr[t] += cr * src[j].x - ci * src[j].y;
+            ti[t] += cr * src[j].y + ci * src[j].x;
+          }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < kKbTables; ++j) acc[j] += av.x * tr[j] + av.y * ti[j];
+#pragma unroll
+    for (int k = 0; k < kKbTables; ++k)
+      if (k < ndg) acc[kKbTables + k] += dg.d[k][g] * dre;
+  }
+  __shared__ double red[256 / 64][kKbQ];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int j = 0; j < kKbQ; ++j) {
+    double x = acc[j];
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o, 64);
+    if (lane == 0) red[wv][j] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < kKbQ) {
+    double x = 0.0;
+    for (int w = 0; w < 256 / 64; ++w) x += red[w][threadIdx.x];
+    part[static_cast<int64_t>(blockIdx.x) * kKbQ + threadIdx.x] = make_double2(x, 0.0);
+  }
+}
+
 // ------------------------------------------------------------------ host objects
 namespace {
 
@@ -943,6 +1045,75 @@ void launch_kblock_chunk(const KBlock* B, const void* x, void* y, int RT, int r0
 constexpr int kKbMaxGrid = 4096;
 
 bool kblock_hermitian(const ftlm_op* op) { return static_cast<const KBlock*>(op)->G->hermitian; }
+
+bool kblock_termsplit(const ftlm_op* op, ftlm_op* const* dops, int npar, KbTermSplit* T) {
+  if (op->kind != kKBlock) return false;
+  const KBlock* H = static_cast<const KBlock*>(op);
+  const OrbitGraph& G = *H->G;
+  // H's bond ids: merged (from, to, spin) in first-occurrence order, as build_orbit_graph numbers them
+  std::map<std::tuple<int, int, int>, int> id;
+  for (const ftlm_hop_t& h : G.hops) {
+    if (h.from == h.to) continue;
+    const auto key = std::make_tuple(h.from, h.to, h.spin);
+    if (id.find(key) == id.end()) {
+      const int b = static_cast<int>(id.size());
+      id.emplace(key, b);
+    }
+  }
+  std::vector<std::complex<double>> tabs;
+  T->tab.assign(static_cast<size_t>(npar), -1);
+  T->dg.assign(static_cast<size_t>(npar), -1);
+  T->ntab = T->ndg = 0;
+  for (int p = 0; p < npar; ++p) {
+    if (dops[p]->kind != kKBlock) return false;
+    const KBlock* D = static_cast<const KBlock*>(dops[p]);
+    const OrbitGraph& GD = *D->G;
+    if (GD.lx != G.lx || GD.ly != G.ly || GD.n_up != G.n_up || GD.n_dn != G.n_dn || D->kx != H->kx ||
+        D->ky != H->ky || D->dim != H->dim || GD.nrep != G.nrep)
+      return false;
+    bool any_hop = false;
+    for (const ftlm_hop_t& h : GD.hops)
+      if (h.from != h.to) any_hop = true;
+    if (any_hop) {
+      if (T->ntab == kKbTables) return false;
+      const int j = T->ntab++;
+      tabs.resize(static_cast<size_t>(T->ntab) * kMaxBondsLds);
+      for (const ftlm_hop_t& h : GD.hops) {
+        if (h.from == h.to) continue;
+        const auto it = id.find(std::make_tuple(h.from, h.to, h.spin));
+        if (it == id.end()) return false;
+        tabs[static_cast<size_t>(j) * kMaxBondsLds + it->second] += std::complex<double>(h.re, h.im);
+      }
+      T->tab[p] = j;
+    }
+    if (GD.U != 0.0 || GD.V != 0.0) {
+      if (T->ndg == kKbTables) return false;
+      T->diag[T->ndg] = GD.diag.p;
+      T->dg[p] = T->ndg++;
+    }
+  }
+  std::vector<double2> h(std::max<size_t>(tabs.size(), 1), make_double2(0.0, 0.0));
+  for (size_t i = 0; i < tabs.size(); ++i) h[i] = make_double2(tabs[i].real(), tabs[i].imag());
+  T->dc.alloc(h.size());
+  T->dc.upload(h);
+  return true;
+}
+
+int kblock_bilinear_blocks(const ftlm_op* op, int R) {
+  const int64_t n = op->dim * R;
+  return static_cast<int>(std::min<int64_t>(kKbMaxGrid, (n + 255) / 256));
+}
+
+void kblock_bilinear(const ftlm_op* op, const KbTermSplit& T, const double2* a, const double2* b, int R,
+                     double2* part, hipStream_t st) {
+  const KBlock* B = static_cast<const KBlock*>(op);
+  const OrbitGraph& G = *B->G;
+  KbDiags dg{};
+  for (int k = 0; k < T.ndg; ++k) dg.d[k] = T.diag[k];
+  hipLaunchKernelGGL(k_bil_kblock, dim3(kblock_bilinear_blocks(op, R)), dim3(256), 0, st, B->rows.p, B->dim,
+                     G.row_off.p, G.ent.p, B->colk.p, B->ph, T.dc.p, T.ntab, dg, T.ndg, a, b, R, part);
+  check_launch("k_bil_kblock");
+}
 
 int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
                     hipStream_t st) {

@@ -247,10 +247,11 @@ int ftlm_hip_ftlm_gradient(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar
  * grad_p = sum_b gw[b] dlnZ(beta_b)/dtheta_p, from one forward Lanczos pass that keeps
  * (q_{k-1}, q_k) every ckpt steps (0: round(sqrt(2M))), the quadrature's derivatives, and one
  * adjoint pass that recomputes each segment's Krylov vectors from its checkpoint.  The parameter
- * adjoints sum_k Re <w-bar_k| dH_p |q_k> come from one term-split pass over a full sector's own
- * hop lists (all derivative tables and the U / V terms at once) when every dH_p is a hop table
- * over H's bonds plus U / V terms (mode FTLM_TS_AUTO or FTLM_TS_FUSED), otherwise from one apply
- * of each dH_p (FTLM_TS_GENERIC, and any operator kind).  The backward sweep applies H as H^dag:
+ * adjoints sum_k Re <w-bar_k| dH_p |q_k> come from one term-split pass over the operator's own
+ * terms -- a full sector's hop lists or a momentum block's orbit terms, all derivative tables and
+ * the U / V terms at once -- when every dH_p is a hop table over H's bonds plus U / V terms (mode
+ * FTLM_TS_AUTO or FTLM_TS_FUSED), otherwise from one apply of each dH_p (FTLM_TS_GENERIC, and
+ * callback operators).  The backward sweep applies H as H^dag:
  * momentum blocks whose hop coefficients are not translation invariant (flux on wrapped bonds)
  * are refused with FTLM_EUNSUPPORTED; callback operators must be Hermitian.  grad is [npar].  No
  * counterpart in the reference. */

@@ -281,9 +281,10 @@ def test_kblock_gradient_vs_central_differences(lx, ly, nu, nd, kx, ky, phi):
 
 
 def test_kblock_vjp_matches_forward_mode():
-    """Momentum block (no term-split form: per-operator parameter adjoints) against forward mode at
-    phi = 0.  With flux the reference's block operator is not Hermitian (its Peierls phases sit on
-    the wrapped bonds, which translations move): the adjoint sweep refuses it."""
+    """Momentum block adjoint (term-split pass over the block's orbit terms, and per-operator
+    parameter adjoints) against forward mode at phi = 0.  With flux the reference's block operator
+    is not Hermitian (its Peierls phases sit on the wrapped bonds, which translations move): the
+    adjoint sweep refuses it."""
     say("kblock vjp")
     p = model(4, 2, 0.0)
     K = api.MomentumSector(1, 0, 4, 2)
@@ -292,12 +293,11 @@ def test_kblock_vjp_matches_forward_mode():
     par = api.FtlmParams(n_random=4, lanczos_steps=24, seed=9, quad=L.FTLM_QUAD_CONVERGED, clamp_M_to_dim=True)
     names = ("t", "tp", "U", "V", "phi_x")
     lnz_f, g = api.ftlm_log_partition_gradient(p, 3, 3, betas, par, params=names, K=K)
-    lnz, got = api.ftlm_log_partition_vjp(p, 3, 3, betas, w, par, params=names, K=K, checkpoint=5)
-    assert np.max(np.abs(lnz - lnz_f)) < 1e-10
-    for k in names:
-        want = float(np.dot(w, g[k]))
-        assert abs(got[k] - want) <= 1e-9 * max(abs(want), 1e-3), (k, got[k], want)
-    with pytest.raises(Exception, match="term-split"):
-        api.ftlm_log_partition_vjp(p, 3, 3, betas, w, par, params=names, K=K, mode=L.FTLM_TS_FUSED)
+    for mode in (L.FTLM_TS_FUSED, L.FTLM_TS_GENERIC):
+        lnz, got = api.ftlm_log_partition_vjp(p, 3, 3, betas, w, par, params=names, K=K, checkpoint=5, mode=mode)
+        assert np.max(np.abs(lnz - lnz_f)) < 1e-10
+        for k in names:
+            want = float(np.dot(w, g[k]))
+            assert abs(got[k] - want) <= 1e-9 * max(abs(want), 1e-3), (mode, k, got[k], want)
     with pytest.raises(Exception, match="not Hermitian"):
         api.ftlm_log_partition_vjp(model(4, 2, 0.05), 3, 3, betas, w, par, params=names, K=K)
