@@ -23,7 +23,7 @@
 //   orc_tridiag_eig_ql         converged symmetric tridiagonal eigensolver (implicit QL with
 //                              Wilkinson shift, textbook), used to state converged values.
 //
-// Pinning (what checks each function; see tests/test_oracle_*.py):
+// Pinning (what checks each function; see tests/test_oracle.py):
 //   geometry / basis / rank / apply / lanczos / extrema / quadrature / ftlm / orbit:
 //       bit-exact against oracle/_ref (the reference sources compiled here) through
 //       tests/golden fixtures; basis against brute force; apply against the published 2x2

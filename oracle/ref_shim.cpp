@@ -1,8 +1,9 @@
 // ref_shim.cpp -- C entry points over the reference's own hot-path sources, compiled in this
 // container from /root/reference (see oracle/Makefile) into oracle/_ref/libftlm_ref.so.
 //
-// TEST INFRASTRUCTURE ONLY: used by tests/golden/make_golden.py to write golden fixtures and by
-// tests/test_oracle_vs_ref.py to pin the oracle restatement.  It calls the reference API exactly
+// TEST INFRASTRUCTURE ONLY: used by tests/golden/make_golden.py and make_nmu_golden.py to write
+// the golden fixtures against which tests/test_oracle.py pins the oracle restatement (and the GPU
+// tests the device path).  It calls the reference API exactly
 // as the reference's drivers do; it contains no numerics of its own except argument marshalling.
 
 #include <complex>
