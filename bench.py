@@ -453,6 +453,45 @@ def graph_workload(F, L, torch):
             "config": {"workload": "sector ladder 4x2 (2,2) .. 4x3 (6,6)"}, "rows": rows}
 
 
+def reorth_workload(F, L, torch):
+    """Plain vs reorthogonalised FTLM Lanczos (FTLM_REORTH_FULL, FTLM_REORTH_PARTIAL): R=16
+    random-phase vectors, M=150 steps, converged Gauss rule, on 4x3 sectors (t=1, t'=-0.25, U=8,
+    V=1, phi_x=0.05), complex64 and complex128 storage.  Wall time of ftlm_ritz per operator
+    (median of 3, after one warm-up) and max |lnZ - lnZ(full, c128)| over beta in [0.1, 20]."""
+    import numpy as np
+
+    betas = np.geomspace(0.1, 20.0, 12)
+    rows = []
+    for lx, ly, nu, nd in [(4, 3, 3, 3), (4, 3, 4, 4), (4, 3, 6, 6)]:
+        p = F.HubbardParams(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.05, 0.0)
+        op = F.HubbardOperator.from_params(p, nu, nd)
+        res, lnz = {}, {}
+        for dname, dt in (("c64", L.FTLM_C64), ("c128", L.FTLM_C128)):
+            for mname, mode in (("plain", L.FTLM_REORTH_NONE), ("full", L.FTLM_REORTH_FULL),
+                                ("partial", L.FTLM_REORTH_PARTIAL)):
+                par = F.FtlmParams(n_random=16, lanczos_steps=150, seed=3, start=L.FTLM_START_PHASE, dtype=dt,
+                                   quad=L.FTLM_QUAD_CONVERGED, path=L.FTLM_PATH_PER_OP, reorth=mode)
+                rd = F.ftlm_ritz([op], par)
+                ts = []
+                for _ in range(3):
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    F.ftlm_ritz([op], par)
+                    ts.append(time.perf_counter() - t0)
+                res[mname, dname] = float(np.median(ts))
+                lnz[mname, dname] = F.ftlm_thermo(rd, betas)["lnZ"][0]
+        ref = lnz["full", "c128"]
+        row = {"sector": f"{lx}x{ly} ({nu},{nd})", "dim": op.dim()}
+        for (m, d), t in res.items():
+            row[f"{m}_{d}_ms"] = round(1e3 * t, 1)
+            row[f"{m}_{d}_dlnZ"] = float(np.max(np.abs(lnz[m, d] - ref)))
+        rows.append(row)
+        log(f"{row}")
+    return {"metric": "FTLM Lanczos with and without reorthogonalisation, R=16, M=150", "unit": "ms",
+            "higher_is_better": False, "n_gpus": 1, "data": "synthetic", "dtype": "c64/c128",
+            "config": {"workload": "4x3 sectors (3,3), (4,4), (6,6)"}, "rows": rows}
+
+
 def traffic_from_profile():
     """HBM bytes per H*V (R=16 complex64) from the committed counter summary: the line
     'hbm_bytes_per_hv = <int>' (FETCH_SIZE + WRITE_SIZE of both passes, copy-calibrated)."""
@@ -542,7 +581,7 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["sector", "kblock", "kblock_all", "gc4x2", "grad", "graph"], default="sector",
+    ap.add_argument("--workload", choices=["sector", "kblock", "kblock_all", "gc4x2", "grad", "graph", "reorth"], default="sector",
                     help="sector: the 4x4 (8,8) full-sector line (default); kblock: the 4x4 (8,8) momentum-block "
                          "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch); grad: "
                          "the differentiated FTLM line (4x3 (6,6) and the 4x4 (8,8) K=(0,0) block)")
@@ -570,6 +609,9 @@ def main():
         return
     if args.workload == "graph":
         print(json.dumps(graph_workload(F, L, torch)), flush=True)
+        return
+    if args.workload == "reorth":
+        print(json.dumps(reorth_workload(F, L, torch)), flush=True)
         return
     if args.workload == "grad":
         print(json.dumps(grad_workload(F, L, torch, validate=not args.no_validate)), flush=True)

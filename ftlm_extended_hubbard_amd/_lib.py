@@ -18,6 +18,7 @@ FTLM_C128, FTLM_C64 = 0, 1
 FTLM_START_GAUSS_MT, FTLM_START_PHASE = 0, 1
 FTLM_QUAD_REFERENCE, FTLM_QUAD_CONVERGED = 0, 1
 FTLM_PATH_AUTO, FTLM_PATH_PER_OP, FTLM_PATH_PERSISTENT, FTLM_PATH_GRAPH = 0, 1, 2, 3
+FTLM_REORTH_NONE, FTLM_REORTH_FULL, FTLM_REORTH_PARTIAL = 0, 1, 2
 FTLM_TS_AUTO, FTLM_TS_GENERIC, FTLM_TS_FUSED = 0, 1, 2
 
 
@@ -42,7 +43,8 @@ class Pair(C.Structure):
 
 class FtlmOpts(C.Structure):
     _fields_ = [("R", C.c_int), ("M", C.c_int), ("seed", C.c_uint), ("start", C.c_int), ("dtype", C.c_int),
-                ("quad", C.c_int), ("clamp_M_to_dim", C.c_int), ("real_variant", C.c_int), ("path", C.c_int)]
+                ("quad", C.c_int), ("clamp_M_to_dim", C.c_int), ("real_variant", C.c_int), ("path", C.c_int),
+                ("reorth", C.c_int)]
 
 
 # C struct layout (ftlm_hop_t is 32 bytes: three ints, 4 bytes of padding, two doubles)

@@ -389,10 +389,11 @@ class FtlmParams:
     quad: int = L.FTLM_QUAD_REFERENCE
     clamp_M_to_dim: bool = False
     path: int = L.FTLM_PATH_AUTO
+    reorth: int = L.FTLM_REORTH_NONE  # FTLM_REORTH_*: the reference never reorthogonalises
 
     def _c(self, real: bool = False) -> L.FtlmOpts:
         return L.FtlmOpts(self.n_random, self.lanczos_steps, self.seed, self.start, self.dtype, self.quad,
-                          int(self.clamp_M_to_dim), int(real), int(self.path))
+                          int(self.clamp_M_to_dim), int(real), int(self.path), int(self.reorth))
 
 
 def _log_partition(op: _Op, betas, par: FtlmParams, real: bool):

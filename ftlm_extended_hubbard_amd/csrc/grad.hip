@@ -828,6 +828,8 @@ void tape_forward(ftlm_tape* T, const ftlm_op* op, ftlm_op* const* dops, int npa
   FTLM_REQUIRE(mode == FTLM_TS_AUTO || mode == FTLM_TS_GENERIC || mode == FTLM_TS_FUSED, FTLM_EINVAL, "bad mode");
   FTLM_REQUIRE(ckpt >= 0, FTLM_EINVAL, "negative checkpoint interval");
   FTLM_REQUIRE(opts->start == FTLM_START_GAUSS_MT || opts->start == FTLM_START_PHASE, FTLM_EINVAL, "bad start");
+  FTLM_REQUIRE(opts->reorth == FTLM_REORTH_NONE, FTLM_EUNSUPPORTED,
+               "derivatives follow the plain (unreorthogonalised) recurrence");
   for (int p = 0; p < npar; ++p)
     FTLM_REQUIRE(dops[p] != nullptr && dops[p]->dim == op->dim, FTLM_EINVAL, "derivative operator of another size");
   FTLM_HIP_CHECK(hipSetDevice(op->device));
@@ -1209,6 +1211,8 @@ int ftlm_hip_ftlm_gradient(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar
     FTLM_REQUIRE(npar >= 0 && npar <= kGMaxPar && (npar == 0 || (dops != nullptr && grad != nullptr)), FTLM_EINVAL,
                  "bad derivative operators");
     FTLM_REQUIRE(opts->start == FTLM_START_GAUSS_MT || opts->start == FTLM_START_PHASE, FTLM_EINVAL, "bad start");
+    FTLM_REQUIRE(opts->reorth == FTLM_REORTH_NONE, FTLM_EUNSUPPORTED,
+                 "derivatives follow the plain (unreorthogonalised) recurrence");
     for (int p = 0; p < npar; ++p)
       FTLM_REQUIRE(dops[p] != nullptr && dops[p]->dim == op->dim, FTLM_EINVAL, "derivative operator of another size");
     FTLM_HIP_CHECK(hipSetDevice(op->device));

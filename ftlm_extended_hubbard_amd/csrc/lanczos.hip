@@ -160,7 +160,8 @@ __global__ __launch_bounds__(kUpdBlock) void k_norm_flat(int64_t n, int R, const
 __global__ __launch_bounds__(kRedBlock) void k_reduce_beta(const double* __restrict__ normp, int nblk, int R,
                                                            int* __restrict__ alive, int* __restrict__ used,
                                                            double* __restrict__ beta, double* __restrict__ sa,
-                                                           double* __restrict__ sb, double* __restrict__ bprev, int k) {
+                                                           double* __restrict__ sb, double* __restrict__ bprev, int k,
+                                                           double* __restrict__ sout) {
   const int r = blockIdx.x;
   double a = 0.0;
   for (int i = threadIdx.x; i < nblk; i += kRedBlock) a += normp[static_cast<int64_t>(i) * R + r];
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_beta(const double* __restr
         bprev[r] = b;
       }
     }
+    if (sout) sout[r] = alive[r] ? sa[r] : 0.0;  // scale of the new Krylov vector (reorthogonalising runs)
   }
 }
 
@@ -237,6 +239,250 @@ __global__ void k_phase_start(int64_t dim, int R, uint64_t seed, const uint32_t*
     }
 }
 
+// ---------------------------------------------------------------- reorthogonalisation
+// With reorthogonalisation every Krylov block stays resident: slot j holds w_j (unnormalised, as
+// the three-buffer recurrence leaves it) and scl[j][r] = 1/beta_{j-1} makes q_j = scl_j * slot_j.
+// A pass computes h_j = <q_j|w> for j <= k (FP64), then w -= sum_j h_j q_j; a second pass, run
+// when the first one cancelled much of w, makes classical Gram-Schmidt as accurate as the modified
+// form (Giraud, Langou, Rozloznik 2005).
+// flags[r] selects the vectors reorthogonalised this step, flags[R + r] forces the next step.
+constexpr int kRoJ = 8;  // Krylov slots whose loads are in flight together
+
+__device__ __forceinline__ bool ro_any(const int* __restrict__ flags, int R) {
+  int any = 0;
+  for (int r = 0; r < R; ++r) any |= flags[r];
+  return any != 0;
+}
+
+// Per-workgroup partials of <slot_j|w> * scl_j for j < nq: part[blk][j][r] (row stride nq_cap).
+template <typename T>
+__global__ __launch_bounds__(kUpdBlock) void k_ro_dots(int64_t n, int R, const typename V2Of<T>::t* __restrict__ C,
+                                                       const typename V2Of<T>::t* __restrict__ slots, int64_t stride,
+                                                       int nq, int nq_cap, const double* __restrict__ scl,
+                                                       const int* __restrict__ flags, double2* __restrict__ part) {
+  using V2 = typename V2Of<T>::t;
+  if (!ro_any(flags, R)) return;
+  const int nthr = blockDim.x, r = threadIdx.x % R;
+  __shared__ double red[2][kUpdBlock];
+  for (int j0 = 0; j0 < nq; j0 += kRoJ) {
+    double ar[kRoJ], ai[kRoJ];
+#pragma unroll
+    for (int jj = 0; jj < kRoJ; ++jj) ar[jj] = ai[jj] = 0.0;
+    for (int64_t f = static_cast<int64_t>(blockIdx.x) * nthr + threadIdx.x; f < n;
+         f += static_cast<int64_t>(gridDim.x) * nthr) {
+      const V2 c = C[f];
+      const double cr = static_cast<double>(c.x), ci = static_cast<double>(c.y);
+      V2 q[kRoJ];
+#pragma unroll
+      for (int jj = 0; jj < kRoJ; ++jj)
+        if (j0 + jj < nq) q[jj] = slots[static_cast<int64_t>(j0 + jj) * stride + f];
+#pragma unroll
+      for (int jj = 0; jj < kRoJ; ++jj)
+        if (j0 + jj < nq) {
+          const double qr = static_cast<double>(q[jj].x), qi = static_cast<double>(q[jj].y);
+          ar[jj] += qr * cr + qi * ci;  // conj(q) * w
+          ai[jj] += qr * ci - qi * cr;
+        }
+    }
+    for (int jj = 0; jj < kRoJ && j0 + jj < nq; ++jj) {
+      red[0][threadIdx.x] = ar[jj];
+      red[1][threadIdx.x] = ai[jj];
+      __syncthreads();
+      if (threadIdx.x < R) {
+        double x = 0.0, y = 0.0;
+        for (int t = threadIdx.x; t < nthr; t += R) {
+          x += red[0][t];
+          y += red[1][t];
+        }
+        const double s = scl[static_cast<int64_t>(j0 + jj) * R + threadIdx.x];
+        part[(static_cast<int64_t>(blockIdx.x) * nq_cap + j0 + jj) * R + threadIdx.x] = make_double2(s * x, s * y);
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// Block (j, r): coef[j][r] = scl_j * <q_j|w> for vectors being reorthogonalised, else 0.
+__global__ __launch_bounds__(kRedBlock) void k_ro_reduce(const double2* __restrict__ part, int nblk, int nq_cap, int R,
+                                                         const double* __restrict__ scl, const int* __restrict__ flags,
+                                                         double2* __restrict__ coef) {
+  if (!ro_any(flags, R)) return;
+  const int j = blockIdx.x, r = blockIdx.y;
+  double a = 0.0, b = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock) {
+    const double2 v = part[(static_cast<int64_t>(i) * nq_cap + j) * R + r];
+    a += v.x;
+    b += v.y;
+  }
+  __shared__ double sh[2][kRedBlock / 64];
+  a = wsum(a);
+  b = wsum(b);
+  if ((threadIdx.x & 63) == 0) {
+    sh[0][threadIdx.x >> 6] = a;
+    sh[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double x = 0.0, y = 0.0;
+    for (int w = 0; w < kRedBlock / 64; ++w) {
+      x += sh[0][w];
+      y += sh[1][w];
+    }
+    const double s = flags[r] ? scl[static_cast<int64_t>(j) * R + r] : 0.0;
+    coef[static_cast<int64_t>(j) * R + r] = make_double2(s * x, s * y);
+  }
+}
+
+// w -= sum_{j<nq} coef_j slot_j for flagged vectors, then |w|^2 partials of every vector in the
+// update kernel's layout (same grid, block and element order, so unchanged vectors reproduce the
+// update's partials bit for bit).
+template <typename T>
+__global__ __launch_bounds__(kUpdBlock) void k_ro_sub(int64_t n, int R, typename V2Of<T>::t* __restrict__ C,
+                                                      const typename V2Of<T>::t* __restrict__ slots, int64_t stride,
+                                                      int nq, const double2* __restrict__ coef,
+                                                      const int* __restrict__ flags, double* __restrict__ normp) {
+  using V2 = typename V2Of<T>::t;
+  if (!ro_any(flags, R)) return;
+  const int nthr = blockDim.x, r = threadIdx.x % R;
+  const bool on = flags[r] != 0;
+  double nrm = 0.0;
+  for (int64_t f = static_cast<int64_t>(blockIdx.x) * nthr + threadIdx.x; f < n;
+       f += static_cast<int64_t>(gridDim.x) * nthr) {
+    V2 c = C[f];
+    if (on) {
+      double wr = static_cast<double>(c.x), wi = static_cast<double>(c.y);
+      for (int j0 = 0; j0 < nq; j0 += kRoJ) {
+        V2 q[kRoJ];
+#pragma unroll
+        for (int jj = 0; jj < kRoJ; ++jj)
+          if (j0 + jj < nq) q[jj] = slots[static_cast<int64_t>(j0 + jj) * stride + f];
+#pragma unroll
+        for (int jj = 0; jj < kRoJ; ++jj)
+          if (j0 + jj < nq) {
+            const double2 h = coef[static_cast<int64_t>(j0 + jj) * R + r];
+            const double qr = static_cast<double>(q[jj].x), qi = static_cast<double>(q[jj].y);
+            wr -= h.x * qr - h.y * qi;
+            wi -= h.x * qi + h.y * qr;
+          }
+      }
+      c.x = static_cast<T>(wr);
+      c.y = static_cast<T>(wi);
+      C[f] = c;
+    }
+    const double sr = static_cast<double>(c.x), si = static_cast<double>(c.y);
+    nrm += sr * sr + si * si;
+  }
+  __shared__ double red[kUpdBlock];
+  red[threadIdx.x] = nrm;
+  __syncthreads();
+  if (threadIdx.x < R) {
+    double s = 0.0;
+    for (int t = threadIdx.x; t < nthr; t += R) s += red[t];
+    normp[static_cast<int64_t>(blockIdx.x) * R + threadIdx.x] = s;
+  }
+}
+
+// Decides, per vector, whether step k reorthogonalises w_{k+1}.  FULL: every live vector.
+// PARTIAL (H. D. Simon, Math. Comp. 42 (1984) 115-142): with beta_k = |w| before
+// reorthogonalisation, the estimates omega_{k+1,j} of |<q_{k+1}|q_j>| follow
+//   beta_k w_{k+1,j} = beta_j w_{k,j+1} + (alpha_j - alpha_k) w_{k,j} + beta_{j-1} w_{k,j-1}
+//                      - beta_{k-1} w_{k-1,j} + sign * eps1 (beta_k + beta_j),
+// with w_{k+1,k} = eps1 = eps sqrt(dim) and w_{k+1,k+1} = 1.  A step whose largest estimate
+// exceeds sqrt(eps) is reorthogonalised, and so is the one after; the estimates of a
+// reorthogonalised vector restart at eps1.  Rows live in om[3][R][ld], row (k mod 3) = omega_k.
+__global__ __launch_bounds__(kRedBlock) void k_ro_omega(int k, int R, int mode, double eps, double eps1,
+                                                        const double* __restrict__ alpha,
+                                                        const double* __restrict__ beta,
+                                                        const double* __restrict__ normp, int nblk,
+                                                        const int* __restrict__ alive, double* __restrict__ om,
+                                                        int ld, int* __restrict__ flags, double* __restrict__ nw0) {
+  const int r = blockIdx.x;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock) a += normp[static_cast<int64_t>(i) * R + r];
+  __shared__ double sh[kRedBlock / 64];
+  __shared__ int sflag;
+  a = wsum(a);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+  __syncthreads();
+  double nw2 = 0.0;
+  for (int w = 0; w < kRedBlock / 64; ++w) nw2 += sh[w];
+  const bool live = alive[r] != 0 && nw2 >= 1e-24;
+  if (threadIdx.x == 0) nw0[r] = nw2;
+  if (mode == FTLM_REORTH_FULL || !live) {
+    if (threadIdx.x == 0) {
+      flags[r] = live ? 1 : 0;
+      flags[R + r] = 0;
+    }
+    return;
+  }
+  const double bk = sqrt(nw2);
+  const double* wk = om + (static_cast<int64_t>(k % 3) * R + r) * ld;
+  const double* wkm = om + (static_cast<int64_t>((k + 2) % 3) * R + r) * ld;
+  double* wn = om + (static_cast<int64_t>((k + 1) % 3) * R + r) * ld;
+  const double ak = alpha[static_cast<int64_t>(k) * R + r];
+  const double bkm = k > 0 ? beta[static_cast<int64_t>(k - 1) * R + r] : 0.0;
+  double mx = 0.0;
+  for (int j = threadIdx.x; j < k; j += kRedBlock) {
+    const double bj = beta[static_cast<int64_t>(j) * R + r];
+    double v = bj * wk[j + 1] + (alpha[static_cast<int64_t>(j) * R + r] - ak) * wk[j] - bkm * wkm[j];
+    if (j > 0) v += beta[static_cast<int64_t>(j - 1) * R + r] * wk[j - 1];
+    v = (v + copysign(eps1 * (bk + bj), v)) / bk;
+    wn[j] = v;
+    mx = fmax(mx, fabs(v));
+  }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int w = 0; w < kRedBlock / 64; ++w) m = fmax(m, sh[w]);
+    const int trig = m > sqrt(eps) ? 1 : 0;
+    const int on = trig | flags[R + r];
+    flags[r] = on;
+    flags[R + r] = trig;
+    sflag = on;
+    wn[k] = eps1;
+    wn[k + 1] = 1.0;
+  }
+  __syncthreads();
+  if (sflag)
+    for (int j = threadIdx.x; j < k; j += kRedBlock) wn[j] = eps1;
+}
+
+// After the first Gram-Schmidt pass: keep a vector for the second pass only when the pass cut
+// its norm below 1/sqrt(2) of what it was (the criterion of Daniel, Gragg, Kaufman and Stewart,
+// Math. Comp. 30 (1976) 772-795); otherwise one pass already left it orthogonal to rounding.
+__global__ __launch_bounds__(kRedBlock) void k_ro_second(const double* __restrict__ normp, int nblk, int R,
+                                                         const double* __restrict__ nw0, int* __restrict__ flags) {
+  const int r = blockIdx.x;
+  if (!flags[r]) return;
+  double a = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += kRedBlock) a += normp[static_cast<int64_t>(i) * R + r];
+  __shared__ double sh[kRedBlock / 64];
+  a = wsum(a);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double nw1 = 0.0;
+    for (int w = 0; w < kRedBlock / 64; ++w) nw1 += sh[w];
+    flags[r] = nw1 < 0.5 * nw0[r] ? 1 : 0;
+  }
+}
+
+// omega_0 = (1), flags cleared: the start of a reorthogonalising recurrence.
+__global__ void k_ro_init(int R, double* __restrict__ om, int64_t om_n, int ld, int* __restrict__ flags) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < om_n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    om[i] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x < 2 * R) flags[threadIdx.x] = 0;
+}
+
+__global__ void k_ro_row0(int R, double* __restrict__ om, int ld) {
+  if (threadIdx.x < R) om[static_cast<int64_t>(threadIdx.x) * ld] = 1.0;
+}
+
 // ---------------------------------------------------------------- host engine
 
 struct Krylov {
@@ -251,15 +497,34 @@ struct Krylov {
   DevBuf<int> alive, used;
   int nblk_hv = 0, nblk_upd = 0;
   bool force_real = false;
+  // reorthogonalisation (FTLM_REORTH_*): every Krylov block in `basis` (slot j = w_j), per-slot
+  // scales scl[M+1][R], overlap partials/coefficients, omega rows and the step flags
+  int reorth = FTLM_REORTH_NONE;
+  int nblk_dot = 0, om_ld = 0;
+  DevBuf<unsigned char> basis;
+  DevBuf<double> scl, om, rnw0;
+  DevBuf<double2> rpart, rcoef;
+  DevBuf<int> rflags;
 
-  void init(const ftlm_op* o, int R_, int M_, int dt) {
+  void init(const ftlm_op* o, int R_, int M_, int dt, int reorth_ = FTLM_REORTH_NONE) {
     op = o;
     R = R_;
     M = M_;
     dtype = dt;
     dim = o->dim;
+    reorth = reorth_;
     vbytes = static_cast<size_t>(dim) * R * (dt == FTLM_C64 ? sizeof(float2) : sizeof(double2));
-    for (auto& b : buf) b.alloc(vbytes);
+    if (reorth == FTLM_REORTH_NONE) {
+      for (auto& b : buf) b.alloc(vbytes);
+    } else {
+      basis.alloc(static_cast<size_t>(M + 1) * vbytes);
+      scl.alloc(static_cast<size_t>(M + 1) * R);
+      rcoef.alloc(static_cast<size_t>(M + 1) * R);
+      rflags.alloc(static_cast<size_t>(2) * R);
+      rnw0.alloc(static_cast<size_t>(R));
+      om_ld = M + 2;
+      if (reorth == FTLM_REORTH_PARTIAL) om.alloc(static_cast<size_t>(3) * R * om_ld);
+    }
     sa.alloc(R);
     sb.alloc(R);
     bprev.alloc(R);
@@ -272,7 +537,14 @@ struct Krylov {
     nblk_upd = static_cast<int>(std::min<int64_t>(kUpdMaxGrid, (dim * R + upd_threads() - 1) / upd_threads()));
     dotp.alloc(static_cast<size_t>(nblk_hv) * R);
     normp.alloc(static_cast<size_t>(std::max(nblk_hv, nblk_upd)) * R);
+    if (reorth != FTLM_REORTH_NONE) {
+      nblk_dot = std::min(nblk_upd, 1024);
+      rpart.alloc(static_cast<size_t>(nblk_dot) * (M + 1) * R);
+    }
   }
+
+  // Krylov block j of a reorthogonalising run; the start block otherwise.
+  void* slot(int j) const { return reorth == FTLM_REORTH_NONE ? static_cast<void*>(buf[0].p) : basis.p + j * vbytes; }
 
   // block size: the largest multiple of R not above kUpdBlock (each thread keeps one vector r)
   int upd_threads() const { return (kUpdBlock / R) * R; }
@@ -291,7 +563,7 @@ struct Krylov {
     check_launch("k_update_flat");
   }
 
-  // Prepare the start block in buf[0]: normalise lazily and reset the recurrence state.
+  // Prepare the start block (buf[0], or slot 0): normalise lazily and reset the recurrence state.
   void start(hipStream_t st) {
     std::vector<int> one(static_cast<size_t>(R), 1), zero(static_cast<size_t>(R), 0);
     std::vector<double> dz(static_cast<size_t>(R), 0.0);
@@ -303,18 +575,19 @@ struct Krylov {
     FTLM_HIP_CHECK(hipMemsetAsync(beta.p, 0, beta.n * sizeof(double), st));
     const dim3 g(nblk_upd), b(upd_threads());
     if (dtype == FTLM_C64)
-      hipLaunchKernelGGL(k_norm_flat<float>, g, b, 0, st, dim * R, R, static_cast<const float2*>((void*)buf[0].p),
+      hipLaunchKernelGGL(k_norm_flat<float>, g, b, 0, st, dim * R, R, static_cast<const float2*>((void*)slot(0)),
                          normp.p);
     else
-      hipLaunchKernelGGL(k_norm_flat<double>, g, b, 0, st, dim * R, R, static_cast<const double2*>((void*)buf[0].p),
+      hipLaunchKernelGGL(k_norm_flat<double>, g, b, 0, st, dim * R, R, static_cast<const double2*>((void*)slot(0)),
                          normp.p);
     hipLaunchKernelGGL(k_start_norm, dim3(R), dim3(kRedBlock), 0, st, normp.p, nblk_upd, R, sa.p);
     if (dtype == FTLM_C64)
-      hipLaunchKernelGGL(k_start_fallback<float>, dim3(1), dim3(64), 0, st, static_cast<float2*>((void*)buf[0].p), R, sa.p);
+      hipLaunchKernelGGL(k_start_fallback<float>, dim3(1), dim3(64), 0, st, static_cast<float2*>((void*)slot(0)), R, sa.p);
     else
-      hipLaunchKernelGGL(k_start_fallback<double>, dim3(1), dim3(64), 0, st, static_cast<double2*>((void*)buf[0].p), R,
+      hipLaunchKernelGGL(k_start_fallback<double>, dim3(1), dim3(64), 0, st, static_cast<double2*>((void*)slot(0)), R,
                          sa.p);
     check_launch("start");
+    start_reorth(st);
   }
 
   // Host-computed start norms (bit-identical to the reference's sequential sum).
@@ -328,6 +601,7 @@ struct Krylov {
     sa.upload(inv_norm, st);
     FTLM_HIP_CHECK(hipMemsetAsync(alpha.p, 0, alpha.n * sizeof(double), st));
     FTLM_HIP_CHECK(hipMemsetAsync(beta.p, 0, beta.n * sizeof(double), st));
+    start_reorth(st);
   }
 
   // One Lanczos step k with (A, B, C) = buffers (a, b, c).
@@ -336,13 +610,67 @@ struct Krylov {
     hipLaunchKernelGGL(k_reduce_alpha, dim3(R), dim3(kRedBlock), 0, st, dotp.p, nb, R, alive.p, z.p, alpha.p, k);
     launch_update(buf[c].p, buf[a].p, buf[b].p, k > 0 ? 1 : 0, st);
     hipLaunchKernelGGL(k_reduce_beta, dim3(R), dim3(kRedBlock), 0, st, normp.p, nblk_upd, R, alive.p, used.p, beta.p,
-                       sa.p, sb.p, bprev.p, k);
+                       sa.p, sb.p, bprev.p, k, static_cast<double*>(nullptr));
     check_launch("lanczos step");
+  }
+
+  // Clears the reorthogonalisation state after the start block (slot 0, scale sa) is in place.
+  void start_reorth(hipStream_t st) {
+    if (reorth == FTLM_REORTH_NONE) return;
+    FTLM_HIP_CHECK(hipMemcpyAsync(scl.p, sa.p, sizeof(double) * R, hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_ro_init, dim3(om.n ? static_cast<unsigned>(std::min<size_t>(1024, (om.n + 255) / 256)) : 1),
+                       dim3(256), 0, st, R, om.p, static_cast<int64_t>(om.n), om_ld, rflags.p);
+    if (om.n) hipLaunchKernelGGL(k_ro_row0, dim3(1), dim3(64), 0, st, R, om.p, om_ld);
+    check_launch("reorthogonalisation start");
+  }
+
+  // One step of a reorthogonalising run: A = slot k, B = slot k-1, C = slot k+1.
+  void step_reorth(int k, hipStream_t st) {
+    void* A = slot(k);
+    void* B = slot(k > 0 ? k - 1 : k);
+    void* C = slot(k + 1);
+    const int nb = hv_apply(op, A, C, R, dtype, sa.p, dotp.p, st, force_real);
+    hipLaunchKernelGGL(k_reduce_alpha, dim3(R), dim3(kRedBlock), 0, st, dotp.p, nb, R, alive.p, z.p, alpha.p, k);
+    launch_update(C, A, B, k > 0 ? 1 : 0, st);
+    const double eps = dtype == FTLM_C64 ? 0x1.0p-24 : 0x1.0p-53;
+    const double eps1 = eps * std::sqrt(static_cast<double>(dim));
+    hipLaunchKernelGGL(k_ro_omega, dim3(R), dim3(kRedBlock), 0, st, k, R, reorth, eps, eps1, alpha.p, beta.p, normp.p,
+                       nblk_upd, alive.p, om.p, om_ld, rflags.p, rnw0.p);
+    const int64_t n = dim * R;
+    const int nq = k + 1;
+    for (int pass = 0; pass < 2; ++pass) {
+      if (dtype == FTLM_C64)
+        hipLaunchKernelGGL(k_ro_dots<float>, dim3(nblk_dot), dim3(upd_threads()), 0, st, n, R,
+                           static_cast<const float2*>(C), static_cast<const float2*>((void*)basis.p), n, nq, M + 1,
+                           scl.p, rflags.p, rpart.p);
+      else
+        hipLaunchKernelGGL(k_ro_dots<double>, dim3(nblk_dot), dim3(upd_threads()), 0, st, n, R,
+                           static_cast<const double2*>(C), static_cast<const double2*>((void*)basis.p), n, nq, M + 1,
+                           scl.p, rflags.p, rpart.p);
+      hipLaunchKernelGGL(k_ro_reduce, dim3(nq, R), dim3(kRedBlock), 0, st, rpart.p, nblk_dot, M + 1, R, scl.p,
+                         rflags.p, rcoef.p);
+      if (dtype == FTLM_C64)
+        hipLaunchKernelGGL(k_ro_sub<float>, dim3(nblk_upd), dim3(upd_threads()), 0, st, n, R, static_cast<float2*>(C),
+                           static_cast<const float2*>((void*)basis.p), n, nq, rcoef.p, rflags.p, normp.p);
+      else
+        hipLaunchKernelGGL(k_ro_sub<double>, dim3(nblk_upd), dim3(upd_threads()), 0, st, n, R,
+                           static_cast<double2*>(C), static_cast<const double2*>((void*)basis.p), n, nq, rcoef.p,
+                           rflags.p, normp.p);
+      if (pass == 0)
+        hipLaunchKernelGGL(k_ro_second, dim3(R), dim3(kRedBlock), 0, st, normp.p, nblk_upd, R, rnw0.p, rflags.p);
+    }
+    hipLaunchKernelGGL(k_reduce_beta, dim3(R), dim3(kRedBlock), 0, st, normp.p, nblk_upd, R, alive.p, used.p, beta.p,
+                       sa.p, sb.p, bprev.p, k, scl.p + static_cast<int64_t>(k + 1) * R);
+    check_launch("reorthogonalising lanczos step");
   }
 
   // Steps [k0, k1); buffer roles rotate (A,B,C) -> (C,A,B) every step, starting from A = buf[0].
   void run(int k0, int k1, hipStream_t st) {
     for (int k = k0; k < k1; ++k) {
+      if (reorth != FTLM_REORTH_NONE) {
+        step_reorth(k, st);
+        continue;
+      }
       const int a = (3 - k % 3) % 3;  // k=0 -> 0, k=1 -> 2, k=2 -> 1
       const int b = (a + 1) % 3, c = (a + 2) % 3;
       step(k, a, b, c, st);
@@ -411,7 +739,7 @@ double host_norm(const std::vector<std::complex<double>>& q) {
   return std::sqrt(s);
 }
 
-// Fill buf[0] with R start vectors.  seeds[r] per sample.
+// Fill the start block with R start vectors.  seeds[r] per sample.
 void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const double* q0, hipStream_t st) {
   const int R = K.R;
   const int64_t dim = K.dim;
@@ -423,10 +751,10 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
     const unsigned g = static_cast<unsigned>(std::min<int64_t>(8192, (dim + 255) / 256));
     if (K.dtype == FTLM_C64)
       hipLaunchKernelGGL(k_phase_start<float>, dim3(g), dim3(256), 0, st, dim, R, 0x5eedull, dids.p,
-                         static_cast<float2*>((void*)K.buf[0].p));
+                         static_cast<float2*>(K.slot(0)));
     else
       hipLaunchKernelGGL(k_phase_start<double>, dim3(g), dim3(256), 0, st, dim, R, 0x5eedull, dids.p,
-                         static_cast<double2*>((void*)K.buf[0].p));
+                         static_cast<double2*>(K.slot(0)));
     check_launch("k_phase_start");
     K.start(st);
     FTLM_HIP_CHECK(hipStreamSynchronize(st));
@@ -464,7 +792,7 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
     }
   }
   (void)fallback;
-  FTLM_HIP_CHECK(hipMemcpyAsync(K.buf[0].p, host.data(), host.size(), hipMemcpyHostToDevice, st));
+  FTLM_HIP_CHECK(hipMemcpyAsync(K.slot(0), host.data(), host.size(), hipMemcpyHostToDevice, st));
   if (K.dtype == FTLM_C64) {
     // complex64 storage: recompute the norm of the stored (rounded) vectors on the device
     K.start(st);
@@ -479,6 +807,7 @@ void check_opts(const ftlm_ftlm_opts_t* o) {
   FTLM_REQUIRE(o->dtype == FTLM_C128 || o->dtype == FTLM_C64, FTLM_EINVAL, "bad dtype");
   FTLM_REQUIRE(o->start == FTLM_START_GAUSS_MT || o->start == FTLM_START_PHASE, FTLM_EINVAL, "bad start");
   FTLM_REQUIRE(o->quad == FTLM_QUAD_REFERENCE || o->quad == FTLM_QUAD_CONVERGED, FTLM_EINVAL, "bad quad");
+  FTLM_REQUIRE(o->reorth >= FTLM_REORTH_NONE && o->reorth <= FTLM_REORTH_PARTIAL, FTLM_EINVAL, "bad reorth");
 }
 
 
@@ -595,8 +924,13 @@ void ritz_persistent(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* o
   }
 }
 
-bool persistent_eligible(ftlm_op_t* const* ops, int n_ops, int path) {
+bool persistent_eligible(ftlm_op_t* const* ops, int n_ops, int path, int reorth) {
   if (path == FTLM_PATH_PER_OP || path == FTLM_PATH_GRAPH) return false;
+  if (reorth != FTLM_REORTH_NONE) {
+    FTLM_REQUIRE(path != FTLM_PATH_PERSISTENT, FTLM_EUNSUPPORTED,
+                 "the persistent path keeps three vectors per recurrence and cannot reorthogonalise");
+    return false;
+  }
   const int64_t lim = path == FTLM_PATH_PERSISTENT ? 262144 : 16384;
   for (int i = 0; i < n_ops; ++i) {
     if (ops[i] == nullptr || ops[i]->kind != kSector || ops[i]->dim > lim) {
@@ -725,7 +1059,7 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
     FTLM_REQUIRE(opts->path >= FTLM_PATH_AUTO && opts->path <= FTLM_PATH_GRAPH, FTLM_EINVAL, "bad path");
     const int R = std::max(1, opts->R), M = std::max(1, opts->M);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (persistent_eligible(ops, n_ops, opts->path)) {
+    if (persistent_eligible(ops, n_ops, opts->path, opts->reorth)) {
       FTLM_HIP_CHECK(hipSetDevice(ops[0]->device));
       ritz_persistent(ops, n_ops, opts, ritz, weight, used, alpha_out, beta_out, st);
       return;
@@ -767,7 +1101,7 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
       for (int g0 = 0; g0 < R; g0 += kMaxSamplesPerBlock) {
         const int Rg = std::min(kMaxSamplesPerBlock, R - g0);
         Krylov K;
-        K.init(op, Rg, Mop, opts->dtype);
+        K.init(op, Rg, Mop, opts->dtype, opts->reorth);
         K.force_real = opts->real_variant != 0;
         std::vector<unsigned> seeds(static_cast<size_t>(Rg));
         for (int r = 0; r < Rg; ++r) seeds[r] = opts->seed + static_cast<unsigned>(g0 + r) * 100003u;

@@ -159,7 +159,22 @@ typedef struct ftlm_ftlm_opts {
   int real_variant;   /* 1: ftlm_log_partition_real semantics (Re/Im applied separately,
                          src/ftlm_thermo.cpp:150-211) -- identical for real H */
   int path;           /* FTLM_PATH_*: how the recurrences of several operators are scheduled */
+  int reorth;         /* FTLM_REORTH_*; the reference never reorthogonalises (src/lanczos.cpp:180-206) */
 } ftlm_ftlm_opts_t;
+
+/* Reorthogonalisation of the Lanczos vectors (no counterpart in the reference).  Every mode other
+ * than NONE keeps all M+1 Krylov blocks resident in device memory ((M+1) * dim * R * 16 bytes at
+ * complex128, 8 at complex64) and runs on the per-operator paths (AUTO never picks the persistent
+ * launch for it).  Overlaps and norms are FP64 in every dtype.
+ *   FULL    classical Gram-Schmidt against every earlier vector, each step, with a second pass
+ *           wherever the first one cut the norm below 1/sqrt(2) of its value (DGKS criterion);
+ *   PARTIAL Simon's omega recurrence estimates |<q_j|q_k+1>| from alpha/beta on the device and
+ *           reorthogonalises (as FULL, against every earlier vector) only at steps where the
+ *           estimate exceeds sqrt(eps) and at the step after; eps is the storage precision's unit
+ *           roundoff, so the basis stays semi-orthogonal. */
+#define FTLM_REORTH_NONE 0
+#define FTLM_REORTH_FULL 1
+#define FTLM_REORTH_PARTIAL 2
 
 #define FTLM_PATH_AUTO 0       /* persistent launch when every operator is a sector of dim <= 16384 */
 #define FTLM_PATH_PER_OP 1     /* per operator, R recurrences per block H*V */

@@ -73,6 +73,17 @@ def test_bad_options_rejected_without_device():
     assert rc == L.FTLM_EINVAL
 
 
+def test_bad_reorth_rejected_without_device():
+    opts = L.FtlmOpts(4, 10, 1, 0, 0, 0, 0, 0, 0, 3)  # reorth = 3 is no FTLM_REORTH_* mode
+    rz = np.zeros(40)
+    us = np.zeros(4, np.int32)
+    arr = (C.c_void_p * 1)(None)
+    rc = L.lib().ftlm_hip_ftlm_ritz(arr, 1, C.byref(opts), rz.ctypes.data, rz.ctypes.data, us.ctypes.data, None,
+                                    None, None)
+    assert rc == L.FTLM_EINVAL
+    assert b"reorth" in L.lib().ftlm_hip_last_error()
+
+
 def test_tridiagonal_extrema_empty_is_zero():
     lo, hi = api.tridiagonal_extrema(np.zeros(0), np.zeros(0), 0)
     assert (lo, hi) == (0.0, 0.0)
