@@ -64,10 +64,14 @@ def kernel_roofline(F, L, op, R, dtype, torch, reps=5):
     return sum(rows) / reps, sum(cols) / reps, R * dim * 2 * s
 
 
-def profile_hv(F, L, torch, R):
+def profile_hv(F, L, torch, R, shape=None):
     """Workload of the committed rocprofv3 summaries: 3 H*V launches (R complex64 vectors) on the
-    4x4 (8,8) sector, and one copy of 4 GiB (the FETCH_SIZE / WRITE_SIZE calibration kernel)."""
+    4x4 (8,8) sector, and one copy of 4 GiB (the FETCH_SIZE / WRITE_SIZE calibration kernel).
+    `shape` (rb, bpx, wlog, grid) sets the launch shape, e.g. the one the bench line's autotuner
+    picked."""
     op = F.HubbardOperator.from_params(F.HubbardParams(**MODEL), NU, ND)
+    if shape is not None:
+        op.set_shape(R, L.FTLM_C64, *shape)
     dim = op.dim()
     X = torch.randn(dim * R, dtype=torch.complex64, device="cuda")
     Y = torch.empty_like(X)
@@ -111,9 +115,15 @@ def kblock_workload(F, L, torch, steps, validate=True):
     out["n_orbits"], out["nnz"], out["dims"] = info["n_orbits"], info["nnz"], dims
     out["sum_dims_equals_dim"] = bool(sum(dims) == 12870 * 12870)
     log(f"block dims {min(dims)}..{max(dims)}, sum {sum(dims)} (sector {12870 * 12870})")
-    # apply timings on the largest block
+    # apply timings on the largest block, with launch shapes autotuned for each (R, dtype) used
     B = blocks[int(np.argmax(dims))]
     d = B.dim()
+    t0 = time.perf_counter()
+    out["autotune"] = {f"R{R}_{n}": B.autotune(R, dt) for R, dt, n in
+                       ((1, L.FTLM_C128, "c128"), (8, L.FTLM_C128, "c128"), (16, L.FTLM_C64, "c64"),
+                        (32, L.FTLM_C64, "c64"))}
+    out["autotune_s"] = round(time.perf_counter() - t0, 2)
+    log(f"autotune: {out['autotune']}")
     ap = {}
     for R, dt, ct, s in [(1, L.FTLM_C128, torch.complex128, 16), (8, L.FTLM_C128, torch.complex128, 16),
                          (16, L.FTLM_C64, torch.complex64, 8)]:
@@ -329,6 +339,10 @@ def grad_workload(F, L, torch, validate=True):
     dB = [F.derivative_operator(pk, NU, ND, q, K) for q in kn]
     torch.cuda.synchronize()
     t_build = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tuned = B.autotune(32, L.FTLM_C128)
+    out["kblock_autotune"] = dict(tuned, seconds=round(time.perf_counter() - t0, 2))
+    log(f"K=(0,0) block launch shape (R=32, c128): {tuned}")
     pk_par = F.FtlmParams(n_random=32, lanczos_steps=100, seed=5, start=L.FTLM_START_PHASE,
                           quad=L.FTLM_QUAD_CONVERGED)
     t0 = time.perf_counter()
@@ -363,6 +377,17 @@ def kblock_all_workload(F, L, torch):
     out = {"metric": "4x4 (8,8) all 16 momentum blocks FTLM, R=32, M=120", "unit": "s", "higher_is_better": False,
            "n_gpus": 1, "data": "synthetic", "config": {"workload": "4x4 (8,8), 16 K blocks", "R": 32, "M": 120,
                                                         "betas": betas.tolist()}, "dims": dims.tolist()}
+    # one launch-shape tuning per dtype on the largest block, shared by all 16 (their sizes and hop
+    # counts are alike); its time is reported apart from the FTLM times
+    big = blocks[int(np.argmax(dims))]
+    t0 = time.perf_counter()
+    out["autotune"] = {n: big.autotune(32, dt) for n, dt in (("c64", L.FTLM_C64), ("c128", L.FTLM_C128))}
+    for n, dt in (("c64", L.FTLM_C64), ("c128", L.FTLM_C128)):
+        sh = {k: out["autotune"][n][k] for k in ("rb", "bpx", "wlog", "grid")}
+        for b in blocks:
+            b.set_shape(32, dt, **sh)
+    out["autotune_s"] = round(time.perf_counter() - t0, 2)
+    log(f"autotune (largest block, shared): {out['autotune']}")
 
     def lnz_stats(rd, R):
         # per-sample L_r(beta) = ln sum_k w e^{-beta lambda}; Z_op = dim mean_r e^{L_r}
@@ -492,6 +517,36 @@ def reorth_workload(F, L, torch):
             "config": {"workload": "4x3 sectors (3,3), (4,4), (6,6)"}, "rows": rows}
 
 
+def tune_workload(F, L, torch):
+    """H*V launch-shape autotuner over a ladder of operators (t=1, t'=-0.25, U=8, V=1; phi_x=0.05
+    for sectors, 0 for momentum blocks): default-shape and tuned-shape time of one apply (median
+    of 5), for R=16 complex64, R=12 complex64 and R=1 complex128."""
+    cases = [("sector", 4, 3, 3, 3), ("sector", 4, 3, 4, 4), ("sector", 4, 3, 6, 6), ("sector", 4, 4, 6, 6),
+             ("sector", 4, 4, 8, 8), ("kblock", 4, 4, 8, 8), ("kblock", 4, 4, 6, 6)]
+    rows = []
+    for kind, lx, ly, nu, nd in cases:
+        if kind == "sector":
+            op = F.HubbardOperator.from_params(F.HubbardParams(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.05, 0.0), nu, nd)
+        else:
+            op = F.KBlockOperator.from_params(F.HubbardParams(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0), nu, nd,
+                                              F.MomentumSector(1, 0, lx, ly))
+        for R, dt, dname in ((16, L.FTLM_C64, "c64"), (12, L.FTLM_C64, "c64"), (1, L.FTLM_C128, "c128")):
+            t0 = time.perf_counter()
+            res = op.autotune(R, dt, reps=5)
+            row = {"op": f"{kind} {lx}x{ly} ({nu},{nd})", "dim": op.dim(), "R": R, "dtype": dname,
+                   "default_ms": round(res["ms_default"], 3), "tuned_ms": round(res["ms"], 3),
+                   "speedup": round(res["ms_default"] / res["ms"], 3),
+                   "shape": {k: res[k] for k in ("rb", "bpx", "wlog", "grid")}, "n_tried": res["n_tried"],
+                   "tune_s": round(time.perf_counter() - t0, 2)}
+            rows.append(row)
+            log(f"{row}")
+        del op
+        torch.cuda.empty_cache()
+    return {"metric": "H*V apply time, default vs autotuned launch shape", "unit": "ms", "higher_is_better": False,
+            "n_gpus": 1, "data": "synthetic", "dtype": "c64/c128",
+            "config": {"workload": "4x3 and 4x4 sectors, 4x4 K=(1,0) momentum blocks"}, "rows": rows}
+
+
 def traffic_from_profile():
     """HBM bytes per H*V (R=16 complex64) from the committed counter summary: the line
     'hbm_bytes_per_hv = <int>' (FETCH_SIZE + WRITE_SIZE of both passes, copy-calibrated)."""
@@ -581,12 +636,16 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["sector", "kblock", "kblock_all", "gc4x2", "grad", "graph", "reorth"], default="sector",
+    ap.add_argument("--workload", choices=["sector", "kblock", "kblock_all", "gc4x2", "grad", "graph", "reorth", "tune"], default="sector",
                     help="sector: the 4x4 (8,8) full-sector line (default); kblock: the 4x4 (8,8) momentum-block "
                          "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch); grad: "
                          "the differentiated FTLM line (4x3 (6,6) and the 4x4 (8,8) K=(0,0) block)")
     ap.add_argument("--profile-hv", action="store_true",
                     help="only run 3 block H*V (R vectors, complex64) and one 4 GiB device copy, for rocprofv3")
+    ap.add_argument("--hv-shape", type=str, default=None,
+                    help="rb,bpx,wlog,grid launch shape for --profile-hv (as reported under 'autotune')")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the library's default H*V launch shapes instead of autotuning them first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -602,7 +661,8 @@ def main():
     assert torch.cuda.is_available(), "no GPU"
     K, W, R = args.steps, args.warmup, args.R
     if args.profile_hv:
-        profile_hv(F, L, torch, R)
+        shape = None if args.hv_shape is None else tuple(int(v) for v in args.hv_shape.split(","))
+        profile_hv(F, L, torch, R, shape)
         return
     if args.workload == "kblock_all":
         print(json.dumps(kblock_all_workload(F, L, torch)), flush=True)
@@ -612,6 +672,9 @@ def main():
         return
     if args.workload == "reorth":
         print(json.dumps(reorth_workload(F, L, torch)), flush=True)
+        return
+    if args.workload == "tune":
+        print(json.dumps(tune_workload(F, L, torch)), flush=True)
         return
     if args.workload == "grad":
         print(json.dumps(grad_workload(F, L, torch, validate=not args.no_validate)), flush=True)
@@ -628,6 +691,16 @@ def main():
     t_setup = time.perf_counter() - t0
     dim = op.dim()
     log(f"sector built: dim={dim} in {t_setup:.2f} s")
+
+    # ---- launch shapes of the H*V kernels, timed on scratch blocks before the timed region
+    tuned = None
+    if not args.no_autotune:
+        t1 = time.perf_counter()
+        tuned = {f"c64_R{R}": op.autotune(R, L.FTLM_C64), "c128_R1": op.autotune(1, L.FTLM_C128)}
+        for v in tuned.values():
+            v.pop("cached", None)
+        tuned["seconds"] = round(time.perf_counter() - t1, 1)
+        log(f"autotune: {tuned}")
 
     # ---- batched complex64, R vectors
     sess = F.KrylovSession(op, R=R, max_steps=W + K, dtype=L.FTLM_C64, start=L.FTLM_START_PHASE, seed=1)
@@ -661,7 +734,7 @@ def main():
     tr1, tc1, B1 = kernel_roofline(F, L, op, 1, L.FTLM_C128, torch)
     log(f"H*V c128 R=1: rows {tr1 * 1e3:.3f} ms + columns {tc1 * 1e3:.3f} ms, "
         f"{B1 / (tr1 + tc1) / 1e9:.0f} GB/s algorithmic")
-    rb = min(R, 16)
+    rb = op.get_shape(R, L.FTLM_C64)["rb"] or 1 << (min(R, 16).bit_length() - 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
                 "kernel": f"H*V = k_hv_dn<float,{rb},false> then k_hv_up<float,{rb},false>",
@@ -705,6 +778,7 @@ def main():
         "plain_fp64": plain,
         "batched_vs_plain": None if plain is None else round(value / plain["value"], 3),
         "dlnZ": dlnz,
+        "autotune": tuned,
     }
     print(json.dumps(out), flush=True)
 

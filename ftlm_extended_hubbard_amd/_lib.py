@@ -41,6 +41,11 @@ class Pair(C.Structure):
     _fields_ = [("i", C.c_int), ("j", C.c_int)]
 
 
+class HvShape(C.Structure):
+    _fields_ = [("rb", C.c_int), ("bpx", C.c_int), ("wlog", C.c_int), ("grid", C.c_int), ("ms", C.c_double),
+                ("ms_default", C.c_double), ("n_tried", C.c_int)]
+
+
 class FtlmOpts(C.Structure):
     _fields_ = [("R", C.c_int), ("M", C.c_int), ("seed", C.c_uint), ("start", C.c_int), ("dtype", C.c_int),
                 ("quad", C.c_int), ("clamp_M_to_dim", C.c_int), ("real_variant", C.c_int), ("path", C.c_int),
@@ -88,6 +93,9 @@ def lib() -> C.CDLL:
     L.ftlm_hip_index_of.argtypes = [vp, vp, vp, C.c_int64, vp]
     L.ftlm_hip_apply.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp]
     L.ftlm_hip_apply_pass_times.argtypes = [vp, vp, vp, C.c_int, C.c_int, vp, vp, vp]
+    L.ftlm_hip_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp, C.POINTER(HvShape)]
+    L.ftlm_hip_set_shape.argtypes = [vp, C.c_int, C.c_int, C.POINTER(HvShape)]
+    L.ftlm_hip_get_shape.argtypes = [vp, C.c_int, C.c_int, C.POINTER(HvShape)]
     L.ftlm_hip_apply_host.argtypes = [vp, vp, vp]
     L.ftlm_hip_lanczos.argtypes = [vp, C.c_int, C.c_uint, vp, vp, vp, ip, ip]
     L.ftlm_hip_lanczos_extrema.argtypes = [vp, C.c_int, C.c_uint, dp, dp, ip]

@@ -14,7 +14,9 @@ operator (HubbardOperator / KBlockOperator): the operator itself lives on the GP
 from __future__ import annotations
 
 import ctypes as C
+import json
 import math
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -141,6 +143,54 @@ class _Op:
                                                   C.c_void_p(stream), C.byref(a), C.byref(b)))
         return a.value, b.value
 
+    # ---- launch shapes
+    def autotune(self, R: int, dtype: int = L.FTLM_C64, reps: int = 3, cache: str | None = None) -> dict:
+        """Times H*V launch shapes for this operator at (R, dtype) and keeps the fastest
+        (ftlm_hip_autotune); every later apply and Lanczos run with that R and dtype uses it.
+        With `cache` (a JSON file), a shape stored under this operator's key is restored instead
+        of tuned, and a new tuning is added to the file."""
+        key = self._shape_key(R, dtype)
+        if cache is not None and os.path.exists(cache):
+            with open(cache) as f:
+                table = json.load(f)
+            if key in table:
+                self.set_shape(R, dtype, **{k: table[key][k] for k in ("rb", "bpx", "wlog", "grid")})
+                return dict(table[key], cached=True)
+        sh = L.HvShape()
+        stream = 0
+        try:
+            import torch
+
+            stream = torch.cuda.current_stream().cuda_stream
+        except Exception:  # noqa: BLE001
+            pass
+        L.check(L.lib().ftlm_hip_autotune(self._h, R, dtype, reps, C.c_void_p(stream), C.byref(sh)))
+        out = {"rb": sh.rb, "bpx": sh.bpx, "wlog": sh.wlog, "grid": sh.grid, "ms": sh.ms,
+               "ms_default": sh.ms_default, "n_tried": sh.n_tried}
+        if cache is not None:
+            table = {}
+            if os.path.exists(cache):
+                with open(cache) as f:
+                    table = json.load(f)
+            table[key] = out
+            with open(cache, "w") as f:
+                json.dump(table, f, indent=1, sort_keys=True)
+        return dict(out, cached=False)
+
+    def set_shape(self, R: int, dtype: int, rb: int = 0, bpx: int = 0, wlog: int = -1, grid: int = 0) -> None:
+        sh = L.HvShape(rb, bpx, wlog, grid, 0.0, 0.0, 0)
+        L.check(L.lib().ftlm_hip_set_shape(self._h, R, dtype, C.byref(sh)))
+
+    def get_shape(self, R: int, dtype: int) -> dict:
+        sh = L.HvShape()
+        L.check(L.lib().ftlm_hip_get_shape(self._h, R, dtype, C.byref(sh)))
+        return {"rb": sh.rb, "bpx": sh.bpx, "wlog": sh.wlog, "grid": sh.grid}
+
+    def _shape_key(self, R: int, dtype: int) -> str:
+        d, du, dd = C.c_int64(0), C.c_int64(0), C.c_int64(0)
+        L.check(L.lib().ftlm_hip_op_dim(self._h, C.byref(d), C.byref(du), C.byref(dd)))
+        return f"{type(self).__name__}:{getattr(self, '_tune_tag', '')}:{d.value}:{du.value}:{dd.value}:R{R}:d{dtype}"
+
     # ---- Lanczos
     def lanczos(self, max_steps: int, seed: int = 1, q0: np.ndarray | None = None):
         """lanczos_tridiagonal (src/lanczos.cpp:125-209): (used, alpha, beta)."""
@@ -220,7 +270,9 @@ class HubbardOperator(_Op):
     @classmethod
     def from_params(cls, p: HubbardParams, n_up: int, n_dn: int) -> "HubbardOperator":
         hops, pairs = build_hubbard_geometry(p)
-        return cls._create(p.Lx * p.Ly, n_up, n_dn, hops, pairs, p.U, p.V)
+        op = cls._create(p.Lx * p.Ly, n_up, n_dn, hops, pairs, p.U, p.V)
+        op._tune_tag = f"{p.Lx}x{p.Ly}"
+        return op
 
 
 # ---------------------------------------------------------------- translations and k-blocks
@@ -298,7 +350,9 @@ class KBlockOperator(_Op):
                     share: "KBlockOperator | None" = None) -> "KBlockOperator":
         hops, pairs = build_hubbard_geometry(p)
         K = MomentumSector(K.kx, K.ky, p.Lx, p.Ly)
-        return cls._create(p.Lx, p.Ly, n_up, n_dn, K, hops, pairs, p.U, p.V, tol, share)
+        op = cls._create(p.Lx, p.Ly, n_up, n_dn, K, hops, pairs, p.U, p.V, tol, share)
+        op._tune_tag = f"{p.Lx}x{p.Ly}:{n_up},{n_dn}:K{K.kx},{K.ky}"
+        return op
 
     def info(self):
         kx, ky, no, nnz = C.c_int(0), C.c_int(0), C.c_int64(0), C.c_int64(0)

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <stdexcept>
 #include <complex>
+#include <map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -159,10 +160,22 @@ constexpr int kMaxBondsLds = 512;  // LDS slots per spin for bond data (ids < kZ
 // Device view of an orbit-representative momentum block (filled by orbit.hip).
 struct KBlockView;
 
+// Launch shape of an operator's H*V kernels for one (R, dtype): the vectors per launch (rb), the
+// workgroups per XCD of each two-pass sweep (bpx), the column pass's stripe width log2 (wlog) and
+// the grid of the single-pass kernels (grid).  0 (wlog: -1) keeps the library's default.
+struct HvShape {
+  int rb = 0, bpx = 0, wlog = -1, grid = 0;
+};
+
 struct OpBase {
   int kind = kSector;
   int device = 0;
   int64_t dim = 0;
+  std::map<int, HvShape> shapes;  // set by the autotuner, key R * 4 + dtype
+  HvShape shape(int R, int dtype) const {
+    const auto it = shapes.find(R * 4 + dtype);
+    return it == shapes.end() ? HvShape{} : it->second;
+  }
   virtual ~OpBase() = default;
 };
 
@@ -227,6 +240,9 @@ void check_launch(const char* what);
 int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
              hipStream_t s, bool force_real = false);
 int hv_partial_blocks(const ftlm_op* op);
+// Default stripe-width log2 of a two-pass sector's column pass for a launch of rb vectors (-1 for
+// a single-pass sector), and the largest one the tile allows.
+int hv_default_wlog(const ftlm_op* op, int R, int dtype, int rb, int* wlog_max);
 // True when a momentum-block operator is Hermitian (translation-invariant hop coefficients).
 bool kblock_hermitian(const ftlm_op* op);
 

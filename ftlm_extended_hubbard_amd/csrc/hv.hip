@@ -574,19 +574,20 @@ struct Plan {
   int grid_up = 0, wlog = 0, tps = 1, ntiles_up = 0;
 };
 
-Plan make_plan(const Sector* Sx, int RT, int esize, int L) {
+Plan make_plan(const Sector* Sx, int RT, int esize, int L, const HvShape& sh) {
   const SectorView S = Sx->view();
   Plan p;
   p.two = S.Ddn >= 128;
   if (!p.two) {
     const int64_t tiles = (S.dim + kHvBlock - 1) / kHvBlock;
-    p.grid_flat = static_cast<int>(std::min<int64_t>(tiles, kHvMaxGrid));
+    p.grid_flat = static_cast<int>(std::min<int64_t>(tiles, sh.grid > 0 ? std::min(sh.grid, kHvMaxGrid) : kHvMaxGrid));
     return p;
   }
   const int EPB = kHvBlock / L;
   const int64_t budget = static_cast<int64_t>(env_int("FTLM_HV_L2_KB", 3072)) * 1024;
   // workgroups per XCD: enough to fill its 32 CUs several times over
-  const int bpx = std::max(1, std::min(kPassMaxGrid / kXcd, env_int("FTLM_HV_BPX", 256)));
+  const int bpx =
+      std::max(1, std::min(kPassMaxGrid / kXcd, sh.bpx > 0 ? sh.bpx : env_int("FTLM_HV_BPX", 256)));
   p.nchunk = (S.Ddn + EPB - 1) / EPB;
   const int64_t ntd = static_cast<int64_t>(S.Dup) * p.nchunk;
   FTLM_REQUIRE(ntd < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many row tiles");
@@ -597,6 +598,7 @@ Plan make_plan(const Sector* Sx, int RT, int esize, int L) {
   int wl = 0;
   while ((1 << (wl + 1)) <= EPB && col_bytes * (int64_t{1} << (wl + 1)) <= budget) ++wl;
   wl = std::max(0, std::min(wl, env_int("FTLM_HV_WLOG", wl)));
+  if (sh.wlog >= 0) wl = sh.wlog;
   while ((1 << wl) > EPB) --wl;
   p.wlog = wl;
   const int rows_per_tile = EPB >> wl;
@@ -610,11 +612,11 @@ Plan make_plan(const Sector* Sx, int RT, int esize, int L) {
 
 template <typename T, int RB, bool REAL>
 int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const double* xs, double2* dotp,
-           hipStream_t st) {
+           hipStream_t st, const HvShape& sh) {
   using V2 = typename Cx<T>::v2;
   using SP = Split<T, RB>;
   const SectorView S = Sx->view();
-  const Plan p = make_plan(Sx, RT, sizeof(V2), SP::L);
+  const Plan p = make_plan(Sx, RT, sizeof(V2), SP::L, sh);
   const V2* xp = static_cast<const V2*>(x);
   V2* yp = static_cast<V2*>(y);
   if (!p.two) {
@@ -645,13 +647,13 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
 
 template <typename T, bool REAL>
 int launch_chunk(const Sector* S, const void* x, void* y, int RT, int r0, int rb, const double* xs, double2* dotp,
-                 hipStream_t st) {
+                 hipStream_t st, const HvShape& sh) {
   switch (rb) {
-    case 16: return launch<T, 16, REAL>(S, x, y, RT, r0, xs, dotp, st);
-    case 8: return launch<T, 8, REAL>(S, x, y, RT, r0, xs, dotp, st);
-    case 4: return launch<T, 4, REAL>(S, x, y, RT, r0, xs, dotp, st);
-    case 2: return launch<T, 2, REAL>(S, x, y, RT, r0, xs, dotp, st);
-    default: return launch<T, 1, REAL>(S, x, y, RT, r0, xs, dotp, st);
+    case 16: return launch<T, 16, REAL>(S, x, y, RT, r0, xs, dotp, st, sh);
+    case 8: return launch<T, 8, REAL>(S, x, y, RT, r0, xs, dotp, st, sh);
+    case 4: return launch<T, 4, REAL>(S, x, y, RT, r0, xs, dotp, st, sh);
+    case 2: return launch<T, 2, REAL>(S, x, y, RT, r0, xs, dotp, st, sh);
+    default: return launch<T, 1, REAL>(S, x, y, RT, r0, xs, dotp, st, sh);
   }
 }
 
@@ -663,13 +665,28 @@ int hv_partial_blocks(const ftlm_op* op) {
   return kHvMaxGrid;
 }
 
+int hv_default_wlog(const ftlm_op* op, int R, int dtype, int rb, int* wlog_max) {
+  if (wlog_max) *wlog_max = -1;
+  if (op->kind != kSector) return -1;
+  const Sector* Sx = static_cast<const Sector*>(op);
+  if (Sx->Ddn < 128) return -1;
+  const int esize = dtype == FTLM_C64 ? static_cast<int>(sizeof(float2)) : static_cast<int>(sizeof(double2));
+  const int bytes = rb * esize, L = bytes >= 32 ? bytes / 16 : 1;
+  const Plan p = make_plan(Sx, R, esize, L, HvShape{});
+  int wm = 0;
+  while ((1 << (wm + 1)) <= kHvBlock / L) ++wm;
+  if (wlog_max) *wlog_max = wm;
+  return p.wlog;
+}
+
 // Block apply over all RT vectors in chunks of RB (<= 8 for complex128, <= 16 for complex64).
 // Chunks are either all two-pass or all flat; the number of dot-partial blocks written is the
 // maximum over chunks, and blocks a chunk does not launch are zero-filled first.
 int hv_apply_sector(const Sector* S, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
                     hipStream_t st, bool force_real) {
   const bool real = S->real_h || force_real;
-  const int maxrb = dtype == FTLM_C64 ? 16 : 8;
+  const HvShape sh = S->shape(R, dtype);
+  const int maxrb = sh.rb > 0 ? std::min(sh.rb, dtype == FTLM_C64 ? 16 : 8) : (dtype == FTLM_C64 ? 16 : 8);
   int nblk = 0;
   std::vector<std::pair<int, int>> chunks;
   for (int r0 = 0; r0 < R;) {
@@ -686,11 +703,11 @@ int hv_apply_sector(const Sector* S, const void* x, void* y, int R, int dtype, c
   for (const auto& c : chunks) {
     int nb;
     if (dtype == FTLM_C64) {
-      nb = real ? launch_chunk<float, true>(S, x, y, R, c.first, c.second, xs, dotp, st)
-                : launch_chunk<float, false>(S, x, y, R, c.first, c.second, xs, dotp, st);
+      nb = real ? launch_chunk<float, true>(S, x, y, R, c.first, c.second, xs, dotp, st, sh)
+                : launch_chunk<float, false>(S, x, y, R, c.first, c.second, xs, dotp, st, sh);
     } else {
-      nb = real ? launch_chunk<double, true>(S, x, y, R, c.first, c.second, xs, dotp, st)
-                : launch_chunk<double, false>(S, x, y, R, c.first, c.second, xs, dotp, st);
+      nb = real ? launch_chunk<double, true>(S, x, y, R, c.first, c.second, xs, dotp, st, sh)
+                : launch_chunk<double, false>(S, x, y, R, c.first, c.second, xs, dotp, st, sh);
     }
     nblk = std::max(nblk, nb);
   }

@@ -1119,8 +1119,10 @@ int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype,
                     hipStream_t st) {
   const KBlock* B = static_cast<const KBlock*>(op);
   if (B->dim == 0) return 0;
-  const int grid = static_cast<int>(std::min<int64_t>(kKbMaxGrid, (B->dim + 255) / 256));
-  const int maxrb = dtype == FTLM_C64 ? 16 : 8;
+  const HvShape sh = B->shape(R, dtype);
+  const int gmax = sh.grid > 0 ? std::min(sh.grid, kKbMaxGrid) : kKbMaxGrid;
+  const int grid = static_cast<int>(std::min<int64_t>(gmax, (B->dim + 255) / 256));
+  const int maxrb = sh.rb > 0 ? std::min(sh.rb, dtype == FTLM_C64 ? 16 : 8) : (dtype == FTLM_C64 ? 16 : 8);
   std::vector<std::pair<int, int>> chunks;
   for (int r0 = 0; r0 < R;) {
     int rb = 1;

@@ -1,0 +1,179 @@
+// tune.hip -- launch-shape autotuner for the H*V kernels.
+//
+// The reference's apply (src/hubbard_hamiltonian.cpp:51-108) is one scalar loop with nothing to
+// tune.  Here the best launch shape of an operator depends on its dimensions, its hop counts,
+// the number of vectors R and the storage precision: how many vectors one launch carries (more
+// amortise the decode, fewer keep more workgroups resident), how many workgroups each XCD runs
+// per pass, and how wide the column pass's stripes are (wider stripes reuse more of each source
+// line, narrower ones keep the stripe inside the XCD's L2).  The tuner times ftlm_hip_apply for a
+// few shapes by coordinate descent and keeps the fastest in the operator, keyed by (R, dtype).
+
+#include <algorithm>
+#include <vector>
+
+#include "common.hpp"
+
+namespace ftlm_hip {
+namespace {
+
+constexpr int kTuneGridMax = 8192;  // the dot-partial buffer of every apply holds this many blocks
+
+struct EventPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  EventPair() {
+    FTLM_HIP_CHECK(hipEventCreate(&a));
+    FTLM_HIP_CHECK(hipEventCreate(&b));
+  }
+  ~EventPair() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  EventPair(const EventPair&) = delete;
+  EventPair& operator=(const EventPair&) = delete;
+};
+
+// Median milliseconds of `reps` applies with shape `sh` (after one untimed apply), or a negative
+// value when the shape is outside the kernels' limits (the apply refuses it before launching).
+double time_shape(ftlm_op* op, const HvShape& sh, int R, int dtype, int reps, const void* x, void* y,
+                  double2* dotp, hipStream_t st, EventPair& ev) {
+  op->shapes[R * 4 + dtype] = sh;
+  try {
+    hv_apply(op, x, y, R, dtype, nullptr, dotp, st);
+  } catch (const Error& e) {
+    if (e.code != FTLM_EUNSUPPORTED) throw;
+    FTLM_HIP_CHECK(hipStreamSynchronize(st));
+    return -1.0;
+  }
+  std::vector<float> t(static_cast<size_t>(reps));
+  for (int i = 0; i < reps; ++i) {
+    FTLM_HIP_CHECK(hipEventRecord(ev.a, st));
+    hv_apply(op, x, y, R, dtype, nullptr, dotp, st);
+    FTLM_HIP_CHECK(hipEventRecord(ev.b, st));
+    FTLM_HIP_CHECK(hipEventSynchronize(ev.b));
+    FTLM_HIP_CHECK(hipEventElapsedTime(&t[static_cast<size_t>(i)], ev.a, ev.b));
+  }
+  std::sort(t.begin(), t.end());
+  return static_cast<double>(t[t.size() / 2]);
+}
+
+}  // namespace
+}  // namespace ftlm_hip
+
+using namespace ftlm_hip;
+
+extern "C" {
+
+int ftlm_hip_autotune(ftlm_op_t* op, int R, int dtype, int reps, void* stream, ftlm_hv_shape_t* out) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && out != nullptr, FTLM_EINVAL, "null argument");
+    FTLM_REQUIRE(R >= 1 && R <= 32, FTLM_EINVAL, "R must be in [1, 32]");
+    FTLM_REQUIRE(dtype == FTLM_C128 || dtype == FTLM_C64, FTLM_EINVAL, "dtype must be FTLM_C128 or FTLM_C64");
+    FTLM_REQUIRE(reps >= 1 && reps <= 100, FTLM_EINVAL, "reps must be in [1, 100]");
+    FTLM_REQUIRE(op->kind == kSector || op->kind == kKBlock, FTLM_EUNSUPPORTED,
+                 "host-callback operators have no launch shape");
+    *out = ftlm_hv_shape_t{0, 0, -1, 0, 0.0, 0.0, 0};
+    if (op->dim <= 0) return;
+    FTLM_HIP_CHECK(hipSetDevice(op->device));
+    const hipStream_t st = static_cast<hipStream_t>(stream);
+    const size_t es = dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2);
+    DevBuf<unsigned char> x(static_cast<size_t>(op->dim) * R * es), y(static_cast<size_t>(op->dim) * R * es);
+    DevBuf<double2> dotp(static_cast<size_t>(hv_partial_blocks(op)) * R);
+    FTLM_HIP_CHECK(hipMemsetAsync(x.p, 0x3c, x.n, st));  // any finite, normal values time the same
+    EventPair ev;
+    const int key = R * 4 + dtype;
+    const auto prior = op->shapes.find(key);
+    const bool had = prior != op->shapes.end();
+    const HvShape kept = had ? prior->second : HvShape{};
+    try {
+      HvShape best{};
+      double tbest = time_shape(op, best, R, dtype, reps, x.p, y.p, dotp.p, st, ev);
+      FTLM_REQUIRE(tbest >= 0.0, FTLM_EUNSUPPORTED, "the default launch shape is outside the kernels' limits");
+      const double tdef = tbest;
+      int tried = 1;
+      auto consider = [&](const HvShape& c) {
+        const double t = time_shape(op, c, R, dtype, reps, x.p, y.p, dotp.p, st, ev);
+        if (t < 0.0) return;
+        ++tried;
+        if (t < 0.99 * tbest) {  // 1% margin against timing noise
+          tbest = t;
+          best = c;
+        }
+      };
+      // 1. vectors per launch: the default is the largest power of two <= min(R, 8 or 16)
+      const int maxrb = dtype == FTLM_C64 ? 16 : 8;
+      int rbdef = 1;
+      while (rbdef * 2 <= std::min(R, maxrb)) rbdef *= 2;
+      for (int rb = rbdef / 2; rb >= std::max(1, rbdef / 4); rb /= 2) {
+        HvShape c = best;
+        c.rb = rb;
+        consider(c);
+      }
+      const int rb = best.rb > 0 ? best.rb : rbdef;
+      int wmax = -1;
+      const int wdef = hv_default_wlog(op, R, dtype, rb, &wmax);
+      if (wdef >= 0) {
+        // 2. two-pass sector: workgroups per XCD and pass (default 256)
+        for (int bpx : {64, 128, 512}) {
+          HvShape c = best;
+          c.bpx = bpx;
+          consider(c);
+        }
+        // 3. column-pass stripe width around the L2-budget default
+        for (int w = std::max(0, wdef - 2); w <= std::min(wmax, wdef + 1); ++w) {
+          if (w == wdef) continue;
+          HvShape c = best;
+          c.wlog = w;
+          consider(c);
+        }
+      } else {
+        // 2'. single-pass kernels: fewer workgroups than the default of one per 256 rows (capped
+        // at 4096 for momentum blocks, 8192 for sectors), each looping over more rows
+        const int64_t dflt = std::min<int64_t>((op->dim + 255) / 256, op->kind == kKBlock ? 4096 : kTuneGridMax);
+        for (int g : {128, 256, 512, 1024, 2048, 4096}) {
+          if (g >= dflt) break;
+          HvShape c = best;
+          c.grid = g;
+          consider(c);
+        }
+      }
+      op->shapes[key] = best;
+      *out = ftlm_hv_shape_t{best.rb, best.bpx, best.wlog, best.grid, tbest, tdef, tried};
+    } catch (...) {
+      if (had)
+        op->shapes[key] = kept;
+      else
+        op->shapes.erase(key);
+      throw;
+    }
+  });
+}
+
+int ftlm_hip_set_shape(ftlm_op_t* op, int R, int dtype, const ftlm_hv_shape_t* shape) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && shape != nullptr, FTLM_EINVAL, "null argument");
+    FTLM_REQUIRE(R >= 1 && R <= 32, FTLM_EINVAL, "R must be in [1, 32]");
+    FTLM_REQUIRE(dtype == FTLM_C128 || dtype == FTLM_C64, FTLM_EINVAL, "dtype must be FTLM_C128 or FTLM_C64");
+    const int maxrb = dtype == FTLM_C64 ? 16 : 8;
+    FTLM_REQUIRE(shape->rb == 0 || (shape->rb >= 1 && shape->rb <= maxrb && (shape->rb & (shape->rb - 1)) == 0),
+                 FTLM_EINVAL, "rb must be 0 or a power of two up to 8 (complex128) / 16 (complex64)");
+    FTLM_REQUIRE(shape->bpx >= 0 && shape->bpx <= 512, FTLM_EINVAL, "bpx must be in [0, 512]");
+    FTLM_REQUIRE(shape->wlog >= -1 && shape->wlog <= 8, FTLM_EINVAL, "wlog must be in [-1, 8]");
+    FTLM_REQUIRE(shape->grid >= 0 && shape->grid <= kTuneGridMax, FTLM_EINVAL, "grid must be in [0, 8192]");
+    HvShape s;
+    s.rb = shape->rb;
+    s.bpx = shape->bpx;
+    s.wlog = shape->wlog;
+    s.grid = shape->grid;
+    op->shapes[R * 4 + dtype] = s;
+  });
+}
+
+int ftlm_hip_get_shape(const ftlm_op_t* op, int R, int dtype, ftlm_hv_shape_t* shape) {
+  return guarded([&] {
+    FTLM_REQUIRE(op != nullptr && shape != nullptr, FTLM_EINVAL, "null argument");
+    const HvShape s = op->shape(R, dtype);
+    *shape = ftlm_hv_shape_t{s.rb, s.bpx, s.wlog, s.grid, 0.0, 0.0, 0};
+  });
+}
+
+}  // extern "C"

@@ -118,6 +118,29 @@ int ftlm_hip_apply(const ftlm_op_t* op, const void* x, void* y, int R, int dtype
 int ftlm_hip_apply_pass_times(const ftlm_op_t* op, const void* x, void* y, int R, int dtype, void* stream,
                               double* ms_row, double* ms_col);
 
+/* H*V launch-shape autotuner (no counterpart in the reference, whose apply is one scalar loop).
+ * A shape holds, for one (R, dtype) of one operator:
+ *   rb    vectors per launch (the block of R is applied in launches of at most rb vectors);
+ *   bpx   workgroups per XCD in each pass of a two-pass sector apply;
+ *   wlog  log2 of the column pass's stripe width (two-pass sectors);
+ *   grid  workgroups of a single-pass apply (small sectors, momentum blocks).
+ * 0 (wlog: -1) means the library default.  ftlm_hip_autotune times ftlm_hip_apply on scratch
+ * blocks for candidate shapes (coordinate descent: rb, then bpx or grid, then wlog; `reps` timed
+ * applies each, median, after one warm-up), keeps the fastest in the operator -- every later
+ * apply, Lanczos step and FTLM run of the operator with that R and dtype uses it -- and reports
+ * it with its time, the default shape's time and the number of shapes tried.  set / get store and
+ * read a shape directly (to restore a cached tuning).  Results never depend on the shape beyond
+ * the summation order of the dot-product partials. */
+typedef struct ftlm_hv_shape {
+  int rb, bpx, wlog, grid;
+  double ms;         /* median time of one apply with this shape (autotune only) */
+  double ms_default; /* the same for the default shape (autotune only) */
+  int n_tried;       /* shapes timed (autotune only) */
+} ftlm_hv_shape_t;
+int ftlm_hip_autotune(ftlm_op_t* op, int R, int dtype, int reps, void* stream, ftlm_hv_shape_t* out);
+int ftlm_hip_set_shape(ftlm_op_t* op, int R, int dtype, const ftlm_hv_shape_t* shape);
+int ftlm_hip_get_shape(const ftlm_op_t* op, int R, int dtype, ftlm_hv_shape_t* shape);
+
 /* Drop-in for apply_extended_hubbard(p, basis, hops, pairs, x, y) on host complex<double> arrays
  * of length dim: copies in, applies, copies out (synchronous). */
 int ftlm_hip_apply_host(const ftlm_op_t* op, const double* x, double* y);
