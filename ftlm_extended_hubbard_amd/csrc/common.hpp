@@ -240,6 +240,9 @@ void check_launch(const char* what);
 int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
              hipStream_t s, bool force_real = false);
 int hv_partial_blocks(const ftlm_op* op);
+// Grid of a single-pass apply (small sector or momentum block) with the default shape, and the
+// largest grid a shape may ask for.
+int hv_default_grid(const ftlm_op* op, int* grid_max);
 // Default stripe-width log2 of a two-pass sector's column pass for a launch of rb vectors (-1 for
 // a single-pass sector), and the largest one the tile allows.
 int hv_default_wlog(const ftlm_op* op, int R, int dtype, int rb, int* wlog_max);

@@ -298,6 +298,8 @@ int run_k(int argc, char** argv, bool ed_only) {
 
   const auto t0 = clock_type::now();
   int64_t blocks_ed = 0, blocks_ftlm = 0;
+  double t_create = 0.0, t_ed = 0.0, t_ftlm = 0.0;  // per-phase wall time, reported on stderr
+  auto since = [](clock_type::time_point a) { return std::chrono::duration<double>(clock_type::now() - a).count(); };
   for (int nu = 0; nu <= n_sites; ++nu)
     for (int nd = 0; nd <= n_sites; ++nd) {
       const int idx = sector_index(nu, nd, n_sites);
@@ -311,9 +313,11 @@ int run_k(int argc, char** argv, bool ed_only) {
       for (int ky = 0; ky < Ly; ++ky)
         for (int kx = 0; kx < Lx; ++kx) {
           ftlm_op_t* op = nullptr;
+          const auto tc = clock_type::now();
           ok(ftlm_hip_kblock_create(Lx, Ly, nu, nd, kx, ky, 1e-12, m.hops.data(), static_cast<int>(m.hops.size()),
                                     m.pairs.data(), static_cast<int>(m.pairs.size()), m.p.U, m.p.V, first, &op),
              "kblock_create");
+          t_create += since(tc);
           const bool keep = first == nullptr;
           if (keep) first = op;
           struct Release {
@@ -329,14 +333,17 @@ int run_k(int argc, char** argv, bool ed_only) {
           if (log_k_dims) std::cerr << "  k=(" << kx << "," << ky << ") dim=" << dk << " [orbit-direct]\n";
           if (dk <= 0) continue;
           double lz_k = -std::numeric_limits<double>::infinity();
+          const auto ts = clock_type::now();
           if (dk <= ed_cutoff) {
             ++n_ed;
             ok(ftlm_hip_exact_log_partition(op, &c.beta, 1, &lz_k, nullptr), "exact_log_partition");
+            t_ed += since(ts);
           } else {
             ++n_ftlm;
             const unsigned seed_k = c.seed + static_cast<unsigned>(idx * 257 + ky * 17 + kx);
             const ftlm_ftlm_opts_t o = make_opts(argc, argv, c.n_rand, c.lz_steps, seed_k, 0);
             ok(ftlm_hip_ftlm_log_partition(op, &c.beta, 1, &o, &lz_k), "ftlm_log_partition");
+            t_ftlm += since(ts);
           }
           logZ_sector = logsumexp2(logZ_sector, lz_k);
         }
@@ -349,6 +356,8 @@ int run_k(int argc, char** argv, bool ed_only) {
     }
   const double t_sectors = std::chrono::duration<double>(clock_type::now() - t0).count();
   std::cout << "[timing] all sectors logZ: " << t_sectors << " s\n";
+  std::cerr << "[timing] kblock_create " << t_create << " s, ed " << t_ed << " s (" << blocks_ed << " blocks), ftlm "
+            << t_ftlm << " s (" << blocks_ftlm << " blocks)\n";
 
   std::ostream* out = &std::cout;
   std::ofstream file;

@@ -665,6 +665,14 @@ int hv_partial_blocks(const ftlm_op* op) {
   return kHvMaxGrid;
 }
 
+int kblock_default_grid(const ftlm_op* op, int* grid_max);
+
+int hv_default_grid(const ftlm_op* op, int* grid_max) {
+  if (op->kind == kKBlock) return kblock_default_grid(op, grid_max);
+  if (grid_max) *grid_max = kHvMaxGrid;
+  return static_cast<int>(std::min<int64_t>((op->dim + kHvBlock - 1) / kHvBlock, kHvMaxGrid));
+}
+
 int hv_default_wlog(const ftlm_op* op, int R, int dtype, int rb, int* wlog_max) {
   if (wlog_max) *wlog_max = -1;
   if (op->kind != kSector) return -1;

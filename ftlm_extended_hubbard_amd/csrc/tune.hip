@@ -126,11 +126,13 @@ int ftlm_hip_autotune(ftlm_op_t* op, int R, int dtype, int reps, void* stream, f
           consider(c);
         }
       } else {
-        // 2'. single-pass kernels: fewer workgroups than the default of one per 256 rows (capped
-        // at 4096 for momentum blocks, 8192 for sectors), each looping over more rows
-        const int64_t dflt = std::min<int64_t>((op->dim + 255) / 256, op->kind == kKBlock ? 4096 : kTuneGridMax);
-        for (int g : {128, 256, 512, 1024, 2048, 4096}) {
-          if (g >= dflt) break;
+        // 2'. single-pass kernels: the grid (rows are grid-strided, so a grid changes both the
+        // occupancy and how many consecutive rows are in flight at once)
+        int gmax = 0;
+        const int dflt = hv_default_grid(op, &gmax);
+        const int64_t tiles = (op->dim + 255) / 256;
+        for (int g : {128, 256, 512, 1024, 2048, 4096, 8192}) {
+          if (g == dflt || g > gmax || g > tiles) continue;
           HvShape c = best;
           c.grid = g;
           consider(c);
