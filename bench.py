@@ -478,7 +478,7 @@ def graph_workload(F, L, torch):
             "config": {"workload": "sector ladder 4x2 (2,2) .. 4x3 (6,6)"}, "rows": rows}
 
 
-def reorth_workload(F, L, torch):
+def reorth_workload(F, L, torch, profile=False):
     """Plain vs reorthogonalised FTLM Lanczos (FTLM_REORTH_FULL, FTLM_REORTH_PARTIAL): R=16
     random-phase vectors, M=150 steps, converged Gauss rule, on 4x3 sectors (t=1, t'=-0.25, U=8,
     V=1, phi_x=0.05), complex64 and complex128 storage.  Wall time of ftlm_ritz per operator
@@ -486,6 +486,17 @@ def reorth_workload(F, L, torch):
     import numpy as np
 
     betas = np.geomspace(0.1, 20.0, 12)
+    if profile:  # one full-mode complex128 run on 4x3 (6,6), for a kernel trace
+        op = F.HubbardOperator.from_params(F.HubbardParams(4, 3, 1.0, -0.25, 8.0, 1.0, 0.05, 0.0), 6, 6)
+        par = F.FtlmParams(n_random=16, lanczos_steps=150, seed=3, start=L.FTLM_START_PHASE, dtype=L.FTLM_C128,
+                           quad=L.FTLM_QUAD_CONVERGED, path=L.FTLM_PATH_PER_OP, reorth=L.FTLM_REORTH_FULL)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        F.ftlm_ritz([op], par)
+        sec = time.perf_counter() - t0
+        log(f"4x3 (6,6) R=16 M=150 c128 full reorthogonalisation: {sec:.3f} s")
+        return {"metric": "FTLM Lanczos with full reorthogonalisation, 4x3 (6,6), R=16, M=150, c128", "unit": "s",
+                "value": round(sec, 3), "higher_is_better": False, "n_gpus": 1, "data": "synthetic"}
     rows = []
     for lx, ly, nu, nd in [(4, 3, 3, 3), (4, 3, 4, 4), (4, 3, 6, 6)]:
         p = F.HubbardParams(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.05, 0.0)
@@ -644,6 +655,8 @@ def main():
                     help="only run 3 block H*V (R vectors, complex64) and one 4 GiB device copy, for rocprofv3")
     ap.add_argument("--hv-shape", type=str, default=None,
                     help="rb,bpx,wlog,grid launch shape for --profile-hv (as reported under 'autotune')")
+    ap.add_argument("--profile-reorth", action="store_true",
+                    help="with --workload reorth: one full-mode complex128 run on 4x3 (6,6) only, for rocprofv3")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the library's default H*V launch shapes instead of autotuning them first")
     args = ap.parse_args()
@@ -671,7 +684,7 @@ def main():
         print(json.dumps(graph_workload(F, L, torch)), flush=True)
         return
     if args.workload == "reorth":
-        print(json.dumps(reorth_workload(F, L, torch)), flush=True)
+        print(json.dumps(reorth_workload(F, L, torch, profile=args.profile_reorth)), flush=True)
         return
     if args.workload == "tune":
         print(json.dumps(tune_workload(F, L, torch)), flush=True)
