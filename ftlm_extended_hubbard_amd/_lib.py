@@ -128,6 +128,7 @@ def lib() -> C.CDLL:
     L.ftlm_hip_kblock_create.argtypes = [C.c_int] * 6 + [C.c_double, vp, C.c_int, vp, C.c_int, C.c_double,
                                                          C.c_double, vp, C.POINTER(vp)]
     L.ftlm_hip_kblock_info.argtypes = [vp, ip, ip, i64p, i64p]
+    L.ftlm_hip_kgram_create.argtypes = [vp, vp, C.POINTER(C.c_void_p)]
     L.ftlm_hip_kblock_entries.argtypes = [vp, C.c_int64, C.c_int64, vp, vp, vp, vp]
     L.ftlm_hip_kblock_index_of_rep.argtypes = [vp, vp, C.c_int64, vp]
     L.ftlm_hip_kblock_map.argtypes = [vp, vp, C.c_int64, vp, vp, vp, vp]

@@ -387,6 +387,24 @@ class KBlockOperator(_Op):
 _APPLY_CACHE: dict = {}
 
 
+class KGramOperator(_Op):
+    """HubbardMomentumBlock (src/hubbard_momentum_action.cpp:37-89) in the Gram basis of
+    MomentumPhiGramBasis (src/orbit_bloch_phi.cpp:226-355): y = Phi^dag H Phi x through the full
+    sector.  `sector` (a HubbardOperator of the same lattice and particle numbers) may be shared by
+    the blocks of every K; it is kept alive by the block."""
+
+    @classmethod
+    def from_params(cls, p: HubbardParams, n_up: int, n_dn: int, K: MomentumSector,
+                    sector: "HubbardOperator | None" = None,
+                    share: "KBlockOperator | None" = None) -> "KGramOperator":
+        sector = HubbardOperator.from_params(p, n_up, n_dn) if sector is None else sector
+        kb = KBlockOperator.from_params(p, n_up, n_dn, K, share=share)
+        self = cls()
+        L.check(L.lib().ftlm_hip_kgram_create(sector._h, kb._h, C.byref(self._h)))
+        self.sector, self.K = sector, MomentumSector(K.kx, K.ky, p.Lx, p.Ly)
+        return self
+
+
 def apply_extended_hubbard(p: HubbardParams, basis: FockBasis, hops, pairs, x: np.ndarray) -> np.ndarray:
     """Drop-in of apply_extended_hubbard (src/hubbard_hamiltonian.cpp:51-108): returns y = H x.
     The device operator for (basis, hops, pairs, U, V) is built once and cached."""

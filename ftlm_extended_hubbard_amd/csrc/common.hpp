@@ -121,7 +121,7 @@ Binom host_binomials();
 int64_t binom(int a, int b);
 
 // ------------------------------------------------------------------ operator (sector / k-block)
-enum OpKind { kSector = 0, kKBlock = 1, kCallback = 2 };
+enum OpKind { kSector = 0, kKBlock = 1, kCallback = 2, kGram = 3 };
 
 // Device view of a full (N_up, N_dn) sector with its per-spin pull-hop tables.
 // Hop entry (uint32): bits 0..19 target string rank, 20..30 bond id, 31 own-spin JW parity.
@@ -240,6 +240,8 @@ void check_launch(const char* what);
 int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
              hipStream_t s, bool force_real = false);
 int hv_partial_blocks(const ftlm_op* op);
+// Sector indices of packed states (u << 16) | d, all in the sector (sector.hip).
+void sector_rank_packs(const Sector* S, const uint32_t* d_packs, int64_t count, int32_t* d_out, hipStream_t st);
 // Grid of a single-pass apply (small sector or momentum block) with the default shape, and the
 // largest grid a shape may ask for.
 int hv_default_grid(const ftlm_op* op, int* grid_max);

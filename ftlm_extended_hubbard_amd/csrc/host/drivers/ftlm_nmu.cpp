@@ -9,12 +9,17 @@
 // METRIC output (:181-184, 232-244).  Every sector or block runs on the device:
 //   nonmom   ftlm_log_partition_real of every (N_up, N_dn) sector, all sectors in one
 //            ftlm_hip_ftlm_ritz call (one persistent launch when every sector is small);
-//   k-orbit  every momentum block of every sector as an orbit-representative operator
-//            (apply_orbit_k_block), FTLM (complex) above the ED cutoff, device dense ED below;
-//   ed       every block by device dense ED.
-// The reference's k-gram solver (Gram blocks, LAPACK zheev) is not built; k-orbit is the default
-// of ftlm_nmu_rect_k.  Device options beyond the reference: --start=gauss|phase,
-// --quad=capped|converged, --dtype=c128|c64 (defaults reproduce the reference).
+//   k-gram   (the default, as in the reference) every momentum block as HubbardMomentumBlock: the
+//            Gram-basis block applied through the full sector (ftlm_hip_kgram_create); FTLM
+//            (complex) above the ED cutoff, device dense ED below;
+//   k-orbit  the blocks of sectors with 0 < N_up, N_dn < N and at most 512 states as
+//            orbit-representative operators (apply_orbit_k_block), every other block as a Gram
+//            block, as the reference gates its orbit-direct path (:797-807, 1003-1019);
+//   ed       every block a Gram block, by device dense ED.
+// The Gram blocks keep the reference's subspace (orbits with trivial bosonic and fermionic
+// stabiliser characters); their orthonormal basis is not the reference's zheev basis, so ED blocks
+// agree exactly and FTLM blocks in distribution.  Device options beyond the reference:
+// --start=gauss|phase, --quad=capped|converged, --dtype=c128|c64 (defaults reproduce the reference).
 #include <sys/resource.h>
 
 #include <algorithm>
@@ -23,6 +28,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <initializer_list>
 #include <iostream>
 #include <limits>
 #include <string>
@@ -270,8 +276,14 @@ int run_nonmom(int argc, char** argv, const char* metric_kind) {
 }
 
 #if FTLM_NMU_K
-// Momentum-block solvers (bench_ftlm_nmu_4x4_k.cpp:546-1262): k-orbit and ed.
-int run_k(int argc, char** argv, bool ed_only) {
+// Momentum-block solvers (bench_ftlm_nmu_4x4_k.cpp:546-1262): k-gram, k-orbit and ed.
+enum class KSolver { kGram, kOrbit, kEd };
+constexpr int kOrbitDirectMaxDim = 512;  // kOrbitMfMaxDim, bench_ftlm_nmu_4x4_k.cpp:616
+
+const char* k_label(KSolver s) { return s == KSolver::kGram ? "k-gram" : s == KSolver::kOrbit ? "k-orbit" : "ed"; }
+
+int run_k(int argc, char** argv, KSolver solver, bool solver_set) {
+  const bool ed_only = solver == KSolver::kEd;
   const int Lx = arg_int(argc, argv, "Lx", 4), Ly = arg_int(argc, argv, "Ly", 3);
   if (Lx <= 0 || Ly <= 0 || Lx * Ly > 16) {
     std::cerr << "ftlm_nmu_rect_k: requires 1 <= Lx*Ly <= 16\n";
@@ -289,15 +301,21 @@ int run_k(int argc, char** argv, bool ed_only) {
   std::vector<double> logZ(static_cast<size_t>(nsec), -std::numeric_limits<double>::infinity());
   std::vector<int> Nelec(static_cast<size_t>(nsec), 0);
   std::cout << "=== FTLM grand-canonical n(mu) " << Lx << "x" << Ly << " with momentum blocks ===\n";
-  std::cout << "[solver] " << (ed_only ? "ed" : "k-orbit");
-  if (ed_only) std::cout << "  ed_cutoff=" << ed_cutoff;
-  std::cout << "\n";
+  if (solver_set) {  // bench_ftlm_nmu_4x4_k.cpp:704-710
+    std::cout << "[solver] " << k_label(solver);
+    if (ed_only) std::cout << "  ed_cutoff=" << ed_cutoff;
+    std::cout << "\n";
+  }
   std::cout << "beta=" << c.beta << "  t=" << m.p.t << " tp=" << m.p.tp << " U=" << m.p.U << " V=" << m.p.V << "\n";
   std::cout << "ftlm_random=" << c.n_rand << " lanczos_steps=" << c.lz_steps << "  sectors=" << nsec << "\n";
   if (!c.no_monitor) std::cout << "[monitor] wall time + peak RSS reported at exit (disable with --no-monitor=1)\n";
 
+  if (solver == KSolver::kOrbit)  // bench_ftlm_nmu_4x4_k.cpp:736-740
+    std::cout << "[kblock-prototype] orbit-direct: OrbitKBlockBasis + apply_orbit_k_block in orbit coords (no Phi "
+                 "Gram, no T bridge); dim_full<="
+              << kOrbitDirectMaxDim << "\n";
   const auto t0 = clock_type::now();
-  int64_t blocks_ed = 0, blocks_ftlm = 0;
+  int64_t blocks_ed = 0, blocks_ftlm = 0, orbit_direct_used = 0, orbit_direct_zero = 0;
   double t_create = 0.0, t_ed = 0.0, t_ftlm = 0.0;  // per-phase wall time, reported on stderr
   auto since = [](clock_type::time_point a) { return std::chrono::duration<double>(clock_type::now() - a).count(); };
   for (int nu = 0; nu <= n_sites; ++nu)
@@ -308,8 +326,31 @@ int run_k(int argc, char** argv, bool ed_only) {
       int64_t dim_k_total = 0;
       int n_ed = 0, n_ftlm = 0;
       // the sector's first block is kept until the sector is done: later blocks share its orbit
-      // graph; every other block is released as soon as its logZ is known
+      // graph; every other block is released as soon as its logZ is known.  Gram blocks apply
+      // through the full sector operator, built once per sector.
       ftlm_op_t* first = nullptr;
+      ftlm_op_t* sector_op = nullptr;
+      int64_t dim_full = 0;
+      {
+        ftlm_op_t* probe = nullptr;  // FockBasis dim (sector_create validates and enumerates)
+        ok(ftlm_hip_sector_create(n_sites, nu, nd, m.hops.data(), static_cast<int>(m.hops.size()),
+                                  m.pairs.data(), static_cast<int>(m.pairs.size()), m.p.U, m.p.V, &probe),
+           "sector_create");
+        ok(ftlm_hip_op_dim(probe, &dim_full, nullptr, nullptr), "op_dim");
+        sector_op = probe;
+      }
+      if (dim_full <= 0) {  // bench_ftlm_nmu_4x4_k.cpp:752-755
+        ftlm_hip_op_destroy(sector_op);
+        continue;
+      }
+      // orbit coordinates only for the k-orbit solver's interior sectors of <= 512 states
+      // (bench_ftlm_nmu_4x4_k.cpp:800-807); HubbardMomentumBlock everywhere else (:1003-1019)
+      const bool orbit_direct = solver == KSolver::kOrbit && nu > 0 && nd > 0 && nu < n_sites && nd < n_sites &&
+                                dim_full <= kOrbitDirectMaxDim;
+      if (orbit_direct) {
+        ftlm_hip_op_destroy(sector_op);
+        sector_op = nullptr;
+      }
       for (int ky = 0; ky < Ly; ++ky)
         for (int kx = 0; kx < Lx; ++kx) {
           ftlm_op_t* op = nullptr;
@@ -327,31 +368,44 @@ int run_k(int argc, char** argv, bool ed_only) {
               if (!keep) ftlm_hip_op_destroy(op);
             }
           } release{op, keep};
+          ftlm_op_t* gram = nullptr;
+          if (!orbit_direct) ok(ftlm_hip_kgram_create(sector_op, op, &gram), "kgram_create");
+          struct ReleaseGram {
+            ftlm_op_t* op;
+            ~ReleaseGram() {
+              if (op) ftlm_hip_op_destroy(op);
+            }
+          } release_gram{gram};
+          ftlm_op_t* const blk = gram ? gram : op;
           int64_t dk = 0;
-          ok(ftlm_hip_op_dim(op, &dk, nullptr, nullptr), "op_dim");
+          ok(ftlm_hip_op_dim(blk, &dk, nullptr, nullptr), "op_dim");
           dim_k_total += dk;
-          if (log_k_dims) std::cerr << "  k=(" << kx << "," << ky << ") dim=" << dk << " [orbit-direct]\n";
+          if (orbit_direct) ++(dk > 0 ? orbit_direct_used : orbit_direct_zero);
+          if (log_k_dims)
+            std::cerr << "  k=(" << kx << "," << ky << ") dim=" << dk << (gram ? "" : " [orbit-direct]") << "\n";
           if (dk <= 0) continue;
           double lz_k = -std::numeric_limits<double>::infinity();
           const auto ts = clock_type::now();
           if (dk <= ed_cutoff) {
             ++n_ed;
-            ok(ftlm_hip_exact_log_partition(op, &c.beta, 1, &lz_k, nullptr), "exact_log_partition");
+            ok(ftlm_hip_exact_log_partition(blk, &c.beta, 1, &lz_k, nullptr), "exact_log_partition");
             t_ed += since(ts);
           } else {
             ++n_ftlm;
             const unsigned seed_k = c.seed + static_cast<unsigned>(idx * 257 + ky * 17 + kx);
             const ftlm_ftlm_opts_t o = make_opts(argc, argv, c.n_rand, c.lz_steps, seed_k, 0);
-            ok(ftlm_hip_ftlm_log_partition(op, &c.beta, 1, &o, &lz_k), "ftlm_log_partition");
+            ok(ftlm_hip_ftlm_log_partition(blk, &c.beta, 1, &o, &lz_k), "ftlm_log_partition");
             t_ftlm += since(ts);
           }
           logZ_sector = logsumexp2(logZ_sector, lz_k);
         }
       if (first) ftlm_hip_op_destroy(first);
+      if (sector_op) ftlm_hip_op_destroy(sector_op);
       blocks_ed += n_ed;
       blocks_ftlm += n_ftlm;
       logZ[static_cast<size_t>(idx)] = logZ_sector;
-      std::cerr << "sector (" << nu << "," << nd << ") dim_k_total=" << dim_k_total << " ed_blocks=" << n_ed
+      std::cerr << "sector (" << nu << "," << nd << ") dim_full=" << dim_full << " dim_k_total=" << dim_k_total
+                << " ed_blocks=" << n_ed
                 << " ftlm_blocks=" << n_ftlm << " logZ=" << logZ_sector << "\n";
     }
   const double t_sectors = std::chrono::duration<double>(clock_type::now() - t0).count();
@@ -385,9 +439,13 @@ int run_k(int argc, char** argv, bool ed_only) {
       file.flush();
     }
   }
-  std::cout << "METRIC kind=FTLM_nmu_rect_k solver=" << (ed_only ? "ed" : "k-orbit") << " wall_time_s=" << t_wall
-            << " wall_sector_logZ_s=" << t_sectors << " peak_rss_bytes=" << peak_b << " peak_rss_mib=" << peak_mib
-            << " n_mu=" << c.n_mu << " ed_blocks=" << blocks_ed << " ftlm_blocks=" << blocks_ftlm << "\n";
+  std::cout << "METRIC kind=FTLM_nmu_rect_k solver=" << (solver_set ? k_label(solver) : "default")
+            << " wall_time_s=" << t_wall << " wall_sector_logZ_s=" << t_sectors << " peak_rss_bytes=" << peak_b
+            << " peak_rss_mib=" << peak_mib << " n_mu=" << c.n_mu << " gram_build_no_reuse=0 kblock_prototype="
+            << (solver == KSolver::kOrbit ? "orbit-direct" : "none");
+  if (solver == KSolver::kOrbit)
+    std::cout << " orbit_direct_blocks_used=" << orbit_direct_used << " orbit_direct_skipped_zero=" << orbit_direct_zero;
+  std::cout << " ed_blocks=" << blocks_ed << " ftlm_blocks=" << blocks_ftlm << "\n";
   return 0;
 }
 #endif
@@ -396,18 +454,20 @@ int run_k(int argc, char** argv, bool ed_only) {
 
 int main(int argc, char** argv) {
 #if FTLM_NMU_K
+  // solver names as parse_solver_name accepts them (bench_ftlm_nmu_4x4_k.cpp:340-362)
   const char* solver = find_arg(argc, argv, "solver");
-  if (solver && std::strcmp(solver, "nonmom") == 0) return run_nonmom(argc, argv, "FTLM_nmu_rect solver=nonmom");
-  if (solver && std::strcmp(solver, "ed") == 0) return run_k(argc, argv, true);
-  if (solver && std::strcmp(solver, "k-gram") == 0) {
-    std::cerr << "ftlm_nmu_rect_k: --solver=k-gram (Gram blocks, LAPACK zheev) is not built; use k-orbit\n";
-    return 2;
-  }
-  if (solver && std::strcmp(solver, "k-orbit") != 0) {
-    std::cerr << "ftlm_nmu_rect_k: unknown --solver=" << solver << " (supported: k-orbit, nonmom, ed)\n";
-    return 2;
-  }
-  return run_k(argc, argv, false);
+  auto is = [&](std::initializer_list<const char*> names) {
+    for (const char* n : names)
+      if (std::strcmp(solver, n) == 0) return true;
+    return false;
+  };
+  if (solver == nullptr) return run_k(argc, argv, KSolver::kGram, false);
+  if (is({"nonmom", "nomom", "non-momentum"})) return run_nonmom(argc, argv, "FTLM_nmu_rect solver=nonmom");
+  if (is({"ed", "exact"})) return run_k(argc, argv, KSolver::kEd, true);
+  if (is({"k-orbit", "orbit", "orbit-direct"})) return run_k(argc, argv, KSolver::kOrbit, true);
+  if (is({"k-gram", "k_block_gram", "gram", "production"})) return run_k(argc, argv, KSolver::kGram, true);
+  std::cerr << "ftlm_nmu_rect_k: unknown --solver=" << solver << " (supported: k-gram, k-orbit, nonmom, ed)\n";
+  return 2;
 #else
   return run_nonmom(argc, argv, "FTLM_nmu_rect");
 #endif

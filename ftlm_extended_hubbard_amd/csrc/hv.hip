@@ -727,6 +727,8 @@ int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype,
                     hipStream_t st);
 int hv_apply_callback(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs,
                       double2* dotp, hipStream_t st);
+int hv_apply_gram(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
+                  hipStream_t st);
 
 int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
              hipStream_t st, bool force_real) {
@@ -734,6 +736,7 @@ int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const 
     return hv_apply_sector(static_cast<const Sector*>(op), x, y, R, dtype, xs, dotp, st, force_real);
   FTLM_REQUIRE(!force_real, FTLM_EUNSUPPORTED, "the real-part apply is defined for full-sector operators only");
   if (op->kind == kCallback) return hv_apply_callback(op, x, y, R, dtype, xs, dotp, st);
+  if (op->kind == kGram) return hv_apply_gram(op, x, y, R, dtype, xs, dotp, st);
   return hv_apply_kblock(op, x, y, R, dtype, xs, dotp, st);
 }
 

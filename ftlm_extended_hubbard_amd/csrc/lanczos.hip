@@ -681,7 +681,8 @@ struct Krylov {
   // anything later on `st`) and launched once.  Every step is kernel launches only: the H*V
   // tile counters reset themselves and partial sums are cleared by a kernel, never a memset node.
   void run_graph(int k0, int k1, hipStream_t st) {
-    FTLM_REQUIRE(op->kind != kCallback, FTLM_EUNSUPPORTED, "host-callback operators cannot be graph-captured");
+    FTLM_REQUIRE(op->kind != kCallback && op->kind != kGram, FTLM_EUNSUPPORTED,
+                 "host-callback operators and momentum Gram blocks cannot be graph-captured");
     if (k1 <= k0) return;
     hipStream_t cs = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1106,7 +1107,7 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
         std::vector<unsigned> seeds(static_cast<size_t>(Rg));
         for (int r = 0; r < Rg; ++r) seeds[r] = opts->seed + static_cast<unsigned>(g0 + r) * 100003u;
         fill_start(K, opts->start, seeds, nullptr, st);
-        if (opts->path == FTLM_PATH_GRAPH && op->kind != kCallback)
+        if (opts->path == FTLM_PATH_GRAPH && op->kind != kCallback && op->kind != kGram)
           K.run_graph(0, Mop, st);
         else
           K.run(0, Mop, st);

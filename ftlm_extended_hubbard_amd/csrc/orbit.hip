@@ -1149,6 +1149,210 @@ int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype,
   return grid;
 }
 
+// ------------------------------------------------------------------ momentum Gram blocks
+// HubbardMomentumBlock (src/hubbard_momentum_action.cpp:37-89): the momentum-K block in the
+// orthonormal basis of MomentumPhiGramBasis (src/orbit_bloch_phi.cpp:226-355), applied as
+// y = Phi^dag H Phi x by lifting into the full sector, applying the sector operator and projecting
+// back.  The reference seeds Phi with every member of every orbit whose stabiliser has a trivial
+// *bosonic* character at K (momentum_phi_seeds, k_basis.hpp:66-84, via compatible_momentum_mask,
+// momentum_sector.hpp:79-97), builds the Bloch sums with the fermionic signs
+// (fill_phi_orbit_bloch_from_seed, orbit_bloch_phi.hpp:36-52) and keeps the range of the Gram
+// matrix.  An orbit's Bloch sum vanishes unless the fermionic signs are trivial on its stabiliser
+// too, and all members of an orbit give the same column up to a phase, so the range is spanned by
+// one normalised Bloch sum per orbit with eta|_Stab = 1 and chi_K|_Stab = 1: those are the columns
+// used here (in the momentum block's pack order).  The reference picks another orthonormal basis
+// of the same space (zheev eigenvectors, which depend on the LAPACK build); spectra, exact lnZ and
+// the FTLM distribution are basis independent.  Orbits with an odd fermionic stabiliser sign drop
+// out of every K, as in the reference.
+struct GramBlock : ftlm_op {
+  const Sector* S = nullptr;  // the full sector (not owned)
+  int nt = 0;
+  Phases ph{};
+  DevBuf<int32_t> idx;  // [dim][nt] sector index of T_t rep; bit 31 set where eta(rep, t) = -1
+  DevBuf<double> cs;    // [dim] 1 / sqrt(|G| |Stab|)
+  mutable DevBuf<unsigned char> xf, yf;  // full-sector scratch blocks
+  mutable size_t xf_layout = 0;           // bytes per state of xf's last clear (0: not cleared)
+};
+
+namespace {
+
+__global__ void k_gram_flag(const int32_t* rows, int64_t n, const uint16_t* stab, const uint16_t* eta,
+                            uint8_t* flag) {
+  const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int32_t g = rows[r];
+  flag[r] = (stab[g] & eta[g]) == 0 ? 1 : 0;
+}
+
+__global__ void k_gram_images(const int32_t* rows, const int32_t* sel, int64_t dk, const uint32_t* pack,
+                              const uint16_t* stab, LatDev L, uint32_t* img, uint8_t* neg, double* cs) {
+  const int64_t a = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= dk) return;
+  const int32_t g = rows[sel[a]];
+  const uint32_t rep = pack[g], u = rep >> 16, d = rep & 0xFFFFu;
+  cs[a] = 1.0 / sqrt(static_cast<double>(L.nt) * static_cast<double>(__popc(static_cast<uint32_t>(stab[g]))));
+  for (int t = 0; t < L.nt; ++t) {
+    img[a * L.nt + t] = pack_state(tr_bits(u, L, t), tr_bits(d, L, t));
+    neg[a * L.nt + t] = tr_parity(u, d, L, t) ? 1 : 0;
+  }
+}
+
+__global__ void k_gram_sign(int32_t* idx, const uint8_t* neg, int64_t n) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && neg[i]) idx[i] = static_cast<int32_t>(static_cast<uint32_t>(idx[i]) | 0x80000000u);
+}
+
+// xf[T_t rep] = |Stab| c eta_t conj(chi_t) x_a for the vectors r0 .. r0+RB of the block of RT:
+// the Bloch sum reaches each member once per stabiliser element, always with the same coefficient,
+// so a member's amplitude is |Stab| c eta conj(chi) = sqrt(|Stab| / |G|) eta conj(chi) and every
+// write of it stores the whole amplitude.  xf holds SB >= RB vectors per state.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram_lift(int64_t n, int RT, int r0, int RB, int SB, int nt,
+                                                   const int32_t* __restrict__ idx, const double* __restrict__ cs,
+                                                   Phases ph, const typename Cx<T>::v2* __restrict__ x,
+                                                   typename Cx<T>::v2* __restrict__ xf) {
+  using V2 = typename Cx<T>::v2;
+  for (int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; f < n;
+       f += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t a = f / RB;
+    const int rr = static_cast<int>(f - a * RB);
+    const V2 v = x[a * RT + r0 + rr];
+    const double vr = static_cast<double>(v.x), vi = static_cast<double>(v.y);
+    const double c = 1.0 / (static_cast<double>(nt) * cs[a]);  // |Stab| c, with c = 1 / sqrt(|G| |Stab|)
+    for (int t = 0; t < nt; ++t) {
+      const int32_t e = idx[a * nt + t];
+      const double sc = e < 0 ? -c : c;
+      const double kr = sc * ph.chi[t].x, ki = -sc * ph.chi[t].y;  // |Stab| c eta conj(chi)
+      V2 o;
+      o.x = static_cast<T>(kr * vr - ki * vi);
+      o.y = static_cast<T>(kr * vi + ki * vr);
+      xf[static_cast<int64_t>(e & 0x7FFFFFFF) * SB + rr] = o;
+    }
+  }
+}
+
+// y_a = xs_r sum_t c eta_t chi_t yf[T_t rep] (= Phi^dag yf) for r = r0 .. r0+RB, with <xs x | y>
+// partials per block in the layout of the sector applies.  The block size is a multiple of RB;
+// yf holds SB >= RB vectors per state.
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram_project(int64_t n, int RT, int r0, int RB, int SB, int nt,
+                                                      const int32_t* __restrict__ idx, const double* __restrict__ cs,
+                                                      Phases ph, const typename Cx<T>::v2* __restrict__ yf,
+                                                      typename Cx<T>::v2* __restrict__ y,
+                                                      const typename Cx<T>::v2* __restrict__ x,
+                                                      const double* __restrict__ xs, double2* __restrict__ dotp) {
+  using V2 = typename Cx<T>::v2;
+  const int nthr = blockDim.x, rr = threadIdx.x % RB;
+  const double sx = xs ? xs[r0 + rr] : 1.0;
+  double dre = 0.0, dim = 0.0;
+  for (int64_t f = static_cast<int64_t>(blockIdx.x) * nthr + threadIdx.x; f < n;
+       f += static_cast<int64_t>(gridDim.x) * nthr) {
+    const int64_t a = f / RB;
+    const double c = cs[a];
+    double ar = 0.0, ai = 0.0;
+    for (int t = 0; t < nt; ++t) {
+      const int32_t e = idx[a * nt + t];
+      const double sc = e < 0 ? -c : c;
+      const double kr = sc * ph.chi[t].x, ki = sc * ph.chi[t].y;  // c eta chi
+      const V2 w = yf[static_cast<int64_t>(e & 0x7FFFFFFF) * SB + rr];
+      const double wr = static_cast<double>(w.x), wi = static_cast<double>(w.y);
+      ar += kr * wr - ki * wi;
+      ai += kr * wi + ki * wr;
+    }
+    V2 o;
+    o.x = static_cast<T>(sx * ar);
+    o.y = static_cast<T>(sx * ai);
+    const int64_t g = a * RT + r0 + rr;
+    y[g] = o;
+    if (dotp) {
+      const V2 q = x[g];
+      const double qr = sx * static_cast<double>(q.x), qi = sx * static_cast<double>(q.y);
+      dre += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+      dim += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+    }
+  }
+  if (dotp) {
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = dre;
+    red[1][threadIdx.x] = dim;
+    __syncthreads();
+    if (threadIdx.x < RB) {
+      double u = 0.0, v = 0.0;
+      for (int t = threadIdx.x; t < nthr; t += RB) {
+        u += red[0][t];
+        v += red[1][t];
+      }
+      dotp[static_cast<int64_t>(blockIdx.x) * RT + r0 + threadIdx.x] = make_double2(u, v);
+    }
+  }
+}
+
+__global__ void k_gram_zero_d2(double2* dst, int64_t n) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = make_double2(0.0, 0.0);
+}
+
+constexpr int kGramGrid = 2048;
+constexpr int kGramChunk = 32;  // vectors per lift / sector apply / projection round
+
+}  // namespace
+
+// Vectors go through in chunks of at most kGramChunk.  The two full-sector scratch blocks hold SB
+// = min(R, 32) vectors per state for every chunk of an apply (a shorter last chunk leaves its
+// spare vectors unused).  The lift writes the block's member states only, so the others must read
+// zero: the scratch is cleared whenever its per-state layout changes.
+int hv_apply_gram(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
+                  hipStream_t st) {
+  const GramBlock* B = static_cast<const GramBlock*>(op);
+  if (B->dim == 0) return 0;
+  const size_t es = dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2);
+  const int SB = std::min(R, kGramChunk);
+  const size_t fb = static_cast<size_t>(B->S->dim) * SB * es;
+  if (B->xf.n < fb) {
+    B->xf.alloc(fb);
+    B->yf.alloc(fb);
+    B->xf_layout = 0;
+  }
+  if (B->xf_layout != SB * es) {
+    FTLM_HIP_CHECK(hipMemsetAsync(B->xf.p, 0, fb, st));
+    B->xf_layout = SB * es;
+  }
+  if (dotp && R > SB) {  // partial blocks a chunk does not write must read as zero
+    const int64_t nz = static_cast<int64_t>(kGramGrid) * R;
+    hipLaunchKernelGGL(k_gram_zero_d2, dim3(static_cast<unsigned>(std::min<int64_t>(1024, (nz + 255) / 256))),
+                       dim3(256), 0, st, dotp, nz);
+  }
+  int nblk = 0;
+  for (int r0 = 0; r0 < R; r0 += SB) {
+    const int RB = std::min(SB, R - r0);
+    const int64_t n = B->dim * RB;
+    const int thr = (256 / RB) * RB;
+    const int grid = static_cast<int>(std::min<int64_t>(kGramGrid, (n + thr - 1) / thr));
+    nblk = std::max(nblk, grid);
+    if (dtype == FTLM_C64)
+      hipLaunchKernelGGL(k_gram_lift<float>, dim3(grid), dim3(256), 0, st, n, R, r0, RB, SB, B->nt, B->idx.p,
+                         B->cs.p, B->ph, static_cast<const float2*>(x),
+                         static_cast<float2*>(static_cast<void*>(B->xf.p)));
+    else
+      hipLaunchKernelGGL(k_gram_lift<double>, dim3(grid), dim3(256), 0, st, n, R, r0, RB, SB, B->nt, B->idx.p,
+                         B->cs.p, B->ph, static_cast<const double2*>(x),
+                         static_cast<double2*>(static_cast<void*>(B->xf.p)));
+    check_launch("k_gram_lift");
+    hv_apply(B->S, B->xf.p, B->yf.p, SB, dtype, nullptr, nullptr, st);
+    if (dtype == FTLM_C64)
+      hipLaunchKernelGGL(k_gram_project<float>, dim3(grid), dim3(thr), 0, st, n, R, r0, RB, SB, B->nt, B->idx.p,
+                         B->cs.p, B->ph, static_cast<const float2*>(static_cast<const void*>(B->yf.p)),
+                         static_cast<float2*>(y), static_cast<const float2*>(x), xs, dotp);
+    else
+      hipLaunchKernelGGL(k_gram_project<double>, dim3(grid), dim3(thr), 0, st, n, R, r0, RB, SB, B->nt, B->idx.p,
+                         B->cs.p, B->ph, static_cast<const double2*>(static_cast<const void*>(B->yf.p)),
+                         static_cast<double2*>(y), static_cast<const double2*>(x), xs, dotp);
+    check_launch("k_gram_project");
+  }
+  return nblk;
+}
+
 }  // namespace ftlm_hip
 
 using namespace ftlm_hip;
@@ -1200,6 +1404,59 @@ int ftlm_hip_kblock_create(int Lx, int Ly, int n_up, int n_dn, int kx, int ky, d
       B->G = build_orbit_graph(Lx, Ly, n_up, n_dn, hops, n_hops, pairs, n_pairs, U, V);
     }
     build_kblock(B.get(), tol);
+    *out = B.release();
+  });
+}
+
+int ftlm_hip_kgram_create(const ftlm_op_t* sector, const ftlm_op_t* kblock, ftlm_op_t** out) {
+  return guarded([&] {
+    FTLM_REQUIRE(out != nullptr, FTLM_EINVAL, "null output handle");
+    *out = nullptr;
+    FTLM_REQUIRE(sector != nullptr && sector->kind == kSector, FTLM_EINVAL, "not a full-sector operator");
+    const KBlock* K = as_kblock(kblock);
+    const Sector* S = static_cast<const Sector*>(sector);
+    const OrbitGraph& G = *K->G;
+    FTLM_REQUIRE(S->n_sites == G.lx * G.ly && S->n_up == G.n_up && S->n_dn == G.n_dn, FTLM_EINVAL,
+                 "the sector and the momentum block belong to different (lattice, N_up, N_dn)");
+    FTLM_REQUIRE(S->dim < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "sector too large for 32-bit state indices");
+    FTLM_HIP_CHECK(hipSetDevice(K->device));
+    auto B = std::make_unique<GramBlock>();
+    B->kind = kGram;
+    B->device = K->device;
+    B->S = S;
+    B->nt = G.L.nt;
+    B->ph = K->ph;
+    const hipStream_t st = nullptr;
+    const int64_t nk = K->dim;
+    int32_t dk = 0;
+    DevBuf<int32_t> sel(std::max<int64_t>(nk, 1)), nsel(1);
+    if (nk > 0) {
+      DevBuf<uint8_t> flag(static_cast<size_t>(nk));
+      hipLaunchKernelGGL(k_gram_flag, dim3(grid_for(nk, 256)), dim3(256), 0, st, K->rows.p, nk, G.stab.p, G.eta.p,
+                         flag.p);
+      check_launch("k_gram_flag");
+      hipcub::CountingInputIterator<int32_t> it(0);
+      const int nn = static_cast<int>(nk);
+      cub_call([&](void* t, size_t& b, hipStream_t s) {
+        return hipcub::DeviceSelect::Flagged(t, b, it, flag.p, sel.p, nsel.p, nn, s);
+      }, st);
+      FTLM_HIP_CHECK(hipMemcpy(&dk, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    B->dim = dk;
+    const int64_t nimg = static_cast<int64_t>(dk) * B->nt;
+    B->idx.alloc(static_cast<size_t>(std::max<int64_t>(nimg, 1)));
+    B->cs.alloc(static_cast<size_t>(std::max<int32_t>(dk, 1)));
+    if (dk > 0) {
+      DevBuf<uint32_t> img(static_cast<size_t>(nimg));
+      DevBuf<uint8_t> neg(static_cast<size_t>(nimg));
+      hipLaunchKernelGGL(k_gram_images, dim3(grid_for(dk, 256)), dim3(256), 0, st, K->rows.p, sel.p,
+                         static_cast<int64_t>(dk), G.pack.p, G.stab.p, G.L, img.p, neg.p, B->cs.p);
+      check_launch("k_gram_images");
+      sector_rank_packs(S, img.p, nimg, B->idx.p, st);
+      hipLaunchKernelGGL(k_gram_sign, dim3(grid_for(nimg, 256)), dim3(256), 0, st, B->idx.p, neg.p, nimg);
+      check_launch("k_gram_sign");
+    }
+    FTLM_HIP_CHECK(hipDeviceSynchronize());
     *out = B.release();
   });
 }

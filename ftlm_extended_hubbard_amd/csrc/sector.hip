@@ -174,6 +174,14 @@ __global__ void k_diag_tables(const uint32_t* masks, int D, const int2* pairs, i
   }
 }
 
+// Sector index of packed states (u << 16) | d of this sector (momentum Gram blocks).
+__global__ void k_rank_packs(const uint32_t* packs, int64_t cnt, int n, int nu, int nd, int Ddn, int32_t* out) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  const uint32_t p = packs[i];
+  out[i] = static_cast<int32_t>(lex_rank(p >> 16, n, nu) * Ddn + lex_rank(p & 0xFFFFu, n, nd));
+}
+
 // index_of with the reference's key semantics: key = (up << n) | dn, decoded back into (u, d).
 __global__ void k_index_of(const uint64_t* up, const uint64_t* dn, int64_t cnt, int n, int nu, int nd, int Ddn,
                            int64_t* out) {
@@ -438,6 +446,18 @@ void geometry(const ftlm_params_t& p, std::vector<ftlm_hop_t>* hops, std::vector
 }  // namespace ftlm_hip
 
 using namespace ftlm_hip;
+
+namespace ftlm_hip {
+// Sector indices of packed states (every state must belong to the sector; its dim < 2^31).
+void sector_rank_packs(const Sector* S, const uint32_t* d_packs, int64_t count, int32_t* d_out, hipStream_t st) {
+  FTLM_REQUIRE(S->dim < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "sector too large for 32-bit state indices");
+  ensure_device_binomials(S->device);
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_rank_packs, dim3(grid_for(count, 256)), dim3(256), 0, st, d_packs, count, S->n_sites, S->n_up,
+                     S->n_dn, S->Ddn, d_out);
+  check_launch("k_rank_packs");
+}
+}  // namespace ftlm_hip
 
 extern "C" {
 

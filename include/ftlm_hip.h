@@ -382,6 +382,21 @@ int ftlm_hip_kblock_index_of_rep(const ftlm_op_t* op, const uint32_t* reps, int6
 int ftlm_hip_kblock_map(const ftlm_op_t* op, const uint32_t* states, int64_t count, int64_t* index, int* dx, int* dy,
                         double* eta);
 
+/* HubbardMomentumBlock (src/hubbard_momentum_action.cpp:37-89) with the Gram basis of
+ * MomentumPhiGramBasis (src/orbit_bloch_phi.cpp:226-355): the momentum-K block of `sector` as
+ * y = Phi^dag H Phi x, applied by lifting into the full sector, the sector apply and a projection
+ * (each apply costs one full-sector apply, as in the reference).  `kblock` is a momentum block of
+ * the same lattice, particle numbers and K (ftlm_hip_kblock_create); it may be destroyed after
+ * this call, `sector` must outlive the result.  The block keeps the reference's subspace: the
+ * Bloch sums of the orbits whose stabiliser has a trivial bosonic character at K
+ * (momentum_phi_seeds, include/ftlm/symmetry/k_basis.hpp:66-84) that survive the Gram rank cut,
+ * i.e. those whose fermionic translation signs are trivial on the stabiliser too; orbits with an
+ * odd stabiliser sign drop out of every K, as in the reference.  The columns are one normalised
+ * Bloch sum per orbit in pack order; the reference's own orthonormal basis of the same space comes
+ * from LAPACK zheev, so block spectra and exact lnZ agree, FTLM samples agree in distribution.
+ * The operator is Hermitian; graph capture and launch-shape tuning do not apply to it. */
+int ftlm_hip_kgram_create(const ftlm_op_t* sector, const ftlm_op_t* kblock, ftlm_op_t** out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,12 +4,15 @@ TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the c
 of bench.py, never by the product package.  ``Oracle`` wraps oracle/libftlm_oracle.so (this
 repository's CPU restatement of the reference hot path); ``Reference`` wraps
 oracle/_ref/libftlm_ref.so (the reference's own sources compiled here, present only where the
-reference checkout was available at build time).
+reference checkout was available at build time).  ``Oracle.kgram_block`` is a numpy restatement
+of the reference's Gram momentum blocks over the library's routines, pinned against the
+reference's own printed 3x2 run by tests/test_oracle.py::test_kgram_blocks_reproduce_reference_run.
 """
 
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -278,6 +281,46 @@ class Oracle(_Common):
         H = np.zeros((d, d), np.complex128)
         assert self.lib.orc_dense_hamiltonian(p8, nu, nd, H.view(np.float64).ravel()) == 0
         return H
+
+    def kgram_block(self, p8, nu, nd, kx, ky):
+        """Dense HubbardMomentumBlock matrix Phi^dag H Phi (src/hubbard_momentum_action.cpp:37-89)
+        of momentum (kx, ky), in one orthonormal basis of the reference's Gram subspace
+        (src/orbit_bloch_phi.cpp:226-355): the Bloch sums fill_phi_orbit_bloch_from_seed
+        (include/ftlm/symmetry/orbit_bloch_phi.hpp:36-52) of the orbits that momentum_phi_seeds
+        admits (trivial bosonic character on the stabiliser, k_basis.hpp:66-84 and
+        momentum_sector.hpp:79-97) and whose column survives the Gram rank cut (nonzero, i.e.
+        fermionic signs trivial on the stabiliser).  Pure numpy over the oracle's translation,
+        sign and dense-H routines."""
+        lx, ly = int(p8[0]), int(p8[1])
+        n = lx * ly
+        up, dn = self.basis_masks(n, nu, nd)
+        dim = len(up)
+        if dim == 0:
+            return np.zeros((0, 0), complex)
+        index = {(int(u) << 16) | int(d): i for i, (u, d) in enumerate(zip(up, dn))}
+        H = self.dense_hamiltonian(p8, nu, nd)
+        reps = sorted({int(self.canonical(int(u), int(d), lx, ly)) for u, d in zip(up, dn)})
+        cols = []
+        for rp in reps:
+            u, d = rp >> 16, rp & 0xFFFF
+            stab = [(dx, dy) for dy in range(ly) for dx in range(lx) if int(self.translate(u, d, lx, ly, dx, dy)) == rp]
+            if any((kx * dx * ly + ky * dy * lx) % (lx * ly) for dx, dy in stab):
+                continue  # K not bosonically compatible with the stabiliser: no seed
+            col = np.zeros(dim, complex)
+            c = 1.0 / math.sqrt(n * len(stab))
+            for ey in range(ly):
+                for ex in range(lx):
+                    i = index[int(self.translate(u, d, lx, ly, ex, ey))]
+                    eta = self.translation_sign(u, d, lx, ly, ex, ey)
+                    th = -2.0 * math.pi * (kx * ex / lx + ky * ey / ly)
+                    col[i] += c * eta * complex(math.cos(th), -math.sin(th))
+            nrm = np.linalg.norm(col)
+            if nrm > 1e-8:
+                cols.append(col / nrm)
+        if not cols:
+            return np.zeros((0, 0), complex)
+        Phi = np.array(cols).T
+        return Phi.conj().T @ H @ Phi
 
     def dense_extrema(self, Hreal, cap=1600):
         A = np.ascontiguousarray(Hreal, np.float64)

@@ -11,6 +11,9 @@ Run where /root/reference exists (after `make -C oracle ref refbench`):
   per-block rule (:1072-1079: ED below the cutoff, else ftlm_log_partition_complex on
   apply_orbit_k_block with seed + idx*257 + ky*17 + kx) is evaluated with the reference's orbit
   basis / apply / FTLM (oracle/_ref) and the oracle's dense ED, then combined as the driver does.
+  Sectors outside its orbit-direct gate (N_up or N_dn in {0, N}, or more than 512 states,
+  :800-807) use the Gram block (:1003-1019), restated by the oracle (kgram_block); on 3x2 all
+  of them are below the ED cutoff, so their logZ is the oracle's ED of that block.
 Fixtures are data only (arguments, n(mu), logZ per sector).
 """
 from __future__ import annotations
@@ -86,20 +89,34 @@ def run_rect_k(c):
     for nu in range(n + 1):
         for nd in range(n + 1):
             idx = nu * (n + 1) + nd
+            dim_full = math.comb(n, nu) * math.comb(n, nd)
+            # orbit coordinates only for interior sectors of <= 512 states; HubbardMomentumBlock
+            # (Gram basis) for the others (bench_ftlm_nmu_4x4_k.cpp:800-807, 1003-1019)
+            orbit = 0 < nu < n and 0 < nd < n and dim_full <= 512
             lzs = []
             for ky in range(ly):
                 for kx in range(lx):
-                    dk = len(ref.orbit_basis(lx, ly, nu, nd, kx, ky)[0])
+                    if orbit:
+                        dk = len(ref.orbit_basis(lx, ly, nu, nd, kx, ky)[0])
+                    else:
+                        Hg = orc.kgram_block(p8, nu, nd, kx, ky)
+                        dk = Hg.shape[0]
                     if dk <= 0:
                         continue
                     if dk <= c["ed_cutoff"]:
-                        H = np.zeros((dk, dk), np.complex128)
-                        for j in range(dk):
-                            e = np.zeros(dk, np.complex128)
-                            e[j] = 1.0
-                            H[:, j] = ref.orbit_apply(p8, nu, nd, kx, ky, e)
+                        if orbit:
+                            H = np.zeros((dk, dk), np.complex128)
+                            for j in range(dk):
+                                e = np.zeros(dk, np.complex128)
+                                e[j] = 1.0
+                                H[:, j] = ref.orbit_apply(p8, nu, nd, kx, ky, e)
+                        else:
+                            H = Hg
                         lz = orc.exact_log_partition(H, c["beta"])
                     else:
+                        # FTLM on a Gram block draws its starts in the reference's zheev basis,
+                        # which no other build reproduces: keep the fixture to orbit FTLM blocks
+                        assert orbit, (nu, nd, kx, ky)
                         seed_k = c["seed"] + idx * 257 + ky * 17 + kx
                         lz = ref.orbit_ftlm_log_partition(p8, nu, nd, kx, ky, c["beta"], c["n_rand"], c["lz"], seed_k)
                     lzs.append(lz)

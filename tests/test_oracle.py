@@ -369,3 +369,41 @@ def test_oracle_translate_group_action(oracle):
                     assert t2 == oracle.translate(u, d, lx, ly, dx + 1, dy + 2)
                     imgs.append(t)
             assert oracle.canonical(u, d, lx, ly) == min(imgs)
+
+
+def _golden_lz_close(got, want):
+    """The reference printed logZ with 6 significant digits (and -0 / -inf)."""
+    if not math.isfinite(want):
+        return not math.isfinite(got)
+    return abs(got - want) <= 6e-6 * max(abs(want), 1.0)
+
+
+def test_kgram_blocks_reproduce_reference_run(oracle):
+    """The oracle's restatement of the reference's Gram momentum blocks (kgram_block) against the
+    reference's own 3x2 run (benchmarks/data/n_vs_mu_3x2_beta20_four_solver_monitoring.txt):
+    per sector, the summed block dims (orbits with an odd fermionic stabiliser sign drop out, e.g.
+    12 of 15 states in (0,2), none in (0,6)), and ln sum_K tr e^{-beta H_K} by the reference's
+    capped-Jacobi ED against the printed logZ of every sector the k-gram run solved by ED and of
+    every sector of the ed run (Gram blocks, all by ED)."""
+    import json
+    import os
+
+    d = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "nmu3x2_solvers.json")))
+    p8 = params8(3, 2, 1.0, -0.35, 5.75, 0.9)
+    kg, ed = d["solvers"]["k-gram"]["sectors"], d["solvers"]["ed"]["sectors"]
+    dropped = 0
+    for key, (dim_full, dk_tot, n_ed, n_ftlm, lz) in kg.items():
+        nu, nd = map(int, key.split(","))
+        blocks = [oracle.kgram_block(p8, nu, nd, kx, ky) for ky in range(2) for kx in range(3)]
+        dims = [b.shape[0] for b in blocks]
+        assert sum(dims) == dk_tot == ed[key][1], key
+        assert max(dims) <= 64 or n_ftlm > 0
+        dropped += dim_full - dk_tot
+        lzs = [oracle.exact_log_partition(b, 20.0) for b in blocks if b.shape[0] > 0]
+        ours = -math.inf
+        for v in lzs:
+            ours = float(np.logaddexp(ours, v))
+        assert _golden_lz_close(ours, ed[key][4]), (key, ours, ed[key][4])
+        if n_ftlm == 0:
+            assert _golden_lz_close(ours, lz), (key, ours, lz)
+    assert dropped == 3 * 4 + 9 * 2 + 1 * 2  # (0,2),(2,0),(4,6),(6,4); (2,4),(4,2); (0,6),(6,0)
