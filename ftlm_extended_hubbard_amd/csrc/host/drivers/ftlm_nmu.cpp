@@ -19,7 +19,8 @@
 // The Gram blocks keep the reference's subspace (orbits with trivial bosonic and fermionic
 // stabiliser characters); their orthonormal basis is not the reference's zheev basis, so ED blocks
 // agree exactly and FTLM blocks in distribution.  Device options beyond the reference:
-// --start=gauss|phase, --quad=capped|converged, --dtype=c128|c64 (defaults reproduce the reference).
+// --start=gauss|phase, --quad=capped|converged, --dtype=c128|c64 (defaults reproduce the reference)
+// and --orbit-all=1 (k-orbit: orbit blocks in every sector).
 #include <sys/resource.h>
 
 #include <algorithm>
@@ -310,6 +311,9 @@ int run_k(int argc, char** argv, KSolver solver, bool solver_set) {
   std::cout << "ftlm_random=" << c.n_rand << " lanczos_steps=" << c.lz_steps << "  sectors=" << nsec << "\n";
   if (!c.no_monitor) std::cout << "[monitor] wall time + peak RSS reported at exit (disable with --no-monitor=1)\n";
 
+  // --orbit-all=1 (not in the reference): orbit blocks in every sector of the k-orbit solver.  They
+  // keep every state and skip the full-sector applies of the Gram blocks.
+  const bool orbit_all = arg_int(argc, argv, "orbit-all", 0) != 0;
   if (solver == KSolver::kOrbit)  // bench_ftlm_nmu_4x4_k.cpp:736-740
     std::cout << "[kblock-prototype] orbit-direct: OrbitKBlockBasis + apply_orbit_k_block in orbit coords (no Phi "
                  "Gram, no T bridge); dim_full<="
@@ -345,8 +349,9 @@ int run_k(int argc, char** argv, KSolver solver, bool solver_set) {
       }
       // orbit coordinates only for the k-orbit solver's interior sectors of <= 512 states
       // (bench_ftlm_nmu_4x4_k.cpp:800-807); HubbardMomentumBlock everywhere else (:1003-1019)
-      const bool orbit_direct = solver == KSolver::kOrbit && nu > 0 && nd > 0 && nu < n_sites && nd < n_sites &&
-                                dim_full <= kOrbitDirectMaxDim;
+      const bool orbit_direct =
+          solver == KSolver::kOrbit &&
+          (orbit_all || (nu > 0 && nd > 0 && nu < n_sites && nd < n_sites && dim_full <= kOrbitDirectMaxDim));
       if (orbit_direct) {
         ftlm_hip_op_destroy(sector_op);
         sector_op = nullptr;

@@ -178,3 +178,28 @@ def test_driver_default_solver_is_k_gram(tmp_path):
                         text=True, timeout=120)
     assert r2.returncode == 0
     assert (tmp_path / "d.tsv").read_text().splitlines()[2:] == (tmp_path / "g.tsv").read_text().splitlines()[2:]
+
+
+def test_orbit_all_keeps_every_state(oracle, tmp_path):
+    """--solver=k-orbit --orbit-all=1 (beyond the reference): orbit blocks in every sector, so
+    every sector's blocks cover all its states and the all-ED sectors give ln tr e^{-beta H}."""
+    args = [os.path.join(BIN, "ftlm_nmu_rect_k"), "--Lx=3", "--Ly=2", "--beta=1.5", "--n-mu=11", "--no-monitor=1",
+            "--solver=k-orbit", "--orbit-all=1", f"--out={tmp_path / 'o.tsv'}"]
+    say("ftlm_nmu_rect_k --solver=k-orbit --orbit-all=1 3x2")
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    p8 = params8(3, 2, 1.0, -0.35, 5.75, 0.9)
+    seen = 0
+    for line in r.stderr.splitlines():
+        m = SECTOR_RE.match(line.strip())
+        if not m:
+            continue
+        nu, nd, df, dk, ne, nf, lz = m.groups()
+        assert int(df) == int(dk), line
+        seen += 1
+        if int(nf) == 0:
+            w = np.linalg.eigvalsh(oracle.dense_hamiltonian(p8, int(nu), int(nd)))
+            mx = np.max(-1.5 * w)
+            exact = mx + np.log(np.sum(np.exp(-1.5 * w - mx)))
+            assert abs(float(lz) - exact) <= 1e-5 * max(1.0, abs(exact)), line
+    assert seen == 49
