@@ -142,6 +142,26 @@ PYBIND11_MODULE(_ftlm_core, m) {
 
   m.def("tridiagonal_extrema", &ftlm::tridiagonal_extrema);
 
+  // HubbardMomentumBlock counterpart: the Gram-subspace momentum block of `sector` (same lattice,
+  // particle numbers and hoppings) at K, applied through the full sector on the device.
+  py::class_<ftlm::hip::GramBlockOperator>(m, "GramBlockOperator")
+      .def(py::init([](const ftlm::hip::SectorOperator& sector, const ftlm::HubbardParams& p, const ftlm::FockBasis& fb,
+                       ftlm::symmetry::MomentumSector K, const std::vector<ftlm::SpinfulHopping>& h,
+                       const std::vector<ftlm::NearestPair>& pr) {
+             const ftlm::hip::KBlockOperator kb(p, fb, K, h, pr);
+             return new ftlm::hip::GramBlockOperator(sector, kb);
+           }),
+           py::arg("sector"), py::arg("params"), py::arg("fb"), py::arg("K"), py::arg("hoppings"), py::arg("nn_pairs"))
+      .def("dim", &ftlm::hip::GramBlockOperator::dim)
+      .def("apply", [](const ftlm::hip::GramBlockOperator& o, const cvec& x) { return apply_host(o.dim(), x, o.apply_fn()); })
+      .def("lanczos_extrema", [](const ftlm::hip::GramBlockOperator& o, int max_steps, unsigned seed) {
+        return ftlm::lanczos_extrema(o.dim(), o.apply_fn(), max_steps, seed);
+      }, py::arg("max_steps"), py::arg("seed") = 1)
+      .def("ftlm_log_partition_complex", [](const ftlm::hip::GramBlockOperator& o, double beta,
+                                             const ftlm::FtlmParams& par) {
+        return ftlm::ftlm_log_partition_complex(o.dim(), o.apply_fn(), beta, par);
+      });
+
   py::class_<ftlm::symmetry::MomentumSector>(m, "MomentumSector")
       .def(py::init([](int kx, int ky, int lx, int ly) { return ftlm::symmetry::MomentumSector{kx, ky, lx, ly}; }),
            py::arg("kx") = 0, py::arg("ky") = 0, py::arg("lx") = 4, py::arg("ly") = 4)

@@ -395,6 +395,14 @@ KBlockOperator::KBlockOperator(const HubbardParams& p, const FockBasis& fb, symm
   dim_ = static_cast<int>(op_dim(op_.get()));
 }
 
+GramBlockOperator::GramBlockOperator(const SectorOperator& sector, const KBlockOperator& kblock)
+    : sector_(sector.shared_handle()) {
+  ftlm_op* op = nullptr;
+  check(ftlm_hip_kgram_create(sector_.get(), kblock.handle(), &op));
+  op_ = own(op);
+  dim_ = static_cast<int>(op_dim(op_.get()));
+}
+
 }  // namespace hip
 
 // ------------------------------------------------------------------ translations and k-blocks

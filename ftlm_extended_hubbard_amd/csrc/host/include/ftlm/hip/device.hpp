@@ -40,6 +40,7 @@ class SectorOperator {
   ApplyFn apply_fn() const { return DeviceApply{op_}; }
   std::function<void(const double*, double*)> apply_real_fn() const { return DeviceApplyReal{op_}; }
   const ftlm_op* handle() const { return op_.get(); }
+  std::shared_ptr<ftlm_op> shared_handle() const { return op_; }
 
  private:
   int dim_ = 0;
@@ -58,6 +59,23 @@ class KBlockOperator {
 
  private:
   int dim_ = 0;
+  std::shared_ptr<ftlm_op> op_;
+};
+
+// HubbardMomentumBlock (include/ftlm/symmetry/hubbard_momentum_action.hpp:66-113): the momentum
+// block of `sector` in the reference's Gram subspace, applied through the full sector on the
+// device.  `kblock` (same sector and K) supplies the orbits; the block keeps the sector operator
+// alive and does not need `kblock` afterwards.
+class GramBlockOperator {
+ public:
+  GramBlockOperator(const SectorOperator& sector, const KBlockOperator& kblock);
+  int dim() const { return dim_; }
+  ApplyFn apply_fn() const { return DeviceApply{op_}; }
+  const ftlm_op* handle() const { return op_.get(); }
+
+ private:
+  int dim_ = 0;
+  std::shared_ptr<ftlm_op> sector_;
   std::shared_ptr<ftlm_op> op_;
 };
 

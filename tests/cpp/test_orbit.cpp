@@ -1,9 +1,10 @@
 // Drop-in C++ API, orbit k-blocks: the reference's tests/test_orbit_k_block_basis.cpp checks
 // (|A| = |Stab| for surviving orbits, self-map of representatives, map count = sum of surviving
 // orbit sizes, a 3x2 block builds) plus the reference's dimensions 12 and 66 and Hermiticity of
-// the block apply.
+// the block apply, and the Gram blocks (HubbardMomentumBlock counterpart) of two 3x2 sectors.
 
 #include <complex>
+#include <utility>
 #include <vector>
 
 #include "check.hpp"
@@ -14,6 +15,7 @@
 #include "ftlm/symmetry/orbit.hpp"
 #include "ftlm/symmetry/orbit_k_block_basis.hpp"
 #include "ftlm/symmetry/packed_raw_state.hpp"
+#include "ftlm/hip/device.hpp"
 
 using namespace ftlm::symmetry;
 
@@ -84,6 +86,49 @@ int main() {
     for (int i = 0; i < d; ++i)
       for (int j = 0; j < d; ++j) res = std::max(res, std::abs(H[i][j] - std::conj(H[j][i])));
     CHECK(res < 1e-12);
+  }
+  {
+    // HubbardMomentumBlock counterpart: the Gram blocks of 3x2 (2,4) cover 216 of its 225 states
+    // (the reference's k-gram run prints dim_k_total=216) and are Hermitian; a sector without
+    // dropped orbits, (1,2), keeps all 90 states.
+    ftlm::HubbardParams p;
+    p.Lx = 3;
+    p.Ly = 2;
+    p.t = 1.0;
+    p.tp = -0.35;
+    p.U = 5.75;
+    p.V = 0.9;
+    std::vector<ftlm::SpinfulHopping> h;
+    std::vector<ftlm::NearestPair> pr;
+    ftlm::build_hubbard_geometry(p, ftlm::RectLattice{3, 2}, &h, &pr);
+    for (const auto& sec : {std::pair<int, int>{2, 4}, std::pair<int, int>{1, 2}}) {
+      ftlm::FockBasis fb(6, sec.first, sec.second);
+      ftlm::hip::SectorOperator full(p, fb, h, pr);
+      int total = 0;
+      for (int ky = 0; ky < 2; ++ky)
+        for (int kx = 0; kx < 3; ++kx) {
+          const MomentumSector K{kx, ky, 3, 2};
+          ftlm::hip::KBlockOperator kb(p, fb, K, h, pr);
+          ftlm::hip::GramBlockOperator g(full, kb);
+          const int d = g.dim();
+          total += d;
+          if (d == 0) continue;
+          const auto apply = g.apply_fn();
+          std::vector<std::vector<std::complex<double>>> H(d, std::vector<std::complex<double>>(d));
+          std::vector<std::complex<double>> e(d), col(d);
+          for (int j = 0; j < d; ++j) {
+            std::fill(e.begin(), e.end(), std::complex<double>(0.0, 0.0));
+            e[j] = 1.0;
+            apply(e.data(), col.data());
+            for (int i = 0; i < d; ++i) H[i][j] = col[i];
+          }
+          double res = 0.0;
+          for (int i = 0; i < d; ++i)
+            for (int j = 0; j < d; ++j) res = std::max(res, std::abs(H[i][j] - std::conj(H[j][i])));
+          CHECK(res < 1e-12);
+        }
+      CHECK(total == (sec.first == 2 ? 216 : 90));
+    }
   }
   std::cout << "test_orbit ok" << std::endl;
   return 0;

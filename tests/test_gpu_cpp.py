@@ -49,3 +49,24 @@ def test_pybind_hot_path(golden):
     wa = golden["lanczos/c1_2x2/2_2/s7/alpha"]
     n = min(12, len(wa))
     assert np.max(np.abs(np.array(a[:n]) - wa[:n])) < 1e-10
+
+
+def test_pybind_gram_blocks():
+    """_ftlm_core.GramBlockOperator (HubbardMomentumBlock counterpart): the 3x2 (2,4) blocks keep
+    216 of 225 states, as the reference's k-gram run prints, and apply Hermitian blocks."""
+    from ftlm_extended_hubbard_amd import _ftlm_core as c
+
+    say("pybind gram blocks 3x2 (2,4)")
+    p = c.HubbardParams(3, 2, 1.0, -0.35, 5.75, 0.9)
+    hops, pairs = c.build_hubbard_geometry(p, c.RectLattice(3, 2))
+    fb = c.FockBasis(6, 2, 4)
+    full = c.SectorOperator(p, fb, hops, pairs)
+    total = 0
+    for ky in range(2):
+        for kx in range(3):
+            g = c.GramBlockOperator(full, p, fb, c.MomentumSector(kx, ky, 3, 2), hops, pairs)
+            d = g.dim()
+            total += d
+            H = np.array([g.apply(np.eye(d, dtype=complex)[:, j]) for j in range(d)]).T
+            assert np.max(np.abs(H - H.conj().T)) < 1e-12 * max(1.0, np.max(np.abs(H)))
+    assert total == 216
