@@ -162,3 +162,27 @@ def test_bench_2x2_nmu_matches_reference_driver():
     assert int(re.search(r"exact diagonalization: (\d+) eigenlevels", txt).group(1)) == c["levels"]
     rows = [line.split("\t") for line in txt.splitlines() if "\t" in line and not line.startswith("#")]
     assert close([float(a) for a, _ in rows], c["mu"]) and close([float(b) for _, b in rows], c["n"]), rows
+
+
+PUB = json.load(open(os.path.join(ROOT, "tests", "golden", "nmu_published.json")))
+
+
+@pytest.mark.parametrize("lat", ["4x2", "3x3", "2x5"])
+def test_nmu_rect_reproduces_published_run(lat, tmp_path):
+    """The reference's published runs of bench_ftlm_nmu_rect with its defaults (benchmarks/data/
+    n_vs_mu_{4x2,3x3,2x5}_beta20_ftlm.*): every logged sector's logZ and the n(mu) table, to the
+    printed 6 digits, with the same seeds and capped-Jacobi rule.  Reports the device wall time next
+    to the reference's published single-core time."""
+    c = PUB[lat]
+    say("ftlm_nmu_rect published", lat, *c["args"])
+    import time
+
+    t0 = time.perf_counter()
+    mu, n, sectors, stdout = run("ftlm_nmu_rect", c["args"], tmp_path)
+    wall = time.perf_counter() - t0
+    print(f"[case] {lat}: device run {wall:.2f} s (reference {c['reference_wall_time_s']} s)", flush=True)
+    ref = np.array(c["n_mu"])
+    assert close(mu, ref[:, 0], rel=1e-5, absol=1e-4)
+    assert close(n, ref[:, 1], rel=2e-5, absol=1e-12), np.max(np.abs(n - ref[:, 1]))
+    bad = [(k, sectors[k], v[1]) for k, v in c["sectors"].items() if not close(sectors[k], v[1], rel=6e-6)]
+    assert not bad, bad
