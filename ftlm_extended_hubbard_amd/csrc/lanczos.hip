@@ -1065,6 +1065,21 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
       ritz_persistent(ops, n_ops, opts, ritz, weight, used, alpha_out, beta_out, st);
       return;
     }
+    // The recurrences run operator by operator; their tridiagonals are gathered into [M][S] device
+    // arrays (S = n_ops * R samples) and diagonalised together in one launch after the loop (a
+    // per-operator launch leaves the GPU idle: R workgroups each).
+    for (int io = 0; io < n_ops; ++io)
+      FTLM_REQUIRE(ops[io] != nullptr && ops[io]->device == ops[0]->device, FTLM_EINVAL,
+                   "the operators of one call must live on one device");
+    FTLM_HIP_CHECK(hipSetDevice(ops[0]->device));
+    const int64_t S = static_cast<int64_t>(n_ops) * R;
+    DevBuf<double> qa(static_cast<size_t>(M) * S), qb(static_cast<size_t>(M) * S);
+    DevBuf<int> qn(static_cast<size_t>(S)), qnb(static_cast<size_t>(S));
+    FTLM_HIP_CHECK(hipMemsetAsync(qa.p, 0, qa.n * sizeof(double), st));
+    FTLM_HIP_CHECK(hipMemsetAsync(qb.p, 0, qb.n * sizeof(double), st));
+    FTLM_HIP_CHECK(hipMemsetAsync(qn.p, 0, qn.n * sizeof(int), st));
+    FTLM_HIP_CHECK(hipMemsetAsync(qnb.p, 0, qnb.n * sizeof(int), st));
+    std::vector<char> from_krylov(static_cast<size_t>(S), 0);
     for (int io = 0; io < n_ops; ++io) {
       const ftlm_op* op0 = ops[io];
       FTLM_REQUIRE(op0 != nullptr, FTLM_EINVAL, "null operator");
@@ -1111,25 +1126,22 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
           K.run_graph(0, Mop, st);
         else
           K.run(0, Mop, st);
-        // quadrature per sample: alpha/beta are [Mop][Rg] (stride Rg)
-        DevBuf<int> nb(static_cast<size_t>(Rg));
+        // this group's alpha/beta [Mop][Rg] into columns s0 .. s0+Rg of the batch arrays
+        const int64_t s0 = static_cast<int64_t>(io) * R + g0;
         std::vector<int> hu(static_cast<size_t>(Rg)), ha(static_cast<size_t>(Rg));
         FTLM_HIP_CHECK(hipMemcpyAsync(hu.data(), K.used.p, sizeof(int) * Rg, hipMemcpyDeviceToHost, st));
         FTLM_HIP_CHECK(hipMemcpyAsync(ha.data(), K.alive.p, sizeof(int) * Rg, hipMemcpyDeviceToHost, st));
+        FTLM_HIP_CHECK(hipMemcpy2DAsync(qa.p + s0, sizeof(double) * S, K.alpha.p, sizeof(double) * Rg,
+                                        sizeof(double) * Rg, Mop, hipMemcpyDeviceToDevice, st));
+        FTLM_HIP_CHECK(hipMemcpy2DAsync(qb.p + s0, sizeof(double) * S, K.beta.p, sizeof(double) * Rg,
+                                        sizeof(double) * Rg, Mop, hipMemcpyDeviceToDevice, st));
         FTLM_HIP_CHECK(hipStreamSynchronize(st));
         std::vector<int> hnb(static_cast<size_t>(Rg));
         for (int r = 0; r < Rg; ++r) hnb[r] = ha[r] ? hu[r] : hu[r] - 1;
-        nb.upload(hnb, st);
-        DevBuf<double> dr(static_cast<size_t>(Rg) * M), dw(static_cast<size_t>(Rg) * M);
-        FTLM_HIP_CHECK(hipMemsetAsync(dr.p, 0, dr.n * sizeof(double), st));
-        FTLM_HIP_CHECK(hipMemsetAsync(dw.p, 0, dw.n * sizeof(double), st));
-        quadrature_device(K.alpha.p, K.beta.p, K.used.p, nb.p, Rg, Rg, Mop, opts->quad, 0, 1e-14, 1, dr.p, dw.p, M,
-                          st);
-        FTLM_HIP_CHECK(hipMemcpyAsync(rz + static_cast<int64_t>(g0) * M, dr.p, sizeof(double) * Rg * M,
-                                      hipMemcpyDeviceToHost, st));
-        FTLM_HIP_CHECK(hipMemcpyAsync(wt + static_cast<int64_t>(g0) * M, dw.p, sizeof(double) * Rg * M,
-                                      hipMemcpyDeviceToHost, st));
+        FTLM_HIP_CHECK(hipMemcpyAsync(qn.p + s0, hu.data(), sizeof(int) * Rg, hipMemcpyHostToDevice, st));
+        FTLM_HIP_CHECK(hipMemcpyAsync(qnb.p + s0, hnb.data(), sizeof(int) * Rg, hipMemcpyHostToDevice, st));
         std::copy(hu.begin(), hu.end(), us + g0);
+        std::fill(from_krylov.begin() + s0, from_krylov.begin() + s0 + Rg, 1);
         if (alpha_out || beta_out) {
           std::vector<double> a(static_cast<size_t>(Mop) * Rg), b(static_cast<size_t>(Mop) * Rg);
           FTLM_HIP_CHECK(hipMemcpyAsync(a.data(), K.alpha.p, a.size() * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -1142,8 +1154,23 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
               if (beta_out) beta_out[o] = k < Mop ? b[static_cast<size_t>(k) * Rg + r] : 0.0;
             }
         }
-        FTLM_HIP_CHECK(hipStreamSynchronize(st));
+        FTLM_HIP_CHECK(hipStreamSynchronize(st));  // the block's buffers are freed at scope end
       }
+    }
+    if (std::find(from_krylov.begin(), from_krylov.end(), 1) != from_krylov.end()) {
+      DevBuf<double> dr(static_cast<size_t>(S) * M), dw(static_cast<size_t>(S) * M);
+      FTLM_HIP_CHECK(hipMemsetAsync(dr.p, 0, dr.n * sizeof(double), st));
+      FTLM_HIP_CHECK(hipMemsetAsync(dw.p, 0, dw.n * sizeof(double), st));
+      quadrature_device(qa.p, qb.p, qn.p, qnb.p, static_cast<int>(S), static_cast<int>(S), M, opts->quad, 0, 1e-14,
+                        1, dr.p, dw.p, M, st);
+      std::vector<double> hr(static_cast<size_t>(S) * M), hw(static_cast<size_t>(S) * M);
+      FTLM_HIP_CHECK(hipMemcpy(hr.data(), dr.p, sizeof(double) * hr.size(), hipMemcpyDeviceToHost));
+      FTLM_HIP_CHECK(hipMemcpy(hw.data(), dw.p, sizeof(double) * hw.size(), hipMemcpyDeviceToHost));
+      for (int64_t sm = 0; sm < S; ++sm)
+        if (from_krylov[static_cast<size_t>(sm)]) {
+          std::copy(hr.begin() + sm * M, hr.begin() + (sm + 1) * M, ritz + sm * M);
+          std::copy(hw.begin() + sm * M, hw.begin() + (sm + 1) * M, weight + sm * M);
+        }
     }
   });
 }
