@@ -30,21 +30,50 @@ struct RowMax {
   int col;
 };
 
-// Rescan row i of the full row-major matrix: max over j > i, first column on ties.
-__device__ __forceinline__ RowMax scan_row(const double* A, int n, int i) {
-  RowMax r{0.0, -1};
-  for (int j = i + 1; j < n; ++j) {
-    const double v = fabs(A[static_cast<int64_t>(i) * n + j]);
-    if (v > r.v) {
-      r.v = v;
-      r.col = j;
+// Rows of the pivot table to rescan are handled one per wavefront: the lanes stride over the row
+// and combine (larger |a|, then smaller column), which is the serial scan's first strict maximum.
+__device__ __forceinline__ void rescan_rows(const double* A, int n, RowMax* rm, const int* todo, int ntodo) {
+  const int lane = static_cast<int>(threadIdx.x) & 63, wv = static_cast<int>(threadIdx.x) >> 6;
+  const int nw = static_cast<int>(blockDim.x) >> 6;
+  for (int t = wv; t < ntodo; t += nw) {
+    const int i = todo[t];
+    double bv = 0.0;
+    int bc = -1;
+    for (int j = i + 1 + lane; j < n; j += 64) {
+      const double v = fabs(A[static_cast<int64_t>(i) * n + j]);
+      if (v > bv) {
+        bv = v;
+        bc = j;
+      }
     }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+      const double ov = __shfl_xor(bv, o, 64);
+      const int oc = __shfl_xor(bc, o, 64);
+      if (ov > bv || (ov == bv && oc >= 0 && (bc < 0 || oc < bc))) {
+        bv = ov;
+        bc = oc;
+      }
+    }
+    if (lane == 0) rm[i] = RowMax{bv, bc};
   }
-  return r;
+}
+
+// Dynamic LDS of k_jacobi for matrices of at most n rows: v0 [n], row maxima [n], rescan list [n],
+// and the matrix itself [n][n] when it is kept in LDS.
+constexpr int kJacLdsMaxN = 85;  // n * n * 8 + 28 n <= 60 KB
+__host__ __device__ inline size_t jacobi_lds_offset(int n) {
+  return (static_cast<size_t>(n) * (sizeof(double) + sizeof(RowMax) + sizeof(int)) + 15) / 16 * 16;
+}
+inline size_t jacobi_shmem(int n, bool lds) {
+  return jacobi_lds_offset(n) + (lds ? static_cast<size_t>(n) * n * sizeof(double) : 0);
 }
 
 // One tridiagonal per workgroup.  alpha/beta are strided: entry i of sample s at [i*stride + s].
-// n[s] = size (used steps), nb_avail = number of valid beta entries (n-1 or fewer).
+// n[s] = size (used steps), nb_avail = number of valid beta entries (n-1 or fewer).  kLds keeps
+// the matrix in LDS (n <= kJacLdsMaxN), otherwise in the workgroup's slice of `scratch`.  Every
+// rotation runs the same arithmetic and pivot order either way.
+template <bool kLds>
 __global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__ alpha, const double* __restrict__ beta,
                                                       const int* __restrict__ nvec, const int* __restrict__ nbvec,
                                                       int stride, int cap_fixed, double tol, int with_weights,
@@ -55,14 +84,15 @@ __global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__
   const int s = blockIdx.x;
   const int n = nvec[s];
   const int nb = nbvec ? nbvec[s] : (n - 1);
-  double* A = scratch + scratch_per * s;
   extern __shared__ unsigned char smem_raw[];
   double* v0 = reinterpret_cast<double*>(smem_raw);                 // [n]
   RowMax* rm = reinterpret_cast<RowMax*>(v0 + n);                    // [n]
+  int* todo = reinterpret_cast<int*>(rm + n);                        // [n]
+  double* A = kLds ? reinterpret_cast<double*>(smem_raw + jacobi_lds_offset(n)) : scratch + scratch_per * s;
   __shared__ double s_val[kJacBlock / 64];
   __shared__ int s_row[kJacBlock / 64];
   __shared__ double s_c, s_s, s_big;
-  __shared__ int s_p, s_q;
+  __shared__ int s_p, s_q, s_ntodo;
   const int tid = threadIdx.x;
   if (n <= 0) return;
 
@@ -85,7 +115,9 @@ __global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__
   __syncthreads();
   const int cap = cap_fixed > 0 ? cap_fixed : max(3000, 120 * n);
   if (n > 1) {
-    for (int i = tid; i < n; i += blockDim.x) rm[i] = scan_row(A, n, i);
+    for (int i = tid; i < n; i += blockDim.x) todo[i] = i;
+    __syncthreads();
+    rescan_rows(A, n, rm, todo, n);
     __syncthreads();
     for (int it = 0; it < cap; ++it) {
       // ---- pivot: largest row maximum, smallest row on ties
@@ -131,6 +163,7 @@ __global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__
         s_big = v;
         s_p = p;
         s_q = q;
+        s_ntodo = 0;
         if (v >= tol) {
           const double app = A[static_cast<int64_t>(p) * n + p];
           const double aqq = A[static_cast<int64_t>(q) * n + q];
@@ -175,16 +208,17 @@ __global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__
         A[static_cast<int64_t>(q) * n + q] = nqq;
       }
       __syncthreads();
-      // ---- row maxima: rows p, q fully; rows i < q see new values in columns p (if i < p), q
+      // ---- row maxima: rows p, q fully; rows i < q see new values in columns p (if i < p), q;
+      // rows whose maximum sat in column p or q are rescanned (by wavefronts, below)
       for (int i = tid; i < n; i += blockDim.x) {
         if (i == p || i == q) {
-          rm[i] = scan_row(A, n, i);
+          todo[atomicAdd(&s_ntodo, 1)] = i;
           continue;
         }
         if (i > q && i > p) continue;
         RowMax r = rm[i];
         if (r.col == p || r.col == q) {
-          rm[i] = scan_row(A, n, i);
+          todo[atomicAdd(&s_ntodo, 1)] = i;
           continue;
         }
         const int c1 = p < q ? p : q, c2 = p < q ? q : p;
@@ -205,12 +239,30 @@ __global__ __launch_bounds__(kJacBlock) void k_jacobi(const double* __restrict__
         rm[i] = r;
       }
       __syncthreads();
+      rescan_rows(A, n, rm, todo, s_ntodo);
+      __syncthreads();
     }
   }
   for (int k = tid; k < n; k += blockDim.x) {
     ritz[static_cast<int64_t>(s) * out_stride + k] = A[static_cast<int64_t>(k) * n + k];
     if (with_weights) weight[static_cast<int64_t>(s) * out_stride + k] = v0[k] * v0[k];
   }
+}
+
+// k_jacobi over `grid` problems of at most max_n rows, in LDS when they fit.
+void launch_jacobi(int grid, int max_n, hipStream_t st, const double* alpha, const double* beta, const int* nvec,
+                   const int* nbvec, int stride, int cap_fixed, double tol, int with_weights, double* scratch,
+                   int64_t scratch_per, double* ritz, double* weight, int out_stride, const double* dense) {
+  const bool lds = max_n <= kJacLdsMaxN;
+  const size_t shmem = jacobi_shmem(max_n, lds);
+  FTLM_REQUIRE(shmem <= 60 * 1024, FTLM_EUNSUPPORTED, "matrix too large for the Jacobi kernel");
+  if (lds)
+    hipLaunchKernelGGL(k_jacobi<true>, dim3(grid), dim3(kJacBlock), shmem, st, alpha, beta, nvec, nbvec, stride,
+                       cap_fixed, tol, with_weights, scratch, scratch_per, ritz, weight, out_stride, dense);
+  else
+    hipLaunchKernelGGL(k_jacobi<false>, dim3(grid), dim3(kJacBlock), shmem, st, alpha, beta, nvec, nbvec, stride,
+                       cap_fixed, tol, with_weights, scratch, scratch_per, ritz, weight, out_stride, dense);
+  check_launch("k_jacobi");
 }
 
 // Converged eigen-decomposition of a symmetric tridiagonal (implicit QL, Wilkinson shift),
@@ -466,11 +518,8 @@ void quadrature_device(const double* d_alpha, const double* d_beta, const int* d
   }
   const int64_t per = static_cast<int64_t>(max_n) * max_n;
   DevBuf<double> scratch(static_cast<size_t>(per) * nsamp);
-  const size_t shmem = static_cast<size_t>(max_n) * (sizeof(double) + sizeof(RowMax));
-  FTLM_REQUIRE(shmem <= 60 * 1024, FTLM_EUNSUPPORTED, "tridiagonal too large for the Jacobi kernel");
-  hipLaunchKernelGGL(k_jacobi, dim3(nsamp), dim3(kJacBlock), shmem, st, d_alpha, d_beta, d_n, d_nb, stride, cap_fixed,
-                     tol, with_weights, scratch.p, per, d_ritz, d_weight, out_stride, nullptr);
-  check_launch("k_jacobi");
+  launch_jacobi(nsamp, max_n, st, d_alpha, d_beta, d_n, d_nb, stride, cap_fixed, tol, with_weights, scratch.p, per,
+                d_ritz, d_weight, out_stride, nullptr);
   FTLM_HIP_CHECK(hipStreamSynchronize(st));
 }
 
@@ -523,12 +572,9 @@ int ftlm_hip_exact_log_partition(const ftlm_op_t* op, const double* betas, int n
     check_launch("k_embed_real");
     DevBuf<int> dn(1);
     dn.upload(&m, 1);
-    const size_t shmem = static_cast<size_t>(m) * (sizeof(double) + sizeof(RowMax));
     // jacobi_symmetric_all(R, max(5000, 150 m), 1e-13) (bench_ftlm_nmu_4x4_k.cpp:182-230, 324)
-    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(kJacBlock), shmem, nullptr, nullptr, nullptr, dn.p, nullptr, 1,
-                       std::max(5000, 150 * m), 1e-13, 0, scratch.p, static_cast<int64_t>(m) * m, lam.p, nullptr, m,
-                       Rm.p);
-    check_launch("k_jacobi(dense)");
+    launch_jacobi(1, m, nullptr, nullptr, nullptr, dn.p, nullptr, 1, std::max(5000, 150 * m), 1e-13, 0, scratch.p,
+                  static_cast<int64_t>(m) * m, lam.p, nullptr, m, Rm.p);
     DevBuf<double> dbt(static_cast<size_t>(nb)), dl(static_cast<size_t>(nb));
     dbt.upload(betas, static_cast<size_t>(nb));
     hipLaunchKernelGGL(k_ed_lnz, dim3(grid_for(nb, 64)), dim3(64), 0, nullptr, lam.p, m, dbt.p, nb, dl.p);
@@ -542,16 +588,15 @@ int ftlm_hip_dense_jacobi(const double* A, int n, int max_rotations, double tol,
                           double* hi) {
   return guarded([&] {
     FTLM_REQUIRE(A != nullptr && n >= 1 && max_rotations >= 1, FTLM_EINVAL, "bad arguments");
-    const size_t shmem = static_cast<size_t>(n) * (sizeof(double) + sizeof(RowMax));
-    FTLM_REQUIRE(shmem <= 60 * 1024 && n <= 2048, FTLM_EUNSUPPORTED, "matrix too large for the Jacobi kernel");
+    FTLM_REQUIRE(jacobi_shmem(n, n <= kJacLdsMaxN) <= 60 * 1024 && n <= 2048, FTLM_EUNSUPPORTED,
+                 "matrix too large for the Jacobi kernel");
     const int64_t nn = static_cast<int64_t>(n) * n;
     DevBuf<double> dA(static_cast<size_t>(nn)), scratch(static_cast<size_t>(nn)), lam(static_cast<size_t>(n));
     dA.upload(A, static_cast<size_t>(nn));
     DevBuf<int> dn(1);
     dn.upload(&n, 1);
-    hipLaunchKernelGGL(k_jacobi, dim3(1), dim3(kJacBlock), shmem, nullptr, nullptr, nullptr, dn.p, nullptr, 1,
-                       max_rotations, tol, 0, scratch.p, nn, lam.p, nullptr, n, dA.p);
-    check_launch("k_jacobi(dense)");
+    launch_jacobi(1, n, nullptr, nullptr, nullptr, dn.p, nullptr, 1, max_rotations, tol, 0, scratch.p, nn, lam.p,
+                  nullptr, n, dA.p);
     std::vector<double> d(static_cast<size_t>(n));
     FTLM_HIP_CHECK(hipMemcpy(d.data(), lam.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     if (diag) std::copy(d.begin(), d.end(), diag);
