@@ -64,6 +64,44 @@ def kernel_roofline(F, L, op, R, dtype, torch, reps=5):
     return sum(rows) / reps, sum(cols) / reps, R * dim * 2 * s
 
 
+def copy_ceiling(torch, gib=4, reps=5):
+    """Device-to-device copy of `gib` GiB (torch's copy kernel), median of `reps` with events: the
+    practical streaming ceiling of this GPU, in GB/s of bytes moved (read + write)."""
+    n = (gib << 30) // 4
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        b.copy_(a)
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) / 1e3)
+    del a, b
+    torch.cuda.empty_cache()
+    t = sorted(ts)[len(ts) // 2]
+    return 2 * (gib << 30) / t / 1e9
+
+
+def hv_flops(F, R):
+    """FP operations of one H*V on R vectors of the 4x4 (8,8) sector: per vector, 8 per complex and
+    4 per real off-diagonal term, 2 per diagonal element.  Terms per merged directed bond of one
+    spin: C(n-2, N-1) strings allow it, times the other spin's D strings."""
+    import collections
+
+    hops, _ = F.build_hubbard_geometry(F.HubbardParams(**MODEL))
+    merged = collections.defaultdict(complex)
+    for h in hops:
+        if h["from"] != h["to"]:
+            merged[(int(h["from"]), int(h["to"]), int(h["spin"]))] += complex(h["re"], h["im"])
+    n, D = MODEL["Lx"] * MODEL["Ly"], 12870
+    per_bond = math.comb(n - 2, NU - 1) * D
+    off = sum((8 if abs(c.imag) > 0 else 4) * per_bond for c in merged.values() if abs(c) > 0)
+    return R * (off + 2 * D * D)
+
+
 def profile_hv(F, L, torch, R, shape=None):
     """Workload of the committed rocprofv3 summaries: 3 H*V launches (R complex64 vectors) on the
     4x4 (8,8) sector, and one copy of 4 GiB (the FETCH_SIZE / WRITE_SIZE calibration kernel).
@@ -619,6 +657,49 @@ def cpu_baseline(budget_s=12.0):
     }
 
 
+def cpu_baseline_threads(budget_s=8.0, threads=None):
+    """The same oracle push-form complex128 H*v on all host cores: T threads (ctypes releases the
+    GIL), each pushing its own column slice into a private y (push form writes scattered rows),
+    for about budget_s of wall time; columns per second scaled to one H*v of the sector, plus the
+    single-thread vector passes divided by T.  Returns sample-steps/s."""
+    import threading
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from pyoracle import Oracle, params8
+
+    T = threads or max(1, min(16, os.cpu_count() or 1))
+    orc = Oracle()
+    p8 = params8(MODEL["Lx"], MODEL["Ly"], MODEL["t"], MODEL["tp"], MODEL["U"], MODEL["V"], MODEL["phi_x"],
+                 MODEL["phi_y"])
+    dim = 12870 * 12870
+    x = np.random.default_rng(0).standard_normal(2 * dim).view(np.complex128)
+    chunk = 4096
+    done = [0] * T
+    stop = time.perf_counter() + budget_s
+
+    def work(k):
+        y = np.zeros(dim, np.complex128)
+        c0 = k * (dim // T)
+        while time.perf_counter() < stop:
+            orc.apply_cols(p8, NU, ND, x, y, c0, c0 + chunk)
+            c0 += chunk
+            done[k] += chunk
+
+    t0 = time.perf_counter()
+    ths = [threading.Thread(target=work, args=(k,)) for k in range(T)]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    wall = time.perf_counter() - t0
+    t_hv = wall / sum(done) * dim
+    return {"value": 1.0 / t_hv, "unit": "sample-steps/s (H*v only)", "cores": T, "kind": "port",
+            "sample": f"{T} threads x oracle push-form complex128 H*v, {sum(done)} columns of the 4x4 (8,8) sector "
+                      f"in {wall:.1f} s wall, scaled by dim: {t_hv:.0f} s per H*v"}
+
+
 def ftlm_lnz(F, L, op, R, M, dtype, betas):
     """lnZ(beta) from one R-sample FTLM run, with the converged (QL) Gauss rule and with the
     reference's capped Jacobi applied to the same tridiagonals."""
@@ -747,6 +828,18 @@ def main():
     tr1, tc1, B1 = kernel_roofline(F, L, op, 1, L.FTLM_C128, torch)
     log(f"H*V c128 R=1: rows {tr1 * 1e3:.3f} ms + columns {tc1 * 1e3:.3f} ms, "
         f"{B1 / (tr1 + tc1) / 1e9:.0f} GB/s algorithmic")
+    # the survey's block H*V table (R in {1, 4, 8}, complex128 and complex64) and the live copy ceiling
+    hv_table = []
+    for Rt in (1, 4, 8):
+        for dname, dt, sz in (("c128", L.FTLM_C128, 16), ("c64", L.FTLM_C64, 8)):
+            a_, b_, Bt = kernel_roofline(F, L, op, Rt, dt, torch)
+            hv_table.append({"R": Rt, "dtype": dname, "ms": round((a_ + b_) * 1e3, 3),
+                             "GB_s_algorithmic": round(Bt / (a_ + b_) / 1e9, 1),
+                             "ideal_ms_at_peak": round(Bt / (HBM_PEAK_GBS * 1e9) * 1e3, 3)})
+            log(f"H*V R={Rt} {dname}: {hv_table[-1]}")
+    copy_gbs = copy_ceiling(torch)
+    log(f"device copy ceiling: {copy_gbs:.0f} GB/s")
+    fl = hv_flops(F, R)
     rb = op.get_shape(R, L.FTLM_C64)["rb"] or 1 << (min(R, 16).bit_length() - 1)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
@@ -754,6 +847,10 @@ def main():
                 "launch_ms": round(t_k * 1e3, 4),
                 "pass_ms": {"k_hv_dn": round(tr * 1e3, 4), "k_hv_up": round(tc * 1e3, 4)},
                 "algorithmic_bytes_per_launch": B,
+                "copy_ceiling_GB_s": round(copy_gbs, 1), "frac_of_copy_ceiling": round(achieved / copy_gbs, 4),
+                "flops_per_launch": fl, "achieved_TFLOP_s": round(fl / t_k / 1e12, 2),
+                "vector_peaks_TFLOP_s": {"fp32": 157.3, "fp64": 78.6},
+                "hv_table": hv_table,
                 "plain_c128_r1": {"launch_ms": round((tr1 + tc1) * 1e3, 4),
                                   "pass_ms": {"k_hv_dn": round(tr1 * 1e3, 4), "k_hv_up": round(tc1 * 1e3, 4)},
                                   "achieved": round(B1 / (tr1 + tc1) / 1e9, 1)}}
@@ -776,8 +873,10 @@ def main():
         log(f"dlnZ c64 vs fp64 (M={M}, converged): {dlnz['abs_diff']}; capped: {dlnz['capped_abs_diff']}")
 
     cpu = None if args.no_cpu else cpu_baseline()
+    cpu_all = None if args.no_cpu else cpu_baseline_threads()
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3e} sample-steps/s ({cpu['sample']})")
+        log(f"cpu baseline, all cores: {cpu_all['value']:.3e} ({cpu_all['sample']})")
 
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "sample-steps/s", "n_gpus": 1, "steps": K,
@@ -788,6 +887,7 @@ def main():
                    "dim": dim, "hamiltonian": MODEL, "R": R, "setup_s": round(t_setup, 2)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "cpu_baseline_all_cores": cpu_all,
         "plain_fp64": plain,
         "batched_vs_plain": None if plain is None else round(value / plain["value"], 3),
         "dlnZ": dlnz,
