@@ -203,3 +203,80 @@ def test_orbit_all_keeps_every_state(oracle, tmp_path):
             exact = mx + np.log(np.sum(np.exp(-1.5 * w - mx)))
             assert abs(float(lz) - exact) <= 1e-5 * max(1.0, abs(exact)), line
     assert seen == 49
+
+
+# ---- the reference's Gram-path tests (they need LAPACK there), on the device Gram blocks
+
+def _dense_from(op):
+    d = op.dim()
+    return np.stack([op.apply(np.eye(d, dtype=complex)[:, j]) for j in range(d)], axis=1) if d else np.zeros((0, 0))
+
+
+def test_single_particle_dispersion_4x4():
+    """tests/test_hubbard_momentum_action.cpp:29-66 and test_lanczos_engine.cpp:34-52: one up
+    electron on 4x4 (t=1, t'=U=V=0): every momentum block has one state with energy
+    -2t(cos(pi kx/2) + cos(pi ky/2)) (1e-10), and 24-step Lanczos extrema equal it (1e-8), for the
+    Gram and the orbit blocks."""
+    p = api.HubbardParams(4, 4, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0)
+    sector = api.HubbardOperator.from_params(p, 1, 0)
+    say("single-particle dispersion 4x4")
+    for ky in range(4):
+        for kx in range(4):
+            K = api.MomentumSector(kx, ky, 4, 4)
+            eps = -2.0 * (math.cos(0.5 * math.pi * kx) + math.cos(0.5 * math.pi * ky))
+            for op in (api.KGramOperator.from_params(p, 1, 0, K, sector=sector),
+                       api.KBlockOperator.from_params(p, 1, 0, K)):
+                assert op.dim() == 1
+                y = op.apply(np.array([1.0 + 0j]))
+                assert abs(y[0].real - eps) < 1e-10 and abs(y[0].imag) < 1e-10
+                ex = api.lanczos_extrema(op, 24, 3 + kx + 4 * ky)
+                assert abs(ex.min_eval - eps) < 1e-8 and abs(ex.max_eval - eps) < 1e-8
+
+
+def test_two_level_gamma_block():
+    """tests/test_lanczos_engine.cpp:54-70: t=0, U=1, one up and one down electron on 4x4: the
+    Gamma block holds the doubly occupied orbit (energy U) and the others (energy 0), so its
+    Lanczos extrema are 0 and 1 (1e-8)."""
+    p = api.HubbardParams(4, 4, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0)
+    say("two-level Gamma block 4x4 (1,1)")
+    g = api.KGramOperator.from_params(p, 1, 1, api.MomentumSector(0, 0, 4, 4))
+    ex = api.lanczos_extrema(g, 32, 99)
+    assert abs(ex.min_eval - 0.0) < 1e-8 and abs(ex.max_eval - 1.0) < 1e-8
+
+
+@pytest.mark.parametrize("lx,ly,tp,U,V,nu,nd,kx,ky", [
+    (2, 2, -0.2, 4.0, 0.5, 1, 1, 0, 0), (2, 2, -0.2, 4.0, 0.5, 1, 1, 1, 0), (2, 2, -0.2, 4.0, 0.5, 2, 1, 1, 1),
+    (3, 2, -0.35, 5.75, 0.9, 2, 3, 1, 0)])
+def test_orbit_block_equals_gram_block(lx, ly, tp, U, V, nu, nd, kx, ky):
+    """tests/test_orbit_k_block_apply_vs_production.cpp:135-207: the orbit block and the Gram block
+    are one operator in two bases (the reference compares them in the full space); here their
+    dimensions and spectra agree (1e-10)."""
+    p = api.HubbardParams(lx, ly, 1.0, tp, U, V, 0.0, 0.0)
+    K = api.MomentumSector(kx, ky, lx, ly)
+    say("orbit vs gram", lx, ly, nu, nd, kx, ky)
+    o = api.KBlockOperator.from_params(p, nu, nd, K)
+    g = api.KGramOperator.from_params(p, nu, nd, K)
+    assert o.dim() == g.dim() > 0
+    wo = np.sort(np.linalg.eigvals(_dense_from(o)).real)
+    wg = np.linalg.eigvalsh(_dense_from(g))
+    assert np.max(np.abs(wo - wg)) < 1e-10 * max(1.0, np.max(np.abs(wg)))
+
+
+def test_momentum_trace_2x2():
+    """tests/test_momentum_trace_2x2.cpp:77-125: on 2x2 (t=1, t'=-0.35, U=5.75, V=0.9), for every
+    sector with a momentum block, the Gram blocks' traces sum to tr H of the sector (1e-8)."""
+    p = api.HubbardParams(2, 2, 1.0, -0.35, 5.75, 0.9, 0.0, 0.0)
+    say("momentum trace 2x2")
+    checked = 0
+    for nu in range(5):
+        for nd in range(5):
+            sector = api.HubbardOperator.from_params(p, nu, nd)
+            blocks = [api.KGramOperator.from_params(p, nu, nd, api.MomentumSector(kx, ky, 2, 2), sector=sector)
+                      for ky in range(2) for kx in range(2)]
+            if sum(b.dim() for b in blocks) == 0:
+                continue
+            tr_k = sum(np.trace(_dense_from(b)).real for b in blocks if b.dim())
+            tr = np.trace(_dense_from(sector)).real
+            assert abs(tr - tr_k) < 1e-8, (nu, nd, tr, tr_k)
+            checked += 1
+    assert checked >= 20
