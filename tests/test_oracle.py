@@ -407,3 +407,75 @@ def test_kgram_blocks_reproduce_reference_run(oracle):
         if n_ftlm == 0:
             assert _golden_lz_close(ours, lz), (key, ours, lz)
     assert dropped == 3 * 4 + 9 * 2 + 1 * 2  # (0,2),(2,0),(4,6),(6,4); (2,4),(4,2); (0,6),(6,0)
+
+
+def _translation_matrix(oracle, lx, ly, up, dn, index, dx, dy):
+    """Dense fermionic translation T(dx, dy) on one (N_up, N_dn) sector: column i holds
+    eta * |T s_i>, from the oracle's translate / translation_sign."""
+    dim = len(up)
+    T = np.zeros((dim, dim))
+    for i, (u, d) in enumerate(zip(up, dn)):
+        j = index[int(oracle.translate(int(u), int(d), lx, ly, dx, dy))]
+        T[j, i] = oracle.translation_sign(int(u), int(d), lx, ly, dx, dy)
+    return T
+
+
+@pytest.mark.parametrize("lx,ly", [(2, 2), (3, 2)])
+def test_kgram_dims_against_dense_translation_projector(oracle, lx, ly):
+    """Mirrors tests/test_k_gram_mismatch_scan.cpp: the Gram block dimension of every sector and
+    momentum against rank(P_K) of the dense translation projector P_K = (1/N) sum_t e^{-iK.t} T(t).
+    What the mathematics fixes and the projector does not take from the Gram code: T(t) is a
+    unitary representation of Z_lx x Z_ly that commutes with H; P_K are orthogonal projectors
+    that sum to the identity, so sum_K rank(P_K) = dim; the Gram subspace lies inside range(P_K),
+    so k_out <= rank(P_K), and where they are equal the Gram block has the spectrum of H on
+    range(P_K).  Where they differ are the orbits whose bosonic character and fermionic sign are
+    both non-trivial on the stabiliser (the product is trivial): the reference's seed rule leaves
+    them out.  On 3x2 these are the sectors and counts its own 3x2 run shows (3 of 15 states in
+    (0,2), the single state of (0,6), and their particle-hole / spin images; (2,4) and (4,2) are
+    past the scan's dim <= 64); on 2x2 every state of (0,2), (2,0), (2,4) and (4,2) is left out,
+    since each two-fermion configuration there is swapped onto itself by one translation."""
+    n = lx * ly
+    p8 = params8(lx, ly, 1.0, -0.35, 5.75, 0.9)
+    shifts = [(dx, dy) for dy in range(ly) for dx in range(lx)]
+    missing = {}
+    for nu in range(n + 1):
+        for nd in range(n + 1):
+            up, dn = oracle.basis_masks(n, nu, nd)
+            dim = len(up)
+            if dim == 0 or dim > 64:
+                continue
+            index = {(int(u) << 16) | int(d): i for i, (u, d) in enumerate(zip(up, dn))}
+            H = oracle.dense_hamiltonian(p8, nu, nd)
+            T = {s: _translation_matrix(oracle, lx, ly, up, dn, index, *s) for s in shifts}
+            for (ax, ay), Ta in T.items():
+                assert np.array_equal(Ta.T @ Ta, np.eye(dim))
+                assert np.abs(Ta @ H - H @ Ta).max() < 1e-12, (nu, nd, ax, ay)
+                for (bx, by), Tb in T.items():
+                    assert np.array_equal(Ta @ Tb, T[(ax + bx) % lx, (ay + by) % ly])
+            full = np.linalg.eigvalsh(H)
+            total, spectra, psum = 0, [], np.zeros((dim, dim), complex)
+            for ky in range(ly):
+                for kx in range(lx):
+                    P = sum(np.exp(-2j * math.pi * (kx * dx / lx + ky * dy / ly)) * T[dx, dy]
+                            for dx, dy in shifts) / n
+                    psum += P
+                    assert np.abs(P @ P - P).max() < 1e-12 and np.abs(P - P.conj().T).max() < 1e-12
+                    w, Q = np.linalg.eigh(P)
+                    Q = Q[:, w > 0.5]
+                    rank = Q.shape[1]
+                    total += rank
+                    spectra.extend(np.linalg.eigvalsh(Q.conj().T @ H @ Q))
+                    B = oracle.kgram_block(p8, nu, nd, kx, ky)
+                    k_out = B.shape[0]
+                    assert k_out <= rank, (nu, nd, kx, ky)
+                    if rank > k_out:
+                        missing[nu, nd] = missing.get((nu, nd), 0) + rank - k_out
+                    if k_out == rank and rank > 0:
+                        assert np.allclose(np.linalg.eigvalsh(B), np.linalg.eigvalsh(Q.conj().T @ H @ Q),
+                                           atol=1e-10), (nu, nd, kx, ky)
+            assert np.abs(psum - np.eye(dim)).max() < 1e-12
+            assert total == dim
+            assert np.allclose(np.sort(spectra), full, atol=1e-10)
+    want = {(2, 2): {(0, 2): 6, (2, 0): 6, (2, 4): 6, (4, 2): 6},
+            (3, 2): {(0, 2): 3, (2, 0): 3, (4, 6): 3, (6, 4): 3, (0, 6): 1, (6, 0): 1}}[lx, ly]
+    assert missing == want, missing
