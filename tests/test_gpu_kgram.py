@@ -280,3 +280,62 @@ def test_momentum_trace_2x2():
             assert abs(tr - tr_k) < 1e-8, (nu, nd, tr, tr_k)
             checked += 1
     assert checked >= 20
+
+
+# The readers of the reference's plot scripts, restated: the TSV loader and its header parser
+# (benchmarks/plot_n_vs_mu.py:27-52, also used by plot_n_vs_mu_3x2_four_solver.py:112-118) and the
+# METRIC patterns (plot_n_vs_mu_3x2_four_solver.py:28-33, plot_n_vs_mu_3x2_three_way.py:21-27).
+_HDR_NUM = r"=([+-]?(?:\d+\.?\d*|\.\d+)(?:[eE][+-]?\d+)?)"
+_METRIC_FOUR = re.compile(
+    r"^METRIC\s+kind=\S+\s+solver=(?P<solver>[^\s]+)\s+"
+    r".*?wall_time_s=(?P<wall>[\d.eE+-]+)\s+.*?wall_sector_logZ_s=(?P<wall_sec>[\d.eE+-]+)\s+"
+    r".*?peak_rss_bytes=(?P<rss_b>[\d.eE+-]+)\s+.*?peak_rss_mib=(?P<rss_mib>[\d.eE+-]+)", re.DOTALL)
+_METRIC_THREE = re.compile(
+    r"^METRIC\s+kind=(?P<kind>\S+)\s+.*?wall_time_s=(?P<wall>[\d.eE+-]+)\s+.*?"
+    r"peak_rss_bytes=(?P<rss_b>[\d.eE+-]+)\s+.*?peak_rss_mib=(?P<rss_mib>[\d.eE+-]+)", re.DOTALL)
+
+
+def _plot_load_tsv(text):
+    mus, ns, meta = [], [], {}
+    for line in text.splitlines():
+        line = line.strip()
+        if not line:
+            continue
+        if line.startswith("#"):
+            for key in ("beta", "t", "tp", "U", "V"):
+                m = re.search(rf"\b{key}" + _HDR_NUM, line)
+                if m:
+                    meta[key] = float(m.group(1))
+            continue
+        parts = line.split()
+        mus.append(float(parts[0]))
+        ns.append(float(parts[1]))
+    return np.array(mus), np.array(ns), meta
+
+
+@pytest.mark.parametrize("driver", ["ftlm_nmu_rect", "ftlm_nmu_rect_k"])
+def test_driver_output_reads_with_reference_plot_scripts(driver, tmp_path):
+    """The TSV and METRIC line of both drivers (monitoring on, as the reference's plot workflows run
+    them) parse with the reference's plot-script readers into the run's model, grid and timings."""
+    out = tmp_path / "n.tsv"
+    args = [a for a in GOLD["args"] if driver == "ftlm_nmu_rect_k" or not a.startswith("--no-log-k-dims")]
+    if driver == "ftlm_nmu_rect_k":
+        args.append("--solver=k-orbit")
+    say(driver, *args, "monitoring on")
+    r = subprocess.run([os.path.join(BIN, driver), *args, f"--out={out}"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    mu, n, meta = _plot_load_tsv(out.read_text())
+    assert meta == {"beta": 20.0, "t": 1.0, "tp": -0.35, "U": 5.75, "V": 0.9}, meta
+    assert len(mu) == 221 and abs(mu[0] + 5.0) < 1e-12 and abs(mu[-1] - 25.0) < 1e-9
+    assert np.all(np.diff(mu) > 0) and np.all((n >= 0) & (n <= 2.0))
+    lines = [ln for ln in r.stdout.splitlines() if ln.startswith("METRIC")]
+    assert len(lines) == 1
+    three = _METRIC_THREE.match(lines[0].strip())
+    assert three and three.group("kind") == ("FTLM_nmu_rect_k" if driver.endswith("_k") else "FTLM_nmu_rect")
+    assert float(three.group("wall")) > 0 and float(three.group("rss_b")) > 0
+    assert abs(float(three.group("rss_mib")) - float(three.group("rss_b")) / 2 ** 20) < 1.0
+    if driver == "ftlm_nmu_rect_k":
+        four = _METRIC_FOUR.match(lines[0].strip())
+        assert four and four.group("solver") == "k-orbit"
+        assert 0 < float(four.group("wall_sec")) <= float(four.group("wall"))
