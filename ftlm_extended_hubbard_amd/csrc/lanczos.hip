@@ -925,6 +925,9 @@ void ritz_persistent(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* o
   }
 }
 
+constexpr int64_t kPersistentMaxDim = 262144;     // FTLM_PATH_PERSISTENT
+constexpr int64_t kPersistentAutoMaxDim = 16384;  // FTLM_PATH_AUTO
+
 bool persistent_eligible(ftlm_op_t* const* ops, int n_ops, int path, int reorth) {
   if (path == FTLM_PATH_PER_OP || path == FTLM_PATH_GRAPH) return false;
   if (reorth != FTLM_REORTH_NONE) {
@@ -932,7 +935,7 @@ bool persistent_eligible(ftlm_op_t* const* ops, int n_ops, int path, int reorth)
                  "the persistent path keeps three vectors per recurrence and cannot reorthogonalise");
     return false;
   }
-  const int64_t lim = path == FTLM_PATH_PERSISTENT ? 262144 : 16384;
+  const int64_t lim = path == FTLM_PATH_PERSISTENT ? kPersistentMaxDim : kPersistentAutoMaxDim;
   for (int i = 0; i < n_ops; ++i) {
     if (ops[i] == nullptr || ops[i]->kind != kSector || ops[i]->dim > lim) {
       FTLM_REQUIRE(path != FTLM_PATH_PERSISTENT, FTLM_EUNSUPPORTED,
@@ -1065,13 +1068,43 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
       ritz_persistent(ops, n_ops, opts, ritz, weight, used, alpha_out, beta_out, st);
       return;
     }
-    // The recurrences run operator by operator; their tridiagonals are gathered into [M][S] device
-    // arrays (S = n_ops * R samples) and diagonalised together in one launch after the loop (a
-    // per-operator launch leaves the GPU idle: R workgroups each).
     for (int io = 0; io < n_ops; ++io)
       FTLM_REQUIRE(ops[io] != nullptr && ops[io]->device == ops[0]->device, FTLM_EINVAL,
                    "the operators of one call must live on one device");
     FTLM_HIP_CHECK(hipSetDevice(ops[0]->device));
+    // AUTO over a mix of sizes: the small full sectors (at least two) share one persistent launch,
+    // whose results are scattered to their operators' slots; the rest run operator by operator.
+    const int64_t per = static_cast<int64_t>(R) * M;
+    std::vector<char> done(static_cast<size_t>(n_ops), 0);
+    if (opts->path == FTLM_PATH_AUTO && opts->reorth == FTLM_REORTH_NONE) {
+      std::vector<ftlm_op_t*> small;
+      std::vector<int> at;
+      for (int io = 0; io < n_ops; ++io)
+        if (ops[io]->kind == kSector && ops[io]->dim <= kPersistentAutoMaxDim) {
+          small.push_back(ops[io]);
+          at.push_back(io);
+        }
+      if (small.size() >= 2) {
+        const size_t ns = small.size();
+        std::vector<double> sr(ns * per), sw(ns * per), sa(alpha_out ? ns * per : 0), sb(beta_out ? ns * per : 0);
+        std::vector<int> su(ns * R);
+        ritz_persistent(small.data(), static_cast<int>(ns), opts, sr.data(), sw.data(), su.data(),
+                        alpha_out ? sa.data() : nullptr, beta_out ? sb.data() : nullptr, st);
+        for (size_t j = 0; j < ns; ++j) {
+          const int64_t io = at[j], src = static_cast<int64_t>(j) * per, dst = io * per;
+          std::copy(sr.begin() + src, sr.begin() + src + per, ritz + dst);
+          std::copy(sw.begin() + src, sw.begin() + src + per, weight + dst);
+          std::copy(su.begin() + static_cast<int64_t>(j) * R, su.begin() + static_cast<int64_t>(j + 1) * R,
+                    used + io * R);
+          if (alpha_out) std::copy(sa.begin() + src, sa.begin() + src + per, alpha_out + dst);
+          if (beta_out) std::copy(sb.begin() + src, sb.begin() + src + per, beta_out + dst);
+          done[static_cast<size_t>(io)] = 1;
+        }
+      }
+    }
+    // The recurrences run operator by operator; their tridiagonals are gathered into [M][S] device
+    // arrays (S = n_ops * R samples) and diagonalised together in one launch after the loop (a
+    // per-operator launch leaves the GPU idle: R workgroups each).
     const int64_t S = static_cast<int64_t>(n_ops) * R;
     DevBuf<double> qa(static_cast<size_t>(M) * S), qb(static_cast<size_t>(M) * S);
     DevBuf<int> qn(static_cast<size_t>(S)), qnb(static_cast<size_t>(S));
@@ -1081,6 +1114,7 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
     FTLM_HIP_CHECK(hipMemsetAsync(qnb.p, 0, qnb.n * sizeof(int), st));
     std::vector<char> from_krylov(static_cast<size_t>(S), 0);
     for (int io = 0; io < n_ops; ++io) {
+      if (done[static_cast<size_t>(io)]) continue;
       const ftlm_op* op0 = ops[io];
       FTLM_REQUIRE(op0 != nullptr, FTLM_EINVAL, "null operator");
       FTLM_HIP_CHECK(hipSetDevice(op0->device));

@@ -199,7 +199,8 @@ typedef struct ftlm_ftlm_opts {
 #define FTLM_REORTH_FULL 1
 #define FTLM_REORTH_PARTIAL 2
 
-#define FTLM_PATH_AUTO 0       /* persistent launch when every operator is a sector of dim <= 16384 */
+#define FTLM_PATH_AUTO 0       /* the full sectors of dim <= 16384 in one persistent launch (when
+                                  there are at least two), every other operator per operator */
 #define FTLM_PATH_PER_OP 1     /* per operator, R recurrences per block H*V */
 #define FTLM_PATH_PERSISTENT 2 /* every (sector, sample) recurrence in one persistent launch (sectors
                                   of dim <= 262144) */

@@ -334,6 +334,29 @@ def test_persistent_path_matches_per_operator(golden):
     assert np.max(np.abs(api.ftlm_thermo(c, betas)["lnZ"] - lb)) < 1e-5
 
 
+def test_auto_path_mixes_persistent_and_per_operator():
+    """FTLM_PATH_AUTO over 4x3 sectors of mixed sizes (dims 1 ... 108,900, listed out of order):
+    the sectors of dim <= 16384 share one persistent launch and the larger ones run per operator;
+    every operator's slot gets its own Ritz data, equal to the per-operator path's (same starts,
+    complex128, capped Jacobi) as in the all-persistent check."""
+    say("auto mixed persistent + per-op, 4x3")
+    p = api.HubbardParams(4, 3, 1.0, -0.25, 8.0, 1.0, 0.05, 0.0)
+    sectors = [(3, 3), (1, 1), (0, 0), (4, 3), (2, 2), (3, 2), (2, 1)]
+    ops = [api.HubbardOperator.from_params(p, nu, nd) for nu, nd in sectors]
+    assert sum(op.dim() <= 16384 for op in ops) == 5
+    kw = dict(n_random=3, lanczos_steps=30, seed=13, clamp_M_to_dim=True)
+    a = api.ftlm_ritz(ops, api.FtlmParams(path=L.FTLM_PATH_AUTO, **kw), keep_tridiagonals=True)
+    b = api.ftlm_ritz(ops, api.FtlmParams(path=L.FTLM_PATH_PER_OP, **kw), keep_tridiagonals=True)
+    assert np.array_equal(a.used, b.used)
+    assert np.max(np.abs(a.alpha[:, :, :6] - b.alpha[:, :, :6])) < 1e-10
+    betas = np.array([0.5, 2.0])
+    la, lb = api.ftlm_thermo(a, betas)["lnZ"], api.ftlm_thermo(b, betas)["lnZ"]
+    assert np.max(np.abs(la - lb)) < 1e-8
+    for i, op in enumerate(ops):  # each slot holds its own operator's data
+        one = api.ftlm_ritz([op], api.FtlmParams(path=L.FTLM_PATH_PER_OP, **kw))
+        assert np.max(np.abs(api.ftlm_thermo(one, betas)["lnZ"][0] - la[i])) < 1e-8, sectors[i]
+
+
 @pytest.mark.parametrize("lx,ly,nu,nd,phi", [(4, 2, 3, 2, 0.05), (4, 2, 4, 4, 0.0), (2, 2, 2, 2, 0.3)])
 def test_gradient_vs_central_differences(lx, ly, nu, nd, phi):
     """d lnZ / d(t, t', U, V, phi_x) of the FTLM estimator (same seeds, converged Gauss rule)
