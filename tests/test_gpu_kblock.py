@@ -301,3 +301,31 @@ def test_kblock_vjp_matches_forward_mode():
             assert abs(got[k] - want) <= 1e-9 * max(abs(want), 1e-3), (mode, k, got[k], want)
     with pytest.raises(Exception, match="not Hermitian"):
         api.ftlm_log_partition_vjp(model(4, 2, 0.05), 3, 3, betas, w, par, params=names, K=K)
+
+
+def test_symmetry_known_answers_on_device():
+    """The reference's translation-symmetry known answers (test_symmetry_translation_stage1.cpp,
+    test_symmetry_stage2.cpp, test_symmetry_stage3.cpp, test_momentum_partition_2x2.cpp) on the
+    device primitives ftlm_hip_translate_states / _canonical_states / _bloch_phase."""
+    import symmetry_kat as kat
+
+    say("symmetry known answers on the device")
+
+    def translate(lx, ly, u, d, dx, dy):
+        out, sg = api.translate_states(lx, ly, np.array([(u << 16) | d], np.uint32), dx, dy)
+        return int(out[0]), float(sg[0])
+
+    def canonical(lx, ly, u, d):
+        can, ss = api.canonical_states(lx, ly, np.array([(u << 16) | d], np.uint32))
+        return int(can[0]), int(ss[0])
+
+    def bloch(lx, ly, kx, ky, dx, dy):
+        return api.translation_bloch_phase(api.MomentumSector(kx, ky, lx, ly), dx, dy)
+
+    def sector(n, nu, nd):
+        return api.FockBasis(n, nu, nd).masks()
+
+    kat.check_translation_stage1(translate)
+    kat.check_symmetry_stage2(translate, canonical)
+    kat.check_symmetry_stage3(translate, canonical, bloch)
+    kat.check_momentum_partition_2x2(translate, canonical, sector)

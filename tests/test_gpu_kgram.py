@@ -339,3 +339,55 @@ def test_driver_output_reads_with_reference_plot_scripts(driver, tmp_path):
         four = _METRIC_FOUR.match(lines[0].strip())
         assert four and four.group("solver") == "k-orbit"
         assert 0 < float(four.group("wall_sec")) <= float(four.group("wall"))
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd,kx,ky,dk", [
+    (2, 2, 1, 1, 0, 0, 4), (2, 2, 1, 1, 1, 0, 4), (2, 2, 1, 1, 1, 1, 4), (2, 2, 2, 2, 0, 0, 12),
+    (2, 2, 2, 2, 1, 0, 8), (2, 2, 0, 2, 0, 0, 0), (3, 2, 1, 1, 0, 0, 6), (3, 2, 1, 1, 1, 0, 6)])
+def test_k_dim_analysis_cases(oracle, lx, ly, nu, nd, kx, ky, dk):
+    """tests/test_k_dim_analysis.cpp:372-398 (t=1, t'=-0.2, U=4, V=0.5): on the reference's cases the
+    device Gram block has dimension rank(P_K) of the dense translation projector, and its spectrum
+    equals that of H restricted to range(P_K) (the reference's bound 1e-8; here 1e-10)."""
+    import symmetry_kat as kat
+
+    say("k-dim analysis", lx, ly, nu, nd, kx, ky)
+    p = api.HubbardParams(lx, ly, 1.0, -0.2, 4.0, 0.5, 0.0, 0.0)
+    g = api.KGramOperator.from_params(p, nu, nd, api.MomentumSector(kx, ky, lx, ly))
+    Q = kat.projector_range(oracle, lx, ly, nu, nd, kx, ky)
+    assert g.dim() == Q.shape[1] == dk
+    if dk:
+        H = oracle.dense_hamiltonian(params8(lx, ly, 1.0, -0.2, 4.0, 0.5), nu, nd)
+        want = np.linalg.eigvalsh(Q.conj().T @ H @ Q)
+        got = np.linalg.eigvalsh(_dense_from(g))
+        assert np.max(np.abs(got - want)) < 1e-10
+
+
+def test_momentum_logz_2x2(oracle):
+    """tests/test_momentum_logz_2x2.cpp:96-166 (2x2, t=1, t'=-0.35, U=5.75, V=0.9, beta=20): the
+    device ED of every full sector (the reference's real embedding and capped Jacobi) is finite and
+    equals ln tr e^{-beta H} from numpy's eigh of the oracle's H (1e-10 relative); the Gram blocks
+    are not all empty, and where they keep every state their ED log-sum equals the sector's."""
+    say("momentum logZ 2x2")
+    p = api.HubbardParams(2, 2, 1.0, -0.35, 5.75, 0.9, 0.0, 0.0)
+    p8 = params8(2, 2, 1.0, -0.35, 5.75, 0.9)
+    beta = np.array([20.0])
+    nonempty = full_cover = 0
+    for nu in range(5):
+        for nd in range(5):
+            sec = api.HubbardOperator.from_params(p, nu, nd)
+            lz = float(api.exact_log_partition(sec, beta)[0])
+            E = np.linalg.eigvalsh(oracle.dense_hamiltonian(p8, nu, nd))
+            want = -20.0 * E[0] + math.log(np.sum(np.exp(-20.0 * (E - E[0]))))
+            assert math.isfinite(lz) and abs(lz - want) <= 1e-10 * max(1.0, abs(want)), (nu, nd, lz, want)
+            blocks = [api.KGramOperator.from_params(p, nu, nd, api.MomentumSector(kx, ky, 2, 2), sector=sec)
+                      for ky in range(2) for kx in range(2)]
+            dks = [b.dim() for b in blocks]
+            nonempty += sum(dks) > 0
+            if sum(dks) == sec.dim():
+                full_cover += 1
+                tot = -math.inf
+                for b in blocks:
+                    if b.dim():
+                        tot = float(np.logaddexp(tot, api.exact_log_partition(b, beta)[0]))
+                assert abs(tot - lz) <= 1e-9 * max(1.0, abs(lz)), (nu, nd, tot, lz)
+    assert nonempty == 21 and full_cover == 21

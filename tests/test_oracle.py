@@ -409,17 +409,6 @@ def test_kgram_blocks_reproduce_reference_run(oracle):
     assert dropped == 3 * 4 + 9 * 2 + 1 * 2  # (0,2),(2,0),(4,6),(6,4); (2,4),(4,2); (0,6),(6,0)
 
 
-def _translation_matrix(oracle, lx, ly, up, dn, index, dx, dy):
-    """Dense fermionic translation T(dx, dy) on one (N_up, N_dn) sector: column i holds
-    eta * |T s_i>, from the oracle's translate / translation_sign."""
-    dim = len(up)
-    T = np.zeros((dim, dim))
-    for i, (u, d) in enumerate(zip(up, dn)):
-        j = index[int(oracle.translate(int(u), int(d), lx, ly, dx, dy))]
-        T[j, i] = oracle.translation_sign(int(u), int(d), lx, ly, dx, dy)
-    return T
-
-
 @pytest.mark.parametrize("lx,ly", [(2, 2), (3, 2)])
 def test_kgram_dims_against_dense_translation_projector(oracle, lx, ly):
     """Mirrors tests/test_k_gram_mismatch_scan.cpp: the Gram block dimension of every sector and
@@ -434,6 +423,8 @@ def test_kgram_dims_against_dense_translation_projector(oracle, lx, ly):
     (0,2), the single state of (0,6), and their particle-hole / spin images; (2,4) and (4,2) are
     past the scan's dim <= 64); on 2x2 every state of (0,2), (2,0), (2,4) and (4,2) is left out,
     since each two-fermion configuration there is swapped onto itself by one translation."""
+    import symmetry_kat as kat
+
     n = lx * ly
     p8 = params8(lx, ly, 1.0, -0.35, 5.75, 0.9)
     shifts = [(dx, dy) for dy in range(ly) for dx in range(lx)]
@@ -444,9 +435,8 @@ def test_kgram_dims_against_dense_translation_projector(oracle, lx, ly):
             dim = len(up)
             if dim == 0 or dim > 64:
                 continue
-            index = {(int(u) << 16) | int(d): i for i, (u, d) in enumerate(zip(up, dn))}
             H = oracle.dense_hamiltonian(p8, nu, nd)
-            T = {s: _translation_matrix(oracle, lx, ly, up, dn, index, *s) for s in shifts}
+            T = {s: kat.dense_translation(oracle, lx, ly, up, dn, *s) for s in shifts}
             for (ax, ay), Ta in T.items():
                 assert np.array_equal(Ta.T @ Ta, np.eye(dim))
                 assert np.abs(Ta @ H - H @ Ta).max() < 1e-12, (nu, nd, ax, ay)
@@ -479,3 +469,28 @@ def test_kgram_dims_against_dense_translation_projector(oracle, lx, ly):
     want = {(2, 2): {(0, 2): 6, (2, 0): 6, (2, 4): 6, (4, 2): 6},
             (3, 2): {(0, 2): 3, (2, 0): 3, (4, 6): 3, (6, 4): 3, (0, 6): 1, (6, 0): 1}}[lx, ly]
     assert missing == want, missing
+
+
+def test_symmetry_known_answers(oracle):
+    """The reference's translation-symmetry known answers (test_symmetry_translation_stage1.cpp,
+    test_symmetry_stage2.cpp, test_symmetry_stage3.cpp, test_momentum_partition_2x2.cpp) on the
+    oracle's translate / sign / canonical routines."""
+    import cmath
+
+    import symmetry_kat as kat
+
+    def translate(lx, ly, u, d, dx, dy):
+        return int(oracle.translate(u, d, lx, ly, dx, dy)), oracle.translation_sign(u, d, lx, ly, dx, dy)
+
+    def canonical(lx, ly, u, d):
+        c = int(oracle.canonical(u, d, lx, ly))
+        return c, sum(int(oracle.translate(u, d, lx, ly, dx, dy)) == ((u << 16) | d)
+                      for dy in range(ly) for dx in range(lx))
+
+    def bloch(lx, ly, kx, ky, dx, dy):
+        return cmath.exp(-2j * math.pi * (kx * dx / lx + ky * dy / ly))
+
+    kat.check_translation_stage1(translate)
+    kat.check_symmetry_stage2(translate, canonical)
+    kat.check_symmetry_stage3(translate, canonical, bloch)
+    kat.check_momentum_partition_2x2(translate, canonical, oracle.basis_masks)
