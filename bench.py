@@ -484,6 +484,61 @@ def kblock_all_workload(F, L, torch):
     return out
 
 
+def hostlink_workload(F, L, torch, reps=3):
+    """PCIe-inclusive H*v on the 4x4 (8,8) sector (complex128, one vector), the rate a caller of
+    the host-pointer drop-in sees: ftlm_hip_apply_host on pageable numpy arrays (device buffers
+    allocated, x copied in, H*v, y copied out, per call), and the same three steps on pinned host
+    buffers into resident device buffers.  Median of `reps` calls after one warm-up; the kernel
+    alone is timed beside it."""
+    import numpy as np
+
+    op = F.HubbardOperator.from_params(F.HubbardParams(**MODEL), NU, ND)
+    dim = op.dim()
+    nbytes = 2 * dim * 16  # x in, y out
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal(dim) + 1j * rng.standard_normal(dim)) / np.sqrt(2.0 * dim)
+    log(f"hostlink: dim={dim}, {nbytes / 1e9:.2f} GB over the host link per H*v")
+
+    def median_ms(fn):
+        fn()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            ts.append((time.perf_counter() - t0) * 1e3)
+        return float(np.median(ts))
+
+    y = None
+
+    def pageable():
+        nonlocal y
+        y = op.apply(x)
+
+    ms_pageable = median_ms(pageable)
+    xh = torch.from_numpy(x).pin_memory()
+    yh = torch.empty_like(xh).pin_memory()
+    xd = torch.empty(dim, dtype=torch.complex128, device="cuda")
+    yd = torch.empty_like(xd)
+
+    def pinned():
+        xd.copy_(xh, non_blocking=True)
+        op.apply_device(xd, yd, R=1, dtype=L.FTLM_C128)
+        yh.copy_(yd, non_blocking=True)
+
+    ms_pinned = median_ms(pinned)
+    ms_kernel = median_ms(lambda: op.apply_device(xd, yd, R=1, dtype=L.FTLM_C128))
+    assert np.array_equal(yh.numpy(), y), "pinned and pageable paths disagree"
+    out = {"metric": "4x4 (8,8) complex128 H*v including the host link", "unit": "ms", "higher_is_better": False,
+           "n_gpus": 1, "data": "synthetic", "dtype": "c128", "dim": dim, "host_link_bytes": nbytes,
+           "pageable_apply_host_ms": round(ms_pageable, 2), "pinned_resident_ms": round(ms_pinned, 2),
+           "kernel_only_ms": round(ms_kernel, 2),
+           "pageable_GB_s": round(nbytes / ms_pageable / 1e6, 1), "pinned_GB_s": round(nbytes / ms_pinned / 1e6, 1)}
+    log(str(out))
+    return out
+
+
 def graph_workload(F, L, torch):
     """Eager vs hipGraph-captured per-operator Lanczos (FTLM_PATH_PER_OP vs FTLM_PATH_GRAPH):
     M=100 steps of R=16 complex64 random-phase vectors on sectors of growing size (t=1,
@@ -728,7 +783,8 @@ def main():
     ap.add_argument("--no-validate", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--workload", choices=["sector", "kblock", "kblock_all", "gc4x2", "grad", "graph", "reorth", "tune"], default="sector",
+    ap.add_argument("--workload", choices=["sector", "kblock", "kblock_all", "gc4x2", "grad", "graph", "reorth", "tune",
+                                                "hostlink"], default="sector",
                     help="sector: the 4x4 (8,8) full-sector line (default); kblock: the 4x4 (8,8) momentum-block "
                          "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch); grad: "
                          "the differentiated FTLM line (4x3 (6,6) and the 4x4 (8,8) K=(0,0) block)")
@@ -769,6 +825,9 @@ def main():
         return
     if args.workload == "tune":
         print(json.dumps(tune_workload(F, L, torch)), flush=True)
+        return
+    if args.workload == "hostlink":
+        print(json.dumps(hostlink_workload(F, L, torch)), flush=True)
         return
     if args.workload == "grad":
         print(json.dumps(grad_workload(F, L, torch, validate=not args.no_validate)), flush=True)
