@@ -18,8 +18,10 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <random>
+#include <thread>
 #include <vector>
 
 #include "common.hpp"
@@ -734,12 +736,6 @@ void gauss_mt(int64_t dim, unsigned seed, std::vector<std::complex<double>>* q) 
   }
 }
 
-double host_norm(const std::vector<std::complex<double>>& q) {
-  double s = 0.0;
-  for (const auto& z : q) s += std::norm(z);
-  return std::sqrt(s);
-}
-
 // Fill the start block with R start vectors.  seeds[r] per sample.
 void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const double* q0, hipStream_t st) {
   const int R = K.R;
@@ -761,38 +757,82 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
     FTLM_HIP_CHECK(hipStreamSynchronize(st));
     return;
   }
-  // host-generated (reference mt19937 Gaussian or caller-supplied), interleaved [dim][R]
-  std::vector<double> inv(static_cast<size_t>(R));
+  // host-generated (reference mt19937 Gaussian or caller-supplied), interleaved [dim][R].  The
+  // samples are independent streams, so they are drawn in parallel, in blocks of kStartBlock
+  // indices: each thread draws its samples' block into a private buffer (the generator and the
+  // distribution carry over between blocks, so every sample sees exactly the serial sequence),
+  // then the threads interleave disjoint index ranges into the host block.  The norms accumulate
+  // per sample in index order, as one serial sum.
+  constexpr int64_t kStartBlock = int64_t(1) << 18;
   const size_t es = K.dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2);
   std::vector<unsigned char> host(static_cast<size_t>(dim) * R * es);
-  std::vector<std::complex<double>> q;
-  bool fallback = false;
+  const int nthr = std::max(1, std::min(R, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))));
+  std::vector<std::mt19937> rngs;
+  std::vector<std::normal_distribution<double>> gs(static_cast<size_t>(R), std::normal_distribution<double>(0.0, 1.0));
+  for (int r = 0; r < R; ++r) rngs.emplace_back(q0 == nullptr ? seeds[r] : 0u);
+  const int64_t blk = std::min(kStartBlock, dim);
+  std::vector<std::vector<std::complex<double>>> part(static_cast<size_t>(R),
+                                                      std::vector<std::complex<double>>(static_cast<size_t>(blk)));
+  std::vector<double> ss(static_cast<size_t>(R), 0.0);
+  auto parallel = [&](int n, const std::function<void(int)>& body) {
+    std::vector<std::thread> th;
+    for (int t = 1; t < n; ++t) th.emplace_back(body, t);
+    body(0);
+    for (auto& x : th) x.join();
+  };
+  for (int64_t b0 = 0; b0 < dim; b0 += blk) {
+    const int64_t nb = std::min(blk, dim - b0);
+    parallel(nthr, [&](int t) {
+      for (int r = t; r < R; r += nthr) {
+        std::complex<double>* q = part[static_cast<size_t>(r)].data();
+        if (q0 != nullptr) {
+          std::memcpy(q, reinterpret_cast<const std::complex<double>*>(q0) + static_cast<int64_t>(r) * dim + b0,
+                      static_cast<size_t>(nb) * sizeof(std::complex<double>));
+        } else {
+          for (int64_t i = 0; i < nb; ++i) {
+            const double re = gs[r](rngs[r]);
+            const double im = gs[r](rngs[r]);
+            q[i] = std::complex<double>(re, im);
+          }
+        }
+        double s2 = ss[static_cast<size_t>(r)];
+        for (int64_t i = 0; i < nb; ++i) s2 += std::norm(q[i]);
+        ss[static_cast<size_t>(r)] = s2;
+      }
+    });
+    parallel(nthr, [&](int t) {
+      const int64_t i0 = nb * t / nthr, i1 = nb * (t + 1) / nthr;
+      for (int64_t i = i0; i < i1; ++i)
+        for (int r = 0; r < R; ++r) {
+          const std::complex<double> z = part[static_cast<size_t>(r)][static_cast<size_t>(i)];
+          unsigned char* dst = &host[(static_cast<size_t>(b0 + i) * R + r) * es];
+          if (K.dtype == FTLM_C64) {
+            const float2 v = make_float2(static_cast<float>(z.real()), static_cast<float>(z.imag()));
+            std::memcpy(dst, &v, es);
+          } else {
+            const double2 v = make_double2(z.real(), z.imag());
+            std::memcpy(dst, &v, es);
+          }
+        }
+    });
+  }
+  std::vector<double> inv(static_cast<size_t>(R));
   for (int r = 0; r < R; ++r) {
-    if (q0 != nullptr) {
-      q.assign(reinterpret_cast<const std::complex<double>*>(q0) + static_cast<int64_t>(r) * dim,
-               reinterpret_cast<const std::complex<double>*>(q0) + static_cast<int64_t>(r + 1) * dim);
-    } else {
-      gauss_mt(dim, seeds[r], &q);
-    }
-    const double n0 = host_norm(q);
-    if (n0 < 1e-18) {
-      q[0] = std::complex<double>(1.0, 0.0);
+    const double n0 = std::sqrt(ss[static_cast<size_t>(r)]);
+    if (n0 < 1e-18) {  // src/lanczos.cpp:170-175: start from e_0 instead
+      unsigned char* dst = &host[static_cast<size_t>(r) * es];
+      if (K.dtype == FTLM_C64) {
+        const float2 v = make_float2(1.0f, 0.0f);
+        std::memcpy(dst, &v, es);
+      } else {
+        const double2 v = make_double2(1.0, 0.0);
+        std::memcpy(dst, &v, es);
+      }
       inv[r] = 1.0;
-      fallback = true;
     } else {
       inv[r] = 1.0 / n0;
     }
-    for (int64_t i = 0; i < dim; ++i) {
-      if (K.dtype == FTLM_C64) {
-        float2 v = make_float2(static_cast<float>(q[i].real()), static_cast<float>(q[i].imag()));
-        std::memcpy(&host[(static_cast<size_t>(i) * R + r) * es], &v, es);
-      } else {
-        double2 v = make_double2(q[i].real(), q[i].imag());
-        std::memcpy(&host[(static_cast<size_t>(i) * R + r) * es], &v, es);
-      }
-    }
   }
-  (void)fallback;
   FTLM_HIP_CHECK(hipMemcpyAsync(K.slot(0), host.data(), host.size(), hipMemcpyHostToDevice, st));
   if (K.dtype == FTLM_C64) {
     // complex64 storage: recompute the norm of the stored (rounded) vectors on the device
