@@ -761,11 +761,11 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
   // samples are independent streams, so they are drawn in parallel, in blocks of kStartBlock
   // indices: each thread draws its samples' block into a private buffer (the generator and the
   // distribution carry over between blocks, so every sample sees exactly the serial sequence),
-  // then the threads interleave disjoint index ranges into the host block.  The norms accumulate
-  // per sample in index order, as one serial sum.
+  // then the threads interleave disjoint index ranges into one of two pinned staging blocks, whose
+  // copy to the device overlaps the next block's draws.  The norms accumulate per sample in index
+  // order, as one serial sum.
   constexpr int64_t kStartBlock = int64_t(1) << 18;
   const size_t es = K.dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2);
-  std::vector<unsigned char> host(static_cast<size_t>(dim) * R * es);
   const int nthr = std::max(1, std::min(R, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))));
   std::vector<std::mt19937> rngs;
   std::vector<std::normal_distribution<double>> gs(static_cast<size_t>(R), std::normal_distribution<double>(0.0, 1.0));
@@ -774,13 +774,39 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
   std::vector<std::vector<std::complex<double>>> part(static_cast<size_t>(R),
                                                       std::vector<std::complex<double>>(static_cast<size_t>(blk)));
   std::vector<double> ss(static_cast<size_t>(R), 0.0);
+  // A single block (dim <= kStartBlock) goes through one pageable buffer and a synchronous copy.
+  struct Staging {
+    unsigned char* h[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    std::vector<unsigned char> pageable;
+    ~Staging() {
+      for (int i = 0; i < 2; ++i) {
+        if (ev[i]) {
+          (void)hipEventSynchronize(ev[i]);
+          (void)hipEventDestroy(ev[i]);
+        }
+        if (h[i] && pageable.empty()) (void)hipHostFree(h[i]);
+      }
+    }
+  } stg;
+  const size_t stg_bytes = static_cast<size_t>(blk) * R * es;
+  if (dim <= blk) {
+    stg.pageable.resize(stg_bytes);
+    stg.h[0] = stg.h[1] = stg.pageable.data();
+  } else {
+    for (int i = 0; i < 2; ++i)
+      FTLM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&stg.h[i]), stg_bytes, hipHostMallocDefault));
+  }
+  for (int i = 0; i < 2; ++i) FTLM_HIP_CHECK(hipEventCreateWithFlags(&stg.ev[i], hipEventDisableTiming));
   auto parallel = [&](int n, const std::function<void(int)>& body) {
     std::vector<std::thread> th;
     for (int t = 1; t < n; ++t) th.emplace_back(body, t);
     body(0);
     for (auto& x : th) x.join();
   };
-  for (int64_t b0 = 0; b0 < dim; b0 += blk) {
+  bool used_stg[2] = {false, false};
+  int which = 0;
+  for (int64_t b0 = 0; b0 < dim; b0 += blk, which ^= 1) {
     const int64_t nb = std::min(blk, dim - b0);
     parallel(nthr, [&](int t) {
       for (int r = t; r < R; r += nthr) {
@@ -789,23 +815,31 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
           std::memcpy(q, reinterpret_cast<const std::complex<double>*>(q0) + static_cast<int64_t>(r) * dim + b0,
                       static_cast<size_t>(nb) * sizeof(std::complex<double>));
         } else {
+          // thread-local copies: the distributions sit side by side in one vector, and drawing
+          // from them in place would share their cache lines between threads
+          std::mt19937 rng = rngs[r];
+          std::normal_distribution<double> g = gs[r];
           for (int64_t i = 0; i < nb; ++i) {
-            const double re = gs[r](rngs[r]);
-            const double im = gs[r](rngs[r]);
+            const double re = g(rng);
+            const double im = g(rng);
             q[i] = std::complex<double>(re, im);
           }
+          rngs[r] = rng;
+          gs[r] = g;
         }
         double s2 = ss[static_cast<size_t>(r)];
         for (int64_t i = 0; i < nb; ++i) s2 += std::norm(q[i]);
         ss[static_cast<size_t>(r)] = s2;
       }
     });
+    if (used_stg[which]) FTLM_HIP_CHECK(hipEventSynchronize(stg.ev[which]));  // its previous copy is done
+    unsigned char* host = stg.h[which];
     parallel(nthr, [&](int t) {
       const int64_t i0 = nb * t / nthr, i1 = nb * (t + 1) / nthr;
       for (int64_t i = i0; i < i1; ++i)
         for (int r = 0; r < R; ++r) {
           const std::complex<double> z = part[static_cast<size_t>(r)][static_cast<size_t>(i)];
-          unsigned char* dst = &host[(static_cast<size_t>(b0 + i) * R + r) * es];
+          unsigned char* dst = &host[(static_cast<size_t>(i) * R + r) * es];
           if (K.dtype == FTLM_C64) {
             const float2 v = make_float2(static_cast<float>(z.real()), static_cast<float>(z.imag()));
             std::memcpy(dst, &v, es);
@@ -815,25 +849,26 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
           }
         }
     });
+    FTLM_HIP_CHECK(hipMemcpyAsync(static_cast<unsigned char*>(K.slot(0)) + static_cast<size_t>(b0) * R * es, host,
+                                  static_cast<size_t>(nb) * R * es, hipMemcpyHostToDevice, st));
+    FTLM_HIP_CHECK(hipEventRecord(stg.ev[which], st));
+    used_stg[which] = true;
   }
   std::vector<double> inv(static_cast<size_t>(R));
   for (int r = 0; r < R; ++r) {
     const double n0 = std::sqrt(ss[static_cast<size_t>(r)]);
     if (n0 < 1e-18) {  // src/lanczos.cpp:170-175: start from e_0 instead
-      unsigned char* dst = &host[static_cast<size_t>(r) * es];
-      if (K.dtype == FTLM_C64) {
-        const float2 v = make_float2(1.0f, 0.0f);
-        std::memcpy(dst, &v, es);
-      } else {
-        const double2 v = make_double2(1.0, 0.0);
-        std::memcpy(dst, &v, es);
-      }
+      static const double2 one = make_double2(1.0, 0.0);
+      static const float2 onef = make_float2(1.0f, 0.0f);
+      FTLM_HIP_CHECK(hipMemcpyAsync(static_cast<unsigned char*>(K.slot(0)) + static_cast<size_t>(r) * es,
+                                    K.dtype == FTLM_C64 ? static_cast<const void*>(&onef)
+                                                        : static_cast<const void*>(&one),
+                                    es, hipMemcpyHostToDevice, st));
       inv[r] = 1.0;
     } else {
       inv[r] = 1.0 / n0;
     }
   }
-  FTLM_HIP_CHECK(hipMemcpyAsync(K.slot(0), host.data(), host.size(), hipMemcpyHostToDevice, st));
   if (K.dtype == FTLM_C64) {
     // complex64 storage: recompute the norm of the stored (rounded) vectors on the device
     K.start(st);
