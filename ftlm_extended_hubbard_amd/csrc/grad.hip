@@ -29,7 +29,7 @@
 
 namespace ftlm_hip {
 
-void gauss_mt(int64_t dim, unsigned seed, std::vector<std::complex<double>>* q);
+void gauss_interleaved(int64_t dim, int R, const std::vector<unsigned>& seeds, std::vector<double2>* h);
 
 namespace {
 
@@ -886,12 +886,8 @@ void tape_forward(ftlm_tape* T, const ftlm_op* op, ftlm_op* const* dops, int npa
     check_launch("k_gphase");
     FTLM_HIP_CHECK(hipStreamSynchronize(st));
   } else {
-    std::vector<double2> h(vb);
-    std::vector<std::complex<double>> g;
-    for (int r = 0; r < R; ++r) {
-      gauss_mt(dim, seeds[r], &g);
-      for (int64_t i = 0; i < dim; ++i) h[static_cast<size_t>(i) * R + r] = make_double2(g[i].real(), g[i].imag());
-    }
+    std::vector<double2> h;
+    gauss_interleaved(dim, R, seeds, &h);
     FTLM_HIP_CHECK(hipMemcpy(q, h.data(), vb * sizeof(double2), hipMemcpyHostToDevice));
   }
   hipLaunchKernelGGL(k_gstep_dots, dim3(gblk), dim3(kGBlock), 0, st, n, R, 0, q, q, GradVecs{}, part.p);
@@ -1263,12 +1259,8 @@ int ftlm_hip_ftlm_gradient(const ftlm_op_t* op, ftlm_op_t* const* dops, int npar
       check_launch("k_gphase");
       FTLM_HIP_CHECK(hipStreamSynchronize(st));
     } else {
-      std::vector<double2> h(vb);
-      std::vector<std::complex<double>> g;
-      for (int r = 0; r < R; ++r) {
-        gauss_mt(dim, seeds[r], &g);
-        for (int64_t i = 0; i < dim; ++i) h[static_cast<size_t>(i) * R + r] = make_double2(g[i].real(), g[i].imag());
-      }
+      std::vector<double2> h;
+      gauss_interleaved(dim, R, seeds, &h);
       q.upload(h, st);
     }
     const int bsz = (kGBlock / R) * R;

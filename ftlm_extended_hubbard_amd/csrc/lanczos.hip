@@ -736,6 +736,50 @@ void gauss_mt(int64_t dim, unsigned seed, std::vector<std::complex<double>>* q) 
   }
 }
 
+// The reference starts of R samples, interleaved h[i * R + r] = (re, im) of sample r at index i,
+// unnormalised: one thread per sample (up to 16) draws its stream in blocks of 2^18 indices into a
+// private buffer, then the threads interleave disjoint index ranges.  Each sample sees the serial
+// mt19937 / normal_distribution sequence of gauss_mt.
+void gauss_interleaved(int64_t dim, int R, const std::vector<unsigned>& seeds, std::vector<double2>* h) {
+  constexpr int64_t kBlock = int64_t(1) << 18;
+  h->resize(static_cast<size_t>(dim) * R);
+  if (dim <= 0) return;
+  const int nthr = std::max(1, std::min(R, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))));
+  std::vector<std::mt19937> rngs;
+  for (int r = 0; r < R; ++r) rngs.emplace_back(seeds[r]);
+  std::vector<std::normal_distribution<double>> gs(static_cast<size_t>(R), std::normal_distribution<double>(0.0, 1.0));
+  const int64_t blk = std::min(kBlock, dim);
+  std::vector<std::vector<double2>> part(static_cast<size_t>(R), std::vector<double2>(static_cast<size_t>(blk)));
+  auto parallel = [&](const std::function<void(int)>& body) {
+    std::vector<std::thread> th;
+    for (int t = 1; t < nthr; ++t) th.emplace_back(body, t);
+    body(0);
+    for (auto& x : th) x.join();
+  };
+  for (int64_t b0 = 0; b0 < dim; b0 += blk) {
+    const int64_t nb = std::min(blk, dim - b0);
+    parallel([&](int t) {
+      for (int r = t; r < R; r += nthr) {
+        std::mt19937 rng = rngs[r];  // thread-local copies (no shared cache lines)
+        std::normal_distribution<double> g = gs[r];
+        double2* q = part[static_cast<size_t>(r)].data();
+        for (int64_t i = 0; i < nb; ++i) {
+          const double re = g(rng);
+          const double im = g(rng);
+          q[i] = make_double2(re, im);
+        }
+        rngs[r] = rng;
+        gs[r] = g;
+      }
+    });
+    parallel([&](int t) {
+      const int64_t i0 = nb * t / nthr, i1 = nb * (t + 1) / nthr;
+      for (int64_t i = i0; i < i1; ++i)
+        for (int r = 0; r < R; ++r) (*h)[static_cast<size_t>(b0 + i) * R + r] = part[static_cast<size_t>(r)][static_cast<size_t>(i)];
+    });
+  }
+}
+
 // Fill the start block with R start vectors.  seeds[r] per sample.
 void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const double* q0, hipStream_t st) {
   const int R = K.R;
