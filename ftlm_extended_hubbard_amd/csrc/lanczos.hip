@@ -736,46 +736,70 @@ void gauss_mt(int64_t dim, unsigned seed, std::vector<std::complex<double>>* q) 
   }
 }
 
-// The reference starts of R samples, interleaved h[i * R + r] = (re, im) of sample r at index i,
-// unnormalised: one thread per sample (up to 16) draws its stream in blocks of 2^18 indices into a
-// private buffer, then the threads interleave disjoint index ranges.  Each sample sees the serial
-// mt19937 / normal_distribution sequence of gauss_mt.
+// ---- the reference starts of many samples, drawn in parallel
+// Each sample is one sequential mt19937 / normal_distribution stream (gauss_mt), so the samples are
+// drawn in parallel, one thread per sample (up to 16), in blocks of kGaussBlock indices.  A
+// sample's generator and distribution carry over between blocks, so it sees exactly the serial
+// sequence.
+constexpr int64_t kGaussBlock = int64_t(1) << 18;
+
+static int gauss_threads(int R) {
+  return std::max(1, std::min(R, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))));
+}
+
+// body(0 .. n-1), one call per thread; the calling thread runs body(0).
+static void on_threads(int n, const std::function<void(int)>& body) {
+  std::vector<std::thread> th;
+  for (int t = 1; t < n; ++t) th.emplace_back(body, t);
+  body(0);
+  for (auto& x : th) x.join();
+}
+
+struct GaussStreams {
+  std::vector<std::mt19937> rng;
+  std::vector<std::normal_distribution<double>> g;
+  GaussStreams(const std::vector<unsigned>& seeds, int R)
+      : g(static_cast<size_t>(R), std::normal_distribution<double>(0.0, 1.0)) {
+    for (int r = 0; r < R; ++r) rng.emplace_back(seeds[static_cast<size_t>(r)]);
+  }
+  // The next nb values of sample r.  The drawing thread works on its own copies: the objects sit
+  // side by side in the vectors, and drawing in place would share cache lines between threads.
+  void draw(int r, int64_t nb, std::complex<double>* out) {
+    std::mt19937 e = rng[static_cast<size_t>(r)];
+    std::normal_distribution<double> d = g[static_cast<size_t>(r)];
+    for (int64_t i = 0; i < nb; ++i) {
+      const double re = d(e);
+      const double im = d(e);
+      out[i] = std::complex<double>(re, im);
+    }
+    rng[static_cast<size_t>(r)] = e;
+    g[static_cast<size_t>(r)] = d;
+  }
+};
+
+// The reference starts of R samples, unnormalised and interleaved: h[i * R + r] = sample r at i.
+// Each block is drawn per sample into private buffers, then the threads interleave disjoint index
+// ranges.
 void gauss_interleaved(int64_t dim, int R, const std::vector<unsigned>& seeds, std::vector<double2>* h) {
-  constexpr int64_t kBlock = int64_t(1) << 18;
   h->resize(static_cast<size_t>(dim) * R);
   if (dim <= 0) return;
-  const int nthr = std::max(1, std::min(R, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))));
-  std::vector<std::mt19937> rngs;
-  for (int r = 0; r < R; ++r) rngs.emplace_back(seeds[r]);
-  std::vector<std::normal_distribution<double>> gs(static_cast<size_t>(R), std::normal_distribution<double>(0.0, 1.0));
-  const int64_t blk = std::min(kBlock, dim);
-  std::vector<std::vector<double2>> part(static_cast<size_t>(R), std::vector<double2>(static_cast<size_t>(blk)));
-  auto parallel = [&](const std::function<void(int)>& body) {
-    std::vector<std::thread> th;
-    for (int t = 1; t < nthr; ++t) th.emplace_back(body, t);
-    body(0);
-    for (auto& x : th) x.join();
-  };
+  const int nthr = gauss_threads(R);
+  GaussStreams gs(seeds, R);
+  const int64_t blk = std::min(kGaussBlock, dim);
+  std::vector<std::vector<std::complex<double>>> part(static_cast<size_t>(R),
+                                                      std::vector<std::complex<double>>(static_cast<size_t>(blk)));
   for (int64_t b0 = 0; b0 < dim; b0 += blk) {
     const int64_t nb = std::min(blk, dim - b0);
-    parallel([&](int t) {
-      for (int r = t; r < R; r += nthr) {
-        std::mt19937 rng = rngs[r];  // thread-local copies (no shared cache lines)
-        std::normal_distribution<double> g = gs[r];
-        double2* q = part[static_cast<size_t>(r)].data();
-        for (int64_t i = 0; i < nb; ++i) {
-          const double re = g(rng);
-          const double im = g(rng);
-          q[i] = make_double2(re, im);
-        }
-        rngs[r] = rng;
-        gs[r] = g;
-      }
+    on_threads(nthr, [&](int t) {
+      for (int r = t; r < R; r += nthr) gs.draw(r, nb, part[static_cast<size_t>(r)].data());
     });
-    parallel([&](int t) {
+    on_threads(nthr, [&](int t) {
       const int64_t i0 = nb * t / nthr, i1 = nb * (t + 1) / nthr;
       for (int64_t i = i0; i < i1; ++i)
-        for (int r = 0; r < R; ++r) (*h)[static_cast<size_t>(b0 + i) * R + r] = part[static_cast<size_t>(r)][static_cast<size_t>(i)];
+        for (int r = 0; r < R; ++r) {
+          const std::complex<double> z = part[static_cast<size_t>(r)][static_cast<size_t>(i)];
+          (*h)[static_cast<size_t>(b0 + i) * R + r] = make_double2(z.real(), z.imag());
+        }
     });
   }
 }
@@ -801,24 +825,18 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
     FTLM_HIP_CHECK(hipStreamSynchronize(st));
     return;
   }
-  // host-generated (reference mt19937 Gaussian or caller-supplied), interleaved [dim][R].  The
-  // samples are independent streams, so they are drawn in parallel, in blocks of kStartBlock
-  // indices: each thread draws its samples' block into a private buffer (the generator and the
-  // distribution carry over between blocks, so every sample sees exactly the serial sequence),
-  // then the threads interleave disjoint index ranges into one of two pinned staging blocks, whose
-  // copy to the device overlaps the next block's draws.  The norms accumulate per sample in index
-  // order, as one serial sum.
-  constexpr int64_t kStartBlock = int64_t(1) << 18;
+  // host-generated (reference mt19937 Gaussian or caller-supplied), interleaved [dim][R], block by
+  // block (GaussStreams): the threads interleave disjoint index ranges of each block into one of
+  // two pinned staging blocks, whose copy to the device overlaps the next block's draws.  The norms
+  // accumulate per sample in index order, as one serial sum.
   const size_t es = K.dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2);
-  const int nthr = std::max(1, std::min(R, std::min(16, static_cast<int>(std::thread::hardware_concurrency()))));
-  std::vector<std::mt19937> rngs;
-  std::vector<std::normal_distribution<double>> gs(static_cast<size_t>(R), std::normal_distribution<double>(0.0, 1.0));
-  for (int r = 0; r < R; ++r) rngs.emplace_back(q0 == nullptr ? seeds[r] : 0u);
-  const int64_t blk = std::min(kStartBlock, dim);
+  const int nthr = gauss_threads(R);
+  GaussStreams gs(q0 == nullptr ? seeds : std::vector<unsigned>(static_cast<size_t>(R), 0u), R);
+  const int64_t blk = std::min(kGaussBlock, dim);
   std::vector<std::vector<std::complex<double>>> part(static_cast<size_t>(R),
                                                       std::vector<std::complex<double>>(static_cast<size_t>(blk)));
   std::vector<double> ss(static_cast<size_t>(R), 0.0);
-  // A single block (dim <= kStartBlock) goes through one pageable buffer and a synchronous copy.
+  // A single block (dim <= kGaussBlock) goes through one pageable buffer and a synchronous copy.
   struct Staging {
     unsigned char* h[2] = {nullptr, nullptr};
     hipEvent_t ev[2] = {nullptr, nullptr};
@@ -842,35 +860,18 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
       FTLM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&stg.h[i]), stg_bytes, hipHostMallocDefault));
   }
   for (int i = 0; i < 2; ++i) FTLM_HIP_CHECK(hipEventCreateWithFlags(&stg.ev[i], hipEventDisableTiming));
-  auto parallel = [&](int n, const std::function<void(int)>& body) {
-    std::vector<std::thread> th;
-    for (int t = 1; t < n; ++t) th.emplace_back(body, t);
-    body(0);
-    for (auto& x : th) x.join();
-  };
   bool used_stg[2] = {false, false};
   int which = 0;
   for (int64_t b0 = 0; b0 < dim; b0 += blk, which ^= 1) {
     const int64_t nb = std::min(blk, dim - b0);
-    parallel(nthr, [&](int t) {
+    on_threads(nthr, [&](int t) {
       for (int r = t; r < R; r += nthr) {
         std::complex<double>* q = part[static_cast<size_t>(r)].data();
-        if (q0 != nullptr) {
+        if (q0 != nullptr)
           std::memcpy(q, reinterpret_cast<const std::complex<double>*>(q0) + static_cast<int64_t>(r) * dim + b0,
                       static_cast<size_t>(nb) * sizeof(std::complex<double>));
-        } else {
-          // thread-local copies: the distributions sit side by side in one vector, and drawing
-          // from them in place would share their cache lines between threads
-          std::mt19937 rng = rngs[r];
-          std::normal_distribution<double> g = gs[r];
-          for (int64_t i = 0; i < nb; ++i) {
-            const double re = g(rng);
-            const double im = g(rng);
-            q[i] = std::complex<double>(re, im);
-          }
-          rngs[r] = rng;
-          gs[r] = g;
-        }
+        else
+          gs.draw(r, nb, q);
         double s2 = ss[static_cast<size_t>(r)];
         for (int64_t i = 0; i < nb; ++i) s2 += std::norm(q[i]);
         ss[static_cast<size_t>(r)] = s2;
@@ -878,7 +879,7 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
     });
     if (used_stg[which]) FTLM_HIP_CHECK(hipEventSynchronize(stg.ev[which]));  // its previous copy is done
     unsigned char* host = stg.h[which];
-    parallel(nthr, [&](int t) {
+    on_threads(nthr, [&](int t) {
       const int64_t i0 = nb * t / nthr, i1 = nb * (t + 1) / nthr;
       for (int64_t i = i0; i < i1; ++i)
         for (int r = 0; r < R; ++r) {
