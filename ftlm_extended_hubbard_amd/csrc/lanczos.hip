@@ -993,17 +993,21 @@ void ritz_persistent(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* o
   const int ns = static_cast<int>(secs.size()), n_out = ns * R;
   std::vector<std::vector<double2>> q0;
   if (opts->start == FTLM_START_GAUSS_MT) {  // the reference's mt19937 Gaussian starts, per sample
+    // every (sector, sample) start is its own serial stream: the threads take them in turn
     q0.resize(static_cast<size_t>(ns));
-    std::vector<std::complex<double>> q;
-    for (int s = 0; s < ns; ++s) {
-      const int64_t dim = secs[s]->dim;
-      q0[s].resize(static_cast<size_t>(R) * dim);
-      for (int r = 0; r < R; ++r) {
+    for (int s = 0; s < ns; ++s) q0[s].resize(static_cast<size_t>(R) * secs[s]->dim);
+    const int njobs = ns * R;
+    const int nthr = std::max(1, std::min(njobs, gauss_threads(16)));
+    on_threads(nthr, [&](int t) {
+      std::vector<std::complex<double>> q;
+      for (int j = t; j < njobs; j += nthr) {
+        const int s = j / R, r = j - s * R;
+        const int64_t dim = secs[s]->dim;
         gauss_mt(dim, opts->seed + static_cast<unsigned>(r) * 100003u, &q);
         for (int64_t i = 0; i < dim; ++i)
           q0[s][static_cast<size_t>(r) * dim + i] = make_double2(q[i].real(), q[i].imag());
       }
-    }
+    });
   }
   DevBuf<double> da(static_cast<size_t>(M) * n_out), db(static_cast<size_t>(M) * n_out);
   DevBuf<int> du(static_cast<size_t>(n_out)), dal(static_cast<size_t>(n_out));
