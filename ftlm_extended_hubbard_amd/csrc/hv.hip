@@ -98,6 +98,48 @@ __device__ __forceinline__ void axpy_regs(T (&ar)[N], T (&ai)[N], const typename
   }
 }
 
+// Streaming accesses of y (written by the row pass, read and rewritten by the column pass) can
+// carry the non-temporal hint, so that they do not displace the gathered source window in L2.
+// Build knob, off by default.
+#ifndef FTLM_HV_NT
+#define FTLM_HV_NT 0
+#endif
+typedef float ftlm_f2v __attribute__((ext_vector_type(2)));
+typedef double ftlm_d2v __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ void st_stream(float2* p, float2 v) {
+#if FTLM_HV_NT
+  ftlm_f2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<ftlm_f2v*>(p));
+#else
+  *p = v;
+#endif
+}
+__device__ __forceinline__ void st_stream(double2* p, double2 v) {
+#if FTLM_HV_NT
+  ftlm_d2v w = {v.x, v.y};
+  __builtin_nontemporal_store(w, reinterpret_cast<ftlm_d2v*>(p));
+#else
+  *p = v;
+#endif
+}
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+#if FTLM_HV_NT
+  const ftlm_f2v w = __builtin_nontemporal_load(reinterpret_cast<const ftlm_f2v*>(p));
+  return make_float2(w.x, w.y);
+#else
+  return *p;
+#endif
+}
+__device__ __forceinline__ double2 ld_stream(const double2* p) {
+#if FTLM_HV_NT
+  const ftlm_d2v w = __builtin_nontemporal_load(reinterpret_cast<const ftlm_d2v*>(p));
+  return make_double2(w.x, w.y);
+#else
+  return *p;
+#endif
+}
+
 #ifndef FTLM_HV_UNROLL
 #define FTLM_HV_UNROLL 8
 #endif
@@ -276,7 +318,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
       V2 o;
       o.x = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
       o.y = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
-      ye[v] = o;
+      st_stream(&ye[v], o);
       if (dotp) {
         const double qr = sxv[v] * static_cast<double>(xo[v].x), qi = sxv[v] * static_cast<double>(xo[v].y);
         dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
@@ -429,13 +471,13 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
     V2* ye = y + eo;
 #pragma unroll
     for (int v = 0; v < VPL; ++v) {
-      const V2 prev = ye[v];
+      const V2 prev = ld_stream(&ye[v]);
       const T ur = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
       const T ui = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
       V2 o;
       o.x = prev.x + ur;
       o.y = prev.y + ui;
-      ye[v] = o;
+      st_stream(&ye[v], o);
       if (dotp) {
         const V2 q = x[eo + v];
         const double qr = sxv[v] * static_cast<double>(q.x), qi = sxv[v] * static_cast<double>(q.y);
