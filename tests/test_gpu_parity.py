@@ -558,19 +558,19 @@ def test_krylov_session_graph_steps():
 def test_gaussian_starts_of_many_samples_on_a_large_sector(oracle):
     """The reference's mt19937 + normal_distribution starts (src/lanczos.cpp:140-175) are drawn on
     the host in parallel over samples and in blocks of indices; each sample must still see the
-    serial sequence.  On 4x3 (5,4) (dim 392,040, two blocks) with 12 samples: the first Lanczos
-    coefficients of every sample equal the oracle's serial recurrence from its own seed
-    (seed + r * 100003) to 1e-12 relative."""
+    serial sequence.  On 4x3 (5,4) (dim 392,040, two blocks) with 40 samples (recurrence groups of
+    32 and 8): the first Lanczos coefficients of samples in both groups equal the oracle's serial
+    recurrence from their own seeds (seed + r * 100003) to 1e-12 relative."""
     lx, ly, nu, nd = 4, 3, 5, 4
-    say("gaussian starts, 12 samples, 4x3 (5,4)")
+    say("gaussian starts, 40 samples, 4x3 (5,4)")
     p = api.HubbardParams(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.05, 0.0)
     p8 = params8(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.05, 0.0)
     op = api.HubbardOperator.from_params(p, nu, nd)
     assert op.dim() == 392040
-    R, M, seed = 12, 3, 17
+    R, M, seed = 40, 3, 17
     rd = api.ftlm_ritz([op], api.FtlmParams(n_random=R, lanczos_steps=M, seed=seed, start=L.FTLM_START_GAUSS_MT,
                                             path=L.FTLM_PATH_PER_OP), keep_tridiagonals=True)
-    for r in (0, 5, 11):
+    for r in (0, 11, 31, 32, 39):
         used, a, b = oracle.lanczos(p8, nu, nd, M, seed + r * 100003)
         assert used == M
         assert np.max(np.abs(rd.alpha[0, r, :M] - a)) <= 1e-12 * np.max(np.abs(a)), r
