@@ -887,7 +887,7 @@ def main():
     tr1, tc1, B1 = kernel_roofline(F, L, op, 1, L.FTLM_C128, torch)
     log(f"H*V c128 R=1: rows {tr1 * 1e3:.3f} ms + columns {tc1 * 1e3:.3f} ms, "
         f"{B1 / (tr1 + tc1) / 1e9:.0f} GB/s algorithmic")
-    # the survey's block H*V table (R in {1, 4, 8}, complex128 and complex64) and the live copy ceiling
+    # the block H*V table (R in {1, 4, 8}, complex128 and complex64) and the live copy ceiling
     hv_table = []
     for Rt in (1, 4, 8):
         for dname, dt, sz in (("c128", L.FTLM_C128, 16), ("c64", L.FTLM_C64, 8)):
