@@ -484,6 +484,53 @@ def kblock_all_workload(F, L, torch):
     return out
 
 
+def kgram_workload(F, L, torch, reps=5):
+    """Momentum Gram blocks (HubbardMomentumBlock) at 4x4 (8,8), t=1, t'=-0.25, U=8, V=1, phi=0,
+    K=(1,0) and K=(0,0): block build time, and the block apply (lift into the full sector, full
+    sector H*V, projection) for R=1 complex128, R=8 complex128 and R=16 complex64 against the full
+    sector apply of the same R beside it.  Median of `reps` HIP-event-timed applies."""
+    import numpy as np
+
+    p = F.HubbardParams(4, 4, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0)
+    sector = F.HubbardOperator.from_params(p, NU, ND)
+    out = {"metric": "4x4 (8,8) momentum Gram block apply", "unit": "ms", "higher_is_better": False,
+           "n_gpus": 1, "data": "synthetic", "dtype": "c128/c64",
+           "config": {"workload": "4x4 (8,8) Gram blocks K=(1,0), (0,0)"}, "blocks": []}
+
+    def timed(op, R, dt, ct):
+        d = op.dim()
+        x = torch.randn(d * R, dtype=ct, device="cuda")
+        y = torch.empty_like(x)
+        op.apply_device(x, y, R=R, dtype=dt)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ts = []
+        for _ in range(reps):
+            e0.record()
+            op.apply_device(x, y, R=R, dtype=dt)
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        del x, y
+        return float(np.median(ts))
+
+    cases = ((1, L.FTLM_C128, torch.complex128), (8, L.FTLM_C128, torch.complex128), (16, L.FTLM_C64, torch.complex64))
+    full = {f"R{R}_{'c64' if dt == L.FTLM_C64 else 'c128'}": round(timed(sector, R, dt, ct), 2) for R, dt, ct in cases}
+    out["full_sector_ms"] = full
+    for kx, ky in ((1, 0), (0, 0)):
+        t0 = time.perf_counter()
+        g = F.KGramOperator.from_params(p, NU, ND, F.MomentumSector(kx, ky, 4, 4), sector=sector)
+        torch.cuda.synchronize()
+        row = {"K": [kx, ky], "dim": g.dim(), "build_s": round(time.perf_counter() - t0, 2)}
+        for R, dt, ct in cases:
+            row[f"R{R}_{'c64' if dt == L.FTLM_C64 else 'c128'}_ms"] = round(timed(g, R, dt, ct), 2)
+        log(str(row))
+        out["blocks"].append(row)
+        del g
+        torch.cuda.empty_cache()
+    return out
+
+
 def hostlink_workload(F, L, torch, reps=3):
     """PCIe-inclusive H*v on the 4x4 (8,8) sector (complex128, one vector), the rate a caller of
     the host-pointer drop-in sees: ftlm_hip_apply_host on pageable numpy arrays (device buffers
@@ -784,7 +831,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", choices=["sector", "kblock", "kblock_all", "gc4x2", "grad", "graph", "reorth", "tune",
-                                                "hostlink"], default="sector",
+                                                "hostlink", "kgram"], default="sector",
                     help="sector: the 4x4 (8,8) full-sector line (default); kblock: the 4x4 (8,8) momentum-block "
                          "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch); grad: "
                          "the differentiated FTLM line (4x3 (6,6) and the 4x4 (8,8) K=(0,0) block)")
@@ -828,6 +875,9 @@ def main():
         return
     if args.workload == "hostlink":
         print(json.dumps(hostlink_workload(F, L, torch)), flush=True)
+        return
+    if args.workload == "kgram":
+        print(json.dumps(kgram_workload(F, L, torch)), flush=True)
         return
     if args.workload == "grad":
         print(json.dumps(grad_workload(F, L, torch, validate=not args.no_validate)), flush=True)
