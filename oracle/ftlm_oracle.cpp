@@ -18,6 +18,8 @@
 //   orc_ftlm_log_partition_*   src/ftlm_thermo.cpp:150-256
 //   orc_gc_density             benchmarks/bench_ftlm_nmu_rect.cpp:186-219
 //   orc_orbit_*                src/orbit_k_block_basis.cpp:56-220, include/ftlm/symmetry/*.hpp
+//   orc_orbit_apply_rows       src/orbit_k_block_basis.cpp:127-220 restricted to chosen rows
+//   orc_orbit_lift             include/ftlm/symmetry/orbit_bloch_phi.hpp:35-51 (Phi_K v)
 //   orc_exact_log_partition    benchmarks/bench_ftlm_nmu_4x4_k.cpp:249-336
 //   orc_dense_extrema          benchmarks/bench_2x2.cpp:28-93,157-178 (1600-rotation Jacobi)
 //   orc_tridiag_eig_ql         converged symmetric tridiagonal eigensolver (implicit QL with
@@ -32,6 +34,10 @@
 //   exact_log_partition: numpy eigvalsh of the dense block (closed form ln sum e^{-beta E}).
 //   tridiag_eig_ql: numpy eigh_tridiagonal / eigvalsh.
 //   orbit dims: Sigma_K dim_K = dim_full identity, reference test values 12 and 66.
+//   orbit_apply_rows: equal to the full block push orc_orbit_apply on 2x2/3x2/4x2/4x3 blocks
+//       (flux included) and refuses a representative list that disagrees with the basis.
+//   orbit_lift: an isometry with H Phi_K v = Phi_K H_K v (orc_apply, orc_orbit_apply) at zero
+//       flux, and not with the conjugated phase convention.
 
 #include <algorithm>
 #include <cmath>
@@ -40,6 +46,7 @@
 #include <cstring>
 #include <limits>
 #include <random>
+#include <thread>
 #include <unordered_map>
 #include <utility>
 #include <vector>
@@ -966,6 +973,139 @@ int orc_orbit_apply(const double* params8, int nu, int nd, int kx, int ky, doubl
   std::vector<Pair> pairs;
   geometry(m, &hops, &pairs);
   orbit_apply(m, ob, hops, pairs, reinterpret_cast<const cplx*>(x), reinterpret_cast<cplx*>(y));
+  return 0;
+}
+
+// Selected rows of y = H_K x for the orbit block (apply_orbit_k_block, src/orbit_k_block_basis.cpp:151-220)
+// without building the sector map: the caller gives the block's sorted representatives (reps,
+// nreps).  For row a every push that lands on it is enumerated: each distinct member s' of a's
+// orbit and each listed hop h that can produce s' name the column state s = h^-1(s'); when s is
+// a representative of the block (canonical, |A| > tol, found in reps) the reference's push term
+//   jw(s, h) * coeff(h) * eta(rep_a, R) * sqrt(|A_a| / |A_b|) * translation_bloch_phase(K, R)
+// is added, with R the first translation (dx outer, dy inner) taking rep_a to s'
+// (map_raw_to_rep, src/orbit_k_block_basis.cpp:127-149) and the jw sign evaluated on the column
+// state s as the reference does.  The diagonal term is added first.  A representative that the
+// oracle finds in the block but the list lacks (or the reverse) returns 3.
+int orc_orbit_apply_rows(const double* params8, int nu, int nd, int kx, int ky, double tol, const uint32_t* reps,
+                         long nreps, const double* x, long nrows, const long* rows, double* out) {
+  const Model m = model_from(params8);
+  const int lx = m.Lx, ly = m.Ly, n = lx * ly;
+  if (n > 16 || nu < 0 || nd < 0 || nu > n || nd > n) return 1;
+  std::vector<Hop> hops;
+  std::vector<Pair> pairs;
+  geometry(m, &hops, &pairs);
+  const cplx* xc = reinterpret_cast<const cplx*>(x);
+  // |A(K)| of a canonical state (orbit_build's stabiliser sum, dy outer / dx inner)
+  auto absA = [&](uint32_t p) {
+    const uint32_t u = p >> 16, d = p & 0xFFFFu;
+    cplx A(0.0, 0.0);
+    for (int dy = 0; dy < ly; ++dy)
+      for (int dx = 0; dx < lx; ++dx)
+        if (pack(tr_bits(u, lx, ly, dx, dy), tr_bits(d, lx, ly, dx, dy)) == p)
+          A += cplx(tr_sign(u, d, lx, ly, dx, dy), 0.0) * bloch(kx, ky, lx, ly, dx, dy);
+    return std::abs(A);
+  };
+  for (long q = 0; q < nrows; ++q) {
+    const long a = rows[q];
+    if (a < 0 || a >= nreps) return 2;
+    const uint32_t pa = reps[a];
+    const uint32_t ua = pa >> 16, da = pa & 0xFFFFu;
+    if (canonical(ua, da, lx, ly) != pa) return 3;
+    const double Aa = absA(pa);
+    if (Aa <= tol) return 3;
+    cplx acc(0.0, 0.0);
+    {
+      cplx dg(static_cast<double>(__builtin_popcount(ua & da)) * m.U, 0.0);
+      for (const Pair& pr : pairs) dg += m.V * static_cast<double>(occ(ua, da, pr.i) * occ(ua, da, pr.j));
+      acc += dg * xc[a];
+    }
+    std::vector<uint32_t> members;
+    for (int dx = 0; dx < lx; ++dx)
+      for (int dy = 0; dy < ly; ++dy) members.push_back(pack(tr_bits(ua, lx, ly, dx, dy), tr_bits(da, lx, ly, dx, dy)));
+    std::sort(members.begin(), members.end());
+    members.erase(std::unique(members.begin(), members.end()), members.end());
+    for (uint32_t sp : members) {
+      const uint32_t u2 = sp >> 16, d2 = sp & 0xFFFFu;
+      // R: first (dx outer, dy inner) translation with T_R rep_a = s'
+      int rx = -1, ry = -1;
+      for (int tx = 0; tx < lx && rx < 0; ++tx)
+        for (int ty = 0; ty < ly; ++ty)
+          if (tr_bits(ua, lx, ly, tx, ty) == u2 && tr_bits(da, lx, ly, tx, ty) == d2) {
+            rx = tx;
+            ry = ty;
+            break;
+          }
+      const double eta = tr_sign(ua, da, lx, ly, rx, ry);
+      const cplx chi = bloch(kx, ky, lx, ly, rx, ry);
+      for (const Hop& h : hops) {
+        const uint32_t tgt = h.spin == 0 ? u2 : d2;  // s' must hold `to` and lack `from`
+        if (h.from == h.to || !((tgt >> h.to) & 1u) || ((tgt >> h.from) & 1u)) continue;
+        const uint32_t flip = (1u << h.from) ^ (1u << h.to);
+        const uint32_t us = h.spin == 0 ? u2 ^ flip : u2, ds = h.spin == 1 ? d2 ^ flip : d2;
+        const uint32_t ps = pack(us, ds);
+        if (canonical(us, ds, lx, ly) != ps) continue;  // columns are representatives
+        const uint32_t* it = std::lower_bound(reps, reps + nreps, ps);
+        const bool listed = it != reps + nreps && *it == ps;
+        const double Ab = absA(ps);
+        if ((Ab > tol) != listed) return 3;
+        if (!listed) continue;
+        const long b = static_cast<long>(it - reps);
+        const cplx xb = xc[b];
+        if (std::norm(xb) < 1e-28) continue;
+        const int nb = between(2 * h.from + h.spin, 2 * h.to + h.spin, us, ds);
+        const cplx hsb = ((nb & 1) ? -1.0 : 1.0) * h.coeff;
+        acc += hsb * cplx(eta * std::sqrt(Aa / Ab), 0.0) * chi * xb;
+      }
+    }
+    out[2 * q] = acc.real();
+    out[2 * q + 1] = acc.imag();
+  }
+  return 0;
+}
+
+// Full-sector lift Phi_K v of a block vector v over the block's sorted representatives: column a
+// of Phi is fill_phi_orbit_bloch_from_seed(seed = rep_a) (include/ftlm/symmetry/orbit_bloch_phi.hpp:35-51),
+//   sum over all (ex, ey) of c * eta(rep_a, e) * conj(translation_bloch_phase(K, e)) at T_e rep_a,
+// c = 1 / sqrt(|G| |Stab_a|) (momentum_sector.hpp:74-77).  Columns of distinct orbits touch
+// disjoint states, so the sum is split over `threads` workers.  out (complex, C(n,nu) C(n,nd))
+// is overwritten.
+int orc_orbit_lift(int lx, int ly, int nu, int nd, int kx, int ky, const uint32_t* reps, long nreps, const double* v,
+                   double* out, int threads) {
+  const int n = lx * ly;
+  if (n > 16 || nu < 0 || nd < 0 || nu > n || nd > n) return 1;
+  Ranker ru, rd;
+  ru.init(n, nu);
+  rd.init(n, nd);
+  const long Dd = rd.binom(n, nd), dim = ru.binom(n, nu) * Dd;
+  const cplx* vc = reinterpret_cast<const cplx*>(v);
+  cplx* oc = reinterpret_cast<cplx*>(out);
+  std::fill(oc, oc + dim, cplx(0.0, 0.0));
+  const double G = static_cast<double>(lx * ly);
+  auto work = [&](long a0, long a1) {
+    for (long a = a0; a < a1; ++a) {
+      const uint32_t u = reps[a] >> 16, d = reps[a] & 0xFFFFu;
+      int stab = 0;
+      for (int ey = 0; ey < ly; ++ey)
+        for (int ex = 0; ex < lx; ++ex)
+          stab += tr_bits(u, lx, ly, ex, ey) == u && tr_bits(d, lx, ly, ex, ey) == d;
+      const double c = 1.0 / std::sqrt(G * static_cast<double>(stab));
+      for (int ey = 0; ey < ly; ++ey)
+        for (int ex = 0; ex < lx; ++ex) {
+          const uint32_t u2 = tr_bits(u, lx, ly, ex, ey), d2 = tr_bits(d, lx, ly, ex, ey);
+          const long idx = ru.rank(u2) * Dd + rd.rank(d2);
+          const double eta = tr_sign(u, d, lx, ly, ex, ey);
+          oc[idx] += c * eta * std::conj(bloch(kx, ky, lx, ly, ex, ey)) * vc[a];
+        }
+    }
+  };
+  const int nt = std::max(1, std::min(threads, 64));
+  std::vector<std::thread> pool;
+  const long per = (nreps + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const long a0 = std::min(nreps, t * per), a1 = std::min(nreps, a0 + per);
+    if (a0 < a1) pool.emplace_back(work, a0, a1);
+  }
+  for (auto& th : pool) th.join();
   return 0;
 }
 

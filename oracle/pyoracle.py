@@ -170,6 +170,9 @@ class Oracle(_Common):
         L.orc_orbit_basis.argtypes = [C.c_int] * 6 + [C.c_double, _u32p, _ip, _ip, _dp, C.c_long]
         L.orc_orbit_map.argtypes = [C.c_int] * 6 + [C.c_double, _u32p, C.c_long, _ip, _ip, _ip, _dp]
         L.orc_orbit_apply.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp]
+        L.orc_orbit_lift.argtypes = [C.c_int] * 6 + [_u32p, C.c_long, C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_orbit_apply_rows.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, _u32p, C.c_long,
+                                           C.c_void_p, C.c_long, _lp, _dp]
         L.orc_translation_sign.argtypes = [C.c_uint32, C.c_uint32] + [C.c_int] * 4
         L.orc_translation_sign.restype = C.c_double
         L.orc_canonical.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.c_int]
@@ -228,6 +231,29 @@ class Oracle(_Common):
         assert x.dtype == np.complex128 and x.flags.c_contiguous
         assert self.lib.orc_apply_rows(p8, nu, nd, x.ctypes.data, len(rows), rows, out) == 0
         return out.view(np.complex128)
+
+    def orbit_apply_rows(self, p8, nu, nd, kx, ky, reps, x, rows):
+        """Rows `rows` of H_K x for the orbit block whose sorted representatives are `reps`
+        (reference push semantics, enumerated from each row's orbit; no sector map)."""
+        reps = np.ascontiguousarray(reps, np.uint32)
+        rows = np.ascontiguousarray(rows, np.int64)
+        out = np.zeros(2 * len(rows))
+        assert x.dtype == np.complex128 and x.flags.c_contiguous and len(x) == len(reps)
+        rc = self.lib.orc_orbit_apply_rows(p8, nu, nd, kx, ky, 1e-12, reps, len(reps), x.ctypes.data, len(rows),
+                                           rows, out)
+        assert rc == 0, f"orc_orbit_apply_rows failed ({rc}: 3 = representative list and basis disagree)"
+        return out.view(np.complex128)
+
+    def orbit_lift(self, lx, ly, nu, nd, kx, ky, reps, v, threads=None):
+        """Phi_K v: the full-sector vector of the block vector v (columns of Phi are the
+        reference's Bloch sums fill_phi_orbit_bloch_from_seed of each representative)."""
+        reps = np.ascontiguousarray(reps, np.uint32)
+        v = np.ascontiguousarray(v, np.complex128)
+        assert len(v) == len(reps)
+        out = np.empty(self.basis_dim(lx * ly, nu, nd), np.complex128)
+        nt = threads or min(16, os.cpu_count() or 1)
+        assert self.lib.orc_orbit_lift(lx, ly, nu, nd, kx, ky, reps, len(reps), v.ctypes.data, out.ctypes.data, nt) == 0
+        return out
 
     def apply_cols(self, p8, nu, nd, x, y, c0, c1):
         assert self.lib.orc_apply_cols(p8, nu, nd, x.view(np.float64), y.view(np.float64), c0, c1) == 0

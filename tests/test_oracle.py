@@ -494,3 +494,43 @@ def test_symmetry_known_answers(oracle):
     kat.check_symmetry_stage2(translate, canonical)
     kat.check_symmetry_stage3(translate, canonical, bloch)
     kat.check_momentum_partition_2x2(translate, canonical, oracle.basis_masks)
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd,kx,ky,phi", [(4, 2, 4, 4, 1, 0, 0.0), (3, 2, 3, 3, 1, 0, 0.0),
+                                                   (4, 2, 2, 3, 1, 1, 0.3), (4, 3, 3, 2, 2, 1, 0.0)])
+def test_orbit_apply_rows_equals_block_push(oracle, lx, ly, nu, nd, kx, ky, phi):
+    """The row-restricted orbit apply (each row's pushes enumerated from its own orbit, no sector
+    map) reproduces the oracle's full block push (itself bit-exact against the reference sources),
+    flux included; a representative list that disagrees with the basis is refused."""
+    p8 = params8(lx, ly, 1.0, -0.25, 8.0, 1.0, phi, 0.0)
+    reps, _, _, _ = oracle.orbit_basis(lx, ly, nu, nd, kx, ky)
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal(len(reps)) + 1j * rng.standard_normal(len(reps))
+    y = oracle.orbit_apply(p8, nu, nd, kx, ky, x)
+    rows = rng.choice(len(reps), size=min(len(reps), 200), replace=False)
+    yr = oracle.orbit_apply_rows(p8, nu, nd, kx, ky, reps, x, rows)
+    assert np.max(np.abs(yr - y[rows])) < 1e-13 * np.max(np.abs(y))
+    with pytest.raises(AssertionError, match="disagree"):
+        oracle.orbit_apply_rows(p8, nu, nd, kx, ky, reps[1:], x[1:], np.arange(len(reps) - 1))
+
+
+@pytest.mark.parametrize("lx,ly,nu,nd,kx,ky", [(4, 2, 4, 4, 1, 0), (3, 2, 3, 3, 1, 0), (4, 2, 2, 3, 1, 1),
+                                               (2, 2, 2, 2, 0, 0), (4, 3, 3, 2, 2, 1)])
+def test_orbit_lift_is_an_isometry_intertwining_h(oracle, lx, ly, nu, nd, kx, ky):
+    """Phi_K (columns: the reference's Bloch sums of each representative) is an isometry and
+    H Phi_K v = Phi_K H_K v at zero flux, with H the sector push and H_K the block push.  This is
+    the identity behind the lifted-start Lanczos check of the momentum blocks; a wrong phase
+    convention, sign or normalisation breaks it."""
+    p8 = params8(lx, ly, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0)
+    reps, _, _, _ = oracle.orbit_basis(lx, ly, nu, nd, kx, ky)
+    rng = np.random.default_rng(2)
+    v = rng.standard_normal(len(reps)) + 1j * rng.standard_normal(len(reps))
+    f = oracle.orbit_lift(lx, ly, nu, nd, kx, ky, reps, v)
+    assert abs(np.linalg.norm(f) - np.linalg.norm(v)) < 1e-13 * np.linalg.norm(v)
+    hf = oracle.apply(p8, nu, nd, f)
+    fh = oracle.orbit_lift(lx, ly, nu, nd, kx, ky, reps, oracle.orbit_apply(p8, nu, nd, kx, ky, v))
+    assert np.max(np.abs(hf - fh)) < 1e-13 * np.max(np.abs(hf))
+    # a conjugated phase convention does not intertwine (except where every phase is real)
+    if (2 * kx) % lx or (2 * ky) % ly:
+        g = oracle.orbit_lift(lx, ly, nu, nd, (-kx) % lx, (-ky) % ly, reps, v)
+        assert np.max(np.abs(oracle.apply(p8, nu, nd, g) - fh)) > 1e-3
