@@ -54,6 +54,12 @@ def build(verbose: bool = False) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
+    # a kernel whose host-side instantiation fails leaves its launch stub undefined without any
+    # diagnostic; refuse such a library here rather than at load time on the GPU
+    r = subprocess.run(["nm", "-D", "--undefined-only", tmp], capture_output=True, text=True)
+    undef = [ln.split()[-1] for ln in r.stdout.splitlines() if "ftlm_hip" in ln]
+    if undef:
+        raise RuntimeError("libftlm_hip.so has undefined library symbols:\n" + "\n".join(undef[:10]))
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

@@ -123,8 +123,23 @@ int64_t binom(int a, int b);
 // ------------------------------------------------------------------ operator (sector / k-block)
 enum OpKind { kSector = 0, kKBlock = 1, kCallback = 2, kGram = 3 };
 
-// Device view of a full (N_up, N_dn) sector with its per-spin pull-hop tables.
-// Hop entry (uint32): bits 0..19 target string rank, 20..30 bond id, 31 own-spin JW parity.
+// Sliced hop tables of the LDS-staged H*V passes.  The strings of one spin are cut into
+// sub-slices of kSliceSub consecutive ranks and the targets into groups of kSliceGroup consecutive
+// ranks.  For group g and sub-slice s the hops of the group's targets whose source lies in s sit
+// at entries [base[g][s], base[g][s+1]) slot-major (slot k of target g*16+i at base + 16k + i),
+// each target's list sorted by source rank and padded with zero-coefficient entries to the
+// group's longest list.  Entry (uint32): bits 0..15 source position inside the sub-slice,
+// bits 16..25 coefficient index (own-spin parity * 512 + bond), as entry >> 20 of the CSR table.
+// Both arrays carry tail padding (kSliceGroupPad empty groups, kSliceEntPad zero entries) so that
+// look-ahead loads past the last group stay in bounds.
+constexpr int kSliceSub = 512;
+constexpr int kSliceGroup = 16;
+constexpr int kSlicePosBits = 16;
+constexpr int kSliceGroupPad = 1024;
+constexpr int kSliceEntPad = 64;
+
+// Device view of a full (N_up, N_dn) sector with its per-spin pull-hop tables (entry layout above:
+// bits 0..19 source string rank, 20..28 bond id, 29 own-spin JW parity).
 struct SectorView {
   int64_t dim;
   int Dup, Ddn;
@@ -153,6 +168,12 @@ struct SectorView {
   int nbond_up, nbond_dn;
   int entT_stride_up, entT_stride_dn;
   int maxcnt_up, maxcnt_dn;
+  // sliced tables (kSliceSub / kSliceGroup layout above); null when not built
+  const int32_t* sl_base_up;   // [ceil(Dup / 16) + kSliceGroupPad][nsub_up + 1]
+  const uint32_t* sl_ent_up;
+  const int32_t* sl_base_dn;   // [ceil(Ddn / 16) + kSliceGroupPad][nsub_dn + 1]
+  const uint32_t* sl_ent_dn;
+  int nsub_up, nsub_dn;
 };
 
 constexpr int kMaxBondsLds = 512;  // LDS slots per spin for bond data (ids < kZeroBond are bonds)
@@ -204,6 +225,9 @@ struct Sector : ftlm_op {
   DevBuf<int32_t> cnt[2];
   int maxcnt[2] = {0, 0};
   int entT_stride[2] = {0, 0};
+  DevBuf<int32_t> sl_base[2];
+  DevBuf<uint32_t> sl_ent[2];
+  int nsub[2] = {0, 0};
   // per-XCD tile counters of the two H*V passes ([0..7] rows pass, [8..15] column pass); each
   // pass zeroes the other pass's counters, so applies on one operator must be stream-ordered
   DevBuf<int32_t> tile_ctr;

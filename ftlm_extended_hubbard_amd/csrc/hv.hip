@@ -490,6 +490,292 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
   if (dotp) store_dot<L, VPL, RB>(dre, dimg, dotp, slot0 + blockIdx.x, RT, r0);
 }
 
+// ------------------------------------------------------------------ LDS-sliced passes
+// One pass handles the hops of one spin.  A "line" is the set of elements that share the other
+// spin's string: a row iu (down hops; targets and sources are ids) or a column id (up hops;
+// targets and sources are ius).  A tile is (line, block of targets); its workgroup keeps the
+// block's sums in registers and streams the line's source elements through LDS, so every hop's
+// source is read from LDS and every element of the line crosses L2 once per tile.
+// Software pipeline:
+//  - two 64 KB LDS buffers; while the gathers of stage st read one, the asynchronous global->LDS
+//    copy of stage st+1 fills the other.  The copy is spread over the gather rounds of the stage:
+//    vector-memory loads retire in order, so a round's wait for its own hop entries then covers
+//    only the copy units issued a round earlier.
+//  - round j's segment bounds are loaded two rounds ahead and its first four entries one round
+//    ahead (the sliced tables carry tail padding, so these loads need no bounds tests).
+// Each target's hops are summed in increasing source order (stages in order, table sorted by
+// source), so the result does not depend on R or on the slice size.  The down pass writes
+// y = diag x + down hops; the up pass adds its sum to y.
+constexpr int kSlBlock = 1024;
+constexpr int kSliceBuf = 64 * 1024;  // bytes of one LDS stage buffer
+
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// Asynchronous copy of `bytes` (4 or 16) from src to LDS: the wavefront's active lanes write
+// lds_wave + lane * bytes (lds_wave must be the same for the whole wavefront).
+template <int BYTES>
+__device__ __forceinline__ void lds_copy_async(const void* src, void* lds_wave);
+template <>
+__device__ __forceinline__ void lds_copy_async<16>(const void* src, void* lds_wave) {
+  __builtin_amdgcn_global_load_lds(src, (lds_void_t*)lds_wave, 16, 0, 0);
+}
+template <>
+__device__ __forceinline__ void lds_copy_async<4>(const void* src, void* lds_wave) {
+  __builtin_amdgcn_global_load_lds(src, (lds_void_t*)lds_wave, 4, 0, 0);
+}
+
+// Waits until every outstanding memory operation of the wavefront has completed.
+__device__ __forceinline__ void wait_all_memory() { __builtin_amdgcn_s_waitcnt(0); }
+
+template <typename T, int RB>
+struct SlSplit {
+  using V2 = typename Cx<T>::v2;
+  static constexpr int BYTES = RB * static_cast<int>(sizeof(V2));  // one element (RB vectors)
+  static constexpr int LB = BYTES >= 32 ? 32 : BYTES;              // bytes per lane
+  static constexpr int L = BYTES / LB;                              // lanes per element
+  static constexpr int VPL = RB / L;                                // vectors per lane
+  static constexpr int TPJ = kSlBlock / L;                          // targets per round
+  static constexpr int NTR = 256 / (VPL * 2 * static_cast<int>(sizeof(T)));  // 64 accumulator registers
+  static constexpr int NT = NTR > 16 ? 16 : NTR;                    // rounds (targets per lane)
+  static constexpr int M = kSliceBuf / (kSliceSub * BYTES);         // sub-slices per stage
+  static constexpr int UB = BYTES >= 16 ? 16 : 4;                   // asynchronous copy unit
+  static constexpr int UPE = BYTES / UB;                            // copy units per element
+  static_assert(RB % L == 0 && M >= 1 && TPJ % kSliceGroup == 0, "bad slice split");
+};
+
+template <typename T, int RB, bool REAL, bool UP>
+__global__ __launch_bounds__(kSlBlock) void k_hv_slice(const SectorView S, const typename Cx<T>::v2* __restrict__ x,
+                                                       typename Cx<T>::v2* __restrict__ y, const int RT, const int r0,
+                                                       const double* __restrict__ xs, double2* __restrict__ dotp,
+                                                       const int64_t slot0, int* __restrict__ ctr, const int nblk,
+                                                       const int ntiles) {
+  using V2 = typename Cx<T>::v2;
+  using SP = SlSplit<T, RB>;
+  constexpr int L = SP::L, VPL = SP::VPL, TPJ = SP::TPJ, NT = SP::NT, M = SP::M;
+  constexpr int GPJ = TPJ / kSliceGroup;  // target groups per round
+  __shared__ uint4 s_buf[2][kSliceBuf / 16];
+  __shared__ V2 s_c[kCoefSlots];  // line-signed coefficients
+  // with one sub-slice per stage: the stage's segment bounds of every group of the block,
+  // copied with the slice ([g][0..1] = base[g][s], base[g][s+1])
+  __shared__ int32_t s_bnd[2][M == 1 ? 2 * NT * GPJ : 1];
+  __shared__ int s_tile;
+  const int D = UP ? S.Dup : S.Ddn;  // targets (and sources) per line
+  const int nsub = UP ? S.nsub_up : S.nsub_dn;
+  const int32_t* __restrict__ sbase = UP ? S.sl_base_up : S.sl_base_dn;
+  const uint32_t* __restrict__ sent = UP ? S.sl_ent_up : S.sl_ent_dn;
+  const double2* __restrict__ coef = UP ? S.coef_up : S.coef_dn;
+  const uint32_t* __restrict__ xmask = UP ? S.xmask_up : S.xmask_dn;
+  const uint32_t* __restrict__ lmask = UP ? S.dn_mask : S.up_mask;  // other spin's string of a line
+  const int nbond = UP ? S.nbond_up : S.nbond_dn;
+  const int64_t lstride = UP ? static_cast<int64_t>(RT) : static_cast<int64_t>(S.Ddn) * RT;
+  const int64_t pstride = UP ? static_cast<int64_t>(S.Ddn) * RT : static_cast<int64_t>(RT);
+  const int xcd = blockIdx.x % kXcd;
+  const int tid = static_cast<int>(threadIdx.x);
+  const int sub = tid % L, el = tid / L;
+  const int gl = el % kSliceGroup;  // lane's place in its target group
+  const int wv = tid / 64;
+  const int per = (ntiles + kXcd - 1) / kXcd;
+  const int t0 = min(ntiles, xcd * per), nmine = min(ntiles, t0 + per) - t0;
+  const int nstage = (nsub + M - 1) / M;
+  double run_re = 0.0, run_im = 0.0;  // threads < RB: this workgroup's <xs x | y> of vector r0 + tid
+  int cur_line = -1;
+  for (int local = next_tile(ctr, xcd, &s_tile); local < nmine; local = next_tile(ctr, xcd, &s_tile)) {
+    const int tile = __builtin_amdgcn_readfirstlane(t0 + local);
+    const int line = tile / nblk;
+    const int tb = (tile - line * nblk) * (NT * TPJ);  // first target of the block (uniform)
+    if (line != cur_line) {  // uniform; the first stage's barrier orders these writes
+      const uint32_t om = lmask[line];
+      for (int b = tid; b < kMaxBondsLds; b += kSlBlock) {
+        V2 c;
+        c.x = c.y = static_cast<T>(0);
+        if (b < nbond) {
+          const double2 cc = coef[b];
+          const bool neg = (__popc(om & xmask[b]) & 1u) != 0;
+          c.x = static_cast<T>(neg ? -cc.x : cc.x);
+          c.y = static_cast<T>(neg ? -cc.y : cc.y);
+        }
+        s_c[b] = c;
+        c.x = -c.x;
+        c.y = -c.y;
+        s_c[b + kMaxBondsLds] = c;
+      }
+      cur_line = line;
+    }
+    const V2* __restrict__ xl = x + line * lstride + r0;  // element p of the line at xl + p * pstride
+    // asynchronous copy of copy-units [u0, u0 + kSlBlock) of stage st into its buffer, one unit per
+    // thread: unit u is piece u % UPE of element u / UPE.  kSlBlock is a multiple of UPE, so a
+    // thread's element offset within each block of units is fixed (one per-lane base).
+    const char* xcp = reinterpret_cast<const char*>(xl + static_cast<int64_t>(tid / SP::UPE) * pstride) +
+                      (tid % SP::UPE) * SP::UB;
+    auto copy_units = [&](int st, int u0) {
+      const int e0 = st * M * kSliceSub;
+      const int nu = min(D - e0, M * kSliceSub) * SP::UPE;
+      if (u0 + tid < nu) {
+        const int64_t eo = static_cast<int64_t>(e0 + u0 / SP::UPE) * pstride;  // uniform
+        lds_copy_async<SP::UB>(xcp + eo * static_cast<int64_t>(sizeof(V2)),
+                               reinterpret_cast<char*>(s_buf[st & 1]) + (u0 + wv * 64) * SP::UB);
+      }
+    };
+    // asynchronous copy of stage st's segment bounds (one sub-slice per stage)
+    auto copy_bounds = [&](int st) {
+      if (M == 1 && tid < 2 * NT * GPJ) {
+        const int32_t* src = sbase + static_cast<int64_t>(tb / kSliceGroup + (tid >> 1)) * (nsub + 1) + st + (tid & 1);
+        lds_copy_async<4>(src, &s_bnd[st & 1][wv * 64]);
+      }
+    };
+    // round j handles target te + j * TPJ: one per-lane base plus uniform offsets
+    const int te = tb + el;
+    const V2* __restrict__ xt = xl + static_cast<int64_t>(te) * pstride + sub * VPL;
+    const int32_t* __restrict__ gbl = sbase + static_cast<int64_t>(te / kSliceGroup) * (nsub + 1);
+    T ar[NT][VPL], ai[NT][VPL];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+#pragma unroll
+      for (int v = 0; v < VPL; ++v) ar[j][v] = ai[j][v] = static_cast<T>(0);
+      if (!UP && te + j * TPJ < D) {  // the down pass starts from diag * x
+        const int t = te + j * TPJ;
+        const uint32_t u = S.up_mask[line], d = S.dn_mask[t];
+        const T dg = static_cast<T>(diag_value(S, u, d, line, t));
+        const V2* xe = xt + static_cast<int64_t>(j * TPJ) * pstride;
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          const V2 q = xe[v];
+          ar[j][v] = dg * q.x;
+          ai[j][v] = dg * q.y;
+        }
+      }
+    }
+    // stage 0's copy
+    copy_bounds(0);
+    for (int u0 = 0; u0 < SP::UPE * M * kSliceSub; u0 += kSlBlock) copy_units(0, u0);
+    for (int st = 0; st < nstage; ++st) {
+      wait_all_memory();
+      __syncthreads();  // stage st has landed; every wave is done with the other buffer
+      const char* sl0 = reinterpret_cast<const char*>(s_buf[st & 1]) + sub * SP::LB;
+      const bool more = st + 1 < nstage;
+      for (int m = 0; m < M; ++m) {
+        const int s = st * M + m;
+        if (s >= nsub) break;
+        const char* sl = sl0 + m * (kSliceSub * SP::BYTES);
+        // raw segment bounds of round j (made wave-uniform only when the round is processed, two
+        // rounds after the load, so that no wait falls on a load issued in the same round)
+        auto seg = [&](int j, int& b0, int& b1) {
+          if (M == 1) {
+            const int w2 = 2 * (j * GPJ + el / kSliceGroup);
+            b0 = s_bnd[st & 1][w2];
+            b1 = s_bnd[st & 1][w2 + 1];
+          } else {
+            const int32_t* gb = gbl + j * GPJ * (nsub + 1) + s;
+            b0 = gb[0];
+            b1 = gb[1];
+          }
+        };
+        auto hop = [&](T (&cr)[VPL], T (&cim)[VPL], uint32_t en) {
+          const V2* sp = reinterpret_cast<const V2*>(sl + (en & ((1u << kSlicePosBits) - 1u)) * SP::BYTES);
+          const V2 c = s_c[en >> kSlicePosBits];
+#pragma unroll
+          for (int v = 0; v < VPL; ++v) {
+            const V2 x2 = sp[v];
+            cr[v] = fmaT(c.x, x2.x, cr[v]);
+            cim[v] = fmaT(c.x, x2.y, cim[v]);
+            if (!REAL) {
+              cr[v] = fmaT(-c.y, x2.y, cr[v]);
+              cim[v] = fmaT(c.y, x2.x, cim[v]);
+            }
+          }
+        };
+        int rA0, rA1, rB0 = 0, rB1 = 0;  // raw bounds of rounds j and j+1 (then j+1 and j+2)
+        uint32_t enA[4];                 // first entries of the next round
+        seg(0, rA0, rA1);
+        if (NT > 1) seg(1, rB0, rB1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) enA[q] = sent[rA0 + gl + q * kSliceGroup];
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          int b0 = rA0, b1 = rA1;
+          if (L == 4) {  // a wavefront is one target group
+            b0 = __builtin_amdgcn_readfirstlane(b0);
+            b1 = __builtin_amdgcn_readfirstlane(b1);
+          }
+          uint32_t en[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) en[q] = enA[q];
+          if (m == 0 && more) {  // a share of the next stage's copy
+            if (j == 0) copy_bounds(st + 1);
+            copy_units(st + 1, j * kSlBlock);
+          }
+          if (j + 1 < NT) {
+            rA0 = rB0;
+            rA1 = rB1;
+            if (j + 2 < NT) seg(j + 2, rB0, rB1);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) enA[q] = sent[rA0 + gl + q * kSliceGroup];
+          }
+          if (te + j * TPJ >= D) continue;
+#pragma unroll
+          for (int q = 0; q < 4; ++q)
+            if (b0 + q * kSliceGroup < b1) hop(ar[j], ai[j], en[q]);
+          for (int e = b0 + 4 * kSliceGroup; e < b1; e += kSliceGroup) hop(ar[j], ai[j], sent[e + gl]);
+        }
+        // copy units beyond NT * 512 (small elements: more units than one per thread and round)
+        if (m == 0 && more)
+          for (int u0 = NT * kSlBlock; u0 < SP::UPE * M * kSliceSub; u0 += kSlBlock) copy_units(st + 1, u0);
+      }
+    }
+    // epilogue: y (down: write; up: add), and the <xs x | y> partials of this pass's part
+    V2* yt = y + (xt - x);
+    double dre[VPL], dimg[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) dre[v] = dimg[v] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+      if (te + j * TPJ >= D) continue;
+      const int64_t jo = static_cast<int64_t>(j * TPJ) * pstride;
+#pragma unroll
+      for (int v = 0; v < VPL; ++v) {
+        const double sx = xs ? xs[r0 + sub * VPL + v] : 1.0;
+        const T ur = static_cast<T>(static_cast<double>(ar[j][v]) * sx);
+        const T ui = static_cast<T>(static_cast<double>(ai[j][v]) * sx);
+        V2 o;
+        if (UP) {
+          const V2 prev = yt[jo + v];
+          o.x = prev.x + ur;
+          o.y = prev.y + ui;
+        } else {
+          o.x = ur;
+          o.y = ui;
+        }
+        yt[jo + v] = o;
+        if (dotp) {
+          const V2 q = xt[jo + v];
+          const double qr = sx * static_cast<double>(q.x), qi = sx * static_cast<double>(q.y);
+          dre[v] += qr * static_cast<double>(ur) + qi * static_cast<double>(ui);
+          dimg[v] += qr * static_cast<double>(ui) - qi * static_cast<double>(ur);
+        }
+      }
+      asm volatile("" ::: "memory");  // keep each round's y traffic together (bounds live registers)
+    }
+    if (dotp) {  // the tile's partials, reduced over the workgroup through the idle stage buffer
+      double* red = reinterpret_cast<double*>(s_buf[(nstage & 1)]);  // [kSlBlock][VPL][2]
+#pragma unroll
+      for (int v = 0; v < VPL; ++v) {
+        red[(tid * VPL + v) * 2] = dre[v];
+        red[(tid * VPL + v) * 2 + 1] = dimg[v];
+      }
+      __syncthreads();
+      if (tid < RB) {
+        const int s2 = tid / VPL, v2 = tid % VPL;
+        for (int jt = s2; jt < kSlBlock; jt += L) {
+          run_re += red[(jt * VPL + v2) * 2];
+          run_im += red[(jt * VPL + v2) * 2 + 1];
+        }
+      }
+    }
+  }
+  queue_done(ctr);
+  if (dotp && tid < RB) dotp[(slot0 + blockIdx.x) * RT + r0 + tid] = make_double2(run_re, run_im);
+}
+
 // ------------------------------------------------------------------ flat kernel (small sectors)
 template <typename T, int RB, bool REAL>
 __global__ __launch_bounds__(kHvBlock) void k_hv_flat(const SectorView S, const typename Cx<T>::v2* __restrict__ x,
@@ -652,6 +938,27 @@ Plan make_plan(const Sector* Sx, int RT, int esize, int L, const HvShape& sh) {
   return p;
 }
 
+// Whether the two-pass sector apply uses the LDS-sliced passes (default) or the L2-gather passes
+// k_hv_dn / k_hv_up (FTLM_HV_SLICE=0, kept for comparison).
+bool use_sliced(const SectorView& S) {
+  static const bool on = env_int("FTLM_HV_SLICE", 1) != 0;
+  return on && S.sl_ent_up != nullptr && S.sl_ent_dn != nullptr;
+}
+
+template <typename T, int RB, bool REAL, bool UP>
+void launch_slice_pass(const SectorView& S, const typename Cx<T>::v2* x, typename Cx<T>::v2* y, int RT, int r0,
+                       const double* xs, double2* dotp, int64_t slot0, int* ctr, int grid, hipStream_t st) {
+  using SP = SlSplit<T, RB>;
+  const int D = UP ? S.Dup : S.Ddn, nline = UP ? S.Ddn : S.Dup;
+  const int nblk = (D + SP::NT * SP::TPJ - 1) / (SP::NT * SP::TPJ);
+  const int64_t nt = static_cast<int64_t>(nline) * nblk;
+  FTLM_REQUIRE(nt < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many sliced tiles");
+  FTLM_REQUIRE(static_cast<int64_t>(SP::NT) * SP::TPJ <= static_cast<int64_t>(kSliceGroupPad) * kSliceGroup + D,
+               FTLM_EUNSUPPORTED, "sliced table padding too small for this launch");
+  hipLaunchKernelGGL((k_hv_slice<T, RB, REAL, UP>), dim3(grid), dim3(kSlBlock), 0, st, S, x, y, RT, r0, xs, dotp,
+                     slot0, ctr, nblk, static_cast<int>(nt));
+}
+
 template <typename T, int RB, bool REAL>
 int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const double* xs, double2* dotp,
            hipStream_t st, const HvShape& sh) {
@@ -661,6 +968,19 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
   const Plan p = make_plan(Sx, RT, sizeof(V2), SP::L, sh);
   const V2* xp = static_cast<const V2*>(x);
   V2* yp = static_cast<V2*>(y);
+  if constexpr (RB * sizeof(V2) <= 128) {  // launches never carry more (16 complex64 / 8 complex128)
+    if (p.two && use_sliced(S)) {
+      // one workgroup per CU (the stage buffers fill most of its LDS): 32 per XCD
+      const int bpx = std::max(1, std::min(32, sh.bpx > 0 ? sh.bpx : 32));
+      const int grid = kXcd * bpx;
+      record_pass_event(0, st);
+      launch_slice_pass<T, RB, REAL, false>(S, xp, yp, RT, r0, xs, dotp, 0, Sx->tile_ctr.p, grid, st);
+      record_pass_event(1, st);
+      launch_slice_pass<T, RB, REAL, true>(S, xp, yp, RT, r0, xs, dotp, grid, Sx->tile_ctr.p + 16, grid, st);
+      record_pass_event(2, st);
+      return 2 * grid;
+    }
+  }
   if (!p.two) {
     record_pass_event(0, st);
     hipLaunchKernelGGL((k_hv_flat<T, RB, REAL>), dim3(p.grid_flat), dim3(kHvBlock), 0, st, S, xp, yp, RT, r0, xs,

@@ -269,10 +269,68 @@ SectorView Sector::view() const {
   v.entT_stride_dn = entT_stride[1];
   v.maxcnt_up = maxcnt[0];
   v.maxcnt_dn = maxcnt[1];
+  v.sl_base_up = sl_base[0].p;
+  v.sl_ent_up = sl_ent[0].p;
+  v.sl_base_dn = sl_base[1].p;
+  v.sl_ent_dn = sl_ent[1].p;
+  v.nsub_up = nsub[0];
+  v.nsub_dn = nsub[1];
   return v;
 }
 
 namespace {
+
+// Sliced copy of one spin's CSR pull-hop table (layout at kSliceSub in common.hpp): every target's
+// hops sorted by source rank, bucketed by source sub-slice, and laid out slot-major per group of
+// kSliceGroup targets with zero-coefficient padding to the group's longest bucket.
+void build_sliced_table(Sector* S, int sp, int D, const std::vector<int32_t>& off, const std::vector<uint32_t>& ent) {
+  const int nsub = (D + kSliceSub - 1) / kSliceSub;
+  const int ng = (D + kSliceGroup - 1) / kSliceGroup;
+  const uint32_t pad = kZeroBond << kSlicePosBits;
+  std::vector<std::vector<uint32_t>> lists(static_cast<size_t>(kSliceGroup));
+  std::vector<int32_t> base(static_cast<size_t>(ng) * (nsub + 1));
+  std::vector<uint32_t> out;
+  out.reserve(ent.size() + ent.size() / 2);
+  std::vector<int> cnt(static_cast<size_t>(kSliceGroup) * nsub);
+  for (int g = 0; g < ng; ++g) {
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (int i = 0; i < kSliceGroup; ++i) {
+      auto& L = lists[static_cast<size_t>(i)];
+      L.clear();
+      const int t = g * kSliceGroup + i;
+      if (t >= D) continue;
+      for (int k = off[t]; k < off[t + 1]; ++k) L.push_back(ent[static_cast<size_t>(k)]);
+      std::sort(L.begin(), L.end(), [](uint32_t a, uint32_t b) { return (a & kTargetMask) < (b & kTargetMask); });
+      for (uint32_t e : L) ++cnt[static_cast<size_t>(i) * nsub + (e & kTargetMask) / kSliceSub];
+    }
+    std::vector<size_t> cur(static_cast<size_t>(kSliceGroup), 0);  // position in each sorted list
+    for (int s = 0; s < nsub; ++s) {
+      base[static_cast<size_t>(g) * (nsub + 1) + s] = static_cast<int32_t>(out.size());
+      int K = 0;
+      for (int i = 0; i < kSliceGroup; ++i) K = std::max(K, cnt[static_cast<size_t>(i) * nsub + s]);
+      const size_t b0 = out.size();
+      out.resize(b0 + static_cast<size_t>(K) * kSliceGroup, pad);
+      for (int i = 0; i < kSliceGroup; ++i) {
+        const int c = cnt[static_cast<size_t>(i) * nsub + s];
+        for (int k = 0; k < c; ++k) {
+          const uint32_t e = lists[static_cast<size_t>(i)][cur[static_cast<size_t>(i)]++];
+          const uint32_t pos = (e & kTargetMask) - static_cast<uint32_t>(s) * kSliceSub;
+          out[b0 + static_cast<size_t>(k) * kSliceGroup + i] = pos | ((e >> kTargetBits) << kSlicePosBits);
+        }
+      }
+    }
+    base[static_cast<size_t>(g) * (nsub + 1) + nsub] = static_cast<int32_t>(out.size());
+    FTLM_REQUIRE(out.size() < (size_t{1} << 31), FTLM_EUNSUPPORTED, "sliced hop table too large");
+  }
+  base.resize(static_cast<size_t>(ng + kSliceGroupPad) * (nsub + 1), static_cast<int32_t>(out.size()));
+  out.resize(out.size() + kSliceEntPad, pad);
+  S->nsub[sp] = nsub;
+  S->sl_base[sp].alloc(base.size());
+  S->sl_base[sp].upload(base);
+  S->sl_ent[sp].alloc(std::max<size_t>(out.size(), 1));
+  S->sl_ent[sp].upload(out);
+  FTLM_HIP_CHECK(hipDeviceSynchronize());
+}
 
 void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair_t* pairs, int n_pairs) {
   const int n = S->n_sites;
@@ -356,6 +414,12 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
                        S->entT[sp].p, S->cnt[sp].p);
     check_launch("k_hop_transpose");
     FTLM_HIP_CHECK(hipDeviceSynchronize());
+    // sliced tables of the LDS-staged passes (sectors whose strings reach the two-pass size)
+    if (S->Ddn >= 128 && S->Dup >= 128) {
+      std::vector<uint32_t> he(static_cast<size_t>(acc));
+      if (acc) FTLM_HIP_CHECK(hipMemcpy(he.data(), S->ent[sp].p, sizeof(uint32_t) * acc, hipMemcpyDeviceToHost));
+      build_sliced_table(S, sp, D, ho, he);
+    }
   }
 
   // diagonal tables
