@@ -734,8 +734,10 @@ __global__ __launch_bounds__(kSlBlock) void k_hv_slice(const SectorView S, const
 #pragma unroll
       for (int v = 0; v < VPL; ++v) {
         const double sx = xs ? xs[r0 + sub * VPL + v] : 1.0;
-        const T ur = static_cast<T>(static_cast<double>(ar[j][v]) * sx);
-        const T ui = static_cast<T>(static_cast<double>(ai[j][v]) * sx);
+        T a_r = ar[j][v], a_i = ai[j][v];
+        asm volatile("" : "+v"(a_r), "+v"(a_i));  // convert round by round (not all rounds up front)
+        const T ur = static_cast<T>(static_cast<double>(a_r) * sx);
+        const T ui = static_cast<T>(static_cast<double>(a_i) * sx);
         V2 o;
         if (UP) {
           const V2 prev = yt[jo + v];
