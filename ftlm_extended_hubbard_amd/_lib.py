@@ -103,6 +103,7 @@ def lib() -> C.CDLL:
     L.ftlm_hip_ftlm_ritz.argtypes = [vp, C.c_int, C.POINTER(FtlmOpts), vp, vp, vp, vp, vp, vp]
     L.ftlm_hip_ftlm_thermo.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_int, vp, C.c_int,
                                        vp, vp, vp, vp, vp]
+    L.ftlm_hip_ftlm_thermo_noalloc.argtypes = L.ftlm_hip_ftlm_thermo.argtypes
     L.ftlm_hip_ftlm_log_partition.argtypes = [vp, vp, C.c_int, C.POINTER(FtlmOpts), vp]
     L.ftlm_hip_quadrature_ritz.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]
     L.ftlm_hip_krylov_create.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint, C.POINTER(vp)]

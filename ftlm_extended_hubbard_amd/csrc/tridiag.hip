@@ -17,6 +17,9 @@
 //   benchmarks/bench_ftlm_nmu_rect.cpp:186-219).
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <string>
 #include <cmath>
 
 #include "common.hpp"
@@ -536,6 +539,49 @@ void thermo_device(const double* d_ritz, const double* d_weight, const int* d_us
   }
 }
 
+// ------------------------------------------------------------------ allocation-free thermodynamics
+// Device workspace of ftlm_hip_ftlm_thermo_noalloc: one file-scope array per device (the loader
+// gives every device its own copy; it lives until the process ends).  Host code never uses the
+// array's name as an address: the per-device address comes from hipGetSymbolAddress.
+constexpr size_t kThermoScratchBytes = size_t{16} << 20;
+static __device__ __align__(256) unsigned char scratch[kThermoScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch_ptr;  // device -> address of that device's `scratch`
+
+// Address of the current device's workspace, looked up once per device.
+unsigned char* thermo_scratch(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return it->second;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr)
+    throw Error(FTLM_EHIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+  g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  return static_cast<unsigned char*>(p);
+}
+
+// Bump allocation of 256-byte aligned pieces of the workspace (no device allocation).
+struct ScratchCarve {
+  unsigned char* base;
+  size_t used = 0;
+  template <class T>
+  T* take(size_t count) {
+    const size_t bytes = (std::max<size_t>(count, 1) * sizeof(T) + 255) / 256 * 256;
+    FTLM_REQUIRE(used + bytes <= kThermoScratchBytes, FTLM_EUNSUPPORTED,
+                 "ftlm_hip_ftlm_thermo_noalloc: inputs exceed the " + std::to_string(kThermoScratchBytes >> 20) +
+                     " MB device workspace (use ftlm_hip_ftlm_thermo)");
+    T* p = reinterpret_cast<T*>(base + used);
+    used += bytes;
+    return p;
+  }
+};
+
+}  // namespace
+
 }  // namespace ftlm_hip
 
 using namespace ftlm_hip;
@@ -659,6 +705,55 @@ int ftlm_hip_quadrature_ritz(const double* alpha, int n, const double* beta, int
     quadrature_device(da.p, db.p, dn.p, dnb.p, 1, 1, n, quad, 0, 1e-14, 1, dr.p, dw.p, n, nullptr);
     FTLM_HIP_CHECK(hipMemcpy(ritz, dr.p, sizeof(double) * n, hipMemcpyDeviceToHost));
     FTLM_HIP_CHECK(hipMemcpy(weight, dw.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+  });
+}
+
+int ftlm_hip_ftlm_thermo_noalloc(const double* ritz, const double* weight, const int* used, int n_ops, int R, int M,
+                                 const int64_t* dims, const int* N, const double* betas, int nb, const double* mus,
+                                 int nmu, double* lnZ_op, double* E_op, double* lnZ_gc, double* N_gc, double* E_gc) {
+  return guarded([&] {
+    FTLM_REQUIRE(n_ops >= 1 && R >= 1 && M >= 1 && nb >= 1 && nmu >= 0, FTLM_EINVAL, "bad sizes");
+    FTLM_REQUIRE(ritz && weight && used && dims && betas, FTLM_EINVAL, "null input");
+    FTLM_REQUIRE(nmu == 0 || (N != nullptr && mus != nullptr), FTLM_EINVAL, "mu grid needs N and mus");
+    int dev = 0;
+    FTLM_HIP_CHECK(hipGetDevice(&dev));
+    // every piece is fully written (inputs by the copies, outputs by the kernels) before it is
+    // read, so the workspace left by an earlier call needs no clearing
+    ScratchCarve sc{thermo_scratch(dev)};
+    const size_t rk = static_cast<size_t>(n_ops) * R * M;
+    const size_t ob = static_cast<size_t>(n_ops) * nb, gb = static_cast<size_t>(nb) * std::max(nmu, 1);
+    double* dr = sc.take<double>(rk);
+    double* dw = sc.take<double>(rk);
+    int* du = sc.take<int>(static_cast<size_t>(n_ops) * R);
+    int64_t* dd = sc.take<int64_t>(static_cast<size_t>(n_ops));
+    double* dbt = sc.take<double>(static_cast<size_t>(nb));
+    double* dmu = sc.take<double>(static_cast<size_t>(std::max(nmu, 1)));
+    int* dN = sc.take<int>(static_cast<size_t>(n_ops));
+    double* dl = sc.take<double>(ob);
+    double* de = sc.take<double>(ob);
+    double* dlg = sc.take<double>(gb);
+    double* dng = sc.take<double>(gb);
+    double* deg = sc.take<double>(gb);
+    FTLM_HIP_CHECK(hipMemcpy(dr, ritz, rk * sizeof(double), hipMemcpyHostToDevice));
+    FTLM_HIP_CHECK(hipMemcpy(dw, weight, rk * sizeof(double), hipMemcpyHostToDevice));
+    FTLM_HIP_CHECK(hipMemcpy(du, used, static_cast<size_t>(n_ops) * R * sizeof(int), hipMemcpyHostToDevice));
+    FTLM_HIP_CHECK(hipMemcpy(dd, dims, static_cast<size_t>(n_ops) * sizeof(int64_t), hipMemcpyHostToDevice));
+    FTLM_HIP_CHECK(hipMemcpy(dbt, betas, static_cast<size_t>(nb) * sizeof(double), hipMemcpyHostToDevice));
+    if (nmu > 0) {
+      FTLM_HIP_CHECK(hipMemcpy(dmu, mus, static_cast<size_t>(nmu) * sizeof(double), hipMemcpyHostToDevice));
+      FTLM_HIP_CHECK(hipMemcpy(dN, N, static_cast<size_t>(n_ops) * sizeof(int), hipMemcpyHostToDevice));
+    }
+    thermo_device(dr, dw, du, n_ops, R, M, dd, nmu > 0 ? dN : nullptr, dbt, nb, dmu, nmu, dl, de, dlg, dng, deg,
+                  nullptr);
+    FTLM_HIP_CHECK(hipDeviceSynchronize());
+    if (lnZ_op) FTLM_HIP_CHECK(hipMemcpy(lnZ_op, dl, ob * sizeof(double), hipMemcpyDeviceToHost));
+    if (E_op) FTLM_HIP_CHECK(hipMemcpy(E_op, de, ob * sizeof(double), hipMemcpyDeviceToHost));
+    if (nmu > 0) {
+      const size_t g = static_cast<size_t>(nb) * nmu * sizeof(double);
+      if (lnZ_gc) FTLM_HIP_CHECK(hipMemcpy(lnZ_gc, dlg, g, hipMemcpyDeviceToHost));
+      if (N_gc) FTLM_HIP_CHECK(hipMemcpy(N_gc, dng, g, hipMemcpyDeviceToHost));
+      if (E_gc) FTLM_HIP_CHECK(hipMemcpy(E_gc, deg, g, hipMemcpyDeviceToHost));
+    }
   });
 }
 

@@ -229,6 +229,17 @@ int ftlm_hip_ftlm_thermo(const double* ritz, const double* weight, const int* us
                          const int64_t* dims, const int* N, const double* betas, int nb, const double* mus,
                          int nmu, double* lnZ_op, double* E_op, double* lnZ_gc, double* N_gc, double* E_gc);
 
+/* ftlm_hip_ftlm_thermo without any device allocation.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (harnesses that forbid hipMalloc, code paths that
+ * must not allocate).  Inputs, outputs and the per-operator intermediate sums are staged in a
+ * file-scope device workspace of 16 MB per device (held until the process ends); requests
+ * that need more return FTLM_EUNSUPPORTED.  Same arguments and results as ftlm_hip_ftlm_thermo, on
+ * the current device, synchronous.  Every call on a device shares that device's workspace:
+ * calls that overlap on one device (other host threads) race. */
+int ftlm_hip_ftlm_thermo_noalloc(const double* ritz, const double* weight, const int* used, int n_ops, int R, int M,
+                                 const int64_t* dims, const int* N, const double* betas, int nb, const double* mus,
+                                 int nmu, double* lnZ_op, double* E_op, double* lnZ_gc, double* N_gc, double* E_gc);
+
 /* ftlm_log_partition_complex / _real (src/ftlm_thermo.cpp:150-256) for a list of betas (the
  * Lanczos data are beta independent).  opts->start = FTLM_START_GAUSS_MT reproduces the
  * reference start vectors.  dim == 1 returns -beta*H00; dim <= 0 returns -inf. */
