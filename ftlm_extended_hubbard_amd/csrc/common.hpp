@@ -123,20 +123,36 @@ int64_t binom(int a, int b);
 // ------------------------------------------------------------------ operator (sector / k-block)
 enum OpKind { kSector = 0, kKBlock = 1, kCallback = 2, kGram = 3 };
 
-// Sliced hop tables of the LDS-staged H*V passes.  The strings of one spin are cut into
-// sub-slices of kSliceSub consecutive ranks and the targets into groups of kSliceGroup consecutive
-// ranks.  For group g and sub-slice s the hops of the group's targets whose source lies in s sit
-// at entries [base[g][s], base[g][s+1]) slot-major (slot k of target g*16+i at base + 16k + i),
-// each target's list sorted by source rank and padded with zero-coefficient entries to the
-// group's longest list.  Entry (uint32): bits 0..15 source position inside the sub-slice,
-// bits 16..25 coefficient index (own-spin parity * 512 + bond), as entry >> 20 of the CSR table.
-// Both arrays carry tail padding (kSliceGroupPad empty groups, kSliceEntPad zero entries) so that
-// look-ahead loads past the last group stay in bounds.
-constexpr int kSliceSub = 512;
-constexpr int kSliceGroup = 16;
-constexpr int kSlicePosBits = 16;
-constexpr int kSliceGroupPad = 1024;
-constexpr int kSliceEntPad = 64;
+// Step tables of the LDS-staged H*V passes (elements of 128 bytes: 16 complex64 or 8 complex128
+// vectors).  One pass handles one spin's hops along lines of D elements that share the other
+// spin's string.  A tile is (line, block of kStepBlock consecutive targets); its 1024 threads own
+// the block's sums in registers: thread tid serves target j * 256 + tid / 4 in round j
+// (kStepRounds rounds) and bytes (tid % 4) * 32 .. +32 of the element, so wavefront w serves the
+// group of kStepGroup targets j * 256 + w * 16 + [0, 16) in round j.  A block's hops are cut into
+// steps; a step covers sources [xoff, xoff + nx) (nx <= kStepSub) and, for each wavefront, at most
+// kStepRows rows of 16 entries: for round j, K_j rows (K_j = the longest hop list of the round's
+// group inside the step's source range), row k holding hop k of each of the 16 targets, padded
+// with zero-coefficient entries.  Each target's hops appear in increasing source order across
+// steps and rows.  Entry (uint32): bits 0..15 the source's byte offset in the step's LDS copy,
+// (source - xoff) * 128, bits 16..31 the byte offset 16 * c of its coefficient c (own-spin
+// parity * 256 + bond; kStepZero is the zero coefficient) in a table of 16-byte slots.
+// Arrays: blk[b] .. blk[b + 1] = the block's steps; desc[step * 16 + w] = {xoff | nx << 20,
+// first row | rows << 26, K_0..K_3 (bytes), K_4..K_7}; ent = rows of 16 entries.
+constexpr int kStepBlock = 2048;
+constexpr int kStepRounds = 8;
+constexpr int kStepWaves = 16;
+constexpr int kStepGroup = 16;
+constexpr int kStepSub = 320;
+constexpr int kStepRows = 32;
+constexpr int kStepPosBits = 16;
+constexpr uint32_t kStepZero = 255;
+constexpr int kStepCoefSlots = 512;
+constexpr int kStepMaxBonds = 255;  // bonds per spin the step tables encode (ids 0..254)
+constexpr int kStepThreads = kStepWaves * 64;         // threads of a step workgroup
+constexpr int kStepRound = kStepBlock / kStepRounds;  // targets per round (4 lanes each)
+constexpr int kStepWgPerCu = 160 * 1024 / ((kStepSub * 128 + kStepWaves * kStepRows * 64) * 2 + 8192 + 64);
+static_assert(kStepRound == kStepWaves * kStepGroup && kStepRound * 4 == kStepThreads, "step tile shape");
+static_assert(kStepRows * 4 <= 128 && kStepWgPerCu >= 1, "step buffers");
 
 // Device view of a full (N_up, N_dn) sector with its per-spin pull-hop tables (entry layout above:
 // bits 0..19 source string rank, 20..28 bond id, 29 own-spin JW parity).
@@ -168,12 +184,13 @@ struct SectorView {
   int nbond_up, nbond_dn;
   int entT_stride_up, entT_stride_dn;
   int maxcnt_up, maxcnt_dn;
-  // sliced tables (kSliceSub / kSliceGroup layout above); null when not built
-  const int32_t* sl_base_up;   // [ceil(Dup / 16) + kSliceGroupPad][nsub_up + 1]
-  const uint32_t* sl_ent_up;
-  const int32_t* sl_base_dn;   // [ceil(Ddn / 16) + kSliceGroupPad][nsub_dn + 1]
-  const uint32_t* sl_ent_dn;
-  int nsub_up, nsub_dn;
+  // step tables (kStepBlock layout above); null when not built
+  const int32_t* st_blk_up;  // [ceil(Dup / kStepBlock) + 1]
+  const uint4* st_desc_up;   // [steps][kStepWaves]
+  const uint32_t* st_ent_up;
+  const int32_t* st_blk_dn;  // [ceil(Ddn / kStepBlock) + 1]
+  const uint4* st_desc_dn;
+  const uint32_t* st_ent_dn;
 };
 
 constexpr int kMaxBondsLds = 512;  // LDS slots per spin for bond data (ids < kZeroBond are bonds)
@@ -225,9 +242,9 @@ struct Sector : ftlm_op {
   DevBuf<int32_t> cnt[2];
   int maxcnt[2] = {0, 0};
   int entT_stride[2] = {0, 0};
-  DevBuf<int32_t> sl_base[2];
-  DevBuf<uint32_t> sl_ent[2];
-  int nsub[2] = {0, 0};
+  DevBuf<int32_t> st_blk[2];
+  DevBuf<uint4> st_desc[2];
+  DevBuf<uint32_t> st_ent[2];
   // per-XCD tile counters of the two H*V passes ([0..7] rows pass, [8..15] column pass); each
   // pass zeroes the other pass's counters, so applies on one operator must be stream-ordered
   DevBuf<int32_t> tile_ctr;

@@ -490,102 +490,89 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
   if (dotp) store_dot<L, VPL, RB>(dre, dimg, dotp, slot0 + blockIdx.x, RT, r0);
 }
 
-// ------------------------------------------------------------------ LDS-sliced passes
-// One pass handles the hops of one spin.  A "line" is the set of elements that share the other
-// spin's string: a row iu (down hops; targets and sources are ids) or a column id (up hops;
-// targets and sources are ius).  A tile is (line, block of targets); its workgroup keeps the
-// block's sums in registers and streams the line's source elements through LDS, so every hop's
-// source is read from LDS and every element of the line crosses L2 once per tile.
-// Software pipeline:
-//  - two 64 KB LDS buffers; while the gathers of stage st read one, the asynchronous global->LDS
-//    copy of stage st+1 fills the other.  The copy is spread over the gather rounds of the stage:
-//    vector-memory loads retire in order, so a round's wait for its own hop entries then covers
-//    only the copy units issued a round earlier.
-//  - round j's segment bounds are loaded two rounds ahead and its first four entries one round
-//    ahead (the sliced tables carry tail padding, so these loads need no bounds tests).
-// Each target's hops are summed in increasing source order (stages in order, table sorted by
-// source), so the result does not depend on R or on the slice size.  The down pass writes
-// y = diag x + down hops; the up pass adds its sum to y.
-constexpr int kSlBlock = 1024;
-constexpr int kSliceBuf = 64 * 1024;  // bytes of one LDS stage buffer
+// ------------------------------------------------------------------ LDS-staged step passes
+// Elements of 128 bytes (16 complex64 or 8 complex128 vectors) use the step tables (layout at
+// kStepBlock in common.hpp).  A workgroup of kStepThreads threads takes tiles (line, block of
+// kStepBlock targets) from its XCD's queue and keeps the block's sums in registers (64 per
+// lane: kStepRounds rounds x 32 bytes of an element, as complex components).  For every step of the
+// block, the step's source elements and each wavefront's entry rows are copied into LDS by
+// asynchronous global->LDS loads one step ahead (two buffers of each, in distinct LDS arrays
+// selected at compile time, so that reads of one buffer never wait for copies into the other);
+// the rounds of a step then read only LDS.  Each step begins with one wait for the copies and
+// one barrier.  A target's hops are added in increasing source order (steps in order, rows in
+// order), the order of the CSR lists, so every H*V kernel gives the same bits.
+// The down pass writes y = diag x + down hops; the up pass adds its sum to y.
 
 typedef __attribute__((address_space(3))) void lds_void_t;
+typedef __attribute__((address_space(4))) const uint32_t const_u32_t;
 
-// Asynchronous copy of `bytes` (4 or 16) from src to LDS: the wavefront's active lanes write
-// lds_wave + lane * bytes (lds_wave must be the same for the whole wavefront).
-template <int BYTES>
-__device__ __forceinline__ void lds_copy_async(const void* src, void* lds_wave);
-template <>
-__device__ __forceinline__ void lds_copy_async<16>(const void* src, void* lds_wave) {
-  __builtin_amdgcn_global_load_lds(src, (lds_void_t*)lds_wave, 16, 0, 0);
+// Read-only 16-byte record at p[i] through the constant address space: with a wave-uniform
+// index the load is a scalar-memory load (counted apart from vector-memory loads).
+__device__ __forceinline__ uint4 load_const4(const uint4* p, int64_t i) {
+  const const_u32_t* q = (const const_u32_t*)(p + i);
+  return make_uint4(q[0], q[1], q[2], q[3]);
 }
-template <>
-__device__ __forceinline__ void lds_copy_async<4>(const void* src, void* lds_wave) {
-  __builtin_amdgcn_global_load_lds(src, (lds_void_t*)lds_wave, 4, 0, 0);
+
+// Asynchronous copy of 16 bytes from src to LDS: the wavefront's active lanes write
+// lds_wave + lane * 16 (lds_wave must be the same for the whole wavefront).
+__device__ __forceinline__ void lds_copy16(const void* src, void* lds_wave) {
+  __builtin_amdgcn_global_load_lds(src, (lds_void_t*)lds_wave, 16, 0, 0);
 }
 
 // Waits until every outstanding memory operation of the wavefront has completed.
 __device__ __forceinline__ void wait_all_memory() { __builtin_amdgcn_s_waitcnt(0); }
 
-template <typename T, int RB>
-struct SlSplit {
+template <typename T, bool REAL, bool UP>
+__global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, const typename Cx<T>::v2* __restrict__ x,
+                                                          typename Cx<T>::v2* __restrict__ y, const int RT, const int r0,
+                                                          const double* __restrict__ xs, double2* __restrict__ dotp,
+                                                          const int64_t slot0, int* __restrict__ ctr, const int nblk,
+                                                          const int ntiles) {
   using V2 = typename Cx<T>::v2;
-  static constexpr int BYTES = RB * static_cast<int>(sizeof(V2));  // one element (RB vectors)
-  static constexpr int LB = BYTES >= 32 ? 32 : BYTES;              // bytes per lane
-  static constexpr int L = BYTES / LB;                              // lanes per element
-  static constexpr int VPL = RB / L;                                // vectors per lane
-  static constexpr int TPJ = kSlBlock / L;                          // targets per round
-  static constexpr int NTR = 256 / (VPL * 2 * static_cast<int>(sizeof(T)));  // 64 accumulator registers
-  static constexpr int NT = NTR > 16 ? 16 : NTR;                    // rounds (targets per lane)
-  static constexpr int M = kSliceBuf / (kSliceSub * BYTES);         // sub-slices per stage
-  static constexpr int UB = BYTES >= 16 ? 16 : 4;                   // asynchronous copy unit
-  static constexpr int UPE = BYTES / UB;                            // copy units per element
-  static_assert(RB % L == 0 && M >= 1 && TPJ % kSliceGroup == 0, "bad slice split");
-};
-
-template <typename T, int RB, bool REAL, bool UP>
-__global__ __launch_bounds__(kSlBlock) void k_hv_slice(const SectorView S, const typename Cx<T>::v2* __restrict__ x,
-                                                       typename Cx<T>::v2* __restrict__ y, const int RT, const int r0,
-                                                       const double* __restrict__ xs, double2* __restrict__ dotp,
-                                                       const int64_t slot0, int* __restrict__ ctr, const int nblk,
-                                                       const int ntiles) {
-  using V2 = typename Cx<T>::v2;
-  using SP = SlSplit<T, RB>;
-  constexpr int L = SP::L, VPL = SP::VPL, TPJ = SP::TPJ, NT = SP::NT, M = SP::M;
-  constexpr int GPJ = TPJ / kSliceGroup;  // target groups per round
-  __shared__ uint4 s_buf[2][kSliceBuf / 16];
-  __shared__ V2 s_c[kCoefSlots];  // line-signed coefficients
-  // with one sub-slice per stage: the stage's segment bounds of every group of the block,
-  // copied with the slice ([g][0..1] = base[g][s], base[g][s+1])
-  __shared__ int32_t s_bnd[2][M == 1 ? 2 * NT * GPJ : 1];
+  constexpr int VPL = 32 / static_cast<int>(sizeof(V2));  // vectors per lane: a lane owns 32 bytes
+  constexpr int RB = 4 * VPL;                             // vectors per launch (128-byte elements)
+  constexpr int NR = kStepRounds;
+  constexpr int kEw = kStepRows * kStepGroup;  // entries of one wavefront's buffer
+  __shared__ uint4 s_x0[kStepSub * 8];
+  __shared__ uint4 s_x1[kStepSub * 8];
+  __shared__ uint32_t s_e0[kStepWaves * kEw];
+  __shared__ uint32_t s_e1[kStepWaves * kEw];
+  __shared__ uint4 s_c[kStepCoefSlots];  // line-signed coefficients [parity * 256 + bond], 16 bytes apart
   __shared__ int s_tile;
-  const int D = UP ? S.Dup : S.Ddn;  // targets (and sources) per line
-  const int nsub = UP ? S.nsub_up : S.nsub_dn;
-  const int32_t* __restrict__ sbase = UP ? S.sl_base_up : S.sl_base_dn;
-  const uint32_t* __restrict__ sent = UP ? S.sl_ent_up : S.sl_ent_dn;
+  const int D = UP ? S.Dup : S.Ddn;
+  const int32_t* __restrict__ blkp = UP ? S.st_blk_up : S.st_blk_dn;
+  // step descriptors are scalar loads (load_const4), so that taking one never waits for a copy
+  const uint4* __restrict__ desc = UP ? S.st_desc_up : S.st_desc_dn;
+  const char* __restrict__ sent = reinterpret_cast<const char*>(UP ? S.st_ent_up : S.st_ent_dn);
   const double2* __restrict__ coef = UP ? S.coef_up : S.coef_dn;
   const uint32_t* __restrict__ xmask = UP ? S.xmask_up : S.xmask_dn;
   const uint32_t* __restrict__ lmask = UP ? S.dn_mask : S.up_mask;  // other spin's string of a line
   const int nbond = UP ? S.nbond_up : S.nbond_dn;
   const int64_t lstride = UP ? static_cast<int64_t>(RT) : static_cast<int64_t>(S.Ddn) * RT;
   const int64_t pstride = UP ? static_cast<int64_t>(S.Ddn) * RT : static_cast<int64_t>(RT);
+  const int64_t pbytes = pstride * static_cast<int64_t>(sizeof(V2));
   const int xcd = blockIdx.x % kXcd;
   const int tid = static_cast<int>(threadIdx.x);
-  const int sub = tid % L, el = tid / L;
-  const int gl = el % kSliceGroup;  // lane's place in its target group
-  const int wv = tid / 64;
+  const int lane = tid & 63;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int sub = tid & 3, el = tid >> 2, gl = el & (kStepGroup - 1);
+  // A lane owns the two 16-byte pieces 2 sub and 2 sub + 1 of its target's elements and reads
+  // them with two 16-byte LDS reads.  Each 8-lane group of such a read serves two targets; the
+  // odd target takes its pieces in the other order, so the group covers all 32 banks whatever
+  // the two source elements.  Accumulator slot v of an odd target holds vector v ^ H.
+  constexpr int H = VPL / 2;
+  const int odd = gl & 1, hx = odd * H;
   const int per = (ntiles + kXcd - 1) / kXcd;
   const int t0 = min(ntiles, xcd * per), nmine = min(ntiles, t0 + per) - t0;
-  const int nstage = (nsub + M - 1) / M;
   double run_re = 0.0, run_im = 0.0;  // threads < RB: this workgroup's <xs x | y> of vector r0 + tid
   int cur_line = -1;
   for (int local = next_tile(ctr, xcd, &s_tile); local < nmine; local = next_tile(ctr, xcd, &s_tile)) {
     const int tile = __builtin_amdgcn_readfirstlane(t0 + local);
     const int line = tile / nblk;
-    const int tb = (tile - line * nblk) * (NT * TPJ);  // first target of the block (uniform)
-    if (line != cur_line) {  // uniform; the first stage's barrier orders these writes
+    const int blk = tile - line * nblk;
+    if (line != cur_line) {  // uniform; the first step's barrier orders these writes
       const uint32_t om = lmask[line];
-      for (int b = tid; b < kMaxBondsLds; b += kSlBlock) {
+      for (int b = tid; b < kStepCoefSlots / 2; b += kStepThreads) {
         V2 c;
         c.x = c.y = static_cast<T>(0);
         if (b < nbond) {
@@ -594,162 +581,199 @@ __global__ __launch_bounds__(kSlBlock) void k_hv_slice(const SectorView S, const
           c.x = static_cast<T>(neg ? -cc.x : cc.x);
           c.y = static_cast<T>(neg ? -cc.y : cc.y);
         }
-        s_c[b] = c;
+        *reinterpret_cast<V2*>(&s_c[b]) = c;
         c.x = -c.x;
         c.y = -c.y;
-        s_c[b + kMaxBondsLds] = c;
+        *reinterpret_cast<V2*>(&s_c[b + kStepCoefSlots / 2]) = c;
       }
       cur_line = line;
     }
     const V2* __restrict__ xl = x + line * lstride + r0;  // element p of the line at xl + p * pstride
-    // asynchronous copy of copy-units [u0, u0 + kSlBlock) of stage st into its buffer, one unit per
-    // thread: unit u is piece u % UPE of element u / UPE.  kSlBlock is a multiple of UPE, so a
-    // thread's element offset within each block of units is fixed (one per-lane base).
-    const char* xcp = reinterpret_cast<const char*>(xl + static_cast<int64_t>(tid / SP::UPE) * pstride) +
-                      (tid % SP::UPE) * SP::UB;
-    auto copy_units = [&](int st, int u0) {
-      const int e0 = st * M * kSliceSub;
-      const int nu = min(D - e0, M * kSliceSub) * SP::UPE;
-      if (u0 + tid < nu) {
-        const int64_t eo = static_cast<int64_t>(e0 + u0 / SP::UPE) * pstride;  // uniform
-        lds_copy_async<SP::UB>(xcp + eo * static_cast<int64_t>(sizeof(V2)),
-                               reinterpret_cast<char*>(s_buf[st & 1]) + (u0 + wv * 64) * SP::UB);
-      }
-    };
-    // asynchronous copy of stage st's segment bounds (one sub-slice per stage)
-    auto copy_bounds = [&](int st) {
-      if (M == 1 && tid < 2 * NT * GPJ) {
-        const int32_t* src = sbase + static_cast<int64_t>(tb / kSliceGroup + (tid >> 1)) * (nsub + 1) + st + (tid & 1);
-        lds_copy_async<4>(src, &s_bnd[st & 1][wv * 64]);
-      }
-    };
-    // round j handles target te + j * TPJ: one per-lane base plus uniform offsets
-    const int te = tb + el;
+    const int te = blk * kStepBlock + el;                 // this thread's target of round 0
     const V2* __restrict__ xt = xl + static_cast<int64_t>(te) * pstride + sub * VPL;
-    const int32_t* __restrict__ gbl = sbase + static_cast<int64_t>(te / kSliceGroup) * (nsub + 1);
-    T ar[NT][VPL], ai[NT][VPL];
+    T ar[NR][VPL], ai[NR][VPL];
 #pragma unroll
-    for (int j = 0; j < NT; ++j) {
+    for (int j = 0; j < NR; ++j)
 #pragma unroll
       for (int v = 0; v < VPL; ++v) ar[j][v] = ai[j][v] = static_cast<T>(0);
-      if (!UP && te + j * TPJ < D) {  // the down pass starts from diag * x
-        const int t = te + j * TPJ;
-        const uint32_t u = S.up_mask[line], d = S.dn_mask[t];
-        const T dg = static_cast<T>(diag_value(S, u, d, line, t));
-        const V2* xe = xt + static_cast<int64_t>(j * TPJ) * pstride;
+    const int st0 = __builtin_amdgcn_readfirstlane(blkp[blk]);
+    const int st1 = __builtin_amdgcn_readfirstlane(blkp[blk + 1]);
+    // copies of one step: thread tid moves 16-byte piece tid % 8 of source element tid / 8 (plus
+    // kStepThreads / 8 elements per further kStepThreads pieces); lane l of wavefront w moves
+    // 16 bytes l (and 64 + l) of w's rows
+    const char* __restrict__ xsrc = reinterpret_cast<const char*>(xl) + (tid >> 3) * pbytes + (tid & 7) * 16;
+    auto issue = [&](const uint4 d, uint4* xb, uint32_t* eb) {
+      const uint32_t xoff = d.x & 0xfffffu, nu = (d.x >> 20) * 8u;
+      const uint32_t row0 = d.y & 0x3ffffffu, ne = (d.y >> 26) * 4u;
+      const char* xs0 = xsrc + static_cast<int64_t>(xoff) * pbytes;
 #pragma unroll
-        for (int v = 0; v < VPL; ++v) {
-          const V2 q = xe[v];
-          ar[j][v] = dg * q.x;
-          ai[j][v] = dg * q.y;
-        }
-      }
-    }
-    // stage 0's copy
-    copy_bounds(0);
-    for (int u0 = 0; u0 < SP::UPE * M * kSliceSub; u0 += kSlBlock) copy_units(0, u0);
-    for (int st = 0; st < nstage; ++st) {
-      wait_all_memory();
-      __syncthreads();  // stage st has landed; every wave is done with the other buffer
-      const char* sl0 = reinterpret_cast<const char*>(s_buf[st & 1]) + sub * SP::LB;
-      const bool more = st + 1 < nstage;
-      for (int m = 0; m < M; ++m) {
-        const int s = st * M + m;
-        if (s >= nsub) break;
-        const char* sl = sl0 + m * (kSliceSub * SP::BYTES);
-        // raw segment bounds of round j (made wave-uniform only when the round is processed, two
-        // rounds after the load, so that no wait falls on a load issued in the same round)
-        auto seg = [&](int j, int& b0, int& b1) {
-          if (M == 1) {
-            const int w2 = 2 * (j * GPJ + el / kSliceGroup);
-            b0 = s_bnd[st & 1][w2];
-            b1 = s_bnd[st & 1][w2 + 1];
-          } else {
-            const int32_t* gb = gbl + j * GPJ * (nsub + 1) + s;
-            b0 = gb[0];
-            b1 = gb[1];
-          }
-        };
-        auto hop = [&](T (&cr)[VPL], T (&cim)[VPL], uint32_t en) {
-          const V2* sp = reinterpret_cast<const V2*>(sl + (en & ((1u << kSlicePosBits) - 1u)) * SP::BYTES);
-          const V2 c = s_c[en >> kSlicePosBits];
-#pragma unroll
-          for (int v = 0; v < VPL; ++v) {
-            const V2 x2 = sp[v];
-            cr[v] = fmaT(c.x, x2.x, cr[v]);
-            cim[v] = fmaT(c.x, x2.y, cim[v]);
-            if (!REAL) {
-              cr[v] = fmaT(-c.y, x2.y, cr[v]);
-              cim[v] = fmaT(c.y, x2.x, cim[v]);
-            }
-          }
-        };
-        int rA0, rA1, rB0 = 0, rB1 = 0;  // raw bounds of rounds j and j+1 (then j+1 and j+2)
-        uint32_t enA[4];                 // first entries of the next round
-        seg(0, rA0, rA1);
-        if (NT > 1) seg(1, rB0, rB1);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) enA[q] = sent[rA0 + gl + q * kSliceGroup];
-#pragma unroll
-        for (int j = 0; j < NT; ++j) {
-          int b0 = rA0, b1 = rA1;
-          if (L == 4) {  // a wavefront is one target group
-            b0 = __builtin_amdgcn_readfirstlane(b0);
-            b1 = __builtin_amdgcn_readfirstlane(b1);
-          }
-          uint32_t en[4];
-#pragma unroll
-          for (int q = 0; q < 4; ++q) en[q] = enA[q];
-          if (m == 0 && more) {  // a share of the next stage's copy
-            if (j == 0) copy_bounds(st + 1);
-            copy_units(st + 1, j * kSlBlock);
-          }
-          if (j + 1 < NT) {
-            rA0 = rB0;
-            rA1 = rB1;
-            if (j + 2 < NT) seg(j + 2, rB0, rB1);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) enA[q] = sent[rA0 + gl + q * kSliceGroup];
-          }
-          if (te + j * TPJ >= D) continue;
-#pragma unroll
-          for (int q = 0; q < 4; ++q)
-            if (b0 + q * kSliceGroup < b1) hop(ar[j], ai[j], en[q]);
-          for (int e = b0 + 4 * kSliceGroup; e < b1; e += kSliceGroup) hop(ar[j], ai[j], sent[e + gl]);
-        }
-        // copy units beyond NT * 512 (small elements: more units than one per thread and round)
-        if (m == 0 && more)
-          for (int u0 = NT * kSlBlock; u0 < SP::UPE * M * kSliceSub; u0 += kSlBlock) copy_units(st + 1, u0);
-      }
-    }
-    // epilogue: y (down: write; up: add), and the <xs x | y> partials of this pass's part
-    V2* yt = y + (xt - x);
-    double dre[VPL], dimg[VPL];
-#pragma unroll
-    for (int v = 0; v < VPL; ++v) dre[v] = dimg[v] = 0.0;
-#pragma unroll
-    for (int j = 0; j < NT; ++j) {
-      if (te + j * TPJ >= D) continue;
-      const int64_t jo = static_cast<int64_t>(j * TPJ) * pstride;
+      for (int it = 0; it < (kStepSub * 8 + kStepThreads - 1) / kStepThreads; ++it)
+        if (static_cast<uint32_t>(it * kStepThreads + tid) < nu)
+          lds_copy16(xs0 + static_cast<int64_t>(it * (kStepThreads / 8)) * pbytes,
+                     reinterpret_cast<char*>(xb) + (it * kStepThreads + w * 64) * 16);
+      const char* es0 = sent + static_cast<int64_t>(row0) * (kStepGroup * 4) + lane * 16;
+      char* ed = reinterpret_cast<char*>(eb + w * kEw);
+      if (static_cast<uint32_t>(lane) < ne) lds_copy16(es0, ed);
+      if (static_cast<uint32_t>(lane + 64) < ne) lds_copy16(es0 + 1024, ed + 1024);
+    };
+    // one hop: its source's two 16-byte pieces and its coefficient (fetch), then the FMAs
+    struct HopData {
+      V2 x[VPL];
+      V2 c;
+    };
+    auto fetch = [&](HopData& h, const char* xb1, const char* xb2, uint32_t en) {
+      // entry = source byte offset | coefficient byte offset << 16 (16-bit halves)
+      const uint32_t eo = en & 0xffffu;
+      // each piece is one 16-byte LDS read (uint4 keeps the compiler from splitting it)
+      const uint4 q1 = *reinterpret_cast<const uint4*>(xb1 + eo);
+      const uint4 q2 = *reinterpret_cast<const uint4*>(xb2 + eo);
+      static_assert(sizeof(V2) * H == sizeof(uint4), "a piece is H vectors");
+      __builtin_memcpy(&h.x[0], &q1, sizeof(uint4));
+      __builtin_memcpy(&h.x[H], &q2, sizeof(uint4));
+      h.c = *static_cast<const V2*>(
+          __builtin_assume_aligned(reinterpret_cast<const char*>(s_c) + (en >> 16), 16));
+    };
+    auto fmas = [&](T (&cr)[VPL], T (&cim)[VPL], const HopData& h) {
 #pragma unroll
       for (int v = 0; v < VPL; ++v) {
-        const double sx = xs ? xs[r0 + sub * VPL + v] : 1.0;
+        const V2 x2 = h.x[v];
+        cr[v] = fmaT(h.c.x, x2.x, cr[v]);
+        cim[v] = fmaT(h.c.x, x2.y, cim[v]);
+        if (!REAL) {
+          cr[v] = fmaT(-h.c.y, x2.y, cr[v]);
+          cim[v] = fmaT(h.c.y, x2.x, cim[v]);
+        }
+      }
+    };
+    uint4 dcur = make_uint4(0u, 0u, 0u, 0u), dnext = dcur;
+    if (st0 < st1) {
+      dcur = load_const4(desc, static_cast<int64_t>(st0) * kStepWaves + w);
+      issue(dcur, s_x0, s_e0);
+    }
+    if (st0 + 1 < st1) dnext = load_const4(desc, static_cast<int64_t>(st0 + 1) * kStepWaves + w);
+    // the down pass starts from diag * x: all loads first (after the first step's copies, so
+    // that their latencies overlap each other and the previous tile's stores), then the products
+    if (!UP) {
+      uint32_t dm[NR];
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const int t = te + j * kStepRound;
+        dm[j] = 0u;
+        if (t < D) {
+          dm[j] = S.dn_mask[t];
+          const V2* xe = xt + static_cast<int64_t>(j * kStepRound) * pstride;
+#pragma unroll
+          for (int v = 0; v < VPL; ++v) {
+            const V2 q = v < H ? xe[hx + v] : xe[(H - hx) + (v - H)];
+            ar[j][v] = q.x;
+            ai[j][v] = q.y;
+          }
+        }
+      }
+      const uint32_t um = S.up_mask[line];
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const int t = te + j * kStepRound;
+        if (t < D) {
+          const T dg = static_cast<T>(diag_value(S, um, dm[j], line, t));
+#pragma unroll
+          for (int v = 0; v < VPL; ++v) {
+            ar[j][v] = dg * ar[j][v];
+            ai[j][v] = dg * ai[j][v];
+          }
+        }
+      }
+    }
+    // step st with buffers P (compile-time): wait for its copies, start the next step's copies,
+    // then the rounds
+    auto step = [&](auto pc, const int st) {
+      constexpr int P = decltype(pc)::value;
+      wait_all_memory();
+      __syncthreads();  // step st has landed; every wavefront is done with the other buffers
+      const uint4 d = dcur;
+      if (st + 1 < st1) issue(dnext, P ? s_x0 : s_x1, P ? s_e0 : s_e1);
+      dcur = dnext;
+      if (st + 2 < st1) dnext = load_const4(desc, static_cast<int64_t>(st + 2) * kStepWaves + w);
+      const char* xb1 = reinterpret_cast<const char*>(P ? s_x1 : s_x0) + (sub * 32 + odd * 16);
+      const char* xb2 = reinterpret_cast<const char*>(P ? s_x1 : s_x0) + (sub * 32 + (16 - odd * 16));
+      // one offset for the whole per-lane base: the compiler's LDS alias information follows only
+      // a few levels of address arithmetic, and reads it loses wait for every pending copy
+      const uint32_t* eb = (P ? s_e1 : s_e0) + (w * kEw + gl);
+      const uint32_t c_lo = __builtin_amdgcn_readfirstlane(d.z), c_hi = __builtin_amdgcn_readfirstlane(d.w);
+      // row r of this wavefront's entries at eb[r * 16]; the row offsets pass through
+      // readfirstlane so that each read stays one offset away from eb (see above).  Rows are
+      // read two ahead of their hops (e0, e1 = rows r, r + 1); reads past the last row are
+      // clamped to row kStepRows - 1 and never used.
+      auto row = [&](int r) { return eb[__builtin_amdgcn_readfirstlane(min(r, kStepRows - 1) * kStepGroup)]; };
+      uint32_t e0 = row(0), e1 = row(1);
+      int r = 0;
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const int cnt = static_cast<int>(((j < 4 ? c_lo : c_hi) >> (8 * (j & 3))) & 0xffu);
+        for (int k = 0; k + 2 <= cnt; k += 2) {
+          HopData ha, hb;
+          fetch(ha, xb1, xb2, e0);
+          fetch(hb, xb1, xb2, e1);
+          e0 = row(r + 2);
+          e1 = row(r + 3);
+          fmas(ar[j], ai[j], ha);
+          fmas(ar[j], ai[j], hb);
+          r += 2;
+        }
+        if (cnt & 1) {
+          HopData ha;
+          fetch(ha, xb1, xb2, e0);
+          e0 = e1;
+          e1 = row(r + 2);
+          fmas(ar[j], ai[j], ha);
+          r += 1;
+        }
+      }
+    };
+    for (int st = st0; st < st1; st += 2) {
+      step(std::integral_constant<int, 0>{}, st);
+      if (st + 1 < st1) step(std::integral_constant<int, 1>{}, st + 1);
+    }
+    // epilogue: y (down: write; up: add), and the <xs x | y> partials of this pass's part.  The
+    // opaque copy of the lane's offset keeps the compiler from holding the diagonal loads'
+    // addresses live across the steps.
+    int64_t eoff = static_cast<int64_t>(te) * pstride + sub * VPL;
+    asm volatile("" : "+v"(eoff));
+    const V2* xta = xl + eoff + hx;        // slots [0, H)
+    const V2* xtb = xl + eoff + (H - hx);  // slots [H, VPL)
+    V2* yta = y + (xta - x);
+    V2* ytb = y + (xtb - x);
+    double dre[VPL], dimg[VPL], sxv[VPL];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+      dre[v] = dimg[v] = 0.0;
+      sxv[v] = xs ? xs[r0 + sub * VPL + (v ^ hx)] : 1.0;
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      if (te + j * kStepRound >= D) continue;
+      const int64_t jo = static_cast<int64_t>(j * kStepRound) * pstride;
+#pragma unroll
+      for (int v = 0; v < VPL; ++v) {
+        const V2* xp = v < H ? xta + v : xtb + (v - H);
+        V2* yp = v < H ? yta + v : ytb + (v - H);
+        const double sx = sxv[v];
         T a_r = ar[j][v], a_i = ai[j][v];
         asm volatile("" : "+v"(a_r), "+v"(a_i));  // convert round by round (not all rounds up front)
         const T ur = static_cast<T>(static_cast<double>(a_r) * sx);
         const T ui = static_cast<T>(static_cast<double>(a_i) * sx);
         V2 o;
         if (UP) {
-          const V2 prev = yt[jo + v];
+          const V2 prev = yp[jo];
           o.x = prev.x + ur;
           o.y = prev.y + ui;
         } else {
           o.x = ur;
           o.y = ui;
         }
-        yt[jo + v] = o;
+        yp[jo] = o;
         if (dotp) {
-          const V2 q = xt[jo + v];
+          const V2 q = xp[jo];
           const double qr = sx * static_cast<double>(q.x), qi = sx * static_cast<double>(q.y);
           dre[v] += qr * static_cast<double>(ur) + qi * static_cast<double>(ui);
           dimg[v] += qr * static_cast<double>(ui) - qi * static_cast<double>(ur);
@@ -757,19 +781,39 @@ __global__ __launch_bounds__(kSlBlock) void k_hv_slice(const SectorView S, const
       }
       asm volatile("" ::: "memory");  // keep each round's y traffic together (bounds live registers)
     }
-    if (dotp) {  // the tile's partials, reduced over the workgroup through the idle stage buffer
-      double* red = reinterpret_cast<double*>(s_buf[(nstage & 1)]);  // [kSlBlock][VPL][2]
+    if (dotp) {
+      // slots back to vector order, then lanes of one wavefront with the same sub hold partials
+      // of the same vectors: fold them (offsets 4..32) and reduce the 16 wavefronts' sums in LDS
 #pragma unroll
-      for (int v = 0; v < VPL; ++v) {
-        red[(tid * VPL + v) * 2] = dre[v];
-        red[(tid * VPL + v) * 2 + 1] = dimg[v];
+      for (int v = 0; v < H; ++v) {
+        const double a = dre[v], b = dimg[v], c2 = dre[v + H], d2 = dimg[v + H];
+        dre[v] = odd ? c2 : a;
+        dimg[v] = odd ? d2 : b;
+        dre[v + H] = odd ? a : c2;
+        dimg[v + H] = odd ? b : d2;
+      }
+#pragma unroll
+      for (int v = 0; v < VPL; ++v)
+#pragma unroll
+        for (int m = 4; m < 64; m <<= 1) {
+          dre[v] += __shfl_xor(dre[v], m);
+          dimg[v] += __shfl_xor(dimg[v], m);
+        }
+      double* red = reinterpret_cast<double*>(s_e0);  // [16 waves][4 subs][VPL][2]
+      __syncthreads();  // every wavefront is done with the last step's entries
+      if (lane < 4) {
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          red[((w * 4 + lane) * VPL + v) * 2] = dre[v];
+          red[((w * 4 + lane) * VPL + v) * 2 + 1] = dimg[v];
+        }
       }
       __syncthreads();
       if (tid < RB) {
         const int s2 = tid / VPL, v2 = tid % VPL;
-        for (int jt = s2; jt < kSlBlock; jt += L) {
-          run_re += red[(jt * VPL + v2) * 2];
-          run_im += red[(jt * VPL + v2) * 2 + 1];
+        for (int ww = 0; ww < kStepWaves; ++ww) {
+          run_re += red[((ww * 4 + s2) * VPL + v2) * 2];
+          run_im += red[((ww * 4 + s2) * VPL + v2) * 2 + 1];
         }
       }
     }
@@ -940,24 +984,21 @@ Plan make_plan(const Sector* Sx, int RT, int esize, int L, const HvShape& sh) {
   return p;
 }
 
-// Whether the two-pass sector apply uses the LDS-sliced passes (default) or the L2-gather passes
-// k_hv_dn / k_hv_up (FTLM_HV_SLICE=0, kept for comparison).
-bool use_sliced(const SectorView& S) {
-  static const bool on = env_int("FTLM_HV_SLICE", 1) != 0;
-  return on && S.sl_ent_up != nullptr && S.sl_ent_dn != nullptr;
+// Whether a two-pass sector apply of 128-byte elements uses the LDS-staged step passes (default)
+// or the L2-gather passes k_hv_dn / k_hv_up (FTLM_HV_STEP=0, kept for comparison).
+bool use_step(const SectorView& S) {
+  static const bool on = env_int("FTLM_HV_STEP", 1) != 0;
+  return on && S.st_ent_up != nullptr && S.st_ent_dn != nullptr;
 }
 
-template <typename T, int RB, bool REAL, bool UP>
-void launch_slice_pass(const SectorView& S, const typename Cx<T>::v2* x, typename Cx<T>::v2* y, int RT, int r0,
-                       const double* xs, double2* dotp, int64_t slot0, int* ctr, int grid, hipStream_t st) {
-  using SP = SlSplit<T, RB>;
+template <typename T, bool REAL, bool UP>
+void launch_step_pass(const SectorView& S, const typename Cx<T>::v2* x, typename Cx<T>::v2* y, int RT, int r0,
+                      const double* xs, double2* dotp, int64_t slot0, int* ctr, int grid, hipStream_t st) {
   const int D = UP ? S.Dup : S.Ddn, nline = UP ? S.Ddn : S.Dup;
-  const int nblk = (D + SP::NT * SP::TPJ - 1) / (SP::NT * SP::TPJ);
+  const int nblk = (D + kStepBlock - 1) / kStepBlock;
   const int64_t nt = static_cast<int64_t>(nline) * nblk;
-  FTLM_REQUIRE(nt < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many sliced tiles");
-  FTLM_REQUIRE(static_cast<int64_t>(SP::NT) * SP::TPJ <= static_cast<int64_t>(kSliceGroupPad) * kSliceGroup + D,
-               FTLM_EUNSUPPORTED, "sliced table padding too small for this launch");
-  hipLaunchKernelGGL((k_hv_slice<T, RB, REAL, UP>), dim3(grid), dim3(kSlBlock), 0, st, S, x, y, RT, r0, xs, dotp,
+  FTLM_REQUIRE(nt < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many step tiles");
+  hipLaunchKernelGGL((k_hv_step<T, REAL, UP>), dim3(grid), dim3(kStepThreads), 0, st, S, x, y, RT, r0, xs, dotp,
                      slot0, ctr, nblk, static_cast<int>(nt));
 }
 
@@ -970,15 +1011,15 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
   const Plan p = make_plan(Sx, RT, sizeof(V2), SP::L, sh);
   const V2* xp = static_cast<const V2*>(x);
   V2* yp = static_cast<V2*>(y);
-  if constexpr (RB * sizeof(V2) <= 128) {  // launches never carry more (16 complex64 / 8 complex128)
-    if (p.two && use_sliced(S)) {
-      // one workgroup per CU (the stage buffers fill most of its LDS): 32 per XCD
-      const int bpx = std::max(1, std::min(32, sh.bpx > 0 ? sh.bpx : 32));
+  if constexpr (RB * sizeof(V2) == 128) {  // 16 complex64 or 8 complex128 vectors
+    if (p.two && use_step(S)) {
+      // as many workgroups per CU as the step buffers fit in its LDS (32 CUs per XCD)
+      const int bpx = std::max(1, std::min(32 * kStepWgPerCu, sh.bpx > 0 ? sh.bpx : 32 * kStepWgPerCu));
       const int grid = kXcd * bpx;
       record_pass_event(0, st);
-      launch_slice_pass<T, RB, REAL, false>(S, xp, yp, RT, r0, xs, dotp, 0, Sx->tile_ctr.p, grid, st);
+      launch_step_pass<T, REAL, false>(S, xp, yp, RT, r0, xs, dotp, 0, Sx->tile_ctr.p, grid, st);
       record_pass_event(1, st);
-      launch_slice_pass<T, RB, REAL, true>(S, xp, yp, RT, r0, xs, dotp, grid, Sx->tile_ctr.p + 16, grid, st);
+      launch_step_pass<T, REAL, true>(S, xp, yp, RT, r0, xs, dotp, grid, Sx->tile_ctr.p + 16, grid, st);
       record_pass_event(2, st);
       return 2 * grid;
     }

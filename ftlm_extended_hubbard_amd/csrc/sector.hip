@@ -127,7 +127,11 @@ __global__ void k_hop_fill(const uint32_t* masks, int D, const BondDev* bonds, i
     const uint32_t src = m ^ (1u << bd.from) ^ (1u << bd.to);
     const uint32_t t = static_cast<uint32_t>(lex_rank(src, n, k));
     const uint32_t par = __popc(m & bd.own) & 1u;
-    ent[w++] = t | (static_cast<uint32_t>(b) << kTargetBits) | (par << kParityBit);
+    const uint32_t e = t | (static_cast<uint32_t>(b) << kTargetBits) | (par << kParityBit);
+    // keep the list sorted by source rank (every H*V kernel adds a target's hops in this order)
+    int q = w++;
+    for (; q > off[i] && (ent[q - 1] & kTargetMask) > t; --q) ent[q] = ent[q - 1];
+    ent[q] = e;
   }
 }
 
@@ -269,66 +273,106 @@ SectorView Sector::view() const {
   v.entT_stride_dn = entT_stride[1];
   v.maxcnt_up = maxcnt[0];
   v.maxcnt_dn = maxcnt[1];
-  v.sl_base_up = sl_base[0].p;
-  v.sl_ent_up = sl_ent[0].p;
-  v.sl_base_dn = sl_base[1].p;
-  v.sl_ent_dn = sl_ent[1].p;
-  v.nsub_up = nsub[0];
-  v.nsub_dn = nsub[1];
+  v.st_blk_up = st_blk[0].p;
+  v.st_desc_up = st_desc[0].p;
+  v.st_ent_up = st_ent[0].p;
+  v.st_blk_dn = st_blk[1].p;
+  v.st_desc_dn = st_desc[1].p;
+  v.st_ent_dn = st_ent[1].p;
   return v;
 }
 
 namespace {
 
-// Sliced copy of one spin's CSR pull-hop table (layout at kSliceSub in common.hpp): every target's
-// hops sorted by source rank, bucketed by source sub-slice, and laid out slot-major per group of
-// kSliceGroup targets with zero-coefficient padding to the group's longest bucket.
-void build_sliced_table(Sector* S, int sp, int D, const std::vector<int32_t>& off, const std::vector<uint32_t>& ent) {
-  const int nsub = (D + kSliceSub - 1) / kSliceSub;
-  const int ng = (D + kSliceGroup - 1) / kSliceGroup;
-  const uint32_t pad = kZeroBond << kSlicePosBits;
-  std::vector<std::vector<uint32_t>> lists(static_cast<size_t>(kSliceGroup));
-  std::vector<int32_t> base(static_cast<size_t>(ng) * (nsub + 1));
-  std::vector<uint32_t> out;
-  out.reserve(ent.size() + ent.size() / 2);
-  std::vector<int> cnt(static_cast<size_t>(kSliceGroup) * nsub);
-  for (int g = 0; g < ng; ++g) {
-    std::fill(cnt.begin(), cnt.end(), 0);
-    for (int i = 0; i < kSliceGroup; ++i) {
-      auto& L = lists[static_cast<size_t>(i)];
-      L.clear();
-      const int t = g * kSliceGroup + i;
-      if (t >= D) continue;
-      for (int k = off[t]; k < off[t + 1]; ++k) L.push_back(ent[static_cast<size_t>(k)]);
-      std::sort(L.begin(), L.end(), [](uint32_t a, uint32_t b) { return (a & kTargetMask) < (b & kTargetMask); });
-      for (uint32_t e : L) ++cnt[static_cast<size_t>(i) * nsub + (e & kTargetMask) / kSliceSub];
-    }
-    std::vector<size_t> cur(static_cast<size_t>(kSliceGroup), 0);  // position in each sorted list
-    for (int s = 0; s < nsub; ++s) {
-      base[static_cast<size_t>(g) * (nsub + 1) + s] = static_cast<int32_t>(out.size());
-      int K = 0;
-      for (int i = 0; i < kSliceGroup; ++i) K = std::max(K, cnt[static_cast<size_t>(i) * nsub + s]);
-      const size_t b0 = out.size();
-      out.resize(b0 + static_cast<size_t>(K) * kSliceGroup, pad);
-      for (int i = 0; i < kSliceGroup; ++i) {
-        const int c = cnt[static_cast<size_t>(i) * nsub + s];
-        for (int k = 0; k < c; ++k) {
-          const uint32_t e = lists[static_cast<size_t>(i)][cur[static_cast<size_t>(i)]++];
-          const uint32_t pos = (e & kTargetMask) - static_cast<uint32_t>(s) * kSliceSub;
-          out[b0 + static_cast<size_t>(k) * kSliceGroup + i] = pos | ((e >> kTargetBits) << kSlicePosBits);
+// Step table of one spin (layout at kStepBlock in common.hpp) from its CSR pull-hop table, whose
+// per-target lists are sorted by source rank.  Each block's hops are taken in increasing source
+// order; a step grows one source at a time and is closed before the source that would make it
+// span more than kStepSub sources or give some wavefront more than kStepRows rows.  One source
+// adds at most one row per group (a target has one hop per source), so at most kStepRounds rows
+// per wavefront, and every step holds at least one source.
+void build_step_table(Sector* S, int sp, int D, const std::vector<int32_t>& off, const std::vector<uint32_t>& ent) {
+  const int nblk = (D + kStepBlock - 1) / kStepBlock;
+  constexpr int kGroups = kStepBlock / kStepGroup;  // 128: group g = j * 16 + w
+  const uint32_t pad = kStepZero * 16u << kStepPosBits;
+  std::vector<int32_t> blk(static_cast<size_t>(nblk) + 1, 0);
+  std::vector<uint4> desc;
+  std::vector<uint32_t> rows;
+  std::vector<uint64_t> trip;                          // source << 32 | local target << 16 | coefficient index
+  std::vector<std::vector<uint32_t>> lst(kStepBlock);  // the step's entries of each target
+  std::vector<int> touched;
+  int K[kGroups], need[kGroups], wrows[kStepWaves];
+  for (int b = 0; b < nblk; ++b) {
+    const int t0 = b * kStepBlock, t1 = std::min(D, t0 + kStepBlock);
+    trip.clear();
+    for (int t = t0; t < t1; ++t)
+      for (int k = off[t]; k < off[t + 1]; ++k) {
+        const uint32_t e = ent[static_cast<size_t>(k)];
+        const uint32_t bond = (e >> kTargetBits) & kBondMask, par = (e >> kParityBit) & 1u;
+        trip.push_back(static_cast<uint64_t>(e & kTargetMask) << 32 | static_cast<uint64_t>(t - t0) << 16 |
+                       (bond | par << 8));
+      }
+    std::sort(trip.begin(), trip.end());
+    size_t i = 0;
+    while (i < trip.size()) {
+      const uint32_t xoff = static_cast<uint32_t>(trip[i] >> 32);
+      uint32_t last = xoff;
+      std::fill(K, K + kGroups, 0);
+      std::fill(wrows, wrows + kStepWaves, 0);
+      for (int t : touched) lst[static_cast<size_t>(t)].clear();
+      touched.clear();
+      while (i < trip.size()) {
+        const uint32_t src = static_cast<uint32_t>(trip[i] >> 32);
+        if (src - xoff >= static_cast<uint32_t>(kStepSub)) break;
+        size_t j = i;
+        while (j < trip.size() && static_cast<uint32_t>(trip[j] >> 32) == src) ++j;
+        std::copy(K, K + kGroups, need);
+        for (size_t q = i; q < j; ++q) {
+          const int t = static_cast<int>((trip[q] >> 16) & 0xffffu), g = t / kStepGroup;
+          need[g] = std::max(need[g], static_cast<int>(lst[static_cast<size_t>(t)].size()) + 1);
         }
+        bool fits = true;
+        int nr[kStepWaves];
+        std::copy(wrows, wrows + kStepWaves, nr);
+        for (int g = 0; g < kGroups; ++g) nr[g % kStepWaves] += need[g] - K[g];
+        for (int w = 0; w < kStepWaves; ++w) fits = fits && nr[w] <= kStepRows;
+        if (!fits) break;  // the step is not empty here (see above)
+        for (size_t q = i; q < j; ++q) {
+          const int t = static_cast<int>((trip[q] >> 16) & 0xffffu);
+          if (lst[static_cast<size_t>(t)].empty()) touched.push_back(t);
+          lst[static_cast<size_t>(t)].push_back((src - xoff) * 128u |
+                                                static_cast<uint32_t>(trip[q] & 0xffffu) * 16u << kStepPosBits);
+        }
+        std::copy(need, need + kGroups, K);
+        std::copy(nr, nr + kStepWaves, wrows);
+        last = src;
+        i = j;
+      }
+      const uint32_t nx = last - xoff + 1;
+      for (int w = 0; w < kStepWaves; ++w) {
+        const size_t row0 = rows.size() / kStepGroup;
+        FTLM_REQUIRE(row0 < (size_t{1} << 26), FTLM_EUNSUPPORTED, "step hop table too large");
+        uint32_t cnt[2] = {0u, 0u};
+        for (int j = 0; j < kStepRounds; ++j) {
+          const int g = j * kStepWaves + w;
+          cnt[j / 4] |= static_cast<uint32_t>(K[g]) << (8 * (j % 4));
+          for (int k = 0; k < K[g]; ++k)
+            for (int q = 0; q < kStepGroup; ++q) {
+              const auto& L = lst[static_cast<size_t>(g * kStepGroup + q)];
+              rows.push_back(k < static_cast<int>(L.size()) ? L[static_cast<size_t>(k)] : pad);
+            }
+        }
+        desc.push_back(make_uint4(xoff | nx << 20, static_cast<uint32_t>(row0) | static_cast<uint32_t>(wrows[w]) << 26,
+                                  cnt[0], cnt[1]));
       }
     }
-    base[static_cast<size_t>(g) * (nsub + 1) + nsub] = static_cast<int32_t>(out.size());
-    FTLM_REQUIRE(out.size() < (size_t{1} << 31), FTLM_EUNSUPPORTED, "sliced hop table too large");
+    blk[static_cast<size_t>(b) + 1] = static_cast<int32_t>(desc.size() / kStepWaves);
   }
-  base.resize(static_cast<size_t>(ng + kSliceGroupPad) * (nsub + 1), static_cast<int32_t>(out.size()));
-  out.resize(out.size() + kSliceEntPad, pad);
-  S->nsub[sp] = nsub;
-  S->sl_base[sp].alloc(base.size());
-  S->sl_base[sp].upload(base);
-  S->sl_ent[sp].alloc(std::max<size_t>(out.size(), 1));
-  S->sl_ent[sp].upload(out);
+  S->st_blk[sp].alloc(blk.size());
+  S->st_blk[sp].upload(blk);
+  S->st_desc[sp].alloc(std::max<size_t>(desc.size(), 1));
+  S->st_desc[sp].upload(desc);
+  S->st_ent[sp].alloc(std::max<size_t>(rows.size(), 1));
+  S->st_ent[sp].upload(rows);
   FTLM_HIP_CHECK(hipDeviceSynchronize());
 }
 
@@ -414,11 +458,11 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
                        S->entT[sp].p, S->cnt[sp].p);
     check_launch("k_hop_transpose");
     FTLM_HIP_CHECK(hipDeviceSynchronize());
-    // sliced tables of the LDS-staged passes (sectors whose strings reach the two-pass size)
-    if (S->Ddn >= 128 && S->Dup >= 128) {
+    // step tables of the LDS-staged passes (sectors whose strings reach the two-pass size)
+    if (S->Ddn >= 128 && S->Dup >= 128 && nb <= kStepMaxBonds) {
       std::vector<uint32_t> he(static_cast<size_t>(acc));
       if (acc) FTLM_HIP_CHECK(hipMemcpy(he.data(), S->ent[sp].p, sizeof(uint32_t) * acc, hipMemcpyDeviceToHost));
-      build_sliced_table(S, sp, D, ho, he);
+      build_step_table(S, sp, D, ho, he);
     }
   }
 
