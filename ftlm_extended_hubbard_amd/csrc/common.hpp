@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <complex>
 #include <map>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -245,9 +246,17 @@ struct Sector : ftlm_op {
   DevBuf<int32_t> st_blk[2];
   DevBuf<uint4> st_desc[2];
   DevBuf<uint32_t> st_ent[2];
-  // per-XCD tile counters of the two H*V passes ([0..7] rows pass, [8..15] column pass); each
-  // pass zeroes the other pass's counters, so applies on one operator must be stream-ordered
-  DevBuf<int32_t> tile_ctr;
+  // per-XCD tile counters of the two H*V passes, one slice of kCtrSlice ints per stream that
+  // applies the operator ([0..8] rows pass, [16..24] column pass; the last workgroup of a pass
+  // returns its counters to zero).  Launches on one stream are ordered, so a slice is never used
+  // by two launches at once; applies from different streams may overlap.
+  static constexpr int kCtrSlice = 32, kCtrStreams = 64;
+  DevBuf<int32_t> tile_ctr;  // [kCtrStreams][kCtrSlice]
+  mutable std::mutex ctr_mu;
+  mutable std::vector<const void*> ctr_stream;  // stream owning each slice, in order of first use
+  // The counter slice of stream st (assigned on its first apply); throws FTLM_EUNSUPPORTED once
+  // kCtrStreams distinct streams hold slices.
+  int32_t* tile_counters(hipStream_t st) const;
   // host copies: directed bond (from, to) and summed coefficient per spin, in bond-id order, and
   // the NN pair list as sorted (min, max) sites (term-split derivative tables are built from them)
   std::vector<std::pair<int, int>> bond_key[2];

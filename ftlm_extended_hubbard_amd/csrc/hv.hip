@@ -1016,10 +1016,11 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
       // as many workgroups per CU as the step buffers fit in its LDS (32 CUs per XCD)
       const int bpx = std::max(1, std::min(32 * kStepWgPerCu, sh.bpx > 0 ? sh.bpx : 32 * kStepWgPerCu));
       const int grid = kXcd * bpx;
+      int32_t* ctr = Sx->tile_counters(st);
       record_pass_event(0, st);
-      launch_step_pass<T, REAL, false>(S, xp, yp, RT, r0, xs, dotp, 0, Sx->tile_ctr.p, grid, st);
+      launch_step_pass<T, REAL, false>(S, xp, yp, RT, r0, xs, dotp, 0, ctr, grid, st);
       record_pass_event(1, st);
-      launch_step_pass<T, REAL, true>(S, xp, yp, RT, r0, xs, dotp, grid, Sx->tile_ctr.p + 16, grid, st);
+      launch_step_pass<T, REAL, true>(S, xp, yp, RT, r0, xs, dotp, grid, ctr + 16, grid, st);
       record_pass_event(2, st);
       return 2 * grid;
     }
@@ -1040,12 +1041,13 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
                "hop table of one H*V tile exceeds 48 KB of LDS");
   FTLM_REQUIRE(static_cast<int64_t>(S.Ddn) * RT < (int64_t{1} << 24), FTLM_EUNSUPPORTED,
                "D_dn * R must stay below 2^24 for the column pass's 24-bit source offsets");
+  int32_t* ctr = Sx->tile_counters(st);
   record_pass_event(0, st);
   hipLaunchKernelGGL((k_hv_dn<T, RB, REAL>), dim3(p.grid_dn), dim3(kHvBlock), lds_dn, st, S, xp, yp, RT, r0, xs,
-                     dotp, Sx->tile_ctr.p, p.nchunk, p.ntiles_dn);
+                     dotp, ctr, p.nchunk, p.ntiles_dn);
   record_pass_event(1, st);
   hipLaunchKernelGGL((k_hv_up<T, RB, REAL>), dim3(p.grid_up), dim3(kHvBlock), lds_up, st, S, xp, yp, RT, r0, xs,
-                     dotp, static_cast<int64_t>(p.grid_dn), Sx->tile_ctr.p + 16, p.wlog, p.tps, p.ntiles_up);
+                     dotp, static_cast<int64_t>(p.grid_dn), ctr + 16, p.wlog, p.tps, p.ntiles_up);
   record_pass_event(2, st);
   return p.grid_dn + p.grid_up;
 }

@@ -242,6 +242,19 @@ int current_device() {
 
 }  // namespace
 
+int32_t* Sector::tile_counters(hipStream_t st) const {
+  std::lock_guard<std::mutex> lk(ctr_mu);
+  const void* key = static_cast<const void*>(st);
+  size_t i = 0;
+  while (i < ctr_stream.size() && ctr_stream[i] != key) ++i;
+  if (i == ctr_stream.size()) {
+    FTLM_REQUIRE(static_cast<int>(i) < kCtrStreams, FTLM_EUNSUPPORTED,
+                 "an operator is applied from at most 64 distinct streams");
+    ctr_stream.push_back(key);
+  }
+  return tile_ctr.p + i * kCtrSlice;
+}
+
 SectorView Sector::view() const {
   SectorView v;
   v.dim = dim;
@@ -484,8 +497,8 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
     for (int s = 0; s < n; ++s)
       FTLM_REQUIRE(deg[s] <= 15, FTLM_EUNSUPPORTED, "a site appears in more than 15 NN pairs");
   }
-  S->tile_ctr.alloc(32);
-  FTLM_HIP_CHECK(hipMemset(S->tile_ctr.p, 0, 32 * sizeof(int32_t)));
+  S->tile_ctr.alloc(static_cast<size_t>(Sector::kCtrStreams) * Sector::kCtrSlice);
+  FTLM_HIP_CHECK(hipMemset(S->tile_ctr.p, 0, S->tile_ctr.n * sizeof(int32_t)));
   DevBuf<int2> dpairs(pr.size());
   dpairs.upload(pr);
   S->nuu.alloc(static_cast<size_t>(S->Dup));

@@ -108,7 +108,9 @@ int ftlm_hip_index_of(const ftlm_op_t* op, const uint64_t* up, const uint64_t* d
 /* ------------------------------------------------------------------ H * v */
 /* y = H x for a block of R vectors (layout above), device pointers, asynchronous on `stream`.
  * Replaces apply_extended_hubbard (src/hubbard_hamiltonian.cpp:51-108) and apply_orbit_k_block
- * (src/orbit_k_block_basis.cpp:151-220).  y is overwritten.  R in [1, 32]. */
+ * (src/orbit_k_block_basis.cpp:151-220).  y is overwritten.  R in [1, 32].  Applies of one
+ * operator on one stream run in stream order; applies from different streams may overlap (each
+ * stream gets its own tile queues, for up to 64 streams per operator; FTLM_EUNSUPPORTED beyond). */
 int ftlm_hip_apply(const ftlm_op_t* op, const void* x, void* y, int R, int dtype, void* stream);
 
 /* ftlm_hip_apply with the row pass (diagonal + down bonds) and the column pass (up bonds) timed
