@@ -461,9 +461,12 @@ def kblock_all_workload(F, L, torch):
         return tot, rel_se
 
     res = {}
-    for name, dt in (("c64", L.FTLM_C64), ("c128", L.FTLM_C128)):
+    for name, dt, path in (("c64", L.FTLM_C64, L.FTLM_PATH_PER_OP), ("c128", L.FTLM_C128, L.FTLM_PATH_PER_OP),
+                           ("c64_stack", L.FTLM_C64, L.FTLM_PATH_STACK)):
+        # per-op: block after block, 16 samples per apply; stack: the 16 blocks side by side, one
+        # momentum-stack apply per step for 2 samples of every block
         par = F.FtlmParams(n_random=32, lanczos_steps=120, seed=11, start=L.FTLM_START_PHASE, dtype=dt,
-                           quad=L.FTLM_QUAD_CONVERGED, path=L.FTLM_PATH_PER_OP)
+                           quad=L.FTLM_QUAD_CONVERGED, path=path)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         rd = F.ftlm_ritz(blocks, par)
@@ -482,6 +485,9 @@ def kblock_all_workload(F, L, torch):
     totf, sef = lnz_stats(rdf, 16)
     log(f"full sector R=16 M=120 c128: {secf:.1f} s")
     d64 = np.abs(res["c64"][0] - res["c128"][0])
+    dstack = np.abs(res["c64_stack"][0] - res["c64"][0])
+    out.update({"c64_stack_s": round(res["c64_stack"][2], 3), "max_abs_dlnZ_stack_vs_per_block": float(dstack.max())})
+    log(f"stacked vs per-block c64 (same starts): max |dlnZ| {dstack.max():.2e}")
     diff = res["c128"][0] - totf
     sig = np.sqrt(res["c128"][1] ** 2 + sef ** 2)
     out.update({"value": round(res["c64"][2], 3), "c128_s": round(res["c128"][2], 3), "full_sector_c128_s": round(secf, 3),

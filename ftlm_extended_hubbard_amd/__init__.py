@@ -18,6 +18,7 @@ from .api import (
     HubbardOperator,
     HubbardParams,
     KBlockOperator,
+    KStackOperator,
     KGramOperator,
     KrylovSession,
     LanczosExtrema,
@@ -58,7 +59,7 @@ __all__ = [
     "HubbardParams", "nsites", "version_string", "RectLattice", "build_hubbard_geometry", "FockBasis",
     "HubbardOperator", "apply_extended_hubbard", "LanczosExtrema", "lanczos_tridiagonal", "lanczos_extrema",
     "tridiagonal_extrema", "FtlmParams", "ftlm_log_partition_complex", "ftlm_log_partition_real", "RitzData",
-    "ftlm_ritz", "ftlm_thermo", "quadrature_ritz", "device_count", "KrylovSession", "MomentumSector", "KBlockOperator", "KGramOperator", "OrbitKRepEntries", "pack_raw_state",
+    "ftlm_ritz", "ftlm_thermo", "quadrature_ritz", "device_count", "KrylovSession", "MomentumSector", "KBlockOperator", "KStackOperator", "KGramOperator", "OrbitKRepEntries", "pack_raw_state",
     "translate_states", "canonical_states", "translation_bloch_phase", "exact_log_partition", "geometry_derivative", "derivative_operator",
     "ftlm_gradient", "ftlm_log_partition_gradient", "ftlm_vjp", "ftlm_log_partition_vjp", "FtlmTape", "ftlm_observables", "ftlm_log_partition_observables", "_lib",
 ]

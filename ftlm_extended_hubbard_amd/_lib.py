@@ -17,7 +17,7 @@ FTLM_OK, FTLM_EINVAL, FTLM_ERANGE, FTLM_EHIP, FTLM_ENOMEM, FTLM_EUNSUPPORTED = r
 FTLM_C128, FTLM_C64 = 0, 1
 FTLM_START_GAUSS_MT, FTLM_START_PHASE = 0, 1
 FTLM_QUAD_REFERENCE, FTLM_QUAD_CONVERGED = 0, 1
-FTLM_PATH_AUTO, FTLM_PATH_PER_OP, FTLM_PATH_PERSISTENT, FTLM_PATH_GRAPH = 0, 1, 2, 3
+FTLM_PATH_AUTO, FTLM_PATH_PER_OP, FTLM_PATH_PERSISTENT, FTLM_PATH_GRAPH, FTLM_PATH_STACK = 0, 1, 2, 3, 4
 FTLM_REORTH_NONE, FTLM_REORTH_FULL, FTLM_REORTH_PARTIAL = 0, 1, 2
 FTLM_TS_AUTO, FTLM_TS_GENERIC, FTLM_TS_FUSED = 0, 1, 2
 
@@ -133,6 +133,9 @@ def lib() -> C.CDLL:
     L.ftlm_hip_kblock_entries.argtypes = [vp, C.c_int64, C.c_int64, vp, vp, vp, vp]
     L.ftlm_hip_kblock_index_of_rep.argtypes = [vp, vp, C.c_int64, vp]
     L.ftlm_hip_kblock_map.argtypes = [vp, vp, C.c_int64, vp, vp, vp, vp]
+    L.ftlm_hip_kstack_create.argtypes = [vp, C.c_int, C.POINTER(vp)]
+    L.ftlm_hip_kstack_scatter.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]
+    L.ftlm_hip_kstack_gather.argtypes = [vp, C.c_int, vp, C.c_int, C.c_int, vp, vp]
     _lib = L
     return L
 

@@ -384,6 +384,42 @@ class KBlockOperator(_Op):
         return idx, dx, dy, eta
 
 
+class KStackOperator(_Op):
+    """The momentum blocks of one sector and Hamiltonian side by side (ftlm_hip_kstack_create):
+    the reference's per-K loop over apply_orbit_k_block (benchmarks/bench_ftlm_nmu_4x4_k.cpp:801-1103)
+    as one operator over the orbits.  Its applies take R = len(blocks) * rk vectors laid out
+    [orbit][block][rk]; `scatter`/`gather` move one block's [dim_j][rk] vectors in and out."""
+
+    @classmethod
+    def from_blocks(cls, blocks) -> "KStackOperator":
+        self = cls()
+        arr = (C.c_void_p * len(blocks))(*[b.handle.value for b in blocks])
+        L.check(L.lib().ftlm_hip_kstack_create(arr, len(blocks), C.byref(self._h)))
+        self._blocks = list(blocks)  # the stack borrows the blocks' row maps
+        return self
+
+    @property
+    def n_blocks(self) -> int:
+        return len(self._blocks)
+
+    def _copy(self, fn, j, src, dst, rk, dtype, stream):
+        sp = src.data_ptr() if hasattr(src, "data_ptr") else int(src)
+        dp = dst.data_ptr() if hasattr(dst, "data_ptr") else int(dst)
+        if stream is None:
+            import torch
+
+            stream = torch.cuda.current_stream().cuda_stream
+        L.check(fn(self._h, j, C.c_void_p(sp), rk, dtype, C.c_void_p(dp), C.c_void_p(stream)))
+
+    def scatter(self, j: int, xb, xs, rk: int = 1, dtype: int = L.FTLM_C128, stream: int | None = None) -> None:
+        """Block j's vectors xb [dim_j][rk] into slots (j, 0..rk) of the stacked block xs."""
+        self._copy(L.lib().ftlm_hip_kstack_scatter, j, xb, xs, rk, dtype, stream)
+
+    def gather(self, j: int, xs, xb, rk: int = 1, dtype: int = L.FTLM_C128, stream: int | None = None) -> None:
+        """Slots (j, 0..rk) of the stacked block xs back into block j's layout xb [dim_j][rk]."""
+        self._copy(L.lib().ftlm_hip_kstack_gather, j, xs, xb, rk, dtype, stream)
+
+
 _APPLY_CACHE: dict = {}
 
 

@@ -8,6 +8,7 @@
 #include <stdexcept>
 #include <complex>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <utility>
@@ -122,7 +123,7 @@ Binom host_binomials();
 int64_t binom(int a, int b);
 
 // ------------------------------------------------------------------ operator (sector / k-block)
-enum OpKind { kSector = 0, kKBlock = 1, kCallback = 2, kGram = 3 };
+enum OpKind { kSector = 0, kKBlock = 1, kCallback = 2, kGram = 3, kKStack = 4 };
 
 // Step tables of the LDS-staged H*V passes (elements of 128 bytes: 16 complex64 or 8 complex128
 // vectors).  One pass handles one spin's hops along lines of D elements that share the other
@@ -298,6 +299,25 @@ int hv_default_grid(const ftlm_op* op, int* grid_max);
 // Default stripe-width log2 of a two-pass sector's column pass for a launch of rb vectors (-1 for
 // a single-pass sector), and the largest one the tile allows.
 int hv_default_wlog(const ftlm_op* op, int R, int dtype, int rb, int* wlog_max);
+// Momentum stacks (orbit.hip): the momentum blocks of one orbit graph side by side, one vector
+// slot per (block j, sample r) over every orbit of the graph, layout [orbit][j][rk].  Slot (j, r) of
+// orbit g holds block j's row map_j[g], or 0 where g is not a row of block j.
+constexpr int kMaxStackBlocks = 16;
+struct KStackMaps {
+  const int32_t* map[kMaxStackBlocks];  // orbit -> row of block j, or -1
+};
+// The orbit graph a momentum block uses (blocks created with share= return the same pointer);
+// nullptr for other operators.
+const void* kblock_graph(const ftlm_op* op);
+// A stack over blocks (all of one orbit graph and device, at most kMaxStackBlocks); the blocks
+// must outlive it.
+std::unique_ptr<ftlm_op> kstack_build(const std::vector<const ftlm_op*>& blocks);
+int kstack_blocks(const ftlm_op* stack);
+KStackMaps kstack_maps(const ftlm_op* stack);
+// Random-phase start of a stack (lanczos.hip): slot (j, r) of orbit g gets the value the per-block
+// start gives block j's row map_j[g] for sample id sample_ids[r] (0 off the block).
+void kstack_phase_start(const ftlm_op* stack, int rk, uint64_t seed, const uint32_t* sample_ids, int dtype, void* q,
+                        hipStream_t st);
 // True when a momentum-block operator is Hermitian (translation-invariant hop coefficients).
 bool kblock_hermitian(const ftlm_op* op);
 

@@ -19,8 +19,10 @@
 // The Gram blocks keep the reference's subspace (orbits with trivial bosonic and fermionic
 // stabiliser characters); their orthonormal basis is not the reference's zheev basis, so ED blocks
 // agree exactly and FTLM blocks in distribution.  Device options beyond the reference:
-// --start=gauss|phase, --quad=capped|converged, --dtype=c128|c64 (defaults reproduce the reference)
-// and --orbit-all=1 (k-orbit: orbit blocks in every sector).
+// --start=gauss|phase, --quad=capped|converged, --dtype=c128|c64 (defaults reproduce the reference),
+// --orbit-all=1 (k-orbit: orbit blocks in every sector) and --path=auto|per-op|stack (the FTLM
+// scheduling: the orbit blocks of a sector run as one ftlm_hip_ftlm_ritz_seeded call, which stacks
+// them under auto where that is faster and under stack always; random-phase starts only).
 #include <sys/resource.h>
 
 #include <algorithm>
@@ -126,7 +128,11 @@ ftlm_ftlm_opts_t make_opts(int argc, char** argv, int R, int M, unsigned seed, i
   o.dtype = (dt && std::strcmp(dt, "c64") == 0) ? FTLM_C64 : FTLM_C128;
   o.clamp_M_to_dim = 0;
   o.real_variant = real_variant;
-  o.path = FTLM_PATH_AUTO;
+  const char* pa = find_arg(argc, argv, "path");  // not in the reference: the device scheduling
+  o.path = !pa                             ? FTLM_PATH_AUTO
+           : std::strcmp(pa, "per-op") == 0 ? FTLM_PATH_PER_OP
+           : std::strcmp(pa, "stack") == 0  ? FTLM_PATH_STACK
+                                            : FTLM_PATH_AUTO;
   return o;
 }
 
@@ -356,6 +362,12 @@ int run_k(int argc, char** argv, KSolver solver, bool solver_set) {
         ftlm_hip_op_destroy(sector_op);
         sector_op = nullptr;
       }
+      // logZ of every K (ED blocks and Gram blocks at once; the orbit blocks' FTLM runs as one call
+      // per sector after the K loop, so that the blocks can share momentum stacks), summed in K order
+      std::vector<double> lz_of_k(static_cast<size_t>(Lx * Ly), -std::numeric_limits<double>::infinity());
+      std::vector<ftlm_op_t*> pend_op, held;
+      std::vector<unsigned> pend_seed;
+      std::vector<int> pend_k;
       for (int ky = 0; ky < Ly; ++ky)
         for (int kx = 0; kx < Lx; ++kx) {
           ftlm_op_t* op = nullptr;
@@ -364,15 +376,15 @@ int run_k(int argc, char** argv, KSolver solver, bool solver_set) {
                                     m.pairs.data(), static_cast<int>(m.pairs.size()), m.p.U, m.p.V, first, &op),
              "kblock_create");
           t_create += since(tc);
-          const bool keep = first == nullptr;
+          bool keep = first == nullptr;
           if (keep) first = op;
           struct Release {
             ftlm_op_t* op;
-            bool keep;
+            const bool* keep;
             ~Release() {
-              if (!keep) ftlm_hip_op_destroy(op);
+              if (!*keep) ftlm_hip_op_destroy(op);
             }
-          } release{op, keep};
+          } release{op, &keep};
           ftlm_op_t* gram = nullptr;
           if (!orbit_direct) ok(ftlm_hip_kgram_create(sector_op, op, &gram), "kgram_create");
           struct ReleaseGram {
@@ -391,19 +403,50 @@ int run_k(int argc, char** argv, KSolver solver, bool solver_set) {
           if (dk <= 0) continue;
           double lz_k = -std::numeric_limits<double>::infinity();
           const auto ts = clock_type::now();
+          const unsigned seed_k = c.seed + static_cast<unsigned>(idx * 257 + ky * 17 + kx);
           if (dk <= ed_cutoff) {
             ++n_ed;
             ok(ftlm_hip_exact_log_partition(blk, &c.beta, 1, &lz_k, nullptr), "exact_log_partition");
             t_ed += since(ts);
+          } else if (gram == nullptr) {  // an orbit block: kept for the sector's batched FTLM
+            ++n_ftlm;
+            pend_op.push_back(op);
+            pend_seed.push_back(seed_k);
+            pend_k.push_back(ky * Lx + kx);
+            if (!keep) held.push_back(op);
+            keep = true;
+            continue;
           } else {
             ++n_ftlm;
-            const unsigned seed_k = c.seed + static_cast<unsigned>(idx * 257 + ky * 17 + kx);
             const ftlm_ftlm_opts_t o = make_opts(argc, argv, c.n_rand, c.lz_steps, seed_k, 0);
             ok(ftlm_hip_ftlm_log_partition(blk, &c.beta, 1, &o, &lz_k), "ftlm_log_partition");
             t_ftlm += since(ts);
           }
-          logZ_sector = logsumexp2(logZ_sector, lz_k);
+          lz_of_k[static_cast<size_t>(ky * Lx + kx)] = lz_k;
         }
+      if (!pend_op.empty()) {
+        // ftlm_log_partition of every pending block, as one call (bench_ftlm_nmu_4x4_k.cpp:1077-1091)
+        const auto ts = clock_type::now();
+        const int np = static_cast<int>(pend_op.size());
+        const ftlm_ftlm_opts_t o = make_opts(argc, argv, c.n_rand, c.lz_steps, pend_seed[0], 0);
+        const size_t per = static_cast<size_t>(c.n_rand) * c.lz_steps;
+        std::vector<double> ritz(per * np), weight(per * np), lz(static_cast<size_t>(np));
+        std::vector<int> used(static_cast<size_t>(c.n_rand) * np);
+        std::vector<int64_t> dims(static_cast<size_t>(np));
+        for (int i = 0; i < np; ++i) ok(ftlm_hip_op_dim(pend_op[i], &dims[i], nullptr, nullptr), "op_dim");
+        ok(ftlm_hip_ftlm_ritz_seeded(pend_op.data(), np, &o, pend_seed.data(), ritz.data(), weight.data(),
+                                     used.data(), nullptr, nullptr, nullptr),
+           "ftlm_ritz_seeded");
+        ok(ftlm_hip_ftlm_thermo(ritz.data(), weight.data(), used.data(), np, c.n_rand, c.lz_steps, dims.data(),
+                                nullptr, &c.beta, 1, nullptr, 0, lz.data(), nullptr, nullptr, nullptr, nullptr),
+           "ftlm_thermo");
+        for (int i = 0; i < np; ++i)  // a single state: -beta H00 (ftlm_log_partition)
+          lz_of_k[static_cast<size_t>(pend_k[i])] = dims[i] == 1 ? -c.beta * ritz[per * i] : lz[i];
+        t_ftlm += since(ts);
+      }
+      for (ftlm_op_t* h : held) ftlm_hip_op_destroy(h);
+      for (double lz_k : lz_of_k)
+        if (lz_k > -std::numeric_limits<double>::infinity()) logZ_sector = logsumexp2(logZ_sector, lz_k);
       if (first) ftlm_hip_op_destroy(first);
       if (sector_op) ftlm_hip_op_destroy(sector_op);
       blocks_ed += n_ed;

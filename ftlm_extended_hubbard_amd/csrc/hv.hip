@@ -1073,9 +1073,11 @@ int hv_partial_blocks(const ftlm_op* op) {
 }
 
 int kblock_default_grid(const ftlm_op* op, int* grid_max);
+int kstack_default_grid(const ftlm_op* op, int* grid_max);
 
 int hv_default_grid(const ftlm_op* op, int* grid_max) {
   if (op->kind == kKBlock) return kblock_default_grid(op, grid_max);
+  if (op->kind == kKStack) return kstack_default_grid(op, grid_max);
   if (grid_max) *grid_max = kHvMaxGrid;
   return static_cast<int>(std::min<int64_t>((op->dim + kHvBlock - 1) / kHvBlock, kHvMaxGrid));
 }
@@ -1136,6 +1138,8 @@ int hv_apply_callback(const ftlm_op* op, const void* x, void* y, int R, int dtyp
                       double2* dotp, hipStream_t st);
 int hv_apply_gram(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
                   hipStream_t st);
+int hv_apply_kstack(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
+                    hipStream_t st);
 
 int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
              hipStream_t st, bool force_real) {
@@ -1144,6 +1148,7 @@ int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const 
   FTLM_REQUIRE(!force_real, FTLM_EUNSUPPORTED, "the real-part apply is defined for full-sector operators only");
   if (op->kind == kCallback) return hv_apply_callback(op, x, y, R, dtype, xs, dotp, st);
   if (op->kind == kGram) return hv_apply_gram(op, x, y, R, dtype, xs, dotp, st);
+  if (op->kind == kKStack) return hv_apply_kstack(op, x, y, R, dtype, xs, dotp, st);
   return hv_apply_kblock(op, x, y, R, dtype, xs, dotp, st);
 }
 

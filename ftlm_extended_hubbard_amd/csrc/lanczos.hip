@@ -241,6 +241,49 @@ __global__ void k_phase_start(int64_t dim, int R, uint64_t seed, const uint32_t*
     }
 }
 
+// The same start on a momentum stack: slot (j, r) of orbit g holds the value k_phase_start gives
+// block j's row i = map_j[g] for sample id sample_ids[j * rk + r], and 0 where g is not a row of
+// block j.
+template <typename T>
+__global__ void k_kstack_phase_start(int64_t nrep, int nk, int rk, KStackMaps maps, uint64_t seed,
+                                     const uint32_t* sample_ids, typename V2Of<T>::t* q) {
+  const int W = nk * rk;
+  for (int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; g < nrep;
+       g += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    for (int j = 0; j < nk; ++j) {
+      const int32_t i = maps.map[j][g];
+      for (int r = 0; r < rk; ++r) {
+        T re = static_cast<T>(0), im = static_cast<T>(0);
+        if (i >= 0) {
+          const uint64_t h =
+              mix64(mix64(seed * 0x9e3779b97f4a7c15ull + sample_ids[j * rk + r]) + static_cast<uint64_t>(i));
+          const double th = 6.283185307179586 * static_cast<double>(h >> 11) * 0x1.0p-53;
+          double s, c;
+          sincos(th, &s, &c);
+          re = static_cast<T>(c);
+          im = static_cast<T>(s);
+        }
+        q[g * W + j * rk + r].x = re;
+        q[g * W + j * rk + r].y = im;
+      }
+    }
+}
+
+void kstack_phase_start(const ftlm_op* stack, int rk, uint64_t seed, const uint32_t* sample_ids, int dtype, void* q,
+                        hipStream_t st) {
+  const int64_t n = stack->dim;
+  if (n <= 0) return;
+  const int nk = kstack_blocks(stack);
+  const unsigned g = static_cast<unsigned>(std::min<int64_t>(8192, (n + 255) / 256));
+  if (dtype == FTLM_C64)
+    hipLaunchKernelGGL(k_kstack_phase_start<float>, dim3(g), dim3(256), 0, st, n, nk, rk, kstack_maps(stack), seed,
+                       sample_ids, static_cast<float2*>(q));
+  else
+    hipLaunchKernelGGL(k_kstack_phase_start<double>, dim3(g), dim3(256), 0, st, n, nk, rk, kstack_maps(stack), seed,
+                       sample_ids, static_cast<double2*>(q));
+  check_launch("k_kstack_phase_start");
+}
+
 // ---------------------------------------------------------------- reorthogonalisation
 // With reorthogonalisation every Krylov block stays resident: slot j holds w_j (unnormalised, as
 // the three-buffer recurrence leaves it) and scl[j][r] = 1/beta_{j-1} makes q_j = scl_j * slot_j.
@@ -808,6 +851,21 @@ void gauss_interleaved(int64_t dim, int R, const std::vector<unsigned>& seeds, s
 void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const double* q0, hipStream_t st) {
   const int R = K.R;
   const int64_t dim = K.dim;
+  if (K.op->kind == kKStack) {  // slot (j, r) starts as block j's sample with seed seeds[j * rk + r]
+    FTLM_REQUIRE(start == FTLM_START_PHASE && q0 == nullptr, FTLM_EUNSUPPORTED,
+                 "a momentum stack takes random-phase starts only");
+    const int nk = kstack_blocks(K.op);
+    FTLM_REQUIRE(R % nk == 0, FTLM_EINVAL, "a momentum stack takes R = blocks * samples per block");
+    const int rk = R / nk;
+    std::vector<uint32_t> ids(static_cast<size_t>(R));
+    for (int v = 0; v < R; ++v) ids[v] = seeds[v];
+    DevBuf<uint32_t> dids(static_cast<size_t>(R));
+    dids.upload(ids, st);
+    kstack_phase_start(K.op, rk, 0x5eedull, dids.p, K.dtype, K.slot(0), st);
+    K.start(st);
+    FTLM_HIP_CHECK(hipStreamSynchronize(st));
+    return;
+  }
   if (start == FTLM_START_PHASE && q0 == nullptr) {
     std::vector<uint32_t> ids(static_cast<size_t>(R));
     for (int r = 0; r < R; ++r) ids[r] = seeds[r];
@@ -1049,6 +1107,104 @@ void ritz_persistent(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* o
   }
 }
 
+// Momentum blocks that share one orbit graph run as momentum stacks: up to 16 blocks side by side,
+// kMaxSamplesPerBlock / blocks samples of each per recurrence group, so one stack apply per step
+// serves every block (the reference loops over K, benchmarks/bench_ftlm_nmu_4x4_k.cpp:801-1103).
+// Slot (j, r) of a group starting at sample g0 is block j's sample g0 + r, started as the per-block
+// path starts it (seed + (g0 + r) * 100003, random phases on the block's rows), and its alpha/beta
+// go to that block's column of the batch arrays [M][S], io * R + g0 + r.  Blocks with fewer than
+// two rows, or fewer rows than M when M is clamped to the dimension, stay on the per-block path.
+void ritz_stacked(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, const unsigned* op_seeds,
+                  double* qa, double* qb, int* qn, int* qnb, int64_t S, int* used, double* alpha_out,
+                  double* beta_out, std::vector<char>* done, std::vector<char>* from_krylov, hipStream_t st) {
+  if (opts->reorth != FTLM_REORTH_NONE || opts->start != FTLM_START_PHASE || opts->real_variant) return;
+  const int R = std::max(1, opts->R), M = std::max(1, opts->M);
+  const int es = opts->dtype == FTLM_C64 ? 8 : 16;
+  // AUTO stacks where the per-block launches would carry elements narrower than 128 bytes (R * es):
+  // at 4x4 (8,8) a stack of 16 blocks then costs 1.8 ms (complex128) / 0.9 ms (complex64) per
+  // sample and block against 5.4 / 2.7 ms per block at R = 1 / 2, while from 128-byte elements up
+  // the per-block applies are as fast per sample (1.7 / 0.83 ms at R = 8 / 16).
+  if (opts->path == FTLM_PATH_AUTO && R * es >= 128) return;
+  std::map<const void*, std::vector<int>> by_graph;  // graph -> operator indices, in call order
+  std::vector<const void*> order;
+  for (int io = 0; io < n_ops; ++io) {
+    if ((*done)[static_cast<size_t>(io)] || ops[io]->kind != kKBlock || ops[io]->dim < 2) continue;
+    if (opts->clamp_M_to_dim && ops[io]->dim < M) continue;
+    const void* g = kblock_graph(ops[io]);
+    if (!by_graph.count(g)) order.push_back(g);
+    by_graph[g].push_back(io);
+  }
+  for (const void* g : order) {
+    const std::vector<int>& all = by_graph[g];
+    if (all.size() < 2) continue;
+    for (size_t c0 = 0; c0 < all.size(); c0 += kMaxStackBlocks) {
+      const std::vector<int> ios(all.begin() + c0, all.begin() + std::min(all.size(), c0 + kMaxStackBlocks));
+      const int nk = static_cast<int>(ios.size());
+      if (nk < 2) break;  // a lone last block runs on the per-block path
+      std::vector<const ftlm_op*> blk;
+      for (int io : ios) blk.push_back(ops[io]);
+      FTLM_HIP_CHECK(hipSetDevice(blk[0]->device));
+      std::unique_ptr<ftlm_op> stack = kstack_build(blk);
+      // 256-byte stacked elements (16 blocks: 2 complex64 or 1 complex128 samples each) were the
+      // fastest per sample at 4x4 (8,8)
+      const int rk_max = std::max(1, std::min(kMaxSamplesPerBlock / nk, 256 / (nk * es)));
+      for (int g0 = 0; g0 < R; g0 += rk_max) {
+        const int rk = std::min(rk_max, R - g0), W = nk * rk;
+        Krylov K;
+        K.init(stack.get(), W, M, opts->dtype, FTLM_REORTH_NONE);
+        std::vector<unsigned> seeds(static_cast<size_t>(W), 0u);
+        for (int j = 0; j < nk; ++j) {
+          const unsigned s0 = op_seeds ? op_seeds[ios[j]] : opts->seed;
+          for (int r = 0; r < rk; ++r) seeds[static_cast<size_t>(j) * rk + r] = s0 + static_cast<unsigned>(g0 + r) * 100003u;
+        }
+        fill_start(K, FTLM_START_PHASE, seeds, nullptr, st);
+        if (opts->path == FTLM_PATH_GRAPH)
+          K.run_graph(0, M, st);
+        else
+          K.run(0, M, st);
+        std::vector<int> hu(static_cast<size_t>(W)), ha(static_cast<size_t>(W));
+        FTLM_HIP_CHECK(hipMemcpyAsync(hu.data(), K.used.p, sizeof(int) * W, hipMemcpyDeviceToHost, st));
+        FTLM_HIP_CHECK(hipMemcpyAsync(ha.data(), K.alive.p, sizeof(int) * W, hipMemcpyDeviceToHost, st));
+        for (int j = 0; j < nk; ++j) {  // slots (j, 0..rk) -> columns io_j * R + g0 ..
+          const int64_t s0 = static_cast<int64_t>(ios[j]) * R + g0;
+          FTLM_HIP_CHECK(hipMemcpy2DAsync(qa + s0, sizeof(double) * S, K.alpha.p + static_cast<int64_t>(j) * rk,
+                                          sizeof(double) * W, sizeof(double) * rk, M, hipMemcpyDeviceToDevice, st));
+          FTLM_HIP_CHECK(hipMemcpy2DAsync(qb + s0, sizeof(double) * S, K.beta.p + static_cast<int64_t>(j) * rk,
+                                          sizeof(double) * W, sizeof(double) * rk, M, hipMemcpyDeviceToDevice, st));
+        }
+        FTLM_HIP_CHECK(hipStreamSynchronize(st));
+        std::vector<double> a, b;
+        if (alpha_out || beta_out) {
+          a.resize(static_cast<size_t>(M) * W);
+          b.resize(static_cast<size_t>(M) * W);
+          FTLM_HIP_CHECK(hipMemcpy(a.data(), K.alpha.p, a.size() * sizeof(double), hipMemcpyDeviceToHost));
+          FTLM_HIP_CHECK(hipMemcpy(b.data(), K.beta.p, b.size() * sizeof(double), hipMemcpyDeviceToHost));
+        }
+        for (int j = 0; j < nk; ++j) {
+          const int io = ios[j];
+          const int64_t s0 = static_cast<int64_t>(io) * R + g0;
+          std::vector<int> hnb(static_cast<size_t>(rk));
+          for (int r = 0; r < rk; ++r) {
+            const int v = j * rk + r;
+            hnb[r] = ha[v] ? hu[v] : hu[v] - 1;
+            used[s0 + r] = hu[v];
+            (*from_krylov)[static_cast<size_t>(s0 + r)] = 1;
+            for (int k = 0; k < M; ++k) {
+              const int64_t o = (s0 + r) * M + k;
+              if (alpha_out) alpha_out[o] = a[static_cast<size_t>(k) * W + v];
+              if (beta_out) beta_out[o] = b[static_cast<size_t>(k) * W + v];
+            }
+          }
+          FTLM_HIP_CHECK(hipMemcpyAsync(qn + s0, hu.data() + j * rk, sizeof(int) * rk, hipMemcpyHostToDevice, st));
+          FTLM_HIP_CHECK(hipMemcpyAsync(qnb + s0, hnb.data(), sizeof(int) * rk, hipMemcpyHostToDevice, st));
+          FTLM_HIP_CHECK(hipStreamSynchronize(st));  // hnb is reused for the next block
+        }
+      }
+      for (int io : ios) (*done)[static_cast<size_t>(io)] = 1;
+    }
+  }
+}
+
 constexpr int64_t kPersistentMaxDim = 262144;     // FTLM_PATH_PERSISTENT
 constexpr int64_t kPersistentAutoMaxDim = 16384;  // FTLM_PATH_AUTO
 
@@ -1087,7 +1243,9 @@ int ftlm_hip_krylov_create(const ftlm_op_t* op, int R, int max_steps, int dtype,
     auto s = std::make_unique<ftlm_krylov>();
     s->K.init(op, R, max_steps, dtype);
     std::vector<unsigned> seeds(static_cast<size_t>(R));
-    for (int r = 0; r < R; ++r) seeds[r] = seed + static_cast<unsigned>(r) * 100003u;
+    // a momentum stack's slot (j, r) starts as block j's sample r
+    const int per = op->kind == kKStack ? R / std::max(1, kstack_blocks(op)) : R;
+    for (int r = 0; r < R; ++r) seeds[r] = seed + static_cast<unsigned>(r % std::max(1, per)) * 100003u;
     fill_start(s->K, start, seeds, nullptr, nullptr);
     *out = s.release();
   });
@@ -1179,15 +1337,27 @@ int ftlm_hip_lanczos_extrema(const ftlm_op_t* op, int max_steps, unsigned seed, 
   return FTLM_OK;
 }
 
-int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, double* ritz, double* weight,
-                       int* used, double* alpha_out, double* beta_out, void* stream) {
-  return guarded([&] {
-    check_opts(opts);
-    FTLM_REQUIRE(ops != nullptr && n_ops >= 1 && ritz && weight && used, FTLM_EINVAL, "null argument");
-    FTLM_REQUIRE(opts->path >= FTLM_PATH_AUTO && opts->path <= FTLM_PATH_GRAPH, FTLM_EINVAL, "bad path");
+// ftlm_hip_ftlm_ritz with an optional seed per operator (op_seeds[io] in place of opts->seed).
+static void ritz_impl(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts0, const unsigned* op_seeds, double* ritz,
+               double* weight, int* used, double* alpha_out, double* beta_out, void* stream) {
+  check_opts(opts0);
+  FTLM_REQUIRE(ops != nullptr && n_ops >= 1 && ritz && weight && used, FTLM_EINVAL, "null argument");
+  FTLM_REQUIRE(opts0->path >= FTLM_PATH_AUTO && opts0->path <= FTLM_PATH_STACK, FTLM_EINVAL, "bad path");
+  ftlm_ftlm_opts_t opts_one = *opts0;
+  const ftlm_ftlm_opts_t* opts = &opts_one;
+  // one seed for every operator: the plain call (the persistent launch draws from opts->seed)
+  bool one_seed = true;
+  if (op_seeds) {
+    for (int io = 1; io < n_ops; ++io) one_seed = one_seed && op_seeds[io] == op_seeds[0];
+    if (one_seed) {
+      opts_one.seed = op_seeds[0];
+      op_seeds = nullptr;
+    }
+  }
+  {
     const int R = std::max(1, opts->R), M = std::max(1, opts->M);
     hipStream_t st = static_cast<hipStream_t>(stream);
-    if (persistent_eligible(ops, n_ops, opts->path, opts->reorth)) {
+    if (one_seed && persistent_eligible(ops, n_ops, opts->path, opts->reorth)) {
       FTLM_HIP_CHECK(hipSetDevice(ops[0]->device));
       ritz_persistent(ops, n_ops, opts, ritz, weight, used, alpha_out, beta_out, st);
       return;
@@ -1200,7 +1370,7 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
     // whose results are scattered to their operators' slots; the rest run operator by operator.
     const int64_t per = static_cast<int64_t>(R) * M;
     std::vector<char> done(static_cast<size_t>(n_ops), 0);
-    if (opts->path == FTLM_PATH_AUTO && opts->reorth == FTLM_REORTH_NONE) {
+    if (one_seed && opts->path == FTLM_PATH_AUTO && opts->reorth == FTLM_REORTH_NONE) {
       std::vector<ftlm_op_t*> small;
       std::vector<int> at;
       for (int io = 0; io < n_ops; ++io)
@@ -1237,6 +1407,9 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
     FTLM_HIP_CHECK(hipMemsetAsync(qn.p, 0, qn.n * sizeof(int), st));
     FTLM_HIP_CHECK(hipMemsetAsync(qnb.p, 0, qnb.n * sizeof(int), st));
     std::vector<char> from_krylov(static_cast<size_t>(S), 0);
+    if (opts->path == FTLM_PATH_AUTO || opts->path == FTLM_PATH_STACK)
+      ritz_stacked(ops, n_ops, opts, op_seeds, qa.p, qb.p, qn.p, qnb.p, S, used, alpha_out, beta_out, &done,
+                   &from_krylov, st);
     for (int io = 0; io < n_ops; ++io) {
       if (done[static_cast<size_t>(io)]) continue;
       const ftlm_op* op0 = ops[io];
@@ -1278,7 +1451,8 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
         K.init(op, Rg, Mop, opts->dtype, opts->reorth);
         K.force_real = opts->real_variant != 0;
         std::vector<unsigned> seeds(static_cast<size_t>(Rg));
-        for (int r = 0; r < Rg; ++r) seeds[r] = opts->seed + static_cast<unsigned>(g0 + r) * 100003u;
+        const unsigned seed0 = op_seeds ? op_seeds[io] : opts->seed;
+        for (int r = 0; r < Rg; ++r) seeds[r] = seed0 + static_cast<unsigned>(g0 + r) * 100003u;
         fill_start(K, opts->start, seeds, nullptr, st);
         if (opts->path == FTLM_PATH_GRAPH && op->kind != kCallback && op->kind != kGram)
           K.run_graph(0, Mop, st);
@@ -1330,6 +1504,20 @@ int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t*
           std::copy(hw.begin() + sm * M, hw.begin() + (sm + 1) * M, weight + sm * M);
         }
     }
+  }
+}
+
+int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, double* ritz, double* weight,
+                       int* used, double* alpha_out, double* beta_out, void* stream) {
+  return guarded([&] { ritz_impl(ops, n_ops, opts, nullptr, ritz, weight, used, alpha_out, beta_out, stream); });
+}
+
+int ftlm_hip_ftlm_ritz_seeded(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, const unsigned* op_seeds,
+                              double* ritz, double* weight, int* used, double* alpha_out, double* beta_out,
+                              void* stream) {
+  return guarded([&] {
+    FTLM_REQUIRE(op_seeds != nullptr, FTLM_EINVAL, "null seed list");
+    ritz_impl(ops, n_ops, opts, op_seeds, ritz, weight, used, alpha_out, beta_out, stream);
   });
 }
 

@@ -371,6 +371,25 @@ __global__ void k_canonical_states(const uint32_t* states, int64_t count, LatDev
 // source orbit is not a row of this block, and the padding of the last group, get coefficient 0
 // and read the row's own element.  Coefficient of a hop term, in the reference's order:
 //   ((c_bond * (sign * sqrt(|Stab_row| / |Stab_col|))) * chi*_K(R)).
+// Rows of a grid-strided momentum-block or momentum-stack apply, dealt per XCD.  Workgroups are
+// placed on the 8 XCDs round-robin (blockIdx % 8), so with a grid that is a multiple of 8, XCD x
+// walks the contiguous x-th eighth of the rows: the rows in flight on one XCD are then
+// (grid / 8) * rpb consecutive orbits, and the sources their down hops share (orbits with the
+// same up-string representative are consecutive) stay in that XCD's L2.  Other grids walk all
+// rows grid-strided.
+struct RowWalk {
+  int64_t first, end, stride;
+};
+
+__device__ __forceinline__ RowWalk row_walk(int64_t n, int rpb, int rl) {
+  if ((gridDim.x & 7u) == 0) {
+    const int64_t x = blockIdx.x & 7u, per = gridDim.x >> 3;
+    const int64_t lo = n * x / 8, hi = n * (x + 1) / 8;
+    return {lo + static_cast<int64_t>(blockIdx.x >> 3) * rpb + rl, hi, per * rpb};
+  }
+  return {static_cast<int64_t>(blockIdx.x) * rpb + rl, n, static_cast<int64_t>(gridDim.x) * rpb};
+}
+
 template <int RB>
 struct KbGroup {
   static constexpr int value = RB <= 2 ? 8 : (RB <= 4 ? 4 : 2);
@@ -395,8 +414,8 @@ __global__ __launch_bounds__(256) void k_kblock_apply(
   double dre[RB], dimg[RB];
 #pragma unroll
   for (int v = 0; v < RB; ++v) dre[v] = dimg[v] = 0.0;
-  for (int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; r < dk;
-       r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+  const RowWalk rw = row_walk(dk, blockDim.x, threadIdx.x);
+  for (int64_t r = rw.first; r < rw.end; r += rw.stride) {
     const int32_t g = rows[r];
     const int32_t e0 = row_off[g], e1 = row_off[g + 1];
     const double dg = diag[g];
@@ -522,7 +541,8 @@ __global__ __launch_bounds__(256) void k_kblock_apply_split(
     dre[v] = dimg[v] = 0.0;
     sxv[v] = xs ? xs[rr + v] : 1.0;
   }
-  for (int64_t r = static_cast<int64_t>(blockIdx.x) * RPB + rl; r < dk; r += static_cast<int64_t>(gridDim.x) * RPB) {
+  const RowWalk rw = row_walk(dk, RPB, rl);
+  for (int64_t r = rw.first; r < rw.end; r += rw.stride) {
     const int32_t g = rows[r];
     const int32_t e0 = row_off[g], e1 = row_off[g + 1];
     const double dg = diag[g];
@@ -605,6 +625,197 @@ __global__ __launch_bounds__(256) void k_kblock_apply_split(
       }
       dotp[static_cast<int64_t>(blockIdx.x) * RT + r0 + threadIdx.x] = make_double2(a, b);
     }
+  }
+}
+
+
+// ------------------------------------------------------------------ momentum stacks
+// One apply for every block of a stack.  The R = nk * rk slots of an orbit are spread over L lanes
+// of VPL slots (16 bytes, or 8 for an odd count of complex64 slots); a workgroup covers 256 / L
+// orbits, grid-strided.  Each term of the K-independent CSR is read once for all blocks: its column
+// orbit's membership bits (memc[e] = mem[b], stored per term so that they stream with the entries)
+// say in which blocks it is a row, and slot (j, r) takes the
+// per-block coefficient ((c_bond * (sign * sqrt(|Stab_row| / |Stab_col|))) * chi*_Kj(R)) computed as
+// k_kblock_apply computes it, or 0 off block j.  Terms are added in CSR order, as there, and an
+// orbit's slots of the blocks it is not a row of are written as 0.
+// ONEJ: every lane's VPL slots belong to one block (rk a multiple of VPL), so a term's coefficient
+// is computed once per lane.
+template <typename T, int VPL, bool ONEJ>
+__global__ __launch_bounds__(256) void k_kstack_apply(
+    int64_t nrep, int nk, int rk, const uint16_t* __restrict__ mem, const uint16_t* __restrict__ memc,
+    const int32_t* __restrict__ row_off, const uint2* __restrict__ ent, const double* __restrict__ diag,
+    const double2* __restrict__ coef, int nbond,
+    const double2* __restrict__ chi, const typename Cx<T>::v2* __restrict__ x, typename Cx<T>::v2* __restrict__ y,
+    const double* __restrict__ xs, double2* __restrict__ dotp) {
+  using V2 = typename Cx<T>::v2;
+  constexpr int G = 8;
+  const int W = nk * rk;
+  const int L = W / VPL;
+  const int RPB = 256 / L;
+  __shared__ double2 s_coef[kMaxBondsLds];
+  __shared__ double2 s_chi[kMaxStackBlocks][kMaxOrbitSites];
+  __shared__ double s_nrm[kMaxOrbitSites * kMaxOrbitSites];
+  for (int b = threadIdx.x; b < kMaxBondsLds; b += blockDim.x) s_coef[b] = b < nbond ? coef[b] : make_double2(0.0, 0.0);
+  for (int i = threadIdx.x; i < kMaxStackBlocks * kMaxOrbitSites; i += blockDim.x)
+    s_chi[i / kMaxOrbitSites][i % kMaxOrbitSites] = i < nk * kMaxOrbitSites ? chi[i] : make_double2(0.0, 0.0);
+  for (int i = threadIdx.x; i < kMaxOrbitSites * kMaxOrbitSites; i += blockDim.x)
+    s_nrm[i] = sqrt(static_cast<double>((i & 15) + 1) / static_cast<double>((i >> 4) + 1));
+  __syncthreads();
+  const int sub = static_cast<int>(threadIdx.x) % L, rl = static_cast<int>(threadIdx.x) / L;
+  const int v0 = sub * VPL;
+  int jb[VPL];
+  double dre[VPL], dimg[VPL], sxv[VPL];
+#pragma unroll
+  for (int v = 0; v < VPL; ++v) {
+    jb[v] = (v0 + v) / rk;
+    dre[v] = dimg[v] = 0.0;
+    sxv[v] = xs ? xs[v0 + v] : 1.0;
+  }
+  if (rl < RPB) {
+    const RowWalk rw = row_walk(nrep, RPB, rl);
+    for (int64_t r = rw.first; r < rw.end; r += rw.stride) {
+      const uint32_t mrow = mem[r];
+      const int32_t e0 = row_off[r], e1 = row_off[r + 1];
+      const double dg = diag[r];
+      T ar[VPL], ai[VPL];
+#pragma unroll
+      for (int v = 0; v < VPL; ++v) ar[v] = ai[v] = static_cast<T>(0);
+      for (int32_t e = e0; e < e1; e += G) {
+        uint2 en[G];
+        uint32_t mb[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) en[j] = ent[min(e + j, e1 - 1)];
+#pragma unroll
+        for (int j = 0; j < G; ++j) mb[j] = e + j < e1 ? static_cast<uint32_t>(memc[min(e + j, e1 - 1)]) : 0u;
+        V2 src[G][VPL];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const V2* sp = x + static_cast<int64_t>(en[j].x & kColMask) * W + v0;
+#pragma unroll
+          for (int v = 0; v < VPL; ++v) src[j][v] = sp[v];
+        }
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const bool dgl = (en[j].x & kDiagFlag) != 0;
+          const double nrm = s_nrm[(en[j].y >> kStabShift) & 0xFFu];
+          const double w = ((en[j].x >> 28) & 1u) ? -nrm : nrm;
+          const double2 cb = s_coef[en[j].y & kBondMask];
+          const double hr = cb.x * w, hi = cb.y * w;
+          const int t = (en[j].x >> 24) & 15u;
+          T trv[VPL], tiv[VPL];
+#pragma unroll
+          for (int v = 0; v < (ONEJ ? 1 : VPL); ++v) {
+            double cr, ci;
+            if (dgl) {
+              cr = dg;
+              ci = 0.0;
+            } else {
+              const double2 ch = s_chi[jb[v]][t];
+              cr = hr * ch.x - hi * ch.y;
+              ci = hr * ch.y + hi * ch.x;
+            }
+            if (!((mb[j] >> jb[v]) & 1u)) cr = ci = 0.0;
+            trv[v] = static_cast<T>(cr);
+            tiv[v] = static_cast<T>(ci);
+          }
+#pragma unroll
+          for (int v = 0; v < VPL; ++v) {
+            const T tr = trv[ONEJ ? 0 : v], ti = tiv[ONEJ ? 0 : v];
+            ar[v] = fma(tr, src[j][v].x, ar[v]);
+            ai[v] = fma(tr, src[j][v].y, ai[v]);
+            ar[v] = fma(-ti, src[j][v].y, ar[v]);
+            ai[v] = fma(ti, src[j][v].x, ai[v]);
+          }
+        }
+      }
+      V2* dst = y + r * W + v0;
+      const V2* own = x + r * W + v0;
+#pragma unroll
+      for (int v = 0; v < VPL; ++v) {
+        V2 o;
+        if ((mrow >> jb[v]) & 1u) {
+          o.x = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
+          o.y = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
+        } else {
+          o.x = o.y = static_cast<T>(0);
+        }
+        dst[v] = o;
+        if (dotp) {
+          const V2 q = own[v];
+          const double qr = sxv[v] * static_cast<double>(q.x), qi = sxv[v] * static_cast<double>(q.y);
+          dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+          dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+        }
+      }
+    }
+  }
+  if (dotp) {  // slot v0 + v of every row slot; sum over the L-strided lanes
+    __shared__ double red[256][2][2];
+#pragma unroll
+    for (int v = 0; v < VPL; ++v) {
+      red[threadIdx.x][v][0] = rl < RPB ? dre[v] : 0.0;
+      red[threadIdx.x][v][1] = rl < RPB ? dimg[v] : 0.0;
+    }
+    __syncthreads();
+    if (static_cast<int>(threadIdx.x) < W) {
+      const int s2 = threadIdx.x / VPL, v2 = threadIdx.x % VPL;
+      double a = 0.0, b = 0.0;
+      for (int j = s2; j < RPB * L; j += L) {
+        a += red[j][v2][0];
+        b += red[j][v2][1];
+      }
+      dotp[static_cast<int64_t>(blockIdx.x) * W + threadIdx.x] = make_double2(a, b);
+    }
+  }
+}
+
+// Membership bits of every orbit: bit j set where the orbit is a row of block j.
+__global__ void k_kstack_mem(int64_t nrep, int nk, KStackMaps maps, uint16_t* mem) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nrep) return;
+  uint32_t m = 0;
+  for (int j = 0; j < nk; ++j) m |= (maps.map[j][g] >= 0 ? 1u : 0u) << j;
+  mem[g] = static_cast<uint16_t>(m);
+}
+
+// Membership bits of every term's column orbit, in term order.
+__global__ void k_kstack_memc(const uint2* ent, int64_t nnz, const uint16_t* mem, uint16_t* memc) {
+  const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e < nnz) memc[e] = mem[ent[e].x & kColMask];
+}
+
+// Block j's vectors [dim_j][rk] into / out of slots (j, 0 .. rk) of a stacked block [nrep][W].
+// Scatter writes every orbit's slots (0 off the block).
+template <typename T>
+__global__ void k_kstack_scatter(int64_t nrep, int W, int j, int rk, const int32_t* __restrict__ map,
+                                 const typename Cx<T>::v2* __restrict__ xb, typename Cx<T>::v2* __restrict__ xs) {
+  using V2 = typename Cx<T>::v2;
+  const int64_t n = nrep * rk;
+  for (int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; f < n;
+       f += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t g = f / rk;
+    const int r = static_cast<int>(f - g * rk);
+    const int32_t i = map[g];
+    V2 v;
+    if (i >= 0) {
+      v = xb[static_cast<int64_t>(i) * rk + r];
+    } else {
+      v.x = v.y = static_cast<T>(0);
+    }
+    xs[g * W + static_cast<int64_t>(j) * rk + r] = v;
+  }
+}
+
+template <typename T>
+__global__ void k_kstack_gather(int64_t nrep, int W, int j, int rk, const int32_t* __restrict__ map,
+                                const typename Cx<T>::v2* __restrict__ xs, typename Cx<T>::v2* __restrict__ xb) {
+  const int64_t n = nrep * rk;
+  for (int64_t f = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; f < n;
+       f += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t g = f / rk;
+    const int r = static_cast<int>(f - g * rk);
+    const int32_t i = map[g];
+    if (i >= 0) xb[static_cast<int64_t>(i) * rk + r] = xs[g * W + static_cast<int64_t>(j) * rk + r];
   }
 }
 
@@ -1149,6 +1360,108 @@ int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype,
   return grid;
 }
 
+// ------------------------------------------------------------------ momentum stacks (host)
+struct KStack : ftlm_op {
+  std::shared_ptr<OrbitGraph> G;
+  int nk = 0;
+  std::vector<const KBlock*> blocks;  // borrowed: the caller keeps them alive
+  KStackMaps maps{};
+  DevBuf<uint16_t> mem;   // [nrep]
+  DevBuf<uint16_t> memc;  // [nnz] mem of every term's column orbit
+  DevBuf<double2> chi;    // [nk][16] Bloch phases of each block
+};
+
+const void* kblock_graph(const ftlm_op* op) {
+  return op != nullptr && op->kind == kKBlock ? static_cast<const KBlock*>(op)->G.get() : nullptr;
+}
+
+std::unique_ptr<ftlm_op> kstack_build(const std::vector<const ftlm_op*>& blocks) {
+  FTLM_REQUIRE(!blocks.empty() && static_cast<int>(blocks.size()) <= kMaxStackBlocks, FTLM_EINVAL,
+               "a momentum stack holds 1 to 16 blocks");
+  auto S = std::make_unique<KStack>();
+  S->kind = kKStack;
+  for (const ftlm_op* op : blocks) {
+    FTLM_REQUIRE(op != nullptr && op->kind == kKBlock, FTLM_EINVAL, "a momentum stack takes momentum blocks only");
+    const KBlock* B = static_cast<const KBlock*>(op);
+    FTLM_REQUIRE(B->G == static_cast<const KBlock*>(blocks[0])->G, FTLM_EINVAL,
+                 "the blocks of a momentum stack must share one orbit graph (create them with share=)");
+    FTLM_REQUIRE(B->device == blocks[0]->device, FTLM_EINVAL, "the blocks of a momentum stack live on one device");
+    FTLM_REQUIRE(B->dim >= 1, FTLM_EINVAL, "every block of a momentum stack must have a row");
+    S->blocks.push_back(B);
+  }
+  const OrbitGraph& G = *S->blocks[0]->G;
+  S->G = S->blocks[0]->G;
+  S->device = blocks[0]->device;
+  S->nk = static_cast<int>(blocks.size());
+  S->dim = G.nrep;
+  std::vector<double2> chi(static_cast<size_t>(S->nk) * kMaxOrbitSites);
+  for (int j = 0; j < S->nk; ++j) {
+    S->maps.map[j] = S->blocks[j]->map.p;
+    for (int t = 0; t < kMaxOrbitSites; ++t) chi[static_cast<size_t>(j) * kMaxOrbitSites + t] = S->blocks[j]->ph.chi[t];
+  }
+  S->chi.alloc(chi.size());
+  S->chi.upload(chi);
+  S->mem.alloc(static_cast<size_t>(std::max<int64_t>(G.nrep, 1)));
+  if (G.nrep > 0) {
+    hipLaunchKernelGGL(k_kstack_mem, dim3(grid_for(G.nrep, 256)), dim3(256), 0, nullptr, G.nrep, S->nk, S->maps,
+                       S->mem.p);
+    check_launch("k_kstack_mem");
+  }
+  S->memc.alloc(static_cast<size_t>(std::max<int64_t>(G.nnz, 1)));
+  if (G.nnz > 0) {
+    hipLaunchKernelGGL(k_kstack_memc, dim3(grid_for(G.nnz, 256)), dim3(256), 0, nullptr, G.ent.p, G.nnz, S->mem.p,
+                       S->memc.p);
+    check_launch("k_kstack_memc");
+  }
+  FTLM_HIP_CHECK(hipDeviceSynchronize());
+  return S;
+}
+
+int kstack_blocks(const ftlm_op* stack) { return static_cast<const KStack*>(stack)->nk; }
+KStackMaps kstack_maps(const ftlm_op* stack) { return static_cast<const KStack*>(stack)->maps; }
+
+// Default grid of a stack apply: 2048 workgroups, 256 per XCD.
+constexpr int kKsDefaultGrid = 2048;
+
+int hv_apply_kstack(const ftlm_op* op, const void* x, void* y, int R, int dtype, const double* xs, double2* dotp,
+                    hipStream_t st) {
+  const KStack* S = static_cast<const KStack*>(op);
+  FTLM_REQUIRE(R % S->nk == 0, FTLM_EINVAL, "a momentum-stack apply takes R = blocks * samples per block");
+  if (S->dim == 0) return 0;
+  const OrbitGraph& G = *S->G;
+  const int rk = R / S->nk;
+  const HvShape sh = S->shape(R, dtype);
+  const int vpl = (dtype == FTLM_C64 && R % 2 == 0) ? 2 : 1;
+  const int rpb = 256 / (R / vpl);
+  const int gmax = sh.grid > 0 ? std::min(sh.grid, kKbMaxGrid) : kKsDefaultGrid;
+  const int grid = static_cast<int>(std::min<int64_t>(gmax, (S->dim + rpb - 1) / rpb));
+  const float2* xf = static_cast<const float2*>(x);
+  float2* yf = static_cast<float2*>(y);
+#define FTLM_KSTACK_ARGS S->dim, S->nk, rk, S->mem.p, S->memc.p, G.row_off.p, G.ent.p, G.diag.p, G.coef.p, G.nbond, S->chi.p
+  if (dtype == FTLM_C64) {
+    if (vpl == 2 && rk % 2 == 0)
+      hipLaunchKernelGGL((k_kstack_apply<float, 2, true>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS, xf, yf, xs,
+                         dotp);
+    else if (vpl == 2)
+      hipLaunchKernelGGL((k_kstack_apply<float, 2, false>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS, xf, yf, xs,
+                         dotp);
+    else
+      hipLaunchKernelGGL((k_kstack_apply<float, 1, true>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS, xf, yf, xs,
+                         dotp);
+  } else {
+    hipLaunchKernelGGL((k_kstack_apply<double, 1, true>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS,
+                       static_cast<const double2*>(x), static_cast<double2*>(y), xs, dotp);
+  }
+#undef FTLM_KSTACK_ARGS
+  check_launch("k_kstack_apply");
+  return grid;
+}
+
+int kstack_default_grid(const ftlm_op* op, int* grid_max) {
+  if (grid_max) *grid_max = kKbMaxGrid;
+  return static_cast<int>(std::min<int64_t>(kKsDefaultGrid, std::max<int64_t>(op->dim, 1)));
+}
+
 // ------------------------------------------------------------------ momentum Gram blocks
 // HubbardMomentumBlock (src/hubbard_momentum_action.cpp:37-89): the momentum-K block in the
 // orthonormal basis of MomentumPhiGramBasis (src/orbit_bloch_phi.cpp:226-355), applied as
@@ -1371,6 +1684,41 @@ const KBlock* as_kblock(const ftlm_op_t* op) {
 
 }  // namespace
 
+namespace {
+
+void kstack_copy(const ftlm_op_t* stack, int j, const void* src, int rk, int dtype, void* dst, void* stream,
+                 bool scatter) {
+  FTLM_REQUIRE(stack != nullptr && stack->kind == kKStack, FTLM_EINVAL, "not a momentum-stack operator");
+  FTLM_REQUIRE(src != nullptr && dst != nullptr, FTLM_EINVAL, "null argument");
+  FTLM_REQUIRE(dtype == FTLM_C128 || dtype == FTLM_C64, FTLM_EINVAL, "dtype must be FTLM_C128 or FTLM_C64");
+  const KStack* S = static_cast<const KStack*>(stack);
+  FTLM_REQUIRE(j >= 0 && j < S->nk, FTLM_ERANGE, "block index outside the stack");
+  FTLM_REQUIRE(rk >= 1 && S->nk * rk <= 32, FTLM_EINVAL, "blocks * samples per block must be in [1, 32]");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  const int64_t n = S->dim * rk;
+  if (n == 0) return;
+  const unsigned g = static_cast<unsigned>(std::min<int64_t>(8192, (n + 255) / 256));
+  const int W = S->nk * rk;
+  const int32_t* map = S->maps.map[j];
+  if (dtype == FTLM_C64) {
+    if (scatter)
+      hipLaunchKernelGGL(k_kstack_scatter<float>, dim3(g), dim3(256), 0, st, S->dim, W, j, rk, map,
+                         static_cast<const float2*>(src), static_cast<float2*>(dst));
+    else
+      hipLaunchKernelGGL(k_kstack_gather<float>, dim3(g), dim3(256), 0, st, S->dim, W, j, rk, map,
+                         static_cast<const float2*>(src), static_cast<float2*>(dst));
+  } else {
+    if (scatter)
+      hipLaunchKernelGGL(k_kstack_scatter<double>, dim3(g), dim3(256), 0, st, S->dim, W, j, rk, map,
+                         static_cast<const double2*>(src), static_cast<double2*>(dst));
+    else
+      hipLaunchKernelGGL(k_kstack_gather<double>, dim3(g), dim3(256), 0, st, S->dim, W, j, rk, map,
+                         static_cast<const double2*>(src), static_cast<double2*>(dst));
+  }
+  check_launch(scatter ? "k_kstack_scatter" : "k_kstack_gather");
+}
+}  // namespace
+
 extern "C" {
 
 int ftlm_hip_kblock_create(int Lx, int Ly, int n_up, int n_dn, int kx, int ky, double tol, const ftlm_hop_t* hops,
@@ -1406,6 +1754,30 @@ int ftlm_hip_kblock_create(int Lx, int Ly, int n_up, int n_dn, int kx, int ky, d
     build_kblock(B.get(), tol);
     *out = B.release();
   });
+}
+
+int ftlm_hip_kstack_create(ftlm_op_t* const* blocks, int n_blocks, ftlm_op_t** out) {
+  return guarded([&] {
+    FTLM_REQUIRE(out != nullptr && blocks != nullptr, FTLM_EINVAL, "null argument");
+    *out = nullptr;
+    FTLM_REQUIRE(n_blocks >= 1 && n_blocks <= kMaxStackBlocks, FTLM_EINVAL, "a momentum stack holds 1 to 16 blocks");
+    std::vector<const ftlm_op*> v(blocks, blocks + n_blocks);
+    FTLM_REQUIRE(v[0] != nullptr, FTLM_EINVAL, "null block");
+    FTLM_HIP_CHECK(hipSetDevice(v[0]->device));
+    std::unique_ptr<ftlm_op> S = kstack_build(v);
+    *out = S.release();
+  });
+}
+
+
+int ftlm_hip_kstack_scatter(const ftlm_op_t* stack, int j, const void* xb, int rk, int dtype, void* xs,
+                            void* stream) {
+  return guarded([&] { kstack_copy(stack, j, xb, rk, dtype, xs, stream, true); });
+}
+
+int ftlm_hip_kstack_gather(const ftlm_op_t* stack, int j, const void* xs, int rk, int dtype, void* xb,
+                           void* stream) {
+  return guarded([&] { kstack_copy(stack, j, xs, rk, dtype, xb, stream, false); });
 }
 
 int ftlm_hip_kgram_create(const ftlm_op_t* sector, const ftlm_op_t* kblock, ftlm_op_t** out) {

@@ -202,12 +202,17 @@ typedef struct ftlm_ftlm_opts {
 #define FTLM_REORTH_PARTIAL 2
 
 #define FTLM_PATH_AUTO 0       /* the full sectors of dim <= 16384 in one persistent launch (when
-                                  there are at least two), every other operator per operator */
+                                  there are at least two); momentum blocks of one orbit graph (at
+                                  least two, random-phase starts) as in FTLM_PATH_STACK; every
+                                  other operator per operator */
 #define FTLM_PATH_PER_OP 1     /* per operator, R recurrences per block H*V */
 #define FTLM_PATH_PERSISTENT 2 /* every (sector, sample) recurrence in one persistent launch (sectors
                                   of dim <= 262144) */
 #define FTLM_PATH_GRAPH 3      /* per operator, the whole M-step recurrence captured as one hipGraph
                                   and launched once (host-callback operators run eagerly) */
+#define FTLM_PATH_STACK 4      /* the momentum blocks that share one orbit graph as momentum stacks
+                                  (ftlm_hip_kstack_create): one recurrence group and one apply per
+                                  step for all of them; every other operator per operator */
 
 /* Runs R independent Lanczos recurrences per operator, all on the device, and diagonalises every
  * tridiagonal with the chosen quadrature eigensolver.  Outputs (host):
@@ -218,6 +223,14 @@ typedef struct ftlm_ftlm_opts {
  * run as one block recurrence (block H*V). */
 int ftlm_hip_ftlm_ritz(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts, double* ritz,
                        double* weight, int* used, double* alpha, double* beta, void* stream);
+
+/* ftlm_hip_ftlm_ritz with a seed per operator: operator io's sample r uses op_seeds[io] + r*100003
+ * (opts->seed is not used).  This is the reference's per-K seed schedule of its momentum loop
+ * (seed + sector*257 + ky*17 + kx, benchmarks/bench_ftlm_nmu_4x4_k.cpp:1077-1078) in one call, so
+ * the blocks of a sector can share momentum stacks; with equal seeds it is ftlm_hip_ftlm_ritz. */
+int ftlm_hip_ftlm_ritz_seeded(ftlm_op_t* const* ops, int n_ops, const ftlm_ftlm_opts_t* opts,
+                              const unsigned* op_seeds, double* ritz, double* weight, int* used, double* alpha,
+                              double* beta, void* stream);
 
 /* Per-operator FTLM thermodynamics from Ritz data, on the device:
  *   lnZ_op[n_ops][nb] = ln dim + log-mean-exp_r ln sum_k w e^{-beta lambda}
@@ -395,6 +408,24 @@ int ftlm_hip_kblock_index_of_rep(const ftlm_op_t* op, const uint32_t* reps, int6
  * eta = 0) where the reference returns false. */
 int ftlm_hip_kblock_map(const ftlm_op_t* op, const uint32_t* states, int64_t count, int64_t* index, int* dx, int* dy,
                         double* eta);
+
+/* Momentum stack: the momentum blocks of one sector and Hamiltonian (created with share= so that
+ * they use one orbit graph) applied side by side, replacing the reference's loop that builds and
+ * runs apply_orbit_k_block once per K (benchmarks/bench_ftlm_nmu_4x4_k.cpp:801-1103 over
+ * src/orbit_k_block_basis.cpp:151-220).  The stack's dim is the number of orbits of the graph; an
+ * apply takes R = n_blocks * rk vectors laid out [orbit][block j][rk]: slot (j, r) of orbit g holds
+ * block j's entry at the row whose ftlm_hip_kblock_orbit_index is g, and the slots of blocks that g is not a row
+ * of hold 0 and are written as 0.  Each block's slots of y equal that block's own apply.  The
+ * blocks must outlive the stack (it borrows their row maps); at most 16 blocks.  ftlm_hip_apply,
+ * ftlm_hip_krylov_* (random-phase starts only: slot (j, r) starts as block j's sample r) and
+ * ftlm_hip_ftlm_ritz take it as an operator. */
+int ftlm_hip_kstack_create(ftlm_op_t* const* blocks, int n_blocks, ftlm_op_t** out);
+/* Block j's vectors xb[dim_j][rk] into slots (j, 0..rk) of a stacked block xs[n_orbits][n_blocks*rk]
+ * (scatter writes 0 into the orbits that are not rows of block j), and back (gather). */
+int ftlm_hip_kstack_scatter(const ftlm_op_t* stack, int j, const void* xb, int rk, int dtype, void* xs,
+                            void* stream);
+int ftlm_hip_kstack_gather(const ftlm_op_t* stack, int j, const void* xs, int rk, int dtype, void* xb,
+                           void* stream);
 
 /* HubbardMomentumBlock (src/hubbard_momentum_action.cpp:37-89) with the Gram basis of
  * MomentumPhiGramBasis (src/orbit_bloch_phi.cpp:226-355): the momentum-K block of `sector` as

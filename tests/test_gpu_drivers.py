@@ -83,6 +83,22 @@ def test_nmu_rect_k_orbit_matches_reference_blocks(tmp_path):
         assert close(sectors[k], v), (k, sectors[k], v)
 
 
+def test_nmu_rect_k_orbit_stacked_equals_per_block(tmp_path):
+    """--solver=k-orbit --orbit-all=1 with random-phase starts on 4x2: the orbit blocks of every
+    sector through momentum stacks (--path=stack, one ftlm_hip_ftlm_ritz_seeded call per sector
+    with the per-K seeds) against the same blocks one by one (--path=per-op).  The printed 6 digits
+    of every sector's logZ and of n(mu) agree (the stacked dot sums differ in the last bits only)."""
+    args = ["--Lx=4", "--Ly=2", "--beta=4", "--n-mu=21", "--ftlm-random=6", "--lanczos-steps=24",
+            "--solver=k-orbit", "--orbit-all=1", "--start=phase", "--quad=converged", "--ed-cutoff=24"]
+    say("ftlm_nmu_rect_k stacked vs per-op", *args)
+    mu0, n0, sec0, _ = run("ftlm_nmu_rect_k", [*args, "--path=per-op"], tmp_path)
+    mu1, n1, sec1, _ = run("ftlm_nmu_rect_k", [*args, "--path=stack"], tmp_path)
+    assert sec0.keys() == sec1.keys() and len(sec0) == 81
+    assert np.array_equal(mu0, mu1) and close(n0, n1, rel=2e-6)
+    for k in sec0:
+        assert close(sec1[k], sec0[k], rel=2e-6), (k, sec0[k], sec1[k])
+
+
 def test_nmu_rect_k_ed_matches_gram_spectra(oracle, tmp_path):
     """--solver=ed (every Gram block by device dense ED, as the reference's ed solver) on 3x2: each
     sector's logZ is ln sum_K tr e^{-beta H_K} over the oracle's Gram blocks (numpy eigvalsh), which
