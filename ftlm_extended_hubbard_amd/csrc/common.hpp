@@ -207,6 +207,29 @@ struct HvShape {
   int rb = 0, bpx = 0, wlog = -1, grid = 0;
 };
 
+// Staging of the host-pointer apply (ftlm_hip_apply_host), per operator: device x / y blocks,
+// two pinned chunks, a stream and two events, made on the first host apply and kept with the
+// operator, so a call allocates nothing.  Calls on one operator are serialised by `mu`.
+struct HostStage {
+  std::mutex mu;
+  DevBuf<double2> dx, dy;
+  double2* pin[2] = {nullptr, nullptr};
+  size_t pin_n = 0;  // complex elements per pinned chunk
+  hipStream_t st = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  HostStage() = default;
+  HostStage(const HostStage&) = delete;
+  HostStage& operator=(const HostStage&) = delete;
+  ~HostStage() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (int i = 0; i < 2; ++i) {
+      if (pin[i]) (void)hipHostFree(pin[i]);
+      if (ev[i]) (void)hipEventDestroy(ev[i]);
+    }
+    if (st) (void)hipStreamDestroy(st);
+  }
+};
+
 struct OpBase {
   int kind = kSector;
   int device = 0;
@@ -216,6 +239,8 @@ struct OpBase {
     const auto it = shapes.find(R * 4 + dtype);
     return it == shapes.end() ? HvShape{} : it->second;
   }
+  mutable std::mutex stage_mu;                    // guards the creation of host_stage
+  mutable std::unique_ptr<HostStage> host_stage;  // ftlm_hip_apply_host staging
   virtual ~OpBase() = default;
 };
 

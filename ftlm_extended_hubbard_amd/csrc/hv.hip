@@ -38,6 +38,9 @@
 #include <array>
 #include <cstdlib>
 
+#include <cstring>
+#include <thread>
+
 #include "common.hpp"
 
 namespace ftlm_hip {
@@ -1156,6 +1159,84 @@ int hv_apply(const ftlm_op* op, const void* x, void* y, int R, int dtype, const 
 
 using namespace ftlm_hip;
 
+namespace {
+
+// memcpy of large host blocks on several threads (pageable <-> pinned staging runs at the speed
+// of one core otherwise, below the PCIe rate).
+void par_copy(void* dst, const void* src, size_t bytes) {
+  constexpr size_t kPart = size_t{4} << 20;
+  const int nt = static_cast<int>(std::min<size_t>(8, (bytes + kPart - 1) / kPart));
+  if (nt <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  std::vector<std::thread> th;
+  auto part = [&](int t) {
+    const size_t a = bytes * t / nt, b = bytes * (t + 1) / nt;
+    std::memcpy(static_cast<char*>(dst) + a, static_cast<const char*>(src) + a, b - a);
+  };
+  for (int t = 1; t < nt; ++t) th.emplace_back(part, t);
+  part(0);
+  for (auto& x : th) x.join();
+}
+
+constexpr size_t kStageChunk = size_t{4} << 20;  // complex128 elements per pinned chunk (64 MB)
+
+// y = H x for host vectors: x is copied into the operator's pinned chunks (on host threads) and
+// on to the device while the next chunk is filled, the apply runs on the operator's own stream,
+// and y comes back the same way.  The device blocks, pinned chunks, stream and events live with
+// the operator (made on its first host apply).
+void apply_host_staged(const ftlm_op* op, const double2* x, double2* y) {
+  const size_t n = static_cast<size_t>(op->dim);
+  if (n == 0) return;
+  HostStage* S = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(op->stage_mu);
+    if (!op->host_stage) {
+      auto st = std::make_unique<HostStage>();
+      st->dx.alloc(n);
+      st->dy.alloc(n);
+      st->pin_n = std::min(n, kStageChunk);
+      for (int i = 0; i < 2; ++i) {
+        FTLM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&st->pin[i]), st->pin_n * sizeof(double2),
+                                     hipHostMallocDefault));
+        FTLM_HIP_CHECK(hipEventCreateWithFlags(&st->ev[i], hipEventDisableTiming));
+      }
+      FTLM_HIP_CHECK(hipStreamCreateWithFlags(&st->st, hipStreamNonBlocking));
+      op->host_stage = std::move(st);
+    }
+    S = op->host_stage.get();
+  }
+  std::lock_guard<std::mutex> lk(S->mu);
+  const size_t cn = S->pin_n, nc = (n + cn - 1) / cn;
+  const size_t es = sizeof(double2);
+  for (size_t c = 0; c < nc; ++c) {  // x: host -> pinned chunk c % 2 -> device
+    const int b = static_cast<int>(c & 1);
+    const size_t off = c * cn, len = std::min(cn, n - off);
+    if (c >= 2) FTLM_HIP_CHECK(hipEventSynchronize(S->ev[b]));  // chunk c - 2's copy has left it
+    par_copy(S->pin[b], x + off, len * es);
+    FTLM_HIP_CHECK(hipMemcpyAsync(S->dx.p + off, S->pin[b], len * es, hipMemcpyHostToDevice, S->st));
+    FTLM_HIP_CHECK(hipEventRecord(S->ev[b], S->st));
+  }
+  hv_apply(op, S->dx.p, S->dy.p, 1, FTLM_C128, nullptr, nullptr, S->st);
+  auto fetch = [&](size_t c) {
+    const int b = static_cast<int>(c & 1);
+    const size_t off = c * cn, len = std::min(cn, n - off);
+    FTLM_HIP_CHECK(hipMemcpyAsync(S->pin[b], S->dy.p + off, len * es, hipMemcpyDeviceToHost, S->st));
+    FTLM_HIP_CHECK(hipEventRecord(S->ev[b], S->st));
+  };
+  fetch(0);
+  for (size_t c = 0; c < nc; ++c) {  // y: device -> pinned chunk c % 2 -> host, one chunk ahead
+    const int b = static_cast<int>(c & 1);
+    const size_t off = c * cn, len = std::min(cn, n - off);
+    FTLM_HIP_CHECK(hipEventSynchronize(S->ev[b]));
+    if (c + 1 < nc) fetch(c + 1);  // the other chunk was emptied into y at step c - 1
+    par_copy(y + off, S->pin[b], len * es);
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 int ftlm_hip_apply(const ftlm_op_t* op, const void* x, void* y, int R, int dtype, void* stream) {
@@ -1207,12 +1288,9 @@ int ftlm_hip_apply_pass_times(const ftlm_op_t* op, const void* x, void* y, int R
 int ftlm_hip_apply_host(const ftlm_op_t* op, const double* x, double* y) {
   return guarded([&] {
     FTLM_REQUIRE(op != nullptr && x != nullptr && y != nullptr, FTLM_EINVAL, "null argument");
+    FTLM_REQUIRE(op->kind != kKStack, FTLM_EINVAL, "a momentum stack is applied to device blocks only");
     FTLM_HIP_CHECK(hipSetDevice(op->device));
-    const size_t n = static_cast<size_t>(op->dim);
-    DevBuf<double2> dx(n), dy(n);
-    dx.upload(reinterpret_cast<const double2*>(x), n);
-    hv_apply(op, dx.p, dy.p, 1, FTLM_C128, nullptr, nullptr, nullptr);
-    FTLM_HIP_CHECK(hipMemcpy(y, dy.p, n * sizeof(double2), hipMemcpyDeviceToHost));
+    apply_host_staged(op, reinterpret_cast<const double2*>(x), reinterpret_cast<double2*>(y));
   });
 }
 

@@ -63,5 +63,5 @@ def test_cpp_tests_built():
     bdir = os.path.join(ROOT, "tests", "cpp", "_bin")
     if not os.path.isdir(bdir):
         pytest.skip("C++ tests not built")
-    for t in ("test_pipeline", "test_thermo", "test_orbit"):
+    for t in ("test_pipeline", "test_thermo", "test_orbit", "test_apply_host"):
         assert os.access(os.path.join(bdir, t), os.X_OK), t

@@ -19,7 +19,7 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("prog", ["test_pipeline", "test_thermo", "test_orbit"])
+@pytest.mark.parametrize("prog", ["test_pipeline", "test_thermo", "test_orbit", "test_apply_host"])
 def test_cpp_program(prog):
     path = os.path.join(BIN, prog)
     if not os.path.exists(path):
