@@ -26,6 +26,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include "driver_host.hpp"
 #include "ftlm_hip.h"
 
 #ifndef FTLM_BENCH
@@ -37,20 +38,10 @@ namespace {
 using clock_type = std::chrono::high_resolution_clock;
 using cplx = std::complex<double>;
 
-const char* find_arg(int argc, char** argv, const char* key) {
-  const std::string prefix = std::string("--") + key + "=";
-  for (int i = 1; i < argc; ++i)
-    if (std::strncmp(argv[i], prefix.c_str(), prefix.size()) == 0) return argv[i] + prefix.size();
-  return nullptr;
-}
-double arg_double(int argc, char** argv, const char* key, double dflt) {
-  const char* v = find_arg(argc, argv, key);
-  return v ? std::strtod(v, nullptr) : dflt;
-}
-int arg_int(int argc, char** argv, const char* key, int dflt) {
-  const char* v = find_arg(argc, argv, key);
-  return v ? std::atoi(v) : dflt;
-}
+using ftlm_driver::arg_double;
+using ftlm_driver::arg_int;
+using ftlm_driver::find_arg;
+
 double seconds_since(clock_type::time_point t0) {
   return std::chrono::duration<double>(clock_type::now() - t0).count();
 }

@@ -37,6 +37,7 @@
 #include <string>
 #include <vector>
 
+#include "driver_host.hpp"
 #include "ftlm_hip.h"
 
 #ifndef FTLM_NMU_K
@@ -45,23 +46,13 @@
 
 namespace {
 
+using ftlm_driver::arg_double;
+using ftlm_driver::arg_int;
+using ftlm_driver::find_arg;
+using ftlm_driver::has_arg;
+using ftlm_driver::logsumexp2;
+using ftlm_driver::write_nmu;
 using clock_type = std::chrono::high_resolution_clock;
-
-const char* find_arg(int argc, char** argv, const char* key) {
-  const std::string prefix = std::string("--") + key + "=";
-  for (int i = 1; i < argc; ++i)
-    if (std::strncmp(argv[i], prefix.c_str(), prefix.size()) == 0) return argv[i] + prefix.size();
-  return nullptr;
-}
-double arg_double(int argc, char** argv, const char* key, double dflt) {
-  const char* v = find_arg(argc, argv, key);
-  return v ? std::strtod(v, nullptr) : dflt;
-}
-int arg_int(int argc, char** argv, const char* key, int dflt) {
-  const char* v = find_arg(argc, argv, key);
-  return v ? std::atoi(v) : dflt;
-}
-bool has_arg(int argc, char** argv, const char* key) { return find_arg(argc, argv, key) != nullptr; }
 
 double peak_rss_bytes() {
   struct rusage ru {};
@@ -75,13 +66,6 @@ double peak_rss_bytes() {
 }
 void ok(int rc, const char* what) {
   if (rc != FTLM_OK) fail(what);
-}
-
-double logsumexp2(double a, double b) {
-  if (!std::isfinite(a)) return b;
-  if (!std::isfinite(b)) return a;
-  const double m = std::max(a, b);
-  return m + std::log(std::exp(a - m) + std::exp(b - m));
 }
 
 struct Model {
@@ -137,34 +121,6 @@ ftlm_ftlm_opts_t make_opts(int argc, char** argv, int R, int M, unsigned seed, i
 }
 
 int sector_index(int nu, int nd, int n_sites) { return nu * (n_sites + 1) + nd; }
-
-// n(mu) lines of bench_ftlm_nmu_rect.cpp:186-219.
-void write_nmu(std::ostream& out, const std::vector<double>& logZ, const std::vector<int>& Nelec, int n_sites,
-               double beta, double mu_min, double mu_max, int n_mu) {
-  const int nsec = static_cast<int>(logZ.size());
-  for (int k = 0; k < n_mu; ++k) {
-    const double tmu = static_cast<double>(k) / static_cast<double>(n_mu - 1);
-    const double mu = mu_min + tmu * (mu_max - mu_min);
-    double mx = -std::numeric_limits<double>::infinity();
-    for (int i = 0; i < nsec; ++i) {
-      if (!std::isfinite(logZ[i])) continue;
-      mx = std::max(mx, logZ[i] + beta * mu * static_cast<double>(Nelec[i]));
-    }
-    if (!std::isfinite(mx)) {
-      out << mu << "\t0\n";
-      continue;
-    }
-    double sum_w = 0.0, sum_Nw = 0.0;
-    for (int i = 0; i < nsec; ++i) {
-      if (!std::isfinite(logZ[i])) continue;
-      const double w = std::exp(logZ[i] + beta * mu * static_cast<double>(Nelec[i]) - mx);
-      sum_w += w;
-      sum_Nw += static_cast<double>(Nelec[i]) * w;
-    }
-    const double n_avg = (sum_w > 0.0) ? (sum_Nw / sum_w) / static_cast<double>(n_sites) : 0.0;
-    out << mu << "\t" << n_avg << "\n";
-  }
-}
 
 struct Common {
   int Lx, Ly;

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "common.hpp"
+#include "gauss_mt.hpp"
 
 namespace ftlm_hip {
 
@@ -217,6 +218,19 @@ __global__ void k_start_fallback(typename V2Of<T>::t* q, int R, double* sa) {
     q[r].x = static_cast<T>(1.0);
     q[r].y = static_cast<T>(0.0);
     sa[r] = 1.0;
+  }
+}
+
+// One sample's start, drawn contiguous (complex128 on the host), into slot r of the interleaved
+// start block q[i * R + r] (converted to T).
+template <typename T>
+__global__ void k_sample_to_slot(int64_t n, int64_t first, int R, int r, const double2* __restrict__ src,
+                                 typename V2Of<T>::t* __restrict__ q) {
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const double2 z = src[i];
+    q[(first + i) * R + r].x = static_cast<T>(z.x);
+    q[(first + i) * R + r].y = static_cast<T>(z.y);
   }
 }
 
@@ -820,12 +834,31 @@ struct GaussStreams {
   }
 };
 
+// Large starts of few samples: each sample's one stream is split over the host threads with
+// MT19937 jump-ahead (gauss_mt.hpp), exactly; with many samples, one thread per sample is as fast.
+constexpr int64_t kSplitGaussMinDim = int64_t(1) << 21;
+
+static int split_gauss_threads() {
+  return std::max(1, std::min(16, static_cast<int>(std::thread::hardware_concurrency())));
+}
+
+static bool split_gauss(int64_t dim, int R) { return dim >= kSplitGaussMinDim && 2 * R <= split_gauss_threads(); }
+
 // The reference starts of R samples, unnormalised and interleaved: h[i * R + r] = sample r at i.
 // Each block is drawn per sample into private buffers, then the threads interleave disjoint index
 // ranges.
 void gauss_interleaved(int64_t dim, int R, const std::vector<unsigned>& seeds, std::vector<double2>* h) {
   h->resize(static_cast<size_t>(dim) * R);
   if (dim <= 0) return;
+  if (split_gauss(dim, R)) {
+    for (int r = 0; r < R; ++r)
+      ftlm_gauss::draw_sample(seeds[static_cast<size_t>(r)], dim, split_gauss_threads(),
+                              [&](int64_t first, int64_t n, const std::complex<double>* v) {
+                                for (int64_t i = 0; i < n; ++i)
+                                  (*h)[static_cast<size_t>(first + i) * R + r] = make_double2(v[i].real(), v[i].imag());
+                              });
+    return;
+  }
   const int nthr = gauss_threads(R);
   GaussStreams gs(seeds, R);
   const int64_t blk = std::min(kGaussBlock, dim);
@@ -845,6 +878,74 @@ void gauss_interleaved(int64_t dim, int R, const std::vector<unsigned>& seeds, s
         }
     });
   }
+}
+
+// The reference Gaussian starts of R samples, sample after sample, each drawn by all host threads
+// (ftlm_gauss::draw_sample hands its entries over in index order): the calling thread sums the
+// norm in index order (one serial sum, as the reference's norm2), packs the entries into one of
+// two pinned chunks whose copy to a contiguous device block overlaps the next chunk, and a kernel
+// moves the sample into its slot of the interleaved start block.
+void finish_start(Krylov& K, const std::vector<double>& ss, hipStream_t st);
+
+void fill_start_split_gauss(Krylov& K, const std::vector<unsigned>& seeds, hipStream_t st) {
+  const int R = K.R;
+  const int64_t dim = K.dim;
+  constexpr int64_t kChunk = int64_t(1) << 21;  // entries per pinned chunk (32 MB)
+  struct Pinned {
+    double2* h[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~Pinned() {
+      for (int i = 0; i < 2; ++i) {
+        if (ev[i]) {
+          (void)hipEventSynchronize(ev[i]);
+          (void)hipEventDestroy(ev[i]);
+        }
+        if (h[i]) (void)hipHostFree(h[i]);
+      }
+    }
+  } pin;
+  for (int i = 0; i < 2; ++i) {
+    FTLM_HIP_CHECK(hipHostMalloc(reinterpret_cast<void**>(&pin.h[i]), kChunk * sizeof(double2), hipHostMallocDefault));
+    FTLM_HIP_CHECK(hipEventCreateWithFlags(&pin.ev[i], hipEventDisableTiming));
+  }
+  DevBuf<double2> dev(static_cast<size_t>(dim));
+  std::vector<double> ss(static_cast<size_t>(R), 0.0);
+  const unsigned g = static_cast<unsigned>(std::min<int64_t>(8192, (dim + 255) / 256));
+  for (int r = 0; r < R; ++r) {
+    double s2 = 0.0;
+    int64_t fill = 0, base = 0;
+    int which = 0;
+    bool used[2] = {false, false};
+    auto flush = [&] {
+      if (fill == 0) return;
+      FTLM_HIP_CHECK(hipMemcpyAsync(dev.p + base, pin.h[which], fill * sizeof(double2), hipMemcpyHostToDevice, st));
+      FTLM_HIP_CHECK(hipEventRecord(pin.ev[which], st));
+      used[which] = true;
+      base += fill;
+      fill = 0;
+      which ^= 1;
+      if (used[which]) FTLM_HIP_CHECK(hipEventSynchronize(pin.ev[which]));  // its last copy is done
+    };
+    ftlm_gauss::draw_sample(seeds[static_cast<size_t>(r)], dim, split_gauss_threads(),
+                            [&](int64_t, int64_t n, const std::complex<double>* v) {
+                              for (int64_t i = 0; i < n; ++i) {
+                                s2 += std::norm(v[i]);
+                                pin.h[which][fill++] = make_double2(v[i].real(), v[i].imag());
+                                if (fill == kChunk) flush();
+                              }
+                            });
+    flush();
+    ss[static_cast<size_t>(r)] = s2;
+    if (K.dtype == FTLM_C64)
+      hipLaunchKernelGGL(k_sample_to_slot<float>, dim3(g), dim3(256), 0, st, dim, int64_t{0}, R, r, dev.p,
+                         static_cast<float2*>(K.slot(0)));
+    else
+      hipLaunchKernelGGL(k_sample_to_slot<double>, dim3(g), dim3(256), 0, st, dim, int64_t{0}, R, r, dev.p,
+                         static_cast<double2*>(K.slot(0)));
+    check_launch("k_sample_to_slot");
+    FTLM_HIP_CHECK(hipStreamSynchronize(st));  // dev is reused by the next sample
+  }
+  finish_start(K, ss, st);
 }
 
 // Fill the start block with R start vectors.  seeds[r] per sample.
@@ -881,6 +982,10 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
     check_launch("k_phase_start");
     K.start(st);
     FTLM_HIP_CHECK(hipStreamSynchronize(st));
+    return;
+  }
+  if (q0 == nullptr && split_gauss(dim, R)) {
+    fill_start_split_gauss(K, seeds, st);
     return;
   }
   // host-generated (reference mt19937 Gaussian or caller-supplied), interleaved [dim][R], block by
@@ -957,6 +1062,14 @@ void fill_start(Krylov& K, int start, const std::vector<unsigned>& seeds, const 
     FTLM_HIP_CHECK(hipEventRecord(stg.ev[which], st));
     used_stg[which] = true;
   }
+  finish_start(K, ss, st);
+}
+
+// Start norms from the host sums ss[r] of |q_r|^2 (src/lanczos.cpp:166-175: 1 / ||q||, or e_0 when
+// the norm is below 1e-18), then the recurrence state.
+void finish_start(Krylov& K, const std::vector<double>& ss, hipStream_t st) {
+  const int R = K.R;
+  const size_t es = K.dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2);
   std::vector<double> inv(static_cast<size_t>(R));
   for (int r = 0; r < R; ++r) {
     const double n0 = std::sqrt(ss[static_cast<size_t>(r)]);

@@ -603,6 +603,41 @@ def test_gaussian_starts_of_many_samples_on_a_large_sector(oracle):
         assert np.max(np.abs(rd.beta[0, r, :M - 1] - b[:M - 1])) <= 1e-12 * np.max(np.abs(b)), r
 
 
+def test_gaussian_starts_split_over_threads_equal_the_serial_stream():
+    """Large sectors with few samples draw each reference start (one mt19937 / normal_distribution
+    stream) on all host threads with MT19937 jump-ahead.  On 4x4 (5,5) (dim 19,079,424, several
+    generation rounds) the start of one sample (split path, R = 1) and the same sample among 16
+    (one thread per sample, R = 16, seed + r * 100003 for r = 0) give the same Lanczos
+    coefficients to 1e-12 relative (the dot sums run in another order; a different start would
+    differ at 1e-3).  A 4x4 (8,8) start (dim 165,636,900) is timed."""
+    import time
+
+    import torch
+
+    p = api.HubbardParams(4, 4, 1.0, -0.25, 8.0, 1.0, 0.1, 0.0)
+    op = api.HubbardOperator.from_params(p, 5, 5)
+    assert op.dim() == 4368 ** 2
+    M, seed = 4, 29
+    say("split gaussian starts 4x4 (5,5)")
+    one = api.ftlm_ritz([op], api.FtlmParams(n_random=1, lanczos_steps=M, seed=seed, start=L.FTLM_START_GAUSS_MT,
+                                             path=L.FTLM_PATH_PER_OP), keep_tridiagonals=True)
+    many = api.ftlm_ritz([op], api.FtlmParams(n_random=16, lanczos_steps=M, seed=seed, start=L.FTLM_START_GAUSS_MT,
+                                              path=L.FTLM_PATH_PER_OP), keep_tridiagonals=True)
+    a1, a16 = one.alpha[0, 0], many.alpha[0, 0]
+    b1, b16 = one.beta[0, 0, :M - 1], many.beta[0, 0, :M - 1]
+    say("alpha", a1, a16)
+    assert np.max(np.abs(a1 - a16)) <= 1e-12 * np.max(np.abs(a16))
+    assert np.max(np.abs(b1 - b16)) <= 1e-12 * np.max(np.abs(b16))
+    del op
+    torch.cuda.empty_cache()
+    big = api.HubbardOperator.from_params(p, 8, 8)
+    t0 = time.perf_counter()
+    ks = api.KrylovSession(big, 1, 1, dtype=L.FTLM_C128, start=L.FTLM_START_GAUSS_MT, seed=seed)
+    sec = time.perf_counter() - t0
+    say(f"one 4x4 (8,8) reference start (draw, norm, upload): {sec:.2f} s")
+    ks.close()
+
+
 def test_gradient_path_draws_the_same_starts_on_a_large_sector():
     """The derivative entry points (forward mode and the adjoint) draw the reference starts through
     their own host path; on a sector past one generation block (4x3 (5,4), dim 392,040, 4 samples)
