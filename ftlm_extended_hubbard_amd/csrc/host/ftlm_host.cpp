@@ -408,45 +408,6 @@ GramBlockOperator::GramBlockOperator(const SectorOperator& sector, const KBlockO
 // ------------------------------------------------------------------ translations and k-blocks
 namespace symmetry {
 
-RawState translate_raw_state_rect(RawState s, int lx, int ly, int dx, int dy) {
-  const uint32_t in = pack_raw_state(s);
-  uint32_t out = 0;
-  check(ftlm_hip_translate_states(lx, ly, &in, 1, dx, dy, &out, nullptr));
-  return unpack_raw_state(out);
-}
-
-double fermionic_translation_sign_rect(RawState s, int lx, int ly, int dx, int dy) {
-  const uint32_t in = pack_raw_state(s);
-  double sg = 0.0;
-  check(ftlm_hip_translate_states(lx, ly, &in, 1, dx, dy, nullptr, &sg));
-  return sg;
-}
-
-RawState canonical_raw_state_rect(RawState s, int lx, int ly) {
-  const uint32_t in = pack_raw_state(s);
-  uint32_t out = 0;
-  check(ftlm_hip_canonical_states(lx, ly, &in, 1, &out, nullptr));
-  return unpack_raw_state(out);
-}
-
-std::vector<std::pair<std::int8_t, std::int8_t>> translation_stabilizer_rect(RawState s, int lx, int ly) {
-  std::vector<std::pair<std::int8_t, std::int8_t>> st;
-  const uint32_t in = pack_raw_state(s);
-  for (int dy = 0; dy < ly; ++dy)
-    for (int dx = 0; dx < lx; ++dx) {
-      uint32_t out = 0;
-      check(ftlm_hip_translate_states(lx, ly, &in, 1, dx, dy, &out, nullptr));
-      if (out == in) st.emplace_back(static_cast<std::int8_t>(dx), static_cast<std::int8_t>(dy));
-    }
-  return st;
-}
-
-std::complex<double> translation_bloch_phase(MomentumSector K, int dx, int dy) {
-  double re = 0.0, im = 0.0;
-  check(ftlm_hip_bloch_phase(K.lx, K.ly, K.kx, K.ky, dx, dy, &re, &im));
-  return {re, im};
-}
-
 std::complex<double> fermionic_stabilizer_character_A_rect(
     const RawState& rep, MomentumSector K, int lx, int ly,
     const std::vector<std::pair<std::int8_t, std::int8_t>>& stabilizer) {

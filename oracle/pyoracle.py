@@ -36,6 +36,9 @@ def build(with_ref: bool | None = None) -> None:
         with_ref = os.path.isdir("/root/reference/src")
     if with_ref:
         targets.append("ref")
+        # the reference's symmetry tests against the drop-in headers need libftlm.so
+        if os.path.exists(os.path.join(os.path.dirname(HERE), "ftlm_extended_hubbard_amd", "libftlm.so")):
+            targets.append("reftests")
     subprocess.run(["make", "-s", "-C", HERE] + targets, check=True)
 
 

@@ -52,9 +52,7 @@ def test_cpp_symbols_exported():
         "ftlm::symmetry::OrbitKBlockBasis::map_raw_to_rep(",
         "ftlm::symmetry::OrbitKBlockBasis::reduced_index_of_rep(",
         "ftlm::symmetry::apply_orbit_k_block(",
-        "ftlm::symmetry::translation_bloch_phase(",
-        "ftlm::symmetry::canonical_raw_state_rect(",
-        "ftlm::symmetry::fermionic_translation_sign_rect(",
+        "ftlm::symmetry::fermionic_stabilizer_character_A_rect(",
     ]:
         assert sym in out, sym
 
@@ -65,3 +63,31 @@ def test_cpp_tests_built():
         pytest.skip("C++ tests not built")
     for t in ("test_pipeline", "test_thermo", "test_orbit", "test_apply_host"):
         assert os.access(os.path.join(bdir, t), os.X_OK), t
+
+
+# The reference's own symmetry tests, compiled against the drop-in headers and libftlm.so by
+# oracle/Makefile (target reftests, where the reference checkout exists; the binaries live in
+# oracle/_ref/tests).  These three use only host code and run here; test_momentum_partition_2x2
+# and test_orbit_k_block_basis need the device (tests/test_gpu_cpp.py).
+REFT = os.path.join(ROOT, "oracle", "_ref", "tests")
+HOST_REF_TESTS = ("test_symmetry_translation_stage1", "test_symmetry_stage2", "test_symmetry_stage3")
+ALL_REF_TESTS = HOST_REF_TESTS + ("test_momentum_partition_2x2", "test_orbit_k_block_basis")
+
+
+def test_reference_symmetry_tests_built_against_dropin_headers():
+    if not os.path.isdir("/root/reference/tests"):
+        pytest.skip("no reference checkout here")
+    for t in ALL_REF_TESTS:
+        assert os.access(os.path.join(REFT, t), os.X_OK), f"{t}: run __graft_entry__.build()"
+
+
+@pytest.mark.parametrize("prog", HOST_REF_TESTS)
+def test_reference_symmetry_tests_pass_on_host(prog):
+    import subprocess
+
+    path = os.path.join(REFT, prog)
+    if not os.path.exists(path):
+        pytest.skip("reference tests not built (no reference checkout)")
+    r = subprocess.run([path], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"{prog} ok" in r.stdout

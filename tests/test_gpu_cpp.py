@@ -31,6 +31,21 @@ def test_cpp_program(prog):
     assert f"{prog} ok" in r.stdout
 
 
+@pytest.mark.parametrize("prog", ["test_momentum_partition_2x2", "test_orbit_k_block_basis"])
+def test_reference_symmetry_tests_on_device(prog):
+    """The reference's own tests (tests/*.cpp of the reference checkout), compiled against the
+    drop-in headers and libftlm.so into oracle/_ref/tests where that checkout exists; these two
+    build FockBasis / OrbitKBlockBasis, which live on the device."""
+    path = os.path.join(ROOT, "oracle", "_ref", "tests", prog)
+    if not os.path.exists(path):
+        pytest.skip("reference tests not built here (no reference checkout when the tree was built)")
+    say("reference test", prog)
+    r = subprocess.run([path], capture_output=True, text=True, timeout=100)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stderr
+    assert "ok" in r.stdout
+
+
 def test_pybind_hot_path(golden):
     from ftlm_extended_hubbard_amd import _ftlm_core as c
 
