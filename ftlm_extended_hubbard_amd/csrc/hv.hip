@@ -504,7 +504,8 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
 // the rounds of a step then read only LDS.  Each step begins with one wait for the copies and
 // one barrier.  A target's hops are added in increasing source order (steps in order, rows in
 // order), the order of the CSR lists, so every H*V kernel gives the same bits.
-// The down pass writes y = diag x + down hops; the up pass adds its sum to y.
+// The down pass writes y = diag x + down hops; the up pass adds its sum to y and takes the
+// <xs x | y> partials of the finished y.
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(4))) const uint32_t const_u32_t;
@@ -775,11 +776,11 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
           o.y = ui;
         }
         yp[jo] = o;
-        if (dotp) {
+        if (dotp) {  // the column pass: <xs x | y> of the finished y (the row pass takes no partials)
           const V2 q = xp[jo];
           const double qr = sx * static_cast<double>(q.x), qi = sx * static_cast<double>(q.y);
-          dre[v] += qr * static_cast<double>(ur) + qi * static_cast<double>(ui);
-          dimg[v] += qr * static_cast<double>(ui) - qi * static_cast<double>(ur);
+          dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+          dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
         }
       }
       asm volatile("" ::: "memory");  // keep each round's y traffic together (bounds live registers)
@@ -1020,12 +1021,14 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
       const int bpx = std::max(1, std::min(32 * kStepWgPerCu, sh.bpx > 0 ? sh.bpx : 32 * kStepWgPerCu));
       const int grid = kXcd * bpx;
       int32_t* ctr = Sx->tile_counters(st);
+      // <xs x | y> comes whole from the column pass, which reads x at the targets anyway: the
+      // row pass would have to read x there a second time for its part
       record_pass_event(0, st);
-      launch_step_pass<T, REAL, false>(S, xp, yp, RT, r0, xs, dotp, 0, ctr, grid, st);
+      launch_step_pass<T, REAL, false>(S, xp, yp, RT, r0, xs, nullptr, 0, ctr, grid, st);
       record_pass_event(1, st);
-      launch_step_pass<T, REAL, true>(S, xp, yp, RT, r0, xs, dotp, grid, ctr + 16, grid, st);
+      launch_step_pass<T, REAL, true>(S, xp, yp, RT, r0, xs, dotp, 0, ctr + 16, grid, st);
       record_pass_event(2, st);
-      return 2 * grid;
+      return grid;
     }
   }
   if (!p.two) {

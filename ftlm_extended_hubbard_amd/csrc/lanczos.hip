@@ -138,6 +138,94 @@ __global__ __launch_bounds__(kUpdBlock) void k_update_flat(int64_t n, int R, typ
   }
 }
 
+// The same update on 16-byte units with two units in flight per thread: a unit is one complex128
+// value, or two complex64 values of vectors (2p, 2p + 1) (R even).  The block size is a multiple
+// of P = R / EPU units and so is the grid stride, so each thread always sees the same vectors; it
+// keeps one norm partial per vector of its unit.  Per value the arithmetic is k_update_flat's.
+template <typename T>
+__global__ __launch_bounds__(kUpdBlock) void k_update_units(int64_t nu, int R, typename V2Of<T>::t* __restrict__ C,
+                                                            const typename V2Of<T>::t* __restrict__ A,
+                                                            const typename V2Of<T>::t* __restrict__ B,
+                                                            const double2* __restrict__ z, const double* __restrict__ sa,
+                                                            const double* __restrict__ sb,
+                                                            const double* __restrict__ bprev,
+                                                            const int* __restrict__ alive, double* __restrict__ normp,
+                                                            const int use_prev) {
+  using V2 = typename V2Of<T>::t;
+  constexpr int EPU = 16 / static_cast<int>(sizeof(V2));  // values per unit
+  struct alignas(16) Unit {
+    V2 v[EPU];
+  };
+  const int P = R / EPU;
+  const int nthr = blockDim.x;  // multiple of P
+  const int p = threadIdx.x % P;
+  bool live[EPU];
+  double zr[EPU], zi[EPU], qa[EPU], qb[EPU], nrm[EPU];
+#pragma unroll
+  for (int e = 0; e < EPU; ++e) {
+    const int r = p * EPU + e;
+    live[e] = alive[r] != 0;
+    zr[e] = z[r].x;
+    zi[e] = z[r].y;
+    qa[e] = sa[r];
+    qb[e] = bprev[r] * sb[r];
+    nrm[e] = 0.0;
+  }
+  Unit* __restrict__ Cu = reinterpret_cast<Unit*>(C);
+  const Unit* __restrict__ Au = reinterpret_cast<const Unit*>(A);
+  const Unit* __restrict__ Bu = reinterpret_cast<const Unit*>(B);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * nthr;
+  auto one = [&](const Unit& c, const Unit& a, const Unit& b, Unit& out) {
+#pragma unroll
+    for (int e = 0; e < EPU; ++e) {
+      double wr = static_cast<double>(c.v[e].x), wi = static_cast<double>(c.v[e].y);
+      const double ar = qa[e] * static_cast<double>(a.v[e].x), ai = qa[e] * static_cast<double>(a.v[e].y);
+      wr -= zr[e] * ar - zi[e] * ai;
+      wi -= zr[e] * ai + zi[e] * ar;
+      if (use_prev) {
+        wr -= qb[e] * static_cast<double>(b.v[e].x);
+        wi -= qb[e] * static_cast<double>(b.v[e].y);
+      }
+      out.v[e].x = static_cast<T>(live[e] ? wr : 0.0);
+      out.v[e].y = static_cast<T>(live[e] ? wi : 0.0);
+      const double sr = static_cast<double>(out.v[e].x), si = static_cast<double>(out.v[e].y);
+      nrm[e] += sr * sr + si * si;
+    }
+  };
+  int64_t f = static_cast<int64_t>(blockIdx.x) * nthr + threadIdx.x;
+  for (; f + stride < nu; f += 2 * stride) {  // two units in flight
+    const Unit c0 = Cu[f], c1 = Cu[f + stride], a0 = Au[f], a1 = Au[f + stride];
+    Unit b0{}, b1{};
+    if (use_prev) {
+      b0 = Bu[f];
+      b1 = Bu[f + stride];
+    }
+    Unit o0, o1;
+    one(c0, a0, b0, o0);
+    one(c1, a1, b1, o1);
+    Cu[f] = o0;
+    Cu[f + stride] = o1;
+  }
+  if (f < nu) {
+    const Unit c0 = Cu[f], a0 = Au[f];
+    Unit b0{};
+    if (use_prev) b0 = Bu[f];
+    Unit o0;
+    one(c0, a0, b0, o0);
+    Cu[f] = o0;
+  }
+  __shared__ double red[EPU][kUpdBlock];
+#pragma unroll
+  for (int e = 0; e < EPU; ++e) red[e][threadIdx.x] = nrm[e];
+  __syncthreads();
+  if (static_cast<int>(threadIdx.x) < R) {
+    const int q = threadIdx.x / EPU, e = threadIdx.x % EPU;
+    double s = 0.0;
+    for (int t = q; t < nthr; t += P) s += red[e][t];
+    normp[static_cast<int64_t>(blockIdx.x) * R + threadIdx.x] = s;
+  }
+}
+
 // |q_r|^2 partials of the start block, same flat layout.
 template <typename T>
 __global__ __launch_bounds__(kUpdBlock) void k_norm_flat(int64_t n, int R, const typename V2Of<T>::t* __restrict__ q,
@@ -609,8 +697,24 @@ struct Krylov {
   int upd_threads() const { return (kUpdBlock / R) * R; }
 
   void launch_update(void* C, const void* A, const void* B, int use_prev, hipStream_t st) {
-    const dim3 g(nblk_upd), b(upd_threads());
     const int64_t n = dim * R;
+    if (dtype == FTLM_C128 || R % 2 == 0) {  // 16-byte units (k_update_units)
+      const int epu = dtype == FTLM_C64 ? 2 : 1, P = R / epu;
+      const int thr = (kUpdBlock / P) * P;
+      const int64_t nu = n / epu;
+      const dim3 g(nblk_upd), b(thr);  // nblk_upd partial blocks, as the readers of normp expect
+      if (dtype == FTLM_C64)
+        hipLaunchKernelGGL(k_update_units<float>, g, b, 0, st, nu, R, static_cast<float2*>(C),
+                           static_cast<const float2*>(A), static_cast<const float2*>(B), z.p, sa.p, sb.p, bprev.p,
+                           alive.p, normp.p, use_prev);
+      else
+        hipLaunchKernelGGL(k_update_units<double>, g, b, 0, st, nu, R, static_cast<double2*>(C),
+                           static_cast<const double2*>(A), static_cast<const double2*>(B), z.p, sa.p, sb.p, bprev.p,
+                           alive.p, normp.p, use_prev);
+      check_launch("k_update_units");
+      return;
+    }
+    const dim3 g(nblk_upd), b(upd_threads());
     if (dtype == FTLM_C64)
       hipLaunchKernelGGL(k_update_flat<float>, g, b, 0, st, n, R, static_cast<float2*>(C),
                          static_cast<const float2*>(A), static_cast<const float2*>(B), z.p, sa.p, sb.p, bprev.p,
