@@ -501,6 +501,58 @@ def kblock_all_workload(F, L, torch):
     return out
 
 
+def kstack_workload(F, L, torch, reps=3):
+    """Momentum-block applies at 4x4 (8,8) (t=1, t'=-0.25, U=8, V=1, phi=0), one block against the
+    momentum stack of all 16: per-block k_kblock_apply at R=1 complex128 and R=16 complex64 on the
+    largest block, and the stack apply at 16 (complex128, one sample per block) and 32 (complex64,
+    two per block) vectors.  HIP-event time per apply (mean of `reps` after one warm-up), the
+    per-sample-and-block cost, and one 4 GiB device copy to calibrate rocprofv3's byte counters."""
+    import numpy as np
+
+    p = F.HubbardParams(4, 4, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0)
+    first = F.KBlockOperator.from_params(p, NU, ND, F.MomentumSector(0, 0, 4, 4))
+    blocks = [first] + [F.KBlockOperator.from_params(p, NU, ND, F.MomentumSector(kx, ky, 4, 4), share=first)
+                        for ky in range(4) for kx in range(4) if (kx, ky) != (0, 0)]
+    S = F.KStackOperator.from_blocks(blocks)
+    dims = [b.dim() for b in blocks]
+    B = blocks[int(np.argmax(dims))]
+    out = {"metric": "4x4 (8,8) momentum-block applies, one block and the 16-block stack", "unit": "ms",
+           "higher_is_better": False, "n_gpus": 1, "data": "synthetic",
+           "config": {"workload": "4x4 (8,8) momentum blocks, 16 K", "n_orbits": S.dim(), "block_dim": B.dim()},
+           "apply": {}}
+
+    def timed(op, R, dt, ct):
+        X = torch.randn(op.dim() * R, dtype=ct, device="cuda")
+        Y = torch.empty_like(X)
+        op.apply_device(X, Y, R=R, dtype=dt)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            op.apply_device(X, Y, R=R, dtype=dt)
+        e1.record()
+        torch.cuda.synchronize()
+        del X, Y
+        torch.cuda.empty_cache()
+        return e0.elapsed_time(e1) / reps
+
+    for name, op, R, dt, ct, per in (("block_R1_c128", B, 1, L.FTLM_C128, torch.complex128, 1),
+                                     ("block_R16_c64", B, 16, L.FTLM_C64, torch.complex64, 16),
+                                     ("stack_W16_c128", S, 16, L.FTLM_C128, torch.complex128, 16),
+                                     ("stack_W32_c64", S, 32, L.FTLM_C64, torch.complex64, 32)):
+        ms = timed(op, R, dt, ct)
+        # per sample and block: a block apply serves R samples of one block, a stack apply W / 16
+        # samples of each of the 16 blocks
+        per_sample_block = ms / R if op is B else ms / (R / 16) / 16
+        out["apply"][name] = {"ms": round(ms, 3), "ms_per_sample_and_block": round(per_sample_block, 3)}
+        log(f"{name}: {ms:.3f} ms per apply, {per_sample_block:.3f} ms per sample and block")
+    src = torch.empty(1 << 30, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    return out
+
+
 def kgram_workload(F, L, torch, reps=5):
     """Momentum Gram blocks (HubbardMomentumBlock) at 4x4 (8,8), t=1, t'=-0.25, U=8, V=1, phi=0,
     K=(1,0) and K=(0,0): block build time, and the block apply (lift into the full sector, full
@@ -848,7 +900,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--workload", choices=["sector", "kblock", "kblock_all", "gc4x2", "grad", "graph", "reorth", "tune",
-                                                "hostlink", "kgram"], default="sector",
+                                                "hostlink", "kgram", "kstack"], default="sector",
                     help="sector: the 4x4 (8,8) full-sector line (default); kblock: the 4x4 (8,8) momentum-block "
                          "line; gc4x2: the 4x2 grand-canonical line (81 sectors, one persistent launch); grad: "
                          "the differentiated FTLM line (4x3 (6,6) and the 4x4 (8,8) K=(0,0) block)")
@@ -877,6 +929,9 @@ def main():
     if args.profile_hv:
         shape = None if args.hv_shape is None else tuple(int(v) for v in args.hv_shape.split(","))
         profile_hv(F, L, torch, R, shape)
+        return
+    if args.workload == "kstack":
+        print(json.dumps(kstack_workload(F, L, torch)), flush=True)
         return
     if args.workload == "kblock_all":
         print(json.dumps(kblock_all_workload(F, L, torch)), flush=True)
