@@ -390,6 +390,24 @@ __device__ __forceinline__ RowWalk row_walk(int64_t n, int rpb, int rl) {
   return {static_cast<int64_t>(blockIdx.x) * rpb + rl, n, static_cast<int64_t>(gridDim.x) * rpb};
 }
 
+// Coefficient of a stored term: the diagonal entry carries the orbit's diagonal dg, a hop term
+// ((c_bond * (sign * sqrt(|Stab_row| / |Stab_col|))) * chi*_K(R)) with chi the momentum's 16
+// Bloch phases.  Branch-free (every term reads its three table entries, all indices in range), so
+// that the LDS reads of a group of terms issue together.
+__device__ __forceinline__ void kb_term_coef(const uint2 en, const double dg, const double* s_nrm,
+                                             const double2* s_coef, const double2* s_chi, double* cr, double* ci) {
+  const double nrm = s_nrm[(en.y >> kStabShift) & 0xFFu];
+  const double w = ((en.x >> 28) & 1u) ? -nrm : nrm;
+  const double2 cb = s_coef[en.y & kBondMask];
+  const double2 ch = s_chi[(en.x >> 24) & 15u];
+  const double hr = cb.x * w, hi = cb.y * w;
+  const double hr2 = hr * ch.x - hi * ch.y;
+  const double hi2 = hr * ch.y + hi * ch.x;
+  const bool diag = (en.x & kDiagFlag) != 0;
+  *cr = diag ? dg : hr2;
+  *ci = diag ? 0.0 : hi2;
+}
+
 template <int RB>
 struct KbGroup {
   static constexpr int value = RB <= 2 ? 8 : (RB <= 4 ? 4 : 2);
@@ -442,18 +460,7 @@ __global__ __launch_bounds__(256) void k_kblock_apply(
 #pragma unroll
       for (int j = 0; j < G; ++j) {
         double cr, ci;
-        if (en[j].x & kDiagFlag) {
-          cr = dg;
-          ci = 0.0;
-        } else {
-          const double nrm = s_nrm[(en[j].y >> kStabShift) & 0xFFu];
-          const double w = ((en[j].x >> 28) & 1u) ? -nrm : nrm;
-          const double2 cb = s_coef[en[j].y & kBondMask];
-          const double hr = cb.x * w, hi = cb.y * w;
-          const double2 ch = s_chi[(en[j].x >> 24) & 15u];
-          cr = hr * ch.x - hi * ch.y;
-          ci = hr * ch.y + hi * ch.x;
-        }
+        kb_term_coef(en[j], dg, s_nrm, s_coef, s_chi, &cr, &ci);
         if (cc[j] < 0) cr = ci = 0.0;
         const T tr = static_cast<T>(cr), ti = static_cast<T>(ci);
 #pragma unroll
@@ -513,7 +520,9 @@ __global__ __launch_bounds__(256) void k_kblock_apply(
 // Wide blocks: the RB vectors of a row are spread over L lanes of 16 bytes (VPL vectors each), as
 // in the full-sector kernels, so a term's gather is one contiguous RB * s run read by L lanes; a
 // workgroup covers 256 / L rows and every lane keeps 8 terms in flight.
-template <typename T, int RB>
+// COOP as in k_kstack_apply: lanes 0 .. G-1 of a row load the next G terms' entries and block
+// columns, and the row's lanes share them by cross-lane reads (L = 8 or 16 lanes per row).
+template <typename T, int RB, bool COOP>
 __global__ __launch_bounds__(256) void k_kblock_apply_split(
     const int32_t* __restrict__ rows, int64_t dk, const int32_t* __restrict__ row_off, const uint2* __restrict__ ent,
     const int32_t* __restrict__ colk, const double* __restrict__ diag, const double2* __restrict__ coef, int nbond,
@@ -552,12 +561,30 @@ __global__ __launch_bounds__(256) void k_kblock_apply_split(
     for (int32_t e = e0; e < e1; e += G) {
       uint2 en[G];
       int32_t cc[G];
+      if constexpr (COOP) {
+        const int lane = static_cast<int>(threadIdx.x) & 63, rbase = lane - sub;
+        uint2 me = make_uint2(0u, 0u);
+        int32_t mc = -1;
+        if (sub < G) {
+          const int32_t ej = min(e + sub, e1 - 1);
+          me = ent[ej];
+          const int32_t c = colk[ej];
+          mc = (e + sub < e1 && c >= 0) ? c : -1;
+        }
 #pragma unroll
-      for (int j = 0; j < G; ++j) {
-        const int32_t ej = min(e + j, e1 - 1);
-        en[j] = ent[ej];
-        const int32_t c = colk[ej];
-        cc[j] = (e + j < e1 && c >= 0) ? c : -1;
+        for (int j = 0; j < G; ++j) {
+          en[j].x = __shfl(me.x, rbase + j, 64);
+          en[j].y = __shfl(me.y, rbase + j, 64);
+          cc[j] = __shfl(mc, rbase + j, 64);
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+          const int32_t ej = min(e + j, e1 - 1);
+          en[j] = ent[ej];
+          const int32_t c = colk[ej];
+          cc[j] = (e + j < e1 && c >= 0) ? c : -1;
+        }
       }
       V2 src[G][VPL];
 #pragma unroll
@@ -569,18 +596,7 @@ __global__ __launch_bounds__(256) void k_kblock_apply_split(
 #pragma unroll
       for (int j = 0; j < G; ++j) {
         double cr, ci;
-        if (en[j].x & kDiagFlag) {
-          cr = dg;
-          ci = 0.0;
-        } else {
-          const double nrm = s_nrm[(en[j].y >> kStabShift) & 0xFFu];
-          const double w = ((en[j].x >> 28) & 1u) ? -nrm : nrm;
-          const double2 cb = s_coef[en[j].y & kBondMask];
-          const double hr = cb.x * w, hi = cb.y * w;
-          const double2 ch = s_chi[(en[j].x >> 24) & 15u];
-          cr = hr * ch.x - hi * ch.y;
-          ci = hr * ch.y + hi * ch.x;
-        }
+        kb_term_coef(en[j], dg, s_nrm, s_coef, s_chi, &cr, &ci);
         if (cc[j] < 0) cr = ci = 0.0;
         const T tr = static_cast<T>(cr), ti = static_cast<T>(ci);
 #pragma unroll
@@ -640,7 +656,11 @@ __global__ __launch_bounds__(256) void k_kblock_apply_split(
 // orbit's slots of the blocks it is not a row of are written as 0.
 // ONEJ: every lane's VPL slots belong to one block (rk a multiple of VPL), so a term's coefficient
 // is computed once per lane.
-template <typename T, int VPL, bool ONEJ>
+// COOP (L a power of two >= 8, so that a row's lanes sit in one wavefront): lanes 0 .. G-1 of a
+// row load the entries and membership bits of the row's next G terms with one load each, and every
+// lane takes term j's from lane j of its row by a cross-lane read, in place of G loads per lane of
+// the same addresses (the vector-memory data path is what these applies saturate).
+template <typename T, int VPL, bool ONEJ, bool COOP>
 __global__ __launch_bounds__(256) void k_kstack_apply(
     int64_t nrep, int nk, int rk, const uint16_t* __restrict__ mem, const uint16_t* __restrict__ memc,
     const int32_t* __restrict__ row_off, const uint2* __restrict__ ent, const double* __restrict__ diag,
@@ -683,10 +703,27 @@ __global__ __launch_bounds__(256) void k_kstack_apply(
       for (int32_t e = e0; e < e1; e += G) {
         uint2 en[G];
         uint32_t mb[G];
+        if constexpr (COOP) {
+          const int lane = static_cast<int>(threadIdx.x) & 63, rbase = lane - sub;
+          uint2 me = make_uint2(0u, 0u);
+          uint32_t mm = 0u;
+          if (sub < G) {
+            const int32_t ej = min(e + sub, e1 - 1);
+            me = ent[ej];
+            mm = e + sub < e1 ? static_cast<uint32_t>(memc[ej]) : 0u;
+          }
 #pragma unroll
-        for (int j = 0; j < G; ++j) en[j] = ent[min(e + j, e1 - 1)];
+          for (int j = 0; j < G; ++j) {
+            en[j].x = __shfl(me.x, rbase + j, 64);
+            en[j].y = __shfl(me.y, rbase + j, 64);
+            mb[j] = __shfl(mm, rbase + j, 64);
+          }
+        } else {
 #pragma unroll
-        for (int j = 0; j < G; ++j) mb[j] = e + j < e1 ? static_cast<uint32_t>(memc[min(e + j, e1 - 1)]) : 0u;
+          for (int j = 0; j < G; ++j) en[j] = ent[min(e + j, e1 - 1)];
+#pragma unroll
+          for (int j = 0; j < G; ++j) mb[j] = e + j < e1 ? static_cast<uint32_t>(memc[min(e + j, e1 - 1)]) : 0u;
+        }
         V2 src[G][VPL];
 #pragma unroll
         for (int j = 0; j < G; ++j) {
@@ -705,15 +742,10 @@ __global__ __launch_bounds__(256) void k_kstack_apply(
           T trv[VPL], tiv[VPL];
 #pragma unroll
           for (int v = 0; v < (ONEJ ? 1 : VPL); ++v) {
-            double cr, ci;
-            if (dgl) {
-              cr = dg;
-              ci = 0.0;
-            } else {
-              const double2 ch = s_chi[jb[v]][t];
-              cr = hr * ch.x - hi * ch.y;
-              ci = hr * ch.y + hi * ch.x;
-            }
+            const double2 ch = s_chi[jb[v]][t];
+            const double hr2 = hr * ch.x - hi * ch.y;
+            const double hi2 = hr * ch.y + hi * ch.x;
+            double cr = dgl ? dg : hr2, ci = dgl ? 0.0 : hi2;
             if (!((mb[j] >> jb[v]) & 1u)) cr = ci = 0.0;
             trv[v] = static_cast<T>(cr);
             tiv[v] = static_cast<T>(ci);
@@ -1228,10 +1260,14 @@ void launch_kblock(const KBlock* B, const void* x, void* y, int RT, int r0, cons
                    hipStream_t st, int grid) {
   using V2 = typename Cx<T>::v2;
   const OrbitGraph& G = *B->G;
-  if (RB * sizeof(V2) >= 32) {
-    hipLaunchKernelGGL((k_kblock_apply_split<T, RB>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, G.row_off.p,
-                       G.ent.p, B->colk.p, G.diag.p, G.coef.p, G.nbond, B->ph, static_cast<const V2*>(x),
-                       static_cast<V2*>(y), RT, r0, xs, dotp);
+  if (RB * sizeof(V2) >= 128) {  // 8 or more lanes per row: shared entry loads
+    hipLaunchKernelGGL((k_kblock_apply_split<T, RB, true>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim,
+                       G.row_off.p, G.ent.p, B->colk.p, G.diag.p, G.coef.p, G.nbond, B->ph,
+                       static_cast<const V2*>(x), static_cast<V2*>(y), RT, r0, xs, dotp);
+  } else if (RB * sizeof(V2) >= 32) {
+    hipLaunchKernelGGL((k_kblock_apply_split<T, RB, false>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim,
+                       G.row_off.p, G.ent.p, B->colk.p, G.diag.p, G.coef.p, G.nbond, B->ph,
+                       static_cast<const V2*>(x), static_cast<V2*>(y), RT, r0, xs, dotp);
   } else {
     hipLaunchKernelGGL((k_kblock_apply<T, RB>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, G.row_off.p,
                        G.ent.p, B->colk.p, G.diag.p, G.coef.p, G.nbond, B->ph, static_cast<const V2*>(x),
@@ -1438,20 +1474,28 @@ int hv_apply_kstack(const ftlm_op* op, const void* x, void* y, int R, int dtype,
   const float2* xf = static_cast<const float2*>(x);
   float2* yf = static_cast<float2*>(y);
 #define FTLM_KSTACK_ARGS S->dim, S->nk, rk, S->mem.p, S->memc.p, G.row_off.p, G.ent.p, G.diag.p, G.coef.p, G.nbond, S->chi.p
+  const int lanes = R / vpl;
+  const bool coop = lanes >= 8 && (lanes & (lanes - 1)) == 0;
+#define FTLM_KSTACK_GO(TT, VV, OJ, XP, YP)                                                                          \
+  do {                                                                                                           \
+    if (coop)                                                                                                    \
+      hipLaunchKernelGGL((k_kstack_apply<TT, VV, OJ, true>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS, XP, YP, \
+                         xs, dotp);                                                                              \
+    else                                                                                                         \
+      hipLaunchKernelGGL((k_kstack_apply<TT, VV, OJ, false>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS, XP, \
+                         YP, xs, dotp);                                                                          \
+  } while (0)
   if (dtype == FTLM_C64) {
     if (vpl == 2 && rk % 2 == 0)
-      hipLaunchKernelGGL((k_kstack_apply<float, 2, true>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS, xf, yf, xs,
-                         dotp);
+      FTLM_KSTACK_GO(float, 2, true, xf, yf);
     else if (vpl == 2)
-      hipLaunchKernelGGL((k_kstack_apply<float, 2, false>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS, xf, yf, xs,
-                         dotp);
+      FTLM_KSTACK_GO(float, 2, false, xf, yf);
     else
-      hipLaunchKernelGGL((k_kstack_apply<float, 1, true>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS, xf, yf, xs,
-                         dotp);
+      FTLM_KSTACK_GO(float, 1, true, xf, yf);
   } else {
-    hipLaunchKernelGGL((k_kstack_apply<double, 1, true>), dim3(grid), dim3(256), 0, st, FTLM_KSTACK_ARGS,
-                       static_cast<const double2*>(x), static_cast<double2*>(y), xs, dotp);
+    FTLM_KSTACK_GO(double, 1, true, static_cast<const double2*>(x), static_cast<double2*>(y));
   }
+#undef FTLM_KSTACK_GO
 #undef FTLM_KSTACK_ARGS
   check_launch("k_kstack_apply");
   return grid;
