@@ -87,6 +87,29 @@ __device__ __forceinline__ void axpy_block(T (&ar)[N], T (&ai)[N], const typenam
   }
 }
 
+// Summation order of the two-pass H*V kernels (gather and step passes), so that all of them give
+// the same bits:
+//   y = (part(S_down) + part(S_up)) + part(T(diag) * x),   part(a) = T(double(a) * xs)
+// where S_down and S_up are the down- and up-bond hop sums, each accumulated from 0 in increasing
+// source order, and the two additions are done in T.  The row pass stores part(S_down); the
+// column pass adds the rest (it reads x at its targets anyway for the <xs x | y> partials).
+// Contraction is off so that no kernel fuses a product into a sum.  The one-pass kernel of small
+// sectors (k_hv_flat) keeps the reference's order, diagonal first: y = part(diag x + S_down) +
+// part(S_up), so that the late, rounding-sensitive Lanczos coefficients of the small sectors the
+// reference's drivers print follow the reference's own.
+template <typename T>
+__device__ __forceinline__ T hv_part(T a, double sx) {
+#pragma clang fp contract(off)
+  return static_cast<T>(static_cast<double>(a) * sx);
+}
+template <typename T>
+__device__ __forceinline__ T hv_finish(T row_part, T s_up, T dg, T xq, double sx) {
+#pragma clang fp contract(off)
+  const T uu = static_cast<T>(static_cast<double>(s_up) * sx);
+  const T ug = static_cast<T>(static_cast<double>(dg * xq) * sx);
+  return (row_part + uu) + ug;
+}
+
 // acc += (cr + i ci) * v for N vectors already in registers.
 template <typename T, int N, bool REAL>
 __device__ __forceinline__ void axpy_regs(T (&ar)[N], T (&ai)[N], const typename Cx<T>::v2 (&v)[N], T cr, T ci) {
@@ -179,10 +202,15 @@ __device__ __forceinline__ void store_dot(const double (&dre)[VPL], const double
   }
 }
 
-__device__ __forceinline__ double diag_value(const SectorView& S, uint32_t u, uint32_t d, int iu, int id) {
-  const uint4 pl = S.planes[iu];
+// Diagonal of element (iu, id) from the strings u, d, the bit planes of row iu and the pair
+// counts nuu[iu], ndd[id].
+__device__ __forceinline__ double diag_from(const SectorView& S, uint32_t u, uint32_t d, uint4 pl, int nu, int nd) {
   const int cross = __popc(d & pl.x) + 2 * __popc(d & pl.y) + 4 * __popc(d & pl.z) + 8 * __popc(d & pl.w);
-  return S.U * static_cast<double>(__popc(u & d)) + S.V * static_cast<double>(S.nuu[iu] + S.ndd[id] + cross);
+  return S.U * static_cast<double>(__popc(u & d)) + S.V * static_cast<double>(nu + nd + cross);
+}
+
+__device__ __forceinline__ double diag_value(const SectorView& S, uint32_t u, uint32_t d, int iu, int id) {
+  return diag_from(S, u, d, S.planes[iu], S.nuu[iu], S.ndd[id]);
 }
 
 // ------------------------------------------------------------------ per-XCD tile queues
@@ -208,7 +236,7 @@ __device__ __forceinline__ void queue_done(int* ctr) {
   }
 }
 
-// ------------------------------------------------------------------ rows pass: diag + down bonds
+// ------------------------------------------------------------------ rows pass: down bonds
 // Tiles: (row iu, chunk of EPB ids), row-major; XCD x owns tiles [x*per, (x+1)*per).
 // Per hop: one entry read, one 24-bit multiply-add for the source offset from the (uniform) row
 // base, one LDS read of the pre-signed coefficient (index = entry >> 20), and the complex FMAs.
@@ -217,7 +245,7 @@ __device__ __forceinline__ void queue_done(int* ctr) {
 template <typename T, int RB, bool REAL>
 __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const typename Cx<T>::v2* __restrict__ x,
                                                     typename Cx<T>::v2* __restrict__ y, const int RT, const int r0,
-                                                    const double* __restrict__ xs, double2* __restrict__ dotp,
+                                                    const double* __restrict__ xs,
                                                     int* __restrict__ ctr, const int nchunk, const int ntiles) {
   using V2 = typename Cx<T>::v2;
   using SP = Split<T, RB>;
@@ -238,12 +266,9 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
   const uint32_t es = static_cast<uint32_t>(RT * sizeof(V2));      // bytes per element
   const uint32_t lane_b = static_cast<uint32_t>(rr * sizeof(V2));  // this lane's bytes in an element
   const int mc = S.maxcnt_dn, estride = S.entT_stride_dn;
-  double sxv[VPL], dre[VPL], dimg[VPL];
+  double sxv[VPL];
 #pragma unroll
-  for (int v = 0; v < VPL; ++v) {
-    sxv[v] = xs ? xs[rr + v] : 1.0;
-    dre[v] = dimg[v] = 0.0;
-  }
+  for (int v = 0; v < VPL; ++v) sxv[v] = xs ? xs[rr + v] : 1.0;
   int cur_row = -1, cur_chunk = -1;
   for (int local = next_tile(ctr, xcd, &s_tile); local < nmine; local = next_tile(ctr, xcd, &s_tile)) {
     const int tile = __builtin_amdgcn_readfirstlane(t0 + local);
@@ -279,18 +304,10 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
     __syncthreads();
     const int id = chunk * EPB + el;
     if (id >= Ddn) continue;
-    const uint32_t d = S.dn_mask[id];
-    const double dg = diag_value(S, u, d, iu, id);
     const char* xb = reinterpret_cast<const char*>(x + static_cast<int64_t>(iu) * row_stride);  // uniform
     T ar[VPL], ai[VPL];
-    V2 xo[VPL];
-    const V2* xe = reinterpret_cast<const V2*>(xb + (__umul24(static_cast<uint32_t>(id), es) + lane_b));
 #pragma unroll
-    for (int v = 0; v < VPL; ++v) {
-      xo[v] = xe[v];
-      ar[v] = static_cast<T>(dg) * xo[v].x;
-      ai[v] = static_cast<T>(dg) * xo[v].y;
-    }
+    for (int v = 0; v < VPL; ++v) ar[v] = ai[v] = static_cast<T>(0);
     const int kd = S.cnt_dn[id];
     // staged: slot k of this element at s_ent[k * EPB + el]; otherwise straight from the table
     const uint32_t* ent = kStage ? s_ent + el : S.entT_dn + id;
@@ -319,18 +336,12 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
 #pragma unroll
     for (int v = 0; v < VPL; ++v) {
       V2 o;
-      o.x = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
-      o.y = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
+      o.x = hv_part(ar[v], sxv[v]);
+      o.y = hv_part(ai[v], sxv[v]);
       st_stream(&ye[v], o);
-      if (dotp) {
-        const double qr = sxv[v] * static_cast<double>(xo[v].x), qi = sxv[v] * static_cast<double>(xo[v].y);
-        dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
-        dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
-      }
     }
   }
   queue_done(ctr);
-  if (dotp) store_dot<L, VPL, RB>(dre, dimg, dotp, blockIdx.x, RT, r0);
 }
 
 // Up-bond accumulation of one element over its padded hop list.  ONE_ID: every lane of the
@@ -367,7 +378,7 @@ __device__ __forceinline__ void up_hops(T (&ar)[VPL], T (&ai)[VPL], const typena
   }
 }
 
-// ------------------------------------------------------------------ column pass: up bonds, y +=
+// ------------------------------------------------------------------ column pass: up bonds + diagonal, y +=
 // Tiles: (stripe of W ids, chunk of EPB/W rows), stripe-major; element e of a tile ->
 // (row chunk*EPB/W + e/W, id stripe*W + e%W).  While an XCD works through one stripe (all rows x
 // W ids <= ~3 MB) every up bond of the stripe reads from that XCD's L2.
@@ -472,20 +483,19 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
       up_hops<T, VPL, REAL, false>(ar, ai, xcol, ent, eld, ku, rs24, s_c, s_xm, d);
     const int64_t eo = iu * row_stride + static_cast<int64_t>(id) * RT + rr;
     V2* ye = y + eo;
+    const T dgt = static_cast<T>(diag_value(S, S.up_mask[iu], d, iu, id));
 #pragma unroll
     for (int v = 0; v < VPL; ++v) {
       const V2 prev = ld_stream(&ye[v]);
-      const T ur = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
-      const T ui = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
+      const V2 q = x[eo + v];
       V2 o;
-      o.x = prev.x + ur;
-      o.y = prev.y + ui;
+      o.x = hv_finish(prev.x, ar[v], dgt, q.x, sxv[v]);
+      o.y = hv_finish(prev.y, ai[v], dgt, q.y, sxv[v]);
       st_stream(&ye[v], o);
-      if (dotp) {
-        const V2 q = x[eo + v];
+      if (dotp) {  // <xs x | y> partials of the finished y
         const double qr = sxv[v] * static_cast<double>(q.x), qi = sxv[v] * static_cast<double>(q.y);
-        dre[v] += qr * static_cast<double>(ur) + qi * static_cast<double>(ui);
-        dimg[v] += qr * static_cast<double>(ui) - qi * static_cast<double>(ur);
+        dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+        dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
       }
     }
   }
@@ -504,8 +514,8 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_up(const SectorView S, const ty
 // the rounds of a step then read only LDS.  Each step begins with one wait for the copies and
 // one barrier.  A target's hops are added in increasing source order (steps in order, rows in
 // order), the order of the CSR lists, so every H*V kernel gives the same bits.
-// The down pass writes y = diag x + down hops; the up pass adds its sum to y and takes the
-// <xs x | y> partials of the finished y.
+// The down pass writes y = part(down hops); the up pass adds part(up hops) and part(diag x) to it
+// (in the order of hv_finish) and takes the <xs x | y> partials of the finished y.
 
 typedef __attribute__((address_space(3))) void lds_void_t;
 typedef __attribute__((address_space(4))) const uint32_t const_u32_t;
@@ -543,6 +553,7 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
   __shared__ uint32_t s_e1[kStepWaves * kEw];
   __shared__ uint4 s_c[kStepCoefSlots];  // line-signed coefficients [parity * 256 + bond], 16 bytes apart
   __shared__ int s_tile;
+  __shared__ double s_xs[RB];  // xs of the launch's vectors (1 without xs)
   const int D = UP ? S.Dup : S.Ddn;
   const int32_t* __restrict__ blkp = UP ? S.st_blk_up : S.st_blk_dn;
   // step descriptors are scalar loads (load_const4), so that taking one never waits for a copy
@@ -569,6 +580,7 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
   const int per = (ntiles + kXcd - 1) / kXcd;
   const int t0 = min(ntiles, xcd * per), nmine = min(ntiles, t0 + per) - t0;
   double run_re = 0.0, run_im = 0.0;  // threads < RB: this workgroup's <xs x | y> of vector r0 + tid
+  if (tid < RB) s_xs[tid] = xs ? xs[r0 + tid] : 1.0;  // ordered before use by the first tile's barriers
   int cur_line = -1;
   for (int local = next_tile(ctr, xcd, &s_tile); local < nmine; local = next_tile(ctr, xcd, &s_tile)) {
     const int tile = __builtin_amdgcn_readfirstlane(t0 + local);
@@ -594,7 +606,6 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
     }
     const V2* __restrict__ xl = x + line * lstride + r0;  // element p of the line at xl + p * pstride
     const int te = blk * kStepBlock + el;                 // this thread's target of round 0
-    const V2* __restrict__ xt = xl + static_cast<int64_t>(te) * pstride + sub * VPL;
     T ar[NR][VPL], ai[NR][VPL];
 #pragma unroll
     for (int j = 0; j < NR; ++j)
@@ -655,39 +666,6 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
       issue(dcur, s_x0, s_e0);
     }
     if (st0 + 1 < st1) dnext = load_const4(desc, static_cast<int64_t>(st0 + 1) * kStepWaves + w);
-    // the down pass starts from diag * x: all loads first (after the first step's copies, so
-    // that their latencies overlap each other and the previous tile's stores), then the products
-    if (!UP) {
-      uint32_t dm[NR];
-#pragma unroll
-      for (int j = 0; j < NR; ++j) {
-        const int t = te + j * kStepRound;
-        dm[j] = 0u;
-        if (t < D) {
-          dm[j] = S.dn_mask[t];
-          const V2* xe = xt + static_cast<int64_t>(j * kStepRound) * pstride;
-#pragma unroll
-          for (int v = 0; v < VPL; ++v) {
-            const V2 q = v < H ? xe[hx + v] : xe[(H - hx) + (v - H)];
-            ar[j][v] = q.x;
-            ai[j][v] = q.y;
-          }
-        }
-      }
-      const uint32_t um = S.up_mask[line];
-#pragma unroll
-      for (int j = 0; j < NR; ++j) {
-        const int t = te + j * kStepRound;
-        if (t < D) {
-          const T dg = static_cast<T>(diag_value(S, um, dm[j], line, t));
-#pragma unroll
-          for (int v = 0; v < VPL; ++v) {
-            ar[j][v] = dg * ar[j][v];
-            ai[j][v] = dg * ai[j][v];
-          }
-        }
-      }
-    }
     // step st with buffers P (compile-time): wait for its copies, start the next step's copies,
     // then the rounds
     auto step = [&](auto pc, const int st) {
@@ -738,52 +716,71 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
       step(std::integral_constant<int, 0>{}, st);
       if (st + 1 < st1) step(std::integral_constant<int, 1>{}, st + 1);
     }
-    // epilogue: y (down: write; up: add), and the <xs x | y> partials of this pass's part.  The
-    // opaque copy of the lane's offset keeps the compiler from holding the diagonal loads'
-    // addresses live across the steps.
+    // epilogue.  Down: y = part(S_down).  Up: y = (y + part(S_up)) + part(diag * x), and the
+    // <xs x | y> partials of the finished y.  The opaque copy of the lane's offset keeps the
+    // compiler from holding addresses live across the steps.
     int64_t eoff = static_cast<int64_t>(te) * pstride + sub * VPL;
     asm volatile("" : "+v"(eoff));
     const V2* xta = xl + eoff + hx;        // slots [0, H)
     const V2* xtb = xl + eoff + (H - hx);  // slots [H, VPL)
     V2* yta = y + (xta - x);
     V2* ytb = y + (xtb - x);
-    double dre[VPL], dimg[VPL], sxv[VPL];
+    // xs of slot v (vector v ^ hx of the lane's piece), read from LDS where used
+    const double* sxl = s_xs + sub * VPL;
+    double dre[VPL], dimg[VPL];
 #pragma unroll
-    for (int v = 0; v < VPL; ++v) {
-      dre[v] = dimg[v] = 0.0;
-      sxv[v] = xs ? xs[r0 + sub * VPL + (v ^ hx)] : 1.0;
-    }
+    for (int v = 0; v < VPL; ++v) dre[v] = dimg[v] = 0.0;
+    if constexpr (UP) {
+      const uint32_t dmask = lmask[line];  // the line's down string
+      const int nd_line = S.ndd[line];
 #pragma unroll
-    for (int j = 0; j < NR; ++j) {
-      if (te + j * kStepRound >= D) continue;
-      const int64_t jo = static_cast<int64_t>(j * kStepRound) * pstride;
+      for (int j = 0; j < NR; ++j) {
+        const int t = te + j * kStepRound;
+        if (t >= D) continue;
+        const int64_t jo = static_cast<int64_t>(j * kStepRound) * pstride;
+        V2 py[VPL], px[VPL];
 #pragma unroll
-      for (int v = 0; v < VPL; ++v) {
-        const V2* xp = v < H ? xta + v : xtb + (v - H);
-        V2* yp = v < H ? yta + v : ytb + (v - H);
-        const double sx = sxv[v];
-        T a_r = ar[j][v], a_i = ai[j][v];
-        asm volatile("" : "+v"(a_r), "+v"(a_i));  // convert round by round (not all rounds up front)
-        const T ur = static_cast<T>(static_cast<double>(a_r) * sx);
-        const T ui = static_cast<T>(static_cast<double>(a_i) * sx);
-        V2 o;
-        if (UP) {
-          const V2 prev = yp[jo];
-          o.x = prev.x + ur;
-          o.y = prev.y + ui;
-        } else {
-          o.x = ur;
-          o.y = ui;
+        for (int v = 0; v < VPL; ++v) {
+          py[v] = v < H ? yta[jo + v] : ytb[jo + (v - H)];
+          px[v] = v < H ? xta[jo + v] : xtb[jo + (v - H)];
         }
-        yp[jo] = o;
-        if (dotp) {  // the column pass: <xs x | y> of the finished y (the row pass takes no partials)
-          const V2 q = xp[jo];
-          const double qr = sx * static_cast<double>(q.x), qi = sx * static_cast<double>(q.y);
-          dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
-          dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+        const T dgt = static_cast<T>(diag_from(S, S.up_mask[t], dmask, S.planes[t], S.nuu[t], nd_line));
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          V2* yp = v < H ? yta + v : ytb + (v - H);
+          const double sx = sxl[v ^ hx];
+          T a_r = ar[j][v], a_i = ai[j][v];
+          asm volatile("" : "+v"(a_r), "+v"(a_i));  // convert round by round (not all rounds up front)
+          const V2 q = px[v];
+          V2 o;
+          o.x = hv_finish(py[v].x, a_r, dgt, q.x, sx);
+          o.y = hv_finish(py[v].y, a_i, dgt, q.y, sx);
+          yp[jo] = o;
+          if (dotp) {
+            const double qr = sx * static_cast<double>(q.x), qi = sx * static_cast<double>(q.y);
+            dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+            dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+          }
         }
+        asm volatile("" ::: "memory");  // keep each round's y traffic together (bounds live registers)
       }
-      asm volatile("" ::: "memory");  // keep each round's y traffic together (bounds live registers)
+    } else {
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        if (te + j * kStepRound >= D) continue;
+        const int64_t jo = static_cast<int64_t>(j * kStepRound) * pstride;
+#pragma unroll
+        for (int v = 0; v < VPL; ++v) {
+          V2* yp = v < H ? yta + v : ytb + (v - H);
+          T a_r = ar[j][v], a_i = ai[j][v];
+          asm volatile("" : "+v"(a_r), "+v"(a_i));
+          V2 o;
+          o.x = hv_part(a_r, sxl[v ^ hx]);
+          o.y = hv_part(a_i, sxl[v ^ hx]);
+          yp[jo] = o;
+        }
+        asm volatile("" ::: "memory");
+      }
     }
     if (dotp) {
       // slots back to vector order, then lanes of one wavefront with the same sub hold partials
@@ -1049,13 +1046,15 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
                "D_dn * R must stay below 2^24 for the column pass's 24-bit source offsets");
   int32_t* ctr = Sx->tile_counters(st);
   record_pass_event(0, st);
+  // the row pass writes part(S_down); the column pass adds the rest and takes the <xs x | y>
+  // partials of the finished y, reading x at its targets for the diagonal anyway
   hipLaunchKernelGGL((k_hv_dn<T, RB, REAL>), dim3(p.grid_dn), dim3(kHvBlock), lds_dn, st, S, xp, yp, RT, r0, xs,
-                     dotp, ctr, p.nchunk, p.ntiles_dn);
+                     ctr, p.nchunk, p.ntiles_dn);
   record_pass_event(1, st);
   hipLaunchKernelGGL((k_hv_up<T, RB, REAL>), dim3(p.grid_up), dim3(kHvBlock), lds_up, st, S, xp, yp, RT, r0, xs,
-                     dotp, static_cast<int64_t>(p.grid_dn), ctr + 16, p.wlog, p.tps, p.ntiles_up);
+                     dotp, int64_t{0}, ctr + 16, p.wlog, p.tps, p.ntiles_up);
   record_pass_event(2, st);
-  return p.grid_dn + p.grid_up;
+  return p.grid_up;
 }
 
 template <typename T, bool REAL>

@@ -542,11 +542,14 @@ def test_graph_path_identical_to_eager(kind, dtype):
     checked.  The single-pass sector and block applies assign work statically: bit-identical.  The
     two-pass sector apply takes tiles from an atomic queue, so its per-block dot partials are summed
     in a run-dependent order (eager runs differ from each other the same way): early alpha/beta to
-    1e-12 and lnZ (converged rule) to 1e-9."""
+    1e-12 and lnZ (converged rule) to 1e-9 in complex128.  In complex64 an alpha that differs in its
+    last bit now and then moves the rounding of one single-precision element of the next vector
+    (one part in 1e7 of that element), which the recurrence carries forward: 1e-8 and 1e-8."""
     say("graph vs eager", kind, dtype)
     op = _graph_case(kind)
     kw = dict(n_random=20, lanczos_steps=30, seed=3, start=L.FTLM_START_PHASE, dtype=dtype)
     exact = kind != "sector_two_pass"
+    tol_ab, tol_lnz = (1e-12, 1e-9) if dtype == L.FTLM_C128 else (1e-8, 1e-8)
     ref = api.ftlm_ritz([op], api.FtlmParams(path=L.FTLM_PATH_PER_OP, **kw), keep_tridiagonals=True)
     betas = np.array([0.5, 2.0])
     pc = dict(kw, quad=L.FTLM_QUAD_CONVERGED)
@@ -560,10 +563,10 @@ def test_graph_path_identical_to_eager(kind, dtype):
                 assert np.array_equal(got.alpha, ref.alpha) and np.array_equal(got.beta, ref.beta), rep
                 assert np.array_equal(got.ritz, ref.ritz) and np.array_equal(got.weight, ref.weight), rep
             else:
-                assert np.allclose(got.alpha[..., :10], ref.alpha[..., :10], rtol=1e-12, atol=1e-12), rep
-                assert np.allclose(got.beta[..., :10], ref.beta[..., :10], rtol=1e-12, atol=1e-12), rep
+                assert np.allclose(got.alpha[..., :10], ref.alpha[..., :10], rtol=tol_ab, atol=tol_ab), rep
+                assert np.allclose(got.beta[..., :10], ref.beta[..., :10], rtol=tol_ab, atol=tol_ab), rep
         lg = api.ftlm_log_partition_complex(op, betas, api.FtlmParams(path=L.FTLM_PATH_GRAPH, **pc))
-        assert np.max(np.abs(lg - ref_lnz)) <= 1e-9 * np.max(np.abs(ref_lnz)), (rep, lg, ref_lnz)
+        assert np.max(np.abs(lg - ref_lnz)) <= tol_lnz * np.max(np.abs(ref_lnz)), (rep, lg, ref_lnz)
 
 
 def test_krylov_session_graph_steps():
