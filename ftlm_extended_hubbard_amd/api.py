@@ -130,7 +130,7 @@ class _Op:
         L.check(L.lib().ftlm_hip_apply(self._h, C.c_void_p(xp), C.c_void_p(yp), R, dtype, C.c_void_p(stream)))
 
     def apply_pass_times(self, x, y, R: int = 1, dtype: int = L.FTLM_C128, stream: int | None = None):
-        """apply_device with the row pass (diagonal + down bonds) and the column pass (up bonds)
+        """apply_device with the row pass (down bonds) and the column pass (up bonds, diagonal)
         timed by HIP events on the launch stream; synchronises and returns (ms_row, ms_col)."""
         xp = x.data_ptr() if hasattr(x, "data_ptr") else int(x)
         yp = y.data_ptr() if hasattr(y, "data_ptr") else int(y)

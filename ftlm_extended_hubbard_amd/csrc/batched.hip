@@ -11,7 +11,7 @@
 // (the reference's recurrence, no reorthogonalisation).  The three vectors live in LDS for
 // complex64 storage (3 * dim * 8 bytes, dim <= 6144) or in a per-workgroup slice of global memory
 // for complex128; H*v is the pull form over the sector's hop tables with FP64 accumulation
-// (diagonal + down bonds, then up bonds, as the full-sector kernels order them), and z, |w|^2 and
+// (diagonal + down bonds, then up bonds, as the one-pass sector kernel orders them), and z, |w|^2 and
 // every scalar are FP64.  alpha/beta are written step-major, [k][item], for the device quadrature.
 
 #include <algorithm>

@@ -20,11 +20,12 @@
 // one complex128 vector, 1.65 MB for 16 complex64 vectors), so an XCD's 4 MB L2 can hold the
 // few rows it is working on but not the ~34 other rows an up bond reaches.  Large sectors
 // therefore use two passes:
-//   k_hv_dn  diagonal + down bonds, tiles = (row, chunk of ids) in row-major order.  u is
-//            uniform per tile: the down-bond coefficients are pre-signed with the row's
-//            cross-spin parity in LDS.
-//   k_hv_up  up bonds, y += ... ; tiles = (stripe of W adjacent ids, chunk of rows) in
-//            stripe-major order; a stripe of all rows (D_up * W * R * s) is kept below ~3 MB.
+//   k_hv_dn  down bonds, tiles = (row, chunk of ids) in row-major order.  u is uniform per
+//            tile: the down-bond coefficients are pre-signed with the row's cross-spin parity
+//            in LDS.
+//   k_hv_up  up bonds and the diagonal, y += ... ; tiles = (stripe of W adjacent ids, chunk
+//            of rows) in stripe-major order; a stripe of all rows (D_up * W * R * s) is kept
+//            below ~3 MB.
 // Workgroups are dealt to the 8 XCDs round-robin (blockIdx % 8).  Each XCD owns a contiguous
 // share of the tiles and hands them out through an atomic counter, so the tiles in flight on an
 // XCD always form one contiguous window (a few rows, or one stripe) that its L2 holds, no

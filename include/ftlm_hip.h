@@ -113,7 +113,7 @@ int ftlm_hip_index_of(const ftlm_op_t* op, const uint64_t* up, const uint64_t* d
  * stream gets its own tile queues, for up to 64 streams per operator; FTLM_EUNSUPPORTED beyond). */
 int ftlm_hip_apply(const ftlm_op_t* op, const void* x, void* y, int R, int dtype, void* stream);
 
-/* ftlm_hip_apply with the row pass (diagonal + down bonds) and the column pass (up bonds) timed
+/* ftlm_hip_apply with the row pass (down bonds) and the column pass (up bonds, diagonal) timed
  * separately by HIP events recorded on `stream`; synchronises the stream.  No counterpart in the
  * reference: instrumentation for roofline reports.  Full-sector operators only; a single-pass
  * (small) sector reports its whole time as *ms_row. */
