@@ -588,7 +588,7 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
     const int line = tile / nblk;
     const int blk = tile - line * nblk;
     if (line != cur_line) {  // uniform; the first step's barrier orders these writes
-      const uint32_t om = lmask[line];
+      const uint32_t om = !UP && S.line_umask ? S.line_umask[line] : lmask[line];
       for (int b = tid; b < kStepCoefSlots / 2; b += kStepThreads) {
         V2 c;
         c.x = c.y = static_cast<T>(0);
@@ -605,7 +605,8 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
       }
       cur_line = line;
     }
-    const V2* __restrict__ xl = x + line * lstride + r0;  // element p of the line at xl + p * pstride
+    // element p of the line at xl + p * pstride
+    const V2* __restrict__ xl = x + (!UP && S.line_base ? S.line_base[line] * RT : line * lstride) + r0;
     const int te = blk * kStepBlock + el;                 // this thread's target of round 0
     T ar[NR][VPL], ai[NR][VPL];
 #pragma unroll
@@ -1100,6 +1101,33 @@ int hv_default_wlog(const ftlm_op* op, int R, int dtype, int rb, int* wlog_max) 
   while ((1 << (wm + 1)) <= kHvBlock / L) ++wm;
   if (wlog_max) *wlog_max = wm;
   return p.wlog;
+}
+
+bool sector_has_step_tables(const Sector* S) { return S->st_ent[0].p != nullptr && S->st_ent[1].p != nullptr; }
+
+void hv_step_rows_lines(const Sector* Sx, const void* x, void* y, int RT, int r0, int dtype, const double* xs,
+                        const int64_t* lbase, const uint32_t* lumask, int nlines, hipStream_t st) {
+  FTLM_REQUIRE(sector_has_step_tables(Sx), FTLM_EUNSUPPORTED, "no step tables for this sector");
+  const int rb = dtype == FTLM_C64 ? 16 : 8;
+  FTLM_REQUIRE(r0 >= 0 && r0 + rb <= RT, FTLM_EINVAL, "row step pass outside the vector block");
+  if (nlines <= 0) return;
+  SectorView S = Sx->view();
+  S.line_base = lbase;
+  S.line_umask = lumask;
+  const int nblk = (S.Ddn + kStepBlock - 1) / kStepBlock;
+  const int64_t nt = static_cast<int64_t>(nlines) * nblk;
+  FTLM_REQUIRE(nt < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many step tiles");
+  const int grid = kXcd * 32 * kStepWgPerCu;
+  int32_t* ctr = Sx->tile_counters(st);
+  if (dtype == FTLM_C64)
+    hipLaunchKernelGGL((k_hv_step<float, true, false>), dim3(grid), dim3(kStepThreads), 0, st, S,
+                       static_cast<const float2*>(x), static_cast<float2*>(y), RT, r0, xs, nullptr, int64_t{0}, ctr,
+                       nblk, static_cast<int>(nt));
+  else
+    hipLaunchKernelGGL((k_hv_step<double, true, false>), dim3(grid), dim3(kStepThreads), 0, st, S,
+                       static_cast<const double2*>(x), static_cast<double2*>(y), RT, r0, xs, nullptr, int64_t{0},
+                       ctr, nblk, static_cast<int>(nt));
+  check_launch("k_hv_step (lines)");
 }
 
 // Block apply over all RT vectors in chunks of RB (<= 8 for complex128, <= 16 for complex64).

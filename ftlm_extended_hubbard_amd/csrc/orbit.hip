@@ -665,8 +665,8 @@ __global__ __launch_bounds__(256) void k_kstack_apply(
     int64_t nrep, int nk, int rk, const uint16_t* __restrict__ mem, const uint16_t* __restrict__ memc,
     const int32_t* __restrict__ row_off, const uint2* __restrict__ ent, const double* __restrict__ diag,
     const double2* __restrict__ coef, int nbond,
-    const double2* __restrict__ chi, const typename Cx<T>::v2* __restrict__ x, typename Cx<T>::v2* __restrict__ y,
-    const double* __restrict__ xs, double2* __restrict__ dotp) {
+    const double2* __restrict__ chi, const uint8_t* __restrict__ acc, const typename Cx<T>::v2* __restrict__ x,
+    typename Cx<T>::v2* __restrict__ y, const double* __restrict__ xs, double2* __restrict__ dotp) {
   using V2 = typename Cx<T>::v2;
   constexpr int G = 8;
   const int W = nk * rk;
@@ -762,12 +762,18 @@ __global__ __launch_bounds__(256) void k_kstack_apply(
       }
       V2* dst = y + r * W + v0;
       const V2* own = x + r * W + v0;
+      const bool add = acc != nullptr && acc[r] != 0;  // the row's down hops are in y already
 #pragma unroll
       for (int v = 0; v < VPL; ++v) {
         V2 o;
         if ((mrow >> jb[v]) & 1u) {
           o.x = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
           o.y = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
+          if (add) {
+            const V2 p = dst[v];
+            o.x = p.x + o.x;
+            o.y = p.y + o.y;
+          }
         } else {
           o.x = o.y = static_cast<T>(0);
         }
@@ -981,6 +987,26 @@ void cub_call(F&& f, hipStream_t st) {
   DevBuf<uint8_t> tmp(std::max<size_t>(bytes, 1));
   FTLM_HIP_CHECK(f(tmp.p, bytes, st));
   FTLM_HIP_CHECK(hipStreamSynchronize(st));
+}
+
+// Terms a momentum stack's split apply leaves to k_kstack_apply: all terms of orbits off the
+// trivial lines, and the diagonal and up-hop terms of those on them (their down hops go to the
+// step row pass).  keep[e] per term, cnt[g] = kept terms of row g.
+__global__ void k_split_keep(const int32_t* row_off, const uint2* ent, int64_t nrep, const uint8_t* triv,
+                             const uint32_t* dnbits, uint8_t* keep, int32_t* cnt) {
+  const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nrep) return;
+  const int32_t e0 = row_off[g], e1 = row_off[g + 1];
+  int32_t c = 0;
+  for (int32_t e = e0; e < e1; ++e) {
+    const uint2 en = ent[e];
+    const uint32_t b = en.y & kBondMask;
+    const bool down = (en.x & kDiagFlag) == 0 && ((dnbits[(b >> 5) & 15u] >> (b & 31u)) & 1u) != 0;
+    const bool k = !(triv[g] && down);
+    keep[e] = k ? 1 : 0;
+    c += k ? 1 : 0;
+  }
+  cnt[g] = c;
 }
 
 }  // namespace
@@ -1405,10 +1431,120 @@ struct KStack : ftlm_op {
   DevBuf<uint16_t> mem;   // [nrep]
   DevBuf<uint16_t> memc;  // [nnz] mem of every term's column orbit
   DevBuf<double2> chi;    // [nk][16] Bloch phases of each block
+  // Split apply (kstack_split_build).  A trivial line is the run of orbits whose representative's
+  // up string u has no translation fixing it: it holds (u, d) for every down string d, in the
+  // sector's order, and its down hops are the full sector's row-u down hops (identity
+  // translation, unit norms, the same Jordan-Wigner signs) for every block.  Those go through
+  // the LDS-staged step row pass of the full sector; k_kstack_apply adds the remaining terms.
+  std::unique_ptr<Sector> sec;  // the full sector in increasing string order, for its step tables
+  int nlines = 0;
+  DevBuf<int64_t> lbase;     // [nlines] first orbit of each trivial line
+  DevBuf<uint32_t> lumask;   // [nlines] its up string
+  DevBuf<uint8_t> triv;      // [nrep] 1 on a trivial line
+  DevBuf<int32_t> row_off2;  // [nrep + 1] the remaining terms
+  DevBuf<uint2> ent2;
+  DevBuf<uint16_t> memc2;
 };
 
 const void* kblock_graph(const ftlm_op* op) {
   return op != nullptr && op->kind == kKBlock ? static_cast<const KBlock*>(op)->G.get() : nullptr;
+}
+
+// Whether momentum stacks split their applies (FTLM_KS_SPLIT=0 keeps the one-kernel apply).
+bool kstack_split_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("FTLM_KS_SPLIT");
+    return v == nullptr || std::atoi(v) != 0;
+  }();
+  return on;
+}
+
+// Builds the split apply's tables (see KStack) when the graph has real bond coefficients and the
+// full sector has step tables; otherwise leaves the stack on the one-kernel apply.
+void kstack_split_build(KStack* S) {
+  const OrbitGraph& G = *S->G;
+  if (!kstack_split_enabled() || !G.real_h || G.nrep == 0 || G.nnz == 0) return;
+  const int n = G.lx * G.ly;
+  std::unique_ptr<Sector> sec =
+      sector_create_numeric(n, G.n_up, G.n_dn, G.hops.data(), static_cast<int>(G.hops.size()), G.pairs.data(),
+                            static_cast<int>(G.pairs.size()), G.U, G.V);
+  const Sector* Sx = sec.get();
+  if (!sector_has_step_tables(Sx) || !Sx->real_h) return;
+  // trivial lines, from the representatives (pack order: up string, then down string)
+  const int64_t nr = G.nrep;
+  std::vector<uint32_t> pack(static_cast<size_t>(nr));
+  FTLM_HIP_CHECK(hipMemcpy(pack.data(), G.pack.p, sizeof(uint32_t) * pack.size(), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> dn(static_cast<size_t>(Sx->Ddn));
+  FTLM_HIP_CHECK(hipMemcpy(dn.data(), Sx->dn_mask.p, sizeof(uint32_t) * dn.size(), hipMemcpyDeviceToHost));
+  std::vector<int64_t> lbase;
+  std::vector<uint32_t> lumask;
+  std::vector<uint8_t> triv(static_cast<size_t>(nr), 0);
+  for (int64_t a = 0; a < nr;) {
+    const uint32_t u = pack[static_cast<size_t>(a)] >> 16;
+    int64_t b = a;
+    while (b < nr && (pack[static_cast<size_t>(b)] >> 16) == u) ++b;
+    bool full = b - a == Sx->Ddn;
+    for (int64_t i = a; full && i < b; ++i) full = (pack[static_cast<size_t>(i)] & 0xFFFFu) == dn[static_cast<size_t>(i - a)];
+    if (full) {
+      lbase.push_back(a);
+      lumask.push_back(u);
+      std::fill(triv.begin() + a, triv.begin() + b, 1);
+    }
+    a = b;
+  }
+  if (lbase.empty()) return;
+  // down-spin bond ids: the graph's merged bond list (first occurrence of (from, to, spin), self
+  // hops skipped), as build_orbit_graph forms it
+  std::vector<uint32_t> dnbits(16, 0u);
+  {
+    std::map<std::tuple<int, int, int>, int> seen;
+    int nb = 0;
+    for (const ftlm_hop_t& e : G.hops) {
+      if (e.from == e.to) continue;
+      const auto key = std::make_tuple(e.from, e.to, e.spin);
+      if (seen.count(key)) continue;
+      seen.emplace(key, nb);
+      if (e.spin == 1) dnbits[static_cast<size_t>(nb >> 5)] |= 1u << (nb & 31);
+      ++nb;
+    }
+    FTLM_REQUIRE(nb == G.nbond, FTLM_EHIP, "momentum stack: bond list mismatch");
+  }
+  S->sec = std::move(sec);
+  S->nlines = static_cast<int>(lbase.size());
+  S->lbase.alloc(lbase.size());
+  S->lbase.upload(lbase);
+  S->lumask.alloc(lumask.size());
+  S->lumask.upload(lumask);
+  S->triv.alloc(triv.size());
+  S->triv.upload(triv);
+  DevBuf<uint32_t> ddn(16);
+  ddn.upload(dnbits);
+  DevBuf<uint8_t> keep(static_cast<size_t>(G.nnz));
+  DevBuf<int32_t> cnt(static_cast<size_t>(nr));
+  const hipStream_t st = nullptr;
+  hipLaunchKernelGGL(k_split_keep, dim3(grid_for(nr, 256)), dim3(256), 0, st, G.row_off.p, G.ent.p, nr, S->triv.p,
+                     ddn.p, keep.p, cnt.p);
+  check_launch("k_split_keep");
+  std::vector<int32_t> c(static_cast<size_t>(nr)), off(static_cast<size_t>(nr) + 1);
+  FTLM_HIP_CHECK(hipMemcpy(c.data(), cnt.p, sizeof(int32_t) * c.size(), hipMemcpyDeviceToHost));
+  off[0] = 0;
+  for (int64_t g = 0; g < nr; ++g) off[static_cast<size_t>(g) + 1] = off[static_cast<size_t>(g)] + c[static_cast<size_t>(g)];
+  const int64_t n2 = off[static_cast<size_t>(nr)];
+  S->row_off2.alloc(off.size());
+  S->row_off2.upload(off);
+  S->ent2.alloc(static_cast<size_t>(std::max<int64_t>(n2, 1)));
+  S->memc2.alloc(static_cast<size_t>(std::max<int64_t>(n2, 1)));
+  DevBuf<int32_t> nsel(1);
+  const int nn = static_cast<int>(G.nnz);
+  cub_call([&](void* t, size_t& b, hipStream_t s2) {
+    return hipcub::DeviceSelect::Flagged(t, b, G.ent.p, keep.p, S->ent2.p, nsel.p, nn, s2);
+  }, st);
+  cub_call([&](void* t, size_t& b, hipStream_t s2) {
+    return hipcub::DeviceSelect::Flagged(t, b, S->memc.p, keep.p, S->memc2.p, nsel.p, nn, s2);
+  }, st);
+  int32_t got = 0;
+  FTLM_HIP_CHECK(hipMemcpy(&got, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+  FTLM_REQUIRE(got == n2, FTLM_EHIP, "momentum stack: split term count mismatch");
 }
 
 std::unique_ptr<ftlm_op> kstack_build(const std::vector<const ftlm_op*>& blocks) {
@@ -1450,6 +1586,7 @@ std::unique_ptr<ftlm_op> kstack_build(const std::vector<const ftlm_op*>& blocks)
     check_launch("k_kstack_memc");
   }
   FTLM_HIP_CHECK(hipDeviceSynchronize());
+  kstack_split_build(S.get());
   return S;
 }
 
@@ -1473,7 +1610,20 @@ int hv_apply_kstack(const ftlm_op* op, const void* x, void* y, int R, int dtype,
   const int grid = static_cast<int>(std::min<int64_t>(gmax, (S->dim + rpb - 1) / rpb));
   const float2* xf = static_cast<const float2*>(x);
   float2* yf = static_cast<float2*>(y);
-#define FTLM_KSTACK_ARGS S->dim, S->nk, rk, S->mem.p, S->memc.p, G.row_off.p, G.ent.p, G.diag.p, G.coef.p, G.nbond, S->chi.p
+  // split apply: the trivial lines' down hops by the step row pass (128-byte chunks of the
+  // slots), then the remaining terms, added on those lines
+  const size_t esz = dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2);
+  const bool split = S->sec != nullptr && (static_cast<size_t>(R) * esz) % 128 == 0;
+  if (split) {
+    const int rb = static_cast<int>(128 / esz);
+    for (int r0 = 0; r0 < R; r0 += rb)
+      hv_step_rows_lines(S->sec.get(), x, y, R, r0, dtype, xs, S->lbase.p, S->lumask.p, S->nlines, st);
+  }
+  const int32_t* roff = split ? S->row_off2.p : G.row_off.p;
+  const uint2* ents = split ? S->ent2.p : G.ent.p;
+  const uint16_t* mcs = split ? S->memc2.p : S->memc.p;
+  const uint8_t* accp = split ? S->triv.p : nullptr;
+#define FTLM_KSTACK_ARGS S->dim, S->nk, rk, S->mem.p, mcs, roff, ents, G.diag.p, G.coef.p, G.nbond, S->chi.p, accp
   const int lanes = R / vpl;
   const bool coop = lanes >= 8 && (lanes & (lanes - 1)) == 0;
 #define FTLM_KSTACK_GO(TT, VV, OJ, XP, YP)                                                                          \

@@ -94,9 +94,39 @@ __device__ __forceinline__ uint32_t lex_unrank(int64_t r, int n, int k) {
   return m;
 }
 
-__global__ void k_unrank(uint32_t* out, int D, int n, int k) {
+// Rank of a k-subset among all k-subsets of [0, n) in increasing order of their masks (the
+// colexicographic order of the position sets): sum_i C(p_i, i), positions p_1 < p_2 < ...
+__device__ __forceinline__ int64_t num_rank(uint32_t m) {
+  int64_t r = 0;
+  int i = 1;
+  while (m) {
+    const int p = __ffs(m) - 1;
+    m &= m - 1;
+    r += c_binom[p][i];
+    ++i;
+  }
+  return r;
+}
+
+// Inverse of num_rank: the largest position first (the largest p with C(p, k) <= r).
+__device__ __forceinline__ uint32_t num_unrank(int64_t r, int n, int k) {
+  uint32_t m = 0;
+  for (int p = n - 1; p >= 0 && k > 0; --p) {
+    const int64_t c = c_binom[p][k];
+    if (c <= r) {
+      m |= 1u << p;
+      r -= c;
+      --k;
+    }
+  }
+  return m;
+}
+
+// Strings of a spin in rank order: the reference's lexicographic order, or (numeric) increasing
+// masks for the internal sectors of momentum stacks.
+__global__ void k_unrank(uint32_t* out, int D, int n, int k, int numeric) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < D) out[i] = lex_unrank(i, n, k);
+  if (i < D) out[i] = numeric ? num_unrank(i, n, k) : lex_unrank(i, n, k);
 }
 
 struct BondDev {
@@ -116,7 +146,7 @@ __global__ void k_hop_count(const uint32_t* masks, int D, const BondDev* bonds, 
 }
 
 __global__ void k_hop_fill(const uint32_t* masks, int D, const BondDev* bonds, int nb, const int32_t* off,
-                           uint32_t* ent, int n, int k) {
+                           uint32_t* ent, int n, int k, int numeric) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= D) return;
   const uint32_t m = masks[i];
@@ -125,7 +155,7 @@ __global__ void k_hop_fill(const uint32_t* masks, int D, const BondDev* bonds, i
     const BondDev bd = bonds[b];
     if (!((m >> bd.to) & 1u) || ((m >> bd.from) & 1u)) continue;
     const uint32_t src = m ^ (1u << bd.from) ^ (1u << bd.to);
-    const uint32_t t = static_cast<uint32_t>(lex_rank(src, n, k));
+    const uint32_t t = static_cast<uint32_t>(numeric ? num_rank(src) : lex_rank(src, n, k));
     const uint32_t par = __popc(m & bd.own) & 1u;
     const uint32_t e = t | (static_cast<uint32_t>(b) << kTargetBits) | (par << kParityBit);
     // keep the list sorted by source rank (every H*V kernel adds a target's hops in this order)
@@ -292,6 +322,8 @@ SectorView Sector::view() const {
   v.st_blk_dn = st_blk[1].p;
   v.st_desc_dn = st_desc[1].p;
   v.st_ent_dn = st_ent[1].p;
+  v.line_base = nullptr;
+  v.line_umask = nullptr;
   return v;
 }
 
@@ -403,8 +435,9 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
   // strings (K1: device unrank)
   S->up_mask.alloc(static_cast<size_t>(Du));
   S->dn_mask.alloc(static_cast<size_t>(Dd));
-  hipLaunchKernelGGL(k_unrank, dim3(grid_for(Du, 256)), dim3(256), 0, 0, S->up_mask.p, S->Dup, n, S->n_up);
-  hipLaunchKernelGGL(k_unrank, dim3(grid_for(Dd, 256)), dim3(256), 0, 0, S->dn_mask.p, S->Ddn, n, S->n_dn);
+  const int numeric = S->numeric_order ? 1 : 0;
+  hipLaunchKernelGGL(k_unrank, dim3(grid_for(Du, 256)), dim3(256), 0, 0, S->up_mask.p, S->Dup, n, S->n_up, numeric);
+  hipLaunchKernelGGL(k_unrank, dim3(grid_for(Dd, 256)), dim3(256), 0, 0, S->dn_mask.p, S->Ddn, n, S->n_dn, numeric);
   check_launch("k_unrank");
 
   // bonds
@@ -455,7 +488,7 @@ void build_sector(Sector* S, const ftlm_hop_t* hops, int n_hops, const ftlm_pair
     S->off[sp].upload(ho);
     S->ent[sp].alloc(static_cast<size_t>(std::max<int64_t>(acc, 1)));
     hipLaunchKernelGGL(k_hop_fill, dim3(grid_for(D, 256)), dim3(256), 0, 0, masks, D, dbonds.p, nb, S->off[sp].p,
-                       S->ent[sp].p, n, k);
+                       S->ent[sp].p, n, k, numeric);
     check_launch("k_hop_fill");
     int mc = 0;
     for (int i = 0; i < D; ++i) mc = std::max(mc, hc[i]);
@@ -563,6 +596,23 @@ void geometry(const ftlm_params_t& p, std::vector<ftlm_hop_t>* hops, std::vector
 }
 
 }  // namespace
+
+std::unique_ptr<Sector> sector_create_numeric(int n_sites, int n_up, int n_dn, const ftlm_hop_t* hops, int n_hops,
+                                              const ftlm_pair_t* pairs, int n_pairs, double U, double V) {
+  FTLM_REQUIRE(n_sites >= 1 && n_sites <= kMaxSites && n_up >= 0 && n_dn >= 0 && n_up <= n_sites &&
+                   n_dn <= n_sites,
+               FTLM_EINVAL, "sector out of range");
+  auto S = std::make_unique<Sector>();
+  S->kind = kSector;
+  S->n_sites = n_sites;
+  S->n_up = n_up;
+  S->n_dn = n_dn;
+  S->U = U;
+  S->V = V;
+  S->numeric_order = true;
+  build_sector(S.get(), hops, n_hops, pairs, n_pairs);
+  return S;
+}
 
 }  // namespace ftlm_hip
 
