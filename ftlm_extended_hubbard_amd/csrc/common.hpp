@@ -328,16 +328,16 @@ void sector_rank_packs(const Sector* S, const uint32_t* d_packs, int64_t count, 
 // Grid of a single-pass apply (small sector or momentum block) with the default shape, and the
 // largest grid a shape may ask for.
 int hv_default_grid(const ftlm_op* op, int* grid_max);
-// Row step pass (down bonds only: y = part(S_down) of the two-pass order) over nlines caller-described
-// lines of D_dn elements each: line l's element p at x + (lbase[l] + p) * RT, with up string
-// lumask[l]; vectors [r0, r0 + 128 bytes / element) of RT.  Needs the sector's step tables
-// (sector_has_step_tables) and real bond coefficients (hv.hip).
+// Row pass (down bonds only: y = part(S_down) of the two-pass order) over nlines caller-described
+// lines of D_dn elements each: line l's element p at x + (lbase[l] + p) * R, with up string
+// lumask[l], for all R vectors (the step passes for 128-byte chunks, the L2-gather pass
+// otherwise, which add in the same order).  Needs D_dn >= 128 and real bond coefficients (hv.hip).
 bool sector_has_step_tables(const Sector* S);
+void hv_rows_lines(const Sector* S, const void* x, void* y, int R, int dtype, const double* xs, const int64_t* lbase,
+                   const uint32_t* lumask, int nlines, hipStream_t st);
 // A sector whose strings are ranked in increasing mask order (sector.hip; see Sector::numeric_order).
 std::unique_ptr<Sector> sector_create_numeric(int n_sites, int n_up, int n_dn, const ftlm_hop_t* hops, int n_hops,
                                               const ftlm_pair_t* pairs, int n_pairs, double U, double V);
-void hv_step_rows_lines(const Sector* S, const void* x, void* y, int RT, int r0, int dtype, const double* xs,
-                        const int64_t* lbase, const uint32_t* lumask, int nlines, hipStream_t st);
 // Default stripe-width log2 of a two-pass sector's column pass for a launch of rb vectors (-1 for
 // a single-pass sector), and the largest one the tile allows.
 int hv_default_wlog(const ftlm_op* op, int R, int dtype, int rb, int* wlog_max);

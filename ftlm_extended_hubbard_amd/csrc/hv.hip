@@ -275,7 +275,9 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
     const int tile = __builtin_amdgcn_readfirstlane(t0 + local);
     const int iu = tile / nchunk;
     const int chunk = tile - iu * nchunk;
-    const uint32_t u = S.up_mask[iu];
+    const uint32_t u = S.line_umask ? S.line_umask[iu] : S.up_mask[iu];
+    // the row's first element (caller-described lines: line_base[iu])
+    const int64_t rowoff = S.line_base ? S.line_base[iu] * RT : static_cast<int64_t>(iu) * row_stride;
     // uniform branches: the whole block works on one tile
     if (kStage && chunk != cur_chunk) {  // the chunk's entries: one 16-byte load per 4 strings and slot
       const uint4* src4 = reinterpret_cast<const uint4*>(S.entT_dn + chunk * EPB);
@@ -305,7 +307,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
     __syncthreads();
     const int id = chunk * EPB + el;
     if (id >= Ddn) continue;
-    const char* xb = reinterpret_cast<const char*>(x + static_cast<int64_t>(iu) * row_stride);  // uniform
+    const char* xb = reinterpret_cast<const char*>(x + rowoff);  // uniform
     T ar[VPL], ai[VPL];
 #pragma unroll
     for (int v = 0; v < VPL; ++v) ar[v] = ai[v] = static_cast<T>(0);
@@ -332,7 +334,7 @@ __global__ __launch_bounds__(kHvBlock) void k_hv_dn(const SectorView S, const ty
         axpy_regs<T, VPL, REAL>(ar, ai, src[j], c.x, c.y);
       }
     }
-    V2* ye = reinterpret_cast<V2*>(reinterpret_cast<char*>(y + static_cast<int64_t>(iu) * row_stride) +
+    V2* ye = reinterpret_cast<V2*>(reinterpret_cast<char*>(y + rowoff) +
                                    (__umul24(static_cast<uint32_t>(id), es) + lane_b));
 #pragma unroll
     for (int v = 0; v < VPL; ++v) {
@@ -1105,29 +1107,72 @@ int hv_default_wlog(const ftlm_op* op, int R, int dtype, int rb, int* wlog_max) 
 
 bool sector_has_step_tables(const Sector* S) { return S->st_ent[0].p != nullptr && S->st_ent[1].p != nullptr; }
 
-void hv_step_rows_lines(const Sector* Sx, const void* x, void* y, int RT, int r0, int dtype, const double* xs,
-                        const int64_t* lbase, const uint32_t* lumask, int nlines, hipStream_t st) {
-  FTLM_REQUIRE(sector_has_step_tables(Sx), FTLM_EUNSUPPORTED, "no step tables for this sector");
-  const int rb = dtype == FTLM_C64 ? 16 : 8;
-  FTLM_REQUIRE(r0 >= 0 && r0 + rb <= RT, FTLM_EINVAL, "row step pass outside the vector block");
+namespace {
+
+// One chunk of RB vectors of hv_rows_lines: the step row pass for 128-byte elements, the L2-gather
+// row pass otherwise (both add a target's down hops in the same order).
+template <typename T, int RB>
+void rows_lines_chunk(const Sector* Sx, const SectorView& S, const void* x, void* y, int RT, int r0, const double* xs,
+                      int nlines, hipStream_t st) {
+  using V2 = typename Cx<T>::v2;
+  using SP = Split<T, RB>;
+  int32_t* ctr = Sx->tile_counters(st);
+  const V2* xp = static_cast<const V2*>(x);
+  V2* yp = static_cast<V2*>(y);
+  if constexpr (RB * sizeof(V2) == 128) {
+    if (use_step(S)) {
+      const int nblk = (S.Ddn + kStepBlock - 1) / kStepBlock;
+      const int64_t nt = static_cast<int64_t>(nlines) * nblk;
+      FTLM_REQUIRE(nt < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many step tiles");
+      hipLaunchKernelGGL((k_hv_step<T, true, false>), dim3(kXcd * 32 * kStepWgPerCu), dim3(kStepThreads), 0, st, S,
+                         xp, yp, RT, r0, xs, nullptr, int64_t{0}, ctr, nblk, static_cast<int>(nt));
+      return;
+    }
+  }
+  const Plan p = make_plan(Sx, RT, sizeof(V2), SP::L, HvShape{});
+  FTLM_REQUIRE(p.two, FTLM_EUNSUPPORTED, "row pass over lines needs D_dn >= 128");
+  const bool stage = StageEntries<SP::EPB>::value;
+  const size_t lds_dn = stage ? static_cast<size_t>(std::max(S.maxcnt_dn, 1)) * SP::EPB * sizeof(uint32_t) : 16;
+  FTLM_REQUIRE(lds_dn <= 48 * 1024, FTLM_EUNSUPPORTED, "hop table of one H*V tile exceeds 48 KB of LDS");
+  const int64_t ntd = static_cast<int64_t>(nlines) * p.nchunk;
+  FTLM_REQUIRE(ntd < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many row tiles");
+  const int grid = kXcd * static_cast<int>(std::min<int64_t>(p.grid_dn / kXcd, (ntd + kXcd - 1) / kXcd));
+  hipLaunchKernelGGL((k_hv_dn<T, RB, true>), dim3(grid), dim3(kHvBlock), lds_dn, st, S, xp, yp, RT, r0, xs, ctr,
+                     p.nchunk, static_cast<int>(ntd));
+}
+
+template <typename T>
+void rows_lines_dispatch(const Sector* Sx, const SectorView& S, const void* x, void* y, int RT, int r0, int rb,
+                         const double* xs, int nlines, hipStream_t st) {
+  switch (rb) {
+    case 16: rows_lines_chunk<T, 16>(Sx, S, x, y, RT, r0, xs, nlines, st); break;
+    case 8: rows_lines_chunk<T, 8>(Sx, S, x, y, RT, r0, xs, nlines, st); break;
+    case 4: rows_lines_chunk<T, 4>(Sx, S, x, y, RT, r0, xs, nlines, st); break;
+    case 2: rows_lines_chunk<T, 2>(Sx, S, x, y, RT, r0, xs, nlines, st); break;
+    default: rows_lines_chunk<T, 1>(Sx, S, x, y, RT, r0, xs, nlines, st); break;
+  }
+}
+
+}  // namespace
+
+void hv_rows_lines(const Sector* Sx, const void* x, void* y, int R, int dtype, const double* xs, const int64_t* lbase,
+                   const uint32_t* lumask, int nlines, hipStream_t st) {
+  FTLM_REQUIRE(Sx->real_h, FTLM_EUNSUPPORTED, "row pass over lines needs real bond coefficients");
   if (nlines <= 0) return;
   SectorView S = Sx->view();
   S.line_base = lbase;
   S.line_umask = lumask;
-  const int nblk = (S.Ddn + kStepBlock - 1) / kStepBlock;
-  const int64_t nt = static_cast<int64_t>(nlines) * nblk;
-  FTLM_REQUIRE(nt < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many step tiles");
-  const int grid = kXcd * 32 * kStepWgPerCu;
-  int32_t* ctr = Sx->tile_counters(st);
-  if (dtype == FTLM_C64)
-    hipLaunchKernelGGL((k_hv_step<float, true, false>), dim3(grid), dim3(kStepThreads), 0, st, S,
-                       static_cast<const float2*>(x), static_cast<float2*>(y), RT, r0, xs, nullptr, int64_t{0}, ctr,
-                       nblk, static_cast<int>(nt));
-  else
-    hipLaunchKernelGGL((k_hv_step<double, true, false>), dim3(grid), dim3(kStepThreads), 0, st, S,
-                       static_cast<const double2*>(x), static_cast<double2*>(y), RT, r0, xs, nullptr, int64_t{0},
-                       ctr, nblk, static_cast<int>(nt));
-  check_launch("k_hv_step (lines)");
+  const int maxrb = dtype == FTLM_C64 ? 16 : 8;
+  for (int r0 = 0; r0 < R;) {
+    int rb = 1;
+    while (rb * 2 <= maxrb && r0 + rb * 2 <= R) rb *= 2;
+    if (dtype == FTLM_C64)
+      rows_lines_dispatch<float>(Sx, S, x, y, R, r0, rb, xs, nlines, st);
+    else
+      rows_lines_dispatch<double>(Sx, S, x, y, R, r0, rb, xs, nlines, st);
+    r0 += rb;
+  }
+  check_launch("H*V row pass over lines");
 }
 
 // Block apply over all RT vectors in chunks of RB (<= 8 for complex128, <= 16 for complex64).

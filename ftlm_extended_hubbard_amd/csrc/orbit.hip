@@ -417,8 +417,8 @@ template <typename T, int RB>
 __global__ __launch_bounds__(256) void k_kblock_apply(
     const int32_t* __restrict__ rows, int64_t dk, const int32_t* __restrict__ row_off, const uint2* __restrict__ ent,
     const int32_t* __restrict__ colk, const double* __restrict__ diag, const double2* __restrict__ coef, int nbond,
-    Phases ph, const typename Cx<T>::v2* __restrict__ x, typename Cx<T>::v2* __restrict__ y, int RT, int r0,
-    const double* __restrict__ xs, double2* __restrict__ dotp) {
+    Phases ph, const uint8_t* __restrict__ acc, const typename Cx<T>::v2* __restrict__ x,
+    typename Cx<T>::v2* __restrict__ y, int RT, int r0, const double* __restrict__ xs, double2* __restrict__ dotp) {
   using V2 = typename Cx<T>::v2;
   constexpr int G = KbGroup<RB>::value;
   __shared__ double2 s_coef[kMaxBondsLds];
@@ -474,12 +474,18 @@ __global__ __launch_bounds__(256) void k_kblock_apply(
     }
     V2* dst = y + r * RT + r0;
     const V2* own = x + r * RT + r0;
+    const bool add = acc != nullptr && acc[g] != 0;  // the row's down hops are in y already
 #pragma unroll
     for (int v = 0; v < RB; ++v) {
       const double sx = xs ? xs[r0 + v] : 1.0;
       V2 o;
       o.x = static_cast<T>(static_cast<double>(ar[v]) * sx);
       o.y = static_cast<T>(static_cast<double>(ai[v]) * sx);
+      if (add) {
+        const V2 p = dst[v];
+        o.x = p.x + o.x;
+        o.y = p.y + o.y;
+      }
       dst[v] = o;
       if (dotp) {
         const V2 q = own[v];
@@ -526,8 +532,8 @@ template <typename T, int RB, bool COOP>
 __global__ __launch_bounds__(256) void k_kblock_apply_split(
     const int32_t* __restrict__ rows, int64_t dk, const int32_t* __restrict__ row_off, const uint2* __restrict__ ent,
     const int32_t* __restrict__ colk, const double* __restrict__ diag, const double2* __restrict__ coef, int nbond,
-    Phases ph, const typename Cx<T>::v2* __restrict__ x, typename Cx<T>::v2* __restrict__ y, int RT, int r0,
-    const double* __restrict__ xs, double2* __restrict__ dotp) {
+    Phases ph, const uint8_t* __restrict__ acc, const typename Cx<T>::v2* __restrict__ x,
+    typename Cx<T>::v2* __restrict__ y, int RT, int r0, const double* __restrict__ xs, double2* __restrict__ dotp) {
   using V2 = typename Cx<T>::v2;
   constexpr int BYTES = RB * static_cast<int>(sizeof(V2));
   constexpr int L = BYTES >= 32 ? BYTES / 16 : 1;
@@ -610,11 +616,17 @@ __global__ __launch_bounds__(256) void k_kblock_apply_split(
     }
     V2* dst = y + r * RT + rr;
     const V2* own = x + r * RT + rr;
+    const bool add = acc != nullptr && acc[g] != 0;  // the row's down hops are in y already
 #pragma unroll
     for (int v = 0; v < VPL; ++v) {
       V2 o;
       o.x = static_cast<T>(static_cast<double>(ar[v]) * sxv[v]);
       o.y = static_cast<T>(static_cast<double>(ai[v]) * sxv[v]);
+      if (add) {
+        const V2 p = dst[v];
+        o.x = p.x + o.x;
+        o.y = p.y + o.y;
+      }
       dst[v] = o;
       if (dotp) {
         const V2 q = own[v];
@@ -989,9 +1001,14 @@ void cub_call(F&& f, hipStream_t st) {
   FTLM_HIP_CHECK(hipStreamSynchronize(st));
 }
 
-// Terms a momentum stack's split apply leaves to k_kstack_apply: all terms of orbits off the
-// trivial lines, and the diagonal and up-hop terms of those on them (their down hops go to the
-// step row pass).  keep[e] per term, cnt[g] = kept terms of row g.
+// Block row of each trivial line's first orbit (the line's orbits are rows, consecutive).
+__global__ void k_line_rows(const int64_t* lbase, int nlines, const int32_t* map, int64_t* out) {
+  const int l = blockIdx.x * blockDim.x + threadIdx.x;
+  if (l < nlines) out[l] = map[lbase[l]];
+}
+
+// Terms a split apply leaves to the momentum kernels: all terms of orbits off the trivial lines,
+// and the diagonal and up-hop terms of those on them (their down hops go to the sector row pass).  keep[e] per term, cnt[g] = kept terms of row g.
 __global__ void k_split_keep(const int32_t* row_off, const uint2* ent, int64_t nrep, const uint8_t* triv,
                              const uint32_t* dnbits, uint8_t* keep, int32_t* cnt) {
   const int64_t g = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1011,6 +1028,25 @@ __global__ void k_split_keep(const int32_t* row_off, const uint2* ent, int64_t n
 
 }  // namespace
 
+// Split applies of an orbit graph (orbit_split_build).  In the pack order an orbit's
+// representative (u, d) has the smallest up string of its orbit.  Where no translation fixes u, every
+// down string d occurs with it: those orbits form a trivial line of D_dn consecutive orbits in
+// increasing-mask order of d, and a down hop inside the line is the full sector's row-u down hop
+// (identity translation, unit norm, the same Jordan-Wigner signs, no Bloch phase), in every block,
+// of which every orbit of the line is a row.  The lines' down hops go through the full sector's
+// row pass (H*V kernels over a sector whose strings are in increasing-mask order); the momentum
+// kernels add the remaining terms, kept in a second CSR.
+struct OrbitSplit {
+  std::unique_ptr<Sector> sec;  // the full sector, strings in increasing-mask order
+  int nlines = 0;
+  DevBuf<int64_t> lbase;        // [nlines] first orbit of each trivial line
+  DevBuf<uint32_t> lumask;      // [nlines] its up string
+  DevBuf<uint8_t> triv;         // [nrep] 1 on a trivial line
+  DevBuf<int32_t> row_off2;     // [nrep + 1] the remaining terms, in CSR order
+  DevBuf<uint2> ent2;
+  int64_t nnz2 = 0;
+};
+
 struct OrbitGraph {
   int lx = 0, ly = 0, n_up = 0, n_dn = 0;
   double U = 0.0, V = 0.0;
@@ -1027,6 +1063,7 @@ struct OrbitGraph {
   DevBuf<int32_t> row_off;
   DevBuf<uint2> ent;
   DevBuf<double2> coef;
+  std::unique_ptr<OrbitSplit> split;  // null: no split applies (see OrbitSplit)
 
   bool same_model(int lx_, int ly_, int nu, int nd, double U_, double V_, const ftlm_hop_t* h, int nh,
                   const ftlm_pair_t* p, int np) const {
@@ -1049,6 +1086,8 @@ struct KBlock : ftlm_op {
   Phases ph{};
   DevBuf<int32_t> rows, map;
   DevBuf<int32_t> colk;  // [nnz] block column of every stored term, -1 when not a row here
+  DevBuf<int32_t> colk2;  // [nnz2] the same for the split's remaining terms
+  DevBuf<int64_t> lbase;  // [nlines] block row of the split's trivial lines' first orbits
   DevBuf<double> sq, isq;
   DevBuf<double2> A;  // [nrep]
 };
@@ -1077,6 +1116,102 @@ bool hops_translation_invariant(int lx, int ly, const ftlm_hop_t* hops, int n_ho
       if (std::abs(c - kv.second) > tol) return false;
     }
   return true;
+}
+
+// Whether orbit graphs get split applies (FTLM_ORBIT_SPLIT=0 keeps the one-kernel applies).
+bool orbit_split_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("FTLM_ORBIT_SPLIT");
+    return v == nullptr || std::atoi(v) != 0;
+  }();
+  return on;
+}
+
+// Builds G->split (see OrbitSplit) when the bond coefficients are real and the full sector has
+// D_up, D_dn >= 128 and some trivial line; otherwise leaves it null.
+void orbit_split_build(OrbitGraph* Gp) {
+  OrbitGraph& G = *Gp;
+  if (!orbit_split_enabled() || !G.real_h || G.nrep == 0 || G.nnz == 0) return;
+  const int n = G.lx * G.ly;
+  if (binom(n, G.n_up) < 128 || binom(n, G.n_dn) < 128) return;
+  auto sp = std::make_unique<OrbitSplit>();
+  sp->sec = sector_create_numeric(n, G.n_up, G.n_dn, G.hops.data(), static_cast<int>(G.hops.size()), G.pairs.data(),
+                                  static_cast<int>(G.pairs.size()), G.U, G.V);
+  const Sector* Sx = sp->sec.get();
+  if (!Sx->real_h || Sx->Ddn < 128) return;
+  // trivial lines, from the representatives (pack order: up string, then down string)
+  const int64_t nr = G.nrep;
+  std::vector<uint32_t> pack(static_cast<size_t>(nr));
+  FTLM_HIP_CHECK(hipMemcpy(pack.data(), G.pack.p, sizeof(uint32_t) * pack.size(), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> dn(static_cast<size_t>(Sx->Ddn));
+  FTLM_HIP_CHECK(hipMemcpy(dn.data(), Sx->dn_mask.p, sizeof(uint32_t) * dn.size(), hipMemcpyDeviceToHost));
+  std::vector<int64_t> lbase;
+  std::vector<uint32_t> lumask;
+  std::vector<uint8_t> triv(static_cast<size_t>(nr), 0);
+  for (int64_t a = 0; a < nr;) {
+    const uint32_t u = pack[static_cast<size_t>(a)] >> 16;
+    int64_t b = a;
+    while (b < nr && (pack[static_cast<size_t>(b)] >> 16) == u) ++b;
+    bool full = b - a == Sx->Ddn;
+    for (int64_t i = a; full && i < b; ++i)
+      full = (pack[static_cast<size_t>(i)] & 0xFFFFu) == dn[static_cast<size_t>(i - a)];
+    if (full) {
+      lbase.push_back(a);
+      lumask.push_back(u);
+      std::fill(triv.begin() + a, triv.begin() + b, 1);
+    }
+    a = b;
+  }
+  if (lbase.empty()) return;
+  // down-spin bond ids: the merged bond list (first occurrence of (from, to, spin), self hops
+  // skipped), as build_orbit_graph numbers them
+  std::vector<uint32_t> dnbits(16, 0u);
+  {
+    std::map<std::tuple<int, int, int>, int> seen;
+    int nb = 0;
+    for (const ftlm_hop_t& e : G.hops) {
+      if (e.from == e.to) continue;
+      const auto key = std::make_tuple(e.from, e.to, e.spin);
+      if (seen.count(key)) continue;
+      seen.emplace(key, nb);
+      if (e.spin == 1) dnbits[static_cast<size_t>(nb >> 5)] |= 1u << (nb & 31);
+      ++nb;
+    }
+    FTLM_REQUIRE(nb == G.nbond, FTLM_EHIP, "orbit graph: bond list mismatch");
+  }
+  sp->nlines = static_cast<int>(lbase.size());
+  sp->lbase.alloc(lbase.size());
+  sp->lbase.upload(lbase);
+  sp->lumask.alloc(lumask.size());
+  sp->lumask.upload(lumask);
+  sp->triv.alloc(triv.size());
+  sp->triv.upload(triv);
+  DevBuf<uint32_t> ddn(16);
+  ddn.upload(dnbits);
+  DevBuf<uint8_t> keep(static_cast<size_t>(G.nnz));
+  DevBuf<int32_t> cnt(static_cast<size_t>(nr));
+  const hipStream_t st = nullptr;
+  hipLaunchKernelGGL(k_split_keep, dim3(grid_for(nr, 256)), dim3(256), 0, st, G.row_off.p, G.ent.p, nr, sp->triv.p,
+                     ddn.p, keep.p, cnt.p);
+  check_launch("k_split_keep");
+  std::vector<int32_t> c(static_cast<size_t>(nr)), off(static_cast<size_t>(nr) + 1);
+  FTLM_HIP_CHECK(hipMemcpy(c.data(), cnt.p, sizeof(int32_t) * c.size(), hipMemcpyDeviceToHost));
+  off[0] = 0;
+  for (int64_t g = 0; g < nr; ++g)
+    off[static_cast<size_t>(g) + 1] = off[static_cast<size_t>(g)] + c[static_cast<size_t>(g)];
+  sp->nnz2 = off[static_cast<size_t>(nr)];
+  sp->row_off2.alloc(off.size());
+  sp->row_off2.upload(off);
+  sp->ent2.alloc(static_cast<size_t>(std::max<int64_t>(sp->nnz2, 1)));
+  DevBuf<int32_t> nsel(1);
+  const int nn = static_cast<int>(G.nnz);
+  cub_call([&](void* t, size_t& b, hipStream_t s2) {
+    return hipcub::DeviceSelect::Flagged(t, b, G.ent.p, keep.p, sp->ent2.p, nsel.p, nn, s2);
+  }, st);
+  int32_t got = 0;
+  FTLM_HIP_CHECK(hipMemcpy(&got, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost));
+  FTLM_REQUIRE(got == sp->nnz2, FTLM_EHIP, "orbit graph: split term count mismatch");
+  G.split = std::move(sp);
 }
 
 std::shared_ptr<OrbitGraph> build_orbit_graph(int lx, int ly, int n_up, int n_dn, const ftlm_hop_t* hops, int n_hops,
@@ -1224,6 +1359,7 @@ std::shared_ptr<OrbitGraph> build_orbit_graph(int lx, int ly, int n_up, int n_dn
     G->ent = std::move(v1);
   }
   FTLM_HIP_CHECK(hipDeviceSynchronize());
+  orbit_split_build(G.get());
   return G;
 }
 
@@ -1274,6 +1410,19 @@ void build_kblock(KBlock* B, double tol) {
                          B->colk.p);
       check_launch("k_kblock_cols");
     }
+    if (G.split) {  // every orbit of a trivial line is a row of every block, in one run of rows
+      const OrbitSplit& sp = *G.split;
+      B->colk2.alloc(static_cast<size_t>(std::max<int64_t>(sp.nnz2, 1)));
+      if (sp.nnz2 > 0) {
+        hipLaunchKernelGGL(k_kblock_cols, dim3(grid_for(sp.nnz2, 256)), dim3(256), 0, st, sp.ent2.p, sp.nnz2,
+                           B->map.p, B->colk2.p);
+        check_launch("k_kblock_cols");
+      }
+      B->lbase.alloc(static_cast<size_t>(sp.nlines));
+      hipLaunchKernelGGL(k_line_rows, dim3(grid_for(sp.nlines, 256)), dim3(256), 0, st, sp.lbase.p, sp.nlines,
+                         B->map.p, B->lbase.p);
+      check_launch("k_line_rows");
+    }
   }
   FTLM_HIP_CHECK(hipDeviceSynchronize());
   B->dim = dk;
@@ -1286,18 +1435,23 @@ void launch_kblock(const KBlock* B, const void* x, void* y, int RT, int r0, cons
                    hipStream_t st, int grid) {
   using V2 = typename Cx<T>::v2;
   const OrbitGraph& G = *B->G;
+  const OrbitSplit* sp = G.split.get();  // with a split, the remaining terms (added on the lines)
+  const int32_t* roff = sp ? sp->row_off2.p : G.row_off.p;
+  const uint2* ents = sp ? sp->ent2.p : G.ent.p;
+  const int32_t* cols = sp ? B->colk2.p : B->colk.p;
+  const uint8_t* acc = sp ? sp->triv.p : nullptr;
   if (RB * sizeof(V2) >= 128) {  // 8 or more lanes per row: shared entry loads
-    hipLaunchKernelGGL((k_kblock_apply_split<T, RB, true>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim,
-                       G.row_off.p, G.ent.p, B->colk.p, G.diag.p, G.coef.p, G.nbond, B->ph,
-                       static_cast<const V2*>(x), static_cast<V2*>(y), RT, r0, xs, dotp);
-  } else if (RB * sizeof(V2) >= 32) {
-    hipLaunchKernelGGL((k_kblock_apply_split<T, RB, false>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim,
-                       G.row_off.p, G.ent.p, B->colk.p, G.diag.p, G.coef.p, G.nbond, B->ph,
-                       static_cast<const V2*>(x), static_cast<V2*>(y), RT, r0, xs, dotp);
-  } else {
-    hipLaunchKernelGGL((k_kblock_apply<T, RB>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, G.row_off.p,
-                       G.ent.p, B->colk.p, G.diag.p, G.coef.p, G.nbond, B->ph, static_cast<const V2*>(x),
+    hipLaunchKernelGGL((k_kblock_apply_split<T, RB, true>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, roff,
+                       ents, cols, G.diag.p, G.coef.p, G.nbond, B->ph, acc, static_cast<const V2*>(x),
                        static_cast<V2*>(y), RT, r0, xs, dotp);
+  } else if (RB * sizeof(V2) >= 32) {
+    hipLaunchKernelGGL((k_kblock_apply_split<T, RB, false>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, roff,
+                       ents, cols, G.diag.p, G.coef.p, G.nbond, B->ph, acc, static_cast<const V2*>(x),
+                       static_cast<V2*>(y), RT, r0, xs, dotp);
+  } else {
+    hipLaunchKernelGGL((k_kblock_apply<T, RB>), dim3(grid), dim3(256), 0, st, B->rows.p, B->dim, roff, ents, cols,
+                       G.diag.p, G.coef.p, G.nbond, B->ph, acc, static_cast<const V2*>(x), static_cast<V2*>(y), RT,
+                       r0, xs, dotp);
   }
 }
 
@@ -1412,6 +1566,9 @@ int hv_apply_kblock(const ftlm_op* op, const void* x, void* y, int R, int dtype,
     chunks.emplace_back(r0, rb);
     r0 += rb;
   }
+  // split: the trivial lines' down hops by the full sector's row pass, then the remaining terms
+  const OrbitSplit* sp = B->G->split.get();
+  if (sp) hv_rows_lines(sp->sec.get(), x, y, R, dtype, xs, B->lbase.p, sp->lumask.p, sp->nlines, st);
   for (const auto& c : chunks) {
     if (dtype == FTLM_C64)
       launch_kblock_chunk<float>(B, x, y, R, c.first, c.second, xs, dotp, st, grid);
@@ -1431,120 +1588,11 @@ struct KStack : ftlm_op {
   DevBuf<uint16_t> mem;   // [nrep]
   DevBuf<uint16_t> memc;  // [nnz] mem of every term's column orbit
   DevBuf<double2> chi;    // [nk][16] Bloch phases of each block
-  // Split apply (kstack_split_build).  A trivial line is the run of orbits whose representative's
-  // up string u has no translation fixing it: it holds (u, d) for every down string d, in the
-  // sector's order, and its down hops are the full sector's row-u down hops (identity
-  // translation, unit norms, the same Jordan-Wigner signs) for every block.  Those go through
-  // the LDS-staged step row pass of the full sector; k_kstack_apply adds the remaining terms.
-  std::unique_ptr<Sector> sec;  // the full sector in increasing string order, for its step tables
-  int nlines = 0;
-  DevBuf<int64_t> lbase;     // [nlines] first orbit of each trivial line
-  DevBuf<uint32_t> lumask;   // [nlines] its up string
-  DevBuf<uint8_t> triv;      // [nrep] 1 on a trivial line
-  DevBuf<int32_t> row_off2;  // [nrep + 1] the remaining terms
-  DevBuf<uint2> ent2;
-  DevBuf<uint16_t> memc2;
+  DevBuf<uint16_t> memc2;  // [nnz2] mem of the graph split's remaining terms' column orbits
 };
 
 const void* kblock_graph(const ftlm_op* op) {
   return op != nullptr && op->kind == kKBlock ? static_cast<const KBlock*>(op)->G.get() : nullptr;
-}
-
-// Whether momentum stacks split their applies (FTLM_KS_SPLIT=0 keeps the one-kernel apply).
-bool kstack_split_enabled() {
-  static const bool on = [] {
-    const char* v = std::getenv("FTLM_KS_SPLIT");
-    return v == nullptr || std::atoi(v) != 0;
-  }();
-  return on;
-}
-
-// Builds the split apply's tables (see KStack) when the graph has real bond coefficients and the
-// full sector has step tables; otherwise leaves the stack on the one-kernel apply.
-void kstack_split_build(KStack* S) {
-  const OrbitGraph& G = *S->G;
-  if (!kstack_split_enabled() || !G.real_h || G.nrep == 0 || G.nnz == 0) return;
-  const int n = G.lx * G.ly;
-  std::unique_ptr<Sector> sec =
-      sector_create_numeric(n, G.n_up, G.n_dn, G.hops.data(), static_cast<int>(G.hops.size()), G.pairs.data(),
-                            static_cast<int>(G.pairs.size()), G.U, G.V);
-  const Sector* Sx = sec.get();
-  if (!sector_has_step_tables(Sx) || !Sx->real_h) return;
-  // trivial lines, from the representatives (pack order: up string, then down string)
-  const int64_t nr = G.nrep;
-  std::vector<uint32_t> pack(static_cast<size_t>(nr));
-  FTLM_HIP_CHECK(hipMemcpy(pack.data(), G.pack.p, sizeof(uint32_t) * pack.size(), hipMemcpyDeviceToHost));
-  std::vector<uint32_t> dn(static_cast<size_t>(Sx->Ddn));
-  FTLM_HIP_CHECK(hipMemcpy(dn.data(), Sx->dn_mask.p, sizeof(uint32_t) * dn.size(), hipMemcpyDeviceToHost));
-  std::vector<int64_t> lbase;
-  std::vector<uint32_t> lumask;
-  std::vector<uint8_t> triv(static_cast<size_t>(nr), 0);
-  for (int64_t a = 0; a < nr;) {
-    const uint32_t u = pack[static_cast<size_t>(a)] >> 16;
-    int64_t b = a;
-    while (b < nr && (pack[static_cast<size_t>(b)] >> 16) == u) ++b;
-    bool full = b - a == Sx->Ddn;
-    for (int64_t i = a; full && i < b; ++i) full = (pack[static_cast<size_t>(i)] & 0xFFFFu) == dn[static_cast<size_t>(i - a)];
-    if (full) {
-      lbase.push_back(a);
-      lumask.push_back(u);
-      std::fill(triv.begin() + a, triv.begin() + b, 1);
-    }
-    a = b;
-  }
-  if (lbase.empty()) return;
-  // down-spin bond ids: the graph's merged bond list (first occurrence of (from, to, spin), self
-  // hops skipped), as build_orbit_graph forms it
-  std::vector<uint32_t> dnbits(16, 0u);
-  {
-    std::map<std::tuple<int, int, int>, int> seen;
-    int nb = 0;
-    for (const ftlm_hop_t& e : G.hops) {
-      if (e.from == e.to) continue;
-      const auto key = std::make_tuple(e.from, e.to, e.spin);
-      if (seen.count(key)) continue;
-      seen.emplace(key, nb);
-      if (e.spin == 1) dnbits[static_cast<size_t>(nb >> 5)] |= 1u << (nb & 31);
-      ++nb;
-    }
-    FTLM_REQUIRE(nb == G.nbond, FTLM_EHIP, "momentum stack: bond list mismatch");
-  }
-  S->sec = std::move(sec);
-  S->nlines = static_cast<int>(lbase.size());
-  S->lbase.alloc(lbase.size());
-  S->lbase.upload(lbase);
-  S->lumask.alloc(lumask.size());
-  S->lumask.upload(lumask);
-  S->triv.alloc(triv.size());
-  S->triv.upload(triv);
-  DevBuf<uint32_t> ddn(16);
-  ddn.upload(dnbits);
-  DevBuf<uint8_t> keep(static_cast<size_t>(G.nnz));
-  DevBuf<int32_t> cnt(static_cast<size_t>(nr));
-  const hipStream_t st = nullptr;
-  hipLaunchKernelGGL(k_split_keep, dim3(grid_for(nr, 256)), dim3(256), 0, st, G.row_off.p, G.ent.p, nr, S->triv.p,
-                     ddn.p, keep.p, cnt.p);
-  check_launch("k_split_keep");
-  std::vector<int32_t> c(static_cast<size_t>(nr)), off(static_cast<size_t>(nr) + 1);
-  FTLM_HIP_CHECK(hipMemcpy(c.data(), cnt.p, sizeof(int32_t) * c.size(), hipMemcpyDeviceToHost));
-  off[0] = 0;
-  for (int64_t g = 0; g < nr; ++g) off[static_cast<size_t>(g) + 1] = off[static_cast<size_t>(g)] + c[static_cast<size_t>(g)];
-  const int64_t n2 = off[static_cast<size_t>(nr)];
-  S->row_off2.alloc(off.size());
-  S->row_off2.upload(off);
-  S->ent2.alloc(static_cast<size_t>(std::max<int64_t>(n2, 1)));
-  S->memc2.alloc(static_cast<size_t>(std::max<int64_t>(n2, 1)));
-  DevBuf<int32_t> nsel(1);
-  const int nn = static_cast<int>(G.nnz);
-  cub_call([&](void* t, size_t& b, hipStream_t s2) {
-    return hipcub::DeviceSelect::Flagged(t, b, G.ent.p, keep.p, S->ent2.p, nsel.p, nn, s2);
-  }, st);
-  cub_call([&](void* t, size_t& b, hipStream_t s2) {
-    return hipcub::DeviceSelect::Flagged(t, b, S->memc.p, keep.p, S->memc2.p, nsel.p, nn, s2);
-  }, st);
-  int32_t got = 0;
-  FTLM_HIP_CHECK(hipMemcpy(&got, nsel.p, sizeof(int32_t), hipMemcpyDeviceToHost));
-  FTLM_REQUIRE(got == n2, FTLM_EHIP, "momentum stack: split term count mismatch");
 }
 
 std::unique_ptr<ftlm_op> kstack_build(const std::vector<const ftlm_op*>& blocks) {
@@ -1585,8 +1633,15 @@ std::unique_ptr<ftlm_op> kstack_build(const std::vector<const ftlm_op*>& blocks)
                        S->memc.p);
     check_launch("k_kstack_memc");
   }
+  if (G.split) {
+    S->memc2.alloc(static_cast<size_t>(std::max<int64_t>(G.split->nnz2, 1)));
+    if (G.split->nnz2 > 0) {
+      hipLaunchKernelGGL(k_kstack_memc, dim3(grid_for(G.split->nnz2, 256)), dim3(256), 0, nullptr, G.split->ent2.p,
+                         G.split->nnz2, S->mem.p, S->memc2.p);
+      check_launch("k_kstack_memc");
+    }
+  }
   FTLM_HIP_CHECK(hipDeviceSynchronize());
-  kstack_split_build(S.get());
   return S;
 }
 
@@ -1610,19 +1665,14 @@ int hv_apply_kstack(const ftlm_op* op, const void* x, void* y, int R, int dtype,
   const int grid = static_cast<int>(std::min<int64_t>(gmax, (S->dim + rpb - 1) / rpb));
   const float2* xf = static_cast<const float2*>(x);
   float2* yf = static_cast<float2*>(y);
-  // split apply: the trivial lines' down hops by the step row pass (128-byte chunks of the
-  // slots), then the remaining terms, added on those lines
-  const size_t esz = dtype == FTLM_C64 ? sizeof(float2) : sizeof(double2);
-  const bool split = S->sec != nullptr && (static_cast<size_t>(R) * esz) % 128 == 0;
-  if (split) {
-    const int rb = static_cast<int>(128 / esz);
-    for (int r0 = 0; r0 < R; r0 += rb)
-      hv_step_rows_lines(S->sec.get(), x, y, R, r0, dtype, xs, S->lbase.p, S->lumask.p, S->nlines, st);
-  }
-  const int32_t* roff = split ? S->row_off2.p : G.row_off.p;
-  const uint2* ents = split ? S->ent2.p : G.ent.p;
-  const uint16_t* mcs = split ? S->memc2.p : S->memc.p;
-  const uint8_t* accp = split ? S->triv.p : nullptr;
+  // split apply: the trivial lines' down hops by the full sector's row pass, then the remaining
+  // terms, added on those lines
+  const OrbitSplit* sp = G.split.get();
+  if (sp) hv_rows_lines(sp->sec.get(), x, y, R, dtype, xs, sp->lbase.p, sp->lumask.p, sp->nlines, st);
+  const int32_t* roff = sp ? sp->row_off2.p : G.row_off.p;
+  const uint2* ents = sp ? sp->ent2.p : G.ent.p;
+  const uint16_t* mcs = sp ? S->memc2.p : S->memc.p;
+  const uint8_t* accp = sp ? sp->triv.p : nullptr;
 #define FTLM_KSTACK_ARGS S->dim, S->nk, rk, S->mem.p, mcs, roff, ents, G.diag.p, G.coef.p, G.nbond, S->chi.p, accp
   const int lanes = R / vpl;
   const bool coop = lanes >= 8 && (lanes & (lanes - 1)) == 0;
