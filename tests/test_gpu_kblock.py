@@ -202,6 +202,50 @@ def test_kblock_dims_sum_4x4():
     assert tot == 1820 * 1820
 
 
+def test_kblock_split_paths_4x4_4_4(oracle):
+    """4x4 (4,4) K=(1,2) block (about 2e5 rows): real bonds and D_up = D_dn = 1820 >= 128, so the
+    apply splits (the down hops of the orbits whose up string no translation fixes go through the
+    full sector's row pass: the gather row pass for R=1 complex128, the LDS-staged step row pass
+    for the 128-byte elements of R=8 complex128 and R=16 complex64; the other terms through the
+    block kernel). 1024 random rows of each column against the oracle's row-restricted orbit push
+    (src/orbit_k_block_basis.cpp:151-220) at 1e-12 relative; the R=8 columns against the R=1
+    applies at 1e-14 (the two row passes add the same down hops in other orders); complex64 at
+    1e-5."""
+    import torch
+
+    say("kblock split 4x4 (4,4) K=(1,2)")
+    B = kblock(4, 4, 4, 4, 1, 2)
+    d = B.dim()
+    reps = B.entries().rep
+    rng = np.random.default_rng(17)
+    R = 8
+    X = rng.standard_normal((d, R)) + 1j * rng.standard_normal((d, R))
+    xd = torch.tensor(X, dtype=torch.complex128, device="cuda").contiguous()
+    yd = torch.empty_like(xd)
+    B.apply_device(xd, yd, R=R, dtype=L.FTLM_C128)
+    Y = yd.cpu().numpy()
+    rows = np.sort(rng.choice(d, size=1024, replace=False))
+    for r in range(R):
+        y1 = B.apply(np.ascontiguousarray(X[:, r]))
+        want = oracle.orbit_apply_rows(params8(4, 4, 1.0, -0.25, 8.0, 1.0, 0.0, 0.0), 4, 4, 1, 2, reps,
+                                       np.ascontiguousarray(X[:, r]), rows)
+        e_or = np.max(np.abs(y1[rows] - want)) / np.max(np.abs(want))
+        e_st = np.max(np.abs(Y[rows, r] - want)) / np.max(np.abs(want))
+        e_12 = rel(Y[:, r], y1)
+        say("  r", r, "R=1 vs oracle", e_or, "R=8 vs oracle", e_st, "R=8 vs R=1", e_12)
+        assert e_or < 1e-12 and e_st < 1e-12 and e_12 < 1e-14
+    X16 = rng.standard_normal((d, 16)) + 1j * rng.standard_normal((d, 16))
+    x16 = torch.tensor(X16, dtype=torch.complex128, device="cuda").contiguous()
+    y16 = torch.empty_like(x16)
+    B.apply_device(x16, y16, R=16, dtype=L.FTLM_C128)
+    x64 = x16.to(torch.complex64)
+    y64 = torch.empty_like(x64)
+    B.apply_device(x64, y64, R=16, dtype=L.FTLM_C64)
+    e64 = rel(y64.cpu().numpy().astype(np.complex128), y16.cpu().numpy())
+    say("  c64 R=16 vs c128", e64)
+    assert e64 < 1e-5
+
+
 def dense_of(B):
     d = B.dim()
     H = np.zeros((d, d), np.complex128)

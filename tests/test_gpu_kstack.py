@@ -2,11 +2,12 @@
 recurrence group) against the per-block operators it replaces, through the C ABI.
 
 - apply: every block's slots of the stacked y equal that block's own apply (complex128: within
-  1e-14 relative, the same terms; on 4x4 sectors the stack adds the down hops of the orbits whose
-  up string has no fixing translation first, through the full sector's step row pass, and the
-  other terms after them; complex64: 1e-6), and the slots of blocks an orbit is not a row of are
-  0, on 3x2, 4x2 (with a flux) and 4x4 (4,4) (16 blocks), and on two blocks of the 4x4 (8,8)
-  stack of all 16;
+  1e-14 relative; on 4x4 sectors with real bonds both applies add the down hops of the orbits
+  whose up string has no fixing translation first, through the full sector's row pass -- the
+  LDS-staged step pass for the stack's wide elements, the gather pass for a block's narrow ones --
+  and the other terms after them, so the two agree to rounding; complex64: 1e-6), and the slots
+  of blocks an orbit is not a row of are 0, on 3x2, 4x2 (with a flux) and 4x4 (4,4) (16 blocks),
+  and on two blocks of the 4x4 (8,8) stack of all 16;
 - FTLM: ftlm_ritz through the stack (FTLM_PATH_STACK, and AUTO, which picks it) against the
   per-block path with the same random-phase starts: used steps equal, alpha/beta within 1e-10
   relative, Ritz values within 1e-9 and lnZ within 1e-10 (complex128; complex64 lnZ 1e-5), with
