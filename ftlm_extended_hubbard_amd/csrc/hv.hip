@@ -530,10 +530,19 @@ __device__ __forceinline__ uint4 load_const4(const uint4* p, int64_t i) {
   return make_uint4(q[0], q[1], q[2], q[3]);
 }
 
-// Asynchronous copy of 16 bytes from src to LDS: the wavefront's active lanes write
-// lds_wave + lane * 16 (lds_wave must be the same for the whole wavefront).
-__device__ __forceinline__ void lds_copy16(const void* src, void* lds_wave) {
-  __builtin_amdgcn_global_load_lds(src, (lds_void_t*)lds_wave, 16, 0, 0);
+// Buffer resource for raw byte offsets from a wave-uniform base (stride 0, every 32-bit offset
+// in range; data format 32 bits, the word that the gfx9 family expects).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), static_cast<short>(0), 0x7fffffff,
+                                           0x00020000);
+}
+
+// Asynchronous copy of 16 bytes from base + voff to LDS: the wavefront's active lanes write
+// lds_wave + lane * 16 (lds_wave must be the same for the whole wavefront).  The buffer form
+// (not global_load_lds) matters for the waits: the compiler counts these copies in vmcnt only,
+// while a global_load_lds in flight makes it wait for every LDS read with lgkmcnt(0).
+__device__ __forceinline__ void lds_copy16(__amdgpu_buffer_rsrc_t rsrc, uint32_t voff, void* lds_wave) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_void_t*)lds_wave, 16, voff, 0, 0, 0);
 }
 
 // Waits until every outstanding memory operation of the wavefront has completed.
@@ -619,21 +628,22 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
     const int st1 = __builtin_amdgcn_readfirstlane(blkp[blk + 1]);
     // copies of one step: thread tid moves 16-byte piece tid % 8 of source element tid / 8 (plus
     // kStepThreads / 8 elements per further kStepThreads pieces); lane l of wavefront w moves
-    // 16 bytes l (and 64 + l) of w's rows
-    const char* __restrict__ xsrc = reinterpret_cast<const char*>(xl) + (tid >> 3) * pbytes + (tid & 7) * 16;
+    // 16 bytes l (and 64 + l) of w's rows.  Offsets from the step's first source element stay
+    // below kStepSub * pbytes, under 2^31 (checked by the launcher).
+    const uint32_t xlane = static_cast<uint32_t>((tid >> 3) * pbytes + (tid & 7) * 16);
     auto issue = [&](const uint4 d, uint4* xb, uint32_t* eb) {
       const uint32_t xoff = d.x & 0xfffffu, nu = (d.x >> 20) * 8u;
       const uint32_t row0 = d.y & 0x3ffffffu, ne = (d.y >> 26) * 4u;
-      const char* xs0 = xsrc + static_cast<int64_t>(xoff) * pbytes;
+      const __amdgpu_buffer_rsrc_t rx = raw_rsrc(reinterpret_cast<const char*>(xl) + static_cast<int64_t>(xoff) * pbytes);
 #pragma unroll
       for (int it = 0; it < (kStepSub * 8 + kStepThreads - 1) / kStepThreads; ++it)
         if (static_cast<uint32_t>(it * kStepThreads + tid) < nu)
-          lds_copy16(xs0 + static_cast<int64_t>(it * (kStepThreads / 8)) * pbytes,
+          lds_copy16(rx, xlane + static_cast<uint32_t>(it * (kStepThreads / 8) * pbytes),
                      reinterpret_cast<char*>(xb) + (it * kStepThreads + w * 64) * 16);
-      const char* es0 = sent + static_cast<int64_t>(row0) * (kStepGroup * 4) + lane * 16;
+      const __amdgpu_buffer_rsrc_t re = raw_rsrc(sent + static_cast<int64_t>(row0) * (kStepGroup * 4));
       char* ed = reinterpret_cast<char*>(eb + w * kEw);
-      if (static_cast<uint32_t>(lane) < ne) lds_copy16(es0, ed);
-      if (static_cast<uint32_t>(lane + 64) < ne) lds_copy16(es0 + 1024, ed + 1024);
+      if (static_cast<uint32_t>(lane) < ne) lds_copy16(re, lane * 16u, ed);
+      if (static_cast<uint32_t>(lane + 64) < ne) lds_copy16(re, lane * 16u + 1024u, ed + 1024);
     };
     // one hop: its source's two 16-byte pieces and its coefficient (fetch), then the FMAs
     struct HopData {
@@ -674,12 +684,17 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
     // then the rounds
     auto step = [&](auto pc, const int st) {
       constexpr int P = decltype(pc)::value;
+      // the descriptor of step st + 2 is loaded before the wait, so that no scalar load is in
+      // flight during the rounds: LDS reads alone then complete in order and each hop waits only
+      // for its own reads
+      uint4 dn2 = dnext;
+      if (st + 2 < st1) dn2 = load_const4(desc, static_cast<int64_t>(st + 2) * kStepWaves + w);
       wait_all_memory();
       __syncthreads();  // step st has landed; every wavefront is done with the other buffers
       const uint4 d = dcur;
       if (st + 1 < st1) issue(dnext, P ? s_x0 : s_x1, P ? s_e0 : s_e1);
       dcur = dnext;
-      if (st + 2 < st1) dnext = load_const4(desc, static_cast<int64_t>(st + 2) * kStepWaves + w);
+      dnext = dn2;
       const char* xb1 = reinterpret_cast<const char*>(P ? s_x1 : s_x0) + (sub * 32 + odd * 16);
       const char* xb2 = reinterpret_cast<const char*>(P ? s_x1 : s_x0) + (sub * 32 + (16 - odd * 16));
       // one offset for the whole per-lane base: the compiler's LDS alias information follows only
@@ -991,9 +1006,12 @@ Plan make_plan(const Sector* Sx, int RT, int esize, int L, const HvShape& sh) {
 
 // Whether a two-pass sector apply of 128-byte elements uses the LDS-staged step passes (default)
 // or the L2-gather passes k_hv_dn / k_hv_up (FTLM_HV_STEP=0, kept for comparison).
-bool use_step(const SectorView& S) {
+// The copies address a step's sources by 32-bit offsets from its first source element: kStepSub
+// elements of the column pass (stride D_dn * RT vectors) must span less than 2^31 bytes.
+bool use_step(const SectorView& S, int RT, size_t vbytes) {
   static const bool on = env_int("FTLM_HV_STEP", 1) != 0;
-  return on && S.st_ent_up != nullptr && S.st_ent_dn != nullptr;
+  const bool fits = static_cast<int64_t>(kStepSub) * S.Ddn * RT * static_cast<int64_t>(vbytes) < (int64_t{1} << 31);
+  return on && fits && S.st_ent_up != nullptr && S.st_ent_dn != nullptr;
 }
 
 template <typename T, bool REAL, bool UP>
@@ -1017,7 +1035,7 @@ int launch(const Sector* Sx, const void* x, void* y, int RT, int r0, const doubl
   const V2* xp = static_cast<const V2*>(x);
   V2* yp = static_cast<V2*>(y);
   if constexpr (RB * sizeof(V2) == 128) {  // 16 complex64 or 8 complex128 vectors
-    if (p.two && use_step(S)) {
+    if (p.two && use_step(S, RT, sizeof(V2))) {
       // as many workgroups per CU as the step buffers fit in its LDS (32 CUs per XCD)
       const int bpx = std::max(1, std::min(32 * kStepWgPerCu, sh.bpx > 0 ? sh.bpx : 32 * kStepWgPerCu));
       const int grid = kXcd * bpx;
@@ -1120,7 +1138,7 @@ void rows_lines_chunk(const Sector* Sx, const SectorView& S, const void* x, void
   const V2* xp = static_cast<const V2*>(x);
   V2* yp = static_cast<V2*>(y);
   if constexpr (RB * sizeof(V2) == 128) {
-    if (use_step(S)) {
+    if (use_step(S, RT, sizeof(V2))) {
       const int nblk = (S.Ddn + kStepBlock - 1) / kStepBlock;
       const int64_t nt = static_cast<int64_t>(nlines) * nblk;
       FTLM_REQUIRE(nt < (int64_t{1} << 31), FTLM_EUNSUPPORTED, "too many step tiles");
