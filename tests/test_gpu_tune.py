@@ -3,7 +3,8 @@
 A launch shape changes how the work is cut into launches and workgroups, never what is summed
 per output element, so y = H x must not change beyond rounding (checked within 1e-13 relative,
 max-norm, against the default shape on the same input), and an FTLM run with a tuned operator
-must give the same lnZ within 1e-10 (the dot partials are summed in another order).
+must give the same lnZ within 1e-11 relative (the dot partials are summed in another order; with
+complex64 vectors and M=60 the last-bit differences of alpha reach 2e-12 of lnZ at beta=8).
 """
 
 import json
@@ -89,7 +90,7 @@ def test_tuned_ftlm_matches_default(tmp_path):
     res = op.autotune(16, L.FTLM_C64, reps=2, cache=cache)
     assert not res["cached"]
     after = api.ftlm_log_partition_complex(op, betas, par)
-    assert np.max(np.abs(after - before)) < 1e-10
+    assert np.all(np.abs(after - before) <= 1e-11 * np.maximum(1.0, np.abs(before))), (before, after)
     # a fresh operator of the same sector restores the shape from the cache without timing
     op2 = api.HubbardOperator.from_params(p, 3, 3)
     res2 = op2.autotune(16, L.FTLM_C64, cache=cache)
