@@ -211,6 +211,39 @@ def test_apply_4x4_hermitian_and_precision():
     torch.cuda.empty_cache()
 
 
+def test_apply_4x4_step_passes_equal_gather_passes():
+    """The LDS-staged step passes (128-byte elements: R=8 complex128, R=16 complex64) at the full
+    4x4 (8,8) size against the L2-gather passes of single vectors (R=1 complex128, the kernels the
+    sampled-row oracle test pins): every two-pass kernel adds a target's terms in the same order,
+    so complex128 columns are equal bit for bit; complex64 R=16 (each vector twice, in other lane
+    slots: equal bit for bit) within 1e-6 of complex128."""
+    import torch
+
+    say("apply 4x4 (8,8) step passes vs gather passes")
+    op = api.HubbardOperator.from_params(api.HubbardParams(*C3), 8, 8)
+    d = op.dim()
+    g = torch.Generator(device="cuda").manual_seed(9)
+    X = torch.randn(d, 8, dtype=torch.complex128, device="cuda", generator=g)
+    Y = torch.empty_like(X)
+    op.apply_device(X, Y, R=8, dtype=L.FTLM_C128)
+    y1 = torch.empty(d, dtype=torch.complex128, device="cuda")
+    for r in (0, 5):
+        op.apply_device(X[:, r].contiguous(), y1, R=1, dtype=L.FTLM_C128)
+        torch.cuda.synchronize()
+        assert torch.equal(Y[:, r].contiguous(), y1), r
+    del y1
+    X64 = torch.cat([X, X], dim=1).to(torch.complex64)  # vectors r and r + 8 in other lane slots
+    Y64 = torch.empty_like(X64)
+    op.apply_device(X64, Y64, R=16, dtype=L.FTLM_C64)
+    torch.cuda.synchronize()
+    assert torch.equal(Y64[:, :8], Y64[:, 8:])
+    e = (torch.linalg.vector_norm(Y64[:, :8].to(torch.complex128) - Y) / torch.linalg.vector_norm(Y)).item()
+    say("  c64 R=16 vs c128 R=8: rel", e)
+    assert e < 1e-6, e
+    del X, Y, X64, Y64
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("name,nu,nd", APPLY)
 @pytest.mark.parametrize("seed", [7, 42])
 def test_lanczos_golden(golden, name, nu, nd, seed):
