@@ -545,6 +545,21 @@ __device__ __forceinline__ void lds_copy16(__amdgpu_buffer_rsrc_t rsrc, uint32_t
   __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_void_t*)lds_wave, 16, voff, 0, 0, 0);
 }
 
+// The same for 4 bytes (lane l writes lds_wave + l * 4).
+__device__ __forceinline__ void lds_copy4(__amdgpu_buffer_rsrc_t rsrc, uint32_t voff, void* lds_wave) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_void_t*)lds_wave, 4, voff, 0, 0, 0);
+}
+
+// Workgroup barrier without a memory fence: __syncthreads() would also wait for every copy to
+// LDS in flight (its release covers them), which the staged epilogue keeps in flight.  Callers
+// order LDS themselves: each wavefront waits for its own copies (vmcnt) or reads (lgkmcnt)
+// before it.  The empty asm statements keep the compiler from moving memory accesses across.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_barrier();
+  asm volatile("" ::: "memory");
+}
+
 // Waits until every outstanding memory operation of the wavefront has completed.
 __device__ __forceinline__ void wait_all_memory() { __builtin_amdgcn_s_waitcnt(0); }
 
@@ -674,6 +689,42 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
         }
       }
     };
+    // Column pass: the epilogue's inputs of each round (y of the row pass and x at the round's
+    // 256 targets, and the diagonal's up-string tables) are staged in LDS two rounds ahead, in
+    // the step buffers (free once the last step is done): round j in set j % 2, y in s_x (32 KB)
+    // with the tables behind it (planes 4 KB, up strings 1 KB, nuu 1 KB), x in s_e (32 KB).  A
+    // target past D copies element D - 1 instead (its lanes store nothing), so every wavefront
+    // issues the same copies: 4 of y and x, and one of a table in wavefronts 0-11.
+    static_assert(sizeof(s_x0) >= kStepRound * (128 + 24) && sizeof(s_e0) >= kStepRound * 128,
+                  "a round's staged inputs fit the step buffers");
+    const int rt0 = blk * kStepBlock;
+    const int nround = min(NR, (D - rt0 + kStepRound - 1) / kStepRound);  // rounds with a target
+    const V2* __restrict__ yl = y + (xl - x);
+    const uint32_t pb32 = static_cast<uint32_t>(pbytes);
+    auto stage = [&](const int j, uint4* sxs, uint32_t* ses) {
+      int tq = static_cast<int>(threadIdx.x);  // per-lane values are recomputed, not kept live
+      asm volatile("" : "+v"(tq));
+      const int rt = rt0 + j * kStepRound;  // below D for every staged round
+      const uint32_t kmax = static_cast<uint32_t>(D - 1 - rt);
+      const int64_t eo0 = static_cast<int64_t>(rt) * pstride;
+      const __amdgpu_buffer_rsrc_t ry = raw_rsrc(yl + eo0), rx = raw_rsrc(xl + eo0);
+#pragma unroll
+      for (int it = 0; it < kStepRound * 8 / kStepThreads; ++it) {  // y and x: 4 copies per thread
+        const uint32_t k = min(static_cast<uint32_t>(it * (kStepThreads / 8) + (tq >> 3)), kmax);
+        const uint32_t vo = k * pb32 + static_cast<uint32_t>(tq & 7) * 16u;
+        const int lo = (it * kStepThreads + w * 64) * 16;
+        lds_copy16(ry, vo, reinterpret_cast<char*>(sxs) + lo);
+        lds_copy16(rx, vo, reinterpret_cast<char*>(ses) + lo);
+      }
+      const uint32_t kd = min(static_cast<uint32_t>(tq & (kStepRound - 1)), kmax);
+      char* sd = reinterpret_cast<char*>(sxs) + kStepRound * 128;
+      if (w < 4)
+        lds_copy16(raw_rsrc(S.planes + rt), kd * 16u, sd + w * 1024);
+      else if (w < 8)
+        lds_copy4(raw_rsrc(S.up_mask + rt), kd * 4u, sd + 4096 + (w - 4) * 256);
+      else if (w < 12)
+        lds_copy4(raw_rsrc(S.nuu + rt), kd * 4u, sd + 5120 + (w - 8) * 256);
+    };
     uint4 dcur = make_uint4(0u, 0u, 0u, 0u), dnext = dcur;
     if (st0 < st1) {
       dcur = load_const4(desc, static_cast<int64_t>(st0) * kStepWaves + w);
@@ -752,36 +803,82 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
     if constexpr (UP) {
       const uint32_t dmask = lmask[line];  // the line's down string
       const int nd_line = S.ndd[line];
+      wait_all_memory();
+      __syncthreads();  // every wavefront is done with both step buffers
+      stage(0, s_x0, s_e0);
+      if (nround > 1) stage(1, s_x1, s_e1);
 #pragma unroll
       for (int j = 0; j < NR; ++j) {
-        const int t = te + j * kStepRound;
-        if (t >= D) continue;
-        const int64_t jo = static_cast<int64_t>(j * kStepRound) * pstride;
+        if (j >= nround) continue;  // uniform
+        // round j's copies have landed in every wavefront: this wavefront's own copies are older
+        // than the next round's 4 or 5 (when there is a next round), which may stay in flight
+        if (j + 1 < nround)
+          __builtin_amdgcn_s_waitcnt(0x0f74);  // vmcnt(4)
+        else
+          __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+        lds_barrier();
+        // per-lane values from an opaque copy of the thread index, so that none stays live
+        // across rounds (registers are short here, and a spill reload waits for the copies)
+        int tq = static_cast<int>(threadIdx.x);
+        asm volatile("" : "+v"(tq));
+        const int eq = tq >> 2, pq = (tq & 3) * 32 + ((eq & 1) ? 16 : 0), pq2 = (tq & 3) * 32 + ((eq & 1) ? 0 : 16);
+        const bool q1 = (j & 1) != 0;  // compile-time: the loop is unrolled
+        const char* sy = reinterpret_cast<const char*>(q1 ? s_x1 : s_x0) + eq * 128;
+        const char* sxe = reinterpret_cast<const char*>(q1 ? s_e1 : s_e0) + eq * 128;
+        const char* sd = reinterpret_cast<const char*>(q1 ? s_x1 : s_x0) + kStepRound * 128;
+        const int t = rt0 + j * kStepRound + eq;
+        const bool live = t < D;
         V2 py[VPL], px[VPL];
-#pragma unroll
-        for (int v = 0; v < VPL; ++v) {
-          py[v] = v < H ? yta[jo + v] : ytb[jo + (v - H)];
-          px[v] = v < H ? xta[jo + v] : xtb[jo + (v - H)];
+        uint4 pl = make_uint4(0u, 0u, 0u, 0u);
+        uint32_t um = 0u;
+        int nu = 0;
+        if (live) {
+          const uint4 y1 = *reinterpret_cast<const uint4*>(sy + pq);
+          const uint4 y2 = *reinterpret_cast<const uint4*>(sy + pq2);
+          const uint4 x1 = *reinterpret_cast<const uint4*>(sxe + pq);
+          const uint4 x2 = *reinterpret_cast<const uint4*>(sxe + pq2);
+          __builtin_memcpy(&py[0], &y1, sizeof(uint4));
+          __builtin_memcpy(&py[H], &y2, sizeof(uint4));
+          __builtin_memcpy(&px[0], &x1, sizeof(uint4));
+          __builtin_memcpy(&px[H], &x2, sizeof(uint4));
+          pl = *reinterpret_cast<const uint4*>(sd + eq * 16);
+          um = *reinterpret_cast<const uint32_t*>(sd + 4096 + eq * 4);
+          nu = *reinterpret_cast<const int*>(sd + 5120 + eq * 4);
         }
-        const T dgt = static_cast<T>(diag_from(S, S.up_mask[t], dmask, S.planes[t], S.nuu[t], nd_line));
+        if (live) {
+          const T dgt = static_cast<T>(diag_from(S, um, dmask, pl, nu, nd_line));
+          // the element's two pieces in y (slots [0, H) at byte pq, [H, VPL) at byte pq2)
+          char* ye = reinterpret_cast<char*>(const_cast<V2*>(yl + static_cast<int64_t>(t) * pstride));
+          V2* ya = reinterpret_cast<V2*>(ye + pq);
+          V2* yb = reinterpret_cast<V2*>(ye + pq2);
+          const double* sxq = s_xs + (tq & 3) * VPL;
+          const int hq = (eq & 1) ? H : 0;
 #pragma unroll
-        for (int v = 0; v < VPL; ++v) {
-          V2* yp = v < H ? yta + v : ytb + (v - H);
-          const double sx = sxl[v ^ hx];
-          T a_r = ar[j][v], a_i = ai[j][v];
-          asm volatile("" : "+v"(a_r), "+v"(a_i));  // convert round by round (not all rounds up front)
-          const V2 q = px[v];
-          V2 o;
-          o.x = hv_finish(py[v].x, a_r, dgt, q.x, sx);
-          o.y = hv_finish(py[v].y, a_i, dgt, q.y, sx);
-          yp[jo] = o;
-          if (dotp) {
-            const double qr = sx * static_cast<double>(q.x), qi = sx * static_cast<double>(q.y);
-            dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
-            dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+          for (int v = 0; v < VPL; ++v) {
+            V2* yp = v < H ? ya + v : yb + (v - H);
+            const double sx = sxq[v ^ hq];
+            T a_r = ar[j][v], a_i = ai[j][v];
+            asm volatile("" : "+v"(a_r), "+v"(a_i));  // convert round by round (not all rounds up front)
+            const V2 q = px[v];
+            V2 o;
+            o.x = hv_finish(py[v].x, a_r, dgt, q.x, sx);
+            o.y = hv_finish(py[v].y, a_i, dgt, q.y, sx);
+            *yp = o;
+            if (dotp) {
+              const double qr = sx * static_cast<double>(q.x), qi = sx * static_cast<double>(q.y);
+              dre[v] += qr * static_cast<double>(o.x) + qi * static_cast<double>(o.y);
+              dimg[v] += qr * static_cast<double>(o.y) - qi * static_cast<double>(o.x);
+            }
           }
         }
-        asm volatile("" ::: "memory");  // keep each round's y traffic together (bounds live registers)
+        if (j + 2 < nround) {  // round j + 2 into the set every wavefront has finished reading
+          __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): this wavefront's reads of round j are done
+          lds_barrier();                        // and so are every wavefront's
+          if (q1)
+            stage(j + 2, s_x1, s_e1);
+          else
+            stage(j + 2, s_x0, s_e0);
+        }
       }
     } else {
 #pragma unroll
