@@ -1028,8 +1028,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profile(),
                 "kernel": f"H*V = {kn_row} then {kn_col}",
                 "limiter": "not HBM: the hop rounds (VALU 62% busy, LDS source reads with inherent bank "
-                           "conflicts at 128-byte elements) and the per-step barrier, which waits for the most "
-                           "loaded wavefront; the step copies are hidden",
+                           "conflicts at 128-byte elements), the per-step barrier, which waits for the most "
+                           "loaded wavefront, and the column epilogue (5 ms); the step copies are hidden",
                 "launch_ms": round(t_k * 1e3, 4),
                 "pass_ms": {kn_row: round(tr * 1e3, 4), kn_col: round(tc * 1e3, 4)},
                 "algorithmic_bytes_per_launch": B,
