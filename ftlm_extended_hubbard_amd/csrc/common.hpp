@@ -136,8 +136,9 @@ enum OpKind { kSector = 0, kKBlock = 1, kCallback = 2, kGram = 3, kKStack = 4 };
 // group inside the step's source range), row k holding hop k of each of the 16 targets, padded
 // with zero-coefficient entries.  Each target's hops appear in increasing source order across
 // steps and rows.  Entry (uint32): bits 0..15 the source's byte offset in the step's LDS copy,
-// (source - xoff) * 128, bits 16..31 the byte offset 16 * c of its coefficient c (own-spin
-// parity * 256 + bond; kStepZero is the zero coefficient) in a table of 16-byte slots.
+// (source - xoff) * 128, bits 16..31 8 * c for its coefficient c (own-spin parity * 256 +
+// bond; kStepZero is the zero coefficient): the byte offset in a table of 8-byte complex64
+// slots, half of it in a table of 16-byte complex128 slots.
 // Arrays: blk[b] .. blk[b + 1] = the block's steps; desc[step * 16 + w] = {xoff | nx << 20,
 // first row | rows << 26, K_0..K_3 (bytes), K_4..K_7}; ent = rows of 16 entries.
 constexpr int kStepBlock = 2048;

@@ -578,7 +578,7 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
   __shared__ uint4 s_x1[kStepSub * 8];
   __shared__ uint32_t s_e0[kStepWaves * kEw];
   __shared__ uint32_t s_e1[kStepWaves * kEw];
-  __shared__ uint4 s_c[kStepCoefSlots];  // line-signed coefficients [parity * 256 + bond], 16 bytes apart
+  __shared__ uint4 s_c[kStepCoefSlots];  // line-signed coefficients [parity * 256 + bond], sizeof(V2) bytes apart
   __shared__ int s_tile;
   __shared__ double s_xs[RB];  // xs of the launch's vectors (1 without xs)
   const int D = UP ? S.Dup : S.Ddn;
@@ -624,10 +624,11 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
           c.x = static_cast<T>(neg ? -cc.x : cc.x);
           c.y = static_cast<T>(neg ? -cc.y : cc.y);
         }
-        *reinterpret_cast<V2*>(&s_c[b]) = c;
+        V2* sc = reinterpret_cast<V2*>(s_c);  // slots sizeof(V2) bytes apart
+        sc[b] = c;
         c.x = -c.x;
         c.y = -c.y;
-        *reinterpret_cast<V2*>(&s_c[b + kStepCoefSlots / 2]) = c;
+        sc[b + kStepCoefSlots / 2] = c;
       }
       cur_line = line;
     }
@@ -666,7 +667,7 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
       V2 c;
     };
     auto fetch = [&](HopData& h, const char* xb1, const char* xb2, uint32_t en) {
-      // entry = source byte offset | coefficient byte offset << 16 (16-bit halves)
+      // entry = source byte offset | coefficient slot * 8 << 16 (16-bit halves)
       const uint32_t eo = en & 0xffffu;
       // each piece is one 16-byte LDS read (uint4 keeps the compiler from splitting it)
       const uint4 q1 = *reinterpret_cast<const uint4*>(xb1 + eo);
@@ -674,8 +675,11 @@ __global__ __launch_bounds__(kStepThreads) void k_hv_step(const SectorView S, co
       static_assert(sizeof(V2) * H == sizeof(uint4), "a piece is H vectors");
       __builtin_memcpy(&h.x[0], &q1, sizeof(uint4));
       __builtin_memcpy(&h.x[H], &q2, sizeof(uint4));
+      // the entry holds the coefficient slot times 8: a byte offset for complex64 slots (8 bytes
+      // apart, twice the bank positions of 16-byte slots), half of one for complex128
+      const uint32_t co = sizeof(V2) == 8 ? (en >> 16) : (en >> 16) << 1;
       h.c = *static_cast<const V2*>(
-          __builtin_assume_aligned(reinterpret_cast<const char*>(s_c) + (en >> 16), 16));
+          __builtin_assume_aligned(reinterpret_cast<const char*>(s_c) + co, sizeof(V2)));
     };
     auto fmas = [&](T (&cr)[VPL], T (&cim)[VPL], const HopData& h) {
 #pragma unroll

@@ -338,7 +338,7 @@ namespace {
 void build_step_table(Sector* S, int sp, int D, const std::vector<int32_t>& off, const std::vector<uint32_t>& ent) {
   const int nblk = (D + kStepBlock - 1) / kStepBlock;
   constexpr int kGroups = kStepBlock / kStepGroup;  // 128: group g = j * 16 + w
-  const uint32_t pad = kStepZero * 16u << kStepPosBits;
+  const uint32_t pad = kStepZero * 8u << kStepPosBits;
   std::vector<int32_t> blk(static_cast<size_t>(nblk) + 1, 0);
   std::vector<uint4> desc;
   std::vector<uint32_t> rows;
@@ -385,7 +385,7 @@ void build_step_table(Sector* S, int sp, int D, const std::vector<int32_t>& off,
           const int t = static_cast<int>((trip[q] >> 16) & 0xffffu);
           if (lst[static_cast<size_t>(t)].empty()) touched.push_back(t);
           lst[static_cast<size_t>(t)].push_back((src - xoff) * 128u |
-                                                static_cast<uint32_t>(trip[q] & 0xffffu) * 16u << kStepPosBits);
+                                                static_cast<uint32_t>(trip[q] & 0xffffu) * 8u << kStepPosBits);
         }
         std::copy(need, need + kGroups, K);
         std::copy(nr, nr + kStepWaves, wrows);
